@@ -1,0 +1,110 @@
+/*
+ * untangle.h -- C ABI of libuntangle_hip, the gfx950 implementation of the
+ * Untangling26 per-step hot path (LBVH, EF intersection detection, contours,
+ * proximity DCD, ACCD, IPC assembly, PCG).
+ *
+ * Conventions
+ *  - Every compute entry point takes DEVICE pointers (HBM) and plain sizes.
+ *    Positions are packed float[3*V]; edges uint32[2*E]; faces uint32[3*F].
+ *  - Work is enqueued on the context's HIP stream; nothing synchronises unless
+ *    the function says so.  Device-side counters may exceed the capacity the
+ *    caller passed: the caller compares and retries with a larger buffer,
+ *    exactly like the reference's resize-and-retry loop
+ *    (newton_solver.cpp:1402-1438).
+ *  - Return value: 0 ok, >0 recoverable (1 capacity, 2 not converged),
+ *    <0 fatal (-1 invalid argument, -2 HIP error, -3 internal).  The message
+ *    of the last failure is returned by ut_last_error(ctx).  No C++ exception
+ *    crosses this boundary.
+ *  - One stream per context; a context is not thread-safe (the reference
+ *    drives one luisa::compute::Stream from one thread, solver_interface.h:39-67).
+ *
+ * Reference citations are paths inside ChengzhuUwU/Untangling26.
+ */
+#ifndef UNTANGLE_H
+#define UNTANGLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UT_API __attribute__((visibility("default")))
+
+typedef struct ut_ctx  ut_ctx;
+typedef struct ut_lbvh ut_lbvh;
+
+/* ---- context ---------------------------------------------------------- */
+
+/* ABI version of this header. */
+UT_API int ut_version(void);
+/* Creates a context on HIP device `device`.  `hip_stream` is an existing
+ * hipStream_t to launch on, or NULL to create an owned stream.  Replaces the
+ * luisa::compute::Device/Stream pair of solver_interface.h:35-67. */
+UT_API int ut_ctx_create(int device, void* hip_stream, ut_ctx** out);
+UT_API int ut_ctx_destroy(ut_ctx* ctx);
+UT_API const char* ut_last_error(const ut_ctx* ctx);
+/* Blocks until all work on the context stream is done. */
+UT_API int ut_ctx_synchronize(ut_ctx* ctx);
+/* The hipStream_t the context launches on. */
+UT_API void* ut_ctx_stream(ut_ctx* ctx);
+
+/* ---- LBVH: lcs::LBVH, Solver/CollisionDetector/lbvh.h:164-230 ---------- */
+
+enum { UT_TREE_FACE = 1, UT_TREE_EDGE = 2 };
+enum { UT_QUERY_VERTS = 0, UT_QUERY_EDGES = 1, UT_QUERY_EDGES_UNCULLED = 2 };
+
+/* LbvhData::allocate (lbvh.h:108-150) for a face or edge tree of num_leaves. */
+UT_API int ut_lbvh_create(ut_ctx* ctx, int tree_type, uint32_t num_leaves, ut_lbvh** out);
+UT_API int ut_lbvh_destroy(ut_lbvh* bvh);
+/* LBVH::reduce_face_tree_aabb / reduce_edge_tree_aabb (lbvh.cpp:1003-1021)
+ * followed by the Morton pass of construct_tree (fn_compute_mortons,
+ * lbvh.cpp:512-528).  prims: faces uint32[3N] or edges uint32[2N]. */
+UT_API int ut_lbvh_reduce_aabb(ut_lbvh* bvh, const float* x, const uint32_t* prims);
+/* LBVH::construct_tree (lbvh.cpp:1023-1047): on-device radix sort instead of
+ * the host sort, then fn_apply_sorted + fn_build_inner_nodes. */
+UT_API int ut_lbvh_construct_tree(ut_lbvh* bvh);
+/* Builds the topology from caller-supplied sorted 64-bit keys
+ * (morton30 << 32 | object index), as LBVH::unit_test does (lbvh.cpp:920-984). */
+UT_API int ut_lbvh_build_from_sorted_keys(ut_lbvh* bvh, const uint64_t* sorted_keys);
+/* LBVH::update_{face,edge}_tree_leave_aabb + LBVH::refit (lbvh.cpp:1050-1092):
+ * swept leaf box of x_start/x_end inflated by thickness[first vertex]. */
+UT_API int ut_lbvh_refit(ut_lbvh* bvh, const float* thickness, const float* x_start, const float* x_end,
+	const uint32_t* prims);
+/* LBVH::broad_phase_query_from_{verts,edges,edges_unculled} (lbvh.cpp:1193-1262).
+ * Appends (query id, leaf primitive id) pairs to list (uint32[2*max_count]) and
+ * adds the number found to *count (device; not reset by this call). */
+UT_API int ut_lbvh_query(ut_lbvh* bvh, int kind, const float* x_begin, const float* x_end, const uint32_t* edges,
+	uint32_t num_queries, const float* d_hat, const float* thickness, uint32_t* count, uint32_t* list,
+	uint32_t max_count);
+/* Device-to-device copies of the tree in the reference numbering (inner nodes
+ * 0..N-2, leaves N-1..2N-2); any output may be NULL.  children is uint32[2(2N-1)]. */
+UT_API int ut_lbvh_tree_arrays(ut_lbvh* bvh, uint64_t* morton_sorted, uint32_t* sorted_get_original,
+	uint32_t* parent, uint32_t* children, uint32_t* object_idx);
+/* Root box after refit (device float[6]) and global centroid box (device float[6]). */
+UT_API int ut_lbvh_boxes(ut_lbvh* bvh, float* root_box, float* global_box);
+/* Host-visible health: *healthy = 1 when the build succeeded, *truncated =
+ * number of queries whose traversal stack was exhausted (always 0 for trees
+ * built from unique keys).  Synchronises the stream. */
+UT_API int ut_lbvh_status(ut_lbvh* bvh, uint32_t* healthy, uint32_t* truncated);
+/* make_morton32 of lbvh.cpp:109-121 for normalised points (device float[3n]). */
+UT_API int ut_morton30(ut_ctx* ctx, const float* points, uint32_t n, uint32_t* codes);
+
+/* ---- EF intersection DCD: IntersectionResolver::ef_dcd_intersection_query
+ *      (intersection_resolver.cpp:365-423, kernel :86-197) ---------------- */
+
+/* broad: (eid, fid) pairs from UT_QUERY_EDGES_UNCULLED on the face tree;
+ * broad_count: device count of them (pairs beyond broad_cap are ignored).
+ * vprop: optional per-vertex property words (bit0 fixed, bit1 rigid,
+ * bits 8.. object id).  Emits EfPair records (48 B: 5 x u32 indices, edge
+ * bary, face bary x/y, gradient[3], stiffness) and (eid, fid) into pairs /
+ * pair_ids, adding the number found to *count (device). */
+UT_API int ut_ef_dcd(ut_ctx* ctx, const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap,
+	const float* x, const uint32_t* edges, const uint32_t* faces, const uint32_t* vprop, const float* d_hat,
+	uint32_t* count, void* pairs, uint32_t* pair_ids, uint32_t max_pairs);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UNTANGLE_H */

@@ -1,0 +1,80 @@
+/*
+ * oracle.h -- CPU restatement of the Untangling26 hot path, used ONLY as the
+ * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * Nothing in the product (untangling26_amd/) links or calls it.
+ *
+ * Built by oracle/Makefile with gcc -O2 -ffp-contract=off: FP32 expressions
+ * are evaluated in the same order as the GPU kernels, with IEEE rounding and
+ * no fused multiply-add, so geometric predicates give identical bits.
+ *
+ * Each function cites the reference file:line it restates (paths inside
+ * ChengzhuUwU/Untangling26).  Pinning of each function is recorded in the
+ * header comment of its definition in the .c file.
+ */
+#ifndef UT_ORACLE_H
+#define UT_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* lbvh.cpp:84-121 make_morton32 on normalised points. */
+void or_morton30(const float* p, uint32_t n, uint32_t* out);
+
+/* Full LBVH topology: global box of leaf centroids (lbvh.cpp:400-496),
+ * Morton keys (lbvh.cpp:512-528), stable sort by 64-bit key (lbvh.cpp:1032-1040),
+ * Karras inner nodes (lbvh.cpp:206-293, 530-600).
+ * tree_type 1 = faces (prims uint32[3n]), 2 = edges (uint32[2n]).
+ * Outputs sized n / 2n-1 / 2(2n-1). */
+void or_lbvh_build(int tree_type, const float* x, const uint32_t* prims, uint32_t n, uint64_t* morton_sorted,
+	uint32_t* sorted_get_original, uint32_t* parent, uint32_t* children, uint32_t* object_idx, float* global_box);
+
+/* Karras topology from given sorted unique keys (lbvh.cpp:530-600). */
+void or_karras(const uint64_t* keys, uint32_t n, uint32_t* parent, uint32_t* children, uint32_t* object_idx);
+
+/* Box of every leaf after update_*_leave_aabb (lbvh.cpp:621-688): out float[6n]. */
+void or_leaf_boxes(int tree_type, const float* xs, const float* xe, const uint32_t* prims, uint32_t n,
+	const float* thickness, float* out);
+
+/* Query boxes (lbvh.cpp:780-894): kind 0 vertices (n = #verts), 1/2 edges. */
+void or_query_boxes(int kind, const float* xb, const float* xe, const uint32_t* edges, uint32_t n, const float* d_hat,
+	const float* thickness, float* out);
+
+/* All (query, leaf) pairs with strict box overlap (aabb.h:94-98).  cull != 0
+ * keeps only query < leaf (fn_query_from_edges_v2).  Uses a uniform grid; the
+ * result is sorted lexicographically.  Returns the number of pairs; writes at
+ * most cap of them to out (uint32[2*cap]). */
+size_t or_overlap_pairs(const float* qbox, uint32_t nq, const float* lbox, uint32_t nl, int cull, uint32_t* out,
+	size_t cap);
+/* Same set by O(nq*nl) enumeration, for pinning or_overlap_pairs. */
+size_t or_overlap_pairs_brute(const float* qbox, uint32_t nq, const float* lbox, uint32_t nl, int cull, uint32_t* out,
+	size_t cap);
+
+/* LineIntersection (intersection_resolver_helper.cpp:333-481): returns 1 on hit. */
+int or_line_intersection(const float* s, const float* e, const float* A, const float* B, const float* C, float* bary,
+	float* t);
+
+/* fn_narrow_phase_ef_dcd_query (intersection_resolver.cpp:86-197) over given
+ * broad pairs; writes EfPair records (12 words) and (eid, fid); returns count. */
+size_t or_ef_dcd(const uint32_t* broad, size_t nb, const float* x, const uint32_t* edges, const uint32_t* faces,
+	const uint32_t* vprop, const float* d_hat, uint32_t* pairs, uint32_t* pair_ids, size_t cap);
+
+/* Complete EF detection: edge query boxes vs face leaf boxes (unculled,
+ * xb = xe = x) followed by or_ef_dcd; (eid, fid) sorted.  Returns count. */
+size_t or_ef_detect(const float* x, uint32_t nv, const uint32_t* edges, uint32_t ne, const uint32_t* faces, uint32_t nf,
+	const uint32_t* vprop, const float* d_hat, const float* thickness, uint32_t* pairs, uint32_t* pair_ids, size_t cap);
+
+/* Node boxes of a Karras tree from leaf boxes (indexed by primitive id):
+ * node_box float[6(2n-1)], reference numbering. */
+void or_lbvh_node_boxes(uint32_t n, const uint32_t* children, const uint32_t* sorted_get_original,
+	const float* leaf_box, float* node_box);
+/* Stack traversal of that tree (query_template2).  Unsorted output; returns count. */
+size_t or_lbvh_query(uint32_t n, const uint32_t* children, const uint32_t* object_idx, const float* node_box,
+	const float* qbox, uint32_t nq, int cull, uint32_t* out, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,173 @@
+"""ctypes access to liboracle.so -- the CPU restatement of the reference path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the CPU baseline, never
+by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import subprocess
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+def build() -> pathlib.Path:
+    subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        _lib = C.CDLL(str(LIB))
+        L = _lib
+        p = C.c_void_p
+        L.or_morton30.argtypes = [p, C.c_uint32, p]
+        L.or_lbvh_build.argtypes = [C.c_int, p, p, C.c_uint32, p, p, p, p, p, p]
+        L.or_karras.argtypes = [p, C.c_uint32, p, p, p]
+        L.or_leaf_boxes.argtypes = [C.c_int, p, p, p, C.c_uint32, p, p]
+        L.or_query_boxes.argtypes = [C.c_int, p, p, p, C.c_uint32, p, p, p]
+        L.or_overlap_pairs.argtypes = [p, C.c_uint32, p, C.c_uint32, C.c_int, p, C.c_size_t]
+        L.or_overlap_pairs.restype = C.c_size_t
+        L.or_overlap_pairs_brute.argtypes = [p, C.c_uint32, p, C.c_uint32, C.c_int, p, C.c_size_t]
+        L.or_overlap_pairs_brute.restype = C.c_size_t
+        L.or_lbvh_node_boxes.argtypes = [C.c_uint32, p, p, p, p]
+        L.or_lbvh_query.argtypes = [C.c_uint32, p, p, p, p, C.c_uint32, C.c_int, p, C.c_size_t]
+        L.or_lbvh_query.restype = C.c_size_t
+        L.or_line_intersection.argtypes = [p, p, p, p, p, p, p]
+        L.or_line_intersection.restype = C.c_int
+        L.or_ef_dcd.argtypes = [p, C.c_size_t, p, p, p, p, p, p, p, C.c_size_t]
+        L.or_ef_dcd.restype = C.c_size_t
+        L.or_ef_detect.argtypes = [p, C.c_uint32, p, C.c_uint32, p, C.c_uint32, p, p, p, p, p, C.c_size_t]
+        L.or_ef_detect.restype = C.c_size_t
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def morton30(points):
+    points = _c(points, np.float32)
+    out = np.empty(points.shape[0], np.uint32)
+    lib().or_morton30(_p(points), points.shape[0], _p(out))
+    return out
+
+
+def lbvh_build(tree_type, x, prims):
+    x = _c(x, np.float32)
+    prims = _c(prims, np.uint32)
+    n = prims.shape[0]
+    nn = 2 * n - 1
+    ms = np.empty(n, np.uint64)
+    sgo = np.empty(n, np.uint32)
+    par = np.empty(nn, np.uint32)
+    ch = np.empty((nn, 2), np.uint32)
+    obj = np.empty(nn, np.uint32)
+    gb = np.empty(6, np.float32)
+    lib().or_lbvh_build(tree_type, _p(x), _p(prims), n, _p(ms), _p(sgo), _p(par), _p(ch), _p(obj), _p(gb))
+    return dict(morton_sorted=ms, sorted_get_original=sgo, parent=par, children=ch, object_idx=obj, global_box=gb)
+
+
+def karras(keys):
+    keys = _c(keys, np.uint64)
+    n = keys.shape[0]
+    nn = 2 * n - 1
+    par = np.empty(nn, np.uint32)
+    ch = np.empty((nn, 2), np.uint32)
+    obj = np.empty(nn, np.uint32)
+    lib().or_karras(_p(keys), n, _p(par), _p(ch), _p(obj))
+    return dict(parent=par, children=ch, object_idx=obj)
+
+
+def leaf_boxes(tree_type, xs, xe, prims, thickness):
+    prims = _c(prims, np.uint32)
+    out = np.empty((prims.shape[0], 6), np.float32)
+    lib().or_leaf_boxes(tree_type, _p(_c(xs, np.float32)), _p(_c(xe, np.float32)), _p(prims), prims.shape[0],
+                        _p(_c(thickness, np.float32)), _p(out))
+    return out
+
+
+def query_boxes(kind, xb, xe, edges, d_hat, thickness):
+    xb = _c(xb, np.float32)
+    n = xb.shape[0] if kind == 0 else edges.shape[0]
+    out = np.empty((n, 6), np.float32)
+    e = None if edges is None else _c(edges, np.uint32)
+    lib().or_query_boxes(kind, _p(xb), _p(_c(xe, np.float32)), _p(e), n, _p(_c(d_hat, np.float32)),
+                         _p(_c(thickness, np.float32)), _p(out))
+    return out
+
+
+def overlap_pairs(qbox, lbox, cull=False, brute=False):
+    qbox = _c(qbox, np.float32)
+    lbox = _c(lbox, np.float32)
+    fn = lib().or_overlap_pairs_brute if brute else lib().or_overlap_pairs
+    n = fn(_p(qbox), qbox.shape[0], _p(lbox), lbox.shape[0], int(cull), None, 0)
+    out = np.empty((max(n, 1), 2), np.uint32)
+    fn(_p(qbox), qbox.shape[0], _p(lbox), lbox.shape[0], int(cull), _p(out), n)
+    out = out[:n]
+    if brute:
+        out = out[np.lexsort((out[:, 1], out[:, 0]))]
+    return out
+
+
+def line_intersection(s, e, a, b, c):
+    bary = np.zeros(3, np.float32)
+    t = np.zeros(1, np.float32)
+    arr = [_c(v, np.float32) for v in (s, e, a, b, c)]
+    hit = lib().or_line_intersection(*(_p(v) for v in arr), _p(bary), _p(t))
+    return bool(hit), bary, float(t[0])
+
+
+def ef_dcd(broad, x, edges, faces, d_hat, vprop=None):
+    broad = _c(broad, np.uint32).reshape(-1, 2)
+    nb = broad.shape[0]
+    pairs = np.empty((max(nb, 1), 12), np.uint32)
+    ids = np.empty((max(nb, 1), 2), np.uint32)
+    n = lib().or_ef_dcd(_p(broad), nb, _p(_c(x, np.float32)), _p(_c(edges, np.uint32)), _p(_c(faces, np.uint32)),
+                        _p(None if vprop is None else _c(vprop, np.uint32)), _p(_c(d_hat, np.float32)),
+                        _p(pairs), _p(ids), nb)
+    return pairs[:n], ids[:n]
+
+
+def ef_detect(x, edges, faces, d_hat, thickness, vprop=None):
+    """Sorted (eid, fid) of all EF intersections and the matching EfPair words."""
+    x = _c(x, np.float32)
+    edges = _c(edges, np.uint32)
+    faces = _c(faces, np.uint32)
+    args = (_p(x), x.shape[0], _p(edges), edges.shape[0], _p(faces), faces.shape[0],
+            _p(None if vprop is None else _c(vprop, np.uint32)), _p(_c(d_hat, np.float32)),
+            _p(_c(thickness, np.float32)))
+    n = lib().or_ef_detect(*args, None, None, 0)
+    pairs = np.empty((max(n, 1), 12), np.uint32)
+    ids = np.empty((max(n, 1), 2), np.uint32)
+    lib().or_ef_detect(*args, _p(pairs), _p(ids), n)
+    return pairs[:n], ids[:n]
+
+
+def lbvh_query(tree, leaf_box, qbox, cull=False):
+    """CPU LBVH traversal over an or_lbvh_build tree; returns sorted pairs."""
+    n = tree["sorted_get_original"].shape[0]
+    nb = np.empty((2 * n - 1, 6), np.float32)
+    ch = _c(tree["children"], np.uint32)
+    obj = _c(tree["object_idx"], np.uint32)
+    lib().or_lbvh_node_boxes(n, _p(ch), _p(_c(tree["sorted_get_original"], np.uint32)),
+                             _p(_c(leaf_box, np.float32)), _p(nb))
+    qbox = _c(qbox, np.float32)
+    cnt = lib().or_lbvh_query(n, _p(ch), _p(obj), _p(nb), _p(qbox), qbox.shape[0], int(cull), None, 0)
+    out = np.empty((max(cnt, 1), 2), np.uint32)
+    lib().or_lbvh_query(n, _p(ch), _p(obj), _p(nb), _p(qbox), qbox.shape[0], int(cull), _p(out), cnt)
+    out = out[:cnt]
+    return out[np.lexsort((out[:, 1], out[:, 0]))]

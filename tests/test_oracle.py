@@ -1,0 +1,190 @@
+"""Pins the CPU oracle against the reference's own known answers, closed forms
+and brute force (no GPU needed)."""
+import numpy as np
+import pytest
+
+import oracle
+
+
+# lbvh.cpp:934-946: LBVH::unit_test expected Morton codes for pos = i / 10.
+ANSWER_MORTON32 = [0, 2064888, 16519104, 117698623, 132152839, 939524096, 941588984, 956043200]
+ANSWER_MORTON64 = [0, 8868626429902849, 70949011439222786, 505511736569233411, 567592121578553348,
+                   4035225266123964421, 4044093892553867270, 4106174277563187207]
+# lbvh.cpp:566-597: expected children / parents of the 8-leaf unit test tree.
+ANSWER_CHILDREN = [[4, 5], [8, 7], [9, 1], [11, 10], [2, 3], [14, 6], [13, 12]]
+ANSWER_PARENT = [0, 2, 4, 4, 0, 0, 5, 1, 1, 2, 3, 3, 6, 6, 5]
+
+
+def unit_test_points():
+    p = (np.arange(8, dtype=np.float32) / np.float32(10.0)).astype(np.float32)
+    return np.repeat(p[:, None], 3, axis=1)
+
+
+def test_morton_known_answers():
+    m = oracle.morton30(unit_test_points())
+    assert m.tolist() == ANSWER_MORTON32
+    keys = (m.astype(np.uint64) << np.uint64(32)) | np.arange(8, dtype=np.uint64)
+    assert keys.tolist() == ANSWER_MORTON64
+
+
+def test_karras_known_answers():
+    t = oracle.karras(np.array(ANSWER_MORTON64, dtype=np.uint64))
+    assert t["children"][:7].tolist() == ANSWER_CHILDREN
+    assert t["parent"][1:].tolist() == ANSWER_PARENT[1:]
+    assert t["parent"][0] == 0xFFFFFFFF  # root has no parent (fn_reset_tree, lbvh.cpp:503)
+    assert t["object_idx"][7:].tolist() == list(range(8))
+
+
+def _karras_invariants(t, n):
+    par, ch = t["parent"], t["children"]
+    ni = n - 1
+    seen = np.zeros(2 * n - 1, int)
+    for i in range(ni):
+        for c in ch[i]:
+            assert par[c] == i
+            seen[c] += 1
+    assert seen[0] == 0 and (seen[1:] == 1).all()
+
+    def leaves(node):
+        if node >= ni:
+            return [node - ni]
+        return leaves(ch[node][0]) + leaves(ch[node][1])
+
+    got = leaves(0)
+    assert sorted(got) == list(range(n))
+    # every subtree covers a contiguous range of sorted leaves
+    for i in range(ni):
+        lv = sorted(leaves(i))
+        assert lv == list(range(lv[0], lv[-1] + 1))
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 17, 64, 257])
+def test_karras_invariants_random(n):
+    rng = np.random.default_rng(n)
+    codes = rng.integers(0, 1 << 30, size=n).astype(np.uint64)
+    codes[: n // 3] = codes[0]  # duplicated Morton codes are separated by the index bits
+    keys = np.sort((codes << np.uint64(32)) | np.arange(n, dtype=np.uint64))
+    _karras_invariants(oracle.karras(keys), n)
+
+
+def test_lbvh_build_sorted_and_consistent():
+    from untangling26_amd import scenes
+
+    m = scenes.two_sheets(3)
+    t = oracle.lbvh_build(1, m.x, m.faces)
+    ms = t["morton_sorted"]
+    assert (np.diff(ms.astype(np.float64)) > 0).all() or (ms[1:] > ms[:-1]).all()
+    assert sorted(t["sorted_get_original"].tolist()) == list(range(m.nf))
+    _karras_invariants(t, m.nf)
+    lo = np.minimum.reduce(m.x[m.faces.ravel()], 0)
+    hi = np.maximum.reduce(m.x[m.faces.ravel()], 0)
+    assert np.array_equal(t["global_box"], np.concatenate([lo, hi]).astype(np.float32))
+
+
+@pytest.mark.parametrize("cull", [False, True])
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_overlap_grid_equals_brute(seed, cull):
+    rng = np.random.default_rng(seed)
+    nq, nl = 300, 400
+    c = rng.uniform(0, 1, (nq, 3)).astype(np.float32)
+    q = np.concatenate([c - rng.uniform(0, 0.05, (nq, 3)), c + rng.uniform(0, 0.05, (nq, 3))], 1).astype(np.float32)
+    c = rng.uniform(0, 1, (nl, 3)).astype(np.float32)
+    lb = np.concatenate([c - rng.uniform(0, 0.1, (nl, 3)), c + rng.uniform(0, 0.1, (nl, 3))], 1).astype(np.float32)
+    # touching boxes: strict comparisons must reject them
+    lb[0, :3] = q[0, 3:]
+    lb[0, 3:] = q[0, 3:] + 0.1
+    lb[1] = q[1]  # identical boxes overlap
+    g = oracle.overlap_pairs(q, lb, cull)
+    b = oracle.overlap_pairs(q, lb, cull, brute=True)
+    assert np.array_equal(g, b)
+    assert len(b) > 100
+    assert not ((b[:, 0] == 0) & (b[:, 1] == 0)).any()
+
+
+def test_overlap_grid_equals_brute_on_sheets():
+    from untangling26_amd import scenes
+
+    m = scenes.two_sheets(1, n=12)
+    d, th = scenes.per_vertex(m)
+    q = oracle.query_boxes(2, m.x, m.x, m.edges, d, th)
+    lb = oracle.leaf_boxes(1, m.x, m.x, m.faces, th)
+    assert np.array_equal(oracle.overlap_pairs(q, lb), oracle.overlap_pairs(q, lb, brute=True))
+
+
+def test_line_intersection_closed_form():
+    A, B, C = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    # vertical segment through (0.25, 0.25) from z=-1 to z=3: t = 0.25, bary = (0.5, 0.25, 0.25)
+    hit, bary, t = oracle.line_intersection([0.25, 0.25, -1], [0.25, 0.25, 3], A, B, C)
+    assert hit and t == 0.25
+    assert np.allclose(bary, [0.5, 0.25, 0.25], atol=1e-7)
+    # misses: outside the triangle, parallel, short of the plane, degenerate
+    assert not oracle.line_intersection([0.8, 0.8, -1], [0.8, 0.8, 1], A, B, C)[0]
+    assert not oracle.line_intersection([0.2, 0.2, 1], [0.3, 0.2, 1], A, B, C)[0]
+    assert not oracle.line_intersection([0.2, 0.2, 1], [0.2, 0.2, 2], A, B, C)[0]
+    assert not oracle.line_intersection([0.2, 0.2, 0], [0.2, 0.2, 0], A, B, C)[0]
+    # reversed segment: t measured from the start
+    hit, _, t = oracle.line_intersection([0.25, 0.25, 3], [0.25, 0.25, -1], A, B, C)
+    assert hit and t == 0.75
+
+
+def _one_ef(p0, p1):
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], p0, p1], np.float32)
+    faces = np.array([[0, 1, 2]], np.uint32)
+    edges = np.array([[3, 4]], np.uint32)
+    d_hat = np.full(5, 1e-3, np.float32)
+    return oracle.ef_dcd(np.array([[0, 0]], np.uint32), x, edges, faces, d_hat)
+
+
+def test_ef_filters():
+    pairs, ids = _one_ef([0.25, 0.25, -1], [0.25, 0.25, 1])
+    assert ids.tolist() == [[0, 0]]
+    w = pairs[0]
+    assert w[:5].tolist() == [3, 4, 0, 1, 2]
+    f = w[5:8].view(np.float32)
+    assert f[0] == np.float32(0.5) and np.allclose(f[1:], [0.5, 0.25], atol=1e-7)
+    # penetration of the shallow end below 1e-4 * sqrt(|N|) = 1e-4 -> rejected (grazing)
+    assert len(_one_ef([0.25, 0.25, -5e-5], [0.25, 0.25, 1])[1]) == 0
+    assert len(_one_ef([0.25, 0.25, -2e-4], [0.25, 0.25, 1])[1]) == 1
+    # face barycentric below 1e-4 -> rejected
+    assert len(_one_ef([0.99995e-4 / 2, 0.5, -1], [0.99995e-4 / 2, 0.5, 1])[1]) == 0
+    # shared vertex -> rejected before the test
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0.25, 0.25, 1]], np.float32)
+    p, i = oracle.ef_dcd(np.array([[0, 0]], np.uint32), x, np.array([[0, 3]], np.uint32),
+                         np.array([[0, 1, 2]], np.uint32), np.full(4, 1e-3, np.float32))
+    assert len(i) == 0
+
+
+def test_ef_detect_two_sheets_nonempty_and_consistent():
+    from untangling26_amd import scenes
+
+    m = scenes.two_sheets(0)
+    d, th = scenes.per_vertex(m)
+    pairs, ids = oracle.ef_detect(m.x, m.edges, m.faces, d, th)
+    assert len(ids) > 10
+    # brute force over all edges x faces gives the same set
+    ne, nf = m.ne, m.nf
+    rng = np.random.default_rng(0)
+    sel = np.unique(ids[:, 0])
+    broad = np.stack(np.meshgrid(sel, np.arange(nf), indexing="ij"), -1).reshape(-1, 2).astype(np.uint32)
+    _, bid = oracle.ef_dcd(broad, m.x, m.edges, m.faces, d)
+    assert np.array_equal(bid, ids)
+    # edges of one sheet only cut faces of the other sheet
+    half_e = m.edges[ids[:, 0], 0] < 32 * 32
+    half_f = ids[:, 1] < m.nf // 2
+    assert (half_e != half_f).all()
+    del rng, ne
+
+
+@pytest.mark.parametrize("kind,tree_type,cull", [(0, 1, False), (1, 2, True), (2, 1, False)])
+def test_cpu_lbvh_traversal_equals_grid(kind, tree_type, cull):
+    from untangling26_amd import scenes
+
+    m = scenes.two_sheets(2)
+    d, th = scenes.per_vertex(m, thickness=1e-4)
+    prims = m.faces if tree_type == 1 else m.edges
+    tree = oracle.lbvh_build(tree_type, m.x, prims)
+    lb = oracle.leaf_boxes(tree_type, m.x, m.x, prims, th)
+    qb = oracle.query_boxes(0 if kind == 0 else 1, m.x, m.x, None if kind == 0 else m.edges, d, th)
+    a = oracle.lbvh_query(tree, lb, qb, cull)
+    b = oracle.overlap_pairs(qb, lb, cull)
+    assert np.array_equal(a, b) and len(a) > 0

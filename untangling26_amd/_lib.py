@@ -1,0 +1,78 @@
+"""ctypes binding of libuntangle_hip.so (the C ABI declared in include/untangle.h).
+
+The shared library is the only compute path.  If it is missing, importing the
+package still works (so CPU-only tooling can inspect it), but every call that
+needs the library raises ``RuntimeError`` -- there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+
+_PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = pathlib.Path(os.environ.get("UT_LIB_PATH", _PKG / "libuntangle_hip.so"))
+
+_lib = None
+
+u32p = C.POINTER(C.c_uint32)
+vp = C.c_void_p
+
+# name -> (restype, argtypes); mirrors include/untangle.h
+SIGNATURES = {
+    "ut_version": (C.c_int, []),
+    "ut_ctx_create": (C.c_int, [C.c_int, vp, C.POINTER(vp)]),
+    "ut_ctx_destroy": (C.c_int, [vp]),
+    "ut_last_error": (C.c_char_p, [vp]),
+    "ut_ctx_synchronize": (C.c_int, [vp]),
+    "ut_ctx_stream": (vp, [vp]),
+    "ut_lbvh_create": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(vp)]),
+    "ut_lbvh_destroy": (C.c_int, [vp]),
+    "ut_lbvh_reduce_aabb": (C.c_int, [vp, vp, vp]),
+    "ut_lbvh_construct_tree": (C.c_int, [vp]),
+    "ut_lbvh_build_from_sorted_keys": (C.c_int, [vp, vp]),
+    "ut_lbvh_refit": (C.c_int, [vp, vp, vp, vp, vp]),
+    "ut_lbvh_query": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_uint32, vp, vp, vp, vp, C.c_uint32]),
+    "ut_lbvh_tree_arrays": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "ut_lbvh_boxes": (C.c_int, [vp, vp, vp]),
+    "ut_lbvh_status": (C.c_int, [vp, u32p, u32p]),
+    "ut_morton30": (C.c_int, [vp, vp, C.c_uint32, vp]),
+    "ut_ef_dcd": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_uint32]),
+}
+
+
+def lib():
+    """Loads the library once; raises RuntimeError if it was not built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} not found: build it with `python -m untangling26_amd.build` "
+                "(there is no CPU fallback)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None passes NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+class UntangleError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"untangle error {code}: {msg}")
+        self.code = code
+
+
+def check(rc: int, ctx_handle) -> int:
+    if rc < 0:
+        msg = lib().ut_last_error(ctx_handle)
+        raise UntangleError(rc, msg.decode() if msg else "")
+    return rc

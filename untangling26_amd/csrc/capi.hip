@@ -1,0 +1,319 @@
+// extern "C" boundary of libuntangle_hip.  Each entry point converts C++
+// exceptions into status codes and records the message on the context.
+#include "../../include/untangle.h"
+#include "ut_common.hpp"
+#include "ut_runtime.hpp"
+#include "lbvh.hpp"
+#include "ef_dcd.hpp"
+
+#include <new>
+
+struct ut_ctx
+{
+	ut::Context c;
+};
+struct ut_lbvh
+{
+	ut::Lbvh b;
+};
+
+namespace
+{
+template <typename F>
+int guarded(ut::Context* c, F&& f)
+{
+	try
+	{
+		if (c)
+			c->activate();
+		f();
+		return ut::kOk;
+	}
+	catch (const ut::Error& e)
+	{
+		if (c)
+			c->last_error = e.what();
+		return e.code;
+	}
+	catch (const std::exception& e)
+	{
+		if (c)
+			c->last_error = e.what();
+		return ut::kInternal;
+	}
+	catch (...)
+	{
+		if (c)
+			c->last_error = "unknown exception";
+		return ut::kInternal;
+	}
+}
+
+__global__ void k_morton_points(const float* __restrict__ p, uint32_t n, uint32_t* __restrict__ out)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	auto expand = [](uint32_t v)
+	{
+		v = (v | (v << 16)) & 0x030000FFu;
+		v = (v | (v << 8)) & 0x0300F00Fu;
+		v = (v | (v << 4)) & 0x030C30C3u;
+		return (v | (v << 2)) & 0x09249249u;
+	};
+	const float x = fmaxf(fminf(p[3 * i + 0] * 1024.0f, 1023.0f), 0.0f);
+	const float y = fmaxf(fminf(p[3 * i + 1] * 1024.0f, 1023.0f), 0.0f);
+	const float z = fmaxf(fminf(p[3 * i + 2] * 1024.0f, 1023.0f), 0.0f);
+	out[i] = (expand((uint32_t)x) << 2) | (expand((uint32_t)y) << 1) | expand((uint32_t)z);
+}
+
+__global__ void k_split_keys(const uint64_t* __restrict__ k, uint32_t n, uint32_t* __restrict__ hi, uint32_t* __restrict__ lo)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	hi[i] = (uint32_t)(k[i] >> 32);
+	lo[i] = (uint32_t)(k[i] & 0xFFFFFFFFull);
+}
+} // namespace
+
+extern "C" {
+
+UT_API int ut_version(void) { return 1; }
+
+UT_API int ut_ctx_create(int device, void* hip_stream, ut_ctx** out)
+{
+	if (!out)
+		return ut::kInvalid;
+	*out = nullptr;
+	ut_ctx* h = new (std::nothrow) ut_ctx();
+	if (!h)
+		return ut::kInternal;
+	int rc = guarded(nullptr,
+		[&]
+		{
+			int n = 0;
+			UT_HIP_CHECK(hipGetDeviceCount(&n));
+			UT_REQUIRE(device >= 0 && device < n, "no such HIP device");
+			h->c.device = device;
+			UT_HIP_CHECK(hipSetDevice(device));
+			if (hip_stream)
+				h->c.stream = reinterpret_cast<hipStream_t>(hip_stream);
+			else
+			{
+				UT_HIP_CHECK(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
+				h->c.own_stream = true;
+			}
+		});
+	if (rc != ut::kOk)
+	{
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return ut::kOk;
+}
+
+UT_API int ut_ctx_destroy(ut_ctx* ctx)
+{
+	if (!ctx)
+		return ut::kOk;
+	if (ctx->c.own_stream && ctx->c.stream)
+	{
+		(void)hipSetDevice(ctx->c.device);
+		(void)hipStreamSynchronize(ctx->c.stream);
+		(void)hipStreamDestroy(ctx->c.stream);
+	}
+	delete ctx;
+	return ut::kOk;
+}
+
+UT_API const char* ut_last_error(const ut_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "null context"; }
+
+UT_API int ut_ctx_synchronize(ut_ctx* ctx)
+{
+	if (!ctx)
+		return ut::kInvalid;
+	return guarded(&ctx->c, [&] { UT_HIP_CHECK(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+UT_API void* ut_ctx_stream(ut_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+// ---- LBVH ----------------------------------------------------------------
+UT_API int ut_lbvh_create(ut_ctx* ctx, int tree_type, uint32_t num_leaves, ut_lbvh** out)
+{
+	if (!ctx || !out)
+		return ut::kInvalid;
+	*out = nullptr;
+	ut_lbvh* h = new (std::nothrow) ut_lbvh();
+	if (!h)
+		return ut::kInternal;
+	int rc = guarded(&ctx->c, [&] { h->b.init(&ctx->c, tree_type, num_leaves); });
+	if (rc != ut::kOk)
+	{
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return ut::kOk;
+}
+
+UT_API int ut_lbvh_destroy(ut_lbvh* bvh)
+{
+	if (bvh)
+	{
+		if (bvh->b.ctx)
+			(void)hipStreamSynchronize(bvh->b.ctx->stream);
+		delete bvh;
+	}
+	return ut::kOk;
+}
+
+UT_API int ut_lbvh_reduce_aabb(ut_lbvh* bvh, const float* x, const uint32_t* prims)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx, [&] { bvh->b.reduce_aabb(x, prims); });
+}
+
+UT_API int ut_lbvh_construct_tree(ut_lbvh* bvh)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx, [&] { bvh->b.construct_tree(); });
+}
+
+UT_API int ut_lbvh_build_from_sorted_keys(ut_lbvh* bvh, const uint64_t* keys)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx,
+		[&]
+		{
+			UT_REQUIRE(keys, "null keys");
+			auto&		   b = bvh->b;
+			const uint32_t n = b.num_leaves;
+			b.sort.result = 0;
+			k_split_keys<<<ut::div_up(n, 256), 256, 0, b.ctx->stream>>>(keys, n, b.sort.keys[0], b.sort.vals[0]);
+			UT_LAUNCH_CHECK();
+			b.construct_tree_presorted();
+		});
+}
+
+UT_API int ut_lbvh_refit(ut_lbvh* bvh, const float* thickness, const float* xs, const float* xe, const uint32_t* prims)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx, [&] { bvh->b.update_leaves_and_refit(thickness, xs, xe, prims); });
+}
+
+UT_API int ut_lbvh_query(ut_lbvh* bvh, int kind, const float* xb, const float* xe, const uint32_t* edges, uint32_t nq,
+	const float* d_hat, const float* thickness, uint32_t* count, uint32_t* list, uint32_t max_count)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx,
+		[&] { bvh->b.query(kind, xb, xe, edges, nq, d_hat, thickness, count, list, max_count); });
+}
+
+UT_API int ut_lbvh_tree_arrays(ut_lbvh* bvh, uint64_t* morton_sorted, uint32_t* sorted_get_original, uint32_t* parent,
+	uint32_t* children, uint32_t* object_idx)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx,
+		[&]
+		{
+			auto&		   b = bvh->b;
+			const size_t   n = b.num_leaves, nn = 2 * n - 1;
+			hipStream_t	   s = b.ctx->stream;
+			const auto	   d2d = hipMemcpyDeviceToDevice;
+			if (morton_sorted)
+				UT_HIP_CHECK(hipMemcpyAsync(morton_sorted, b.morton_sorted.get(), n * 8, d2d, s));
+			if (sorted_get_original)
+				UT_HIP_CHECK(hipMemcpyAsync(sorted_get_original, b.sorted_get_original.get(), n * 4, d2d, s));
+			if (parent)
+				UT_HIP_CHECK(hipMemcpyAsync(parent, b.parent.get(), nn * 4, d2d, s));
+			if (children)
+				UT_HIP_CHECK(hipMemcpyAsync(children, b.children.get(), nn * 8, d2d, s));
+			if (object_idx)
+				UT_HIP_CHECK(hipMemcpyAsync(object_idx, b.object_idx.get(), nn * 4, d2d, s));
+		});
+}
+
+UT_API int ut_lbvh_boxes(ut_lbvh* bvh, float* root_box, float* global_box)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx,
+		[&]
+		{
+			auto&		b = bvh->b;
+			hipStream_t s = b.ctx->stream;
+			if (root_box)
+				UT_HIP_CHECK(hipMemcpyAsync(root_box, b.root_box.get(), 24, hipMemcpyDeviceToDevice, s));
+			if (global_box)
+				UT_HIP_CHECK(hipMemcpyAsync(global_box, b.global_box.get(), 24, hipMemcpyDeviceToDevice, s));
+		});
+}
+
+UT_API int ut_lbvh_status(ut_lbvh* bvh, uint32_t* healthy, uint32_t* truncated)
+{
+	if (!bvh)
+		return ut::kInvalid;
+	return guarded(bvh->b.ctx,
+		[&]
+		{
+			uint32_t st[2] = { 0, 0 };
+			UT_HIP_CHECK(hipMemcpyAsync(st, bvh->b.status.get(), 8, hipMemcpyDeviceToHost, bvh->b.ctx->stream));
+			UT_HIP_CHECK(hipStreamSynchronize(bvh->b.ctx->stream));
+			if (healthy)
+				*healthy = st[0];
+			if (truncated)
+				*truncated = st[1];
+		});
+}
+
+UT_API int ut_morton30(ut_ctx* ctx, const float* points, uint32_t n, uint32_t* codes)
+{
+	if (!ctx)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			UT_REQUIRE(points && codes, "null pointer");
+			if (n == 0)
+				return;
+			k_morton_points<<<ut::div_up(n, 256), 256, 0, ctx->c.stream>>>(points, n, codes);
+			UT_LAUNCH_CHECK();
+		});
+}
+
+// ---- EF DCD ----------------------------------------------------------------
+UT_API int ut_ef_dcd(ut_ctx* ctx, const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap,
+	const float* x, const uint32_t* edges, const uint32_t* faces, const uint32_t* vprop, const float* d_hat,
+	uint32_t* count, void* pairs, uint32_t* pair_ids, uint32_t max_pairs)
+{
+	if (!ctx)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			ut::EfDcdArgs a;
+			a.broad = broad;
+			a.broad_count = broad_count;
+			a.broad_cap = broad_cap;
+			a.x = x;
+			a.edges = edges;
+			a.faces = faces;
+			a.vprop = vprop;
+			a.d_hat = d_hat;
+			a.count = count;
+			a.pairs = reinterpret_cast<uint32_t*>(pairs);
+			a.pair_ids = pair_ids;
+			a.max_pairs = max_pairs;
+			ut::ef_dcd(&ctx->c, a);
+		});
+}
+
+} // extern "C"
