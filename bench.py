@@ -6,8 +6,8 @@ A step is one pass of the per-step collision pipeline over the folded
 256x256-vertex cloth (130,050 triangles), inputs resident in HBM:
   face + edge LBVH rebuild (global box, Morton, radix sort, Karras),
   leaf update + refit of both trees,
-  VF, EE and unculled EF broad-phase queries,
-  EF intersection narrow phase.
+  VF and EE broad-phase queries,
+  EF intersection detection (edge-vs-face traversal fused with the EF test).
 Prints one JSON line (see the "roofline" and "cpu_baseline" keys).
 """
 from __future__ import annotations
@@ -47,7 +47,7 @@ def main():
 
     from untangling26_amd import scenes
     from untangling26_amd.lbvh import LBVH, TREE_EDGE, TREE_FACE
-    from untangling26_amd.intersection import ef_dcd_intersection_query
+    from untangling26_amd.intersection import ef_detect_fused
     from untangling26_amd.runtime import Context
 
     ctx = Context(0)
@@ -65,7 +65,7 @@ def main():
     face_tree = LBVH(ctx, TREE_FACE, mesh.nf)
     edge_tree = LBVH(ctx, TREE_EDGE, mesh.ne)
     counts = torch.zeros(8, dtype=torch.int32, device="cuda")  # vf, ee, ef broad, ef narrow
-    caps = {"vf": 1 << 20, "ee": 1 << 20, "ef": 1 << 20, "efn": 1 << 18}
+    caps = {"vf": 1 << 20, "ee": 1 << 21, "ef": 1 << 22, "efn": 1 << 18}
     bufs = {}
 
     def alloc():
@@ -90,13 +90,13 @@ def main():
         edge_tree.refit()
 
     def st_query():
-        counts.zero_()
+        counts[0:2].zero_()
         face_tree.broad_phase_query_from_verts(X, X, counts[0:1], bufs["vf"], D, T)
         edge_tree.broad_phase_query_from_edges(X, X, E, counts[1:2], bufs["ee"], D, T)
-        face_tree.broad_phase_query_from_edges_unculled(X, X, E, counts[2:3], bufs["ef"], D, T)
 
     def st_ef():
-        ef_dcd_intersection_query(ctx, bufs["ef"], counts[2:3], X, E, F, D, counts[3:4], bufs["efp"], bufs["efi"])
+        counts[3:4].zero_()
+        ef_detect_fused(ctx, face_tree, X, E, F, D, T, counts[3:4], bufs["efp"], bufs["efi"])
 
     def step():
         st_build()
@@ -108,14 +108,18 @@ def main():
     for _ in range(4):
         step()
         c = counts.cpu().tolist()
-        need = {"vf": c[0], "ee": c[1], "ef": c[2], "efn": c[3]}
+        need = {"vf": c[0], "ee": c[1], "efn": c[3]}
         if all(need[k] <= caps[k] for k in need):
             break
         for k in need:
             caps[k] = max(caps[k], 2 * need[k])
         alloc()
     c = counts.cpu().tolist()
-    n_vf, n_ee, n_ef_broad, n_ef = c[0], c[1], c[2], c[3]
+    n_vf, n_ee, n_ef = c[0], c[1], c[3]
+    # candidate count of the fused EF kernel (for its byte count): one broad query
+    counts[2:3].zero_()
+    face_tree.broad_phase_query_from_edges_unculled(X, X, E, counts[2:3], bufs["ef"], D, T)
+    n_ef_broad = int(counts[2].item())
 
     for _ in range(args.warmup):
         step()
@@ -144,10 +148,12 @@ def main():
 
     stage_ms = {"build": timed(st_build), "refit": timed(st_refit), "query": timed(st_query), "ef_dcd": timed(st_ef)}
 
-    # Roofline of the EF narrow-phase kernel: per broad pair 8 B pair + 5 x 12 B
-    # vertex positions + 8 B edge + 12 B face topology = 88 B, plus 56 B per
-    # emitted pair (48 B EfPair + 8 B ids).
-    ef_bytes = 88.0 * n_ef_broad + 56.0 * n_ef
+    # Roofline of the fused EF kernel (broad traversal + narrow test): per
+    # candidate pair 5 x 12 B vertex positions + 8 B edge + 12 B face topology
+    # = 80 B, per edge query 8 B edge + 24 B positions + 8 B d_hat/thickness,
+    # plus 56 B per emitted pair (48 B EfPair + 8 B ids); node visits (64 B)
+    # hit in L2 and are not counted.
+    ef_bytes = 80.0 * n_ef_broad + 40.0 * mesh.ne + 56.0 * n_ef
     ef_ms = stage_ms["ef_dcd"]
     achieved = ef_bytes / (ef_ms * 1e-3) / 1e9
 
@@ -156,7 +162,7 @@ def main():
         cpu = cpu_baseline(mesh, d_hat_np, th_np)
 
     out = {
-        "metric": "steps/s (per-step collision pipeline: LBVH rebuild+refit, VF/EE/EF queries, EF DCD)",
+        "metric": "steps/s (per-step collision pipeline: LBVH rebuild+refit, VF/EE queries, fused EF detection)",
         "value": 1000.0 / ms,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -172,7 +178,7 @@ def main():
                    "vf_pairs": n_vf, "ee_pairs": n_ee, "ef_broad_pairs": n_ef_broad, "ef_pairs": n_ef,
                    "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_ef_dcd"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_ef_detect"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

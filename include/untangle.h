@@ -38,10 +38,13 @@ typedef struct ut_lbvh ut_lbvh;
 
 /* ABI version of this header. */
 UT_API int ut_version(void);
-/* Creates a context on HIP device `device`.  `hip_stream` is an existing
- * hipStream_t to launch on, or NULL to create an owned stream.  Replaces the
- * luisa::compute::Device/Stream pair of solver_interface.h:35-67. */
-UT_API int ut_ctx_create(int device, void* hip_stream, ut_ctx** out);
+enum { UT_CTX_OWN_STREAM = 1 };
+/* Creates a context on HIP device `device`.  Kernels are launched on
+ * `hip_stream` (a hipStream_t; NULL is the device's default stream) unless
+ * flags has UT_CTX_OWN_STREAM, in which case the context creates and owns a
+ * non-blocking stream.  Replaces the luisa::compute::Device/Stream pair of
+ * solver_interface.h:35-67. */
+UT_API int ut_ctx_create(int device, void* hip_stream, int flags, ut_ctx** out);
 UT_API int ut_ctx_destroy(ut_ctx* ctx);
 UT_API const char* ut_last_error(const ut_ctx* ctx);
 /* Blocks until all work on the context stream is done. */
@@ -102,6 +105,15 @@ UT_API int ut_morton30(ut_ctx* ctx, const float* points, uint32_t n, uint32_t* c
 UT_API int ut_ef_dcd(ut_ctx* ctx, const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap,
 	const float* x, const uint32_t* edges, const uint32_t* faces, const uint32_t* vprop, const float* d_hat,
 	uint32_t* count, void* pairs, uint32_t* pair_ids, uint32_t max_pairs);
+
+/* Fused form of broad_phase_query_from_edges_unculled + fn_narrow_phase_ef_dcd_query
+ * (newton_solver.cpp:1283-1297 followed by intersection_resolver.cpp:365-423):
+ * traverses the refit face tree with every edge's box and runs the EF test on
+ * each candidate in the same kernel, so the broad EF list is never stored.
+ * Same output set as the two-step form. */
+UT_API int ut_ef_detect(ut_lbvh* face_tree, const float* x, const uint32_t* edges, uint32_t num_edges,
+	const uint32_t* faces, const uint32_t* vprop, const float* d_hat, const float* thickness, uint32_t* count,
+	void* pairs, uint32_t* pair_ids, uint32_t max_pairs);
 
 #ifdef __cplusplus
 }

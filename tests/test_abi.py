@@ -41,5 +41,5 @@ def test_no_device_gives_clean_error():
     from untangling26_amd import _lib
 
     h = ctypes.c_void_p()
-    rc = _lib.lib().ut_ctx_create(0, None, ctypes.byref(h))
+    rc = _lib.lib().ut_ctx_create(0, None, 0, ctypes.byref(h))
     assert rc < 0 and not h.value

@@ -171,13 +171,14 @@ def test_query_capacity_overflow_reports_true_count(ctx):
     assert (small >= 0).all()
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("name", ["two_sheets", "two_sheets_s7", "folded_cloth", "nested_layers"])
-def test_ef_detect_matches_oracle(ctx, name):
+def test_ef_detect_matches_oracle(ctx, name, fused):
     m = _mesh(name)
     d_hat, th = scenes.per_vertex(m)
     want_pairs, want_ids = oracle.ef_detect(m.x, m.edges, m.faces, d_hat, th)
     det = EfDetector(ctx, m.nv, dev(m.edges), dev(m.faces), broad_cap=1024, ef_cap=64)  # forces resize-retry
-    n = det.detect(dev(m.x), dev(d_hat), dev(th))
+    n = det.detect(dev(m.x), dev(d_hat), dev(th), fused=fused)
     assert n == len(want_ids) > 0
     ids = host_u32(det.pair_ids[: 2 * n]).reshape(-1, 2)
     order = np.lexsort((ids[:, 1], ids[:, 0]))

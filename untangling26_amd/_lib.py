@@ -21,7 +21,7 @@ vp = C.c_void_p
 # name -> (restype, argtypes); mirrors include/untangle.h
 SIGNATURES = {
     "ut_version": (C.c_int, []),
-    "ut_ctx_create": (C.c_int, [C.c_int, vp, C.POINTER(vp)]),
+    "ut_ctx_create": (C.c_int, [C.c_int, vp, C.c_int, C.POINTER(vp)]),
     "ut_ctx_destroy": (C.c_int, [vp]),
     "ut_last_error": (C.c_char_p, [vp]),
     "ut_ctx_synchronize": (C.c_int, [vp]),
@@ -38,6 +38,7 @@ SIGNATURES = {
     "ut_lbvh_status": (C.c_int, [vp, u32p, u32p]),
     "ut_morton30": (C.c_int, [vp, vp, C.c_uint32, vp]),
     "ut_ef_dcd": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_uint32]),
+    "ut_ef_detect": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, C.c_uint32]),
 }
 
 

@@ -81,7 +81,7 @@ extern "C" {
 
 UT_API int ut_version(void) { return 1; }
 
-UT_API int ut_ctx_create(int device, void* hip_stream, ut_ctx** out)
+UT_API int ut_ctx_create(int device, void* hip_stream, int flags, ut_ctx** out)
 {
 	if (!out)
 		return ut::kInvalid;
@@ -97,13 +97,13 @@ UT_API int ut_ctx_create(int device, void* hip_stream, ut_ctx** out)
 			UT_REQUIRE(device >= 0 && device < n, "no such HIP device");
 			h->c.device = device;
 			UT_HIP_CHECK(hipSetDevice(device));
-			if (hip_stream)
-				h->c.stream = reinterpret_cast<hipStream_t>(hip_stream);
-			else
+			if (flags & UT_CTX_OWN_STREAM)
 			{
 				UT_HIP_CHECK(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
 				h->c.own_stream = true;
 			}
+			else
+				h->c.stream = reinterpret_cast<hipStream_t>(hip_stream);
 		});
 	if (rc != ut::kOk)
 	{
@@ -313,6 +313,29 @@ UT_API int ut_ef_dcd(ut_ctx* ctx, const uint32_t* broad, const uint32_t* broad_c
 			a.pair_ids = pair_ids;
 			a.max_pairs = max_pairs;
 			ut::ef_dcd(&ctx->c, a);
+		});
+}
+
+UT_API int ut_ef_detect(ut_lbvh* face_tree, const float* x, const uint32_t* edges, uint32_t num_edges,
+	const uint32_t* faces, const uint32_t* vprop, const float* d_hat, const float* thickness, uint32_t* count,
+	void* pairs, uint32_t* pair_ids, uint32_t max_pairs)
+{
+	if (!face_tree)
+		return ut::kInvalid;
+	return guarded(face_tree->b.ctx,
+		[&]
+		{
+			ut::EfDcdArgs a;
+			a.x = x;
+			a.edges = edges;
+			a.faces = faces;
+			a.vprop = vprop;
+			a.d_hat = d_hat;
+			a.count = count;
+			a.pairs = reinterpret_cast<uint32_t*>(pairs);
+			a.pair_ids = pair_ids;
+			a.max_pairs = max_pairs;
+			ut::ef_detect(face_tree->b.ctx, face_tree->b, a, num_edges, thickness);
 		});
 }
 

@@ -7,6 +7,7 @@
 #include "ut_common.hpp"
 #include "ut_runtime.hpp"
 #include "ef_dcd.hpp"
+#include "lbvh_traverse.hpp"
 
 namespace ut
 {
@@ -144,6 +145,77 @@ __global__ __launch_bounds__(256) void k_ef_dcd(const uint2* __restrict__ broad,
 		}
 		ef_emit(hit, rec, p.x, p.y, count, pairs, pair_ids, max_pairs);
 	}
+}
+
+// Fused broad + narrow phase: unculled edge-vs-face traversal whose staged
+// candidates go straight into the EF test, so the broad list is never written.
+struct EfNarrow
+{
+	const float*	x;
+	const uint32_t* edges;
+	const uint32_t* faces;
+	const uint32_t* vprop;
+	const float*	d_hat;
+	uint32_t*		count;
+	EfPairRec*		pairs;
+	uint2*			pair_ids;
+	uint32_t		max_pairs;
+	__device__ __forceinline__ void operator()(bool valid, uint32_t eid, uint32_t fid)
+	{
+		EfPairRec rec;
+		const bool hit = valid && ef_test(eid, fid, x, edges, faces, vprop, d_hat, &rec);
+		ef_emit(hit, rec, eid, fid, count, pairs, pair_ids, max_pairs);
+	}
+};
+
+__global__ __launch_bounds__(kTravThreads) void k_ef_detect(uint32_t ne,
+	const float* __restrict__ x,
+	const uint32_t* __restrict__ edges,
+	const uint32_t* __restrict__ faces,
+	const uint32_t* __restrict__ vprop,
+	const float* __restrict__ d_hat,
+	const float* __restrict__ thickness,
+	const BvhNode* __restrict__ nodes,
+	uint32_t* __restrict__ status,
+	uint32_t* __restrict__ spill,
+	uint32_t max_visits,
+	uint32_t* __restrict__ count,
+	EfPairRec* __restrict__ pairs,
+	uint2* __restrict__ pair_ids,
+	uint32_t max_pairs)
+{
+	__shared__ TravShared sh;
+	const uint32_t		  q = blockIdx.x * kTravThreads + threadIdx.x;
+	const bool			  active = q < ne;
+	const box3 qb = active ? query_box(kQueryEdgesUnculled, q, x, x, edges, d_hat, thickness) : box_empty();
+	EfNarrow   nar{ x, edges, faces, vprop, d_hat, count, pairs, pair_ids, max_pairs };
+	traverse_staged(sh, active, q, qb, false, nodes, status, spill, max_visits, nar);
+}
+
+void ef_detect(Context* ctx, Lbvh& face_tree, const EfDcdArgs& a, uint32_t num_edges, const float* thickness)
+{
+	UT_REQUIRE(face_tree.type == kTreeFace, "ef_detect needs the face tree");
+	UT_REQUIRE(a.x && a.edges && a.faces && a.d_hat && a.count && thickness, "ef_detect: null input");
+	UT_REQUIRE(a.max_pairs == 0 || (a.pairs && a.pair_ids), "ef_detect: null output");
+	if (num_edges == 0)
+		return;
+	face_tree.spill.reserve((size_t)num_edges * (kStackTotal - kStackLds));
+	k_ef_detect<<<div_up(num_edges, kTravThreads), kTravThreads, 0, ctx->stream>>>(num_edges,
+		a.x,
+		a.edges,
+		a.faces,
+		a.vprop,
+		a.d_hat,
+		thickness,
+		face_tree.nodes.get(),
+		face_tree.status.get(),
+		face_tree.spill.get(),
+		face_tree.num_leaves + 1,
+		a.count,
+		reinterpret_cast<EfPairRec*>(a.pairs),
+		reinterpret_cast<uint2*>(a.pair_ids),
+		a.max_pairs);
+	UT_LAUNCH_CHECK();
 }
 
 void ef_dcd(Context* ctx, const EfDcdArgs& a)

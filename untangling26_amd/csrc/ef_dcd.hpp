@@ -1,5 +1,6 @@
 #pragma once
 #include "ut_runtime.hpp"
+#include "lbvh.hpp"
 
 namespace ut
 {
@@ -37,5 +38,8 @@ struct EfDcdArgs
 };
 
 void ef_dcd(Context* ctx, const EfDcdArgs& a);
+// Broad (edge query on the refit face tree, unculled) + narrow in one kernel;
+// a.broad / a.broad_count / a.broad_cap are unused.
+void ef_detect(Context* ctx, Lbvh& face_tree, const EfDcdArgs& a, uint32_t num_edges, const float* thickness);
 
 } // namespace ut

@@ -1,13 +1,15 @@
 // LBVH kernels for gfx950.  See lbvh.hpp for the data layout.
 #include "ut_common.hpp"
 #include "lbvh.hpp"
+#include "lbvh_traverse.hpp"
 
 namespace ut
 {
 
 constexpr int kBoxThreads = 256;
-constexpr int kQueryThreads = 256;
-constexpr int kStack = 32;
+// Leaves handled by one refit workgroup; inner nodes whose leaf range lies in
+// one such block are refit with workgroup-scope synchronisation only.
+constexpr int kRefitBlock = 256;
 
 // ---------------------------------------------------------------------------
 // Morton code, lbvh.cpp:84-135.  10 bits per axis, x in the highest lane.
@@ -47,19 +49,16 @@ __device__ __forceinline__ void prim_points(
 	}
 }
 
-// Leaf centroid + block box, last block folds all block boxes into the
-// global box (lbvh.cpp:400-496 do this in two launches).
+// Leaf centroid + per-block box of the primitive vertices (first pass of
+// fn_reduce_{face,edge}_tree_global_aabb, lbvh.cpp:400-496).
 __global__ __launch_bounds__(kBoxThreads) void k_leaf_center_box(int type,
 	const float* __restrict__ x,
 	const uint32_t* __restrict__ prims,
 	uint32_t n,
 	float* __restrict__ center,
-	float* __restrict__ block_box,
-	uint32_t* __restrict__ done,
-	float* __restrict__ global_box)
+	float* __restrict__ block_box)
 {
 	__shared__ float red[6][kBoxThreads / kWave];
-	__shared__ bool	 is_last;
 	const uint32_t	 i = blockIdx.x * kBoxThreads + threadIdx.x;
 	box3			 bx = box_empty();
 	if (i < n)
@@ -95,41 +94,37 @@ __global__ __launch_bounds__(kBoxThreads) void k_leaf_center_box(int type,
 			red[k][w] = v[k];
 	}
 	__syncthreads();
-	if (threadIdx.x == 0)
+	if (threadIdx.x < 6)
 	{
-		float r[6];
-#pragma unroll
-		for (int k = 0; k < 6; ++k)
-			r[k] = red[k][0];
+		const int k = threadIdx.x;
+		float	  r = red[k][0];
 		for (int q = 1; q < kBoxThreads / kWave; ++q)
-		{
-#pragma unroll
-			for (int k = 0; k < 3; ++k)
-				r[k] = fminf(r[k], red[k][q]);
-#pragma unroll
-			for (int k = 3; k < 6; ++k)
-				r[k] = fmaxf(r[k], red[k][q]);
-		}
-#pragma unroll
-		for (int k = 0; k < 6; ++k)
-			block_box[6 * blockIdx.x + k] = r[k];
-		const uint32_t ticket =
-			__hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-		is_last = (ticket == gridDim.x - 1);
+			r = k < 3 ? fminf(r, red[k][q]) : fmaxf(r, red[k][q]);
+		block_box[6 * blockIdx.x + k] = r;
 	}
-	__syncthreads();
-	if (!is_last)
-		return;
-	__atomic_thread_fence(__ATOMIC_ACQUIRE); // other blocks' boxes are visible
-	float r[6] = { 1e8f, 1e8f, 1e8f, -1e8f, -1e8f, -1e8f };
-	for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBoxThreads)
+}
+
+// Second pass of the global box (every block folds all block boxes; they sit
+// in L2) followed by fn_compute_mortons, lbvh.cpp:512-528.
+__global__ __launch_bounds__(256) void k_morton(uint32_t n,
+	const float* __restrict__ center,
+	const float* __restrict__ block_box,
+	uint32_t num_blocks,
+	float* __restrict__ global_box,
+	uint32_t* __restrict__ keys,
+	uint32_t* __restrict__ vals)
+{
+	__shared__ float red[6][256 / kWave];
+	__shared__ float gb[6];
+	float			 r[6] = { 1e8f, 1e8f, 1e8f, -1e8f, -1e8f, -1e8f };
+	for (uint32_t b = threadIdx.x; b < num_blocks; b += 256)
 	{
 #pragma unroll
 		for (int k = 0; k < 3; ++k)
-			r[k] = fminf(r[k], __builtin_nontemporal_load(&block_box[6 * b + k]));
+			r[k] = fminf(r[k], block_box[6 * b + k]);
 #pragma unroll
 		for (int k = 3; k < 6; ++k)
-			r[k] = fmaxf(r[k], __builtin_nontemporal_load(&block_box[6 * b + k]));
+			r[k] = fmaxf(r[k], block_box[6 * b + k]);
 	}
 #pragma unroll
 	for (int k = 0; k < 3; ++k)
@@ -137,7 +132,7 @@ __global__ __launch_bounds__(kBoxThreads) void k_leaf_center_box(int type,
 #pragma unroll
 	for (int k = 3; k < 6; ++k)
 		r[k] = wave_max(r[k]);
-	__syncthreads();
+	const uint32_t w = threadIdx.x / kWave;
 	if (__lane_id() == 0)
 	{
 #pragma unroll
@@ -145,36 +140,22 @@ __global__ __launch_bounds__(kBoxThreads) void k_leaf_center_box(int type,
 			red[k][w] = r[k];
 	}
 	__syncthreads();
-	if (threadIdx.x == 0)
+	if (threadIdx.x < 6)
 	{
-		for (int q = 1; q < kBoxThreads / kWave; ++q)
-		{
-#pragma unroll
-			for (int k = 0; k < 3; ++k)
-				red[k][0] = fminf(red[k][0], red[k][q]);
-#pragma unroll
-			for (int k = 3; k < 6; ++k)
-				red[k][0] = fmaxf(red[k][0], red[k][q]);
-		}
-#pragma unroll
-		for (int k = 0; k < 6; ++k)
-			global_box[k] = red[k][0];
-		*done = 0u; // ready for the next build
+		const int k = threadIdx.x;
+		float	  v = red[k][0];
+		for (int q = 1; q < 256 / kWave; ++q)
+			v = k < 3 ? fminf(v, red[k][q]) : fmaxf(v, red[k][q]);
+		gb[k] = v;
+		if (blockIdx.x == 0)
+			global_box[k] = v;
 	}
-}
-
-// fn_compute_mortons, lbvh.cpp:512-528.
-__global__ void k_morton(uint32_t n,
-	const float* __restrict__ center,
-	const float* __restrict__ global_box,
-	uint32_t* __restrict__ keys,
-	uint32_t* __restrict__ vals)
-{
+	__syncthreads();
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	const f3 lo = mk3(global_box[0], global_box[1], global_box[2]);
-	const f3 hi = mk3(global_box[3], global_box[4], global_box[5]);
+	const f3 lo = mk3(gb[0], gb[1], gb[2]);
+	const f3 hi = mk3(gb[3], gb[4], gb[5]);
 	const f3 ext = max3(hi - lo, mk3(1e-6f, 1e-6f, 1e-6f));
 	const f3 inv = mk3(1.0f / ext.x, 1.0f / ext.y, 1.0f / ext.z);
 	const f3 nrm = mul3(ld3(center, i) - lo, inv);
@@ -239,7 +220,7 @@ __global__ void k_karras(uint32_t n,
 		r.c = make_float4(INFINITY, -INFINITY, -INFINITY, -INFINITY);
 		r.d = make_uint4(kLeafBit | orig, kLeafBit | orig, 0u, 0u);
 		nodes[0] = r;
-		leaf_ps[0] = 0u;
+		leaf_ps[0] = 2u; // node 0, local, slot 0
 		return;
 	}
 	if (i >= ni)
@@ -304,18 +285,22 @@ __global__ void k_karras(uint32_t n,
 	children[2 * i + 0] = (uint32_t)cl;
 	children[2 * i + 1] = (uint32_t)cr;
 	object_idx[i] = kNone;
-	// traversal record + refit bookkeeping
+	// traversal record + refit bookkeeping: children learn parent, slot and
+	// whether the parent's leaf range [min(r0,r1), max(r0,r1)] lies in one refit block
 	const uint32_t ref0 = cl >= ni ? (kLeafBit | svals[cl - ni]) : (uint32_t)cl;
 	const uint32_t ref1 = cr >= ni ? (kLeafBit | svals[cr - ni]) : (uint32_t)cr;
 	nodes[i].d = make_uint4(ref0, ref1, 0u, 0u);
+	const uint32_t local = ((uint32_t)min(r0, r1) / kRefitBlock == (uint32_t)max(r0, r1) / kRefitBlock) ? 2u : 0u;
+	const uint32_t ps0 = ((uint32_t)i << 2) | local | 0u;
+	const uint32_t ps1 = ((uint32_t)i << 2) | local | 1u;
 	if (cl >= ni)
-		leaf_ps[cl - ni] = ((uint32_t)i << 1) | 0u;
+		leaf_ps[cl - ni] = ps0;
 	else
-		inner_ps[cl] = ((uint32_t)i << 1) | 0u;
+		inner_ps[cl] = ps0;
 	if (cr >= ni)
-		leaf_ps[cr - ni] = ((uint32_t)i << 1) | 1u;
+		leaf_ps[cr - ni] = ps1;
 	else
-		inner_ps[cr] = ((uint32_t)i << 1) | 1u;
+		inner_ps[cr] = ps1;
 	if (i == 0)
 		inner_ps[0] = kNone;
 	refit_flag[i] = 0u;
@@ -342,7 +327,22 @@ __device__ __forceinline__ box3 load_slot(const BvhNode* node, uint32_t slot)
 	return box3{ mk3(f[0], f[1], f[2]), mk3(f[3], f[4], f[5]) };
 }
 
-__global__ void k_leaf_refit(int type,
+__device__ __forceinline__ void store_root(float* root_box, const box3& b)
+{
+	root_box[0] = b.lo.x;
+	root_box[1] = b.lo.y;
+	root_box[2] = b.lo.z;
+	root_box[3] = b.hi.x;
+	root_box[4] = b.hi.y;
+	root_box[5] = b.hi.z;
+}
+
+// Phase 1: one thread per sorted leaf.  Climbs while the parent is local to
+// this workgroup (both children are then refit by threads of the same
+// workgroup, so a workgroup-scope acq_rel counter orders the box writes).  At
+// the first non-local parent the thread stores its box into the parent's slot
+// and records the arrival in `exits` for phase 2.
+__global__ __launch_bounds__(kRefitBlock) void k_refit_local(int type,
 	uint32_t n,
 	const uint32_t* __restrict__ sorted_get_original,
 	const uint32_t* __restrict__ prims,
@@ -353,6 +353,8 @@ __global__ void k_leaf_refit(int type,
 	const uint32_t* __restrict__ leaf_ps,
 	const uint32_t* __restrict__ inner_ps,
 	uint32_t* refit_flag,
+	uint32_t* __restrict__ exits,
+	uint32_t* __restrict__ exit_count,
 	float* __restrict__ root_box,
 	uint32_t* __restrict__ status)
 {
@@ -385,46 +387,79 @@ __global__ void k_leaf_refit(int type,
 			status[0] = 0u;
 			return;
 		}
-		const uint32_t node = slot_ref >> 1;
+		const uint32_t node = slot_ref >> 2;
 		store_slot(&nodes[node], slot_ref & 1u, bx);
 		if (n == 1)
 		{
-			for (int k = 0; k < 3; ++k)
-				root_box[k] = (&bx.lo.x)[k], root_box[3 + k] = (&bx.hi.x)[k];
+			store_root(root_box, bx);
+			return;
+		}
+		if (!(slot_ref & 2u))
+		{
+			exits[atomicAdd(exit_count, 1u)] = node;
 			return;
 		}
 		const uint32_t old =
-			__hip_atomic_fetch_add(&refit_flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+			__hip_atomic_fetch_add(&refit_flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
 		if (old == 0u)
-			return; // sibling not done yet; it will carry the union upward
+			return; // sibling not done yet; it carries the union upward
 		refit_flag[node] = 0u;
-		const box3 b0 = load_slot(&nodes[node], 0);
-		const box3 b1 = load_slot(&nodes[node], 1);
-		bx = box_union(b0, b1);
+		bx = box_union(load_slot(&nodes[node], 0), load_slot(&nodes[node], 1));
 		slot_ref = inner_ps[node];
 		if (slot_ref == kNone)
 		{
-			root_box[0] = bx.lo.x;
-			root_box[1] = bx.lo.y;
-			root_box[2] = bx.lo.z;
-			root_box[3] = bx.hi.x;
-			root_box[4] = bx.hi.y;
-			root_box[5] = bx.hi.z;
+			store_root(root_box, bx);
 			return;
 		}
 	}
 }
 
-// ---------------------------------------------------------------------------
-// Stack traversal (query_template2, lbvh.cpp:295-363), one thread per query
-// primitive.  The first kStack entries of the stack live in LDS; deeper
-// entries spill to a per-query global area.  Keys are unique 64-bit values, so
-// the Karras tree is at most 64 levels deep and kStackTotal = 128 is never
-// reached; if it were, status[1] counts the truncated queries (the reference
-// silently drops them, lbvh.cpp:343-351).
-constexpr int kStackTotal = 128;
+// Phase 2: one workgroup finishes the nodes whose leaf range spans several
+// refit blocks (about two per block).  Phase-1 boxes are visible through the
+// kernel boundary; arrivals inside this workgroup use workgroup scope.
+__global__ __launch_bounds__(1024) void k_refit_top(BvhNode* nodes,
+	const uint32_t* __restrict__ inner_ps,
+	uint32_t* refit_flag,
+	const uint32_t* __restrict__ exits,
+	uint32_t* __restrict__ exit_count,
+	float* __restrict__ root_box,
+	uint32_t* __restrict__ status)
+{
+	const uint32_t ne = *exit_count;
+	for (uint32_t e = threadIdx.x; e < ne; e += blockDim.x)
+	{
+		uint32_t node = exits[e];
+		for (int level = 0;; ++level)
+		{
+			if (level > 96)
+			{
+				status[0] = 0u;
+				break;
+			}
+			const uint32_t old =
+				__hip_atomic_fetch_add(&refit_flag[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+			if (old == 0u)
+				break;
+			refit_flag[node] = 0u;
+			const box3	   bx = box_union(load_slot(&nodes[node], 0), load_slot(&nodes[node], 1));
+			const uint32_t ps = inner_ps[node];
+			if (ps == kNone)
+			{
+				store_root(root_box, bx);
+				break;
+			}
+			node = ps >> 2;
+			store_slot(&nodes[node], ps & 1u, bx);
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x == 0)
+		*exit_count = 0u;
+}
 
-__global__ __launch_bounds__(kQueryThreads) void k_query(int kind,
+// ---------------------------------------------------------------------------
+// Broad-phase queries (query_template2, lbvh.cpp:295-363) with staged output.
+__global__ __launch_bounds__(kTravThreads) void k_query(int kind,
 	uint32_t nq,
 	const float* __restrict__ xb,
 	const float* __restrict__ xe,
@@ -439,90 +474,12 @@ __global__ __launch_bounds__(kQueryThreads) void k_query(int kind,
 	uint32_t* __restrict__ spill,
 	uint32_t max_visits)
 {
-	__shared__ uint32_t lds_stack[kStack * kQueryThreads];
-	const uint32_t		q = blockIdx.x * kQueryThreads + threadIdx.x;
-	const bool			active = q < nq;
-	box3				qb = box_empty();
-	if (active)
-	{
-		if (kind == kQueryVerts)
-		{
-			qb = box_add(box_of(ld3(xb, q)), ld3(xe, q));
-			qb = box_inflate(qb, thickness[q] + d_hat[q]);
-		}
-		else
-		{
-			const uint32_t e0 = edges[2 * q], e1 = edges[2 * q + 1];
-			qb = box_add(box_add(box_add(box_of(ld3(xb, e0)), ld3(xb, e1)), ld3(xe, e0)), ld3(xe, e1));
-			qb = box_inflate(qb, thickness[e0] + d_hat[e0]);
-		}
-	}
-	int		 sp = 0;
-	uint32_t node = 0;
-	bool	 go = active;
-	uint32_t visits = 0;
-	while (__ballot(go) != 0ull)
-	{
-		bool	 emit0 = false, emit1 = false;
-		uint32_t obj0 = 0, obj1 = 0;
-		if (go && ++visits > max_visits)
-		{
-			// every inner node is visited at most once per query; more means a corrupted tree
-			status[0] = 0u;
-			go = false;
-		}
-		if (go)
-		{
-			const BvhNode* rec = &nodes[node];
-			const float4   a = rec->a, b = rec->b, c = rec->c;
-			const uint2	   refs = make_uint2(rec->d.x, rec->d.y);
-			const box3	   b0{ mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y) };
-			const box3	   b1{ mk3(b.z, b.w, c.x), mk3(c.y, c.z, c.w) };
-			const bool	   h0 = box_overlap(b0, qb);
-			const bool	   h1 = box_overlap(b1, qb);
-			const bool	   l0 = (refs.x & kLeafBit) != 0u, l1 = (refs.y & kLeafBit) != 0u;
-			obj0 = refs.x & ~kLeafBit;
-			obj1 = refs.y & ~kLeafBit;
-			emit0 = h0 && l0 && (kind != kQueryEdges || q < obj0);
-			emit1 = h1 && l1 && (kind != kQueryEdges || q < obj1);
-			const bool t0 = h0 && !l0, t1 = h1 && !l1;
-			if (t0 && t1)
-			{
-				if (sp < kStack)
-					lds_stack[sp * kQueryThreads + threadIdx.x] = refs.y;
-				else if (sp < kStackTotal)
-					spill[(size_t)q * (kStackTotal - kStack) + (sp - kStack)] = refs.y;
-				if (sp < kStackTotal)
-				{
-					++sp;
-					node = refs.x;
-				}
-				else
-				{
-					atomicAdd(&status[1], 1u);
-					go = false;
-				}
-			}
-			else if (t0)
-				node = refs.x;
-			else if (t1)
-				node = refs.y;
-			else if (sp > 0)
-			{
-				--sp;
-				node = sp < kStack ? lds_stack[sp * kQueryThreads + threadIdx.x]
-								   : spill[(size_t)q * (kStackTotal - kStack) + (sp - kStack)];
-			}
-			else
-				go = false;
-		}
-		const uint32_t s0 = wave_append(count, emit0);
-		if (emit0 && s0 < max_count)
-			list[s0] = make_uint2(q, obj0);
-		const uint32_t s1 = wave_append(count, emit1);
-		if (emit1 && s1 < max_count)
-			list[s1] = make_uint2(q, obj1);
-	}
+	__shared__ TravShared sh;
+	const uint32_t		  q = blockIdx.x * kTravThreads + threadIdx.x;
+	const bool			  active = q < nq;
+	const box3			  qb = active ? query_box(kind, q, xb, xe, edges, d_hat, thickness) : box_empty();
+	EmitPairs			  emit{ count, list, max_count };
+	traverse_staged(sh, active, q, qb, kind == kQueryEdges, nodes, status, spill, max_visits, emit);
 }
 
 // ---------------------------------------------------------------------------
@@ -538,7 +495,6 @@ void Lbvh::init(Context* c, int tree_type, uint32_t n)
 	leaf_center.reserve(3 * (size_t)n);
 	block_box.reserve(6 * (size_t)div_up(n, kBoxThreads));
 	global_box.reserve(6);
-	done_counter.reserve(1);
 	for (int k = 0; k < 2; ++k)
 	{
 		sort_keys[k].reserve(n);
@@ -557,10 +513,12 @@ void Lbvh::init(Context* c, int tree_type, uint32_t n)
 	leaf_parent_slot.reserve(n);
 	inner_parent_slot.reserve(inner);
 	refit_flag.reserve(inner);
+	refit_exits.reserve(2 * (size_t)n + 2);
+	refit_exit_count.reserve(1);
 	root_box.reserve(6);
 	status.reserve(2);
 	ctx->activate();
-	UT_HIP_CHECK(hipMemsetAsync(done_counter.get(), 0, sizeof(uint32_t), ctx->stream));
+	UT_HIP_CHECK(hipMemsetAsync(refit_exit_count.get(), 0, sizeof(uint32_t), ctx->stream));
 	UT_HIP_CHECK(hipMemsetAsync(refit_flag.get(), 0, sizeof(uint32_t) * inner, ctx->stream));
 	UT_HIP_CHECK(hipMemsetAsync(status.get(), 0, sizeof(uint32_t) * 2, ctx->stream));
 }
@@ -570,9 +528,14 @@ void Lbvh::reduce_aabb(const float* x, const uint32_t* prims)
 	UT_REQUIRE(x && prims, "reduce_aabb: null input");
 	const uint32_t blocks = div_up(num_leaves, kBoxThreads);
 	k_leaf_center_box<<<blocks, kBoxThreads, 0, ctx->stream>>>(
-		type, x, prims, num_leaves, leaf_center.get(), block_box.get(), done_counter.get(), global_box.get());
-	k_morton<<<div_up(num_leaves, 256), 256, 0, ctx->stream>>>(
-		num_leaves, leaf_center.get(), global_box.get(), sort.keys[0], sort.vals[0]);
+		type, x, prims, num_leaves, leaf_center.get(), block_box.get());
+	k_morton<<<div_up(num_leaves, 256), 256, 0, ctx->stream>>>(num_leaves,
+		leaf_center.get(),
+		block_box.get(),
+		blocks,
+		global_box.get(),
+		sort.keys[0],
+		sort.vals[0]);
 	UT_LAUNCH_CHECK();
 }
 
@@ -604,7 +567,7 @@ void Lbvh::construct_tree_presorted()
 void Lbvh::update_leaves_and_refit(const float* thickness, const float* xs, const float* xe, const uint32_t* prims)
 {
 	UT_REQUIRE(thickness && xs && xe && prims, "update_leaves_and_refit: null input");
-	k_leaf_refit<<<div_up(num_leaves, 256), 256, 0, ctx->stream>>>(type,
+	k_refit_local<<<div_up(num_leaves, kRefitBlock), kRefitBlock, 0, ctx->stream>>>(type,
 		num_leaves,
 		sorted_get_original.get(),
 		prims,
@@ -615,6 +578,15 @@ void Lbvh::update_leaves_and_refit(const float* thickness, const float* xs, cons
 		leaf_parent_slot.get(),
 		inner_parent_slot.get(),
 		refit_flag.get(),
+		refit_exits.get(),
+		refit_exit_count.get(),
+		root_box.get(),
+		status.get());
+	k_refit_top<<<1, 1024, 0, ctx->stream>>>(nodes.get(),
+		inner_parent_slot.get(),
+		refit_flag.get(),
+		refit_exits.get(),
+		refit_exit_count.get(),
 		root_box.get(),
 		status.get());
 	UT_LAUNCH_CHECK();
@@ -637,8 +609,8 @@ void Lbvh::query(int kind,
 	UT_REQUIRE(max_count == 0 || list, "query: null output list");
 	if (nq == 0)
 		return;
-	spill.reserve((size_t)nq * (kStackTotal - kStack));
-	k_query<<<div_up(nq, kQueryThreads), kQueryThreads, 0, ctx->stream>>>(kind,
+	spill.reserve((size_t)nq * (kStackTotal - kStackLds));
+	k_query<<<div_up(nq, kTravThreads), kTravThreads, 0, ctx->stream>>>(kind,
 		nq,
 		xb,
 		xe,

@@ -49,7 +49,6 @@ struct Lbvh
 	DevBuf<float>	 leaf_center; // [3N]
 	DevBuf<float>	 block_box;	  // [6 * blocks]
 	DevBuf<float>	 global_box;  // [6]
-	DevBuf<uint32_t> done_counter;
 	DevBuf<uint32_t> sort_keys[2], sort_vals[2], sort_hist;
 	RadixSortBuffers sort;
 
@@ -61,9 +60,11 @@ struct Lbvh
 	DevBuf<uint32_t> object_idx;		  // [2N-1]
 	// traversal records and refit bookkeeping
 	DevBuf<BvhNode>	 nodes;			  // [max(N-1,1)]
-	DevBuf<uint32_t> leaf_parent_slot;	// [N]   parent << 1 | slot
-	DevBuf<uint32_t> inner_parent_slot; // [N-1]
-	DevBuf<uint32_t> refit_flag;		// [N-1]
+	DevBuf<uint32_t> leaf_parent_slot;	// [N]   parent << 2 | parent-is-block-local << 1 | slot
+	DevBuf<uint32_t> inner_parent_slot; // [N-1] same encoding, kNone for the root
+	DevBuf<uint32_t> refit_flag;		// [N-1] arrivals per inner node, 0 between refits
+	DevBuf<uint32_t> refit_exits;		// nodes reached from below whose range spans refit blocks
+	DevBuf<uint32_t> refit_exit_count;
 	DevBuf<float>	 root_box;			// [6]
 	DevBuf<uint32_t> status;			// [2] : [0] health (1 ok), [1] truncated queries
 	DevBuf<uint32_t> spill;				// traversal stack entries beyond the LDS part

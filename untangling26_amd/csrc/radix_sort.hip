@@ -6,9 +6,10 @@
 // the same total order.
 //
 // Per pass three launches:
-//   upsweep   per-tile digit histogram in LDS            -> hist[digit][tile]
-//   scan      one workgroup, exclusive scan over hist     (digit-major)
-//   downsweep stable per-tile ranking with wave64 ballot multisplit, scatter
+//   upsweep     per-tile digit histogram in LDS          -> hist[digit][tile]
+//   digit scan  one workgroup per digit: exclusive scan over tiles + digit total
+//   downsweep   digit base (scan of totals) + stable per-tile ranking with
+//               wave64 ballot multisplit, scatter
 #include "ut_common.hpp"
 #include "ut_runtime.hpp"
 #include "radix_sort.hpp"
@@ -42,49 +43,44 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_upsweep(
 	hist[tid * num_tiles + tile] = h[tid];
 }
 
-// Exclusive scan of `total` counters with one 1024-thread workgroup.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ hist, uint32_t total)
+// One workgroup per digit: exclusive scan of that digit's per-tile counts
+// (coalesced, 256 at a time), total of the digit into totals[digit].
+__global__ __launch_bounds__(256) void k_rs_digit_scan(uint32_t* __restrict__ hist, uint32_t num_tiles,
+	uint32_t* __restrict__ totals)
 {
-	__shared__ uint32_t wsum[1024 / kWave];
+	__shared__ uint32_t wsum[256 / kWave];
+	const uint32_t		d = blockIdx.x;
 	const uint32_t		tid = threadIdx.x;
-	const uint32_t		per = (total + 1023) / 1024;
-	const uint32_t		beg = tid * per;
-	const uint32_t		end = min(beg + per, total);
-	uint32_t			local = 0;
-	for (uint32_t i = beg; i < end; ++i)
-		local += hist[i];
-	// inclusive wave scan
-	uint32_t v = local;
-	for (int o = 1; o < kWave; o <<= 1)
+	uint32_t*			row = hist + (size_t)d * num_tiles;
+	uint32_t			carry = 0;
+	for (uint32_t base = 0; base < num_tiles; base += 256)
 	{
-		uint32_t t = __shfl_up(v, o, kWave);
-		if ((int)__lane_id() >= o)
-			v += t;
-	}
-	const uint32_t w = tid / kWave;
-	if (__lane_id() == kWave - 1)
-		wsum[w] = v;
-	__syncthreads();
-	if (tid < kWave)
-	{
-		uint32_t s = tid < 1024 / kWave ? wsum[tid] : 0u;
+		const uint32_t i = base + tid;
+		const uint32_t c = i < num_tiles ? row[i] : 0u;
+		uint32_t	   v = c;
 		for (int o = 1; o < kWave; o <<= 1)
 		{
-			uint32_t t = __shfl_up(s, o, kWave);
+			const uint32_t t = __shfl_up(v, o, kWave);
 			if ((int)__lane_id() >= o)
-				s += t;
+				v += t;
 		}
-		if (tid < 1024 / kWave)
-			wsum[tid] = s; // inclusive
+		const uint32_t w = tid / kWave;
+		if (__lane_id() == kWave - 1)
+			wsum[w] = v;
+		__syncthreads();
+		uint32_t before = carry;
+		for (uint32_t q = 0; q < w; ++q)
+			before += wsum[q];
+		uint32_t chunk = 0;
+		for (uint32_t q = 0; q < 256 / kWave; ++q)
+			chunk += wsum[q];
+		if (i < num_tiles)
+			row[i] = before + v - c;
+		carry += chunk;
+		__syncthreads();
 	}
-	__syncthreads();
-	uint32_t run = (w ? wsum[w - 1] : 0u) + v - local; // exclusive start of this thread's chunk
-	for (uint32_t i = beg; i < end; ++i)
-	{
-		uint32_t c = hist[i];
-		hist[i] = run;
-		run += c;
-	}
+	if (tid == 0)
+		totals[d] = carry;
 }
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const uint32_t* __restrict__ keys_in,
@@ -94,14 +90,33 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_downsweep(const uint32_t* __r
 	uint32_t n,
 	int		 shift,
 	const uint32_t* __restrict__ hist,
-	uint32_t num_tiles)
+	uint32_t num_tiles,
+	const uint32_t* __restrict__ totals)
 {
 	__shared__ uint32_t wave_cnt[kRsWaves][kRsBins];
 	__shared__ uint32_t digit_base[kRsBins];
+	__shared__ uint32_t wsum[kRsWaves];
 	const uint32_t		tid = threadIdx.x;
 	const uint32_t		tile = blockIdx.x;
 	const uint32_t		w = tid / kWave;
-	digit_base[tid] = hist[tid * num_tiles + tile];
+	{
+		// exclusive scan of the 256 digit totals
+		const uint32_t c = totals[tid];
+		uint32_t	   v = c;
+		for (int o = 1; o < kWave; o <<= 1)
+		{
+			const uint32_t t = __shfl_up(v, o, kWave);
+			if ((int)__lane_id() >= o)
+				v += t;
+		}
+		if (__lane_id() == kWave - 1)
+			wsum[w] = v;
+		__syncthreads();
+		uint32_t before = 0;
+		for (uint32_t q = 0; q < w; ++q)
+			before += wsum[q];
+		digit_base[tid] = before + v - c + hist[tid * num_tiles + tile];
+	}
 #pragma unroll
 	for (int q = 0; q < kRsWaves; ++q)
 		wave_cnt[q][tid] = 0;
@@ -168,15 +183,15 @@ void radix_sort_pairs(RadixSortBuffers& b, uint32_t n, int key_bits, hipStream_t
 	{
 		const int dst = src ^ 1;
 		k_rs_upsweep<<<tiles, kRsThreads, 0, stream>>>(b.keys[src], n, shift, b.hist, tiles);
-		k_rs_scan<<<1, 1024, 0, stream>>>(b.hist, tiles * kRsBins);
+		k_rs_digit_scan<<<kRsBins, 256, 0, stream>>>(b.hist, tiles, b.hist + (size_t)tiles * kRsBins);
 		k_rs_downsweep<<<tiles, kRsThreads, 0, stream>>>(
-			b.keys[src], b.vals[src], b.keys[dst], b.vals[dst], n, shift, b.hist, tiles);
+			b.keys[src], b.vals[src], b.keys[dst], b.vals[dst], n, shift, b.hist, tiles, b.hist + (size_t)tiles * kRsBins);
 		src = dst;
 	}
 	UT_LAUNCH_CHECK();
 	b.result = src;
 }
 
-uint32_t radix_sort_hist_size(uint32_t n) { return div_up(n ? n : 1, kRsTile) * kRsBins; }
+uint32_t radix_sort_hist_size(uint32_t n) { return (div_up(n ? n : 1, kRsTile) + 1) * kRsBins; }
 
 } // namespace ut

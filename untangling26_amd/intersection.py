@@ -25,6 +25,14 @@ def ef_dcd_intersection_query(ctx: Context, broad, broad_count, x, edges, faces,
                               ptr(vprop), ptr(d_hat), ptr(count), ptr(pairs), ptr(pair_ids), cap))
 
 
+def ef_detect_fused(ctx: Context, face_tree: LBVH, x, edges, faces, d_hat, thickness, count, pairs, pair_ids,
+                    vprop=None):
+    """Broad + narrow EF detection in one kernel (ut_ef_detect); face_tree must be refit."""
+    cap = pair_ids.numel() // 2
+    ctx.check(lib().ut_ef_detect(face_tree.handle, ptr(x), ptr(edges), edges.shape[0], ptr(faces), ptr(vprop),
+                                 ptr(d_hat), ptr(thickness), ptr(count), ptr(pairs), ptr(pair_ids), cap))
+
+
 def decode_ef_pairs(pairs: torch.Tensor, n: int):
     """Splits EfPair words into (indices uint32[n,5], baries float[n,3])."""
     w = pairs[: n * EF_PAIR_WORDS].view(n, EF_PAIR_WORDS)
@@ -69,10 +77,27 @@ class EfDetector:
         ef_dcd_intersection_query(self.ctx, self.broad, self.counts[0:1], x, self.edges, self.faces, d_hat,
                                   self.counts[1:2], self.pairs, self.pair_ids, vprop)
 
-    def detect(self, x, d_hat, thickness, vprop=None, rebuild: bool = True, max_attempts: int = 10):
-        """Returns the number of EF pairs; pair_ids[:n] holds (eid, fid)."""
+    def detect(self, x, d_hat, thickness, vprop=None, rebuild: bool = True, max_attempts: int = 10,
+               fused: bool = True):
+        """Returns the number of EF pairs; pair_ids[:n] holds (eid, fid).
+
+        fused=True runs ut_ef_detect (no broad list); fused=False runs the
+        reference's two steps (broad query, then the EF narrow phase)."""
         if rebuild:
             self.build(x)
+        if fused:
+            self.face_tree.update_face_tree_leave_aabb(thickness, x, x, self.faces)
+            self.face_tree.refit()
+            for _ in range(max_attempts):
+                self.counts[1].zero_()
+                ef_detect_fused(self.ctx, self.face_tree, x, self.edges, self.faces, d_hat, thickness,
+                                self.counts[1:2], self.pairs, self.pair_ids, vprop)
+                n = int(self.counts[1].item())
+                if n <= self.ef_cap:
+                    return n
+                self.ef_cap = 2 * n
+                self._alloc()
+            raise RuntimeError("EF count still out of range after resizing")
         for _ in range(max_attempts):
             self._broad(x, d_hat, thickness)
             nb = int(self.counts[0].item())

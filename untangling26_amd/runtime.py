@@ -21,7 +21,7 @@ class Context:
         self.device = torch.device("cuda", device)
         self.torch_stream = stream if stream is not None else torch.cuda.current_stream(self.device)
         h = C.c_void_p()
-        rc = lib().ut_ctx_create(device, C.c_void_p(self.torch_stream.cuda_stream), C.byref(h))
+        rc = lib().ut_ctx_create(device, C.c_void_p(self.torch_stream.cuda_stream), 0, C.byref(h))
         if rc != 0:
             raise RuntimeError(f"ut_ctx_create failed with {rc}")
         self.handle = h
