@@ -31,8 +31,10 @@ extern "C" {
 
 #define UT_API __attribute__((visibility("default")))
 
-typedef struct ut_ctx  ut_ctx;
-typedef struct ut_lbvh ut_lbvh;
+typedef struct ut_ctx      ut_ctx;
+typedef struct ut_lbvh     ut_lbvh;
+typedef struct ut_contours ut_contours;
+typedef struct ut_pcg      ut_pcg;
 
 /* ---- context ---------------------------------------------------------- */
 
@@ -114,6 +116,83 @@ UT_API int ut_ef_dcd(ut_ctx* ctx, const uint32_t* broad, const uint32_t* broad_c
 UT_API int ut_ef_detect(ut_lbvh* face_tree, const float* x, const uint32_t* edges, uint32_t num_edges,
 	const uint32_t* faces, const uint32_t* vprop, const float* d_hat, const float* thickness, uint32_t* count,
 	void* pairs, uint32_t* pair_ids, uint32_t max_pairs);
+
+/* ---- contours: IntersectionResolver::host_preprocess_ef_pairs
+ *      (intersection_resolver.cpp:683-1096) + intersection_contour_construction_from_ext_adjacent
+ *      (intersection_resolver2.cpp:254-309, contour_flood_filling_template
+ *      intersection_resolver_helper.cpp:518-575) ----------------------------- */
+
+enum {
+	UT_CT_PAIR_IDS = 0,		 /* uint32[2*total] (eid, fid), canonical order, loop pairs appended */
+	UT_CT_PAIRS = 1,		 /* EfPair[total] (12 words each) */
+	UT_CT_CONTOUR_INDEX = 2, /* uint32[total] ef_pair_contour_index */
+	UT_CT_MESH_INDEX = 3,	 /* uint32[total] ef_pair_mesh_index (parity) */
+	UT_CT_ADJ = 4,			 /* uint32[12*total] ef_pair_adj_pairs_ext, 12 slots per pair */
+	UT_CT_ADJ_N = 5,		 /* uint32[total] used slots */
+	UT_CT_INCONSISTENT = 6,	 /* uint32[total] 1 at the smallest pair of an odd-cycle component */
+	UT_CT_COMP_MIN = 7		 /* uint32[total] smallest pair index of the pair's contour */
+};
+/* Mesh topology (device arrays kept by reference): face_adj_edges uint32[3F]
+ * (edge between face[j] and face[j+1], init_mesh_data.cpp:419-483) and
+ * edge_adj_faces uint32[2E] (faces around each edge in ascending order,
+ * 0xFFFFFFFF when absent; init_mesh_data.cpp:486-500). */
+UT_API int ut_contours_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges,
+	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, ut_contours** out);
+UT_API int ut_contours_destroy(ut_contours* ct);
+/* num_pairs EF pairs (EfPair records + (eid, fid)) in any order.  Sorts them
+ * canonically, adds loop-vertex pairs, builds the EF adjacency and the
+ * contours.  Synchronises; *num_total = EF pairs including loop pairs (what
+ * narrow_phase_collision_count[1] holds after preprocessing), *num_contours. */
+UT_API int ut_contours_build(ut_contours* ct, uint32_t num_pairs, const void* pairs, const uint32_t* pair_ids,
+	uint32_t* num_total, uint32_t* num_contours);
+/* Device-to-device copy of one UT_CT_* output of the last build. */
+UT_API int ut_contours_get(ut_contours* ct, int what, void* dst);
+/* Components with parity of an undirected CSR graph with 0/1 flip weights
+ * (UnitTest/test_parallel_contour_construction.cpp:281-448).  scratch is
+ * uint32[2n]; outputs uint32[n]. */
+UT_API int ut_graph_components(ut_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+	const uint32_t* flip_w, uint32_t* scratch, uint32_t* comp_min, uint32_t* parity, uint32_t* inconsistent);
+
+/* ---- PCG: ConjugateGradientSolver (Solver/LinearSolver/precond_cg.h:471-543) ---- */
+
+enum { UT_PCG_FP64 = 0, UT_PCG_MIXED = 1, UT_PCG_FP32 = 2 };
+enum { UT_PREC_BLOCK_JACOBI = 0, UT_PREC_POINT_JACOBI = 1 };
+
+typedef struct ut_pcg_params {
+	int		 precision;		 /* UT_PCG_* */
+	int		 preconditioner; /* UT_PREC_* */
+	uint32_t max_iters;		 /* total PCG iterations; 0 skips the solve (precond_cg.cpp:443-452) */
+	double	 rel_tol;		 /* ||b - A x|| / ||b|| (kPcgRelativeTolerance = 1e-6, precond_cg.cpp:17) */
+	double	 inner_tol;		 /* mixed: relative tolerance of each FP32 inner solve */
+	uint32_t max_refine;	 /* mixed: FP64 refinement steps */
+	double	 regularization; /* lambda: A + lambda * mean(diag block) I (precond_cg.cpp:286-339) */
+	uint32_t check_every;	 /* iterations between host convergence polls */
+} ut_pcg_params;
+
+/* PcgSolveReport (precond_cg.h:18-28). failure: 0 none, 1 non-finite rhs,
+ * 2 non-finite or breakdown during iteration, 3 iteration budget 0. */
+typedef struct ut_pcg_report {
+	double	 relative_residual; /* final true ||b - A x|| / ||b||, FP64 */
+	double	 infinity_norm;		/* max over blocks of |x_i| */
+	uint32_t iterations;
+	uint32_t refinements;
+	int		 converged;
+	int		 failure;
+} ut_pcg_report;
+
+/* Matrix pattern in SELL-64 block layout (device arrays, kept by reference):
+ * slice_off uint32[ceil(n/64)+1], col uint32[nslots], diag_slot uint32[n].
+ * Values are planar: vals[c * nslots + slot], c = 3*i + j of the 3x3 block. */
+UT_API int ut_pcg_create(ut_ctx* ctx, uint32_t n, uint32_t nslices, uint32_t nslots, const uint32_t* slice_off,
+	const uint32_t* col, const uint32_t* diag_slot, ut_pcg** out);
+UT_API int ut_pcg_destroy(ut_pcg* pcg);
+/* ConjugateGradientSolver::device_solve: solves A x = b (b, x FP64 device
+ * arrays of 3n; vals FP64 for UT_PCG_FP64, FP32 otherwise).  Synchronises.
+ * Returns 0, or 2 (not converged) with the last iterate in x. */
+UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* vals, const double* b, double* x,
+	ut_pcg_report* report);
+/* y = A x (FP64 vectors and accumulation; vals FP32 when vals_fp32 != 0). */
+UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const double* x, double* y);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,24 @@ void or_lbvh_node_boxes(uint32_t n, const uint32_t* children, const uint32_t* so
 size_t or_lbvh_query(uint32_t n, const uint32_t* children, const uint32_t* object_idx, const float* node_box,
 	const float* qbox, uint32_t nq, int cull, uint32_t* out, size_t cap);
 
+/* init_mesh_data.cpp:419-500: face_adj_edges uint32[3F], edge_adj_faces uint32[2E]. */
+int or_mesh_topology(uint32_t ne, uint32_t nf, const uint32_t* edges, const uint32_t* faces, uint32_t* face_adj_edges,
+	uint32_t* edge_adj_faces);
+/* host_preprocess_ef_pairs (intersection_resolver.cpp:683-1096) for DCD pairs. */
+long or_ef_preprocess(uint32_t P, const uint32_t* in_ids, const uint32_t* in_pairs, const uint32_t* edges,
+	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, uint32_t* out_ids,
+	uint32_t* out_pairs, uint32_t* adj, uint32_t* adj_n);
+/* contour_flood_filling_template (intersection_resolver_helper.cpp:518-575). */
+uint32_t or_contours_dfs(uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adj_n,
+	uint32_t* contour_index, uint32_t* mesh_index);
+/* cpu_reference of UnitTest/test_parallel_contour_construction.cpp:155-198. */
+int or_graph_bfs(uint32_t n, const uint32_t* row_ptr, const uint32_t* col, const uint32_t* w, uint32_t* comp_min,
+	uint32_t* parity, uint32_t* comp_inconsistent);
+
+/* host_solve_core (precond_cg.cpp:367-693) in FP64 on a block-CSR matrix. */
+uint32_t or_pcg(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const double* b, double* x,
+	int precond, uint32_t max_iters, double rel_tol, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,16 @@ def lib():
         L.or_lbvh_node_boxes.argtypes = [C.c_uint32, p, p, p, p]
         L.or_lbvh_query.argtypes = [C.c_uint32, p, p, p, p, C.c_uint32, C.c_int, p, C.c_size_t]
         L.or_lbvh_query.restype = C.c_size_t
+        L.or_mesh_topology.argtypes = [C.c_uint32, C.c_uint32, p, p, p, p]
+        L.or_mesh_topology.restype = C.c_int
+        L.or_ef_preprocess.argtypes = [C.c_uint32, p, p, p, p, p, p, p, p, p, p]
+        L.or_ef_preprocess.restype = C.c_long
+        L.or_contours_dfs.argtypes = [C.c_uint32, p, p, p, p, p]
+        L.or_contours_dfs.restype = C.c_uint32
+        L.or_graph_bfs.argtypes = [C.c_uint32, p, p, p, p, p, p]
+        L.or_graph_bfs.restype = C.c_int
+        L.or_pcg.argtypes = [C.c_uint32, p, p, p, p, p, C.c_int, C.c_uint32, C.c_double, p]
+        L.or_pcg.restype = C.c_uint32
         L.or_line_intersection.argtypes = [p, p, p, p, p, p, p]
         L.or_line_intersection.restype = C.c_int
         L.or_ef_dcd.argtypes = [p, C.c_size_t, p, p, p, p, p, p, p, C.c_size_t]
@@ -171,3 +181,75 @@ def lbvh_query(tree, leaf_box, qbox, cull=False):
     lib().or_lbvh_query(n, _p(ch), _p(obj), _p(nb), _p(qbox), qbox.shape[0], int(cull), _p(out), cnt)
     out = out[:cnt]
     return out[np.lexsort((out[:, 1], out[:, 0]))]
+
+
+def mesh_topology(edges, faces):
+    edges = _c(edges, np.uint32)
+    faces = _c(faces, np.uint32)
+    fae = np.empty((faces.shape[0], 3), np.uint32)
+    eaf = np.empty((edges.shape[0], 2), np.uint32)
+    ok = lib().or_mesh_topology(edges.shape[0], faces.shape[0], _p(edges), _p(faces), _p(fae), _p(eaf))
+    if not ok:
+        raise ValueError("non-manifold or inconsistent mesh")
+    return fae, eaf
+
+
+def ef_preprocess(ids, pairs, edges, faces, fae, eaf):
+    """Canonical EF list with loop pairs appended, and the 12-slot adjacency."""
+    ids = _c(ids, np.uint32).reshape(-1, 2)
+    pairs = _c(pairs, np.uint32).reshape(-1, 12)
+    P = ids.shape[0]
+    cap = 3 * max(P, 1)
+    oid = np.zeros((cap, 2), np.uint32)
+    opr = np.zeros((cap, 12), np.uint32)
+    adj = np.zeros((cap, 12), np.uint32)
+    adjn = np.zeros(cap, np.uint32)
+    n = lib().or_ef_preprocess(P, _p(ids), _p(pairs), _p(_c(edges, np.uint32)), _p(_c(faces, np.uint32)),
+                               _p(_c(fae, np.uint32)), _p(_c(eaf, np.uint32)), _p(oid), _p(opr), _p(adj), _p(adjn))
+    if n < 0:
+        raise RuntimeError("EF adjacency exceeded 12 entries")
+    return oid[:n], opr[:n], adj[:n], adjn[:n]
+
+
+def contours_dfs(pairs, adj, adjn):
+    pairs = _c(pairs, np.uint32)
+    N = pairs.shape[0]
+    ci = np.zeros(max(N, 1), np.uint32)
+    mi = np.zeros(max(N, 1), np.uint32)
+    nc = lib().or_contours_dfs(N, _p(pairs), _p(_c(adj, np.uint32)), _p(_c(adjn, np.uint32)), _p(ci), _p(mi))
+    return int(nc), ci[:N], mi[:N]
+
+
+def graph_bfs(row_ptr, col, w):
+    row_ptr = _c(row_ptr, np.uint32)
+    n = row_ptr.shape[0] - 1
+    cm = np.zeros(max(n, 1), np.uint32)
+    par = np.zeros(max(n, 1), np.uint32)
+    inc = np.zeros(max(n, 1), np.uint32)
+    any_inc = lib().or_graph_bfs(n, _p(row_ptr), _p(_c(col, np.uint32)), _p(_c(w, np.uint32)), _p(cm), _p(par),
+                                 _p(inc))
+    return cm[:n], par[:n], inc[:n], bool(any_inc)
+
+
+def block_csr(n, rows, cols, blocks):
+    """Block-CSR (row_ptr, col, blocks[k,9]) from block coordinates, duplicates summed."""
+    rows = np.asarray(rows, np.int64)
+    cols = np.asarray(cols, np.int64)
+    key = rows * n + cols
+    uk, inv = np.unique(key, return_inverse=True)
+    bl = np.zeros((len(uk), 9))
+    np.add.at(bl, inv, np.asarray(blocks, np.float64).reshape(-1, 9))
+    r = uk // n
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum(np.bincount(r, minlength=n))
+    return rp, (uk % n).astype(np.uint32), bl
+
+
+def pcg(n, rp, ci, bl, b, precond=0, max_iters=1000, rel_tol=1e-6):
+    """FP64 PCG restating host_solve_core; returns (x, iterations, rel_residual, converged, failure)."""
+    b = _c(b, np.float64).ravel()
+    x = np.zeros(3 * n)
+    out = np.zeros(3)
+    it = lib().or_pcg(n, _p(_c(rp, np.uint32)), _p(_c(ci, np.uint32)), _p(_c(bl, np.float64)), _p(b), _p(x),
+                      precond, max_iters, rel_tol, _p(out))
+    return x, int(it), float(out[0]), bool(out[1]), int(out[2])

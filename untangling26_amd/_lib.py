@@ -39,7 +39,27 @@ SIGNATURES = {
     "ut_morton30": (C.c_int, [vp, vp, C.c_uint32, vp]),
     "ut_ef_dcd": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, C.c_uint32]),
     "ut_ef_detect": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, C.c_uint32]),
+    "ut_contours_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.POINTER(vp)]),
+    "ut_contours_destroy": (C.c_int, [vp]),
+    "ut_contours_build": (C.c_int, [vp, C.c_uint32, vp, vp, u32p, u32p]),
+    "ut_contours_get": (C.c_int, [vp, C.c_int, vp]),
+    "ut_graph_components": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp]),
+    "ut_pcg_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
+    "ut_pcg_destroy": (C.c_int, [vp]),
+    "ut_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "ut_bsr_spmv": (C.c_int, [vp, vp, C.c_int, vp, vp]),
 }
+
+
+class PcgParams(C.Structure):
+    _fields_ = [("precision", C.c_int), ("preconditioner", C.c_int), ("max_iters", C.c_uint32),
+                ("rel_tol", C.c_double), ("inner_tol", C.c_double), ("max_refine", C.c_uint32),
+                ("regularization", C.c_double), ("check_every", C.c_uint32)]
+
+
+class PcgReport(C.Structure):
+    _fields_ = [("relative_residual", C.c_double), ("infinity_norm", C.c_double), ("iterations", C.c_uint32),
+                ("refinements", C.c_uint32), ("converged", C.c_int), ("failure", C.c_int)]
 
 
 def lib():

@@ -5,6 +5,8 @@
 #include "ut_runtime.hpp"
 #include "lbvh.hpp"
 #include "ef_dcd.hpp"
+#include "contour.hpp"
+#include "pcg.hpp"
 
 #include <new>
 
@@ -15,6 +17,14 @@ struct ut_ctx
 struct ut_lbvh
 {
 	ut::Lbvh b;
+};
+struct ut_contours
+{
+	ut::ContourBuilder b;
+};
+struct ut_pcg
+{
+	ut::PcgSolver s;
 };
 
 namespace
@@ -337,6 +347,167 @@ UT_API int ut_ef_detect(ut_lbvh* face_tree, const float* x, const uint32_t* edge
 			a.max_pairs = max_pairs;
 			ut::ef_detect(face_tree->b.ctx, face_tree->b, a, num_edges, thickness);
 		});
+}
+
+// ---- contours ---------------------------------------------------------------
+UT_API int ut_contours_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges,
+	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, ut_contours** out)
+{
+	if (!ctx || !out)
+		return ut::kInvalid;
+	*out = nullptr;
+	ut_contours* h = new (std::nothrow) ut_contours();
+	if (!h)
+		return ut::kInternal;
+	int rc = guarded(&ctx->c, [&] { h->b.init(&ctx->c, nv, ne, nf, edges, faces, face_adj_edges, edge_adj_faces); });
+	if (rc != ut::kOk)
+	{
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return ut::kOk;
+}
+
+UT_API int ut_contours_destroy(ut_contours* ct)
+{
+	if (ct)
+	{
+		if (ct->b.ctx)
+			(void)hipStreamSynchronize(ct->b.ctx->stream);
+		delete ct;
+	}
+	return ut::kOk;
+}
+
+UT_API int ut_contours_build(ut_contours* ct, uint32_t num_pairs, const void* pairs, const uint32_t* pair_ids,
+	uint32_t* num_total, uint32_t* num_contours)
+{
+	if (!ct)
+		return ut::kInvalid;
+	return guarded(ct->b.ctx,
+		[&]
+		{
+			ct->b.build(num_pairs, reinterpret_cast<const ut::EfPairRec*>(pairs), pair_ids);
+			if (num_total)
+				*num_total = ct->b.num_total;
+			if (num_contours)
+				*num_contours = ct->b.num_contours;
+		});
+}
+
+UT_API int ut_contours_get(ut_contours* ct, int what, void* dst)
+{
+	if (!ct)
+		return ut::kInvalid;
+	return guarded(ct->b.ctx,
+		[&]
+		{
+			auto&		 b = ct->b;
+			const size_t n = b.num_total;
+			const void*	 src = nullptr;
+			size_t		 bytes = 0;
+			switch (what)
+			{
+			case UT_CT_PAIR_IDS: src = b.ids.get(), bytes = 8 * n; break;
+			case UT_CT_PAIRS: src = b.pairs.get(), bytes = sizeof(ut::EfPairRec) * n; break;
+			case UT_CT_CONTOUR_INDEX: src = b.contour_index.get(), bytes = 4 * n; break;
+			case UT_CT_MESH_INDEX: src = b.mesh_index.get(), bytes = 4 * n; break;
+			case UT_CT_ADJ: src = b.adj.get(), bytes = 4 * ut::kMaxEfAdj * n; break;
+			case UT_CT_ADJ_N: src = b.adj_n.get(), bytes = 4 * n; break;
+			case UT_CT_INCONSISTENT: src = b.comp_inconsistent.get(), bytes = 4 * n; break;
+			case UT_CT_COMP_MIN: src = b.comp_min.get(), bytes = 4 * n; break;
+			default: throw ut::Error(ut::kInvalid, "unknown contour output");
+			}
+			UT_REQUIRE(dst || bytes == 0, "null destination");
+			if (bytes)
+				UT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
+		});
+}
+
+UT_API int ut_graph_components(ut_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+	const uint32_t* flip_w, uint32_t* scratch, uint32_t* comp_min, uint32_t* parity, uint32_t* inconsistent)
+{
+	if (!ctx)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			UT_REQUIRE(n == 0 || (row_ptr && scratch && comp_min && parity && inconsistent), "null pointer");
+			ut::graph_components(&ctx->c, n, row_ptr, col_idx, flip_w, scratch, comp_min, parity, inconsistent, nullptr);
+		});
+}
+
+// ---- PCG ---------------------------------------------------------------------
+UT_API int ut_pcg_create(ut_ctx* ctx, uint32_t n, uint32_t nslices, uint32_t nslots, const uint32_t* slice_off,
+	const uint32_t* col, const uint32_t* diag_slot, ut_pcg** out)
+{
+	if (!ctx || !out)
+		return ut::kInvalid;
+	*out = nullptr;
+	ut_pcg* h = new (std::nothrow) ut_pcg();
+	if (!h)
+		return ut::kInternal;
+	int rc = guarded(&ctx->c, [&] { h->s.init(&ctx->c, n, nslices, nslots, slice_off, col, diag_slot); });
+	if (rc != ut::kOk)
+	{
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return ut::kOk;
+}
+
+UT_API int ut_pcg_destroy(ut_pcg* pcg)
+{
+	if (pcg)
+	{
+		if (pcg->s.ctx)
+			(void)hipStreamSynchronize(pcg->s.ctx->stream);
+		delete pcg;
+	}
+	return ut::kOk;
+}
+
+UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* vals, const double* b, double* x,
+	ut_pcg_report* report)
+{
+	if (!pcg || !params)
+		return ut::kInvalid;
+	ut::PcgReport rep;
+	int			  rc = guarded(pcg->s.ctx,
+		 [&]
+		 {
+			 ut::PcgParams p;
+			 p.precision = params->precision;
+			 p.preconditioner = params->preconditioner;
+			 p.max_iters = params->max_iters;
+			 p.rel_tol = params->rel_tol;
+			 p.inner_tol = params->inner_tol;
+			 p.max_refine = params->max_refine;
+			 p.regularization = params->regularization;
+			 p.check_every = params->check_every ? params->check_every : 8;
+			 rep = pcg->s.solve(p, vals, b, x);
+		 });
+	if (rc != ut::kOk)
+		return rc;
+	if (report)
+	{
+		report->relative_residual = rep.relative_residual;
+		report->infinity_norm = rep.infinity_norm;
+		report->iterations = rep.iterations;
+		report->refinements = rep.refinements;
+		report->converged = rep.converged;
+		report->failure = rep.failure;
+	}
+	return rep.converged ? ut::kOk : ut::kNotConverged;
+}
+
+UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const double* x, double* y)
+{
+	if (!pcg)
+		return ut::kInvalid;
+	return guarded(pcg->s.ctx, [&] { pcg->s.spmv64(vals, vals_fp32 != 0, x, y); });
 }
 
 } // extern "C"

@@ -1,0 +1,63 @@
+// Intersection contours from EF pairs on the GPU.
+//
+// Replaces the host side of IntersectionResolver::host_preprocess_ef_pairs
+// (intersection_resolver.cpp:683-1096: canonical sort, EF adjacency,
+// loop-vertex pairs) and intersection_contour_construction_from_ext_adjacent /
+// contour_flood_filling_template (intersection_resolver2.cpp:254-309,
+// intersection_resolver_helper.cpp:518-575: components with mesh parity).
+#pragma once
+
+#include "ut_runtime.hpp"
+#include "ef_dcd.hpp"
+
+namespace ut
+{
+
+constexpr int kMaxEfAdj = 12; // 4 (shared face) + 6 (edge of the face) + 2 loop pairs
+
+struct ContourBuilder
+{
+	Context*		ctx = nullptr;
+	uint32_t		nv = 0, ne = 0, nf = 0;
+	const uint32_t* edges = nullptr;		  // [2E]  caller's device arrays
+	const uint32_t* faces = nullptr;		  // [3F]
+	const uint32_t* face_adj_edges = nullptr; // [3F]  edge between face[j], face[j+1]
+	const uint32_t* edge_adj_faces = nullptr; // [2E]  ascending, kNone when absent
+
+	uint32_t num_pairs = 0;	   // DCD pairs (before loop pairs)
+	uint32_t num_total = 0;	   // + loop pairs
+	uint32_t num_contours = 0;
+	uint32_t cap = 0;
+
+	DevBuf<uint32_t>  sort_k[2], sort_v[2], sort_h, scan_tmp, scalars;
+	DevBuf<uint64_t>  keys;		  // sorted (eid << 32 | fid) of the DCD pairs
+	DevBuf<uint32_t>  ids;		  // [2 * total] (eid, fid), sorted, loop pairs appended
+	DevBuf<EfPairRec> pairs;	  // [total]
+	DevBuf<uint32_t>  loop_cnt;	  // [P]
+	DevBuf<uint32_t>  loop_off;	  // [P]
+	DevBuf<uint32_t>  adj;		  // [kMaxEfAdj * total]
+	DevBuf<uint32_t>  adj_n;	  // [total]
+	DevBuf<uint32_t>  uf;		  // [2 * total] union-find parents of the parity-doubled graph
+	DevBuf<uint32_t>  comp_min;	  // [total] smallest pair index of the component
+	DevBuf<uint32_t>  mesh_index; // [total] parity relative to that pair
+	DevBuf<uint32_t>  is_root;	  // [total]
+	DevBuf<uint32_t>  rank;		  // [total]
+	DevBuf<uint32_t>  contour_index;	// [total]
+	DevBuf<uint32_t>  comp_inconsistent; // [total] indexed by component min pair
+
+	void init(Context* c, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges, const uint32_t* faces,
+		const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces);
+	// num_pairs DCD pairs in any order; synchronises to learn the loop-pair and contour counts.
+	void build(uint32_t num_pairs, const EfPairRec* in_pairs, const uint32_t* in_ids);
+	void reserve(uint32_t p);
+};
+
+// Connected components with parity of an undirected graph in CSR form (the
+// interface of UnitTest/test_parallel_contour_construction.cpp:281-448):
+// comp_min[v] = smallest vertex of v's component, parity[v] = parity of v
+// relative to it, inconsistent[comp_min] = 1 when the component has an odd cycle.
+void graph_components(Context* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+	const uint32_t* flip_w, uint32_t* uf, uint32_t* comp_min, uint32_t* parity, uint32_t* inconsistent,
+	uint32_t* status);
+
+} // namespace ut

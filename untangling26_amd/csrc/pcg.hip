@@ -1,0 +1,643 @@
+// PCG kernels for gfx950.  See pcg.hpp for the matrix layout and modes.
+//
+// One iteration is three launches with no host round trip: alpha and beta are
+// recomputed on the device by every workgroup from per-workgroup FP64
+// partials (fixed summation order, so the result is deterministic), and a
+// device flag stops the iteration once ||r|| <= tol ||b||; the host only polls
+// that flag every `check_every` iterations.  Compare the reference's two host
+// synchronisations per iteration (precond_cg.cpp:810-816, 837-841).
+#include "ut_common.hpp"
+#include "pcg.hpp"
+
+#include <cmath>
+#include <cstring>
+
+namespace ut
+{
+
+constexpr int kPcgThreads = 256;
+constexpr int kSlice = 64;
+
+// ---- reductions -------------------------------------------------------------
+__device__ __forceinline__ void block_partial(double v, double* out)
+{
+	__shared__ double ws[kPcgThreads / kWave];
+	v = wave_sum(v);
+	if (__lane_id() == 0)
+		ws[threadIdx.x / kWave] = v;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		double s = 0.0;
+		for (int q = 0; q < kPcgThreads / kWave; ++q)
+			s += ws[q];
+		out[blockIdx.x] = s;
+	}
+	__syncthreads(); // ws is reused by the next call
+}
+
+// Reads the stop flag once per workgroup so that every thread of the group
+// takes the same branch (block 0 of the same launch may be writing it).
+__device__ __forceinline__ bool group_stopped(const uint32_t* flags)
+{
+	__shared__ uint32_t s_stop;
+	if (threadIdx.x == 0)
+		s_stop = __hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	__syncthreads();
+	return s_stop != 0u;
+}
+
+// Sum of nb partials, identical in every workgroup (fixed order).
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, uint32_t nb)
+{
+	__shared__ double s_val;
+	if (threadIdx.x < kWave)
+	{
+		double v = 0.0;
+		for (uint32_t i = threadIdx.x; i < nb; i += kWave)
+			v += part[i];
+		v = wave_sum(v);
+		if (threadIdx.x == 0)
+			s_val = v;
+	}
+	__syncthreads();
+	const double r = s_val;
+	__syncthreads();
+	return r;
+}
+
+template <typename T>
+struct V3
+{
+	T x, y, z;
+};
+template <typename T>
+__device__ __forceinline__ V3<T> ldv(const T* __restrict__ p, uint32_t i)
+{
+	return V3<T>{ p[3 * i], p[3 * i + 1], p[3 * i + 2] };
+}
+template <typename T>
+__device__ __forceinline__ void stv(T* __restrict__ p, uint32_t i, V3<T> v)
+{
+	p[3 * i] = v.x;
+	p[3 * i + 1] = v.y;
+	p[3 * i + 2] = v.z;
+}
+template <typename T>
+__device__ __forceinline__ double dotd(V3<T> a, V3<T> b)
+{
+	return (double)a.x * (double)b.x + (double)a.y * (double)b.y + (double)a.z * (double)b.z;
+}
+
+// y_row = sum_k B_k x_col(k) over the SELL slots of `row`; TA = accumulation type.
+template <typename TV, typename TX, typename TA>
+__device__ __forceinline__ V3<TA> row_spmv(uint32_t row, const uint32_t* __restrict__ slice_off,
+	const uint32_t* __restrict__ col, const TV* __restrict__ vals, uint32_t nslots, const TX* __restrict__ x)
+{
+	const uint32_t s = row / kSlice, lane = row % kSlice;
+	const uint32_t off = slice_off[s];
+	const uint32_t w = (slice_off[s + 1] - off) / kSlice;
+	TA			   a0 = 0, a1 = 0, a2 = 0;
+	for (uint32_t k = 0; k < w; ++k)
+	{
+		const uint32_t slot = off + k * kSlice + lane;
+		const uint32_t c = col[slot];
+		const TA	   x0 = (TA)x[3 * c], x1 = (TA)x[3 * c + 1], x2 = (TA)x[3 * c + 2];
+		a0 += (TA)vals[0 * (size_t)nslots + slot] * x0 + (TA)vals[1 * (size_t)nslots + slot] * x1
+			+ (TA)vals[2 * (size_t)nslots + slot] * x2;
+		a1 += (TA)vals[3 * (size_t)nslots + slot] * x0 + (TA)vals[4 * (size_t)nslots + slot] * x1
+			+ (TA)vals[5 * (size_t)nslots + slot] * x2;
+		a2 += (TA)vals[6 * (size_t)nslots + slot] * x0 + (TA)vals[7 * (size_t)nslots + slot] * x1
+			+ (TA)vals[8 * (size_t)nslots + slot] * x2;
+	}
+	return V3<TA>{ a0, a1, a2 };
+}
+
+// ---- preconditioner -----------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_precond(uint32_t n, const T* __restrict__ vals, uint32_t nslots,
+	const uint32_t* __restrict__ diag_slot, int mode, double lambda, T* __restrict__ minv, T* __restrict__ scale)
+{
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= n)
+		return;
+	const uint32_t slot = diag_slot[row];
+	double		   D[9];
+	for (int c = 0; c < 9; ++c)
+		D[c] = (double)vals[(size_t)c * nslots + slot];
+	// pcg_regularization_scale, precond_cg.cpp:88-91
+	const double sc = fmax((D[0] + D[4] + D[8]) * (1.0 / 3.0), 0.0);
+	scale[row] = (T)sc;
+	D[0] += lambda * sc;
+	D[4] += lambda * sc;
+	D[8] += lambda * sc;
+	double M[9] = { 0, 0, 0, 0, 0, 0, 0, 0, 0 };
+	bool   done = false;
+	if (mode == kPrecBlockJacobi)
+	{
+		const double c00 = D[4] * D[8] - D[5] * D[7], c01 = D[5] * D[6] - D[3] * D[8], c02 = D[3] * D[7] - D[4] * D[6];
+		const double det = D[0] * c00 + D[1] * c01 + D[2] * c02;
+		const double mag = fabs(D[0]) + fabs(D[4]) + fabs(D[8]);
+		if (isfinite(det) && fabs(det) > 1e-30 * mag * mag * mag && mag > 0.0)
+		{
+			const double id = 1.0 / det;
+			M[0] = c00 * id;
+			M[1] = (D[2] * D[7] - D[1] * D[8]) * id;
+			M[2] = (D[1] * D[5] - D[2] * D[4]) * id;
+			M[3] = c01 * id;
+			M[4] = (D[0] * D[8] - D[2] * D[6]) * id;
+			M[5] = (D[2] * D[3] - D[0] * D[5]) * id;
+			M[6] = c02 * id;
+			M[7] = (D[1] * D[6] - D[0] * D[7]) * id;
+			M[8] = (D[0] * D[4] - D[1] * D[3]) * id;
+			done = true;
+		}
+	}
+	if (!done)
+	{
+		// make_pcg_jacobi_inverse, precond_cg.cpp:93-111
+		const double s = fmax(fmax(fabs(D[0]), fabs(D[4])), fabs(D[8]));
+		if (s >= 1e-10)
+		{
+			const double mn = fmax(1e-6, 1e-6 * s);
+			M[0] = D[0] > mn ? 1.0 / D[0] : 0.0;
+			M[4] = D[4] > mn ? 1.0 / D[4] : 0.0;
+			M[8] = D[8] > mn ? 1.0 / D[8] : 0.0;
+		}
+	}
+	for (int c = 0; c < 9; ++c)
+		minv[(size_t)c * n + row] = (T)M[c];
+}
+
+template <typename T>
+__device__ __forceinline__ V3<T> apply_minv(const T* __restrict__ minv, uint32_t n, uint32_t row, V3<T> r)
+{
+	const T* m = minv;
+	return V3<T>{ m[0 * (size_t)n + row] * r.x + m[1 * (size_t)n + row] * r.y + m[2 * (size_t)n + row] * r.z,
+		m[3 * (size_t)n + row] * r.x + m[4 * (size_t)n + row] * r.y + m[5 * (size_t)n + row] * r.z,
+		m[6 * (size_t)n + row] * r.x + m[7 * (size_t)n + row] * r.y + m[8 * (size_t)n + row] * r.z };
+}
+
+// ---- init: x = 0, r = rhs, z = M r, p = z ------------------------------------
+struct PcgParts
+{
+	double* pq;		// [nb]
+	double* rz[2];	// [nb] ping-pong by iteration parity
+	double* rr;		// [nb]
+	double* bb;		// [nb]
+};
+
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_init(uint32_t n, const double* __restrict__ rhs,
+	const T* __restrict__ minv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, T* __restrict__ p,
+	PcgParts parts, uint32_t* __restrict__ flags)
+{
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   rz = 0.0, bb = 0.0;
+	if (row < n)
+	{
+		const V3<T> rv{ (T)rhs[3 * row], (T)rhs[3 * row + 1], (T)rhs[3 * row + 2] };
+		const V3<T> zv = apply_minv(minv, n, row, rv);
+		stv(x, row, V3<T>{ 0, 0, 0 });
+		stv(r, row, rv);
+		stv(z, row, zv);
+		stv(p, row, zv);
+		rz = dotd(rv, zv);
+		bb = rhs[3 * row] * rhs[3 * row] + rhs[3 * row + 1] * rhs[3 * row + 1] + rhs[3 * row + 2] * rhs[3 * row + 2];
+	}
+	block_partial(rz, parts.rz[1]);
+	block_partial(bb, parts.bb);
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		flags[0] = 0u;
+		flags[1] = 0u;
+		flags[2] = 0u;
+	}
+}
+
+// Marks the solve done before the first iteration when rhs == 0 or non-finite.
+__global__ void k_pcg_check0(uint32_t nb, PcgParts parts, uint32_t* flags)
+{
+	const double bb = sum_partials(parts.bb, nb);
+	const double rz = sum_partials(parts.rz[1], nb);
+	if (threadIdx.x == 0)
+	{
+		if (!isfinite(bb) || !isfinite(rz))
+		{
+			flags[2] = 1u;
+			flags[0] = 1u;
+		}
+		else if (bb == 0.0)
+			flags[0] = 1u;
+	}
+}
+
+// ---- iteration ------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint32_t* __restrict__ slice_off,
+	const uint32_t* __restrict__ col, const T* __restrict__ vals, uint32_t nslots, const T* __restrict__ p,
+	T* __restrict__ q, double lambda, const T* __restrict__ scale, PcgParts parts, const uint32_t* __restrict__ flags)
+{
+	if (group_stopped(flags))
+		return;
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   pq = 0.0;
+	if (row < n)
+	{
+		V3<T>		a = row_spmv<T, T, T>(row, slice_off, col, vals, nslots, p);
+		const V3<T> pr = ldv(p, row);
+		if (lambda != 0.0)
+		{
+			const T ls = (T)lambda * scale[row];
+			a.x += ls * pr.x;
+			a.y += ls * pr.y;
+			a.z += ls * pr.z;
+		}
+		stv(q, row, a);
+		pq = dotd(pr, a);
+	}
+	block_partial(pq, parts.pq);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_update(uint32_t it, uint32_t n, uint32_t nb,
+	const T* __restrict__ minv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+	const T* __restrict__ q, PcgParts parts, uint32_t* __restrict__ flags)
+{
+	if (group_stopped(flags))
+		return;
+	const double pq = sum_partials(parts.pq, nb);
+	const double rz = sum_partials(parts.rz[(it + 1) & 1], nb);
+	const double alpha = rz / pq;
+	const bool	 ok = isfinite(alpha) && pq > 0.0;
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   rzn = 0.0, rr = 0.0;
+	if (ok && row < n)
+	{
+		const T		a = (T)alpha;
+		const V3<T> pv = ldv(p, row), qv = ldv(q, row);
+		V3<T>		xv = ldv(x, row), rv = ldv(r, row);
+		xv.x += a * pv.x;
+		xv.y += a * pv.y;
+		xv.z += a * pv.z;
+		rv.x -= a * qv.x;
+		rv.y -= a * qv.y;
+		rv.z -= a * qv.z;
+		const V3<T> zv = apply_minv(minv, n, row, rv);
+		stv(x, row, xv);
+		stv(r, row, rv);
+		stv(z, row, zv);
+		rzn = dotd(rv, zv);
+		rr = dotd(rv, rv);
+	}
+	if (!ok && blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		flags[2] = 2u; // breakdown: p^T A p <= 0 or non-finite
+		flags[0] = 1u;
+	}
+	block_partial(rzn, parts.rz[it & 1]);
+	block_partial(rr, parts.rr);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(uint32_t it, uint32_t n, uint32_t nb, double tol2,
+	const T* __restrict__ z, T* __restrict__ p, PcgParts parts, uint32_t* __restrict__ flags)
+{
+	if (group_stopped(flags))
+		return;
+	const double rzn = sum_partials(parts.rz[it & 1], nb);
+	const double rzo = sum_partials(parts.rz[(it + 1) & 1], nb);
+	const double rr = sum_partials(parts.rr, nb);
+	const double bb = sum_partials(parts.bb, nb);
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		flags[1] = it + 1;
+		if (!isfinite(rr))
+		{
+			flags[2] = 2u;
+			flags[0] = 1u;
+		}
+		else if (rr <= tol2 * bb)
+			flags[0] = 1u;
+	}
+	const double beta = rzn / rzo;
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	if (row < n && isfinite(beta))
+	{
+		const T		b = (T)beta;
+		const V3<T> zv = ldv(z, row);
+		V3<T>		pv = ldv(p, row);
+		pv.x = zv.x + b * pv.x;
+		pv.y = zv.y + b * pv.y;
+		pv.z = zv.z + b * pv.z;
+		stv(p, row, pv);
+	}
+}
+
+// ---- FP64 residual and helpers -----------------------------------------------------
+// res = b - A x with FP64 accumulation (TV = stored value type); partial ||res||^2.
+template <typename TV>
+__global__ __launch_bounds__(kPcgThreads) void k_residual64(uint32_t n, const uint32_t* __restrict__ slice_off,
+	const uint32_t* __restrict__ col, const TV* __restrict__ vals, uint32_t nslots, const double* __restrict__ x,
+	const double* __restrict__ b, double* __restrict__ res, double lambda, const double* __restrict__ scale64,
+	double* __restrict__ part)
+{
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   rr = 0.0;
+	if (row < n)
+	{
+		V3<double> a = row_spmv<TV, double, double>(row, slice_off, col, vals, nslots, x);
+		if (lambda != 0.0)
+		{
+			const double ls = lambda * scale64[row];
+			a.x += ls * x[3 * row];
+			a.y += ls * x[3 * row + 1];
+			a.z += ls * x[3 * row + 2];
+		}
+		const V3<double> rv{ b[3 * row] - a.x, b[3 * row + 1] - a.y, b[3 * row + 2] - a.z };
+		if (res)
+			stv(res, row, rv);
+		rr = dotd(rv, rv);
+	}
+	block_partial(rr, part);
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kPcgThreads) void k_spmv64(uint32_t n, const uint32_t* __restrict__ slice_off,
+	const uint32_t* __restrict__ col, const TV* __restrict__ vals, uint32_t nslots, const double* __restrict__ x,
+	double* __restrict__ y)
+{
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	if (row < n)
+		stv(y, row, row_spmv<TV, double, double>(row, slice_off, col, vals, nslots, x));
+}
+
+template <typename T>
+__global__ void k_axpy_to64(uint32_t m, const T* __restrict__ d, double* __restrict__ x)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m)
+		x[i] += (double)d[i];
+}
+template <typename T>
+__global__ void k_copy_to64(uint32_t m, const T* __restrict__ d, double* __restrict__ x)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m)
+		x[i] = (double)d[i];
+}
+__global__ void k_scale_to64(uint32_t n, const float* __restrict__ s, double* __restrict__ o)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		o[i] = (double)s[i];
+}
+__global__ void k_sum_partials_out(const double* __restrict__ part, uint32_t nb, double* __restrict__ out)
+{
+	const double s = sum_partials(part, nb);
+	if (threadIdx.x == 0)
+		*out = s;
+}
+__global__ void k_inf_norm(uint32_t n, const double* __restrict__ x, uint32_t* __restrict__ out_bits)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const double v = sqrt(x[3 * i] * x[3 * i] + x[3 * i + 1] * x[3 * i + 1] + x[3 * i + 2] * x[3 * i + 2]);
+	const float	 f = (float)v;
+	atomicMax(out_bits, __float_as_uint(isfinite(f) ? f : 3.0e38f)); // non-negative floats order like uints
+}
+
+// ---- host -----------------------------------------------------------------------------
+void PcgSolver::init(Context* c, uint32_t n_, uint32_t nslices_, uint32_t nslots_, const uint32_t* so,
+	const uint32_t* cl, const uint32_t* ds)
+{
+	UT_REQUIRE(so && cl && ds, "pcg: null pattern");
+	UT_REQUIRE(nslices_ == div_up(n_, kSlice), "pcg: slices must cover rows in groups of 64");
+	ctx = c;
+	n = n_;
+	nslices = nslices_;
+	nslots = nslots_;
+	slice_off = so;
+	col = cl;
+	diag_slot = ds;
+	const uint32_t nb = div_up(n, kPcgThreads);
+	// x, r, z, p, q (3n each), minv (9n), scale (n), r64 (3n), scale64 (n), d64 spare (3n) in doubles
+	// + 256 B of alignment slack per carved array (10 arrays)
+	work.reserve(sizeof(double) * ((size_t)5 * 3 * n + 9 * n + n + 3 * n + n + 3 * n) + 10 * 256);
+	partials.reserve(8 * (size_t)nb + 8);
+	flags.reserve(4);
+	if (!h_flags)
+		UT_HIP_CHECK(hipHostMalloc((void**)&h_flags, 16, hipHostMallocDefault));
+	if (!h_norm)
+		UT_HIP_CHECK(hipHostMalloc((void**)&h_norm, 16, hipHostMallocDefault));
+}
+
+PcgSolver::~PcgSolver()
+{
+	if (h_flags)
+		(void)hipHostFree(h_flags);
+	if (h_norm)
+		(void)hipHostFree(h_norm);
+}
+
+namespace
+{
+template <typename T>
+struct Vecs
+{
+	T *x, *r, *z, *p, *q, *minv, *scale;
+	double *r64, *scale64, *spare;
+};
+template <typename T>
+Vecs<T> carve(uint8_t* base, uint32_t n)
+{
+	Vecs<T> v;
+	size_t	off = 0;
+	auto	take = [&](size_t count, size_t elem) -> uint8_t*
+	{
+		uint8_t* p = base + off;
+		off += ((count * elem + 255) / 256) * 256;
+		return p;
+	};
+	v.x = (T*)take(3 * (size_t)n, sizeof(T));
+	v.r = (T*)take(3 * (size_t)n, sizeof(T));
+	v.z = (T*)take(3 * (size_t)n, sizeof(T));
+	v.p = (T*)take(3 * (size_t)n, sizeof(T));
+	v.q = (T*)take(3 * (size_t)n, sizeof(T));
+	v.minv = (T*)take(9 * (size_t)n, sizeof(T));
+	v.scale = (T*)take(n, sizeof(T));
+	v.r64 = (double*)take(3 * (size_t)n, sizeof(double));
+	v.scale64 = (double*)take(n, sizeof(double));
+	v.spare = (double*)take(3 * (size_t)n, sizeof(double));
+	return v;
+}
+
+// Runs PCG iterations of one (inner) solve until done or the budget is used.
+template <typename T>
+uint32_t run_iterations(PcgSolver& s, const Vecs<T>& v, const T* vals, PcgParts parts, double lambda, double tol,
+	uint32_t budget, uint32_t check_every)
+{
+	hipStream_t	   st = s.ctx->stream;
+	const uint32_t nb = div_up(s.n, kPcgThreads);
+	const double   tol2 = tol * tol;
+	uint32_t	   it = 0;
+	while (it < budget)
+	{
+		const uint32_t chunk = std::min(check_every, budget - it);
+		for (uint32_t k = 0; k < chunk; ++k, ++it)
+		{
+			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
+				s.n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, lambda, v.scale, parts, s.flags.get());
+			k_pcg_update<T><<<nb, kPcgThreads, 0, st>>>(
+				it, s.n, nb, v.minv, v.x, v.r, v.z, v.p, v.q, parts, s.flags.get());
+			k_pcg_dir<T><<<nb, kPcgThreads, 0, st>>>(it, s.n, nb, tol2, v.z, v.p, parts, s.flags.get());
+		}
+		UT_LAUNCH_CHECK();
+		UT_HIP_CHECK(hipMemcpyAsync(s.h_flags, s.flags.get(), 12, hipMemcpyDeviceToHost, st));
+		UT_HIP_CHECK(hipStreamSynchronize(st));
+		if (s.h_flags[0])
+			break;
+	}
+	return s.h_flags[1];
+}
+
+template <typename T>
+PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const double* b, double* x)
+{
+	hipStream_t	   st = s.ctx->stream;
+	const uint32_t n = s.n, nb = div_up(n, kPcgThreads);
+	Vecs<T>		   v = carve<T>(s.work.get(), n);
+	double*		   pb = s.partials.get();
+	PcgParts	   parts{ pb, { pb + nb, pb + 2 * nb }, pb + 3 * nb, pb + 4 * nb };
+	double*		   res_part = pb + 5 * nb;
+	double*		   d_norm = pb + 6 * nb;
+	PcgReport	   rep;
+	const bool	   mixed = prm.precision == kPcgMixed;
+
+	k_pcg_precond<T><<<nb, kPcgThreads, 0, st>>>(
+		n, vals, s.nslots, s.diag_slot, prm.preconditioner, prm.regularization, v.minv, v.scale);
+	k_copy_to64<T><<<nb, kPcgThreads, 0, st>>>(n, v.scale, v.scale64);
+	UT_HIP_CHECK(hipMemsetAsync(x, 0, sizeof(double) * 3 * n, st));
+
+	auto residual_norm = [&]() -> double
+	{
+		k_residual64<T><<<nb, kPcgThreads, 0, st>>>(
+			n, s.slice_off, s.col, vals, s.nslots, x, b, v.r64, prm.regularization, v.scale64, res_part);
+		k_sum_partials_out<<<1, kPcgThreads, 0, st>>>(res_part, nb, d_norm);
+		UT_HIP_CHECK(hipMemcpyAsync(s.h_norm, d_norm, 8, hipMemcpyDeviceToHost, st));
+		UT_HIP_CHECK(hipStreamSynchronize(st));
+		return std::sqrt(s.h_norm[0]);
+	};
+
+	// ||b|| (x = 0, so the residual is b)
+	const double bn = residual_norm();
+	if (!std::isfinite(bn))
+	{
+		rep.failure = 1;
+		return rep;
+	}
+	if (bn == 0.0)
+	{
+		rep.converged = 1;
+		return rep;
+	}
+	if (prm.max_iters == 0)
+	{
+		rep.failure = 3;
+		rep.relative_residual = 1.0;
+		return rep;
+	}
+	uint32_t used = 0;
+	double	 rn = bn;
+	bool	 tol_reached = false;
+	if (!mixed)
+	{
+		k_pcg_init<T><<<nb, kPcgThreads, 0, st>>>(n, b, v.minv, v.x, v.r, v.z, v.p, parts, s.flags.get());
+		k_pcg_check0<<<1, kPcgThreads, 0, st>>>(nb, parts, s.flags.get());
+		used = run_iterations<T>(s, v, vals, parts, prm.regularization, prm.rel_tol, prm.max_iters, prm.check_every);
+		if (s.h_flags[2])
+			rep.failure = 2;
+		// converged by the iteration's own criterion (recursive residual), as the reference does
+		tol_reached = s.h_flags[0] != 0u && s.h_flags[2] == 0u;
+		k_copy_to64<T><<<div_up(3 * n, 256), 256, 0, st>>>(3 * n, v.x, x);
+		rn = residual_norm();
+	}
+	else
+	{
+		// iterative refinement: inner FP32 solves of A d = r, x += d, r = b - A x in FP64
+		for (uint32_t k = 0; k < prm.max_refine && used < prm.max_iters; ++k)
+		{
+			if (!(rn > prm.rel_tol * bn))
+				break;
+			k_pcg_init<T><<<nb, kPcgThreads, 0, st>>>(n, v.r64, v.minv, v.x, v.r, v.z, v.p, parts, s.flags.get());
+			k_pcg_check0<<<1, kPcgThreads, 0, st>>>(nb, parts, s.flags.get());
+			// the inner solve only has to gain the remaining factor, bounded by inner_tol
+			const double want = std::max(prm.inner_tol, 0.5 * prm.rel_tol * bn / rn);
+			used += run_iterations<T>(s, v, vals, parts, prm.regularization, want, prm.max_iters - used, prm.check_every);
+			if (s.h_flags[2] == 1u)
+			{
+				rep.failure = 2;
+				break;
+			}
+			k_axpy_to64<T><<<div_up(3 * n, 256), 256, 0, st>>>(3 * n, v.x, x);
+			rep.refinements = k + 1;
+			const double prev = rn;
+			rn = residual_norm();
+			if (!std::isfinite(rn))
+			{
+				rep.failure = 2;
+				break;
+			}
+			if (rn >= prev)
+				break; // no progress: stop refining
+		}
+	}
+	rep.iterations = used;
+	rep.relative_residual = rn / bn; // true residual, recomputed in FP64
+	if (mixed)
+		tol_reached = rep.relative_residual <= prm.rel_tol;
+	rep.converged = (rep.failure == 0 && tol_reached) ? 1 : 0;
+	uint32_t* inf_bits = reinterpret_cast<uint32_t*>(d_norm + 1);
+	UT_HIP_CHECK(hipMemsetAsync(inf_bits, 0, 4, st));
+	k_inf_norm<<<nb, kPcgThreads, 0, st>>>(n, x, inf_bits);
+	uint32_t bits = 0;
+	UT_HIP_CHECK(hipMemcpyAsync(&bits, inf_bits, 4, hipMemcpyDeviceToHost, st));
+	UT_HIP_CHECK(hipStreamSynchronize(st));
+	float f;
+	std::memcpy(&f, &bits, 4);
+	rep.infinity_norm = f;
+	return rep;
+}
+} // namespace
+
+PcgReport PcgSolver::solve(const PcgParams& p, const void* vals, const double* b, double* x)
+{
+	UT_REQUIRE(vals && b && x, "pcg: null input");
+	UT_REQUIRE(p.check_every >= 1, "pcg: check_every must be >= 1");
+	if (n == 0)
+	{
+		PcgReport r;
+		r.converged = 1;
+		return r;
+	}
+	if (p.precision == kPcgFp64)
+		return solve_typed<double>(*this, p, reinterpret_cast<const double*>(vals), b, x);
+	UT_REQUIRE(p.precision == kPcgMixed || p.precision == kPcgFp32, "pcg: unknown precision");
+	return solve_typed<float>(*this, p, reinterpret_cast<const float*>(vals), b, x);
+}
+
+void PcgSolver::spmv64(const void* vals, bool vals_fp32, const double* x, double* y)
+{
+	UT_REQUIRE(vals && x && y, "spmv: null input");
+	const uint32_t nb = div_up(n, kPcgThreads);
+	if (vals_fp32)
+		k_spmv64<float><<<nb, kPcgThreads, 0, ctx->stream>>>(
+			n, slice_off, col, reinterpret_cast<const float*>(vals), nslots, x, y);
+	else
+		k_spmv64<double><<<nb, kPcgThreads, 0, ctx->stream>>>(
+			n, slice_off, col, reinterpret_cast<const double*>(vals), nslots, x, y);
+	UT_LAUNCH_CHECK();
+}
+
+} // namespace ut

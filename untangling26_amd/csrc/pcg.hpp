@@ -1,0 +1,82 @@
+// Preconditioned conjugate gradients on a 3x3-block sparse matrix.
+//
+// Replaces ConjugateGradientSolver::device_solve (Solver/LinearSolver/
+// precond_cg.cpp:706-914) and its SpMV callback (newton_solver.cpp:1517-1555).
+//
+// Matrix layout: SELL-64 block rows.  Rows are grouped in slices of 64 (one
+// wavefront); slice s has width w_s = its longest block row and owns the slots
+// slice_off[s] .. slice_off[s] + 64 * w_s - 1, slot = slice_off[s] + k*64 + lane
+// for the k-th block of row 64*s + lane (short rows are padded with zero
+// blocks whose column is the row itself).  Block values are stored planar,
+// vals[c * nslots + slot] for entry c = 3*i + j of the block, so each of the 9
+// loads of a block is one contiguous 256 B (FP32) or 512 B (FP64) wave access.
+//
+// Precision modes:
+//   kPcgFp64   matrix, vectors and dots in FP64
+//   kPcgMixed  FP32 matrix and inner PCG vectors, FP64 dots, FP64 residual
+//              b - A x (FP32 values, FP64 accumulation) between inner solves
+//   kPcgFp32   FP32 matrix and vectors, FP64 dots (reference-like)
+#pragma once
+
+#include "ut_runtime.hpp"
+
+namespace ut
+{
+
+enum PcgPrecision : int
+{
+	kPcgFp64 = 0,
+	kPcgMixed = 1,
+	kPcgFp32 = 2,
+};
+enum PcgPreconditioner : int
+{
+	kPrecBlockJacobi = 0, // inverse of each 3x3 diagonal block
+	kPrecPointJacobi = 1, // make_pcg_jacobi_inverse, precond_cg.cpp:93-111
+};
+
+struct PcgParams
+{
+	int		 precision = kPcgMixed;
+	int		 preconditioner = kPrecBlockJacobi;
+	uint32_t max_iters = 100;	   // total PCG iterations (all inner solves together)
+	double	 rel_tol = 1e-6;	   // ||b - A x|| / ||b||
+	double	 inner_tol = 1e-2;	   // mixed: relative tolerance of each inner solve
+	uint32_t max_refine = 20;	   // mixed: refinement steps
+	double	 regularization = 0.0; // lambda: q += lambda * mean(diag) * p, precond_cg.cpp:286-339
+	uint32_t check_every = 8;	   // iterations launched between host convergence checks
+};
+
+struct PcgReport
+{
+	double	 relative_residual = 0.0; // final ||b - A x|| / ||b|| (recomputed in FP64 for mixed)
+	double	 infinity_norm = 0.0;	  // max_i |x_i| (block 2-norm)
+	uint32_t iterations = 0;
+	uint32_t refinements = 0;
+	int		 converged = 0;
+	int		 failure = 0; // 0 none, 1 non-finite rhs, 2 non-finite during iteration, 3 iteration budget 0
+};
+
+struct PcgSolver
+{
+	Context*		ctx = nullptr;
+	uint32_t		n = 0, nslices = 0, nslots = 0;
+	const uint32_t* slice_off = nullptr; // [nslices+1] device
+	const uint32_t* col = nullptr;		 // [nslots]
+	const uint32_t* diag_slot = nullptr; // [n]
+
+	DevBuf<uint8_t> work; // vectors (sized for FP64)
+	DevBuf<double>	partials;
+	DevBuf<uint32_t> flags; // [0] done, [1] iterations, [2] non-finite
+	uint32_t*		h_flags = nullptr; // pinned host mirror
+	double*			h_norm = nullptr;
+
+	void init(Context* c, uint32_t n, uint32_t nslices, uint32_t nslots, const uint32_t* slice_off, const uint32_t* col,
+		const uint32_t* diag_slot);
+	~PcgSolver();
+	PcgReport solve(const PcgParams& p, const void* vals, const double* b, double* x);
+	// y = A x (FP64 vectors; vals FP32 when vals_fp32, else FP64)
+	void spmv64(const void* vals, bool vals_fp32, const double* x, double* y);
+};
+
+} // namespace ut
