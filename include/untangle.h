@@ -194,6 +194,43 @@ UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* va
 /* y = A x (FP64 vectors and accumulation; vals FP32 when vals_fp32 != 0). */
 UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const double* x, double* y);
 
+/* ---- proximity DCD and CCD: NarrowPhasesDetector (narrow_phase.h:25-346) ---- */
+
+enum { UT_CONTACT_VF_DCD = 0, UT_CONTACT_EE_DCD = 1, UT_CONTACT_VF_CCD = 2, UT_CONTACT_EE_CCD = 3 };
+enum { UT_ENERGY_QUADRATIC = 0, UT_ENERGY_BARRIER = 1 };
+
+typedef struct ut_contact_args {
+	uint32_t		num_verts, num_edges;
+	const float*	x;		/* DCD positions, CCD start positions (float[3V]) */
+	const float*	x_end;	/* CCD end positions */
+	const uint32_t* edges;
+	const uint32_t* faces;
+	const uint32_t* vprop;	/* optional per-vertex property words */
+	const float*	d_hat;	/* per vertex */
+	const float*	offset; /* per vertex contact offset */
+	const float*	rest_vert_area;
+	const float*	rest_edge_area;
+	const float*	rest_face_area;
+	double			kappa;	/* stiffness_collision (scene_params.h:116) */
+	int				energy; /* UT_ENERGY_* (contact_energy_type) */
+	uint32_t*		count;	/* DCD: device count (may exceed max_pairs) */
+	void*			pairs;	/* DCD: CollisionPairTemplate[max_pairs] (64 B each) */
+	uint32_t*		pair_ids; /* DCD: uint32[2*max_pairs] (vid, fid) or (eid, eid) */
+	uint32_t		max_pairs;
+	float*			toi;	/* CCD: device float, minimum time of impact (reset with ut_toi_reset) */
+} ut_contact_args;
+
+/* kind UT_CONTACT_VF_DCD: fn_narrow_phase_vf_dcd_query (narrow_phase.cpp:980-1157);
+ * UT_CONTACT_EE_DCD: fn_narrow_phase_ee_dcd_query (:1159-1346); UT_CONTACT_VF_CCD /
+ * UT_CONTACT_EE_CCD: fn_narrow_phase_{vf,ee}_ccd_query (:412-800) with the global
+ * TOI minimum (use_global_ccd).  With tree != NULL (the refit face tree for VF,
+ * edge tree for EE) the broad phase runs in the same kernel; otherwise the
+ * broad pairs of ut_lbvh_query are used. */
+UT_API int ut_contact_query(ut_ctx* ctx, int kind, const ut_contact_args* args, ut_lbvh* tree,
+	const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap);
+/* Sets *toi = 1 (line_search_max_t) on the context stream (fn_reset_toi). */
+UT_API int ut_toi_reset(ut_ctx* ctx, float* toi);
+
 #ifdef __cplusplus
 }
 #endif

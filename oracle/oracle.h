@@ -92,6 +92,29 @@ int or_graph_bfs(uint32_t n, const uint32_t* row_ptr, const uint32_t* col, const
 uint32_t or_pcg(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const double* b, double* x,
 	int precond, uint32_t max_iters, double rel_tol, double* out);
 
+/* distance.hpp:69-160, 255-310 (FP64) */
+void or_pt_coeff(const double* p, const double* t0, const double* t1, const double* t2, double* out);
+void or_ee_coeff(const double* a0, const double* a1, const double* b0, const double* b1, double* out);
+/* cipc_kernel.hpp:15-36: deriv 0 = b, 1 = b', 2 = b'' */
+double or_barrier(double d, double dhat, int deriv);
+/* narrow_phase.cpp:980-1346 pair tests; rec = 16 words of CollisionPairTemplate */
+int or_vf_dcd(uint32_t vid, uint32_t fid, const float* x, const uint32_t* faces, const uint32_t* vprop,
+	const float* dh, const float* off, const float* rva, const float* rfa, double kappa, int energy, uint32_t* rec);
+int or_ee_dcd(uint32_t left, uint32_t right, const float* x, const uint32_t* edges, const uint32_t* vprop,
+	const float* dh, const float* off, const float* rea, double kappa, int energy, uint32_t* rec);
+/* accd.hpp:48-110 on 4 points x0[12] with displacement dx[12] */
+double or_accd(int is_ee, const double* x0, const double* dx, double offset);
+double or_toi_postprocess(double toi);
+double or_vf_ccd(uint32_t vid, uint32_t fid, const float* xs, const float* xe, const uint32_t* faces,
+	const uint32_t* vprop, const float* off);
+double or_ee_ccd(uint32_t left, uint32_t right, const float* xs, const float* xe, const uint32_t* edges,
+	const uint32_t* vprop, const float* off);
+size_t or_dcd_list(int kind, const uint32_t* broad, size_t nb, const float* x, const uint32_t* edges,
+	const uint32_t* faces, const uint32_t* vprop, const float* dh, const float* off, const float* rva,
+	const float* rea, const float* rfa, double kappa, int energy, uint32_t* recs, uint32_t* ids, size_t cap);
+double or_ccd_list(int kind, const uint32_t* broad, size_t nb, const float* xs, const float* xe,
+	const uint32_t* edges, const uint32_t* faces, const uint32_t* vprop, const float* off);
+
 #ifdef __cplusplus
 }
 #endif

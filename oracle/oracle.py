@@ -52,6 +52,19 @@ def lib():
         L.or_graph_bfs.restype = C.c_int
         L.or_pcg.argtypes = [C.c_uint32, p, p, p, p, p, C.c_int, C.c_uint32, C.c_double, p]
         L.or_pcg.restype = C.c_uint32
+        L.or_pt_coeff.argtypes = [p, p, p, p, p]
+        L.or_ee_coeff.argtypes = [p, p, p, p, p]
+        L.or_barrier.argtypes = [C.c_double, C.c_double, C.c_int]
+        L.or_barrier.restype = C.c_double
+        L.or_accd.argtypes = [C.c_int, p, p, C.c_double]
+        L.or_accd.restype = C.c_double
+        L.or_toi_postprocess.argtypes = [C.c_double]
+        L.or_toi_postprocess.restype = C.c_double
+        L.or_dcd_list.argtypes = [C.c_int, p, C.c_size_t, p, p, p, p, p, p, p, p, p, C.c_double, C.c_int, p, p,
+                                  C.c_size_t]
+        L.or_dcd_list.restype = C.c_size_t
+        L.or_ccd_list.argtypes = [C.c_int, p, C.c_size_t, p, p, p, p, p, p]
+        L.or_ccd_list.restype = C.c_double
         L.or_line_intersection.argtypes = [p, p, p, p, p, p, p]
         L.or_line_intersection.restype = C.c_int
         L.or_ef_dcd.argtypes = [p, C.c_size_t, p, p, p, p, p, p, p, C.c_size_t]
@@ -253,3 +266,49 @@ def pcg(n, rp, ci, bl, b, precond=0, max_iters=1000, rel_tol=1e-6):
     it = lib().or_pcg(n, _p(_c(rp, np.uint32)), _p(_c(ci, np.uint32)), _p(_c(bl, np.float64)), _p(b), _p(x),
                       precond, max_iters, rel_tol, _p(out))
     return x, int(it), float(out[0]), bool(out[1]), int(out[2])
+
+
+def pt_coeff(p, t0, t1, t2):
+    out = np.zeros(3)
+    lib().or_pt_coeff(*(_p(_c(v, np.float64)) for v in (p, t0, t1, t2)), _p(out))
+    return out
+
+
+def ee_coeff(a0, a1, b0, b1):
+    out = np.zeros(4)
+    lib().or_ee_coeff(*(_p(_c(v, np.float64)) for v in (a0, a1, b0, b1)), _p(out))
+    return out
+
+
+def barrier(d, dhat, deriv=0):
+    return lib().or_barrier(float(d), float(dhat), int(deriv))
+
+
+def accd(is_ee, x0, dx, offset):
+    return lib().or_accd(int(is_ee), _p(_c(x0, np.float64)), _p(_c(dx, np.float64)), float(offset))
+
+
+def toi_postprocess(t):
+    return lib().or_toi_postprocess(float(t))
+
+
+def dcd_list(kind, broad, x, edges, faces, d_hat, offset, rva, rea, rfa, kappa=1e8, energy=1, vprop=None):
+    """VF (kind 0) or EE (kind 1) DCD over broad pairs: (records uint32[n,16], ids uint32[n,2])."""
+    broad = _c(broad, np.uint32).reshape(-1, 2)
+    nb = broad.shape[0]
+    recs = np.zeros((max(nb, 1), 16), np.uint32)
+    ids = np.zeros((max(nb, 1), 2), np.uint32)
+    n = lib().or_dcd_list(kind, _p(broad), nb, _p(_c(x, np.float32)), _p(_c(edges, np.uint32)),
+                          _p(_c(faces, np.uint32)), _p(None if vprop is None else _c(vprop, np.uint32)),
+                          _p(_c(d_hat, np.float32)), _p(_c(offset, np.float32)), _p(_c(rva, np.float32)),
+                          _p(_c(rea, np.float32)), _p(_c(rfa, np.float32)), float(kappa), int(energy), _p(recs),
+                          _p(ids), nb)
+    return recs[:n], ids[:n]
+
+
+def ccd_list(kind, broad, xs, xe, edges, faces, offset, vprop=None):
+    """Global minimum TOI (FP32-rounded) of VF (kind 2) or EE (kind 3) pairs; 1.0 when none."""
+    broad = _c(broad, np.uint32).reshape(-1, 2)
+    return lib().or_ccd_list(kind, _p(broad), broad.shape[0], _p(_c(xs, np.float32)), _p(_c(xe, np.float32)),
+                             _p(_c(edges, np.uint32)), _p(_c(faces, np.uint32)),
+                             _p(None if vprop is None else _c(vprop, np.uint32)), _p(_c(offset, np.float32)))

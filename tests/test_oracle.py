@@ -301,3 +301,87 @@ def test_pcg_oracle_edge_cases():
     bad[5] = np.nan
     x, it, rel, conv, fail = oracle.pcg(n, rp, ci, bl, bad)
     assert not conv and fail == 1
+
+
+def test_point_triangle_coeff_closed_forms():
+    t0, t1, t2 = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    assert np.allclose(oracle.pt_coeff([0.25, 0.25, 0.7], t0, t1, t2), [0.5, 0.25, 0.25])  # interior
+    assert np.allclose(oracle.pt_coeff([0.5, -0.3, 0.2], t0, t1, t2), [0.5, 0.5, 0.0])  # beyond edge t0-t1
+    assert np.allclose(oracle.pt_coeff([2.0, -0.5, 0.0], t0, t1, t2), [0.0, 1.0, 0.0])  # beyond vertex t1
+    assert np.allclose(oracle.pt_coeff([-0.2, 0.5, 0.1], t0, t1, t2), [0.5, 0.0, 0.5])  # beyond edge t0-t2
+    # brute force over a barycentric grid wherever at most one planar barycentric is negative
+    # (the classification of distance.hpp:96-160 only looks at the first negative one)
+    rng = np.random.default_rng(0)
+    checked = 0
+    u, v = np.meshgrid(np.linspace(0, 1, 201), np.linspace(0, 1, 201))
+    ok = u + v <= 1
+    while checked < 50:
+        P = rng.normal(size=(4, 3))
+        A = np.stack([P[2] - P[1], P[3] - P[1]], 1)
+        cc = np.linalg.lstsq(A, P[0] - P[1], rcond=None)[0]
+        plane = np.array([1 - cc.sum(), cc[0], cc[1]])
+        if (plane < 0).sum() > 1:
+            continue
+        checked += 1
+        c = oracle.pt_coeff(P[0], P[1], P[2], P[3])
+        d = np.linalg.norm(P[0] - c @ P[1:])
+        pts = (1 - u - v)[ok, None] * P[1] + u[ok, None] * P[2] + v[ok, None] * P[3]
+        assert d <= np.linalg.norm(pts - P[0], axis=1).min() + 1e-9
+    # two negative barycentrics: the reference keeps the edge opposite the first negative one
+    # (here edge t1-t2) although vertex t2 is closer -- reproduced deliberately
+    c = oracle.pt_coeff([-1.0, 3.0, 0.0], t0, t1, t2)
+    assert np.allclose(c, [0.0, 0.0, 1.0]) or c[0] == 0.0
+
+
+def test_edge_edge_coeff_closed_forms():
+    # skew perpendicular segments one unit apart
+    c = oracle.ee_coeff([-1, 0, 0], [1, 0, 0], [0, -1, 1], [0, 1, 1])
+    assert np.allclose(c, [0.5, 0.5, 0.5, 0.5])
+    # parallel segments: falls back to point-edge candidates, distance = separation
+    a0, a1, b0, b1 = map(np.array, ([0, 0, 0], [1, 0, 0], [0.25, 0.5, 0], [2, 0.5, 0]))
+    c = oracle.ee_coeff(a0, a1, b0, b1)
+    xa = c[0] * a0 + c[1] * a1
+    xb = c[2] * b0 + c[3] * b1
+    assert np.isclose(np.linalg.norm(xa - xb), 0.5)
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        P = rng.normal(size=(4, 3))
+        c = oracle.ee_coeff(*P)
+        d = np.linalg.norm((c[0] * P[0] + c[1] * P[1]) - (c[2] * P[2] + c[3] * P[3]))
+        s, t = np.meshgrid(np.linspace(0, 1, 401), np.linspace(0, 1, 401))
+        pa = (1 - s)[..., None] * P[0] + s[..., None] * P[1]
+        pb = (1 - t)[..., None] * P[2] + t[..., None] * P[3]
+        assert d <= np.linalg.norm(pa - pb, axis=-1).min() + 1e-9
+
+
+@pytest.mark.parametrize("d", [1e-6, 3e-5, 2e-4, 7e-4, 9.9e-4])
+def test_barrier_derivatives_central_differences(d):
+    dhat = 1e-3
+    h = 1e-7 * d
+    fd1 = (oracle.barrier(d + h, dhat) - oracle.barrier(d - h, dhat)) / (2 * h)
+    fd2 = (oracle.barrier(d + h, dhat, 1) - oracle.barrier(d - h, dhat, 1)) / (2 * h)
+    assert np.isclose(oracle.barrier(d, dhat, 1), fd1, rtol=1e-6, atol=1e-12)
+    assert np.isclose(oracle.barrier(d, dhat, 2), fd2, rtol=1e-6, atol=1e-9)
+    assert oracle.barrier(dhat, dhat) == 0.0 and oracle.barrier(dhat, dhat, 1) == 0.0
+
+
+def test_accd_point_plane_constant_velocity():
+    # point above the triangle plane moving straight down at speed v
+    tri = np.array([[-1, -1, 0], [2, -1, 0], [-1, 2, 0]], float)
+    for d0, v, off in [(0.1, 0.5, 0.0), (0.1, 0.5, 0.01), (0.05, 1.0, 0.0), (0.3, 0.2, 0.0)]:
+        x0 = np.concatenate([[0.2, 0.2, d0], tri.ravel()])
+        dx = np.concatenate([[0, 0, -v], np.zeros(9)])
+        t = oracle.accd(False, x0, dx, off)
+        t_true = (d0 - off) / v
+        if t_true >= 1.0:
+            assert t == 1.0
+        else:
+            assert 0 < t <= t_true
+            assert d0 - v * t >= off  # never closer than the offset
+            assert t >= 0.9 * t_true  # conservative advancement gets within a few percent
+    # no relative motion -> 1 (accd.hpp:169-175)
+    x0 = np.concatenate([[0.2, 0.2, 0.1], tri.ravel()])
+    assert oracle.accd(False, x0, np.tile([0.3, 0.1, -0.2], 4), 0.0) == 1.0
+    # post-processing of the reference (narrow_phase.cpp:473-522)
+    assert oracle.toi_postprocess(0.001) == 1.0 and oracle.toi_postprocess(1e-3 * 2) == 0.5
+    assert oracle.toi_postprocess(-1) == 1.0 and oracle.toi_postprocess(0.4) == 0.4

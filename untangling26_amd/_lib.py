@@ -48,6 +48,8 @@ SIGNATURES = {
     "ut_pcg_destroy": (C.c_int, [vp]),
     "ut_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "ut_bsr_spmv": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+    "ut_contact_query": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_uint32]),
+    "ut_toi_reset": (C.c_int, [vp, vp]),
 }
 
 
@@ -60,6 +62,13 @@ class PcgParams(C.Structure):
 class PcgReport(C.Structure):
     _fields_ = [("relative_residual", C.c_double), ("infinity_norm", C.c_double), ("iterations", C.c_uint32),
                 ("refinements", C.c_uint32), ("converged", C.c_int), ("failure", C.c_int)]
+
+
+class ContactArgs(C.Structure):
+    _fields_ = [("num_verts", C.c_uint32), ("num_edges", C.c_uint32), ("x", vp), ("x_end", vp), ("edges", vp),
+                ("faces", vp), ("vprop", vp), ("d_hat", vp), ("offset", vp), ("rest_vert_area", vp),
+                ("rest_edge_area", vp), ("rest_face_area", vp), ("kappa", C.c_double), ("energy", C.c_int),
+                ("count", vp), ("pairs", vp), ("pair_ids", vp), ("max_pairs", C.c_uint32), ("toi", vp)]
 
 
 def lib():

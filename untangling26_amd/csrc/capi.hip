@@ -7,6 +7,7 @@
 #include "ef_dcd.hpp"
 #include "contour.hpp"
 #include "pcg.hpp"
+#include "contact.hpp"
 
 #include <new>
 
@@ -508,6 +509,51 @@ UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const doubl
 	if (!pcg)
 		return ut::kInvalid;
 	return guarded(pcg->s.ctx, [&] { pcg->s.spmv64(vals, vals_fp32 != 0, x, y); });
+}
+
+// ---- contacts ----------------------------------------------------------------
+UT_API int ut_contact_query(ut_ctx* ctx, int kind, const ut_contact_args* args, ut_lbvh* tree,
+	const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap)
+{
+	if (!ctx || !args)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			ut::ContactArgs a;
+			a.num_verts = args->num_verts;
+			a.num_edges = args->num_edges;
+			a.x = args->x;
+			a.x_end = args->x_end;
+			a.edges = args->edges;
+			a.faces = args->faces;
+			a.vprop = args->vprop;
+			a.d_hat = args->d_hat;
+			a.offset = args->offset;
+			a.rest_vert_area = args->rest_vert_area;
+			a.rest_edge_area = args->rest_edge_area;
+			a.rest_face_area = args->rest_face_area;
+			a.kappa = args->kappa;
+			a.energy = args->energy;
+			a.count = args->count;
+			a.pairs = reinterpret_cast<ut::ContactRec*>(args->pairs);
+			a.pair_ids = reinterpret_cast<uint2*>(args->pair_ids);
+			a.max_pairs = args->max_pairs;
+			a.toi = args->toi;
+			ut::contact_query(&ctx->c, kind, a, tree ? &tree->b : nullptr, broad, broad_count, broad_cap);
+		});
+}
+
+UT_API int ut_toi_reset(ut_ctx* ctx, float* toi)
+{
+	if (!ctx)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			UT_REQUIRE(toi, "null toi");
+			ut::toi_reset(&ctx->c, toi);
+		});
 }
 
 } // extern "C"

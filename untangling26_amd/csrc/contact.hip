@@ -1,0 +1,265 @@
+// Proximity DCD (VF / EE with the IPC barrier) and additive CCD on gfx950.
+//
+// fn_narrow_phase_{vf,ee}_dcd_query (narrow_phase.cpp:980-1346) and
+// fn_narrow_phase_{vf,ee}_ccd_query (narrow_phase.cpp:412-800).  Distances,
+// barrier derivatives and the time of impact are evaluated in FP64 (the
+// reference runs them in FP32 fast-math); the emitted CollisionPairTemplate
+// keeps the reference's FP32 fields.
+//
+// Each query comes in two forms: over a broad-phase pair list (the
+// reference's two steps) and fused with the LBVH traversal, where the staged
+// candidate batches of lbvh_traverse.hpp feed the narrow test directly.
+#include "ut_common.hpp"
+#include "contact.hpp"
+#include "contact_math.hpp"
+#include "lbvh_traverse.hpp"
+
+namespace ut
+{
+
+constexpr double kMinDist = 5e-6;						 // min_distance_threshold, narrow_phase.cpp:959
+constexpr double kMinDistSqr = kMinDist * kMinDist;		 // narrow_phase.cpp:960
+constexpr uint32_t kTypeMask = 0x80000000u;			 // mask_collision_type, collision_data.h:36
+
+__device__ __forceinline__ bool invalid_combo(const uint32_t* vprop, uint32_t a, uint32_t b)
+{
+	if (!vprop)
+		return false;
+	const uint32_t pa = vprop[a], pb = vprop[b];
+	return ((pa & kPropFixed) && (pb & kPropFixed))
+		|| ((pa & kPropRigid) && (pb & kPropRigid) && (pa >> kPropObjectShift) == (pb >> kPropObjectShift));
+}
+
+// stiffness pair (k1, k2) of narrow_phase.cpp:1091-1112
+__device__ __forceinline__ void contact_stiffness(int energy, double d, double thickness, double d_hat, double area,
+	double kappa, double* k1, double* k2)
+{
+	if (energy == kEnergyQuadratic)
+	{
+		const double C = (d - thickness) - d_hat;
+		const double stiff = area * kappa;
+		*k1 = stiff * C;
+		*k2 = stiff;
+	}
+	else
+	{
+		*k1 = area * kappa * barrier_d1(d - thickness, d_hat);
+		*k2 = area * kappa * barrier_d2(d - thickness, d_hat);
+	}
+}
+
+__device__ __forceinline__ bool vf_dcd_test(uint32_t vid, uint32_t fid, const ContactArgs& a, ContactRec* out)
+{
+	const uint32_t f0 = a.faces[3 * fid], f1 = a.faces[3 * fid + 1], f2 = a.faces[3 * fid + 2];
+	if (vid == f0 || vid == f1 || vid == f2 || invalid_combo(a.vprop, vid, f0))
+		return false;
+	const v3<double> p = load_v3<double>(a.x, vid);
+	const v3<double> t0 = load_v3<double>(a.x, f0), t1 = load_v3<double>(a.x, f1), t2 = load_v3<double>(a.x, f2);
+	const v3<double> b = pt_coeff(p, t0, t1, t2);
+	const v3<double> diff = b.x * (p - t0) + b.y * (p - t1) + b.z * (p - t2);
+	const double	 d2 = dot(diff, diff);
+	const double	 d_hat = ((double)a.d_hat[vid] + (double)a.d_hat[f0]) * 0.5;
+	const double	 th = (double)a.offset[vid] + (double)a.offset[f0];
+	if (!(d2 < (th + d_hat) * (th + d_hat) && (d2 > kMinDistSqr || a.energy == kEnergyQuadratic)))
+		return false;
+	const double d = sqrt(d2);
+	v3<double>	 n = V3m(diff.x / d, diff.y / d, diff.z / d);
+	if (d < kMinDistSqr) // the reference compares d with the squared threshold
+		n = V3m(0.0, 1.0, 0.0);
+	const double area = 0.5 * ((double)a.rest_vert_area[vid] + (double)a.rest_face_area[fid]);
+	double		 k1, k2;
+	contact_stiffness(a.energy, d, th, d_hat, area, a.kappa, &k1, &k2);
+	if (!isfinite(k1) || !isfinite(k2))
+		return false;
+	out->idx[0] = vid;
+	out->idx[1] = f0;
+	out->idx[2] = f1 | kTypeMask;
+	out->idx[3] = f2;
+	out->normal[0] = (float)n.x;
+	out->normal[1] = (float)n.y;
+	out->normal[2] = (float)n.z;
+	out->area = (float)area;
+	out->k1 = (float)k1;
+	out->k2 = (float)k2;
+	out->bary[0] = (float)b.x;
+	out->bary[1] = (float)b.y;
+	out->dv[0] = out->dv[1] = out->dv[2] = 0.0f;
+	out->lambda = 0.0f;
+	return true;
+}
+
+__device__ __forceinline__ bool ee_dcd_test(uint32_t left, uint32_t right, const ContactArgs& a, ContactRec* out)
+{
+	const uint32_t l0 = a.edges[2 * left], l1 = a.edges[2 * left + 1];
+	const uint32_t r0 = a.edges[2 * right], r1 = a.edges[2 * right + 1];
+	if (l0 == r0 || l0 == r1 || l1 == r0 || l1 == r1 || invalid_combo(a.vprop, l0, r0))
+		return false;
+	const v3<double> a0 = load_v3<double>(a.x, l0), a1 = load_v3<double>(a.x, l1);
+	const v3<double> b0 = load_v3<double>(a.x, r0), b1 = load_v3<double>(a.x, r1);
+	const c4<double> c = ee_coeff(a0, a1, b0, b1);
+	const v3<double> x0 = c.a * a0 + c.b * a1;
+	const v3<double> x1 = c.c * b0 + c.d * b1;
+	const v3<double> diff = x0 - x1;
+	const double	 d2 = dot(diff, diff);
+	const double	 d_hat = ((double)a.d_hat[l0] + (double)a.d_hat[r0]) * 0.5;
+	const double	 th = (double)a.offset[l0] + (double)a.offset[r0];
+	if (!(d2 < (th + d_hat) * (th + d_hat) && (d2 > kMinDistSqr || a.energy == kEnergyQuadratic)))
+		return false;
+	const double d = sqrt(d2);
+	v3<double>	 n = V3m(diff.x / d, diff.y / d, diff.z / d);
+	if (d < kMinDistSqr)
+		n = V3m(0.0, 1.0, 0.0);
+	const double area = 0.5 * ((double)a.rest_edge_area[left] + (double)a.rest_edge_area[right]);
+	double		 k1, k2;
+	contact_stiffness(a.energy, d, th, d_hat, area, a.kappa, &k1, &k2);
+	if (!isfinite(k1) || !isfinite(k2))
+		return false;
+	out->idx[0] = l0;
+	out->idx[1] = l1;
+	out->idx[2] = r0;
+	out->idx[3] = r1 | kTypeMask;
+	out->normal[0] = (float)n.x;
+	out->normal[1] = (float)n.y;
+	out->normal[2] = (float)n.z;
+	out->area = (float)area;
+	out->k1 = (float)k1;
+	out->k2 = (float)k2;
+	out->bary[0] = (float)c.a;
+	out->bary[1] = (float)c.c;
+	out->dv[0] = out->dv[1] = out->dv[2] = 0.0f;
+	out->lambda = 0.0f;
+	return true;
+}
+
+__device__ __forceinline__ double vf_ccd(uint32_t vid, uint32_t fid, const ContactArgs& a)
+{
+	const uint32_t f0 = a.faces[3 * fid], f1 = a.faces[3 * fid + 1], f2 = a.faces[3 * fid + 2];
+	if (vid == f0 || vid == f1 || vid == f2 || invalid_combo(a.vprop, vid, f0))
+		return kLineSearchMaxT;
+	v3<double> x0[4] = { load_v3<double>(a.x, vid), load_v3<double>(a.x, f0), load_v3<double>(a.x, f1),
+		load_v3<double>(a.x, f2) };
+	v3<double> dx[4] = { load_v3<double>(a.x_end, vid) - x0[0], load_v3<double>(a.x_end, f0) - x0[1],
+		load_v3<double>(a.x_end, f1) - x0[2], load_v3<double>(a.x_end, f2) - x0[3] };
+	const double th = (double)a.offset[vid] + (double)a.offset[f0];
+	return toi_postprocess(accd_toi(false, x0, dx, th));
+}
+
+__device__ __forceinline__ double ee_ccd(uint32_t left, uint32_t right, const ContactArgs& a)
+{
+	const uint32_t l0 = a.edges[2 * left], l1 = a.edges[2 * left + 1];
+	const uint32_t r0 = a.edges[2 * right], r1 = a.edges[2 * right + 1];
+	if (l0 == r0 || l0 == r1 || l1 == r0 || l1 == r1 || invalid_combo(a.vprop, l0, r0))
+		return kLineSearchMaxT;
+	v3<double> x0[4] = { load_v3<double>(a.x, l0), load_v3<double>(a.x, l1), load_v3<double>(a.x, r0),
+		load_v3<double>(a.x, r1) };
+	v3<double> dx[4] = { load_v3<double>(a.x_end, l0) - x0[0], load_v3<double>(a.x_end, l1) - x0[1],
+		load_v3<double>(a.x_end, r0) - x0[2], load_v3<double>(a.x_end, r1) - x0[3] };
+	const double th = (double)a.offset[l0] + (double)a.offset[r1]; // narrow_phase.cpp:656 uses right_edge[1]
+	return toi_postprocess(accd_toi(true, x0, dx, th));
+}
+
+// Non-negative floats order like their bit patterns.
+__device__ __forceinline__ void toi_min(float* toi, double t)
+{
+	atomicMin(reinterpret_cast<uint32_t*>(toi), __float_as_uint((float)t));
+}
+
+struct NarrowOp
+{
+	const ContactArgs* a;
+	int				   kind; // 0 VF DCD, 1 EE DCD, 2 VF CCD, 3 EE CCD
+	__device__ __forceinline__ void operator()(bool valid, uint32_t q, uint32_t leaf)
+	{
+		if (kind <= 1)
+		{
+			ContactRec rec;
+			const bool hit = valid && (kind == 0 ? vf_dcd_test(q, leaf, *a, &rec) : ee_dcd_test(q, leaf, *a, &rec));
+			const uint32_t s = wave_append(a->count, hit);
+			if (hit && s < a->max_pairs)
+			{
+				a->pairs[s] = rec;
+				a->pair_ids[s] = make_uint2(q, leaf);
+			}
+		}
+		else if (valid)
+		{
+			const double t = kind == 2 ? vf_ccd(q, leaf, *a) : ee_ccd(q, leaf, *a);
+			if (t != kLineSearchMaxT)
+				toi_min(a->toi, t);
+		}
+	}
+};
+
+__global__ __launch_bounds__(kTravThreads) void k_contact_fused(int kind, ContactArgs a, const BvhNode* __restrict__ nodes,
+	uint32_t* __restrict__ status, uint32_t* __restrict__ spill, uint32_t max_visits)
+{
+	__shared__ TravShared sh;
+	const bool			  vf = (kind == 0 || kind == 2);
+	const uint32_t		  nq = vf ? a.num_verts : a.num_edges;
+	const uint32_t		  q = blockIdx.x * kTravThreads + threadIdx.x;
+	const bool			  active = q < nq;
+	const float*		  xe = kind >= 2 ? a.x_end : a.x;
+	const box3			  qb = active ? query_box(vf ? kQueryVerts : kQueryEdges, q, a.x, xe, a.edges, a.d_hat, a.offset)
+										: box_empty();
+	NarrowOp			  op{ &a, kind };
+	traverse_staged(sh, active, q, qb, !vf, nodes, status, spill, max_visits, op);
+}
+
+__global__ __launch_bounds__(256) void k_contact_list(int kind, ContactArgs a, const uint2* __restrict__ broad,
+	const uint32_t* __restrict__ broad_count, uint32_t broad_cap)
+{
+	const uint32_t nb = min(*broad_count, broad_cap);
+	NarrowOp	   op{ &a, kind };
+	for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x)
+	{
+		const uint32_t i = base + threadIdx.x;
+		const bool	   valid = i < nb;
+		const uint2	   p = valid ? broad[i] : make_uint2(0u, 0u);
+		op(valid, p.x, p.y);
+	}
+}
+
+__global__ void k_toi_init(float* toi) { *toi = (float)kLineSearchMaxT; }
+
+void contact_query(Context* ctx, int kind, const ContactArgs& a, Lbvh* tree, const uint32_t* broad,
+	const uint32_t* broad_count, uint32_t broad_cap)
+{
+	UT_REQUIRE(kind >= 0 && kind <= 3, "contact query: unknown kind");
+	UT_REQUIRE(a.x && a.d_hat && a.offset, "contact query: null positions or per-vertex arrays");
+	if (kind == 0 || kind == 2)
+		UT_REQUIRE(a.faces, "VF query needs faces");
+	else
+		UT_REQUIRE(a.edges, "EE query needs edges");
+	if (kind <= 1)
+		UT_REQUIRE(a.count && (a.max_pairs == 0 || (a.pairs && a.pair_ids)) && a.rest_vert_area && a.rest_face_area
+				&& a.rest_edge_area,
+			"DCD query: null output or rest areas");
+	else
+		UT_REQUIRE(a.x_end && a.toi, "CCD query needs x_end and toi");
+	if (tree)
+	{
+		const bool vf = (kind == 0 || kind == 2);
+		UT_REQUIRE(tree->type == (vf ? kTreeFace : kTreeEdge), "fused query: wrong tree type");
+		const uint32_t nq = vf ? a.num_verts : a.num_edges;
+		if (nq == 0)
+			return;
+		tree->spill.reserve((size_t)nq * (kStackTotal - kStackLds));
+		k_contact_fused<<<div_up(nq, kTravThreads), kTravThreads, 0, ctx->stream>>>(
+			kind, a, tree->nodes.get(), tree->status.get(), tree->spill.get(), tree->num_leaves + 1);
+	}
+	else
+	{
+		UT_REQUIRE(broad && broad_count, "list query needs the broad-phase list");
+		k_contact_list<<<256 * 8, 256, 0, ctx->stream>>>(kind, a, reinterpret_cast<const uint2*>(broad), broad_count,
+			broad_cap);
+	}
+	UT_LAUNCH_CHECK();
+}
+
+void toi_reset(Context* ctx, float* toi)
+{
+	k_toi_init<<<1, 1, 0, ctx->stream>>>(toi);
+	UT_LAUNCH_CHECK();
+}
+
+} // namespace ut

@@ -1,0 +1,57 @@
+#pragma once
+#include "ut_runtime.hpp"
+#include "ef_dcd.hpp"
+#include "lbvh.hpp"
+
+namespace ut
+{
+
+enum ContactEnergy : int
+{
+	kEnergyQuadratic = 0, // contact_energy_type 0
+	kEnergyBarrier = 1,	  // contact_energy_type 1 (log barrier, the default)
+};
+
+// CollisionPair::CollisionPairTemplate (collision_data.h:48-150): 64 B.
+struct ContactRec
+{
+	uint32_t idx[4];	// VF: v, f0, f1 | type bit, f2;  EE: a0, a1, b0, b1 | type bit
+	float	 normal[3]; // vec1.xyz
+	float	 area;		// vec1.w
+	float	 k1, k2;	// vec2.xy
+	float	 bary[2];	// vec2.zw: VF face bary x, y;  EE edge-a bary x, edge-b bary x
+	float	 dv[3];		// vec3.xyz (friction)
+	float	 lambda;	// vec3.w
+};
+static_assert(sizeof(ContactRec) == 64, "CollisionPairTemplate layout");
+
+struct ContactArgs
+{
+	uint32_t		num_verts = 0, num_edges = 0;
+	const float*	x = nullptr;	 // DCD positions / CCD start
+	const float*	x_end = nullptr; // CCD end positions
+	const uint32_t* edges = nullptr;
+	const uint32_t* faces = nullptr;
+	const uint32_t* vprop = nullptr;
+	const float*	d_hat = nullptr;  // per vertex
+	const float*	offset = nullptr; // per vertex contact offset (thickness)
+	const float*	rest_vert_area = nullptr;
+	const float*	rest_edge_area = nullptr;
+	const float*	rest_face_area = nullptr;
+	double			kappa = 1e8;
+	int				energy = kEnergyBarrier;
+	uint32_t*		count = nullptr;
+	ContactRec*		pairs = nullptr;
+	uint2*			pair_ids = nullptr;
+	uint32_t		max_pairs = 0;
+	float*			toi = nullptr; // CCD: global minimum time of impact (non-negative float)
+};
+
+// kind: 0 VF DCD, 1 EE DCD, 2 VF CCD, 3 EE CCD.  With `tree` the query is fused
+// with the traversal (face tree for VF, edge tree for EE, refit beforehand);
+// otherwise it runs over `broad` pairs.
+void contact_query(Context* ctx, int kind, const ContactArgs& a, Lbvh* tree, const uint32_t* broad,
+	const uint32_t* broad_count, uint32_t broad_cap);
+void toi_reset(Context* ctx, float* toi);
+
+} // namespace ut

@@ -40,3 +40,23 @@ def mesh_topology(edges: np.ndarray, faces: np.ndarray):
     eaf[has1, 0] = f_sorted[first[has1]]
     eaf[has2, 1] = f_sorted[first[has2] + 1]
     return fae, eaf
+
+
+def rest_areas(x_rest: np.ndarray, edges: np.ndarray, faces: np.ndarray, edge_adj_faces: np.ndarray,
+               min_area: float = 1e-10):
+    """Face, vertex and edge rest areas (init_mesh_data.cpp:621-700): face area
+    = |cross|/2, vertex and edge areas = sum of adjacent face areas / 3, all
+    clamped from below like safe_area."""
+    x = np.asarray(x_rest, np.float64)
+    f = np.asarray(faces, np.int64)
+    fa = 0.5 * np.linalg.norm(np.cross(x[f[:, 1]] - x[f[:, 0]], x[f[:, 2]] - x[f[:, 0]]), axis=1)
+    fa = np.maximum(fa, min_area)
+    va = np.zeros(x.shape[0])
+    np.add.at(va, f.ravel(), np.repeat(fa / 3.0, 3))
+    eaf = np.asarray(edge_adj_faces, np.int64)
+    ea = np.zeros(eaf.shape[0])
+    for k in range(2):
+        m = eaf[:, k] != 0xFFFFFFFF
+        ea[m] += fa[eaf[m, k]] / 3.0
+    return (np.maximum(va, min_area).astype(np.float32), np.maximum(ea, min_area).astype(np.float32),
+            fa.astype(np.float32))
