@@ -35,6 +35,7 @@ typedef struct ut_ctx      ut_ctx;
 typedef struct ut_lbvh     ut_lbvh;
 typedef struct ut_contours ut_contours;
 typedef struct ut_pcg      ut_pcg;
+typedef struct ut_solver   ut_solver;
 
 /* ---- context ---------------------------------------------------------- */
 
@@ -230,6 +231,105 @@ UT_API int ut_contact_query(ut_ctx* ctx, int kind, const ut_contact_args* args, 
 	const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap);
 /* Sets *toi = 1 (line_search_max_t) on the context stream (fn_reset_toi). */
 UT_API int ut_toi_reset(ut_ctx* ctx, float* toi);
+
+/* ---- assembly of the incremental potential ------------------------------------------
+ * NewtonSolver's material assembly (newton_solver.cpp:2641-2714 with
+ * Energies/{soft_inertia_energy,stretch_face_energy,bending_energy_kernel}.cpp)
+ * and NarrowPhasesDetector::device_perPair_evaluate_gradient_hessian
+ * (narrow_phase.cpp:2619-2746).  Writes grad = dE/dx (double[3V]) and the
+ * block values (double[9 * nslots], planar) of the material Hessian plus the
+ * contacts' diagonal blocks; vals32 (optional) receives an FP32 copy. */
+typedef struct ut_assemble_args {
+	uint32_t		nv, nf, nh, nslots;
+	const float*	x;
+	const double*	x_tilde;	 /* inertia: m / h^2 (x - x_tilde); NULL skips */
+	const float*	mass;
+	double			inv_h2;
+	const uint32_t* faces;
+	const float*	dm;
+	const float*	face_area;
+	const float*	mu_lambda;
+	const uint32_t* face_slots;
+	const uint32_t* hinges;
+	const float*	rest_angle;
+	const float*	bend_stiff_area;
+	const uint32_t* hinge_slots;
+	const void*		contacts;	 /* CollisionPairTemplate list or NULL */
+	const uint32_t* contact_count;
+	uint32_t		contact_cap;
+	const uint32_t* diag_slot;
+	double*			grad;
+	double*			vals;
+	float*			vals32;
+	double*			energy;		 /* optional double[4]: inertia, stretch, bending */
+} ut_assemble_args;
+UT_API int ut_assemble(ut_ctx* ctx, const ut_assemble_args* args);
+
+/* ---- Newton step: lcs::NewtonSolver (newton_solver.h:283-326) ------------------ */
+
+/* Static mesh data prepared by the initializer (Solver/Initializer/*); all
+ * pointers are device arrays kept by reference for the solver's lifetime. */
+typedef struct ut_solver_desc {
+	uint32_t		nv, ne, nf, nh;
+	const uint32_t* edges;			/* [2E] */
+	const uint32_t* faces;			/* [3F] */
+	const uint32_t* face_adj_edges; /* [3F], see ut_contours_create */
+	const uint32_t* edge_adj_faces; /* [2E] */
+	const float*	mass;			/* [V] */
+	const float*	d_hat;			/* [V] */
+	const float*	offset;			/* [V] contact offset */
+	const uint32_t* vprop;			/* [V] optional property words */
+	const float*	rest_vert_area; /* [V] */
+	const float*	rest_edge_area; /* [E] */
+	const float*	rest_face_area; /* [F] */
+	const float*	dm;				/* [4F] inverse rest 2x2 (d0, d1, d2, d3), Dm = [[d0, d2], [d1, d3]] */
+	const float*	mu_lambda;		/* [2F] BW98 stretch and shear stiffness */
+	const uint32_t* hinges;			/* [4H] (e0, e1, o0, o1) */
+	const float*	rest_angle;		/* [H] */
+	const float*	bend_stiff_area; /* [H] */
+	uint32_t		nslices, nslots; /* SELL-64 block pattern, see ut_pcg_create */
+	const uint32_t* slice_off;
+	const uint32_t* col;
+	const uint32_t* diag_slot;
+	const uint32_t* face_slots;	 /* [9F] slot of block (a, b) of face vertices a, b */
+	const uint32_t* hinge_slots; /* [16H] */
+} ut_solver_desc;
+
+/* SceneParams fields the step uses (scene_params.h:24-157). */
+typedef struct ut_solver_params {
+	double		  dt;
+	double		  gravity[3];
+	double		  kappa;
+	int			  energy; /* UT_ENERGY_* */
+	double		  damping;
+	int			  nonlinear_iters;
+	int			  use_self_collision;
+	int			  use_ccd;
+	int			  use_untangling; /* EF detection + contours each iteration */
+	ut_pcg_params pcg;
+} ut_solver_params;
+
+typedef struct ut_step_report {
+	uint32_t vf_pairs, ee_pairs, ef_pairs, ef_total, contours;
+	uint32_t pcg_iterations, pcg_refinements;
+	double	 pcg_relative_residual;
+	int		 pcg_converged;
+	float	 toi;
+	double	 energy[3]; /* inertia, stretch, bending at the last assembly */
+	float	 ms_detect, ms_contours, ms_assemble, ms_pcg, ms_ccd, ms_total;
+} ut_step_report;
+
+UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,
+	const float* v0, ut_solver** out);
+UT_API int ut_solver_destroy(ut_solver* s);
+/* NewtonSolver::physics_step_GPU (newton_solver.cpp:2482-2767).  Synchronises. */
+UT_API int ut_solver_step(ut_solver* s, ut_step_report* report);
+UT_API int ut_solver_set_params(ut_solver* s, const ut_solver_params* params);
+/* Device-to-device copies of the state: float[3V]. */
+UT_API int ut_solver_get_state(ut_solver* s, float* x, float* v);
+UT_API int ut_solver_set_state(ut_solver* s, const float* x, const float* v);
+/* UT_CT_* output of the contours built in the last step (see ut_contours_get). */
+UT_API int ut_solver_contour_get(ut_solver* s, int what, void* dst);
 
 #ifdef __cplusplus
 }

@@ -115,6 +115,21 @@ size_t or_dcd_list(int kind, const uint32_t* broad, size_t nb, const float* x, c
 double or_ccd_list(int kind, const uint32_t* broad, size_t nb, const float* xs, const float* xe,
 	const uint32_t* edges, const uint32_t* faces, const uint32_t* vprop, const float* off);
 
+/* BW98 face term (stretch_energy.h:200-520): x[9] (3 vertices), dm[4]; grad[9], hess[81]. */
+double or_bw98_face(const double* x, const float* dm, double area, double mu, double lam, double* grad, double* hess);
+/* dihedral bending hinge (bending_energy.h:164-260): x[12] = (e0, e1, o0, o1); grad[12], hess[144]. */
+double or_bending_hinge(const double* x, double theta0, double k, double* grad, double* hess, double* theta_out);
+
+/* Full incremental-potential assembly into the SELL slot numbering (planar vals). */
+void or_assemble(uint32_t nv, uint32_t nf, uint32_t nh, const float* x, const double* x_tilde, const float* mass,
+	double inv_h2, const uint32_t* faces, const float* dm, const float* area, const float* mu_lambda,
+	const uint32_t* face_slots, const uint32_t* hinges, const float* rest_angle, const float* bend,
+	const uint32_t* hinge_slots, const uint32_t* contacts, uint32_t ncontacts, const uint32_t* diag_slot,
+	uint32_t nslots, double* grad, double* vals, double* energy);
+/* or_pcg with matrix-free contact off-diagonal blocks (16-word records). */
+uint32_t or_pcg_contacts(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const uint32_t* contacts,
+	uint32_t ncontacts, const double* b, double* x, int precond, uint32_t max_iters, double rel_tol, double* out);
+
 #ifdef __cplusplus
 }
 #endif

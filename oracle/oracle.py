@@ -65,6 +65,14 @@ def lib():
         L.or_dcd_list.restype = C.c_size_t
         L.or_ccd_list.argtypes = [C.c_int, p, C.c_size_t, p, p, p, p, p, p]
         L.or_ccd_list.restype = C.c_double
+        L.or_bw98_face.argtypes = [p, p, C.c_double, C.c_double, C.c_double, p, p]
+        L.or_bw98_face.restype = C.c_double
+        L.or_bending_hinge.argtypes = [p, C.c_double, C.c_double, p, p, p]
+        L.or_bending_hinge.restype = C.c_double
+        L.or_assemble.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, p, p, p, C.c_double, p, p, p, p, p, p, p, p,
+                                  p, p, C.c_uint32, p, C.c_uint32, p, p, p]
+        L.or_pcg_contacts.argtypes = [C.c_uint32, p, p, p, p, C.c_uint32, p, p, C.c_int, C.c_uint32, C.c_double, p]
+        L.or_pcg_contacts.restype = C.c_uint32
         L.or_line_intersection.argtypes = [p, p, p, p, p, p, p]
         L.or_line_intersection.restype = C.c_int
         L.or_ef_dcd.argtypes = [p, C.c_size_t, p, p, p, p, p, p, p, C.c_size_t]
@@ -312,3 +320,55 @@ def ccd_list(kind, broad, xs, xe, edges, faces, offset, vprop=None):
     return lib().or_ccd_list(kind, _p(broad), broad.shape[0], _p(_c(xs, np.float32)), _p(_c(xe, np.float32)),
                              _p(_c(edges, np.uint32)), _p(_c(faces, np.uint32)),
                              _p(None if vprop is None else _c(vprop, np.uint32)), _p(_c(offset, np.float32)))
+
+
+def bw98_face(x3, dm, area, mu, lam):
+    """(energy, grad[3,3], hess[3,3,3,3] as block (a,b) entry (i,j)) of one BW98 face."""
+    x = _c(np.asarray(x3, np.float64).ravel(), np.float64)
+    g = np.zeros(9)
+    h = np.zeros(81)
+    e = lib().or_bw98_face(_p(x), _p(_c(dm, np.float32)), float(area), float(mu), float(lam), _p(g), _p(h))
+    return e, g.reshape(3, 3), h.reshape(3, 3, 3, 3)
+
+
+def bending_hinge(x4, theta0, k):
+    x = _c(np.asarray(x4, np.float64).ravel(), np.float64)
+    g = np.zeros(12)
+    h = np.zeros(144)
+    th = np.zeros(1)
+    e = lib().or_bending_hinge(_p(x), float(theta0), float(k), _p(g), _p(h), _p(th))
+    return e, g.reshape(4, 3), h.reshape(4, 4, 3, 3), float(th[0])
+
+
+def assemble(st, x, x_tilde, inv_h2, contacts=None):
+    """(grad[3V], planar vals[9, nslots], energies[3]) of the setup st (initializer.ClothSetup)."""
+    pat = st.pattern
+    grad = np.zeros(3 * st.nv)
+    vals = np.zeros(9 * pat.nslots)
+    en = np.zeros(3)
+    c = None if contacts is None or len(contacts) == 0 else _c(contacts, np.uint32)
+    lib().or_assemble(st.nv, len(st.faces), len(st.hinges), _p(_c(x, np.float32)), _p(_c(x_tilde, np.float64)),
+                      _p(_c(st.mass, np.float32)), float(inv_h2), _p(_c(st.faces, np.uint32)),
+                      _p(_c(st.dm, np.float32)), _p(_c(st.rest_face_area, np.float32)),
+                      _p(_c(st.mu_lambda, np.float32)), _p(_c(st.face_slots, np.uint32)),
+                      _p(_c(st.hinges, np.uint32)), _p(_c(st.rest_angle, np.float32)),
+                      _p(_c(st.bend_stiff_area, np.float32)), _p(_c(st.hinge_slots, np.uint32)), _p(c),
+                      0 if c is None else len(c), _p(_c(pat.diag_slot, np.uint32)), pat.nslots, _p(grad), _p(vals),
+                      _p(en))
+    return grad, vals.reshape(9, -1), en
+
+
+def pcg_sell(pattern, vals, b, contacts=None, precond=0, max_iters=100, rel_tol=1e-6):
+    """FP64 PCG on planar SELL values (+ matrix-free contact off-diagonals)."""
+    n = pattern.n
+    rows = pattern.keys // n
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum(np.bincount(rows, minlength=n))
+    ci = (pattern.keys % n).astype(np.uint32)
+    bl = np.ascontiguousarray(np.asarray(vals)[:, pattern.key_slots].T)
+    x = np.zeros(3 * n)
+    out = np.zeros(3)
+    c = None if contacts is None or len(contacts) == 0 else _c(contacts, np.uint32)
+    it = lib().or_pcg_contacts(n, _p(rp), _p(ci), _p(bl), _p(c), 0 if c is None else len(c),
+                               _p(_c(b, np.float64)), _p(x), precond, max_iters, rel_tol, _p(out))
+    return x, int(it), float(out[0]), bool(out[1]), int(out[2])

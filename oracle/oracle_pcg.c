@@ -14,6 +14,39 @@
 #include <stdlib.h>
 #include <string.h>
 
+static const uint32_t* g_contacts = NULL;
+static uint32_t		   g_ncontacts = 0;
+
+/* y += off-diagonal contact blocks w_i w_j k2 n n^T (i != j) times x */
+static void contact_mv(const double* x, double* y)
+{
+	for (uint32_t c = 0; c < g_ncontacts; ++c)
+	{
+		const uint32_t* r = g_contacts + 16 * c;
+		float			f[12];
+		memcpy(f, r + 4, 48);
+		const double n[3] = { f[0], f[1], f[2] }, k2 = f[5], b0 = f[6], b1 = f[7];
+		double		 w[4], s[4], S = 0;
+		uint32_t	 v[4];
+		if (r[2] & 0x80000000u)
+			w[0] = 1.0, w[1] = -b0, w[2] = -b1, w[3] = b0 + b1 - 1.0;
+		else
+			w[0] = b0, w[1] = 1.0 - b0, w[2] = -b1, w[3] = b1 - 1.0;
+		for (int a = 0; a < 4; ++a)
+		{
+			v[a] = r[a] & 0x7FFFFFFFu;
+			s[a] = n[0] * x[3 * v[a]] + n[1] * x[3 * v[a] + 1] + n[2] * x[3 * v[a] + 2];
+			S += w[a] * s[a];
+		}
+		for (int a = 0; a < 4; ++a)
+		{
+			const double cf = k2 * w[a] * (S - w[a] * s[a]);
+			for (int k = 0; k < 3; ++k)
+				y[3 * v[a] + k] += cf * n[k];
+		}
+	}
+}
+
 static void bsr_mv(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const double* x, double* y)
 {
 	for (uint32_t i = 0; i < n; ++i)
@@ -28,6 +61,7 @@ static void bsr_mv(uint32_t n, const uint32_t* rp, const uint32_t* ci, const dou
 		}
 		y[3 * i] = a[0], y[3 * i + 1] = a[1], y[3 * i + 2] = a[2];
 	}
+	contact_mv(x, y);
 }
 
 static double dotn(const double* a, const double* b, size_t m)
@@ -156,5 +190,16 @@ done:
 	free(z);
 	free(r);
 	free(Minv);
+	return it;
+}
+
+uint32_t or_pcg_contacts(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const uint32_t* contacts,
+	uint32_t ncontacts, const double* b, double* x, int precond, uint32_t max_iters, double rel_tol, double* out)
+{
+	g_contacts = contacts;
+	g_ncontacts = ncontacts;
+	uint32_t it = or_pcg(n, rp, ci, bl, b, x, precond, max_iters, rel_tol, out);
+	g_contacts = NULL;
+	g_ncontacts = 0;
 	return it;
 }

@@ -385,3 +385,113 @@ def test_accd_point_plane_constant_velocity():
     # post-processing of the reference (narrow_phase.cpp:473-522)
     assert oracle.toi_postprocess(0.001) == 1.0 and oracle.toi_postprocess(1e-3 * 2) == 0.5
     assert oracle.toi_postprocess(-1) == 1.0 and oracle.toi_postprocess(0.4) == 0.4
+
+
+def _fd_grad(f, x, h):
+    g = np.zeros_like(x)
+    for i in range(x.size):
+        xp, xm = x.copy(), x.copy()
+        xp.flat[i] += h
+        xm.flat[i] -= h
+        g.flat[i] = (f(xp) - f(xm)) / (2 * h)
+    return g
+
+
+def test_bw98_gradient_and_hessian():
+    from untangling26_amd import initializer
+
+    rng = np.random.default_rng(0)
+    rest = np.array([[0, 0, 0], [0.01, 0, 0], [0.003, 0.009, 0]], np.float64)
+    dm = initializer.dm_inv(rest, np.array([[0, 1, 2]]))[0]
+    area = 0.5 * np.linalg.norm(np.cross(rest[1] - rest[0], rest[2] - rest[0]))
+    mu, lam = 370.0, 864.0
+    e0, g0, _ = oracle.bw98_face(rest, dm, area, mu, lam)
+    assert abs(e0) < 1e-12 * mu * area and np.abs(g0).max() < 1e-6 * mu * 0.01  # rest state
+    for trial in range(20):
+        x = rest + rng.normal(0, 2e-3, rest.shape)
+        e, g, H = oracle.bw98_face(x, dm, area, mu, lam)
+        fd = _fd_grad(lambda y: oracle.bw98_face(y, dm, area, mu, lam)[0], x, 1e-7 * 0.01)
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-6 * np.abs(g).max())
+        Hm = H.transpose(0, 2, 1, 3).reshape(9, 9)
+        assert np.allclose(Hm, Hm.T, atol=1e-9 * np.abs(Hm).max())
+        assert np.linalg.eigvalsh(Hm).min() >= -1e-8 * np.abs(Hm).max()
+    # pure stretch along the fibres (no shear, |Fu|, |Fv| > 1): the stretch projection is inactive and
+    # the shear term vanishes to second order, so the Hessian equals the true one
+    x = rest * np.array([1.2, 1.1, 1.0])
+    _, _, H = oracle.bw98_face(x, dm, area, mu, 0.0)
+    Hm = H.transpose(0, 2, 1, 3).reshape(9, 9)
+    fdH = np.stack([_fd_grad(lambda y: oracle.bw98_face(y, dm, area, mu, 0.0)[1].ravel()[i], x, 1e-9).ravel()
+                    for i in range(9)])
+    assert np.allclose(Hm, fdH, rtol=1e-4, atol=1e-4 * np.abs(Hm).max())
+
+
+def test_bending_gradient_and_rest_angle():
+    from untangling26_amd import initializer
+
+    rng = np.random.default_rng(1)
+    # hinge: edge along x, two wings
+    e0, e1 = np.array([0.0, 0, 0]), np.array([0.01, 0, 0])
+    o0, o1 = np.array([0.004, 0.008, 0.0]), np.array([0.005, -0.008, 0.002])
+    x = np.stack([e0, e1, o0, o1])
+    th_np = initializer.dihedral_angles(x, np.array([[0, 1, 2, 3]]))[0]
+    _, _, _, th = oracle.bending_hinge(x, 0.0, 1.0)
+    assert np.isclose(th, th_np, atol=1e-6)
+    e, g, H, _ = oracle.bending_hinge(x, th, 1.0)
+    assert e == 0.0 and not g.any()
+    for _ in range(20):
+        y = x + rng.normal(0, 1e-3, x.shape)
+        e, g, H, th = oracle.bending_hinge(y, 0.1, 2.0)
+        fd = _fd_grad(lambda z: oracle.bending_hinge(z, 0.1, 2.0)[0], y, 1e-9)
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-6 * np.abs(g).max())
+        Hm = H.transpose(0, 2, 1, 3).reshape(12, 12)
+        assert np.linalg.eigvalsh(Hm).min() >= -1e-9 * np.abs(Hm).max()
+
+
+def test_assemble_equals_term_sum():
+    """The C assembly into SELL slots equals the element-by-element sum of the
+    FD-pinned per-element functions (pins the slot scatter and contact weights)."""
+    from assembly_cases import contact_state, term_sum
+
+    m, st, x_tilde, inv_h2, recs = contact_state(n=12)
+    assert len(recs) > 0
+    g, vals, en = oracle.assemble(st, m.x, x_tilde, inv_h2, recs)
+    g_want, b_want = term_sum(st, m.x, x_tilde, inv_h2, recs)
+    assert np.allclose(g.reshape(-1, 3), g_want, rtol=0, atol=1e-12 * np.abs(g_want).max())
+    r, c, B = st.pattern.blocks_of(vals)
+    want = np.stack([b_want.get((int(i), int(j)), np.zeros((3, 3))) for i, j in zip(r, c)])
+    assert np.allclose(B, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    assert set(b_want) <= set(zip(r.tolist(), c.tolist()))
+    # padding slots stay zero
+    mask = np.ones(st.pattern.nslots, bool)
+    mask[st.pattern.key_slots] = False
+    assert not vals[:, mask].any()
+    # inertia energy closed form
+    d = m.x.astype(np.float64).ravel() - x_tilde
+    assert np.isclose(en[0], 0.5 * inv_h2 * (np.repeat(st.mass.astype(np.float64), 3) * d * d).sum(), rtol=1e-12)
+
+
+def test_pcg_with_contact_offdiagonals_matches_dense_solve():
+    """The matrix-free contact off-diagonal blocks w_a w_b k2 n n^T (a != b),
+    added to the assembled matrix, give the dense system a direct solve answers."""
+    from assembly_cases import contact_state, contact_weights
+
+    m, st, x_tilde, inv_h2, recs = contact_state(n=12)
+    g, vals, _ = oracle.assemble(st, m.x, x_tilde, inv_h2, recs)
+    n = st.nv
+    A = np.zeros((3 * n, 3 * n))
+    r, c, B = st.pattern.blocks_of(vals)
+    for i, j, blk in zip(r, c, B):
+        A[3 * i:3 * i + 3, 3 * j:3 * j + 3] += blk
+    for rec in recs:
+        idx, w, nn, _, k2 = contact_weights(rec)
+        for a in range(4):
+            for b in range(4):
+                if a != b:
+                    A[3 * idx[a]:3 * idx[a] + 3, 3 * idx[b]:3 * idx[b] + 3] += w[a] * w[b] * k2 * np.outer(nn, nn)
+    want = np.linalg.solve(A, -g)
+    x, it, rel, conv, fail = oracle.pcg_sell(st.pattern, vals, -g, recs, max_iters=20000, rel_tol=1e-12)
+    assert conv and fail == 0
+    assert np.linalg.norm(x - want) <= 1e-8 * np.linalg.norm(want)
+    # without the off-diagonal contact terms the answer is measurably different
+    x2, *_ = oracle.pcg_sell(st.pattern, vals, -g, None, max_iters=20000, rel_tol=1e-12)
+    assert np.linalg.norm(x2 - want) > 1e-6 * np.linalg.norm(want)

@@ -1,0 +1,70 @@
+"""End-to-end Newton steps through the lcs_py mirror (NewtonSolver ->
+ut_solver_step): the step runs on the GPU, stays finite, never crosses a
+contact (CCD line search), reports the same EF pairs and contours as the
+oracle on the state it started from, and the FP64 and mixed-precision PCG
+paths take the same step within the solver tolerance."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from untangling26_amd import scenes, topology
+from untangling26_amd.lcs_py import PCG_FP64, PCG_MIXED, NewtonSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(name="two_sheets", precision=PCG_MIXED, n=None):
+    if name == "two_sheets":
+        m = scenes.two_sheets(0)
+        rest = m.x
+    else:
+        m = scenes.folded_cloth(0, n=n or 64)
+        rest = scenes.folded_cloth_rest(n=n or 64)
+    s = NewtonSolver()
+    s.init_device("hip")
+    s.register_world_data(s.create_world_data_from_array(name, m.x, m.faces, rest))
+    cfg = s.get_config()
+    cfg.pcg_precision = precision
+    cfg.pcg_iter_count = 2000
+    s.init_solver()
+    return s, m
+
+
+@pytest.mark.parametrize("name", ["two_sheets", "folded"])
+def test_step_runs_and_matches_oracle_contours(ctx, name):
+    s, m = _solver(name)
+    x0 = s.positions().cpu().numpy()
+    rep = s.physics_step_gpu()
+    assert rep["pcg_converged"] and rep["pcg_relative_residual"] <= 1e-6
+    assert 0.0 < rep["toi"] <= 1.0
+    x1 = s.positions().cpu().numpy()
+    assert np.isfinite(x1).all() and not np.array_equal(x0, x1)
+    # EF pairs and contours of the step's start state, against the oracle
+    d_hat, th = scenes.per_vertex(m)
+    pairs, ids = oracle.ef_detect(x0, m.edges, m.faces, d_hat, th)
+    assert rep["ef_pairs"] == len(ids)
+    fae, eaf = topology.mesh_topology(m.edges, m.faces)
+    oid, opr, oadj, oadjn = oracle.ef_preprocess(ids, pairs, m.edges, m.faces, fae, eaf)
+    nc, ci, mi = oracle.contours_dfs(opr, oadj, oadjn)
+    data = s.get_intersection_contour_data()
+    assert data["num_pairs"] == len(oid) and data["num_contours"] == nc
+    assert np.array_equal(data["ef_indices"], oid)
+    assert np.array_equal(data["contour_index"], ci)
+    verts, tris = s.get_sim_result()
+    assert verts[0].shape == (m.nv, 3) and np.array_equal(tris[0], m.faces)
+
+
+def test_fp64_and_mixed_take_the_same_step(ctx):
+    s64, m = _solver("folded", PCG_FP64)
+    smx, _ = _solver("folded", PCG_MIXED)
+    r64 = s64.physics_step_gpu()
+    rmx = smx.physics_step_gpu()
+    assert r64["pcg_converged"] and rmx["pcg_converged"]
+    x64 = s64.positions().cpu().numpy().astype(np.float64)
+    xmx = smx.positions().cpu().numpy().astype(np.float64)
+    x0 = m.x.astype(np.float64)
+    d64 = x64 - x0
+    # both solve to relative residual 1e-6; the steps differ by the solver tolerance times the
+    # conditioning, plus FP32 rounding of the positions (measured bound)
+    assert np.linalg.norm(xmx - x64) <= 1e-3 * np.linalg.norm(d64) + 1e-6 * np.sqrt(m.nv)

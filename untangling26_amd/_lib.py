@@ -50,6 +50,14 @@ SIGNATURES = {
     "ut_bsr_spmv": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "ut_contact_query": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_uint32]),
     "ut_toi_reset": (C.c_int, [vp, vp]),
+    "ut_assemble": (C.c_int, [vp, vp]),
+    "ut_solver_create": (C.c_int, [vp, vp, vp, vp, vp, C.POINTER(vp)]),
+    "ut_solver_destroy": (C.c_int, [vp]),
+    "ut_solver_step": (C.c_int, [vp, vp]),
+    "ut_solver_set_params": (C.c_int, [vp, vp]),
+    "ut_solver_get_state": (C.c_int, [vp, vp, vp]),
+    "ut_solver_set_state": (C.c_int, [vp, vp, vp]),
+    "ut_solver_contour_get": (C.c_int, [vp, C.c_int, vp]),
 }
 
 
@@ -69,6 +77,37 @@ class ContactArgs(C.Structure):
                 ("faces", vp), ("vprop", vp), ("d_hat", vp), ("offset", vp), ("rest_vert_area", vp),
                 ("rest_edge_area", vp), ("rest_face_area", vp), ("kappa", C.c_double), ("energy", C.c_int),
                 ("count", vp), ("pairs", vp), ("pair_ids", vp), ("max_pairs", C.c_uint32), ("toi", vp)]
+
+
+class AssembleArgs(C.Structure):
+    _fields_ = [("nv", C.c_uint32), ("nf", C.c_uint32), ("nh", C.c_uint32), ("nslots", C.c_uint32), ("x", vp),
+                ("x_tilde", vp), ("mass", vp), ("inv_h2", C.c_double), ("faces", vp), ("dm", vp), ("face_area", vp),
+                ("mu_lambda", vp), ("face_slots", vp), ("hinges", vp), ("rest_angle", vp), ("bend_stiff_area", vp),
+                ("hinge_slots", vp), ("contacts", vp), ("contact_count", vp), ("contact_cap", C.c_uint32),
+                ("diag_slot", vp), ("grad", vp), ("vals", vp), ("vals32", vp), ("energy", vp)]
+
+
+class SolverDesc(C.Structure):
+    _fields_ = [("nv", C.c_uint32), ("ne", C.c_uint32), ("nf", C.c_uint32), ("nh", C.c_uint32)] + [
+        (n, vp) for n in ("edges", "faces", "face_adj_edges", "edge_adj_faces", "mass", "d_hat", "offset", "vprop",
+                          "rest_vert_area", "rest_edge_area", "rest_face_area", "dm", "mu_lambda", "hinges",
+                          "rest_angle", "bend_stiff_area")] + [
+        ("nslices", C.c_uint32), ("nslots", C.c_uint32)] + [
+        (n, vp) for n in ("slice_off", "col", "diag_slot", "face_slots", "hinge_slots")]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("kappa", C.c_double), ("energy", C.c_int),
+                ("damping", C.c_double), ("nonlinear_iters", C.c_int), ("use_self_collision", C.c_int),
+                ("use_ccd", C.c_int), ("use_untangling", C.c_int), ("pcg", PcgParams)]
+
+
+class StepReport(C.Structure):
+    _fields_ = [("vf_pairs", C.c_uint32), ("ee_pairs", C.c_uint32), ("ef_pairs", C.c_uint32),
+                ("ef_total", C.c_uint32), ("contours", C.c_uint32), ("pcg_iterations", C.c_uint32),
+                ("pcg_refinements", C.c_uint32), ("pcg_relative_residual", C.c_double), ("pcg_converged", C.c_int),
+                ("toi", C.c_float), ("energy", C.c_double * 3), ("ms_detect", C.c_float), ("ms_contours", C.c_float),
+                ("ms_assemble", C.c_float), ("ms_pcg", C.c_float), ("ms_ccd", C.c_float), ("ms_total", C.c_float)]
 
 
 def lib():

@@ -8,6 +8,7 @@
 #include "contour.hpp"
 #include "pcg.hpp"
 #include "contact.hpp"
+#include "solver.hpp"
 
 #include <new>
 
@@ -26,6 +27,10 @@ struct ut_contours
 struct ut_pcg
 {
 	ut::PcgSolver s;
+};
+struct ut_solver
+{
+	ut::NewtonSolver s;
 };
 
 namespace
@@ -553,6 +558,239 @@ UT_API int ut_toi_reset(ut_ctx* ctx, float* toi)
 		{
 			UT_REQUIRE(toi, "null toi");
 			ut::toi_reset(&ctx->c, toi);
+		});
+}
+
+// ---- Newton solver -------------------------------------------------------------
+namespace
+{
+ut::PcgParams to_pcg(const ut_pcg_params& q)
+{
+	ut::PcgParams p;
+	p.precision = q.precision;
+	p.preconditioner = q.preconditioner;
+	p.max_iters = q.max_iters;
+	p.rel_tol = q.rel_tol;
+	p.inner_tol = q.inner_tol;
+	p.max_refine = q.max_refine;
+	p.regularization = q.regularization;
+	p.check_every = q.check_every ? q.check_every : 8;
+	return p;
+}
+ut::SolverParams to_params(const ut_solver_params& q)
+{
+	ut::SolverParams p;
+	p.dt = q.dt;
+	for (int k = 0; k < 3; ++k)
+		p.gravity[k] = q.gravity[k];
+	p.kappa = q.kappa;
+	p.energy = q.energy;
+	p.damping = q.damping;
+	p.nonlinear_iters = q.nonlinear_iters;
+	p.use_self_collision = q.use_self_collision;
+	p.use_ccd = q.use_ccd;
+	p.use_untangling = q.use_untangling;
+	p.pcg = to_pcg(q.pcg);
+	return p;
+}
+} // namespace
+
+UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,
+	const float* v0, ut_solver** out)
+{
+	if (!ctx || !desc || !params || !out)
+		return ut::kInvalid;
+	*out = nullptr;
+	ut_solver* h = new (std::nothrow) ut_solver();
+	if (!h)
+		return ut::kInternal;
+	int rc = guarded(&ctx->c,
+		[&]
+		{
+			UT_REQUIRE(x0, "null initial positions");
+			ut::SolverDesc d;
+			d.nv = desc->nv;
+			d.ne = desc->ne;
+			d.nf = desc->nf;
+			d.nh = desc->nh;
+			d.edges = desc->edges;
+			d.faces = desc->faces;
+			d.face_adj_edges = desc->face_adj_edges;
+			d.edge_adj_faces = desc->edge_adj_faces;
+			d.mass = desc->mass;
+			d.d_hat = desc->d_hat;
+			d.offset = desc->offset;
+			d.vprop = desc->vprop;
+			d.rest_vert_area = desc->rest_vert_area;
+			d.rest_edge_area = desc->rest_edge_area;
+			d.rest_face_area = desc->rest_face_area;
+			d.dm = desc->dm;
+			d.mu_lambda = desc->mu_lambda;
+			d.hinges = desc->hinges;
+			d.rest_angle = desc->rest_angle;
+			d.bend_stiff_area = desc->bend_stiff_area;
+			d.nslices = desc->nslices;
+			d.nslots = desc->nslots;
+			d.slice_off = desc->slice_off;
+			d.col = desc->col;
+			d.diag_slot = desc->diag_slot;
+			d.face_slots = desc->face_slots;
+			d.hinge_slots = desc->hinge_slots;
+			h->s.init(&ctx->c, d, to_params(*params), x0, v0);
+		});
+	if (rc != ut::kOk)
+	{
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return ut::kOk;
+}
+
+UT_API int ut_solver_destroy(ut_solver* s)
+{
+	if (s)
+	{
+		if (s->s.ctx)
+			(void)hipStreamSynchronize(s->s.ctx->stream);
+		delete s;
+	}
+	return ut::kOk;
+}
+
+UT_API int ut_solver_step(ut_solver* s, ut_step_report* report)
+{
+	if (!s)
+		return ut::kInvalid;
+	ut::StepReport r;
+	int			   rc = guarded(s->s.ctx, [&] { r = s->s.physics_step_GPU(); });
+	if (rc != ut::kOk)
+		return rc;
+	if (report)
+	{
+		report->vf_pairs = r.vf_pairs;
+		report->ee_pairs = r.ee_pairs;
+		report->ef_pairs = r.ef_pairs;
+		report->ef_total = r.ef_total;
+		report->contours = r.contours;
+		report->pcg_iterations = r.pcg_iterations;
+		report->pcg_refinements = r.pcg_refinements;
+		report->pcg_relative_residual = r.pcg_relative_residual;
+		report->pcg_converged = r.pcg_converged;
+		report->toi = r.toi;
+		for (int k = 0; k < 3; ++k)
+			report->energy[k] = r.energy[k];
+		report->ms_detect = r.ms_detect;
+		report->ms_contours = r.ms_contours;
+		report->ms_assemble = r.ms_assemble;
+		report->ms_pcg = r.ms_pcg;
+		report->ms_ccd = r.ms_ccd;
+		report->ms_total = r.ms_total;
+	}
+	return ut::kOk;
+}
+
+UT_API int ut_solver_set_params(ut_solver* s, const ut_solver_params* params)
+{
+	if (!s || !params)
+		return ut::kInvalid;
+	return guarded(s->s.ctx, [&] { s->s.p = to_params(*params); });
+}
+
+UT_API int ut_solver_get_state(ut_solver* s, float* x, float* v)
+{
+	if (!s)
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			const size_t bytes = 12 * (size_t)s->s.d.nv;
+			if (x)
+				UT_HIP_CHECK(hipMemcpyAsync(x, s->s.x.get(), bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+			if (v)
+				UT_HIP_CHECK(hipMemcpyAsync(v, s->s.v.get(), bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+		});
+}
+
+UT_API int ut_solver_set_state(ut_solver* s, const float* x, const float* v)
+{
+	if (!s)
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			const size_t bytes = 12 * (size_t)s->s.d.nv;
+			if (x)
+				UT_HIP_CHECK(hipMemcpyAsync(s->s.x.get(), x, bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+			if (v)
+				UT_HIP_CHECK(hipMemcpyAsync(s->s.v.get(), v, bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+		});
+}
+
+UT_API int ut_solver_contour_get(ut_solver* s, int what, void* dst)
+{
+	if (!s)
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			auto&		 b = s->s.contours;
+			UT_REQUIRE(b.ctx, "solver has no contour builder (topology not given)");
+			const size_t n = b.num_total;
+			const void*	 src = nullptr;
+			size_t		 bytes = 0;
+			switch (what)
+			{
+			case UT_CT_PAIR_IDS: src = b.ids.get(), bytes = 8 * n; break;
+			case UT_CT_PAIRS: src = b.pairs.get(), bytes = sizeof(ut::EfPairRec) * n; break;
+			case UT_CT_CONTOUR_INDEX: src = b.contour_index.get(), bytes = 4 * n; break;
+			case UT_CT_MESH_INDEX: src = b.mesh_index.get(), bytes = 4 * n; break;
+			case UT_CT_ADJ: src = b.adj.get(), bytes = 4 * ut::kMaxEfAdj * n; break;
+			case UT_CT_ADJ_N: src = b.adj_n.get(), bytes = 4 * n; break;
+			case UT_CT_INCONSISTENT: src = b.comp_inconsistent.get(), bytes = 4 * n; break;
+			case UT_CT_COMP_MIN: src = b.comp_min.get(), bytes = 4 * n; break;
+			default: throw ut::Error(ut::kInvalid, "unknown contour output");
+			}
+			UT_REQUIRE(dst || bytes == 0, "null destination");
+			if (bytes)
+				UT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
+		});
+}
+
+UT_API int ut_assemble(ut_ctx* ctx, const ut_assemble_args* q)
+{
+	if (!ctx || !q)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			ut::AssembleArgs a;
+			a.nv = q->nv;
+			a.nf = q->nf;
+			a.nh = q->nh;
+			a.nslots = q->nslots;
+			a.x = q->x;
+			a.x_tilde = q->x_tilde;
+			a.mass = q->mass;
+			a.inv_h2 = q->inv_h2;
+			a.faces = q->faces;
+			a.dm = q->dm;
+			a.face_area = q->face_area;
+			a.mu_lambda = q->mu_lambda;
+			a.face_slots = q->face_slots;
+			a.hinges = q->hinges;
+			a.rest_angle = q->rest_angle;
+			a.bend_stiff_area = q->bend_stiff_area;
+			a.hinge_slots = q->hinge_slots;
+			a.contacts = reinterpret_cast<const ut::ContactRec*>(q->contacts);
+			a.contact_count = q->contact_count;
+			a.contact_cap = q->contact_cap;
+			a.diag_slot = q->diag_slot;
+			a.grad = q->grad;
+			a.vals = q->vals;
+			a.vals32 = q->vals32;
+			a.energy = q->energy;
+			ut::assemble(&ctx->c, a);
 		});
 }
 

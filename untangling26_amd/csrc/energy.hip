@@ -1,0 +1,231 @@
+// Gradient and Hessian assembly of the incremental potential on gfx950.
+//
+// Material terms (inertia, BW98 stretch/shear, dihedral bending) follow
+// Energies/{soft_inertia_energy,stretch_face_energy,bending_energy_kernel}.cpp;
+// contact terms follow fn_perPair_assemble_gradient_hessian
+// (narrow_phase.cpp:2619-2746): gradient k1 w_i n, diagonal blocks
+// w_i^2 k2 n n^T.  The contacts' off-diagonal blocks (w_i w_j k2 n n^T) are not
+// stored: the PCG applies them matrix-free (pcg.hip), so the block pattern
+// stays the static cloth pattern.
+//
+// Everything is evaluated in FP64 and accumulated with FP64 atomics into the
+// gradient (double[3V]) and the planar block values (double[9][nslots]); the
+// mixed-precision solve rounds the blocks to FP32 afterwards.
+#include "ut_common.hpp"
+#include "energy.hpp"
+#include "energy_math.hpp"
+
+namespace ut
+{
+
+__device__ __forceinline__ void add3(double* g, uint32_t v, const v3<double>& a)
+{
+	atomicAdd(&g[3 * v + 0], a.x);
+	atomicAdd(&g[3 * v + 1], a.y);
+	atomicAdd(&g[3 * v + 2], a.z);
+}
+
+__global__ void k_zero64(double* p, size_t n)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = 0.0;
+}
+
+// E = m / (2 h^2) |x - x_tilde|^2 (soft_inertia_energy.cpp:25-110)
+__global__ void k_asm_inertia(uint32_t nv, const float* __restrict__ x, const double* __restrict__ x_tilde,
+	const float* __restrict__ mass, double inv_h2, const uint32_t* __restrict__ diag_slot, uint32_t nslots,
+	double* __restrict__ grad, double* __restrict__ vals, double* __restrict__ energy_part)
+{
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   e = 0.0;
+	if (v < nv)
+	{
+		const double k = (double)mass[v] * inv_h2;
+		double		 d[3];
+		for (int a = 0; a < 3; ++a)
+		{
+			d[a] = (double)x[3 * v + a] - x_tilde[3 * v + a];
+			atomicAdd(&grad[3 * v + a], k * d[a]);
+		}
+		e = 0.5 * k * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+		const uint32_t s = diag_slot[v];
+		atomicAdd(&vals[0 * (size_t)nslots + s], k);
+		atomicAdd(&vals[4 * (size_t)nslots + s], k);
+		atomicAdd(&vals[8 * (size_t)nslots + s], k);
+	}
+	if (energy_part)
+	{
+		e = wave_sum(e);
+		if (__lane_id() == 0)
+			atomicAdd(energy_part, e);
+	}
+}
+
+__global__ void k_asm_stretch(uint32_t nf, const float* __restrict__ x, const uint32_t* __restrict__ faces,
+	const float* __restrict__ dm, const float* __restrict__ area, const float* __restrict__ mu_lambda,
+	const uint32_t* __restrict__ face_slots, uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals,
+	double* __restrict__ energy_part)
+{
+	const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   e = 0.0;
+	if (f < nf)
+	{
+		const uint32_t vi[3] = { faces[3 * f], faces[3 * f + 1], faces[3 * f + 2] };
+		const v3<double> x0 = load_v3<double>(x, vi[0]), x1 = load_v3<double>(x, vi[1]), x2 = load_v3<double>(x, vi[2]);
+		const double	 d0 = dm[4 * f], d1 = dm[4 * f + 1], d2 = dm[4 * f + 2], d3 = dm[4 * f + 3];
+		const v3<double> r1 = x1 - x0, r2 = x2 - x0;
+		const v3<double> Fu = d0 * r1 + d1 * r2, Fv = d2 * r1 + d3 * r2;
+		const double	 A = area[f];
+		const Bw98		 b = bw98_eval(Fu, Fv, (double)mu_lambda[2 * f], (double)mu_lambda[2 * f + 1]);
+		e = A * b.energy;
+		const double c[3][2] = { { -(d0 + d1), -(d2 + d3) }, { d0, d2 }, { d1, d3 } };
+		for (int a = 0; a < 3; ++a)
+		{
+			v3<double> g{ 0, 0, 0 };
+			for (int k = 0; k < 3; ++k)
+				(&g.x)[k] = A * (c[a][0] * b.dedF[k] + c[a][1] * b.dedF[3 + k]);
+			add3(grad, vi[a], g);
+		}
+		for (int a = 0; a < 3; ++a)
+			for (int bb = 0; bb < 3; ++bb)
+			{
+				const uint32_t s = face_slots[9 * f + 3 * a + bb];
+				for (int i = 0; i < 3; ++i)
+					for (int j = 0; j < 3; ++j)
+					{
+						double h = 0.0;
+						for (int p = 0; p < 2; ++p)
+							for (int q = 0; q < 2; ++q)
+								h += c[a][p] * c[bb][q] * b.H(3 * p + i, 3 * q + j);
+						atomicAdd(&vals[(size_t)(3 * i + j) * nslots + s], A * h);
+					}
+			}
+	}
+	if (energy_part)
+	{
+		e = wave_sum(e);
+		if (__lane_id() == 0)
+			atomicAdd(energy_part, e);
+	}
+}
+
+// E = 1/2 k A (theta - theta0)^2, Gauss-Newton Hessian k A grad grad^T
+__global__ void k_asm_bending(uint32_t nh, const float* __restrict__ x, const uint32_t* __restrict__ hinges,
+	const float* __restrict__ rest_angle, const float* __restrict__ stiff_area, const uint32_t* __restrict__ hinge_slots,
+	uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals, double* __restrict__ energy_part)
+{
+	const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   e = 0.0;
+	if (h < nh)
+	{
+		uint32_t   vi[4];
+		v3<double> p[4];
+		for (int k = 0; k < 4; ++k)
+		{
+			vi[k] = hinges[4 * h + k];
+			p[k] = load_v3<double>(x, vi[k]);
+		}
+		v3<double>	 g[4];
+		const double theta = dihedral(p[0], p[1], p[2], p[3], g);
+		const double ks = stiff_area[h];
+		const double dth = theta - (double)rest_angle[h];
+		e = 0.5 * ks * dth * dth;
+		for (int a = 0; a < 4; ++a)
+			add3(grad, vi[a], (ks * dth) * g[a]);
+		for (int a = 0; a < 4; ++a)
+			for (int b = 0; b < 4; ++b)
+			{
+				const uint32_t s = hinge_slots[16 * h + 4 * a + b];
+				const double   ga[3] = { g[a].x, g[a].y, g[a].z }, gb[3] = { g[b].x, g[b].y, g[b].z };
+				for (int i = 0; i < 3; ++i)
+					for (int j = 0; j < 3; ++j)
+						atomicAdd(&vals[(size_t)(3 * i + j) * nslots + s], ks * ga[i] * gb[j]);
+			}
+	}
+	if (energy_part)
+	{
+		e = wave_sum(e);
+		if (__lane_id() == 0)
+			atomicAdd(energy_part, e);
+	}
+}
+
+// CollisionPairTemplate weights (collision_data.h:131-136)
+__device__ __forceinline__ void contact_weights(const ContactRec& c, double w[4])
+{
+	const bool vf = (c.idx[2] & 0x80000000u) != 0u;
+	const double b0 = c.bary[0], b1 = c.bary[1];
+	if (vf)
+	{
+		w[0] = 1.0;
+		w[1] = -b0;
+		w[2] = -b1;
+		w[3] = b0 + b1 - 1.0;
+	}
+	else
+	{
+		w[0] = b0;
+		w[1] = 1.0 - b0;
+		w[2] = -b1;
+		w[3] = b1 - 1.0;
+	}
+}
+
+__global__ void k_asm_contact(const ContactRec* __restrict__ pairs, const uint32_t* __restrict__ count, uint32_t cap,
+	const uint32_t* __restrict__ diag_slot, uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals)
+{
+	const uint32_t n = min(*count, cap);
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const ContactRec c = pairs[i];
+		double			 w[4];
+		contact_weights(c, w);
+		const double nn[3] = { c.normal[0], c.normal[1], c.normal[2] };
+		const double k1 = c.k1, k2 = c.k2;
+		for (int a = 0; a < 4; ++a)
+		{
+			const uint32_t v = c.idx[a] & 0x7FFFFFFFu;
+			add3(grad, v, V3m(w[a] * k1 * nn[0], w[a] * k1 * nn[1], w[a] * k1 * nn[2]));
+			const uint32_t s = diag_slot[v];
+			for (int r = 0; r < 3; ++r)
+				for (int q = 0; q < 3; ++q)
+					atomicAdd(&vals[(size_t)(3 * r + q) * nslots + s], w[a] * w[a] * k2 * nn[r] * nn[q]);
+		}
+	}
+}
+
+__global__ void k_round_to32(size_t n, const double* __restrict__ in, float* __restrict__ out)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = (float)in[i];
+}
+
+void assemble(Context* ctx, const AssembleArgs& a)
+{
+	UT_REQUIRE(a.x && a.grad && a.vals && a.diag_slot, "assemble: null input");
+	hipStream_t	   s = ctx->stream;
+	const size_t   nv3 = 3 * (size_t)a.nv, nval = 9 * (size_t)a.nslots;
+	k_zero64<<<div_up((uint32_t)nv3, 256), 256, 0, s>>>(a.grad, nv3);
+	k_zero64<<<(uint32_t)((nval + 255) / 256), 256, 0, s>>>(a.vals, nval);
+	if (a.energy)
+		k_zero64<<<1, 64, 0, s>>>(a.energy, 4);
+	if (a.mass && a.x_tilde)
+		k_asm_inertia<<<div_up(a.nv, 256), 256, 0, s>>>(a.nv, a.x, a.x_tilde, a.mass, a.inv_h2, a.diag_slot, a.nslots,
+			a.grad, a.vals, a.energy ? a.energy + 0 : nullptr);
+	if (a.nf && a.faces && a.dm)
+		k_asm_stretch<<<div_up(a.nf, 256), 256, 0, s>>>(a.nf, a.x, a.faces, a.dm, a.face_area, a.mu_lambda,
+			a.face_slots, a.nslots, a.grad, a.vals, a.energy ? a.energy + 1 : nullptr);
+	if (a.nh && a.hinges)
+		k_asm_bending<<<div_up(a.nh, 256), 256, 0, s>>>(a.nh, a.x, a.hinges, a.rest_angle, a.bend_stiff_area,
+			a.hinge_slots, a.nslots, a.grad, a.vals, a.energy ? a.energy + 2 : nullptr);
+	if (a.contacts && a.contact_count && a.contact_cap)
+		k_asm_contact<<<1024, 256, 0, s>>>(a.contacts, a.contact_count, a.contact_cap, a.diag_slot, a.nslots, a.grad,
+			a.vals);
+	if (a.vals32)
+		k_round_to32<<<(uint32_t)((nval + 255) / 256), 256, 0, s>>>(nval, a.vals, a.vals32);
+	UT_LAUNCH_CHECK();
+}
+
+} // namespace ut

@@ -1,0 +1,41 @@
+#pragma once
+#include "ut_runtime.hpp"
+#include "contact.hpp"
+
+namespace ut
+{
+
+struct AssembleArgs
+{
+	uint32_t nv = 0, nf = 0, nh = 0, nslots = 0;
+	const float* x = nullptr; // current positions float[3V]
+	// inertia: m / h^2 (x - x_tilde)
+	const double* x_tilde = nullptr; // double[3V]
+	const float*  mass = nullptr;	 // per vertex
+	double		  inv_h2 = 0.0;
+	// BW98 stretch / shear per face
+	const uint32_t* faces = nullptr;
+	const float*	dm = nullptr;		 // float[4F]: d0, d1, d2, d3 with Dm = [[d0, d2], [d1, d3]]
+	const float*	face_area = nullptr; // rest area
+	const float*	mu_lambda = nullptr; // float[2F]
+	const uint32_t* face_slots = nullptr; // uint32[9F]: slot of block (a, b)
+	// dihedral bending per hinge (e0, e1, o0, o1)
+	const uint32_t* hinges = nullptr;
+	const float*	rest_angle = nullptr;
+	const float*	bend_stiff_area = nullptr;
+	const uint32_t* hinge_slots = nullptr; // uint32[16H]
+	// contacts (diagonal blocks + gradient)
+	const ContactRec* contacts = nullptr;
+	const uint32_t*	  contact_count = nullptr;
+	uint32_t		  contact_cap = 0;
+	// outputs
+	const uint32_t* diag_slot = nullptr;
+	double*			grad = nullptr;	  // dE/dx, double[3V] (the Newton rhs is -grad)
+	double*			vals = nullptr;	  // double[9 * nslots]
+	float*			vals32 = nullptr; // optional FP32 copy of vals
+	double*			energy = nullptr; // optional double[4]: inertia, stretch, bending, (unused)
+};
+
+void assemble(Context* ctx, const AssembleArgs& a);
+
+} // namespace ut

@@ -113,6 +113,61 @@ __device__ __forceinline__ V3<TA> row_spmv(uint32_t row, const uint32_t* __restr
 	return V3<TA>{ a0, a1, a2 };
 }
 
+// ---- matrix-free contact off-diagonal blocks ------------------------------------
+constexpr int kContactBlocks = 256;
+
+__device__ __forceinline__ void pair_weights(const ContactRec& c, double w[4])
+{
+	const double b0 = c.bary[0], b1 = c.bary[1];
+	if (c.idx[2] & 0x80000000u) // VF (collision_data.h:133)
+	{
+		w[0] = 1.0, w[1] = -b0, w[2] = -b1, w[3] = b0 + b1 - 1.0;
+	}
+	else // EE (collision_data.h:134)
+	{
+		w[0] = b0, w[1] = 1.0 - b0, w[2] = -b1, w[3] = b1 - 1.0;
+	}
+}
+
+// q_i += sign * k2 w_i (S - w_i s_i) n with s_j = n . p_j, S = sum_j w_j s_j;
+// partial of p^T A_off p = k2 (S^2 - sum_j w_j^2 s_j^2).
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_contact_offdiag(const ContactRec* __restrict__ pairs,
+	const uint32_t* __restrict__ count, uint32_t cap, const T* __restrict__ p, T* __restrict__ q, double sign,
+	double* __restrict__ part, const uint32_t* __restrict__ flags)
+{
+	if (flags && group_stopped(flags))
+		return;
+	const uint32_t n = min(*count, cap);
+	double		   acc = 0.0;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const ContactRec c = pairs[i];
+		double			 w[4], sv[4];
+		pair_weights(c, w);
+		const double nx = c.normal[0], ny = c.normal[1], nz = c.normal[2], k2 = c.k2;
+		double		 S = 0.0, diag = 0.0;
+		uint32_t	 v[4];
+		for (int a = 0; a < 4; ++a)
+		{
+			v[a] = c.idx[a] & 0x7FFFFFFFu;
+			sv[a] = nx * (double)p[3 * v[a]] + ny * (double)p[3 * v[a] + 1] + nz * (double)p[3 * v[a] + 2];
+			S += w[a] * sv[a];
+			diag += w[a] * w[a] * sv[a] * sv[a];
+		}
+		for (int a = 0; a < 4; ++a)
+		{
+			const double cf = sign * k2 * w[a] * (S - w[a] * sv[a]);
+			atomicAdd(&q[3 * v[a]], (T)(cf * nx));
+			atomicAdd(&q[3 * v[a] + 1], (T)(cf * ny));
+			atomicAdd(&q[3 * v[a] + 2], (T)(cf * nz));
+		}
+		acc += k2 * (S * S - diag);
+	}
+	if (part)
+		block_partial(acc, part);
+}
+
 // ---- preconditioner -----------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_precond(uint32_t n, const T* __restrict__ vals, uint32_t nslots,
@@ -182,6 +237,7 @@ __device__ __forceinline__ V3<T> apply_minv(const T* __restrict__ minv, uint32_t
 struct PcgParts
 {
 	double* pq;		// [nb]
+	double* pqc;	// [kContactBlocks] contact part of p^T A p (zeros when no contacts)
 	double* rz[2];	// [nb] ping-pong by iteration parity
 	double* rr;		// [nb]
 	double* bb;		// [nb]
@@ -266,7 +322,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(uint32_t it, uint32_
 {
 	if (group_stopped(flags))
 		return;
-	const double pq = sum_partials(parts.pq, nb);
+	const double pq = sum_partials(parts.pq, nb) + sum_partials(parts.pqc, kContactBlocks);
 	const double rz = sum_partials(parts.rz[(it + 1) & 1], nb);
 	const double alpha = rz / pq;
 	const bool	 ok = isfinite(alpha) && pq > 0.0;
@@ -372,6 +428,16 @@ __global__ __launch_bounds__(kPcgThreads) void k_spmv64(uint32_t n, const uint32
 		stv(y, row, row_spmv<TV, double, double>(row, slice_off, col, vals, nslots, x));
 }
 
+__global__ __launch_bounds__(kPcgThreads) void k_norm2_64(uint32_t n, const double* __restrict__ r,
+	double* __restrict__ part)
+{
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   v = 0.0;
+	if (row < n)
+		v = r[3 * row] * r[3 * row] + r[3 * row + 1] * r[3 * row + 1] + r[3 * row + 2] * r[3 * row + 2];
+	block_partial(v, part);
+}
+
 template <typename T>
 __global__ void k_axpy_to64(uint32_t m, const T* __restrict__ d, double* __restrict__ x)
 {
@@ -425,7 +491,7 @@ void PcgSolver::init(Context* c, uint32_t n_, uint32_t nslices_, uint32_t nslots
 	// x, r, z, p, q (3n each), minv (9n), scale (n), r64 (3n), scale64 (n), d64 spare (3n) in doubles
 	// + 256 B of alignment slack per carved array (10 arrays)
 	work.reserve(sizeof(double) * ((size_t)5 * 3 * n + 9 * n + n + 3 * n + n + 3 * n) + 10 * 256);
-	partials.reserve(8 * (size_t)nb + 8);
+	partials.reserve(8 * (size_t)nb + 8 + kContactBlocks);
 	flags.reserve(4);
 	if (!h_flags)
 		UT_HIP_CHECK(hipHostMalloc((void**)&h_flags, 16, hipHostMallocDefault));
@@ -489,6 +555,9 @@ uint32_t run_iterations(PcgSolver& s, const Vecs<T>& v, const T* vals, PcgParts 
 		{
 			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
 				s.n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, lambda, v.scale, parts, s.flags.get());
+			if (s.contacts)
+				k_contact_offdiag<T><<<kContactBlocks, kPcgThreads, 0, st>>>(
+					s.contacts, s.contact_count, s.contact_cap, v.p, v.q, 1.0, parts.pqc, s.flags.get());
 			k_pcg_update<T><<<nb, kPcgThreads, 0, st>>>(
 				it, s.n, nb, v.minv, v.x, v.r, v.z, v.p, v.q, parts, s.flags.get());
 			k_pcg_dir<T><<<nb, kPcgThreads, 0, st>>>(it, s.n, nb, tol2, v.z, v.p, parts, s.flags.get());
@@ -509,9 +578,11 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 	const uint32_t n = s.n, nb = div_up(n, kPcgThreads);
 	Vecs<T>		   v = carve<T>(s.work.get(), n);
 	double*		   pb = s.partials.get();
-	PcgParts	   parts{ pb, { pb + nb, pb + 2 * nb }, pb + 3 * nb, pb + 4 * nb };
+	PcgParts	   parts{ pb, pb + 8 * nb + 8, { pb + nb, pb + 2 * nb }, pb + 3 * nb, pb + 4 * nb };
 	double*		   res_part = pb + 5 * nb;
 	double*		   d_norm = pb + 6 * nb;
+	if (!s.contacts)
+		UT_HIP_CHECK(hipMemsetAsync(parts.pqc, 0, sizeof(double) * kContactBlocks, st));
 	PcgReport	   rep;
 	const bool	   mixed = prm.precision == kPcgMixed;
 
@@ -524,6 +595,12 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 	{
 		k_residual64<T><<<nb, kPcgThreads, 0, st>>>(
 			n, s.slice_off, s.col, vals, s.nslots, x, b, v.r64, prm.regularization, v.scale64, res_part);
+		if (s.contacts)
+		{
+			k_contact_offdiag<double><<<kContactBlocks, kPcgThreads, 0, st>>>(
+				s.contacts, s.contact_count, s.contact_cap, x, v.r64, -1.0, nullptr, nullptr);
+			k_norm2_64<<<nb, kPcgThreads, 0, st>>>(n, v.r64, res_part);
+		}
 		k_sum_partials_out<<<1, kPcgThreads, 0, st>>>(res_part, nb, d_norm);
 		UT_HIP_CHECK(hipMemcpyAsync(s.h_norm, d_norm, 8, hipMemcpyDeviceToHost, st));
 		UT_HIP_CHECK(hipStreamSynchronize(st));
@@ -637,6 +714,9 @@ void PcgSolver::spmv64(const void* vals, bool vals_fp32, const double* x, double
 	else
 		k_spmv64<double><<<nb, kPcgThreads, 0, ctx->stream>>>(
 			n, slice_off, col, reinterpret_cast<const double*>(vals), nslots, x, y);
+	if (contacts)
+		k_contact_offdiag<double><<<kContactBlocks, kPcgThreads, 0, ctx->stream>>>(
+			contacts, contact_count, contact_cap, x, y, 1.0, nullptr, nullptr);
 	UT_LAUNCH_CHECK();
 }
 

@@ -19,6 +19,7 @@
 #pragma once
 
 #include "ut_runtime.hpp"
+#include "contact.hpp"
 
 namespace ut
 {
@@ -64,6 +65,13 @@ struct PcgSolver
 	const uint32_t* slice_off = nullptr; // [nslices+1] device
 	const uint32_t* col = nullptr;		 // [nslots]
 	const uint32_t* diag_slot = nullptr; // [n]
+
+	// Optional contact pairs whose off-diagonal Hessian blocks w_i w_j k2 n n^T
+	// (i != j) are applied matrix-free in every product with A; their diagonal
+	// blocks must already be in the block values.
+	const ContactRec* contacts = nullptr;
+	const uint32_t*	  contact_count = nullptr;
+	uint32_t		  contact_cap = 0;
 
 	DevBuf<uint8_t> work; // vectors (sized for FP64)
 	DevBuf<double>	partials;

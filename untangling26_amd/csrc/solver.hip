@@ -1,0 +1,331 @@
+// NewtonSolver::physics_step_GPU on gfx950 (newton_solver.cpp:2482-2767):
+// LBVH rebuild, prediction, then per Newton iteration: refit + fused VF/EE
+// proximity DCD + fused EF intersection detection + contours, assembly of the
+// incremental potential, PCG (FP64 or FP32 with FP64 refinement), and the
+// global additive-CCD line search; finally the velocity update.
+//
+// Host synchronisations per iteration: one count readback after detection
+// (resize-and-retry when a list overflowed, newton_solver.cpp:1402-1438), the
+// contour build's two, and the PCG convergence polls.  The line-search step
+// x = x0 + 0.9 toi dx reads toi on the device.
+#include "ut_common.hpp"
+#include "solver.hpp"
+
+#include <cmath>
+
+namespace ut
+{
+
+__global__ void k_predict(uint32_t nv, const float* __restrict__ x, const float* __restrict__ v, double h, double gx,
+	double gy, double gz, const uint32_t* __restrict__ vprop, double* __restrict__ x_tilde)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nv)
+		return;
+	// fn_predict_position (newton_solver.cpp:561-598): v_pred = v + h g, x_tilde = x + h v_pred
+	const bool fixed = vprop && (vprop[i] & kPropFixed);
+	const double g[3] = { gx, gy, gz };
+	for (int a = 0; a < 3; ++a)
+	{
+		const double vp = (double)v[3 * i + a] + (fixed ? 0.0 : h * g[a]);
+		x_tilde[3 * i + a] = (double)x[3 * i + a] + (fixed ? 0.0 : h * vp);
+	}
+}
+
+__global__ void k_negate(size_t n, const double* __restrict__ a, double* __restrict__ b)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		b[i] = -a[i];
+}
+
+__global__ void k_trial(size_t n, const float* __restrict__ x0, const double* __restrict__ dx, float* __restrict__ xe)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		xe[i] = (float)((double)x0[i] + dx[i]);
+}
+
+// x = x0 + alpha dx with alpha = toi, scaled by 0.9 when an impact limited it
+// (fn_scaling_toi, newton_solver.cpp:191-202, 1481-1482).
+__global__ void k_apply_step(size_t n, const float* __restrict__ x0, const double* __restrict__ dx,
+	const float* __restrict__ toi, float* __restrict__ x)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	float t = toi ? *toi : 1.0f;
+	if (t != 1.0f)
+		t = 0.9f * t;
+	x[i] = (float)((double)x0[i] + (double)t * dx[i]);
+}
+
+__global__ void k_update_velocity(size_t n, const float* __restrict__ x, const float* __restrict__ x_start, double h,
+	double damping, float* __restrict__ v)
+{
+	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		v[i] = (float)(((double)x[i] - (double)x_start[i]) / h * exp(-damping * h)); // fn_update_velocity
+}
+
+__global__ void k_zero_u32(uint32_t* p, uint32_t n)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = 0u;
+}
+
+static uint32_t g(size_t n) { return (uint32_t)((n + 255) / 256); }
+
+void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& params, const float* x0, const float* v0)
+{
+	ctx = c;
+	d = desc;
+	p = params;
+	UT_REQUIRE(d.nv && d.nf && d.ne && d.edges && d.faces && d.mass && d.d_hat && d.offset, "solver: missing mesh data");
+	UT_REQUIRE(d.slice_off && d.col && d.diag_slot && d.face_slots, "solver: missing block pattern");
+	const size_t n3 = 3 * (size_t)d.nv;
+	x.reserve(n3);
+	v.reserve(n3);
+	x_step_start.reserve(n3);
+	x_iter_start.reserve(n3);
+	x_end.reserve(n3);
+	x_tilde.reserve(n3);
+	grad.reserve(n3);
+	rhs.reserve(n3);
+	dx.reserve(n3);
+	vals64.reserve(9 * (size_t)d.nslots);
+	vals32.reserve(9 * (size_t)d.nslots);
+	energy.reserve(4);
+	toi.reserve(1);
+	counts.reserve(8);
+	UT_HIP_CHECK(hipMemcpyAsync(x.get(), x0, 4 * n3, hipMemcpyDeviceToDevice, ctx->stream));
+	if (v0)
+		UT_HIP_CHECK(hipMemcpyAsync(v.get(), v0, 4 * n3, hipMemcpyDeviceToDevice, ctx->stream));
+	else
+		UT_HIP_CHECK(hipMemsetAsync(v.get(), 0, 4 * n3, ctx->stream));
+	face_tree.init(ctx, kTreeFace, d.nf);
+	edge_tree.init(ctx, kTreeEdge, d.ne);
+	pcg.init(ctx, d.nv, d.nslices, d.nslots, d.slice_off, d.col, d.diag_slot);
+	if (d.face_adj_edges && d.edge_adj_faces)
+		contours.init(ctx, d.nv, d.ne, d.nf, d.edges, d.faces, d.face_adj_edges, d.edge_adj_faces);
+	ensure_contact_capacity(std::max<uint32_t>(4096, 4 * d.nv), std::max<uint32_t>(1024, d.ne / 4));
+	UT_HIP_CHECK(hipHostMalloc((void**)&h_counts, 32, hipHostMallocDefault));
+	for (auto& e : ev)
+		UT_HIP_CHECK(hipEventCreate(&e));
+}
+
+NewtonSolver::~NewtonSolver()
+{
+	if (h_counts)
+		(void)hipHostFree(h_counts);
+	for (auto& e : ev)
+		if (e)
+			(void)hipEventDestroy(e);
+}
+
+void NewtonSolver::ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef)
+{
+	if (need_contacts > cap_contacts)
+	{
+		cap_contacts = need_contacts;
+		contact_pairs.reserve(cap_contacts);
+		contact_ids.reserve(2 * (size_t)cap_contacts);
+	}
+	if (need_ef > cap_ef)
+	{
+		cap_ef = need_ef;
+		ef_pairs.reserve(cap_ef);
+		ef_ids.reserve(2 * (size_t)cap_ef);
+	}
+}
+
+// device_update_contact_list (newton_solver.cpp:1393-1445) with fused queries
+void NewtonSolver::detect(StepReport& rep)
+{
+	hipStream_t s = ctx->stream;
+	face_tree.update_leaves_and_refit(d.offset, x.get(), x.get(), d.faces);
+	edge_tree.update_leaves_and_refit(d.offset, x.get(), x.get(), d.edges);
+	for (int attempt = 0; attempt < 10; ++attempt)
+	{
+		k_zero_u32<<<1, 64, 0, s>>>(counts.get(), 8);
+		ContactArgs a;
+		a.num_verts = d.nv;
+		a.num_edges = d.ne;
+		a.x = x.get();
+		a.edges = d.edges;
+		a.faces = d.faces;
+		a.vprop = d.vprop;
+		a.d_hat = d.d_hat;
+		a.offset = d.offset;
+		a.rest_vert_area = d.rest_vert_area;
+		a.rest_edge_area = d.rest_edge_area;
+		a.rest_face_area = d.rest_face_area;
+		a.kappa = p.kappa;
+		a.energy = p.energy;
+		a.count = counts.get() + 3; // VF and EE share one list
+		a.pairs = contact_pairs.get();
+		a.pair_ids = reinterpret_cast<uint2*>(contact_ids.get());
+		a.max_pairs = cap_contacts;
+		contact_query(ctx, 0, a, &face_tree, nullptr, nullptr, 0);
+		UT_HIP_CHECK(hipMemcpyAsync(counts.get() + 0, counts.get() + 3, 4, hipMemcpyDeviceToDevice, s));
+		contact_query(ctx, 1, a, &edge_tree, nullptr, nullptr, 0);
+		if (p.use_untangling)
+		{
+			EfDcdArgs e;
+			e.x = x.get();
+			e.edges = d.edges;
+			e.faces = d.faces;
+			e.vprop = d.vprop;
+			e.d_hat = d.d_hat;
+			e.count = counts.get() + 2;
+			e.pairs = reinterpret_cast<uint32_t*>(ef_pairs.get());
+			e.pair_ids = ef_ids.get();
+			e.max_pairs = cap_ef;
+			ef_detect(ctx, face_tree, e, d.ne, d.offset);
+		}
+		UT_HIP_CHECK(hipMemcpyAsync(h_counts, counts.get(), 16, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipStreamSynchronize(s));
+		const uint32_t nc = h_counts[3], nef = h_counts[2];
+		if (nc <= cap_contacts && nef <= cap_ef)
+		{
+			rep.vf_pairs = h_counts[0];
+			rep.ee_pairs = nc - h_counts[0];
+			rep.ef_pairs = nef;
+			return;
+		}
+		ensure_contact_capacity(nc > cap_contacts ? 2 * nc : cap_contacts, nef > cap_ef ? 2 * nef : cap_ef);
+	}
+	throw Error(kCapacity, "contact lists still out of range after 10 resize attempts");
+}
+
+StepReport NewtonSolver::physics_step_GPU()
+{
+	StepReport	   rep;
+	hipStream_t	   s = ctx->stream;
+	const size_t   n3 = 3 * (size_t)d.nv;
+	const double   h = p.dt;
+	UT_HIP_CHECK(hipEventRecord(ev[0], s));
+	UT_HIP_CHECK(hipMemcpyAsync(x_step_start.get(), x.get(), 4 * n3, hipMemcpyDeviceToDevice, s));
+	if (p.use_self_collision)
+	{
+		face_tree.reduce_aabb(x.get(), d.faces);
+		face_tree.construct_tree();
+		edge_tree.reduce_aabb(x.get(), d.edges);
+		edge_tree.construct_tree();
+	}
+	k_predict<<<g(d.nv), 256, 0, s>>>(d.nv, x.get(), v.get(), h, p.gravity[0], p.gravity[1], p.gravity[2], d.vprop,
+		x_tilde.get());
+	const bool fp64 = p.pcg.precision == kPcgFp64;
+	float	   t_det = 0, t_ct = 0, t_asm = 0, t_pcg = 0, t_ccd = 0;
+	for (int it = 0; it < std::max(1, p.nonlinear_iters); ++it)
+	{
+		UT_HIP_CHECK(hipMemcpyAsync(x_iter_start.get(), x.get(), 4 * n3, hipMemcpyDeviceToDevice, s));
+		UT_HIP_CHECK(hipEventRecord(ev[1], s));
+		uint32_t ncontacts = 0;
+		if (p.use_self_collision)
+		{
+			detect(rep);
+			ncontacts = rep.vf_pairs + rep.ee_pairs;
+		}
+		UT_HIP_CHECK(hipEventRecord(ev[2], s));
+		if (p.use_self_collision && p.use_untangling && contours.ctx)
+		{
+			contours.build(rep.ef_pairs, ef_pairs.get(), ef_ids.get());
+			rep.ef_total = contours.num_total;
+			rep.contours = contours.num_contours;
+		}
+		UT_HIP_CHECK(hipEventRecord(ev[3], s));
+		AssembleArgs a;
+		a.nv = d.nv;
+		a.nf = d.nf;
+		a.nh = d.nh;
+		a.nslots = d.nslots;
+		a.x = x.get();
+		a.x_tilde = x_tilde.get();
+		a.mass = d.mass;
+		a.inv_h2 = 1.0 / (h * h);
+		a.faces = d.faces;
+		a.dm = d.dm;
+		a.face_area = d.rest_face_area;
+		a.mu_lambda = d.mu_lambda;
+		a.face_slots = d.face_slots;
+		a.hinges = d.hinges;
+		a.rest_angle = d.rest_angle;
+		a.bend_stiff_area = d.bend_stiff_area;
+		a.hinge_slots = d.hinge_slots;
+		a.contacts = ncontacts ? contact_pairs.get() : nullptr;
+		a.contact_count = counts.get() + 3;
+		a.contact_cap = cap_contacts;
+		a.diag_slot = d.diag_slot;
+		a.grad = grad.get();
+		a.vals = vals64.get();
+		a.vals32 = fp64 ? nullptr : vals32.get();
+		a.energy = energy.get();
+		assemble(ctx, a);
+		k_negate<<<g(n3), 256, 0, s>>>(n3, grad.get(), rhs.get());
+		UT_HIP_CHECK(hipEventRecord(ev[4], s));
+		pcg.contacts = ncontacts ? contact_pairs.get() : nullptr;
+		pcg.contact_count = counts.get() + 3;
+		pcg.contact_cap = cap_contacts;
+		const PcgReport pr =
+			pcg.solve(p.pcg, fp64 ? (const void*)vals64.get() : (const void*)vals32.get(), rhs.get(), dx.get());
+		rep.pcg_iterations += pr.iterations;
+		rep.pcg_refinements += pr.refinements;
+		rep.pcg_relative_residual = pr.relative_residual;
+		rep.pcg_converged = pr.converged;
+		UT_HIP_CHECK(hipEventRecord(ev[5], s));
+		// device_ccd_line_search (newton_solver.cpp:1446-1483)
+		k_trial<<<g(n3), 256, 0, s>>>(n3, x_iter_start.get(), dx.get(), x_end.get());
+		toi_reset(ctx, toi.get());
+		if (p.use_self_collision && p.use_ccd)
+		{
+			face_tree.update_leaves_and_refit(d.offset, x_iter_start.get(), x_end.get(), d.faces);
+			edge_tree.update_leaves_and_refit(d.offset, x_iter_start.get(), x_end.get(), d.edges);
+			ContactArgs c;
+			c.num_verts = d.nv;
+			c.num_edges = d.ne;
+			c.x = x_iter_start.get();
+			c.x_end = x_end.get();
+			c.edges = d.edges;
+			c.faces = d.faces;
+			c.vprop = d.vprop;
+			c.d_hat = d.d_hat;
+			c.offset = d.offset;
+			c.toi = toi.get();
+			contact_query(ctx, 2, c, &face_tree, nullptr, nullptr, 0);
+			contact_query(ctx, 3, c, &edge_tree, nullptr, nullptr, 0);
+		}
+		k_apply_step<<<g(n3), 256, 0, s>>>(n3, x_iter_start.get(), dx.get(), toi.get(), x.get());
+		UT_HIP_CHECK(hipEventRecord(ev[6], s));
+		UT_HIP_CHECK(hipEventSynchronize(ev[6]));
+		float ms;
+		UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+		t_det += ms;
+		UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[2], ev[3]));
+		t_ct += ms;
+		UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[3], ev[4]));
+		t_asm += ms;
+		UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[4], ev[5]));
+		t_pcg += ms;
+		UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[5], ev[6]));
+		t_ccd += ms;
+	}
+	k_update_velocity<<<g(n3), 256, 0, s>>>(n3, x.get(), x_step_start.get(), h, p.damping, v.get());
+	UT_HIP_CHECK(hipEventRecord(ev[7], s));
+	UT_HIP_CHECK(hipMemcpyAsync(&rep.toi, toi.get(), 4, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipMemcpyAsync(rep.energy, energy.get(), 24, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipEventSynchronize(ev[7]));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
+	float total;
+	UT_HIP_CHECK(hipEventElapsedTime(&total, ev[0], ev[7]));
+	rep.ms_detect = t_det;
+	rep.ms_contours = t_ct;
+	rep.ms_assemble = t_asm;
+	rep.ms_pcg = t_pcg;
+	rep.ms_ccd = t_ccd;
+	rep.ms_total = total;
+	return rep;
+}
+
+} // namespace ut

@@ -1,0 +1,105 @@
+// One implicit Newton step of the untangling simulator on one GPU:
+// NewtonSolver::physics_step_GPU (Solver/SimulationSolver/newton_solver.cpp:2482-2767).
+#pragma once
+
+#include "contact.hpp"
+#include "contour.hpp"
+#include "energy.hpp"
+#include "lbvh.hpp"
+#include "pcg.hpp"
+
+#include <hip/hip_runtime.h>
+
+namespace ut
+{
+
+struct SolverDesc
+{
+	uint32_t nv = 0, ne = 0, nf = 0, nh = 0;
+	// topology (device)
+	const uint32_t* edges = nullptr;
+	const uint32_t* faces = nullptr;
+	const uint32_t* face_adj_edges = nullptr;
+	const uint32_t* edge_adj_faces = nullptr;
+	// per-vertex data (device)
+	const float*	mass = nullptr;
+	const float*	d_hat = nullptr;
+	const float*	offset = nullptr;
+	const uint32_t* vprop = nullptr;
+	const float*	rest_vert_area = nullptr;
+	const float*	rest_edge_area = nullptr;
+	const float*	rest_face_area = nullptr;
+	// material (device)
+	const float*	dm = nullptr;
+	const float*	mu_lambda = nullptr;
+	const uint32_t* hinges = nullptr;
+	const float*	rest_angle = nullptr;
+	const float*	bend_stiff_area = nullptr;
+	// block pattern (device)
+	uint32_t		nslices = 0, nslots = 0;
+	const uint32_t* slice_off = nullptr;
+	const uint32_t* col = nullptr;
+	const uint32_t* diag_slot = nullptr;
+	const uint32_t* face_slots = nullptr;
+	const uint32_t* hinge_slots = nullptr;
+};
+
+struct SolverParams
+{
+	double dt = 1.0 / 60.0;
+	double gravity[3] = { 0.0, -9.8, 0.0 };
+	double kappa = 1e8;
+	int	   energy = kEnergyBarrier;
+	double damping = 2.0;
+	int	   nonlinear_iters = 1;
+	int	   use_self_collision = 1;
+	int	   use_ccd = 1;
+	int	   use_untangling = 1; // EF detection + contours every iteration
+	PcgParams pcg;
+};
+
+struct StepReport
+{
+	uint32_t vf_pairs = 0, ee_pairs = 0, ef_pairs = 0, ef_total = 0, contours = 0;
+	uint32_t pcg_iterations = 0, pcg_refinements = 0;
+	double	 pcg_relative_residual = 0.0;
+	int		 pcg_converged = 0;
+	float	 toi = 1.0f;
+	double	 energy[3] = { 0, 0, 0 };
+	float	 ms_detect = 0, ms_contours = 0, ms_assemble = 0, ms_pcg = 0, ms_ccd = 0, ms_total = 0;
+};
+
+struct NewtonSolver
+{
+	Context*	 ctx = nullptr;
+	SolverDesc	 d;
+	SolverParams p;
+
+	DevBuf<float>  x, v, x_step_start, x_iter_start, x_end;
+	DevBuf<double> x_tilde, grad, rhs, dx, vals64, energy;
+	DevBuf<float>  vals32;
+	DevBuf<float>  toi; // [1]
+	Lbvh		   face_tree, edge_tree;
+	PcgSolver	   pcg;
+	ContourBuilder contours;
+
+	// contact lists: VF pairs then EE pairs in one array
+	uint32_t			cap_contacts = 0, cap_ef = 0;
+	DevBuf<ContactRec>	contact_pairs;
+	DevBuf<uint32_t>	contact_ids;
+	DevBuf<EfPairRec>	ef_pairs;
+	DevBuf<uint32_t>	ef_ids;
+	DevBuf<uint32_t>	counts; // [0] VF, [1] EE, [2] EF, [3] total contacts
+	uint32_t*			h_counts = nullptr;
+	hipEvent_t			ev[8] = {};
+
+	void init(Context* c, const SolverDesc& desc, const SolverParams& params, const float* x0, const float* v0);
+	~NewtonSolver();
+	StepReport physics_step_GPU();
+
+private:
+	void detect(StepReport& rep);
+	void ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef);
+};
+
+} // namespace ut

@@ -123,6 +123,29 @@ def nested_layers(seed: int = 0, layers: int = 8, n: int = 128, side: float = 1.
     return Mesh(np.concatenate(xs, 0), np.concatenate(fs, 0), name=f"nested_{layers}x{n}x{n}")
 
 
+def folded_cloth_rest(n: int = 256, side: float = 2.0) -> np.ndarray:
+    """Rest shape of folded_cloth: the flat, unfolded sheet (same vertex order)."""
+    return grid_sheet(n, n, side, side).astype(np.float32)
+
+
+def nested_layers_rest(layers: int = 8, n: int = 128, side: float = 1.0, spacing: float = 0.004) -> np.ndarray:
+    xs = []
+    for k in range(layers):
+        x = grid_sheet(n, n, side, side)
+        x[:, 1] = k * spacing
+        xs.append(x)
+    return np.concatenate(xs, 0).astype(np.float32)
+
+
+def rest_shape(name: str, mesh: Mesh, **kw) -> np.ndarray:
+    """Rest positions of a named scene (two_sheets is at rest as generated)."""
+    if name == "folded_cloth":
+        return folded_cloth_rest(**kw)
+    if name == "nested_layers":
+        return nested_layers_rest(**kw)
+    return mesh.x.copy()
+
+
 def per_vertex(mesh: Mesh, d_hat: float = 1e-3, thickness: float = 0.0):
     """Per-vertex d_hat and contact offset arrays (init_collision_data.cpp:119-240)."""
     return (np.full(mesh.nv, d_hat, dtype=np.float32), np.full(mesh.nv, thickness, dtype=np.float32))
