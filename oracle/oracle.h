@@ -130,6 +130,33 @@ void or_assemble(uint32_t nv, uint32_t nf, uint32_t nh, const float* x, const do
 uint32_t or_pcg_contacts(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const uint32_t* contacts,
 	uint32_t ncontacts, const double* b, double* x, int precond, uint32_t max_iters, double rel_tol, double* out);
 
+/* PRP untangling path (oracle_prp.c).  Opaque mesh/contour state. */
+typedef struct or_prp_mesh or_prp_mesh;
+or_prp_mesh* or_prp_mesh_create(uint32_t nv, uint32_t ne, uint32_t nf, const float* x, const float* x_rest,
+	const uint32_t* edges, const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces,
+	const float* vert_area, const float* edge_area, const float* face_area, const float* mass);
+void	 or_prp_mesh_destroy(or_prp_mesh* m);
+void	 or_prp_normals(const or_prp_mesh* m, float* fn, float* en, float* vn);
+void	 or_prp_distances(const or_prp_mesh* m, uint32_t nb, const uint32_t* bverts, uint32_t* hop, float* geo);
+int		 or_prp_owns(float own, float opp, float tau);
+int		 or_prp_owns_dual(float rest_own, float rest_opp, float cur_own, float cur_opp, float tau, int use_rest,
+			 int use_cur);
+uint32_t or_prp_directions(uint32_t np, const float* points, const float* weights, uint32_t nb0, const uint32_t* b0,
+	uint32_t nb1, const uint32_t* b1, const float* x, const float* vert_area, uint32_t nv, const double* global_c,
+	uint32_t total, float* out);
+void	 or_prp_prepare(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adjn,
+		 const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t total_dirs);
+uint32_t or_prp_eval_slot(or_prp_mesh* m, uint32_t c, const float* dir, uint32_t k_src, uint32_t k_dst,
+	void* out_hits, uint32_t cap, void* out_metrics);
+void	 or_prp_get_state(const or_prp_mesh* m, uint32_t c, float* dirs, uint32_t* ndirs, uint32_t* diam, uint32_t* nb);
+void	 or_prp_set_dirs(or_prp_mesh* m, uint32_t c, uint32_t n, const float* dirs);
+const uint32_t* or_prp_hop(const or_prp_mesh* m, uint32_t c, uint32_t role);
+const float*	or_prp_geo(const or_prp_mesh* m, uint32_t c, uint32_t role);
+int		 or_prp_select(uint32_t n, const float* objective, const uint32_t* covered, const uint32_t* ef_count,
+		 const float* depth, const float* cost, const uint32_t* hits, const uint32_t* eligible);
+uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa,
+	void* choices_out, uint32_t* recs, uint32_t cap);
+
 #ifdef __cplusplus
 }
 #endif

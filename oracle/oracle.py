@@ -73,6 +73,29 @@ def lib():
                                   p, p, C.c_uint32, p, C.c_uint32, p, p, p]
         L.or_pcg_contacts.argtypes = [C.c_uint32, p, p, p, p, C.c_uint32, p, p, C.c_int, C.c_uint32, C.c_double, p]
         L.or_pcg_contacts.restype = C.c_uint32
+        L.or_prp_mesh_create.argtypes = [C.c_uint32] * 3 + [p] * 10
+        L.or_prp_mesh_create.restype = C.c_void_p
+        L.or_prp_mesh_destroy.argtypes = [C.c_void_p]
+        L.or_prp_normals.argtypes = [C.c_void_p, p, p, p]
+        L.or_prp_distances.argtypes = [C.c_void_p, C.c_uint32, p, p, p]
+        L.or_prp_owns.argtypes = [C.c_float, C.c_float, C.c_float]
+        L.or_prp_owns_dual.argtypes = [C.c_float] * 5 + [C.c_int, C.c_int]
+        L.or_prp_directions.argtypes = [C.c_uint32, p, p, C.c_uint32, p, C.c_uint32, p, p, p, C.c_uint32, p,
+                                        C.c_uint32, p]
+        L.or_prp_directions.restype = C.c_uint32
+        L.or_prp_prepare.argtypes = [C.c_void_p, C.c_uint32, p, p, p, p, p, C.c_uint32, C.c_uint32]
+        L.or_prp_eval_slot.argtypes = [C.c_void_p, C.c_uint32, p, C.c_uint32, C.c_uint32, p, C.c_uint32, p]
+        L.or_prp_eval_slot.restype = C.c_uint32
+        L.or_prp_get_state.argtypes = [C.c_void_p, C.c_uint32, p, p, p, p]
+        L.or_prp_set_dirs.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, p]
+        L.or_prp_hop.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        L.or_prp_hop.restype = C.c_void_p
+        L.or_prp_geo.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+        L.or_prp_geo.restype = C.c_void_p
+        L.or_prp_select.argtypes = [C.c_uint32, p, p, p, p, p, p, p]
+        L.or_prp_hits_adjacent.argtypes = [C.c_void_p, p, p]
+        L.or_prp_resolve.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_float, p, p, C.c_uint32]
+        L.or_prp_resolve.restype = C.c_uint32
         L.or_line_intersection.argtypes = [p, p, p, p, p, p, p]
         L.or_line_intersection.restype = C.c_int
         L.or_ef_dcd.argtypes = [p, C.c_size_t, p, p, p, p, p, p, p, C.c_size_t]
@@ -372,3 +395,121 @@ def pcg_sell(pattern, vals, b, contacts=None, precond=0, max_iters=100, rel_tol=
     it = lib().or_pcg_contacts(n, _p(rp), _p(ci), _p(bl), _p(c), 0 if c is None else len(c),
                                _p(_c(b, np.float64)), _p(x), precond, max_iters, rel_tol, _p(out))
     return x, int(it), float(out[0]), bool(out[1]), int(out[2])
+
+
+PRP_HIT = np.dtype([("type", "u4"), ("src", "u4"), ("dst", "u4"), ("b0", "f4"), ("b1", "f4"), ("dist", "f4"),
+                    ("valid", "u4"), ("pad", "u4")])
+PRP_METRICS = np.dtype([("raw_hits", "u4"), ("hit_count", "u4"), ("covered", "u4"), ("ef_count", "u4"),
+                        ("depth", "f4"), ("volume", "f4"), ("objective", "f4"), ("area", "f4"), ("disp_cost", "f4"),
+                        ("max_hop_src", "u4"), ("max_hop_dst", "u4"), ("usable", "u4")])
+PRP_CHOICE = np.dtype([("combo", "i4"), ("ks", "u4"), ("kd", "u4"), ("hit_count", "u4"), ("covered", "u4"),
+                       ("ef_count", "u4"), ("levels", "u4"), ("suppressed", "u4"), ("dir", "f4", 3),
+                       ("depth", "f4"), ("objective", "f4"), ("area", "f4"), ("disp_cost", "f4")])
+HOP_INF = 0xFFFFFFFF
+
+
+def prp_owns(own, opp, tau=0.25):
+    return bool(lib().or_prp_owns(own, opp, tau))
+
+
+def prp_owns_dual(rest_own, rest_opp, cur_own, cur_opp, tau=0.25, use_rest=True, use_cur=True):
+    return bool(lib().or_prp_owns_dual(rest_own, rest_opp, cur_own, cur_opp, tau, int(use_rest), int(use_cur)))
+
+
+def prp_directions(points, weights, b0, b1, x, vert_area, global_c=None, total=10):
+    points = _c(points, np.float32).reshape(-1, 3)
+    out = np.zeros((64, 3), np.float32)
+    b0 = _c(b0, np.uint32)
+    b1 = _c(b1, np.uint32)
+    gc = None if global_c is None else _c(global_c, np.float64)
+    n = lib().or_prp_directions(points.shape[0], _p(points), _p(_c(weights, np.float32)), len(b0), _p(b0), len(b1),
+                                _p(b1), _p(_c(x, np.float32)), _p(_c(vert_area, np.float32)),
+                                int(np.asarray(x).size // 3), _p(gc), total, _p(out))
+    return out[:n]
+
+
+def prp_select(objective, covered, ef_count, depth, cost, hits, eligible):
+    n = len(objective)
+    return lib().or_prp_select(n, _p(_c(objective, np.float32)), _p(_c(covered, np.uint32)),
+                               _p(_c(ef_count, np.uint32)), _p(_c(depth, np.float32)), _p(_c(cost, np.float32)),
+                               _p(_c(hits, np.uint32)), _p(_c(eligible, np.uint32)))
+
+
+class PrpOracle:
+    """CPU restatement of the PRP path on one mesh (oracle_prp.c)."""
+
+    def __init__(self, x, x_rest, edges, faces, fae, eaf, vert_area, edge_area, face_area, mass):
+        self._keep = [_c(a, dt) for a, dt in ((x, np.float32), (x_rest, np.float32), (edges, np.uint32),
+                                              (faces, np.uint32), (fae, np.uint32), (eaf, np.uint32),
+                                              (vert_area, np.float32), (edge_area, np.float32),
+                                              (face_area, np.float32), (mass, np.float32))]
+        k = self._keep
+        self.nv, self.ne, self.nf = k[0].size // 3, k[2].size // 2, k[3].size // 3
+        self.h = lib().or_prp_mesh_create(self.nv, self.ne, self.nf, *[_p(a) for a in k])
+
+    def __del__(self):
+        try:
+            lib().or_prp_mesh_destroy(self.h)
+        except Exception:
+            pass
+
+    def normals(self):
+        fn = np.zeros((self.nf, 3), np.float32)
+        en = np.zeros((self.ne, 3), np.float32)
+        vn = np.zeros((self.nv, 3), np.float32)
+        lib().or_prp_normals(self.h, _p(fn), _p(en), _p(vn))
+        return fn, en, vn
+
+    def distances(self, bverts):
+        b = _c(bverts, np.uint32)
+        hop = np.zeros(self.nv, np.uint32)
+        geo = np.zeros(self.nv, np.float32)
+        lib().or_prp_distances(self.h, len(b), _p(b), _p(hop), _p(geo))
+        return hop, geo
+
+    def prepare(self, pairs, adj, adjn, contour_index, mesh_index, nc, total_dirs=10):
+        self._ct = [_c(pairs, np.uint32), _c(adj, np.uint32), _c(adjn, np.uint32), _c(contour_index, np.uint32),
+                    _c(mesh_index, np.uint32)]
+        self.nc = int(nc)
+        N = self._ct[0].size // 12
+        lib().or_prp_prepare(self.h, N, *[_p(a) for a in self._ct], self.nc, total_dirs)
+
+    def state(self, c):
+        dirs = np.zeros((64, 3), np.float32)
+        nd = np.zeros(1, np.uint32)
+        diam = np.zeros(2, np.uint32)
+        nb = np.zeros(2, np.uint32)
+        lib().or_prp_get_state(self.h, c, _p(dirs), _p(nd), _p(diam), _p(nb))
+        return dirs[:nd[0]].copy(), diam, nb
+
+    def set_dirs(self, c, dirs):
+        d = _c(dirs, np.float32).reshape(-1, 3)
+        lib().or_prp_set_dirs(self.h, c, d.shape[0], _p(d))
+
+    def hop(self, c, role):
+        return np.ctypeslib.as_array(C.cast(lib().or_prp_hop(self.h, c, role), C.POINTER(C.c_uint32)),
+                                     (self.nv,)).copy()
+
+    def geo(self, c, role):
+        return np.ctypeslib.as_array(C.cast(lib().or_prp_geo(self.h, c, role), C.POINTER(C.c_float)),
+                                     (self.nv,)).copy()
+
+    def eval_slot(self, c, direction, k_src, k_dst):
+        d = _c(direction, np.float32)
+        met = np.zeros(1, PRP_METRICS)
+        n = lib().or_prp_eval_slot(self.h, c, _p(d), k_src, k_dst, None, 0, _p(met))
+        hits = np.zeros(max(n, 1), PRP_HIT)
+        lib().or_prp_eval_slot(self.h, c, _p(d), k_src, k_dst, _p(hits), n, _p(met))
+        return hits[:n], met[0]
+
+    def hits_adjacent(self, a, b):
+        a = np.ascontiguousarray(np.asarray([a], PRP_HIT))
+        b = np.ascontiguousarray(np.asarray([b], PRP_HIT))
+        return bool(lib().or_prp_hits_adjacent(self.h, _p(a), _p(b)))
+
+    def resolve(self, max_contours=256, response_depth=1.0, kappa=1e8):
+        ch = np.zeros(max(self.nc, 1), PRP_CHOICE)
+        n = lib().or_prp_resolve(self.h, max_contours, response_depth, kappa, _p(ch), None, 0)
+        recs = np.zeros((max(n, 1), 16), np.uint32)
+        lib().or_prp_resolve(self.h, max_contours, response_depth, kappa, _p(ch), _p(recs), n)
+        return ch[:self.nc], recs[:n]

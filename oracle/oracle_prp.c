@@ -1,0 +1,1490 @@
+/*
+ * oracle_prp.c -- CPU restatement of the PRP (per-contour ray-casting
+ * untangling) path of the reference, default configuration only.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h).
+ *
+ * Follows Solver/CollisionDetector/intersection_resolver2.cpp:
+ *   normals                     prp_compute_host_normals            :2605-2644
+ *   boundary vertices / flags   prp_initialize_runtime_boundary_primitives :2456-2485,
+ *                               prp_build_vertex_boundary_flags     :2646-2680
+ *   hop + rest geodesic         calculate_hop_dist_and_geom_dist_in_mesh :618-720
+ *   directions                  get_raycasting_direction            :1488-1569
+ *   candidates at (k_src,k_dst) build_standard_candidates_at_level  :1571-1804
+ *                               (intrinsic ownership, intersection_resolver2.h:349-380)
+ *   ray casting                 cached_bvh.cpp:131-555 (leaf tests; the BVH only culls)
+ *   hit adjacency               find_adjacent_hits (intersection_resolver_helper.cpp:771-902)
+ *   cluster culling             :3506-3685 (is_hit_adj_contour :378-412)
+ *   J(r) and coverage           prp_fill_combo_eval_info / prp_eval_penetration_v2 :1806-1954
+ *   adaptive k-ring driver      evaluate_current_runtime_directions_template :1058-1230
+ *   comparator                  prp_response_is_better (intersection_resolver2.h:852-884)
+ *   contour suppression         :450-590
+ *   responses                   make_response_from_min_hitinfo      :2105-2221
+ *
+ * One object with one topology component (cloth); solid-interior hit links
+ * (rigid/tetrahedral only) and direction optimisation (0 iterations by
+ * default) are not part of this path.  The sums of the reference run in FP32
+ * in a thread-pool order; here they run in FP32 in hit order.  Contour
+ * centres, scatter matrices and eigenvectors are computed in FP64 (cyclic
+ * Jacobi) and rounded to FP32; the eigenvector sign is fixed so that the
+ * largest-magnitude component is positive (the reference takes Eigen's sign:
+ * parity unpinned for the sign, the direction set is sign-symmetric).
+ *
+ * Pinning (tests/test_oracle_prp.py): the ownership and comparator cases of
+ * UnitTest/test_prp_selection.cpp:74-290 (known answers); direction slots
+ * against numpy.linalg.eigh on the fixture of test_prp_selection.cpp:201-270;
+ * hop distances against the closed form on grids; rest geodesics against
+ * scipy.sparse.csgraph.dijkstra; ray hits against closed-form rays on
+ * axis-aligned sheets; components against scipy connected_components of the
+ * same adjacency; J(r) against its closed form.
+ */
+#include "oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NONE 0xFFFFFFFFu
+#define HOP_INF 0xFFFFFFFFu
+
+typedef struct
+{
+	float x, y, z;
+} f3;
+static f3	 mk(float x, float y, float z) { f3 r = { x, y, z }; return r; }
+static f3	 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static f3	 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static f3	 scl(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static f3	 crs(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static float len(f3 a) { return sqrtf(dot(a, a)); }
+static f3	 safe_norm(f3 v)
+{
+	const float l2 = dot(v, v);
+	return l2 > 1e-20f ? scl(v, 1.0f / sqrtf(l2)) : mk(0, 0, 0);
+}
+
+struct or_prp_mesh
+{
+	uint32_t		nv, ne, nf;
+	const float*	x;
+	const float*	xr;
+	const uint32_t* edges;
+	const uint32_t* faces;
+	const uint32_t* fae;
+	const uint32_t* eaf;
+	const float *	va, *ea, *fa, *mass;
+	/* CSR adjacency, neighbours in ascending order */
+	uint32_t *vv_off, *vv, *ve_off, *ve, *vf_off, *vf;
+	/* contour state */
+	uint32_t		N, nc;
+	const uint32_t* pairs; /* 12 words per pair */
+	const uint32_t* adj;
+	const uint32_t* adjn;
+	const uint32_t* cidx;
+	const uint32_t* midx;
+	uint32_t *		c_off, *c_pairs;	   /* pairs of contour c, ascending */
+	uint32_t *		b_off, *b_verts;	   /* boundary verts of (c, role) at 2c+role */
+	uint32_t *		flag_off, *flags;	   /* per vertex sorted boundary flags 2c+role */
+	uint32_t*		hop;				   /* [2c+role][V] */
+	float*			geo;				   /* [2c+role][V] */
+	uint32_t*		diameter;			   /* [2c+role] */
+	float*			dirs;				   /* [c][kMaxDirs][3] */
+	float *			fn, *en, *vn;		   /* normals at the prepared positions */
+	uint32_t*		ndirs;				   /* [c] */
+	float			global_centroid[3];
+};
+
+enum
+{
+	kMaxDirs = 64
+};
+
+static const uint32_t* pidx(const or_prp_mesh* m, uint32_t p) { return m->pairs + 12 * p; }
+static int			   is_loop(const or_prp_mesh* m, uint32_t p) { return pidx(m, p)[0] == pidx(m, p)[1]; }
+static f3			   X(const or_prp_mesh* m, uint32_t v) { return mk(m->x[3 * v], m->x[3 * v + 1], m->x[3 * v + 2]); }
+static f3			   XR(const or_prp_mesh* m, uint32_t v) { return mk(m->xr[3 * v], m->xr[3 * v + 1], m->xr[3 * v + 2]); }
+
+/* ---------------------------------------------------------------- CSR helpers */
+static int cmp_u32(const void* a, const void* b)
+{
+	const uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+	return x < y ? -1 : x > y;
+}
+
+static void build_csr(uint32_t n, uint32_t m, const uint32_t* src, const uint32_t* dst, uint32_t** off_out,
+	uint32_t** val_out)
+{
+	uint32_t* off = calloc(n + 1, 4);
+	uint32_t* val = malloc(sizeof(uint32_t) * (m ? m : 1));
+	for (uint32_t i = 0; i < m; ++i)
+		off[src[i] + 1]++;
+	for (uint32_t i = 0; i < n; ++i)
+		off[i + 1] += off[i];
+	uint32_t* fill = malloc(sizeof(uint32_t) * (n ? n : 1));
+	memcpy(fill, off, sizeof(uint32_t) * n);
+	for (uint32_t i = 0; i < m; ++i)
+		val[fill[src[i]]++] = dst[i];
+	for (uint32_t i = 0; i < n; ++i)
+		qsort(val + off[i], off[i + 1] - off[i], 4, cmp_u32);
+	free(fill);
+	*off_out = off;
+	*val_out = val;
+}
+
+or_prp_mesh* or_prp_mesh_create(uint32_t nv, uint32_t ne, uint32_t nf, const float* x, const float* x_rest,
+	const uint32_t* edges, const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces,
+	const float* vert_area, const float* edge_area, const float* face_area, const float* mass)
+{
+	or_prp_mesh* m = calloc(1, sizeof(or_prp_mesh));
+	m->nv = nv, m->ne = ne, m->nf = nf;
+	m->x = x, m->xr = x_rest, m->edges = edges, m->faces = faces, m->fae = face_adj_edges, m->eaf = edge_adj_faces;
+	m->va = vert_area, m->ea = edge_area, m->fa = face_area, m->mass = mass;
+	uint32_t* s = calloc(6 * (size_t)ne + 3 * (size_t)nf + 1, sizeof(uint32_t));
+	uint32_t* d = calloc(6 * (size_t)ne + 3 * (size_t)nf + 1, sizeof(uint32_t));
+	for (uint32_t e = 0; e < ne; ++e)
+	{
+		s[2 * e] = edges[2 * e], d[2 * e] = edges[2 * e + 1];
+		s[2 * e + 1] = edges[2 * e + 1], d[2 * e + 1] = edges[2 * e];
+	}
+	build_csr(nv, 2 * ne, s, d, &m->vv_off, &m->vv);
+	for (uint32_t e = 0; e < ne; ++e)
+	{
+		s[2 * e] = edges[2 * e], d[2 * e] = e;
+		s[2 * e + 1] = edges[2 * e + 1], d[2 * e + 1] = e;
+	}
+	build_csr(nv, 2 * ne, s, d, &m->ve_off, &m->ve);
+	for (uint32_t f = 0; f < nf; ++f)
+		for (int k = 0; k < 3; ++k)
+			s[3 * f + k] = faces[3 * f + k], d[3 * f + k] = f;
+	build_csr(nv, 3 * nf, s, d, &m->vf_off, &m->vf);
+	free(s);
+	free(d);
+	return m;
+}
+
+static void free_contours(or_prp_mesh* m)
+{
+	free(m->c_off), free(m->c_pairs), free(m->b_off), free(m->b_verts), free(m->flag_off), free(m->flags);
+	free(m->hop), free(m->geo), free(m->diameter), free(m->dirs), free(m->ndirs);
+	free(m->fn), free(m->en), free(m->vn);
+	m->fn = m->en = m->vn = NULL;
+	m->c_off = m->c_pairs = m->b_off = m->b_verts = m->flag_off = m->flags = m->hop = m->diameter = m->ndirs = NULL;
+	m->geo = m->dirs = NULL;
+	m->nc = 0;
+}
+
+void or_prp_mesh_destroy(or_prp_mesh* m)
+{
+	if (!m)
+		return;
+	free_contours(m);
+	free(m->vv_off), free(m->vv), free(m->ve_off), free(m->ve), free(m->vf_off), free(m->vf);
+	free(m);
+}
+
+static int contains(const uint32_t* a, uint32_t n, uint32_t v)
+{
+	uint32_t lo = 0, hi = n;
+	while (lo < hi)
+	{
+		const uint32_t mid = (lo + hi) / 2;
+		if (a[mid] < v)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo < n && a[lo] == v;
+}
+
+/* ---------------------------------------------------------------- normals (:2605-2644) */
+void or_prp_normals(const or_prp_mesh* m, float* fn, float* en, float* vn)
+{
+	for (uint32_t f = 0; f < m->nf; ++f)
+	{
+		const uint32_t* t = m->faces + 3 * f;
+		const f3		n = safe_norm(crs(sub(X(m, t[1]), X(m, t[0])), sub(X(m, t[2]), X(m, t[0]))));
+		fn[3 * f] = n.x, fn[3 * f + 1] = n.y, fn[3 * f + 2] = n.z;
+	}
+	for (uint32_t e = 0; e < m->ne; ++e)
+	{
+		f3 s = mk(0, 0, 0);
+		for (int k = 0; k < 2; ++k)
+		{
+			const uint32_t f = m->eaf[2 * e + k];
+			if (f != NONE)
+				s = add(s, scl(mk(fn[3 * f], fn[3 * f + 1], fn[3 * f + 2]), m->fa[f]));
+		}
+		const f3 n = safe_norm(s);
+		en[3 * e] = n.x, en[3 * e + 1] = n.y, en[3 * e + 2] = n.z;
+	}
+	for (uint32_t v = 0; v < m->nv; ++v)
+	{
+		f3 s = mk(0, 0, 0);
+		for (uint32_t i = m->vf_off[v]; i < m->vf_off[v + 1]; ++i)
+		{
+			const uint32_t f = m->vf[i];
+			s = add(s, scl(mk(fn[3 * f], fn[3 * f + 1], fn[3 * f + 2]), m->fa[f]));
+		}
+		const f3 n = safe_norm(s);
+		vn[3 * v] = n.x, vn[3 * v + 1] = n.y, vn[3 * v + 2] = n.z;
+	}
+}
+
+/* ---------------------------------------------------------------- distances (:618-720) */
+typedef struct
+{
+	double	 key;
+	uint32_t v;
+} heap_item;
+
+static void heap_push(heap_item* h, uint32_t* n, heap_item it)
+{
+	uint32_t i = (*n)++;
+	h[i] = it;
+	while (i > 0)
+	{
+		const uint32_t p = (i - 1) / 2;
+		if (h[p].key < h[i].key || (h[p].key == h[i].key && h[p].v <= h[i].v))
+			break;
+		heap_item t = h[p];
+		h[p] = h[i], h[i] = t, i = p;
+	}
+}
+
+static heap_item heap_pop(heap_item* h, uint32_t* n)
+{
+	heap_item top = h[0];
+	h[0] = h[--(*n)];
+	uint32_t i = 0;
+	for (;;)
+	{
+		uint32_t l = 2 * i + 1, r = l + 1, b = i;
+		if (l < *n && (h[l].key < h[b].key || (h[l].key == h[b].key && h[l].v < h[b].v)))
+			b = l;
+		if (r < *n && (h[r].key < h[b].key || (h[r].key == h[b].key && h[r].v < h[b].v)))
+			b = r;
+		if (b == i)
+			break;
+		heap_item t = h[b];
+		h[b] = h[i], h[i] = t, i = b;
+	}
+	return top;
+}
+
+/* BFS hop distance and Dijkstra rest-geodesic distance (FP32 sums) from a vertex set. */
+void or_prp_distances(const or_prp_mesh* m, uint32_t nb, const uint32_t* bverts, uint32_t* hop, float* geo)
+{
+	const uint32_t V = m->nv;
+	for (uint32_t v = 0; v < V; ++v)
+		hop[v] = HOP_INF, geo[v] = FLT_MAX;
+	uint32_t* q = malloc(sizeof(uint32_t) * (V + 1));
+	uint32_t  qh = 0, qt = 0;
+	for (uint32_t i = 0; i < nb; ++i)
+		if (hop[bverts[i]] != 0)
+			hop[bverts[i]] = 0, q[qt++] = bverts[i];
+	while (qh < qt)
+	{
+		const uint32_t u = q[qh++];
+		for (uint32_t i = m->vv_off[u]; i < m->vv_off[u + 1]; ++i)
+			if (hop[m->vv[i]] == HOP_INF)
+				hop[m->vv[i]] = hop[u] + 1, q[qt++] = m->vv[i];
+	}
+	free(q);
+	heap_item* h = malloc(sizeof(heap_item) * (2 * (size_t)m->ne + nb + 1));
+	uint32_t   hn = 0;
+	for (uint32_t i = 0; i < nb; ++i)
+	{
+		geo[bverts[i]] = 0.0f;
+		heap_item it = { 0.0, bverts[i] };
+		heap_push(h, &hn, it);
+	}
+	while (hn)
+	{
+		const heap_item it = heap_pop(h, &hn);
+		const uint32_t	u = it.v;
+		if ((float)it.key > geo[u])
+			continue;
+		for (uint32_t i = m->vv_off[u]; i < m->vv_off[u + 1]; ++i)
+		{
+			const uint32_t w = m->vv[i];
+			const f3	   d = sub(XR(m, u), XR(m, w));
+			const float	   nd = geo[u] + len(d);
+			if (nd < geo[w])
+			{
+				geo[w] = nd;
+				heap_item ni = { nd, w };
+				heap_push(h, &hn, ni);
+			}
+		}
+	}
+	free(h);
+}
+
+/* ---------------------------------------------------------------- ownership (h:349-380) */
+static float normalized_phi(float own, float opp)
+{
+	const float unreachable = FLT_MAX * 0.5f;
+	const int	g0 = isfinite(own) && own < unreachable, g1 = isfinite(opp) && opp < unreachable;
+	if (!g0 && !g1)
+		return 0.0f;
+	if (!g0)
+		return 1.0f;
+	if (!g1)
+		return -1.0f;
+	return (own - opp) / (own + opp + 1e-6f);
+}
+
+int or_prp_owns(float own, float opp, float tau)
+{
+	const float unreachable = FLT_MAX * 0.5f;
+	if (!isfinite(own) || own >= unreachable)
+		return 0;
+	if (!isfinite(opp) || opp >= unreachable)
+		return 1;
+	return normalized_phi(own, opp) <= tau;
+}
+
+int or_prp_owns_dual(float rest_own, float rest_opp, float cur_own, float cur_opp, float tau, int use_rest, int use_cur)
+{
+	return (!use_rest || or_prp_owns(rest_own, rest_opp, tau)) && (!use_cur || or_prp_owns(cur_own, cur_opp, tau));
+}
+
+/* ---------------------------------------------------------------- directions (:1488-1569) */
+static void jacobi3(const double A_in[9], double evals[3], double evecs[9])
+{
+	double A[9], V[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };
+	memcpy(A, A_in, sizeof(A));
+	for (int sweep = 0; sweep < 64; ++sweep)
+	{
+		const double off = A[1] * A[1] + A[2] * A[2] + A[5] * A[5];
+		if (off < 1e-300)
+			break;
+		for (int p = 0; p < 2; ++p)
+			for (int q = p + 1; q < 3; ++q)
+			{
+				const double apq = A[3 * p + q];
+				if (fabs(apq) < 1e-300)
+					continue;
+				const double theta = (A[3 * q + q] - A[3 * p + p]) / (2.0 * apq);
+				const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+				const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+				for (int k = 0; k < 3; ++k)
+				{
+					const double akp = A[3 * k + p], akq = A[3 * k + q];
+					A[3 * k + p] = c * akp - s * akq;
+					A[3 * k + q] = s * akp + c * akq;
+				}
+				for (int k = 0; k < 3; ++k)
+				{
+					const double apk = A[3 * p + k], aqk = A[3 * q + k];
+					A[3 * p + k] = c * apk - s * aqk;
+					A[3 * q + k] = s * apk + c * aqk;
+				}
+				for (int k = 0; k < 3; ++k)
+				{
+					const double vkp = V[3 * k + p], vkq = V[3 * k + q];
+					V[3 * k + p] = c * vkp - s * vkq;
+					V[3 * k + q] = s * vkp + c * vkq;
+				}
+			}
+	}
+	int idx[3] = { 0, 1, 2 };
+	for (int i = 0; i < 3; ++i)
+		for (int j = i + 1; j < 3; ++j)
+			if (A[4 * idx[j]] < A[4 * idx[i]])
+			{
+				int t = idx[i];
+				idx[i] = idx[j], idx[j] = t;
+			}
+	for (int i = 0; i < 3; ++i)
+	{
+		evals[i] = A[4 * idx[i]];
+		double v[3] = { V[idx[i]], V[3 + idx[i]], V[6 + idx[i]] };
+		int	   big = 0;
+		for (int k = 1; k < 3; ++k)
+			if (fabs(v[k]) > fabs(v[big]))
+				big = k;
+		const double sg = v[big] < 0 ? -1.0 : 1.0;
+		for (int k = 0; k < 3; ++k)
+			evecs[3 * i + k] = sg * v[k];
+	}
+}
+
+static int valid_dir(const double* d)
+{
+	return isfinite(d[0]) && isfinite(d[1]) && isfinite(d[2]) && (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) > 1e-12;
+}
+
+static void push_dir(float* out, uint32_t* n, const double* d, double sign)
+{
+	if (!valid_dir(d) || *n >= kMaxDirs)
+		return;
+	const double l = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+	for (int k = 0; k < 3; ++k)
+		out[3 * *n + k] = (float)(sign * d[k] / l);
+	++*n;
+}
+
+/* Directions of one contour: points/weights = EF pair positions and lengths,
+ * boundary vertices per role, area weights of every vertex; extra > 0 appends
+ * Fibonacci-sphere directions up to `total` slots. */
+uint32_t or_prp_directions(uint32_t np, const float* points, const float* weights, uint32_t nb0, const uint32_t* b0,
+	uint32_t nb1, const uint32_t* b1, const float* x, const float* vert_area, uint32_t nv, const double* global_c,
+	uint32_t total, float* out)
+{
+	double c[3] = { 0, 0, 0 }, u[3] = { 0, 0, 0 }, sw = 0;
+	for (uint32_t i = 0; i < np; ++i)
+		for (int k = 0; k < 3; ++k)
+			c[k] += (double)points[3 * i + k] * weights[i], u[k] += points[3 * i + k];
+	for (uint32_t i = 0; i < np; ++i)
+		sw += weights[i];
+	for (int k = 0; k < 3; ++k)
+		c[k] = sw == 0.0 ? u[k] / np : c[k] / sw;
+	double			S[9] = { 0 }, mc[2][3] = { { 0 } };
+	const uint32_t* bl[2] = { b0, b1 };
+	const uint32_t	bn[2] = { nb0, nb1 };
+	for (int r = 0; r < 2; ++r)
+	{
+		double ws = 0;
+		for (uint32_t i = 0; i < bn[r]; ++i)
+		{
+			const uint32_t v = bl[r][i];
+			const double   w = fmax(vert_area[v], 1e-8);
+			double		   d[3];
+			for (int k = 0; k < 3; ++k)
+				mc[r][k] += (double)x[3 * v + k] * w, d[k] = x[3 * v + k] - c[k];
+			for (int a = 0; a < 3; ++a)
+				for (int b = 0; b < 3; ++b)
+					S[3 * a + b] += w * d[a] * d[b];
+			ws += w;
+		}
+		for (int k = 0; k < 3; ++k)
+			mc[r][k] = ws > 1e-8 ? mc[r][k] / ws : c[k];
+	}
+	double ev[3], E[9] = { 0, 1, 0, 1, 0, 0, 0, 0, 1 };
+	const double tr = S[0] + S[4] + S[8];
+	double		 fro = 0;
+	for (int k = 0; k < 9; ++k)
+		fro += S[k] * S[k];
+	int ok = 1;
+	for (int k = 0; k < 9; ++k)
+		ok &= isfinite(S[k]);
+	if (ok && tr >= 1e-10 && fro >= 1e-20)
+		jacobi3(S, ev, E);
+	uint32_t n = 0;
+	for (int i = 0; i < 3; ++i)
+	{
+		push_dir(out, &n, E + 3 * i, 1.0);
+		push_dir(out, &n, E + 3 * i, -1.0);
+	}
+	for (int r = 0; r < 2; ++r)
+	{
+		double d[3] = { c[0] - mc[r][0], c[1] - mc[r][1], c[2] - mc[r][2] };
+		if (!valid_dir(d))
+			memcpy(d, E + 3 * (r + 1), sizeof(d));
+		push_dir(out, &n, d, 1.0);
+	}
+	if (global_c)
+	{
+		const double d[3] = { c[0] - global_c[0], c[1] - global_c[1], c[2] - global_c[2] };
+		push_dir(out, &n, d, 1.0);
+		push_dir(out, &n, d, -1.0);
+	}
+	(void)nv;
+	/* extra directions: Fibonacci sphere */
+	for (uint32_t i = 0; n < total && n < kMaxDirs; ++i)
+	{
+		const uint32_t m = total > 10 ? total - 10 : 1;
+		const double   zz = 1.0 - (2.0 * i + 1.0) / m;
+		const double   rr = sqrt(fmax(0.0, 1.0 - zz * zz));
+		const double   ph = i * 2.399963229728653;
+		const double   d[3] = { rr * cos(ph), zz, rr * sin(ph) };
+		push_dir(out, &n, d, 1.0);
+		if (i > 4 * kMaxDirs)
+			break;
+	}
+	return n;
+}
+
+/* ---------------------------------------------------------------- contour preparation */
+/* EF pair position: edge point at the pair's edge barycentre. */
+static f3 pair_pos(const or_prp_mesh* m, uint32_t p)
+{
+	const uint32_t* q = pidx(m, p);
+	float			b0;
+	memcpy(&b0, q + 5, 4);
+	return add(scl(X(m, q[0]), b0), scl(X(m, q[1]), 1.0f - b0));
+}
+
+static int flag_has(const or_prp_mesh* m, uint32_t v, uint32_t flag)
+{
+	return contains(m->flags + m->flag_off[v], m->flag_off[v + 1] - m->flag_off[v], flag);
+}
+
+/* Per-contour lists, boundary vertices, vertex flags, distances, directions. */
+void or_prp_prepare(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adjn,
+	const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t total_dirs)
+{
+	free_contours(m);
+	m->N = N, m->nc = nc, m->pairs = pairs, m->adj = adj, m->adjn = adjn, m->cidx = contour_index, m->midx = mesh_index;
+	const uint32_t V = m->nv;
+	m->c_off = calloc(nc + 1, 4);
+	m->c_pairs = malloc(4 * (N + 1));
+	for (uint32_t p = 0; p < N; ++p)
+		m->c_off[contour_index[p] + 1]++;
+	for (uint32_t c = 0; c < nc; ++c)
+		m->c_off[c + 1] += m->c_off[c];
+	uint32_t* fill = malloc(4 * (nc + 1));
+	memcpy(fill, m->c_off, 4 * nc);
+	for (uint32_t p = 0; p < N; ++p)
+		m->c_pairs[fill[contour_index[p]]++] = p;
+	/* boundary verts per (c, role): edge verts to the pair's role, face verts to the other */
+	uint32_t* bs = malloc(4 * (5 * (size_t)N + 1));
+	uint32_t* bd = malloc(4 * (5 * (size_t)N + 1));
+	uint32_t  nbv = 0;
+	for (uint32_t p = 0; p < N; ++p)
+	{
+		const uint32_t* q = pidx(m, p);
+		const uint32_t	r = mesh_index[p], c = contour_index[p];
+		for (int k = 0; k < 5; ++k)
+			bs[nbv] = 2 * c + (k < 2 ? r : r ^ 1u), bd[nbv++] = q[k];
+	}
+	build_csr(2 * nc, nbv, bs, bd, &m->b_off, &m->b_verts);
+	/* dedupe rows in place */
+	{
+		uint32_t w = 0, start = 0;
+		for (uint32_t s = 0; s < 2 * nc; ++s)
+		{
+			const uint32_t end = m->b_off[s + 1];
+			const uint32_t row0 = w;
+			for (uint32_t i = start; i < end; ++i)
+				if (w == row0 || m->b_verts[w - 1] != m->b_verts[i])
+					m->b_verts[w++] = m->b_verts[i];
+			start = end;
+			m->b_off[s + 1] = w;
+			m->b_off[s] = row0;
+		}
+	}
+	/* vertex flags from non-loop pairs */
+	nbv = 0;
+	for (uint32_t p = 0; p < N; ++p)
+	{
+		if (is_loop(m, p))
+			continue;
+		const uint32_t* q = pidx(m, p);
+		const uint32_t	ef = 2 * contour_index[p] + mesh_index[p];
+		for (int k = 0; k < 5; ++k)
+			bs[nbv] = q[k], bd[nbv++] = k < 2 ? ef : ef ^ 1u;
+	}
+	build_csr(V, nbv, bs, bd, &m->flag_off, &m->flags);
+	{
+		uint32_t w = 0, start = 0;
+		for (uint32_t v = 0; v < V; ++v)
+		{
+			const uint32_t end = m->flag_off[v + 1], row0 = w;
+			for (uint32_t i = start; i < end; ++i)
+				if (w == row0 || m->flags[w - 1] != m->flags[i])
+					m->flags[w++] = m->flags[i];
+			start = end;
+			m->flag_off[v + 1] = w;
+			m->flag_off[v] = row0;
+		}
+	}
+	free(bs), free(bd), free(fill);
+	m->hop = malloc(sizeof(uint32_t) * 2 * (size_t)nc * V + 4);
+	m->geo = malloc(sizeof(float) * 2 * (size_t)nc * V + 4);
+	m->diameter = calloc(2 * nc + 1, 4);
+	for (uint32_t s = 0; s < 2 * nc; ++s)
+	{
+		uint32_t* hop = m->hop + (size_t)s * V;
+		or_prp_distances(m, m->b_off[s + 1] - m->b_off[s], m->b_verts + m->b_off[s], hop, m->geo + (size_t)s * V);
+		for (uint32_t v = 0; v < V; ++v)
+			if (hop[v] != HOP_INF && hop[v] > m->diameter[s])
+				m->diameter[s] = hop[v];
+	}
+	m->fn = malloc(sizeof(float) * 3 * ((size_t)m->nf + 1));
+	m->en = malloc(sizeof(float) * 3 * ((size_t)m->ne + 1));
+	m->vn = malloc(sizeof(float) * 3 * ((size_t)V + 1));
+	or_prp_normals(m, m->fn, m->en, m->vn);
+	/* global area-weighted centroid */
+	double g[3] = { 0, 0, 0 }, gw = 0;
+	for (uint32_t v = 0; v < V; ++v)
+	{
+		const double w = fmax(m->va[v], 1e-8);
+		for (int k = 0; k < 3; ++k)
+			g[k] += w * m->x[3 * v + k];
+		gw += w;
+	}
+	for (int k = 0; k < 3; ++k)
+		g[k] /= gw, m->global_centroid[k] = (float)g[k];
+	/* directions */
+	m->dirs = calloc((size_t)nc * kMaxDirs * 3 + 1, sizeof(float));
+	m->ndirs = calloc(nc + 1, 4);
+	for (uint32_t c = 0; c < nc; ++c)
+	{
+		const uint32_t n = m->c_off[c + 1] - m->c_off[c];
+		float*		   pts = malloc(sizeof(float) * 3 * (n + 1));
+		float*		   wts = malloc(sizeof(float) * (n + 1));
+		for (uint32_t i = 0; i < n; ++i)
+		{
+			const uint32_t p = m->c_pairs[m->c_off[c] + i];
+			const f3	   pp = pair_pos(m, p);
+			pts[3 * i] = pp.x, pts[3 * i + 1] = pp.y, pts[3 * i + 2] = pp.z;
+			float s = 0.0f;
+			for (uint32_t j = 0; j < adjn[p]; ++j)
+				s += len(sub(pair_pos(m, adj[12 * p + j]), pp));
+			wts[i] = 0.5f * s;
+		}
+		m->ndirs[c] = or_prp_directions(n, pts, wts, m->b_off[2 * c + 1] - m->b_off[2 * c], m->b_verts + m->b_off[2 * c],
+			m->b_off[2 * c + 2] - m->b_off[2 * c + 1], m->b_verts + m->b_off[2 * c + 1], m->x, m->va, V, g,
+			total_dirs, m->dirs + (size_t)c * kMaxDirs * 3);
+		free(pts), free(wts);
+	}
+}
+
+/* ---------------------------------------------------------------- one (contour, direction, k) evaluation */
+typedef struct
+{
+	uint32_t type, src, dst; /* 0 VF (vert, face), 1 EE (edge, edge), 2 FV (face, vert) */
+	float	 b0, b1, dist;
+	uint32_t valid, pad;
+} or_prp_hit;
+
+typedef struct
+{
+	uint32_t raw_hits, hit_count, covered, ef_count;
+	float	 depth, volume, objective, area, disp_cost;
+	uint32_t max_hop_src, max_hop_dst, usable;
+} or_prp_metrics;
+
+
+static uint32_t hit_src_verts(const or_prp_mesh* m, const or_prp_hit* h, uint32_t* out)
+{
+	if (h->type == 0)
+		return out[0] = h->src, 1;
+	if (h->type == 1)
+		return out[0] = m->edges[2 * h->src], out[1] = m->edges[2 * h->src + 1], 2;
+	for (int k = 0; k < 3; ++k)
+		out[k] = m->faces[3 * h->src + k];
+	return 3;
+}
+
+static uint32_t hit_dst_verts(const or_prp_mesh* m, const or_prp_hit* h, uint32_t* out)
+{
+	if (h->type == 0)
+	{
+		for (int k = 0; k < 3; ++k)
+			out[k] = m->faces[3 * h->dst + k];
+		return 3;
+	}
+	if (h->type == 1)
+		return out[0] = m->edges[2 * h->dst], out[1] = m->edges[2 * h->dst + 1], 2;
+	return out[0] = h->dst, 1;
+}
+
+static float cur_dist(const or_prp_mesh* m, uint32_t s, uint32_t v)
+{
+	float		   best = FLT_MAX;
+	const f3	   xv = X(m, v);
+	for (uint32_t i = m->b_off[s]; i < m->b_off[s + 1]; ++i)
+	{
+		const float d = len(sub(xv, X(m, m->b_verts[i])));
+		if (d < best)
+			best = d;
+	}
+	return best;
+}
+
+/* Candidate flags of (contour c, role r) at hop radius k (intersection_resolver2.cpp:1571-1804). */
+static void candidates(const or_prp_mesh* m, uint32_t c, uint32_t r, uint32_t k, uint8_t* cv, uint8_t* ce, uint8_t* cf)
+{
+	const uint32_t	V = m->nv, s = 2 * c + r, so = 2 * c + (r ^ 1u);
+	const uint32_t* hop = m->hop + (size_t)s * V;
+	uint8_t*		own = calloc(V + 1, 1);
+	for (uint32_t v = 0; v < V; ++v)
+	{
+		cv[v] = 0;
+		if (hop[v] > k)
+			continue;
+		const float ro = m->geo[(size_t)s * V + v], rp = m->geo[(size_t)so * V + v];
+		own[v] = (uint8_t)or_prp_owns_dual(ro, rp, cur_dist(m, s, v), cur_dist(m, so, v), 0.25f, 1, 1);
+		cv[v] = own[v];
+	}
+	for (uint32_t e = 0; e < m->ne; ++e)
+	{
+		const uint32_t a = m->edges[2 * e], b = m->edges[2 * e + 1];
+		ce[e] = hop[a] <= k && hop[b] <= k && (own[a] || own[b]);
+	}
+	for (uint32_t f = 0; f < m->nf; ++f)
+	{
+		const uint32_t* t = m->faces + 3 * f;
+		cf[f] = hop[t[0]] <= k && hop[t[1]] <= k && hop[t[2]] <= k && (own[t[0]] || own[t[1]] || own[t[2]]);
+	}
+	free(own);
+}
+
+static void push_hit(or_prp_hit** hits, uint32_t* n, uint32_t* cap, or_prp_hit h)
+{
+	if (*n == *cap)
+	{
+		*cap = *cap ? 2 * *cap : 1024;
+		*hits = realloc(*hits, sizeof(or_prp_hit) * *cap);
+	}
+	(*hits)[(*n)++] = h;
+}
+
+/* Segment-triangle test of the EF pass (oracle_geom.c). */
+int or_line_intersection(const float* s, const float* e, const float* a, const float* b, const float* c, float* bary,
+	float* t);
+
+static int seg_tri(f3 o, f3 e, f3 A, f3 B, f3 C, float* bary, float* t)
+{
+	const float s[3] = { o.x, o.y, o.z }, en[3] = { e.x, e.y, e.z }, a[3] = { A.x, A.y, A.z }, b[3] = { B.x, B.y, B.z },
+				cc[3] = { C.x, C.y, C.z };
+	return or_line_intersection(s, en, a, b, cc, bary, t);
+}
+
+/* VF / EE / FV rays of one contour and direction (cached_bvh.cpp:131-555,
+ * request packing intersection_resolver2.cpp:3066-3124). */
+static uint32_t raycast(const or_prp_mesh* m, f3 dir, const uint8_t* cv0, const uint8_t* ce0, const uint8_t* cf0,
+	const uint8_t* cv1, const uint8_t* ce1, const uint8_t* cf1, or_prp_hit** hits, uint32_t* cap)
+{
+	uint32_t n = 0;
+	int		 any_f1 = 0, any_e1 = 0, any_f0 = 0;
+	for (uint32_t f = 0; f < m->nf; ++f)
+		any_f1 |= cf1[f], any_f0 |= cf0[f];
+	float pmin = FLT_MAX, pmax = -FLT_MAX;
+	for (uint32_t e = 0; e < m->ne; ++e)
+		if (ce1[e])
+		{
+			any_e1 = 1;
+			for (int k = 0; k < 2; ++k)
+			{
+				const float pr = dot(X(m, m->edges[2 * e + k]), dir);
+				pmin = pr < pmin ? pr : pmin;
+				pmax = pr > pmax ? pr : pmax;
+			}
+		}
+	if (!any_e1)
+		pmin = pmax = 0.0f;
+	const float dl2 = dot(dir, dir);
+	/* VF: role-0 vertices along +dir against role-1 faces */
+	if (any_f1)
+		for (uint32_t v = 0; v < m->nv; ++v)
+		{
+			if (!cv0[v])
+				continue;
+			const f3 o = X(m, v), en = add(o, scl(dir, 1.0f));
+			for (uint32_t f = 0; f < m->nf; ++f)
+			{
+				const uint32_t* t = m->faces + 3 * f;
+				if (!cf1[f] || t[0] == v || t[1] == v || t[2] == v)
+					continue;
+				float b[3], tt;
+				if (seg_tri(o, en, X(m, t[0]), X(m, t[1]), X(m, t[2]), b, &tt) && tt >= 0.0f && tt <= 1.0f)
+				{
+					or_prp_hit h = { 0, v, f, b[0], b[1], tt * 1.0f, 0, 0 };
+					push_hit(hits, &n, cap, h);
+				}
+			}
+		}
+	/* EE: role-0 edges swept along +dir against role-1 edges */
+	if (any_e1)
+		for (uint32_t e = 0; e < m->ne; ++e)
+		{
+			if (!ce0[e])
+				continue;
+			const uint32_t a0 = m->edges[2 * e], a1 = m->edges[2 * e + 1];
+			const f3	   p0 = X(m, a0), p1 = X(m, a1), e1 = sub(p1, p0);
+			float		   rmax = 1.0f;
+			if (dl2 > 1e-12f)
+			{
+				const float q0 = dot(p0, dir), q1 = dot(p1, dir);
+				const float smin = q0 < q1 ? q0 : q1;
+				rmax = (pmax - smin) / dl2 + 1e-5f;
+				rmax = rmax < 0.0f ? 0.0f : (rmax > 1.0f ? 1.0f : rmax);
+			}
+			if (rmax <= 0.0f)
+				continue;
+			f3			pn = crs(e1, dir);
+			const float pl = len(pn);
+			const int	use_plane = pl >= 1e-10f;
+			float		pd = 0.0f;
+			if (use_plane)
+			{
+				pn = scl(pn, 1.0f / pl);
+				pd = dot(pn, p0);
+			}
+			for (uint32_t g = 0; g < m->ne; ++g)
+			{
+				if (!ce1[g])
+					continue;
+				const uint32_t b0 = m->edges[2 * g], b1 = m->edges[2 * g + 1];
+				if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1)
+					continue;
+				const f3 q0 = X(m, b0), q1 = X(m, b1);
+				if (use_plane)
+				{
+					const float d0 = dot(pn, q0) - pd, d1 = dot(pn, q1) - pd;
+					if ((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f))
+						continue;
+				}
+				const f3	e2 = sub(q1, q0), r = sub(q0, p0), nn = crs(e1, e2);
+				const float det = -dot(dir, nn);
+				if (fabsf(det) < 1e-10f)
+					continue;
+				const float inv = 1.0f / det;
+				const float s_ = -dot(r, crs(e1, e2)) * inv;
+				const float t1 = -dot(dir, crs(r, e2)) * inv;
+				const float t2 = dot(dir, crs(e1, r)) * inv;
+				if (s_ >= 0.0f && s_ <= rmax && t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f)
+				{
+					or_prp_hit h = { 1, e, g, 1.0f - t1, 1.0f - t2, s_, 0, 0 };
+					push_hit(hits, &n, cap, h);
+				}
+			}
+		}
+	/* FV: role-1 vertices along -dir against role-0 faces */
+	if (any_f0)
+		for (uint32_t v = 0; v < m->nv; ++v)
+		{
+			if (!cv1[v])
+				continue;
+			const f3 o = X(m, v), en = add(o, scl(scl(dir, -1.0f), 1.0f));
+			for (uint32_t f = 0; f < m->nf; ++f)
+			{
+				const uint32_t* t = m->faces + 3 * f;
+				if (!cf0[f] || t[0] == v || t[1] == v || t[2] == v)
+					continue;
+				float b[3], tt;
+				if (seg_tri(o, en, X(m, t[0]), X(m, t[1]), X(m, t[2]), b, &tt) && tt >= 0.0f && tt <= 1.0f)
+				{
+					or_prp_hit h = { 2, f, v, b[0], b[1], tt * 1.0f, 0, 0 };
+					push_hit(hits, &n, cap, h);
+				}
+			}
+		}
+	return n;
+}
+
+/* find_adjacent_hits predicates (intersection_resolver_helper.cpp:771-902) */
+static int face_has_vert(const or_prp_mesh* m, uint32_t f, uint32_t v)
+{
+	const uint32_t* t = m->faces + 3 * f;
+	return t[0] == v || t[1] == v || t[2] == v;
+}
+static int face_has_edge(const or_prp_mesh* m, uint32_t f, uint32_t e)
+{
+	return m->eaf[2 * e] == f || m->eaf[2 * e + 1] == f;
+}
+static int edge_has_vert(const or_prp_mesh* m, uint32_t e, uint32_t v)
+{
+	return m->edges[2 * e] == v || m->edges[2 * e + 1] == v;
+}
+static int verts_adjacent(const or_prp_mesh* m, uint32_t a, uint32_t b)
+{
+	return contains(m->vv + m->vv_off[a], m->vv_off[a + 1] - m->vv_off[a], b);
+}
+/* e2 is one of the other edges of a face adjacent to e1 */
+static int edges_adjacent(const or_prp_mesh* m, uint32_t e1, uint32_t e2)
+{
+	if (e1 == e2)
+		return 0;
+	for (int k = 0; k < 2; ++k)
+	{
+		const uint32_t f = m->eaf[2 * e1 + k];
+		if (f == NONE)
+			continue;
+		for (int j = 0; j < 3; ++j)
+			if (m->fae[3 * f + j] == e2)
+				return 1;
+	}
+	return 0;
+}
+
+int or_prp_hits_adjacent(const or_prp_mesh* m, const or_prp_hit* L, const or_prp_hit* R)
+{
+	if (L->type == 0) /* VF(v, f) */
+	{
+		const uint32_t v = L->src, f = L->dst;
+		if (R->type == 0)
+			return R->dst == f && verts_adjacent(m, v, R->src);
+		if (R->type == 2) /* FV(f', v'): f' adjacent to v, v' in f */
+			return face_has_vert(m, R->src, v) && face_has_vert(m, f, R->dst);
+		/* EE(e1, e2): e1 adjacent to v, e2 an edge of f */
+		return edge_has_vert(m, R->src, v) && face_has_edge(m, f, R->dst);
+	}
+	if (L->type == 2) /* FV(f, v) */
+	{
+		const uint32_t f = L->src, v = L->dst;
+		if (R->type == 0) /* VF(v', f'): v' in f, f' contains v */
+			return face_has_vert(m, f, R->src) && face_has_vert(m, R->dst, v);
+		if (R->type == 2)
+			return R->src == f && verts_adjacent(m, v, R->dst);
+		return face_has_edge(m, f, R->src) && edge_has_vert(m, R->dst, v);
+	}
+	/* EE(e1, e2) */
+	const uint32_t e1 = L->src, e2 = L->dst;
+	if (R->type == 0) /* VF(v', f'): v' in e1, f' contains e2 */
+		return edge_has_vert(m, e1, R->src) && face_has_edge(m, R->dst, e2);
+	if (R->type == 2) /* FV(f', v'): f' adjacent to e1, v' in e2 */
+		return face_has_edge(m, R->src, e1) && edge_has_vert(m, e2, R->dst);
+	return (R->src == e1 && edges_adjacent(m, e2, R->dst)) || (R->dst == e2 && edges_adjacent(m, e1, R->src));
+}
+
+static uint32_t uf_find(uint32_t* p, uint32_t x)
+{
+	while (p[x] != x)
+		p[x] = p[p[x]], x = p[x];
+	return x;
+}
+
+static int hit_adj_contour(const or_prp_mesh* m, uint32_t c, const or_prp_hit* h)
+{
+	uint32_t sv[3], dv[3];
+	const uint32_t ns = hit_src_verts(m, h, sv), nd = hit_dst_verts(m, h, dv);
+	for (uint32_t i = 0; i < ns; ++i)
+		if (!flag_has(m, sv[i], 2 * c))
+			return 0;
+	for (uint32_t i = 0; i < nd; ++i)
+		if (!flag_has(m, dv[i], 2 * c + 1))
+			return 0;
+	return 1;
+}
+
+/* covered EF pairs of contour c by the support of hits[idx[0..n)] */
+static uint32_t coverage(const or_prp_mesh* m, uint32_t c, const or_prp_hit* hits, const uint32_t* idx, uint32_t n,
+	uint8_t* src_set, uint8_t* dst_set)
+{
+	uint32_t sv[3], dv[3];
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		const or_prp_hit* h = hits + idx[i];
+		const uint32_t	  ns = hit_src_verts(m, h, sv), nd = hit_dst_verts(m, h, dv);
+		for (uint32_t k = 0; k < ns; ++k)
+			src_set[sv[k]] = 1;
+		for (uint32_t k = 0; k < nd; ++k)
+			dst_set[dv[k]] = 1;
+	}
+	uint32_t cov = 0;
+	for (uint32_t i = m->c_off[c]; i < m->c_off[c + 1]; ++i)
+	{
+		const uint32_t	p = m->c_pairs[i];
+		const uint32_t* q = pidx(m, p);
+		/* role 0 = sources, role 1 = targets */
+		const uint8_t* es = m->midx[p] == 0 ? src_set : dst_set;
+		const uint8_t* fs = m->midx[p] == 0 ? dst_set : src_set;
+		if ((es[q[0]] || es[q[1]]) && (fs[q[2]] || fs[q[3]] || fs[q[4]]))
+			++cov;
+	}
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		const or_prp_hit* h = hits + idx[i];
+		const uint32_t	  ns = hit_src_verts(m, h, sv), nd = hit_dst_verts(m, h, dv);
+		for (uint32_t k = 0; k < ns; ++k)
+			src_set[sv[k]] = 0;
+		for (uint32_t k = 0; k < nd; ++k)
+			dst_set[dv[k]] = 0;
+	}
+	return cov;
+}
+
+static uint64_t hit_key(const or_prp_hit* h)
+{
+	return ((uint64_t)h->type << 62) | ((uint64_t)(h->src & 0x3fffffffu) << 31) | (uint64_t)(h->dst & 0x7fffffffu);
+}
+
+static float inv_mass(const or_prp_mesh* m, const or_prp_hit* h)
+{
+#define IM(v) (1.0f / fmaxf(m->mass[v], 1e-8f))
+	if (h->type == 0)
+	{
+		const uint32_t* t = m->faces + 3 * h->dst;
+		const float		bz = 1.0f - h->b0 - h->b1;
+		return IM(h->src) + h->b0 * h->b0 * IM(t[0]) + h->b1 * h->b1 * IM(t[1]) + bz * bz * IM(t[2]);
+	}
+	if (h->type == 2)
+	{
+		const uint32_t* t = m->faces + 3 * h->src;
+		const float		bz = 1.0f - h->b0 - h->b1;
+		return h->b0 * h->b0 * IM(t[0]) + h->b1 * h->b1 * IM(t[1]) + bz * bz * IM(t[2]) + IM(h->dst);
+	}
+	const float x0 = h->b0, x1 = 1.0f - h->b0, y0 = h->b1, y1 = 1.0f - h->b1;
+	return x0 * x0 * IM(m->edges[2 * h->src]) + x1 * x1 * IM(m->edges[2 * h->src + 1])
+		+ y0 * y0 * IM(m->edges[2 * h->dst]) + y1 * y1 * IM(m->edges[2 * h->dst + 1]);
+#undef IM
+}
+
+/* prp_eval_penetration_v2 + coverage + frontier hops over the valid hits */
+static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit* hits, uint32_t n,
+	or_prp_metrics* out, uint8_t* src_set, uint8_t* dst_set)
+{
+	const uint32_t V = m->nv;
+	uint32_t*	   idx = malloc(sizeof(uint32_t) * (n + 1));
+	uint32_t	   nv = 0;
+	float		   vol = 0, depth = 0, area = 0, cost = 0;
+	uint32_t	   hs = 0, hd = 0;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		const or_prp_hit* h = hits + i;
+		if (!h->valid)
+			continue;
+		idx[nv++] = i;
+		float		   a, al;
+		const float*   nn;
+		if (h->type == 0)
+			a = m->va[h->src], nn = m->vn + 3 * h->src;
+		else if (h->type == 1)
+			a = m->ea[h->src], nn = m->en + 3 * h->src;
+		else
+			a = m->fa[h->src], nn = m->fn + 3 * h->src;
+		al = fabsf(dot(mk(nn[0], nn[1], nn[2]), dir));
+		vol += a * al * h->dist;
+		area += a;
+		depth = h->dist > depth ? h->dist : depth;
+		const float sh = h->dist + 3e-3f, im = inv_mass(m, h);
+		cost += im > 1e-8f ? sh * sh / im : 0.0f;
+		uint32_t sv[3], dv[3];
+		const uint32_t ns = hit_src_verts(m, h, sv), nd = hit_dst_verts(m, h, dv);
+		for (uint32_t k = 0; k < ns; ++k)
+		{
+			const uint32_t hh = m->hop[(size_t)(2 * c) * V + sv[k]];
+			if (hh != HOP_INF && hh > hs)
+				hs = hh;
+		}
+		for (uint32_t k = 0; k < nd; ++k)
+		{
+			const uint32_t hh = m->hop[(size_t)(2 * c + 1) * V + dv[k]];
+			if (hh != HOP_INF && hh > hd)
+				hd = hh;
+		}
+	}
+	out->hit_count = nv;
+	out->depth = depth;
+	out->area = area;
+	out->objective = area * depth * depth;
+	out->volume = vol;
+	out->disp_cost = cost;
+	out->max_hop_src = hs;
+	out->max_hop_dst = hd;
+	out->ef_count = m->c_off[c + 1] - m->c_off[c];
+	out->covered = nv ? coverage(m, c, hits, idx, nv, src_set, dst_set) : 0;
+	const float cov = out->ef_count ? (float)out->covered / (float)out->ef_count : 0.0f;
+	out->usable = nv > 0 && out->covered > 0 && isfinite(out->objective) && isfinite(cov) && cov > 0.0f;
+	free(idx);
+}
+
+/* Evaluate contour c along dir at (k_src, k_dst): raw hits (all of them,
+ * with .valid set on the kept cluster) and the metrics of the kept hits. */
+static uint32_t eval_slot(const or_prp_mesh* m, uint32_t c, f3 dir, uint32_t ks, uint32_t kd, or_prp_hit** hits,
+	uint32_t* cap, or_prp_metrics* met)
+{
+	const uint32_t V = m->nv, E = m->ne, F = m->nf;
+	uint8_t*	   buf = calloc(2 * ((size_t)V + E + F) + 2 * (size_t)V + 8, 1);
+	uint8_t *	   cv0 = buf, *ce0 = cv0 + V, *cf0 = ce0 + E, *cv1 = cf0 + F, *ce1 = cv1 + V, *cf1 = ce1 + E;
+	uint8_t *	   src_set = cf1 + F, *dst_set = src_set + V;
+	candidates(m, c, 0, ks, cv0, ce0, cf0);
+	candidates(m, c, 1, kd, cv1, ce1, cf1);
+	const uint32_t n = raycast(m, dir, cv0, ce0, cf0, cv1, ce1, cf1, hits, cap);
+	or_prp_hit*	   H = *hits;
+	/* components of the hit adjacency graph (all-pairs test: the bucketed
+	 * search of the reference visits exactly the pairs the predicates accept) */
+	uint32_t* par = malloc(sizeof(uint32_t) * (n + 1));
+	for (uint32_t i = 0; i < n; ++i)
+		par[i] = i, H[i].valid = 0;
+	for (uint32_t i = 0; i < n; ++i)
+		for (uint32_t j = i + 1; j < n; ++j)
+			if (or_prp_hits_adjacent(m, H + i, H + j))
+			{
+				const uint32_t a = uf_find(par, i), b = uf_find(par, j);
+				if (a != b)
+					par[a > b ? a : b] = a < b ? a : b;
+			}
+	uint8_t*  attached = calloc(n + 1, 1);
+	uint32_t* idx = malloc(sizeof(uint32_t) * (n + 1));
+	for (uint32_t i = 0; i < n; ++i)
+		if (hit_adj_contour(m, c, H + i))
+			attached[uf_find(par, i)] = 1;
+	int		 found = 0;
+	uint32_t best_root = 0, best_cov = 0, best_hits = 0;
+	uint64_t best_key = ~0ull;
+	for (uint32_t r = 0; r < n; ++r)
+	{
+		if (uf_find(par, r) != r || !attached[r])
+			continue;
+		uint32_t cnt = 0;
+		uint64_t key = ~0ull;
+		for (uint32_t i = 0; i < n; ++i)
+			if (uf_find(par, i) == r)
+			{
+				idx[cnt++] = i;
+				const uint64_t k = hit_key(H + i);
+				key = k < key ? k : key;
+			}
+		const uint32_t cov = coverage(m, c, H, idx, cnt, src_set, dst_set);
+		if (!found || cov > best_cov || (cov == best_cov && cnt > best_hits)
+			|| (cov == best_cov && cnt == best_hits && key < best_key))
+			found = 1, best_root = r, best_cov = cov, best_hits = cnt, best_key = key;
+	}
+	if (found)
+		for (uint32_t i = 0; i < n; ++i)
+			H[i].valid = uf_find(par, i) == best_root;
+	memset(met, 0, sizeof(*met));
+	eval_hits(m, c, dir, H, n, met, src_set, dst_set);
+	met->raw_hits = n;
+	free(par), free(attached), free(idx), free(buf);
+	return n;
+}
+
+/* Public single-slot entry: hits written to out (cap entries), returns raw count. */
+uint32_t or_prp_eval_slot(or_prp_mesh* m, uint32_t c, const float* dir, uint32_t k_src, uint32_t k_dst, void* out_hits,
+	uint32_t cap, void* out_metrics)
+{
+	or_prp_hit*	   hits = NULL;
+	uint32_t	   hcap = 0;
+	or_prp_metrics met;
+	const uint32_t n = eval_slot(m, c, mk(dir[0], dir[1], dir[2]), k_src, k_dst, &hits, &hcap, &met);
+	if (out_hits)
+		memcpy(out_hits, hits, sizeof(or_prp_hit) * (n < cap ? n : cap));
+	if (out_metrics)
+		memcpy(out_metrics, &met, sizeof(met));
+	free(hits);
+	return n;
+}
+
+void or_prp_get_state(const or_prp_mesh* m, uint32_t c, float* dirs, uint32_t* ndirs, uint32_t* diam, uint32_t* nb)
+{
+	if (dirs)
+		memcpy(dirs, m->dirs + (size_t)c * kMaxDirs * 3, sizeof(float) * 3 * m->ndirs[c]);
+	if (ndirs)
+		*ndirs = m->ndirs[c];
+	if (diam)
+		diam[0] = m->diameter[2 * c], diam[1] = m->diameter[2 * c + 1];
+	if (nb)
+		nb[0] = m->b_off[2 * c + 1] - m->b_off[2 * c], nb[1] = m->b_off[2 * c + 2] - m->b_off[2 * c + 1];
+}
+
+void or_prp_set_dirs(or_prp_mesh* m, uint32_t c, uint32_t n, const float* dirs)
+{
+	m->ndirs[c] = n < kMaxDirs ? n : kMaxDirs;
+	memcpy(m->dirs + (size_t)c * kMaxDirs * 3, dirs, sizeof(float) * 3 * m->ndirs[c]);
+}
+
+const uint32_t* or_prp_hop(const or_prp_mesh* m, uint32_t c, uint32_t role) { return m->hop + (size_t)(2 * c + role) * m->nv; }
+const float*	or_prp_geo(const or_prp_mesh* m, uint32_t c, uint32_t role) { return m->geo + (size_t)(2 * c + role) * m->nv; }
+
+/* ---------------------------------------------------------------- comparator (h:830-912) */
+typedef struct
+{
+	uint32_t combo, hit_count, covered, ef_count, eligible, usable, present, converged, pruned, ks, kd, hs, hd;
+	float	 depth, objective, area, disp_cost, volume;
+} combo_result;
+
+static float response_objective(const combo_result* r)
+{
+	const float cov = r->ef_count ? (float)r->covered / (float)r->ef_count : 0.0f;
+	if (!isfinite(r->objective) || !isfinite(cov) || cov <= 0.0f)
+		return INFINITY;
+	const double o = (double)r->objective / ((double)cov * cov);
+	return isfinite(o) && o <= (double)FLT_MAX ? (float)o : INFINITY;
+}
+
+static int scalar_cmp(float l, float r)
+{
+	const int lf = isfinite(l), rf = isfinite(r);
+	if (lf != rf)
+		return lf ? -1 : 1;
+	if (!lf)
+		return 0;
+	const float tol = 1e-7f + 1e-5f * fmaxf(fabsf(l), fabsf(r));
+	if (l + tol < r)
+		return -1;
+	if (r + tol < l)
+		return 1;
+	return 0;
+}
+
+static int better(const combo_result* a, const combo_result* b)
+{
+	int c = scalar_cmp(response_objective(a), response_objective(b));
+	if (c)
+		return c < 0;
+	if ((c = scalar_cmp(a->depth, b->depth)))
+		return c < 0;
+	if ((c = scalar_cmp(a->disp_cost, b->disp_cost)))
+		return c < 0;
+	if (a->hit_count != b->hit_count)
+		return a->hit_count < b->hit_count;
+	return a->combo < b->combo;
+}
+
+/* Known-answer access to the comparator: objective, coverage (as covered/ef),
+ * depth, cost, hits, combo per candidate; returns the selected index or -1. */
+int or_prp_select(uint32_t n, const float* objective, const uint32_t* covered, const uint32_t* ef_count,
+	const float* depth, const float* cost, const uint32_t* hits, const uint32_t* eligible)
+{
+	int			 sel = -1;
+	combo_result best;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		combo_result r;
+		memset(&r, 0, sizeof(r));
+		r.combo = i, r.objective = objective[i], r.covered = covered[i], r.ef_count = ef_count[i];
+		r.depth = depth[i], r.disp_cost = cost[i], r.hit_count = hits[i], r.eligible = eligible[i];
+		const float cov = r.ef_count ? (float)r.covered / (float)r.ef_count : 0.0f;
+		r.usable = r.hit_count > 0 && r.covered > 0 && isfinite(r.objective) && cov > 0.0f;
+		if (!(r.eligible && r.usable))
+			continue;
+		if (sel < 0 || better(&r, &best))
+			sel = (int)i, best = r;
+	}
+	return sel;
+}
+
+/* ---------------------------------------------------------------- driver + responses */
+typedef struct
+{
+	int32_t	 combo;			/* selected direction slot, -1 when no response */
+	uint32_t ks, kd;		/* k-ring radii it was evaluated at */
+	uint32_t hit_count, covered, ef_count, levels, suppressed;
+	float	 dir[3], depth, objective, area, disp_cost;
+} or_prp_choice;
+
+static uint32_t initial_k(uint32_t nb, uint32_t diameter)
+{
+	if (diameter == 0)
+		return 3;
+	const uint32_t ka = (uint32_t)ceilf(sqrtf((float)nb));
+	const uint32_t m = ka < diameter ? ka : diameter;
+	return m > 3 ? m : 3;
+}
+
+static uint32_t grow(uint32_t cur, uint32_t diam)
+{
+	if (cur >= diam)
+		return cur;
+	uint64_t n = (uint64_t)cur * 2;
+	if (n < 1)
+		n = 1;
+	return (uint32_t)(n < diam ? n : diam);
+}
+
+static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
+{
+	const uint32_t nd = m->ndirs[c];
+	const uint32_t Ds = m->diameter[2 * c], Dd = m->diameter[2 * c + 1];
+	uint32_t	   ks = initial_k(m->b_off[2 * c + 1] - m->b_off[2 * c], Ds);
+	uint32_t	   kd = initial_k(m->b_off[2 * c + 2] - m->b_off[2 * c + 1], Dd);
+	combo_result*  R = calloc(nd + 1, sizeof(combo_result));
+	or_prp_hit*	   hits = NULL;
+	uint32_t	   cap = 0, level = 0;
+	memset(out, 0, sizeof(*out));
+	out->combo = -1;
+	for (;; ++level)
+	{
+		for (uint32_t ci = 0; ci < nd; ++ci)
+		{
+			combo_result* r = R + ci;
+			if (r->converged || r->pruned)
+				continue;
+			const float*   d = m->dirs + ((size_t)c * kMaxDirs + ci) * 3;
+			or_prp_metrics met;
+			eval_slot(m, c, mk(d[0], d[1], d[2]), ks, kd, &hits, &cap, &met);
+			memset(r, 0, sizeof(*r));
+			r->combo = ci, r->hit_count = met.hit_count, r->covered = met.covered, r->ef_count = met.ef_count;
+			r->depth = met.depth, r->objective = met.objective, r->area = met.area, r->disp_cost = met.disp_cost;
+			r->volume = met.volume, r->usable = met.usable, r->ks = ks, r->kd = kd, r->present = 1;
+			const int full = ks >= Ds && kd >= Dd;
+			const int src_ok = met.max_hop_src < ks || ks >= Ds;
+			const int dst_ok = met.max_hop_dst < kd || kd >= Dd;
+			r->eligible = met.usable && src_ok && dst_ok;
+			r->converged = r->eligible || full;
+			r->pruned = 0;
+			r->hs = met.max_hop_src, r->hd = met.max_hop_dst;
+		}
+		int	  has = 0;
+		float best_obj = INFINITY;
+		for (uint32_t ci = 0; ci < nd; ++ci)
+			if (R[ci].eligible && R[ci].usable)
+			{
+				has = 1;
+				const float o = response_objective(R + ci);
+				best_obj = o < best_obj ? o : best_obj;
+			}
+		if (has)
+			for (uint32_t ci = 0; ci < nd; ++ci)
+				if (!R[ci].converged && R[ci].present && response_objective(R + ci) >= best_obj)
+					R[ci].pruned = 1;
+		int pending = 0, gs = 0, gd = 0;
+		for (uint32_t ci = 0; ci < nd; ++ci)
+		{
+			const combo_result* r = R + ci;
+			if (r->converged || r->pruned)
+				continue;
+			pending = 1;
+			gs |= ks < Ds && (!r->usable || r->hs >= ks);
+			gd |= kd < Dd && (!r->usable || r->hd >= kd);
+		}
+		if (!pending)
+			break;
+		const uint32_t ns = gs ? grow(ks, Ds) : ks, ndd = gd ? grow(kd, Dd) : kd;
+		if (ns <= ks && ndd <= kd)
+			break; /* the reference raises here */
+		ks = ns, kd = ndd;
+	}
+	int sel = -1;
+	for (uint32_t ci = 0; ci < nd; ++ci)
+		if (R[ci].eligible && R[ci].usable && (sel < 0 || better(R + ci, R + sel)))
+			sel = (int)ci;
+	out->levels = level + 1;
+	if (sel >= 0)
+	{
+		const combo_result* r = R + sel;
+		const float*		d = m->dirs + ((size_t)c * kMaxDirs + sel) * 3;
+		out->combo = sel, out->ks = r->ks, out->kd = r->kd, out->hit_count = r->hit_count, out->covered = r->covered;
+		out->ef_count = r->ef_count, out->depth = r->depth, out->objective = r->objective, out->area = r->area;
+		out->disp_cost = r->disp_cost;
+		memcpy(out->dir, d, 12);
+	}
+	free(R), free(hits);
+}
+
+/* get_adj_contours (intersection_resolver2.cpp:414-448): other contours whose
+ * boundary flags cover all source vertices on one side and all target vertices
+ * on the other. */
+static void adj_contours(const or_prp_mesh* m, uint32_t c, const or_prp_hit* h, uint8_t* mark)
+{
+	uint32_t	   sv[3], dv[3];
+	const uint32_t ns = hit_src_verts(m, h, sv), nd = hit_dst_verts(m, h, dv);
+	for (uint32_t i = m->flag_off[sv[0]]; i < m->flag_off[sv[0] + 1]; ++i)
+	{
+		const uint32_t fs = m->flags[i];
+		if (fs / 2 == c)
+			continue;
+		int ok = 1;
+		for (uint32_t k = 1; k < ns && ok; ++k)
+			ok = flag_has(m, sv[k], fs);
+		for (uint32_t k = 0; k < nd && ok; ++k)
+			ok = flag_has(m, dv[k], fs ^ 1u);
+		if (ok)
+			mark[fs / 2] = 1;
+	}
+}
+
+/* Full PRP of the prepared contours: choices[nc]; response records (16 words
+ * each, CollisionPairTemplate) written to recs (cap), returns their count. */
+uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa, void* choices_out,
+	uint32_t* recs, uint32_t cap)
+{
+	const uint32_t nc = m->nc, active = nc < max_contours ? nc : max_contours;
+	or_prp_choice* ch = calloc(nc + 1, sizeof(or_prp_choice));
+	for (uint32_t c = 0; c < nc; ++c)
+		ch[c].combo = -1;
+	or_prp_hit** lists = calloc(nc + 1, sizeof(or_prp_hit*));
+	uint32_t*	 counts = calloc(nc + 1, 4);
+	for (uint32_t c = 0; c < active; ++c)
+	{
+		choose_contour(m, c, ch + c);
+		if (ch[c].combo < 0)
+			continue;
+		uint32_t	   cap2 = 0;
+		or_prp_metrics met;
+		const uint32_t n = eval_slot(m, c, mk(ch[c].dir[0], ch[c].dir[1], ch[c].dir[2]), ch[c].ks, ch[c].kd, lists + c,
+			&cap2, &met);
+		uint32_t w = 0;
+		for (uint32_t i = 0; i < n; ++i)
+			if (lists[c][i].valid)
+				lists[c][w++] = lists[c][i];
+		counts[c] = w;
+	}
+	/* suppression of adjacent contours (:450-590) */
+	uint8_t* adjm = calloc((size_t)nc * nc + 1, 1);
+	uint8_t* hasadj = calloc(nc + 1, 1);
+	for (uint32_t c = 0; c < active; ++c)
+		for (uint32_t i = 0; i < counts[c]; ++i)
+			adj_contours(m, c, lists[c] + i, adjm + (size_t)c * nc);
+	for (uint32_t c = 0; c < active; ++c)
+		for (uint32_t o = 0; o < nc; ++o)
+			hasadj[c] |= adjm[(size_t)c * nc + o] && o != c;
+	uint32_t* order = malloc(4 * (nc + 1));
+	uint32_t  no = 0;
+	for (uint32_t c = 0; c < active; ++c)
+		if (hasadj[c])
+			order[no++] = c;
+	for (uint32_t i = 0; i < no; ++i) /* area descending, index ascending */
+		for (uint32_t j = i + 1; j < no; ++j)
+		{
+			const float ai = ch[order[i]].area, aj = ch[order[j]].area;
+			if (aj > ai || (aj == ai && order[j] < order[i]))
+			{
+				uint32_t t = order[i];
+				order[i] = order[j], order[j] = t;
+			}
+		}
+	uint8_t* sup = calloc(nc + 1, 1);
+	for (uint32_t i = 0; i < no; ++i)
+	{
+		const uint32_t o = order[i];
+		if (sup[o])
+			continue;
+		for (uint32_t a = 0; a < active; ++a)
+			if (a != o && adjm[(size_t)o * nc + a])
+				sup[a] = 1;
+	}
+	/* responses */
+	uint32_t nr = 0;
+	for (uint32_t c = 0; c < active; ++c)
+	{
+		if (sup[c])
+		{
+			ch[c].suppressed = 1;
+			continue;
+		}
+		const float C = fmaxf(-ch[c].depth - 3e-3f, -response_depth);
+		for (uint32_t i = 0; i < counts[c]; ++i)
+		{
+			const or_prp_hit* h = lists[c] + i;
+			uint32_t		  rec[16];
+			float			  f[12];
+			memset(rec, 0, sizeof(rec));
+			memset(f, 0, sizeof(f));
+			float a;
+			float nn[3] = { ch[c].dir[0], ch[c].dir[1], ch[c].dir[2] };
+			if (h->type == 0)
+			{
+				const uint32_t* t = m->faces + 3 * h->dst;
+				a = fmaxf(0.5f * (m->va[h->src] + m->fa[h->dst]), 3e-6f);
+				rec[0] = h->src, rec[1] = t[0], rec[2] = t[1] | 0x80000000u, rec[3] = t[2];
+				f[6] = h->b0, f[7] = h->b1;
+			}
+			else if (h->type == 2)
+			{
+				const uint32_t* t = m->faces + 3 * h->src;
+				a = fmaxf(0.5f * (m->fa[h->src] + m->va[h->dst]), 3e-6f);
+				rec[0] = h->dst, rec[1] = t[0], rec[2] = t[1] | 0x80000000u, rec[3] = t[2];
+				nn[0] = -nn[0], nn[1] = -nn[1], nn[2] = -nn[2];
+				f[6] = h->b0, f[7] = h->b1;
+			}
+			else
+			{
+				a = fmaxf(0.5f * (m->ea[h->src] + m->ea[h->dst]), 3e-6f);
+				rec[0] = m->edges[2 * h->src], rec[1] = m->edges[2 * h->src + 1];
+				rec[2] = m->edges[2 * h->dst], rec[3] = m->edges[2 * h->dst + 1];
+				f[6] = h->b0, f[7] = h->b1;
+			}
+			const float stiff = a * kappa;
+			f[0] = nn[0], f[1] = nn[1], f[2] = nn[2], f[3] = a, f[4] = stiff * C, f[5] = stiff;
+			memcpy(rec + 4, f, 48);
+			if (nr < cap)
+				memcpy(recs + 16 * (size_t)nr, rec, 64);
+			++nr;
+		}
+	}
+	if (choices_out)
+		memcpy(choices_out, ch, sizeof(or_prp_choice) * nc);
+	for (uint32_t c = 0; c < nc; ++c)
+		free(lists[c]);
+	free(lists), free(counts), free(adjm), free(hasadj), free(order), free(sup), free(ch);
+	return nr;
+}

@@ -1,0 +1,239 @@
+"""Pinning of the PRP oracle (oracle_prp.c) on the CPU: known answers of the
+reference's UnitTest/test_prp_selection.cpp, closed forms, and independent
+library routines (numpy eigh, scipy Dijkstra and connected components)."""
+import numpy as np
+import pytest
+
+import oracle
+from prp_cases import prp_fixture
+
+FMAX = np.finfo(np.float32).max
+
+
+def test_ownership_known_answers():
+    # test_prp_selection.cpp:109-160
+    assert oracle.prp_owns(0.25, 1.0)
+    assert not oracle.prp_owns(1.0, 0.25)
+    assert oracle.prp_owns(0.5, 0.5)
+    assert oracle.prp_owns(0.5, FMAX)
+    assert not oracle.prp_owns(FMAX, 0.5)
+    assert not oracle.prp_owns_dual(10.0, 1.0, 0.1, 0.2)
+    assert not oracle.prp_owns_dual(1.0, 10.0, 0.9, 0.1)
+    assert oracle.prp_owns_dual(2.0, 3.0, 0.3, 0.2)
+    assert not oracle.prp_owns_dual(2.0, 3.0, 0.3, 0.2, 0.0)
+    assert oracle.prp_owns_dual(5.0, 3.0, 0.5, 0.3)
+    assert oracle.prp_owns_dual(0.5, FMAX, 0.1, 0.2)
+    assert not oracle.prp_owns_dual(0.5, FMAX, 0.9, 0.1)
+    assert not oracle.prp_owns_dual(FMAX, 0.5, 0.1, 0.2)
+    assert not oracle.prp_owns_dual(8.0, 2.0, 4.0, 6.0)
+    for tau in (0.0, 0.25, 1.0):
+        assert oracle.prp_owns_dual(1.0, 10.0, 0.9, 0.1, tau, True, False)
+        assert oracle.prp_owns_dual(10.0, 1.0, 0.1, 0.2, tau, False, True)
+        assert oracle.prp_owns_dual(10.0, 1.0, 0.1, 0.2, tau, True, False) == (tau == 1.0)
+        assert oracle.prp_owns_dual(1.0, 10.0, 0.9, 0.1, tau, False, True) == (tau == 1.0)
+
+
+def test_comparator_known_answers():
+    # test_prp_selection.cpp:272-290: eligibility first, then objective / coverage^2
+    one = np.ones(2, np.uint32)
+    assert oracle.prp_select([10.0, 0.01], one, one, [1, 1], [1, 1], one, [1, 0]) == 0
+    assert oracle.prp_select([4.0, 1.0], [4, 1], [4, 4], [1, 1], [1, 1], one, [1, 1]) == 0
+    # ties inside the relative tolerance fall through to depth, cost, hit count, combo index
+    assert oracle.prp_select([1.0, 1.0 + 5e-6], [1, 1], [1, 1], [2.0, 1.0], [1, 1], one, [1, 1]) == 1
+    assert oracle.prp_select([1.0, 1.0], [1, 1], [1, 1], [1.0, 1.0], [3.0, 2.0], one, [1, 1]) == 1
+    assert oracle.prp_select([1.0, 1.0], [1, 1], [1, 1], [1.0, 1.0], [1.0, 1.0], [5, 3], [1, 1]) == 1
+    assert oracle.prp_select([1.0, 1.0], [1, 1], [1, 1], [1.0, 1.0], [1.0, 1.0], [3, 3], [1, 1]) == 0
+    assert oracle.prp_select([1.0], [0], [1], [1.0], [1.0], [1], [1]) == -1  # no coverage: unusable
+
+
+def test_direction_slots_fixture():
+    # test_prp_selection.cpp:201-270: 10 slots [+-e0, +-e1, +-e2, c-c0, c-c1, +-(c-cG)]
+    x = np.array([[-3, -2, -1], [-3, 2, -1], [-3, 0, 1], [3, -2, -1], [3, 2, -1], [3, 0, 1]], np.float32)
+    va = np.ones(6, np.float32)
+    b0, b1 = [0, 1, 2], [3, 4, 5]
+    pts = np.array([[0, .4, .4], [0, .6, .4], [0, .4, .6], [0, .6, .6]], np.float32)
+    w = np.ones(4, np.float32)
+    gc = x.astype(np.float64).mean(0)
+    d = oracle.prp_directions(pts, w, b0, b1, x, va, gc)
+    assert d.shape == (10, 3)
+    c = pts.astype(np.float64).mean(0)
+    S = sum(np.outer(x[v] - c, x[v] - c) for v in range(6))
+    ev, E = np.linalg.eigh(S)
+    for i in range(3):
+        e = E[:, i]
+        assert min(np.abs(d[2 * i] - e).max(), np.abs(d[2 * i] + e).max()) <= 1e-6
+        assert np.abs(d[2 * i + 1] + d[2 * i]).max() <= 1e-7
+    for r, verts in enumerate((b0, b1)):
+        want = c - x[verts].mean(0)
+        assert np.abs(d[6 + r] - want / np.linalg.norm(want)).max() <= 1e-6
+    want = c - gc
+    assert np.abs(d[8] - want / np.linalg.norm(want)).max() <= 1e-6
+    assert np.abs(d[9] + d[8]).max() <= 1e-7
+    assert np.allclose((d.astype(np.float64) ** 2).sum(1), 1.0, atol=1e-6)
+    # extra directions for the larger configurations are unit and distinct
+    d64 = oracle.prp_directions(pts, w, b0, b1, x, va, gc, total=64)
+    assert d64.shape == (64, 3) and np.array_equal(d64[:10], d)
+    assert np.allclose(np.linalg.norm(d64, axis=1), 1.0, atol=1e-6)
+
+
+def _grid_oracle(n=9):
+    from untangling26_amd import initializer, scenes
+
+    x = scenes.grid_sheet(n, n, 1.0, 1.0).astype(np.float32)
+    m = scenes.Mesh(x, scenes.grid_faces(n, n))
+    st = initializer.setup_cloth(m)
+    o = oracle.PrpOracle(x, x, st.edges, st.faces, st.face_adj_edges, st.edge_adj_faces, st.rest_vert_area,
+                         st.rest_edge_area, st.rest_face_area, st.mass)
+    return o, st, x
+
+
+def test_hop_distance_closed_form_and_geodesic_vs_scipy():
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import dijkstra
+
+    n = 9
+    o, st, x = _grid_oracle(n)
+    src = 4 * n + 4  # centre vertex
+    hop, geo = o.distances([src])
+    # graph distance on the triangulated grid: diagonal steps along the shared diagonal
+    e = st.edges.astype(np.int64)
+    d = x[e[:, 0]] - x[e[:, 1]]
+    diag = np.abs(d[:, 0]) > 0.5 / (n - 1)
+    diag &= np.abs(d[:, 2]) > 0.5 / (n - 1)
+    sgn = np.sign(d[diag][0, 0] * d[diag][0, 2])
+    ii, jj = np.divmod(np.arange(n * n), n)
+    di, dj = ii - 4, jj - 4
+    want = np.where(np.sign(di * dj) == sgn, np.maximum(np.abs(di), np.abs(dj)), np.abs(di) + np.abs(dj))
+    assert np.array_equal(hop.astype(np.int64), want)
+    w = np.linalg.norm(x[e[:, 0]].astype(np.float64) - x[e[:, 1]], axis=1)
+    G = sp.coo_matrix((np.r_[w, w], (np.r_[e[:, 0], e[:, 1]], np.r_[e[:, 1], e[:, 0]])), shape=(n * n, n * n))
+    ref = dijkstra(G.tocsr(), indices=src)
+    assert np.allclose(geo, ref, rtol=1e-6, atol=1e-7)
+
+
+def _check_hit_geometry(f, hits, direction):
+    x = f["mesh"].x.astype(np.float64)
+    F = f["setup"].faces.astype(np.int64)
+    E = f["setup"].edges.astype(np.int64)
+    r = np.asarray(direction, np.float64)
+    for h in hits:
+        if h["type"] == 0:
+            p = x[h["src"]] + h["dist"] * r
+            t = F[h["dst"]]
+            q = h["b0"] * x[t[0]] + h["b1"] * x[t[1]] + (1 - h["b0"] - h["b1"]) * x[t[2]]
+        elif h["type"] == 2:
+            p = x[h["dst"]] - h["dist"] * r
+            t = F[h["src"]]
+            q = h["b0"] * x[t[0]] + h["b1"] * x[t[1]] + (1 - h["b0"] - h["b1"]) * x[t[2]]
+        else:
+            a, b = E[h["src"]], E[h["dst"]]
+            e1, e2 = x[a[1]] - x[a[0]], x[b[1]] - x[b[0]]
+            if abs(np.dot(r, np.cross(e1, e2))) < 1e-5 * np.linalg.norm(e1) * np.linalg.norm(e2):
+                # the reference's Cramer solve accepts |det| >= 1e-10 in FP32: for (nearly)
+                # parallel edges the rounded determinant passes and the hit is arbitrary
+                # (cached_bvh.cpp:497-512); the oracle keeps that behaviour
+                continue
+            p = h["b0"] * x[a[0]] + (1 - h["b0"]) * x[a[1]] + h["dist"] * r
+            q = h["b1"] * x[b[0]] + (1 - h["b1"]) * x[b[1]]
+        assert np.linalg.norm(p - q) <= 2e-5
+        assert 0.0 <= h["dist"] <= 1.0
+
+
+@pytest.mark.parametrize("name", ["two_sheets", "folded_small"])
+def test_slot_hits_geometry_components_and_metrics(name):
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    f = prp_fixture(name)
+    o, st = f["oracle"], f["setup"]
+    assert f["nc"] >= 1
+    checked = 0
+    slots = []
+    for c in range(min(f["nc"], 6)):
+        dirs, diam, nb = o.state(c)
+        assert len(dirs) == 10 and (diam > 0).all() and (nb > 0).all()
+        k = (3, 3)
+        slots += [(c, d, k) for d in dirs]
+    for c, direction, k in slots:
+        hits, met = o.eval_slot(c, direction, *k)
+        dirs = {0: direction}
+        ci = 0
+        _check_hit_geometry(f, hits, dirs[ci])
+        n = len(hits)
+        assert met["raw_hits"] == n
+        if n == 0 or n > 900 or checked >= 3:
+            continue
+        # hit graph: symmetric predicates, components by scipy
+        rows, cols = [], []
+        for i in range(n):
+            for j in range(n):
+                if i != j and o.hits_adjacent(hits[i], hits[j]):
+                    assert o.hits_adjacent(hits[j], hits[i])
+                    rows.append(i), cols.append(j)
+        _, lab = connected_components(coo_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n)), directed=False)
+        valid = hits["valid"] != 0
+        if valid.any():
+            assert len(set(lab[valid])) == 1 and (lab == lab[valid][0]).sum() == valid.sum()
+        # J(r) closed form over the kept hits
+        v = hits[valid]
+        area = np.where(v["type"] == 0, st.rest_vert_area[v["src"]],
+                        np.where(v["type"] == 1, st.rest_edge_area[v["src"]], st.rest_face_area[v["src"]]))
+        depth = v["dist"].max() if len(v) else 0.0
+        assert met["hit_count"] == valid.sum()
+        assert np.isclose(met["area"], area.astype(np.float64).sum(), rtol=1e-5)
+        assert np.isclose(met["objective"], area.astype(np.float64).sum() * depth ** 2, rtol=1e-5)
+        assert met["depth"] == depth
+        checked += 1
+    assert checked >= 1
+
+
+def test_closed_form_ray_between_parallel_layers():
+    """A vertex of the lower sheet casts along +z into the upper sheet: the
+    VF hit distance is the layer gap and the barycentre is the vertex's
+    position in the upper triangle."""
+    from untangling26_amd import initializer, scenes
+
+    n = 5
+    a = scenes.grid_sheet(n, n, 1.0, 1.0).astype(np.float32)  # y = 0 plane
+    a = a[:, [0, 2, 1]]  # z = 0 plane
+    b = a.copy()
+    b[:, 2] += 0.1
+    b[:, :2] += np.float32(0.125 / 2)
+    x = np.concatenate([a, b]).astype(np.float32)
+    faces = np.concatenate([scenes.grid_faces(n, n), scenes.grid_faces(n, n) + n * n]).astype(np.uint32)
+    m = scenes.Mesh(x, faces)
+    st = initializer.setup_cloth(m)
+    o = oracle.PrpOracle(x, x, st.edges, st.faces, st.face_adj_edges, st.edge_adj_faces, st.rest_vert_area,
+                         st.rest_edge_area, st.rest_face_area, st.mass)
+    # one synthetic EF pair so that contour 0 has role 0 = lower sheet, role 1 = upper sheet
+    e0 = int(np.where((st.edges[:, 0] == 12) | (st.edges[:, 1] == 12))[0][0])
+    f1 = 32 + 5
+    rec = np.zeros((1, 12), np.uint32)
+    rec[0, :2] = st.edges[e0]
+    rec[0, 2:5] = st.faces[f1]
+    rec[0, 5:8] = np.array([0.5, 0.3, 0.3], np.float32).view(np.uint32)
+    o.prepare(rec, np.zeros((1, 12), np.uint32), np.zeros(1, np.uint32), np.zeros(1, np.uint32),
+              np.zeros(1, np.uint32), 1)
+    hits, met = o.eval_slot(0, np.array([0, 0, 1], np.float32), 100, 100)
+    vf = hits[hits["type"] == 0]
+    assert len(vf) > 0
+    assert np.allclose(vf["dist"], 0.1, atol=1e-6)
+    assert (vf["src"] < n * n).all() and (vf["dst"] >= 32).all()
+    fv = hits[hits["type"] == 2]
+    assert np.allclose(fv["dist"], 0.1, atol=1e-6) and (fv["dst"] >= n * n).all()
+
+
+def test_resolve_responses_two_sheets():
+    f = prp_fixture("two_sheets")
+    o, st = f["oracle"], f["setup"]
+    ch, recs = o.resolve(max_contours=256, response_depth=1.0, kappa=1e8)
+    assert len(ch) == f["nc"] and (ch["combo"] >= 0).any() and len(recs) > 0
+    fl = recs[:, 4:16].view(np.float32)
+    # every response: |normal| = 1, k2 = area * kappa, k1 = k2 * C with C from its contour's depth
+    assert np.allclose(np.linalg.norm(fl[:, 0:3], axis=1), 1.0, atol=1e-6)
+    assert np.allclose(fl[:, 5], fl[:, 3] * np.float32(1e8), rtol=1e-6)
+    sel = ch[ch["combo"] >= 0]
+    Cs = np.maximum(-sel["depth"] - 3e-3, -1.0).astype(np.float32)
+    C = fl[:, 4] / fl[:, 5]
+    assert np.all([np.isclose(C, c, rtol=1e-5).any() for c in Cs])
+    assert (C < 0).all()
