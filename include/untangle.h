@@ -192,6 +192,12 @@ UT_API int ut_pcg_destroy(ut_pcg* pcg);
  * Returns 0, or 2 (not converged) with the last iterate in x. */
 UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* vals, const double* b, double* x,
 	ut_pcg_report* report);
+/* Mixed-precision solve of the FP64 system: FP32 inner PCG on vals32 (the
+ * FP32 rounding of vals64), refinement residuals r = b - A x with vals64, so the
+ * converged x solves the FP64 matrix (precond_cg.cpp:706-914 solves in FP32 only;
+ * params->precision must be UT_PCG_MIXED). */
+UT_API int ut_pcg_solve_refined(ut_pcg* pcg, const ut_pcg_params* params, const float* vals32, const double* vals64,
+	const double* b, double* x, ut_pcg_report* report);
 /* y = A x (FP64 vectors and accumulation; vals FP32 when vals_fp32 != 0). */
 UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const double* x, double* y);
 
@@ -330,6 +336,82 @@ UT_API int ut_solver_get_state(ut_solver* s, float* x, float* v);
 UT_API int ut_solver_set_state(ut_solver* s, const float* x, const float* v);
 /* UT_CT_* output of the contours built in the last step (see ut_contours_get). */
 UT_API int ut_solver_contour_get(ut_solver* s, int what, void* dst);
+
+/* ---- PRP untangling: IntersectionResolver::host_resolve with PRP
+ *      (intersection_resolver.cpp:2131-2217 -> intersection_resolver2.cpp:2750-4069) ---- */
+
+typedef struct ut_prp ut_prp;
+
+typedef struct ut_prp_params {
+	uint32_t max_contours;	 /* untangling_process_contours_count (scene_params.h:67), default 256 */
+	uint32_t total_dirs;	 /* directions per contour: 10 = get_raycasting_direction; more append a Fibonacci set */
+	uint32_t batch_contours; /* contours evaluated together (PRP_cpu_contour_batch_size, scene_params.h:68) */
+	float	 response_depth; /* untangling_response_depth */
+	float	 kappa;			 /* stiffness_untangling */
+	float	 tau;			 /* PRP_intrinsic_tau */
+} ut_prp_params;
+
+/* PRPMinContourHitInfo essentials of the chosen direction of one contour. */
+typedef struct ut_prp_choice {
+	int32_t	 combo; /* direction slot, -1 when the contour has no response */
+	uint32_t ks, kd, hit_count, covered, ef_count, levels, suppressed;
+	float	 dir[3], depth, objective, area, disp_cost;
+	uint32_t pad;
+} ut_prp_choice;
+
+typedef struct ut_prp_stats {
+	uint32_t contours, levels, slots, hits, responses;
+	float	 ms_prepare, ms_levels, ms_final;
+} ut_prp_stats;
+
+/* One (contour, direction) evaluation at hop radii (ks, kd); contour is the
+ * index inside the batch given to ut_prp_prepare. */
+typedef struct ut_prp_slot {
+	uint32_t contour, combo, ks, kd;
+	float	 dir[3], ee_pmin, ee_pmax;
+	uint32_t pad;
+} ut_prp_slot;
+
+typedef struct ut_prp_summary {
+	uint32_t raw_hits, hit_count, covered, ef_count;
+	float	 depth, volume, objective, area, disp_cost;
+	uint32_t max_hop_src, max_hop_dst, usable, root, pad[3];
+} ut_prp_summary;
+
+/* RayCasting::HitInfo (intersection_resolver_helper.h:260-448) compacted:
+ * type 0 VF (src vertex, dst face), 1 EE (src edge, dst edge), 2 FV (src face,
+ * dst vertex); b0, b1 the face or edge barycentres; root its cluster. */
+typedef struct ut_prp_hit {
+	uint32_t slot, type, src, dst;
+	float	 b0, b1, dist;
+	uint32_t root;
+} ut_prp_hit;
+
+/* Topology and rest data are the caller's device arrays (kept by reference). */
+UT_API int ut_prp_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges,
+	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, const float* rest_x,
+	const float* vert_area, const float* edge_area, const float* face_area, const float* mass,
+	const ut_prp_params* params, ut_prp** out);
+UT_API int ut_prp_destroy(ut_prp* p);
+UT_API int ut_prp_set_params(ut_prp* p, const ut_prp_params* params);
+/* host_resolve_intersections_PRP: contours of ct at positions x (device), rays
+ * through the LBVHs built at x; response pairs (CollisionPairTemplate, 64 B)
+ * appended to out_pairs at device counter out_count.  choices[c] for every
+ * contour (choices_cap entries).  Synchronises. */
+UT_API int ut_prp_resolve(ut_prp* p, const float* x, ut_contours* ct, ut_lbvh* face_tree, ut_lbvh* edge_tree,
+	void* out_pairs, uint32_t* out_count, uint32_t out_cap, ut_prp_choice* choices, uint32_t choices_cap,
+	ut_prp_stats* stats);
+/* Parity hooks: prepare contour batch [c0, c0 + n) (distances, directions),
+ * read its directions and distances, evaluate slots and read the sorted hits of
+ * the last evaluation, read the face/edge/vertex normals. */
+UT_API int ut_prp_prepare(ut_prp* p, const float* x, ut_contours* ct, uint32_t c0, uint32_t n);
+UT_API int ut_prp_directions(ut_prp* p, uint32_t contour, float* dirs, uint32_t* count);
+UT_API int ut_prp_distances(ut_prp* p, uint32_t local_contour, uint32_t role, uint32_t* hop, float* geo,
+	uint32_t* diameter);
+UT_API int ut_prp_eval(ut_prp* p, ut_lbvh* face_tree, ut_lbvh* edge_tree, const ut_prp_slot* slots, uint32_t nslots,
+	const uint32_t* kring, ut_prp_summary* out, uint32_t* raw_hits);
+UT_API int ut_prp_hits(ut_prp* p, ut_prp_hit* out, uint32_t cap);
+UT_API int ut_prp_normals(ut_prp* p, float* out);
 
 #ifdef __cplusplus
 }

@@ -493,7 +493,7 @@ uint32_t or_prp_directions(uint32_t np, const float* points, const float* weight
 	}
 	(void)nv;
 	/* extra directions: Fibonacci sphere */
-	for (uint32_t i = 0; n < total && n < kMaxDirs; ++i)
+	for (uint32_t i = 0; total > 10 && n < total && n < kMaxDirs; ++i)
 	{
 		const uint32_t m = total > 10 ? total - 10 : 1;
 		const double   zz = 1.0 - (2.0 * i + 1.0) / m;

@@ -47,6 +47,17 @@ SIGNATURES = {
     "ut_pcg_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     "ut_pcg_destroy": (C.c_int, [vp]),
     "ut_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, vp]),
+    "ut_pcg_solve_refined": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 11 + [C.POINTER(vp)]),
+    "ut_prp_destroy": (C.c_int, [vp]),
+    "ut_prp_set_params": (C.c_int, [vp, vp]),
+    "ut_prp_resolve": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp]),
+    "ut_prp_prepare": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32]),
+    "ut_prp_directions": (C.c_int, [vp, C.c_uint32, vp, u32p]),
+    "ut_prp_distances": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, u32p]),
+    "ut_prp_eval": (C.c_int, [vp, vp, vp, vp, C.c_uint32, vp, vp, u32p]),
+    "ut_prp_hits": (C.c_int, [vp, vp, C.c_uint32]),
+    "ut_prp_normals": (C.c_int, [vp, vp]),
     "ut_bsr_spmv": (C.c_int, [vp, vp, C.c_int, vp, vp]),
     "ut_contact_query": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, C.c_uint32]),
     "ut_toi_reset": (C.c_int, [vp, vp]),
@@ -108,6 +119,24 @@ class StepReport(C.Structure):
                 ("pcg_refinements", C.c_uint32), ("pcg_relative_residual", C.c_double), ("pcg_converged", C.c_int),
                 ("toi", C.c_float), ("energy", C.c_double * 3), ("ms_detect", C.c_float), ("ms_contours", C.c_float),
                 ("ms_assemble", C.c_float), ("ms_pcg", C.c_float), ("ms_ccd", C.c_float), ("ms_total", C.c_float)]
+
+
+class PrpParams(C.Structure):
+    _fields_ = [("max_contours", C.c_uint32), ("total_dirs", C.c_uint32), ("batch_contours", C.c_uint32),
+                ("response_depth", C.c_float), ("kappa", C.c_float), ("tau", C.c_float)]
+
+
+class PrpChoice(C.Structure):
+    _fields_ = [("combo", C.c_int32), ("ks", C.c_uint32), ("kd", C.c_uint32), ("hit_count", C.c_uint32),
+                ("covered", C.c_uint32), ("ef_count", C.c_uint32), ("levels", C.c_uint32),
+                ("suppressed", C.c_uint32), ("dir", C.c_float * 3), ("depth", C.c_float), ("objective", C.c_float),
+                ("area", C.c_float), ("disp_cost", C.c_float), ("pad", C.c_uint32)]
+
+
+class PrpStats(C.Structure):
+    _fields_ = [("contours", C.c_uint32), ("levels", C.c_uint32), ("slots", C.c_uint32), ("hits", C.c_uint32),
+                ("responses", C.c_uint32), ("ms_prepare", C.c_float), ("ms_levels", C.c_float),
+                ("ms_final", C.c_float)]
 
 
 def lib():

@@ -9,8 +9,12 @@
 #include "pcg.hpp"
 #include "contact.hpp"
 #include "solver.hpp"
+#include "prp.hpp"
 
+#include <algorithm>
+#include <cstring>
 #include <new>
+#include <vector>
 
 struct ut_ctx
 {
@@ -31,6 +35,10 @@ struct ut_pcg
 struct ut_solver
 {
 	ut::NewtonSolver s;
+};
+struct ut_prp
+{
+	ut::PrpResolver r;
 };
 
 namespace
@@ -475,8 +483,8 @@ UT_API int ut_pcg_destroy(ut_pcg* pcg)
 	return ut::kOk;
 }
 
-UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* vals, const double* b, double* x,
-	ut_pcg_report* report)
+static int pcg_solve_impl(ut_pcg* pcg, const ut_pcg_params* params, const void* vals, const double* vals64,
+	const double* b, double* x, ut_pcg_report* report)
 {
 	if (!pcg || !params)
 		return ut::kInvalid;
@@ -493,7 +501,7 @@ UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* va
 			 p.max_refine = params->max_refine;
 			 p.regularization = params->regularization;
 			 p.check_every = params->check_every ? params->check_every : 8;
-			 rep = pcg->s.solve(p, vals, b, x);
+			 rep = pcg->s.solve(p, vals, b, x, vals64);
 		 });
 	if (rc != ut::kOk)
 		return rc;
@@ -507,6 +515,20 @@ UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* va
 		report->failure = rep.failure;
 	}
 	return rep.converged ? ut::kOk : ut::kNotConverged;
+}
+
+UT_API int ut_pcg_solve(ut_pcg* pcg, const ut_pcg_params* params, const void* vals, const double* b, double* x,
+	ut_pcg_report* report)
+{
+	return pcg_solve_impl(pcg, params, vals, nullptr, b, x, report);
+}
+
+UT_API int ut_pcg_solve_refined(ut_pcg* pcg, const ut_pcg_params* params, const float* vals32, const double* vals64,
+	const double* b, double* x, ut_pcg_report* report)
+{
+	if (!vals64)
+		return ut::kInvalid;
+	return pcg_solve_impl(pcg, params, vals32, vals64, b, x, report);
 }
 
 UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const double* x, double* y)
@@ -795,3 +817,205 @@ UT_API int ut_assemble(ut_ctx* ctx, const ut_assemble_args* q)
 }
 
 } // extern "C"
+
+/* ------------------------------------------------------------------ PRP */
+
+static_assert(sizeof(ut_prp_slot) == sizeof(ut::PrpSlot), "slot layout");
+static_assert(sizeof(ut_prp_summary) == sizeof(ut::PrpSummary), "summary layout");
+static_assert(sizeof(ut_prp_hit) == sizeof(ut::PrpHit), "hit layout");
+
+static ut::PrpParams prp_params(const ut_prp_params* p)
+{
+	ut::PrpParams q;
+	if (p)
+	{
+		q.max_contours = p->max_contours;
+		q.total_dirs = p->total_dirs;
+		q.batch_contours = p->batch_contours;
+		q.response_depth = p->response_depth;
+		q.kappa = p->kappa;
+		q.tau = p->tau;
+	}
+	return q;
+}
+
+UT_API int ut_prp_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges,
+	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, const float* rest_x,
+	const float* vert_area, const float* edge_area, const float* face_area, const float* mass,
+	const ut_prp_params* params, ut_prp** out)
+{
+	if (!ctx || !out || !edges || !faces || !face_adj_edges || !edge_adj_faces || !rest_x || !vert_area
+		|| !edge_area || !face_area || !mass)
+		return ut::kInvalid;
+	*out = nullptr;
+	ut_prp* h = new (std::nothrow) ut_prp();
+	if (!h)
+		return ut::kInternal;
+	int rc = guarded(&ctx->c,
+		[&]
+		{
+			h->r.init(&ctx->c, nv, ne, nf, edges, faces, face_adj_edges, edge_adj_faces, rest_x, vert_area, edge_area,
+				face_area, mass, prp_params(params));
+		});
+	if (rc != ut::kOk)
+	{
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return ut::kOk;
+}
+
+UT_API int ut_prp_destroy(ut_prp* p)
+{
+	if (p)
+	{
+		if (p->r.ctx)
+			(void)hipStreamSynchronize(p->r.ctx->stream);
+		delete p;
+	}
+	return ut::kOk;
+}
+
+UT_API int ut_prp_set_params(ut_prp* p, const ut_prp_params* params)
+{
+	if (!p || !params)
+		return ut::kInvalid;
+	return guarded(p->r.ctx,
+		[&]
+		{
+			const ut::PrpParams q = prp_params(params);
+			UT_REQUIRE(q.total_dirs >= 1 && q.total_dirs <= 64, "prp: total_dirs must be in [1, 64]");
+			UT_REQUIRE(q.batch_contours >= 1, "prp: batch_contours must be positive");
+			p->r.params = q;
+		});
+}
+
+UT_API int ut_prp_resolve(ut_prp* p, const float* x, ut_contours* ct, ut_lbvh* face_tree, ut_lbvh* edge_tree,
+	void* out_pairs, uint32_t* out_count, uint32_t out_cap, ut_prp_choice* choices, uint32_t choices_cap,
+	ut_prp_stats* stats)
+{
+	if (!p || !x || !ct || !face_tree || !edge_tree || (!out_pairs && out_cap) || !out_count)
+		return ut::kInvalid;
+	return guarded(p->r.ctx,
+		[&]
+		{
+			std::vector<ut::PrpChoice> ch;
+			ut::PrpStats			   st;
+			p->r.resolve(x, ct->b, face_tree->b, edge_tree->b, ch, reinterpret_cast<ut::ContactRec*>(out_pairs),
+				out_count, out_cap, &st);
+			if (choices)
+				for (uint32_t c = 0; c < ch.size() && c < choices_cap; ++c)
+				{
+					const ut::PrpChoice& a = ch[c];
+					ut_prp_choice&		 b = choices[c];
+					b.combo = a.combo, b.ks = a.ks, b.kd = a.kd, b.hit_count = a.hit_count, b.covered = a.covered;
+					b.ef_count = a.ef_count, b.levels = a.levels, b.suppressed = a.suppressed;
+					for (int k = 0; k < 3; ++k)
+						b.dir[k] = a.dir[k];
+					b.depth = a.depth, b.objective = a.objective, b.area = a.area, b.disp_cost = a.disp_cost;
+				}
+			if (stats)
+			{
+				stats->contours = st.contours, stats->levels = st.levels, stats->slots = st.slots;
+				stats->hits = st.hits, stats->responses = st.responses;
+				stats->ms_prepare = st.ms_prepare, stats->ms_levels = st.ms_levels, stats->ms_final = st.ms_final;
+			}
+		});
+}
+
+UT_API int ut_prp_prepare(ut_prp* p, const float* x, ut_contours* ct, uint32_t c0, uint32_t n)
+{
+	if (!p || !x || !ct)
+		return ut::kInvalid;
+	return guarded(p->r.ctx, [&] { p->r.prepare(x, ct->b, c0, n); });
+}
+
+UT_API int ut_prp_directions(ut_prp* p, uint32_t contour, float* dirs, uint32_t* count)
+{
+	if (!p || !count)
+		return ut::kInvalid;
+	return guarded(p->r.ctx,
+		[&]
+		{
+			UT_REQUIRE(contour < p->r.h_dirs.size(), "prp: contour out of range");
+			const auto& d = p->r.h_dirs[contour];
+			*count = (uint32_t)(d.size() / 3);
+			if (dirs)
+				std::copy(d.begin(), d.end(), dirs);
+		});
+}
+
+UT_API int ut_prp_distances(ut_prp* p, uint32_t local_contour, uint32_t role, uint32_t* hop, float* geo,
+	uint32_t* diameter)
+{
+	if (!p || role > 1)
+		return ut::kInvalid;
+	return guarded(p->r.ctx,
+		[&]
+		{
+			auto&		   r = p->r;
+			UT_REQUIRE(local_contour < r.batch_n, "prp: contour not in the prepared batch");
+			const size_t   off = (size_t)(2 * local_contour + role) * r.nv;
+			const hipStream_t s = r.ctx->stream;
+			if (hop)
+				UT_HIP_CHECK(hipMemcpyAsync(hop, r.hop.get() + off, 4 * (size_t)r.nv, hipMemcpyDeviceToHost, s));
+			if (geo)
+				UT_HIP_CHECK(hipMemcpyAsync(geo, r.geo.get() + off, 4 * (size_t)r.nv, hipMemcpyDeviceToHost, s));
+			UT_HIP_CHECK(hipStreamSynchronize(s));
+			if (diameter)
+				*diameter = r.h_diam[2 * local_contour + role];
+		});
+}
+
+UT_API int ut_prp_eval(ut_prp* p, ut_lbvh* face_tree, ut_lbvh* edge_tree, const ut_prp_slot* slots,
+	uint32_t nslots, const uint32_t* kring, ut_prp_summary* out, uint32_t* raw_hits)
+{
+	if (!p || !face_tree || !edge_tree || (!slots && nslots) || !kring)
+		return ut::kInvalid;
+	return guarded(p->r.ctx,
+		[&]
+		{
+			std::vector<ut::PrpSlot> sl(nslots);
+			std::memcpy(sl.data(), slots, sizeof(ut::PrpSlot) * nslots);
+			for (const auto& q : sl)
+				UT_REQUIRE(q.contour < p->r.batch_n, "prp: slot contour not in the prepared batch");
+			std::vector<uint32_t>	   kr(kring, kring + 2 * p->r.batch_n);
+			std::vector<ut::PrpSummary> res;
+			const uint32_t			   nh = p->r.eval_slots(sl, face_tree->b, edge_tree->b, kr, res);
+			if (out)
+				std::memcpy(out, res.data(), sizeof(ut::PrpSummary) * res.size());
+			if (raw_hits)
+				*raw_hits = nh;
+		});
+}
+
+UT_API int ut_prp_hits(ut_prp* p, ut_prp_hit* out, uint32_t cap)
+{
+	if (!p || (!out && cap))
+		return ut::kInvalid;
+	return guarded(p->r.ctx,
+		[&]
+		{
+			const uint32_t n = std::min(cap, p->r.last_raw_hits);
+			if (n)
+			{
+				UT_HIP_CHECK(hipMemcpyAsync(out, p->r.hits_sorted.get(), sizeof(ut::PrpHit) * n, hipMemcpyDeviceToHost,
+					p->r.ctx->stream));
+				UT_HIP_CHECK(hipStreamSynchronize(p->r.ctx->stream));
+			}
+		});
+}
+
+UT_API int ut_prp_normals(ut_prp* p, float* out)
+{
+	if (!p || !out)
+		return ut::kInvalid;
+	return guarded(p->r.ctx,
+		[&]
+		{
+			const size_t n = 3 * ((size_t)p->r.nf + p->r.ne + p->r.nv);
+			UT_HIP_CHECK(hipMemcpyAsync(out, p->r.normals.get(), 4 * n, hipMemcpyDeviceToHost, p->r.ctx->stream));
+			UT_HIP_CHECK(hipStreamSynchronize(p->r.ctx->stream));
+		});
+}

@@ -51,12 +51,14 @@ __device__ __forceinline__ box3 query_box(int kind,
 }
 
 // Runs the traversal for query q (valid when active).  narrow(valid, q, leaf)
-// is called by all 64 lanes together.
-template <typename Narrow>
-__device__ __forceinline__ void traverse_staged(TravShared& sh,
+// is called by all 64 lanes together.  test(box) decides whether a child box
+// is entered; spill_slot selects this lane's region of the global stack.
+template <typename NodeTest, typename Narrow>
+__device__ __forceinline__ void traverse_staged_test(TravShared& sh,
 	bool							 active,
 	uint32_t						 q,
-	const box3&						 qb,
+	uint32_t						 spill_slot,
+	const NodeTest&					 test,
 	bool							 cull_lt,
 	const BvhNode* __restrict__ nodes,
 	uint32_t* __restrict__ status,
@@ -99,8 +101,8 @@ __device__ __forceinline__ void traverse_staged(TravShared& sh,
 			const uint2	   refs = make_uint2(rec->d.x, rec->d.y);
 			const box3	   b0{ mk3(a.x, a.y, a.z), mk3(a.w, b.x, b.y) };
 			const box3	   b1{ mk3(b.z, b.w, c.x), mk3(c.y, c.z, c.w) };
-			const bool	   h0 = box_overlap(b0, qb);
-			const bool	   h1 = box_overlap(b1, qb);
+			const bool	   h0 = test(b0);
+			const bool	   h1 = test(b1);
 			const bool	   l0 = (refs.x & kLeafBit) != 0u, l1 = (refs.y & kLeafBit) != 0u;
 			o0 = refs.x & ~kLeafBit;
 			o1 = refs.y & ~kLeafBit;
@@ -112,7 +114,7 @@ __device__ __forceinline__ void traverse_staged(TravShared& sh,
 				if (sp < kStackLds)
 					sh.stack[sp * kTravThreads + threadIdx.x] = refs.y;
 				else if (sp < kStackTotal)
-					spill[(size_t)q * (kStackTotal - kStackLds) + (sp - kStackLds)] = refs.y;
+					spill[(size_t)spill_slot * (kStackTotal - kStackLds) + (sp - kStackLds)] = refs.y;
 				if (sp < kStackTotal)
 				{
 					++sp;
@@ -132,7 +134,7 @@ __device__ __forceinline__ void traverse_staged(TravShared& sh,
 			{
 				--sp;
 				node = sp < kStackLds ? sh.stack[sp * kTravThreads + threadIdx.x]
-									  : spill[(size_t)q * (kStackTotal - kStackLds) + (sp - kStackLds)];
+									  : spill[(size_t)spill_slot * (kStackTotal - kStackLds) + (sp - kStackLds)];
 			}
 			else
 				go = false;
@@ -155,6 +157,23 @@ __device__ __forceinline__ void traverse_staged(TravShared& sh,
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 	if (pending > 0)
 		run_batch(pending);
+}
+
+// Box-overlap traversal of a query box (broad phase); the query index is its spill slot.
+template <typename Narrow>
+__device__ __forceinline__ void traverse_staged(TravShared& sh,
+	bool							 active,
+	uint32_t						 q,
+	const box3&						 qb,
+	bool							 cull_lt,
+	const BvhNode* __restrict__ nodes,
+	uint32_t* __restrict__ status,
+	uint32_t* __restrict__ spill,
+	uint32_t max_visits,
+	Narrow&	 narrow)
+{
+	auto test = [&](const box3& b) { return box_overlap(b, qb); };
+	traverse_staged_test(sh, active, q, q, test, cull_lt, nodes, status, spill, max_visits, narrow);
 }
 
 // Narrow functor of the plain broad phase: write (query, leaf) pairs.

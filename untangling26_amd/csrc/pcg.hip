@@ -572,7 +572,8 @@ uint32_t run_iterations(PcgSolver& s, const Vecs<T>& v, const T* vals, PcgParts 
 }
 
 template <typename T>
-PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const double* b, double* x)
+PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const double* b, double* x,
+	const double* vals64)
 {
 	hipStream_t	   st = s.ctx->stream;
 	const uint32_t n = s.n, nb = div_up(n, kPcgThreads);
@@ -593,8 +594,13 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 
 	auto residual_norm = [&]() -> double
 	{
-		k_residual64<T><<<nb, kPcgThreads, 0, st>>>(
-			n, s.slice_off, s.col, vals, s.nslots, x, b, v.r64, prm.regularization, v.scale64, res_part);
+		// the refinement residual uses the FP64 matrix when one is given
+		if (vals64)
+			k_residual64<double><<<nb, kPcgThreads, 0, st>>>(
+				n, s.slice_off, s.col, vals64, s.nslots, x, b, v.r64, prm.regularization, v.scale64, res_part);
+		else
+			k_residual64<T><<<nb, kPcgThreads, 0, st>>>(
+				n, s.slice_off, s.col, vals, s.nslots, x, b, v.r64, prm.regularization, v.scale64, res_part);
 		if (s.contacts)
 		{
 			k_contact_offdiag<double><<<kContactBlocks, kPcgThreads, 0, st>>>(
@@ -688,7 +694,7 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 }
 } // namespace
 
-PcgReport PcgSolver::solve(const PcgParams& p, const void* vals, const double* b, double* x)
+PcgReport PcgSolver::solve(const PcgParams& p, const void* vals, const double* b, double* x, const double* vals64)
 {
 	UT_REQUIRE(vals && b && x, "pcg: null input");
 	UT_REQUIRE(p.check_every >= 1, "pcg: check_every must be >= 1");
@@ -699,9 +705,10 @@ PcgReport PcgSolver::solve(const PcgParams& p, const void* vals, const double* b
 		return r;
 	}
 	if (p.precision == kPcgFp64)
-		return solve_typed<double>(*this, p, reinterpret_cast<const double*>(vals), b, x);
+		return solve_typed<double>(*this, p, reinterpret_cast<const double*>(vals), b, x, nullptr);
 	UT_REQUIRE(p.precision == kPcgMixed || p.precision == kPcgFp32, "pcg: unknown precision");
-	return solve_typed<float>(*this, p, reinterpret_cast<const float*>(vals), b, x);
+	UT_REQUIRE(!vals64 || p.precision == kPcgMixed, "pcg: an FP64 refinement matrix needs the mixed mode");
+	return solve_typed<float>(*this, p, reinterpret_cast<const float*>(vals), b, x, vals64);
 }
 
 void PcgSolver::spmv64(const void* vals, bool vals_fp32, const double* x, double* y)

@@ -82,7 +82,9 @@ struct PcgSolver
 	void init(Context* c, uint32_t n, uint32_t nslices, uint32_t nslots, const uint32_t* slice_off, const uint32_t* col,
 		const uint32_t* diag_slot);
 	~PcgSolver();
-	PcgReport solve(const PcgParams& p, const void* vals, const double* b, double* x);
+	// vals64 (mixed mode only): FP64 copy of the matrix for the refinement residual,
+	// so that the refined solution is that of the FP64 system.
+	PcgReport solve(const PcgParams& p, const void* vals, const double* b, double* x, const double* vals64 = nullptr);
 	// y = A x (FP64 vectors; vals FP32 when vals_fp32, else FP64)
 	void spmv64(const void* vals, bool vals_fp32, const double* x, double* y);
 };

@@ -1,0 +1,2072 @@
+// PRP untangling on gfx950 (see prp.hpp for the split between host and device).
+//
+// Reference semantics: intersection_resolver2.cpp (CPU PRP, the default
+// resolver) with the ray tests of cached_bvh.cpp:131-555 and the hit
+// adjacency of intersection_resolver_helper.cpp:771-902.  Kernels are FP32 with
+// no contraction, so every ray test rounds like the reference's scalar code.
+#include "prp.hpp"
+
+#include "lbvh_traverse.hpp"
+#include "radix_sort.hpp"
+#include "scan.hpp"
+#include "seg_tri.hpp"
+#include "ut_common.hpp"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace ut
+{
+
+namespace
+{
+
+constexpr uint32_t kHopInf = 0xFFFFFFFFu;
+constexpr uint32_t kOwnUnknown = 0xFFu;
+constexpr int	   kBfsThreads = 1024;
+constexpr int	   kRayBlocks = 1024; // persistent ray grid (spill stack per lane)
+constexpr uint32_t kMaxDirs = 64;
+constexpr uint64_t kHashEmpty = ~0ull;
+
+// ------------------------------------------------------------------ normals (:2605-2644)
+__device__ __forceinline__ f3 safe_normalize(f3 v)
+{
+	const float l2 = dot3(v, v);
+	return l2 > 1e-20f ? v * (1.0f / sqrtf(l2)) : mk3(0.0f, 0.0f, 0.0f);
+}
+
+__global__ void k_prp_face_normals(uint32_t nf, const float* __restrict__ x, const uint32_t* __restrict__ faces,
+	float* __restrict__ fn)
+{
+	const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+	if (f >= nf)
+		return;
+	const f3 a = ld3(x, faces[3 * f]), b = ld3(x, faces[3 * f + 1]), c = ld3(x, faces[3 * f + 2]);
+	st3(fn, f, safe_normalize(cross3(b - a, c - a)));
+}
+
+// edge normals (area-weighted faces of the edge), then vertex normals (faces of the vertex, ascending)
+__global__ void k_prp_edge_vert_normals(uint32_t ne, uint32_t nv, const uint32_t* __restrict__ eaf,
+	const uint32_t* __restrict__ vf_off, const uint32_t* __restrict__ vf, const float* __restrict__ fa,
+	const float* __restrict__ fn, float* __restrict__ en, float* __restrict__ vn)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < ne)
+	{
+		f3 s = mk3(0.0f, 0.0f, 0.0f);
+		for (int k = 0; k < 2; ++k)
+		{
+			const uint32_t f = eaf[2 * i + k];
+			if (f != kNone)
+				s = s + ld3(fn, f) * fa[f];
+		}
+		st3(en, i, safe_normalize(s));
+	}
+	else if (i < ne + nv)
+	{
+		const uint32_t v = i - ne;
+		f3			   s = mk3(0.0f, 0.0f, 0.0f);
+		for (uint32_t k = vf_off[v]; k < vf_off[v + 1]; ++k)
+			s = s + ld3(fn, vf[k]) * fa[vf[k]];
+		st3(vn, v, safe_normalize(s));
+	}
+}
+
+// ------------------------------------------------------------------ distances (:618-720)
+// One workgroup per (contour, role): BFS hop distance, then Bellman-Ford on
+// the rest edge lengths with a work list (same fixpoint as Dijkstra's FP32
+// relaxations: every final value is min over neighbours of d[u] + |u - w|).
+__global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint32_t c0,
+	const uint32_t* __restrict__ b_off, const uint32_t* __restrict__ b_verts, const uint32_t* __restrict__ vv_off,
+	const uint32_t* __restrict__ vv, const float* __restrict__ rest_x, uint32_t* __restrict__ hop_all,
+	float* __restrict__ geo_all, uint32_t* __restrict__ queue_all, uint32_t* __restrict__ inq_all,
+	uint32_t* __restrict__ diameter)
+{
+	const uint32_t s = blockIdx.x;
+	const uint32_t bs = 2 * c0 + s; // global (contour, role) slot
+	uint32_t*	   H = hop_all + (size_t)s * V;
+	float*		   G = geo_all + (size_t)s * V;
+	uint32_t*	   Q = queue_all + (size_t)s * 2 * V;
+	uint32_t*	   I = inq_all + (size_t)s * V;
+	__shared__ uint32_t nnext;
+	for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
+		H[v] = kHopInf, G[v] = FLT_MAX, I[v] = 0u;
+	__syncthreads();
+	const uint32_t b0 = b_off[bs], nb = b_off[bs + 1] - b0;
+	for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+	{
+		const uint32_t v = b_verts[b0 + i];
+		H[v] = 0u;
+		G[v] = 0.0f;
+		Q[i] = v;
+	}
+	__syncthreads();
+	// BFS, level synchronous inside the workgroup
+	uint32_t  ncur = nb, level = 0;
+	uint32_t *cur = Q, *nxt = Q + V;
+	while (ncur > 0)
+	{
+		if (threadIdx.x == 0)
+			nnext = 0;
+		__syncthreads();
+		for (uint32_t i = threadIdx.x; i < ncur; i += blockDim.x)
+		{
+			const uint32_t u = cur[i];
+			for (uint32_t k = vv_off[u]; k < vv_off[u + 1]; ++k)
+			{
+				const uint32_t w = vv[k];
+				if (atomicCAS(&H[w], kHopInf, level + 1) == kHopInf)
+					nxt[atomicAdd(&nnext, 1u)] = w;
+			}
+		}
+		__syncthreads();
+		ncur = nnext;
+		uint32_t* t = cur;
+		cur = nxt;
+		nxt = t;
+		++level;
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		diameter[s] = level > 0 ? level - 1 : 0;
+	// Bellman-Ford from the same sources
+	for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+		Q[i] = b_verts[b0 + i];
+	__syncthreads();
+	ncur = nb;
+	cur = Q, nxt = Q + V;
+	uint32_t gen = 1;
+	while (ncur > 0)
+	{
+		if (threadIdx.x == 0)
+			nnext = 0;
+		__syncthreads();
+		for (uint32_t i = threadIdx.x; i < ncur; i += blockDim.x)
+		{
+			const uint32_t u = cur[i];
+			const float	   du = __uint_as_float(__hip_atomic_load(
+				  reinterpret_cast<uint32_t*>(&G[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+			const f3 xu = ld3(rest_x, u);
+			for (uint32_t k = vv_off[u]; k < vv_off[u + 1]; ++k)
+			{
+				const uint32_t w = vv[k];
+				const f3	   d = xu - ld3(rest_x, w);
+				const float	   nd = du + sqrtf(dot3(d, d));
+				const uint32_t bits = __float_as_uint(nd);
+				const uint32_t old = atomicMin(reinterpret_cast<uint32_t*>(&G[w]), bits);
+				if (bits < old && atomicExch(&I[w], gen) != gen)
+					nxt[atomicAdd(&nnext, 1u)] = w;
+			}
+		}
+		__syncthreads();
+		ncur = nnext;
+		uint32_t* t = cur;
+		cur = nxt;
+		nxt = t;
+		++gen;
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------------ ownership + candidates (:1571-1804)
+__device__ __forceinline__ float normalized_phi(float own, float opp)
+{
+	const float unreachable = FLT_MAX * 0.5f;
+	const bool	g0 = isfinite(own) && own < unreachable, g1 = isfinite(opp) && opp < unreachable;
+	if (!g0 && !g1)
+		return 0.0f;
+	if (!g0)
+		return 1.0f;
+	if (!g1)
+		return -1.0f;
+	return (own - opp) / (own + opp + 1e-6f);
+}
+
+__device__ __forceinline__ bool owns1(float own, float opp, float tau)
+{
+	const float unreachable = FLT_MAX * 0.5f;
+	if (!isfinite(own) || own >= unreachable)
+		return false;
+	if (!isfinite(opp) || opp >= unreachable)
+		return true;
+	return normalized_phi(own, opp) <= tau;
+}
+
+__device__ __forceinline__ float min_dist_to(f3 p, const float* __restrict__ x, const uint32_t* __restrict__ b,
+	uint32_t n)
+{
+	float best = FLT_MAX;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		const f3	d = p - ld3(x, b[i]);
+		const float l = sqrtf(dot3(d, d));
+		best = l < best ? l : best;
+	}
+	return best;
+}
+
+// Ownership of every vertex within hop k of its role (computed once per vertex;
+// k only grows), then the candidate vertex and edge lists of each (contour, role).
+__global__ void k_prp_own(uint32_t V, uint32_t c0, uint32_t nslots2, const uint32_t* __restrict__ kring,
+	const uint32_t* __restrict__ hop_all, const float* __restrict__ geo_all, const float* __restrict__ x,
+	const uint32_t* __restrict__ b_off, const uint32_t* __restrict__ b_verts, float tau, uint8_t* __restrict__ own_all)
+{
+	const uint32_t s = blockIdx.y;
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nslots2 || v >= V)
+		return;
+	const size_t o = (size_t)s * V + v;
+	if (hop_all[o] > kring[s] || own_all[o] != kOwnUnknown)
+		return;
+	const uint32_t so = s ^ 1u;
+	const uint32_t bs = 2 * c0 + s, bo = 2 * c0 + so;
+	const f3	   p = ld3(x, v);
+	const float	   cur_own = min_dist_to(p, x, b_verts + b_off[bs], b_off[bs + 1] - b_off[bs]);
+	const float	   cur_opp = min_dist_to(p, x, b_verts + b_off[bo], b_off[bo + 1] - b_off[bo]);
+	const float	   rest_own = geo_all[o], rest_opp = geo_all[(size_t)so * V + v];
+	own_all[o] = (uint8_t)(owns1(rest_own, rest_opp, tau) && owns1(cur_own, cur_opp, tau));
+}
+
+__device__ __forceinline__ bool cand_vert(const uint32_t* H, const uint8_t* O, uint32_t k, uint32_t v)
+{
+	return H[v] <= k && O[v] == 1u;
+}
+__device__ __forceinline__ bool cand_edge(const uint32_t* H, const uint8_t* O, uint32_t k, uint32_t a, uint32_t b)
+{
+	return H[a] <= k && H[b] <= k && (O[a] == 1u || O[b] == 1u);
+}
+__device__ __forceinline__ bool cand_face(const uint32_t* H, const uint8_t* O, uint32_t k, uint32_t a, uint32_t b,
+	uint32_t c)
+{
+	return H[a] <= k && H[b] <= k && H[c] <= k && (O[a] == 1u || O[b] == 1u || O[c] == 1u);
+}
+
+// cand lists: per slot s, vertices at [s * (V + E)], edges at [s * (V + E) + V]
+__global__ void k_prp_cand_lists(uint32_t V, uint32_t E, uint32_t nslots2, const uint32_t* __restrict__ kring,
+	const uint32_t* __restrict__ hop_all, const uint8_t* __restrict__ own_all, const uint32_t* __restrict__ edges,
+	uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_cnt)
+{
+	const uint32_t s = blockIdx.y;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nslots2)
+		return;
+	const uint32_t* H = hop_all + (size_t)s * V;
+	const uint8_t*	O = own_all + (size_t)s * V;
+	const uint32_t	k = kring[s];
+	uint32_t*		C = cand + (size_t)s * (V + E);
+	const bool		vin = i < V && cand_vert(H, O, k, i);
+	const uint32_t	pv = wave_append(&cand_cnt[2 * s], vin);
+	if (vin)
+		C[pv] = i;
+	const bool	   ein = i < E && cand_edge(H, O, k, edges[2 * i], edges[2 * i + 1]);
+	const uint32_t pe = wave_append(&cand_cnt[2 * s + 1], ein);
+	if (ein)
+		C[V + pe] = i;
+}
+
+// Per slot: projections of the candidate target edges (role 1) on the slot
+// direction (intersection_resolver2.h:572-595) and the request offsets of the
+// three ray kinds.  One workgroup per slot; offsets by a single-workgroup scan.
+__global__ void k_prp_slot_prep(uint32_t V, uint32_t E, uint32_t nslots, PrpSlot* __restrict__ slots,
+	const uint32_t* __restrict__ cand, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ x,
+	const uint32_t* __restrict__ edges)
+{
+	const uint32_t j = blockIdx.x;
+	if (j >= nslots)
+		return;
+	PrpSlot		   sl = slots[j];
+	const uint32_t s1 = 2 * sl.contour + 1;
+	const uint32_t n = cand_cnt[2 * s1 + 1];
+	const uint32_t* C = cand + (size_t)s1 * (V + E) + V;
+	const f3		dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+	float			lo = FLT_MAX, hi = -FLT_MAX;
+	for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+	{
+		const uint32_t e = C[i];
+		const float	   p0 = dot3(ld3(x, edges[2 * e]), dir), p1 = dot3(ld3(x, edges[2 * e + 1]), dir);
+		lo = fminf(lo, fminf(p0, p1));
+		hi = fmaxf(hi, fmaxf(p0, p1));
+	}
+	lo = wave_min(lo);
+	hi = wave_max(hi);
+	__shared__ float slo[16], shi[16];
+	const uint32_t	 w = threadIdx.x / kWave;
+	if (__lane_id() == 0)
+		slo[w] = lo, shi[w] = hi;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		float a = slo[0], b = shi[0];
+		for (uint32_t k = 1; k < blockDim.x / kWave; ++k)
+			a = fminf(a, slo[k]), b = fmaxf(b, shi[k]);
+		slots[j].ee_pmin = n ? a : 0.0f;
+		slots[j].ee_pmax = n ? b : 0.0f;
+	}
+}
+
+// request counts per slot: [VF, EE, FV] = (|cand verts role 0|, |cand edges role 0|, |cand verts role 1|),
+// a kind is skipped when its target set is empty (intersection_resolver2.cpp:3040-3056)
+__global__ void k_prp_request_offsets(uint32_t nslots, const PrpSlot* __restrict__ slots,
+	const uint32_t* __restrict__ cand_cnt, const uint32_t* __restrict__ target_cnt, uint32_t* __restrict__ off)
+{
+	// single workgroup of 1024 threads, nslots <= 4096
+	__shared__ uint32_t part[3][1024];
+	const uint32_t		per = (nslots + blockDim.x - 1) / blockDim.x;
+	uint32_t			loc[3] = { 0, 0, 0 };
+	for (uint32_t q = 0; q < per; ++q)
+	{
+		const uint32_t j = threadIdx.x * per + q;
+		if (j >= nslots)
+			break;
+		const uint32_t c = slots[j].contour;
+		const uint32_t s0 = 2 * c, s1 = 2 * c + 1;
+		loc[0] += target_cnt[s1] ? cand_cnt[2 * s0] : 0u;			 // VF needs target faces of role 1
+		loc[1] += cand_cnt[2 * s1 + 1] ? cand_cnt[2 * s0 + 1] : 0u; // EE needs target edges of role 1
+		loc[2] += target_cnt[s0] ? cand_cnt[2 * s1] : 0u;			 // FV needs source faces of role 0
+	}
+	for (int t = 0; t < 3; ++t)
+		part[t][threadIdx.x] = loc[t];
+	__syncthreads();
+	if (threadIdx.x < 3)
+	{
+		uint32_t acc = 0;
+		for (uint32_t i = 0; i < blockDim.x; ++i)
+		{
+			const uint32_t v = part[threadIdx.x][i];
+			part[threadIdx.x][i] = acc;
+			acc += v;
+		}
+		off[threadIdx.x * (nslots + 1) + nslots] = acc;
+	}
+	__syncthreads();
+	uint32_t run[3] = { part[0][threadIdx.x], part[1][threadIdx.x], part[2][threadIdx.x] };
+	for (uint32_t q = 0; q < per; ++q)
+	{
+		const uint32_t j = threadIdx.x * per + q;
+		if (j >= nslots)
+			break;
+		const uint32_t c = slots[j].contour;
+		const uint32_t s0 = 2 * c, s1 = 2 * c + 1;
+		const uint32_t n[3] = { target_cnt[s1] ? cand_cnt[2 * s0] : 0u, cand_cnt[2 * s1 + 1] ? cand_cnt[2 * s0 + 1] : 0u,
+			target_cnt[s0] ? cand_cnt[2 * s1] : 0u };
+		for (int t = 0; t < 3; ++t)
+		{
+			off[t * (nslots + 1) + j] = run[t];
+			run[t] += n[t];
+		}
+	}
+}
+
+// number of candidate faces of each (contour, role): needed only as "any"
+__global__ void k_prp_any_faces(uint32_t F, uint32_t nslots2, const uint32_t* __restrict__ kring, uint32_t V,
+	const uint32_t* __restrict__ hop_all, const uint8_t* __restrict__ own_all, const uint32_t* __restrict__ faces,
+	uint32_t* __restrict__ target_cnt)
+{
+	const uint32_t s = blockIdx.y;
+	const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nslots2)
+		return;
+	const uint32_t* H = hop_all + (size_t)s * V;
+	const uint8_t*	O = own_all + (size_t)s * V;
+	const bool		in = f < F && cand_face(H, O, kring[s], faces[3 * f], faces[3 * f + 1], faces[3 * f + 2]);
+	if (__ballot(in) != 0ull && __lane_id() == 0)
+		atomicOr(&target_cnt[s], 1u);
+}
+
+
+// ------------------------------------------------------------------ rays (cached_bvh.cpp:131-555)
+struct RayArgs
+{
+	uint32_t		V, E, nslots;
+	const PrpSlot*	slots;
+	const uint32_t* off; // [3][nslots + 1]
+	const uint32_t* cand;
+	const uint32_t* hop_all;
+	const uint8_t*	own_all;
+	const uint32_t* kring;
+	const float*	x;
+	const uint32_t* edges;
+	const uint32_t* faces;
+	PrpHit*			hits;
+	uint32_t*		count;
+	uint32_t		cap;
+	uint32_t*		status;
+	uint32_t*		spill;
+	const BvhNode*	nodes;
+};
+
+__device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots, uint32_t r)
+{
+	uint32_t lo = 0, hi = nslots; // last j with off[j] <= r
+	while (hi - lo > 1)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (off[mid] <= r)
+			lo = mid;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ void emit_hit(const RayArgs& a, bool ok, const PrpHit& h)
+{
+	const uint32_t i = wave_append(a.count, ok);
+	if (ok && i < a.cap)
+		a.hits[i] = h;
+}
+
+// Per-lane request state shared by the three ray kinds.
+struct RayReq
+{
+	uint32_t slot, src, contour;
+	f3		 o, dir;
+	float	 tmax; // ray length along dir
+};
+
+// VF (kind 0) and FV (kind 2): segment from a vertex along +dir (VF) or -dir (FV), length 1,
+// against candidate faces of the other role (LineIntersection, t in [0, 1], incident faces skipped).
+template <int Kind>
+__global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
+{
+	__shared__ TravShared sh;
+	const uint32_t		  total = a.off[Kind * (a.nslots + 1) + a.nslots];
+	const uint32_t*		  off = a.off + Kind * (a.nslots + 1);
+	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
+	for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x)
+	{
+		const uint32_t r = base + threadIdx.x;
+		const bool	   active = r < total;
+		RayReq		   q{};
+		if (active)
+		{
+			q.slot = slot_of(off, a.nslots, r);
+			const PrpSlot& sl = a.slots[q.slot];
+			q.contour = sl.contour;
+			const uint32_t s_src = 2 * sl.contour + (Kind == 0 ? 0u : 1u);
+			q.src = a.cand[(size_t)s_src * (a.V + a.E) + (r - off[q.slot])];
+			q.dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+			q.o = ld3(a.x, q.src);
+			q.tmax = 1.0f;
+		}
+		const f3 cast = Kind == 0 ? q.dir : -q.dir;
+		const f3 inv = mk3(1.0f / cast.x, 1.0f / cast.y, 1.0f / cast.z);
+		auto	 test = [&](const box3& b)
+		{
+			// slab test of cached_bvh.cpp:147-165 with a small outward tolerance
+			const float t1x = (b.lo.x - q.o.x) * inv.x, t2x = (b.hi.x - q.o.x) * inv.x;
+			const float t1y = (b.lo.y - q.o.y) * inv.y, t2y = (b.hi.y - q.o.y) * inv.y;
+			const float t1z = (b.lo.z - q.o.z) * inv.z, t2z = (b.hi.z - q.o.z) * inv.z;
+			float		tmin = fminf(t1x, t2x), tmx = fmaxf(t1x, t2x);
+			tmin = fmaxf(tmin, fminf(t1y, t2y));
+			tmx = fminf(tmx, fmaxf(t1y, t2y));
+			tmin = fmaxf(tmin, fminf(t1z, t2z));
+			tmx = fminf(tmx, fmaxf(t1z, t2z));
+			return tmx + 1e-5f >= fmaxf(tmin, 0.0f) && tmin <= q.tmax + 1e-5f;
+		};
+		auto narrow = [&](bool valid, uint32_t rr, uint32_t f)
+		{
+			// the lane's own request is r; staged candidates carry their request id
+			bool   ok = false;
+			PrpHit h{};
+			if (valid)
+			{
+				const uint32_t slot = slot_of(off, a.nslots, rr);
+				const PrpSlot& sl = a.slots[slot];
+				const uint32_t s_dst = 2 * sl.contour + (Kind == 0 ? 1u : 0u);
+				const uint32_t s_src = s_dst ^ 1u;
+				const uint32_t v = a.cand[(size_t)s_src * (a.V + a.E) + (rr - off[slot])];
+				const uint32_t t0 = a.faces[3 * f], t1 = a.faces[3 * f + 1], t2 = a.faces[3 * f + 2];
+				const uint32_t* H = a.hop_all + (size_t)s_dst * a.V;
+				const uint8_t*	O = a.own_all + (size_t)s_dst * a.V;
+				if (v != t0 && v != t1 && v != t2 && cand_face(H, O, a.kring[s_dst], t0, t1, t2))
+				{
+					const f3 d = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+					const f3 o = ld3(a.x, v);
+					const f3 e = Kind == 0 ? o + d * 1.0f : o + (-d) * 1.0f;
+					f3		 bary;
+					float	 t;
+					if (line_intersection(o, e, ld3(a.x, t0), ld3(a.x, t1), ld3(a.x, t2), &bary, &t) && t >= 0.0f
+						&& t <= 1.0f)
+					{
+						ok = true;
+						h.slot = slot;
+						h.type = Kind == 0 ? 0u : 2u;
+						h.src = Kind == 0 ? v : f;
+						h.dst = Kind == 0 ? f : v;
+						h.b0 = bary.x;
+						h.b1 = bary.y;
+						h.dist = t * 1.0f;
+						h.root = 0;
+					}
+				}
+			}
+			emit_hit(a, ok, h);
+		};
+		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow);
+	}
+}
+
+// EE (kind 1): source edge swept along dir by ray_max (h:597-613) against candidate edges of role 1
+// (plane test, |det| >= 1e-10, Cramer solve with s in [0, ray_max], t1, t2 in [0, 1], shared vertices skipped).
+__global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
+{
+	__shared__ TravShared sh;
+	const uint32_t		  total = a.off[1 * (a.nslots + 1) + a.nslots];
+	const uint32_t*		  off = a.off + (a.nslots + 1);
+	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
+	for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x)
+	{
+		const uint32_t r = base + threadIdx.x;
+		bool		   active = r < total;
+		box3		   sw = box_empty();
+		if (active)
+		{
+			const uint32_t slot = slot_of(off, a.nslots, r);
+			const PrpSlot& sl = a.slots[slot];
+			const uint32_t e = a.cand[(size_t)(2 * sl.contour) * (a.V + a.E) + a.V + (r - off[slot])];
+			const f3	   dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+			const f3	   p0 = ld3(a.x, a.edges[2 * e]), p1 = ld3(a.x, a.edges[2 * e + 1]);
+			const float	   dl2 = dot3(dir, dir);
+			float		   rmax = 1.0f;
+			if (dl2 > 1e-12f)
+			{
+				const float smin = fminf(dot3(p0, dir), dot3(p1, dir));
+				rmax = fminf(fmaxf((sl.ee_pmax - smin) / dl2 + 1e-5f, 0.0f), 1.0f);
+			}
+			if (rmax <= 0.0f)
+				active = false;
+			sw = box_add(box_add(box_add(box_of(p0), p1), p0 + dir * rmax), p1 + dir * rmax);
+			sw = box_inflate(sw, 1e-5f);
+		}
+		auto test = [&](const box3& b)
+		{
+			return (sw.lo.x <= b.hi.x) & (sw.hi.x >= b.lo.x) & (sw.lo.y <= b.hi.y) & (sw.hi.y >= b.lo.y)
+				& (sw.lo.z <= b.hi.z) & (sw.hi.z >= b.lo.z);
+		};
+		auto narrow = [&](bool valid, uint32_t rr, uint32_t g)
+		{
+			bool   ok = false;
+			PrpHit h{};
+			if (valid)
+			{
+				const uint32_t	slot = slot_of(off, a.nslots, rr);
+				const PrpSlot&	sl = a.slots[slot];
+				const uint32_t	s1 = 2 * sl.contour + 1;
+				const uint32_t	e = a.cand[(size_t)(2 * sl.contour) * (a.V + a.E) + a.V + (rr - off[slot])];
+				const uint32_t	a0 = a.edges[2 * e], a1 = a.edges[2 * e + 1];
+				const uint32_t	b0 = a.edges[2 * g], b1 = a.edges[2 * g + 1];
+				const uint32_t* H = a.hop_all + (size_t)s1 * a.V;
+				const uint8_t*	O = a.own_all + (size_t)s1 * a.V;
+				if (a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1 && cand_edge(H, O, a.kring[s1], b0, b1))
+				{
+					const f3	dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+					const f3	p0 = ld3(a.x, a0), p1 = ld3(a.x, a1), e1 = p1 - p0;
+					const float dl2 = dot3(dir, dir);
+					float		rmax = 1.0f;
+					if (dl2 > 1e-12f)
+					{
+						const float smin = fminf(dot3(p0, dir), dot3(p1, dir));
+						rmax = fminf(fmaxf((sl.ee_pmax - smin) / dl2 + 1e-5f, 0.0f), 1.0f);
+					}
+					f3			pn = cross3(e1, dir);
+					const float pl = sqrtf(dot3(pn, pn));
+					const bool	use_plane = pl >= 1e-10f;
+					float		pd = 0.0f;
+					if (use_plane)
+					{
+						pn = pn * (1.0f / pl);
+						pd = dot3(pn, p0);
+					}
+					const f3 q0 = ld3(a.x, b0), q1 = ld3(a.x, b1);
+					bool	 keep = true;
+					if (use_plane)
+					{
+						const float d0 = dot3(pn, q0) - pd, d1 = dot3(pn, q1) - pd;
+						keep = !((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f));
+					}
+					if (keep)
+					{
+						const f3	e2 = q1 - q0, rv = q0 - p0, nn = cross3(e1, e2);
+						const float det = -dot3(dir, nn);
+						if (fabsf(det) >= 1e-10f)
+						{
+							const float inv = 1.0f / det;
+							const float s_ = -dot3(rv, cross3(e1, e2)) * inv;
+							const float t1 = -dot3(dir, cross3(rv, e2)) * inv;
+							const float t2 = dot3(dir, cross3(e1, rv)) * inv;
+							if (s_ >= 0.0f && s_ <= rmax && t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f)
+							{
+								ok = true;
+								h.slot = slot, h.type = 1u, h.src = e, h.dst = g;
+								h.b0 = 1.0f - t1, h.b1 = 1.0f - t2, h.dist = s_, h.root = 0;
+							}
+						}
+					}
+				}
+			}
+			emit_hit(a, ok, h);
+		};
+		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow);
+	}
+}
+
+// ------------------------------------------------------------------ hit graph
+struct HitArgs
+{
+	uint32_t		n, V, prim_bits;
+	const PrpHit*	h;		 // sorted by (slot, type, src)
+	const uint32_t* keys;	 // sorted keys
+	uint32_t*		parent;	 // union-find
+	const uint32_t *vv_off, *vv, *ve_off, *ve, *vf_off, *vf;
+	const uint32_t *edges, *faces, *fae, *eaf;
+};
+
+__device__ __forceinline__ uint32_t hit_key(uint32_t slot, uint32_t type, uint32_t prim, uint32_t pb)
+{
+	return (slot << (pb + 2)) | (type << pb) | prim;
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* k, uint32_t n, uint32_t v)
+{
+	uint32_t lo = 0, hi = n;
+	while (lo < hi)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (k[mid] < v)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+__device__ __forceinline__ uint32_t uf_find(uint32_t* p, uint32_t x)
+{
+	uint32_t r = x;
+	for (;;)
+	{
+		const uint32_t q = __hip_atomic_load(&p[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (q == r)
+			return r;
+		r = q;
+	}
+}
+
+__device__ __forceinline__ void uf_union(uint32_t* p, uint32_t a, uint32_t b)
+{
+	for (;;)
+	{
+		a = uf_find(p, a);
+		b = uf_find(p, b);
+		if (a == b)
+			return;
+		if (a < b)
+		{
+			const uint32_t t = a;
+			a = b;
+			b = t;
+		}
+		if (atomicCAS(&p[a], a, b) == a)
+			return;
+	}
+}
+
+__device__ __forceinline__ bool face_has_vert(const uint32_t* faces, uint32_t f, uint32_t v)
+{
+	return faces[3 * f] == v || faces[3 * f + 1] == v || faces[3 * f + 2] == v;
+}
+__device__ __forceinline__ bool face_has_edge(const uint32_t* eaf, uint32_t f, uint32_t e)
+{
+	return eaf[2 * e] == f || eaf[2 * e + 1] == f;
+}
+__device__ __forceinline__ bool edge_has_vert(const uint32_t* edges, uint32_t e, uint32_t v)
+{
+	return edges[2 * e] == v || edges[2 * e + 1] == v;
+}
+__device__ __forceinline__ bool verts_adjacent(const HitArgs& a, uint32_t u, uint32_t v)
+{
+	for (uint32_t k = a.vv_off[u]; k < a.vv_off[u + 1]; ++k)
+		if (a.vv[k] == v)
+			return true;
+	return false;
+}
+// e2 is one of the other two edges of a face adjacent to e1 (edge_adj_edges_ext, init_mesh_data.cpp:505-531)
+__device__ __forceinline__ bool edges_adjacent(const HitArgs& a, uint32_t e1, uint32_t e2)
+{
+	if (e1 == e2)
+		return false;
+	for (int k = 0; k < 2; ++k)
+	{
+		const uint32_t f = a.eaf[2 * e1 + k];
+		if (f != kNone && (a.fae[3 * f] == e2 || a.fae[3 * f + 1] == e2 || a.fae[3 * f + 2] == e2))
+			return true;
+	}
+	return false;
+}
+
+// visit every hit of bucket (slot, type, prim) and union with i when pred(j) holds
+template <typename Pred>
+__device__ __forceinline__ void probe(const HitArgs& a, uint32_t i, uint32_t slot, uint32_t type, uint32_t prim,
+	const Pred& pred)
+{
+	const uint32_t key = hit_key(slot, type, prim, a.prim_bits);
+	for (uint32_t j = lower_bound_u32(a.keys, a.n, key); j < a.n && a.keys[j] == key; ++j)
+		if (j != i && pred(a.h[j]))
+			uf_union(a.parent, i, j);
+}
+
+// find_adjacent_hits (intersection_resolver_helper.cpp:771-902) as bucket probes
+__global__ void k_prp_hit_union(HitArgs a)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= a.n)
+		return;
+	const PrpHit L = a.h[i];
+	const uint32_t s = L.slot;
+	if (L.type == 0) // VF(v, f)
+	{
+		const uint32_t v = L.src, f = L.dst;
+		for (uint32_t k = a.vv_off[v]; k < a.vv_off[v + 1]; ++k)
+			probe(a, i, s, 0, a.vv[k], [&](const PrpHit& R) { return R.dst == f; });
+		for (uint32_t k = a.vf_off[v]; k < a.vf_off[v + 1]; ++k)
+			probe(a, i, s, 2, a.vf[k], [&](const PrpHit& R) { return face_has_vert(a.faces, f, R.dst); });
+		for (uint32_t k = a.ve_off[v]; k < a.ve_off[v + 1]; ++k)
+			probe(a, i, s, 1, a.ve[k], [&](const PrpHit& R) { return face_has_edge(a.eaf, f, R.dst); });
+	}
+	else if (L.type == 2) // FV(f, v)
+	{
+		const uint32_t f = L.src, v = L.dst;
+		for (int k = 0; k < 3; ++k)
+			probe(a, i, s, 0, a.faces[3 * f + k], [&](const PrpHit& R) { return face_has_vert(a.faces, R.dst, v); });
+		probe(a, i, s, 2, f, [&](const PrpHit& R) { return verts_adjacent(a, v, R.dst); });
+		for (int k = 0; k < 3; ++k)
+			probe(a, i, s, 1, a.fae[3 * f + k], [&](const PrpHit& R) { return edge_has_vert(a.edges, R.dst, v); });
+	}
+	else // EE(e1, e2)
+	{
+		const uint32_t e1 = L.src, e2 = L.dst;
+		for (int k = 0; k < 2; ++k)
+			probe(a, i, s, 0, a.edges[2 * e1 + k], [&](const PrpHit& R) { return face_has_edge(a.eaf, R.dst, e2); });
+		for (int k = 0; k < 2; ++k)
+		{
+			const uint32_t f = a.eaf[2 * e1 + k];
+			if (f != kNone)
+				probe(a, i, s, 2, f, [&](const PrpHit& R) { return edge_has_vert(a.edges, e2, R.dst); });
+		}
+		probe(a, i, s, 1, e1, [&](const PrpHit& R) { return edges_adjacent(a, e2, R.dst); });
+		for (int k = 0; k < 2; ++k)
+		{
+			const uint32_t f = a.eaf[2 * e1 + k];
+			if (f == kNone)
+				continue;
+			for (int j = 0; j < 3; ++j)
+			{
+				const uint32_t ae = a.fae[3 * f + j];
+				if (ae != e1)
+					probe(a, i, s, 1, ae, [&](const PrpHit& R) { return R.dst == e2; });
+			}
+		}
+	}
+}
+
+// ------------------------------------------------------------------ clusters (:3506-3685)
+struct CompArgs
+{
+	uint32_t		n, nslots, V, c0;
+	const PrpHit*	h;
+	const uint32_t* parent;
+	const PrpSlot*	slots;
+	const uint32_t *flag_off, *flags, *edges, *faces;
+	uint32_t*		attached;  // [n] per root
+	uint32_t*		count;	   // [n] per root
+	uint64_t*		key;	   // [n] per root, min canonical key
+	uint32_t*		cov;	   // [n] per root
+	uint32_t*		roots;	   // attached roots
+	uint32_t*		nroots;
+	uint64_t*		hash;
+	uint32_t		hash_cap;
+	uint64_t*		slot_best; // [2 * nslots]: (cov << 32 | count), then key
+	uint32_t*		slot_root; // [nslots]
+	const uint32_t *cpair_off, *cpair_rec;
+};
+
+__device__ __forceinline__ uint32_t src_verts(const uint32_t* edges, const uint32_t* faces, const PrpHit& h,
+	uint32_t* o)
+{
+	if (h.type == 0)
+		return o[0] = h.src, 1u;
+	if (h.type == 1)
+		return o[0] = edges[2 * h.src], o[1] = edges[2 * h.src + 1], 2u;
+	o[0] = faces[3 * h.src], o[1] = faces[3 * h.src + 1], o[2] = faces[3 * h.src + 2];
+	return 3u;
+}
+__device__ __forceinline__ uint32_t dst_verts(const uint32_t* edges, const uint32_t* faces, const PrpHit& h,
+	uint32_t* o)
+{
+	if (h.type == 0)
+	{
+		o[0] = faces[3 * h.dst], o[1] = faces[3 * h.dst + 1], o[2] = faces[3 * h.dst + 2];
+		return 3u;
+	}
+	if (h.type == 1)
+		return o[0] = edges[2 * h.dst], o[1] = edges[2 * h.dst + 1], 2u;
+	return o[0] = h.dst, 1u;
+}
+
+__device__ __forceinline__ bool has_flag(const uint32_t* off, const uint32_t* flags, uint32_t v, uint32_t f)
+{
+	uint32_t lo = off[v], hi = off[v + 1];
+	while (lo < hi)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (flags[mid] < f)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo < off[v + 1] && flags[lo] == f;
+}
+
+__device__ __forceinline__ uint64_t canonical_key(const PrpHit& h)
+{
+	return ((uint64_t)h.type << 62) | ((uint64_t)(h.src & 0x3fffffffu) << 31) | (uint64_t)(h.dst & 0x7fffffffu);
+}
+
+__global__ void k_prp_uf_init(uint32_t n, uint32_t* parent, uint32_t* attached, uint32_t* count, uint64_t* key,
+	uint32_t* cov)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	parent[i] = i, attached[i] = 0, count[i] = 0, key[i] = ~0ull, cov[i] = 0;
+}
+
+// root of every hit, boundary attachment (is_hit_adj_contour :378-412), size and canonical key per root
+__global__ void k_prp_components(CompArgs a, PrpHit* hs)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= a.n)
+		return;
+	const uint32_t r = uf_find(const_cast<uint32_t*>(a.parent), i);
+	PrpHit		   h = hs[i];
+	h.root = r;
+	hs[i] = h;
+	const uint32_t c = a.c0 + a.slots[h.slot].contour;
+	uint32_t	   sv[3], dv[3];
+	const uint32_t ns = src_verts(a.edges, a.faces, h, sv), nd = dst_verts(a.edges, a.faces, h, dv);
+	bool		   adj = true;
+	for (uint32_t k = 0; k < ns; ++k)
+		adj = adj && has_flag(a.flag_off, a.flags, sv[k], 2 * c);
+	for (uint32_t k = 0; k < nd; ++k)
+		adj = adj && has_flag(a.flag_off, a.flags, dv[k], 2 * c + 1);
+	if (adj)
+		atomicOr(&a.attached[r], 1u);
+	atomicAdd(&a.count[r], 1u);
+	atomicMin((unsigned long long*)&a.key[r], (unsigned long long)canonical_key(h));
+}
+
+__device__ __forceinline__ uint64_t hmix(uint64_t k)
+{
+	k ^= k >> 33;
+	k *= 0xff51afd7ed558ccdull;
+	k ^= k >> 33;
+	k *= 0xc4ceb9fe1a85ec53ull;
+	k ^= k >> 33;
+	return k;
+}
+
+__device__ __forceinline__ void hash_insert(uint64_t* t, uint32_t cap, uint64_t k)
+{
+	for (uint32_t i = (uint32_t)hmix(k) & (cap - 1), n = 0; n < cap; i = (i + 1) & (cap - 1), ++n)
+	{
+		const uint64_t old = atomicCAS((unsigned long long*)&t[i], (unsigned long long)kHashEmpty,
+			(unsigned long long)k);
+		if (old == kHashEmpty || old == k)
+			return;
+	}
+}
+
+__device__ __forceinline__ bool hash_has(const uint64_t* t, uint32_t cap, uint64_t k)
+{
+	for (uint32_t i = (uint32_t)hmix(k) & (cap - 1), n = 0; n < cap; i = (i + 1) & (cap - 1), ++n)
+	{
+		const uint64_t v = t[i];
+		if (v == k)
+			return true;
+		if (v == kHashEmpty)
+			return false;
+	}
+	return false;
+}
+
+__device__ __forceinline__ uint64_t support_key(uint32_t root, uint32_t v, uint32_t side)
+{
+	return ((uint64_t)root << 32) | ((uint64_t)v << 1) | side;
+}
+
+// support vertices of every hit in an attached cluster: (root, vertex, 0 = source / 1 = target)
+__global__ void k_prp_support(CompArgs a)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= a.n)
+		return;
+	const PrpHit h = a.h[i];
+	if (!a.attached[h.root])
+		return;
+	if (h.root == i)
+		a.roots[atomicAdd(a.nroots, 1u)] = i;
+	uint32_t	   sv[3], dv[3];
+	const uint32_t ns = src_verts(a.edges, a.faces, h, sv), nd = dst_verts(a.edges, a.faces, h, dv);
+	for (uint32_t k = 0; k < ns; ++k)
+		hash_insert(a.hash, a.hash_cap, support_key(h.root, sv[k], 0));
+	for (uint32_t k = 0; k < nd; ++k)
+		hash_insert(a.hash, a.hash_cap, support_key(h.root, dv[k], 1));
+}
+
+// EF-anchor coverage of each attached cluster (component_anchor_coverage :3587-3621):
+// pair (edge in role m, face in role m^1) is covered when an edge vertex is a
+// source vertex (m = 0) / target vertex (m = 1) of the cluster and a face vertex
+// is on the other side.  One workgroup per cluster.
+__global__ void k_prp_coverage(CompArgs a)
+{
+	const uint32_t nr = *a.nroots;
+	for (uint32_t b = blockIdx.x; b < nr; b += gridDim.x)
+	{
+		const uint32_t r = a.roots[b];
+		const PrpHit   h = a.h[r];
+		const uint32_t c = a.c0 + a.slots[h.slot].contour;
+		const uint32_t p0 = a.cpair_off[c], p1 = a.cpair_off[c + 1];
+		uint32_t	   cnt = 0;
+		for (uint32_t p = p0 + threadIdx.x; p < p1; p += blockDim.x)
+		{
+			const uint32_t* q = a.cpair_rec + 6 * p;
+			const uint32_t	m = q[5];
+			const uint32_t	es = m == 0 ? 0u : 1u, fs = es ^ 1u;
+			const bool		e_ok = hash_has(a.hash, a.hash_cap, support_key(r, q[0], es))
+				|| hash_has(a.hash, a.hash_cap, support_key(r, q[1], es));
+			const bool f_ok = hash_has(a.hash, a.hash_cap, support_key(r, q[2], fs))
+				|| hash_has(a.hash, a.hash_cap, support_key(r, q[3], fs))
+				|| hash_has(a.hash, a.hash_cap, support_key(r, q[4], fs));
+			cnt += (e_ok && f_ok) ? 1u : 0u;
+		}
+		__shared__ uint32_t part[16];
+		for (int o = 32; o > 0; o >>= 1)
+			cnt += __shfl_xor(cnt, o, kWave);
+		if (__lane_id() == 0)
+			part[threadIdx.x / kWave] = cnt;
+		__syncthreads();
+		if (threadIdx.x == 0)
+		{
+			uint32_t t = 0;
+			for (uint32_t k = 0; k < blockDim.x / kWave; ++k)
+				t += part[k];
+			a.cov[r] = t;
+			atomicMax((unsigned long long*)&a.slot_best[2 * h.slot], ((unsigned long long)t << 32) | a.count[r]);
+		}
+		__syncthreads();
+	}
+}
+
+// best cluster per slot: max coverage, then max hits, then min canonical key
+__global__ void k_prp_select_key(CompArgs a)
+{
+	const uint32_t nr = *a.nroots;
+	const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nr)
+		return;
+	const uint32_t r = a.roots[b];
+	const uint32_t slot = a.h[r].slot;
+	if ((((uint64_t)a.cov[r] << 32) | a.count[r]) == a.slot_best[2 * slot])
+		atomicMin((unsigned long long*)&a.slot_best[2 * slot + 1], (unsigned long long)a.key[r]);
+}
+
+__global__ void k_prp_init_keys(uint32_t ns, uint64_t* best)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < ns)
+		best[2 * j + 1] = ~0ull;
+}
+
+__global__ void k_prp_select_root(CompArgs a)
+{
+	const uint32_t nr = *a.nroots;
+	const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b >= nr)
+		return;
+	const uint32_t r = a.roots[b];
+	const uint32_t slot = a.h[r].slot;
+	if ((((uint64_t)a.cov[r] << 32) | a.count[r]) == a.slot_best[2 * slot] && a.key[r] == a.slot_best[2 * slot + 1])
+		a.slot_root[slot] = r;
+}
+
+// ------------------------------------------------------------------ J(r) per slot (:1883-1954, :1806-1881)
+struct EvalArgs
+{
+	uint32_t		nslots, V, c0;
+	const PrpHit*	h;
+	const uint32_t* range; // [2 * nslots]
+	const uint32_t* slot_root;
+	const uint32_t* cov;
+	const PrpSlot*	slots;
+	const uint32_t *cpair_off, *edges, *faces, *hop_all;
+	const float *	va, *ea, *fa, *mass, *fn, *en, *vn;
+	PrpSummary*		out;
+};
+
+__device__ __forceinline__ float inv_mass_of(const EvalArgs& a, uint32_t v) { return 1.0f / fmaxf(a.mass[v], 1e-8f); }
+
+__global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
+{
+	const uint32_t j = blockIdx.x;
+	if (j >= a.nslots)
+		return;
+	const uint32_t lo = a.range[2 * j], hi = a.range[2 * j + 1];
+	const uint32_t root = a.slot_root[j];
+	const PrpSlot  sl = a.slots[j];
+	const f3	   dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+	const uint32_t* H0 = a.hop_all + (size_t)(2 * sl.contour) * a.V;
+	const uint32_t* H1 = a.hop_all + (size_t)(2 * sl.contour + 1) * a.V;
+	float		   area = 0.0f, vol = 0.0f, cost = 0.0f, depth = 0.0f;
+	uint32_t	   cnt = 0, hs = 0, hd = 0;
+	if (root != kNone)
+		for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+		{
+			const PrpHit h = a.h[i];
+			if (h.root != root)
+				continue;
+			++cnt;
+			float ar;
+			f3	  nn;
+			if (h.type == 0)
+				ar = a.va[h.src], nn = ld3(a.vn, h.src);
+			else if (h.type == 1)
+				ar = a.ea[h.src], nn = ld3(a.en, h.src);
+			else
+				ar = a.fa[h.src], nn = ld3(a.fn, h.src);
+			area += ar;
+			vol += ar * fabsf(dot3(nn, dir)) * h.dist;
+			depth = fmaxf(depth, h.dist);
+			float im;
+			if (h.type == 1)
+			{
+				const float x0 = h.b0, x1 = 1.0f - h.b0, y0 = h.b1, y1 = 1.0f - h.b1;
+				im = x0 * x0 * inv_mass_of(a, a.edges[2 * h.src]) + x1 * x1 * inv_mass_of(a, a.edges[2 * h.src + 1])
+					+ y0 * y0 * inv_mass_of(a, a.edges[2 * h.dst]) + y1 * y1 * inv_mass_of(a, a.edges[2 * h.dst + 1]);
+			}
+			else
+			{
+				const uint32_t f = h.type == 0 ? h.dst : h.src, v = h.type == 0 ? h.src : h.dst;
+				const float	   bz = 1.0f - h.b0 - h.b1;
+				const float	   fpart = h.b0 * h.b0 * inv_mass_of(a, a.faces[3 * f])
+					+ h.b1 * h.b1 * inv_mass_of(a, a.faces[3 * f + 1]) + bz * bz * inv_mass_of(a, a.faces[3 * f + 2]);
+				im = h.type == 0 ? inv_mass_of(a, v) + fpart : fpart + inv_mass_of(a, v);
+			}
+			const float shd = h.dist + 3e-3f;
+			cost += im > 1e-8f ? shd * shd / im : 0.0f;
+			uint32_t	   sv[3], dv[3];
+			const uint32_t ns = src_verts(a.edges, a.faces, h, sv), nd = dst_verts(a.edges, a.faces, h, dv);
+			for (uint32_t k = 0; k < ns; ++k)
+				if (H0[sv[k]] != kHopInf)
+					hs = max(hs, H0[sv[k]]);
+			for (uint32_t k = 0; k < nd; ++k)
+				if (H1[dv[k]] != kHopInf)
+					hd = max(hd, H1[dv[k]]);
+		}
+	area = wave_sum(area);
+	vol = wave_sum(vol);
+	cost = wave_sum(cost);
+	depth = wave_max(depth);
+	for (int o = 32; o > 0; o >>= 1)
+	{
+		cnt += __shfl_xor(cnt, o, kWave);
+		hs = max(hs, (uint32_t)__shfl_xor(hs, o, kWave));
+		hd = max(hd, (uint32_t)__shfl_xor(hd, o, kWave));
+	}
+	__shared__ float	sf[4][4];
+	__shared__ uint32_t su[3][4];
+	const uint32_t		w = threadIdx.x / kWave;
+	if (__lane_id() == 0)
+		sf[0][w] = area, sf[1][w] = vol, sf[2][w] = cost, sf[3][w] = depth, su[0][w] = cnt, su[1][w] = hs,
+		su[2][w] = hd;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		PrpSummary s{};
+		for (uint32_t k = 0; k < blockDim.x / kWave; ++k)
+		{
+			s.area += sf[0][k], s.volume += sf[1][k], s.disp_cost += sf[2][k];
+			s.depth = fmaxf(s.depth, sf[3][k]);
+			s.hit_count += su[0][k];
+			s.max_hop_src = max(s.max_hop_src, su[1][k]);
+			s.max_hop_dst = max(s.max_hop_dst, su[2][k]);
+		}
+		s.raw_hits = hi - lo;
+		s.ef_count = a.cpair_off[a.c0 + sl.contour + 1] - a.cpair_off[a.c0 + sl.contour];
+		s.covered = root != kNone ? a.cov[root] : 0u;
+		s.objective = s.area * s.depth * s.depth;
+		s.root = root;
+		const float cv = s.ef_count ? (float)s.covered / (float)s.ef_count : 0.0f;
+		s.usable = s.hit_count > 0 && s.covered > 0 && isfinite(s.objective) && isfinite(cv) && cv > 0.0f;
+		a.out[j] = s;
+	}
+}
+
+__global__ void k_prp_keys(uint32_t n, const PrpHit* __restrict__ h, uint32_t pb, uint32_t* __restrict__ keys,
+	uint32_t* __restrict__ vals)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	keys[i] = hit_key(h[i].slot, h[i].type, h[i].src, pb);
+	vals[i] = i;
+}
+
+__global__ void k_prp_gather(uint32_t n, const PrpHit* __restrict__ in, const uint32_t* __restrict__ perm,
+	PrpHit* __restrict__ out, uint32_t nslots, uint32_t* __restrict__ range)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const PrpHit h = in[perm[i]];
+	out[i] = h;
+	const uint32_t prev = i > 0 ? in[perm[i - 1]].slot : kNone;
+	const uint32_t next = i + 1 < n ? in[perm[i + 1]].slot : kNone;
+	if (prev != h.slot)
+		range[2 * h.slot] = i;
+	if (next != h.slot)
+		range[2 * h.slot + 1] = i + 1;
+}
+
+__global__ void k_prp_fill_u32(uint32_t n, uint32_t* p, uint32_t v)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+__global__ void k_prp_fill_u64(uint32_t n, uint64_t* p, uint64_t v)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+__global__ void k_prp_fill_u8(uint32_t n, uint8_t* p, uint8_t v)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+// ------------------------------------------------------------------ contour adjacency + responses
+// get_adj_contours (:414-448): another contour o whose flags cover every source vertex
+// with 2o + s and every target vertex with 2o + s ^ 1 (s in {0, 1})
+__global__ void k_prp_adjacent_contours(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ hc,
+	const uint32_t* __restrict__ flag_off, const uint32_t* __restrict__ flags, const uint32_t* __restrict__ edges,
+	const uint32_t* __restrict__ faces, uint32_t nc, uint8_t* __restrict__ adjm)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const PrpHit   x = h[i];
+	const uint32_t c = hc[i];
+	uint32_t	   sv[3], dv[3];
+	const uint32_t ns = src_verts(edges, faces, x, sv), nd = dst_verts(edges, faces, x, dv);
+	for (uint32_t k = flag_off[sv[0]]; k < flag_off[sv[0] + 1]; ++k)
+	{
+		const uint32_t fs = flags[k];
+		if (fs / 2 == c)
+			continue;
+		bool ok = true;
+		for (uint32_t q = 1; q < ns && ok; ++q)
+			ok = has_flag(flag_off, flags, sv[q], fs);
+		for (uint32_t q = 0; q < nd && ok; ++q)
+			ok = has_flag(flag_off, flags, dv[q], fs ^ 1u);
+		if (ok)
+			adjm[(size_t)c * nc + fs / 2] = 1u;
+	}
+}
+
+// make_response_from_min_hitinfo (:2105-2221): one CollisionPairTemplate per kept hit
+__global__ void k_prp_responses(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ hc,
+	const float* __restrict__ cdir, const float* __restrict__ cC, const uint8_t* __restrict__ suppressed,
+	const uint32_t* __restrict__ edges, const uint32_t* __restrict__ faces, const float* __restrict__ va,
+	const float* __restrict__ ea, const float* __restrict__ fa, float kappa, ContactRec* __restrict__ out,
+	uint32_t* __restrict__ out_count, uint32_t cap)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	const bool	   live = i < n && !suppressed[hc[i]];
+	const uint32_t slot = wave_append(out_count, live);
+	if (!live || slot >= cap)
+		return;
+	const PrpHit   x = h[i];
+	const uint32_t c = hc[i];
+	ContactRec	   r{};
+	float		   nn[3] = { cdir[3 * c], cdir[3 * c + 1], cdir[3 * c + 2] };
+	float		   ar;
+	if (x.type == 0)
+	{
+		ar = fmaxf(0.5f * (va[x.src] + fa[x.dst]), 3e-6f);
+		r.idx[0] = x.src, r.idx[1] = faces[3 * x.dst], r.idx[2] = faces[3 * x.dst + 1] | 0x80000000u;
+		r.idx[3] = faces[3 * x.dst + 2];
+	}
+	else if (x.type == 2)
+	{
+		ar = fmaxf(0.5f * (fa[x.src] + va[x.dst]), 3e-6f);
+		r.idx[0] = x.dst, r.idx[1] = faces[3 * x.src], r.idx[2] = faces[3 * x.src + 1] | 0x80000000u;
+		r.idx[3] = faces[3 * x.src + 2];
+		nn[0] = -nn[0], nn[1] = -nn[1], nn[2] = -nn[2];
+	}
+	else
+	{
+		ar = fmaxf(0.5f * (ea[x.src] + ea[x.dst]), 3e-6f);
+		r.idx[0] = edges[2 * x.src], r.idx[1] = edges[2 * x.src + 1];
+		r.idx[2] = edges[2 * x.dst], r.idx[3] = edges[2 * x.dst + 1];
+	}
+	const float stiff = ar * kappa;
+	r.normal[0] = nn[0], r.normal[1] = nn[1], r.normal[2] = nn[2];
+	r.area = ar;
+	r.k1 = stiff * cC[c];
+	r.k2 = stiff;
+	r.bary[0] = x.b0, r.bary[1] = x.b1;
+	out[slot] = r;
+}
+
+// copy the kept hits of the chosen slots (root == slot root) to the final list
+__global__ void k_prp_keep(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ slot_root,
+	const PrpSlot* __restrict__ slots, uint32_t c0, PrpHit* __restrict__ out, uint32_t* __restrict__ out_c,
+	uint32_t* __restrict__ count, uint32_t cap)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	const bool	   keep = i < n && h[i].root == slot_root[h[i].slot];
+	const uint32_t s = wave_append(count, keep);
+	if (keep && s < cap)
+	{
+		out[s] = h[i];
+		out_c[s] = c0 + slots[h[i].slot].contour;
+	}
+}
+
+} // namespace
+
+// ================================================================== host side
+
+namespace
+{
+
+void build_csr(uint32_t n, const std::vector<uint32_t>& src, const std::vector<uint32_t>& dst,
+	std::vector<uint32_t>& off, std::vector<uint32_t>& val, bool unique)
+{
+	off.assign(n + 1, 0);
+	for (uint32_t s : src)
+		off[s + 1]++;
+	for (uint32_t i = 0; i < n; ++i)
+		off[i + 1] += off[i];
+	val.resize(src.size());
+	std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+	for (size_t i = 0; i < src.size(); ++i)
+		val[fill[src[i]]++] = dst[i];
+	uint32_t w = 0;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		const uint32_t b = off[i], e = off[i + 1];
+		std::sort(val.begin() + b, val.begin() + e);
+		const uint32_t row = w;
+		for (uint32_t k = b; k < e; ++k)
+			if (!unique || w == row || val[w - 1] != val[k])
+				val[w++] = val[k];
+		off[i] = row;
+	}
+	off[n] = w;
+	val.resize(w);
+}
+
+template <typename T>
+void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
+{
+	d.reserve(h.size());
+	if (!h.empty())
+		UT_HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+}
+
+// symmetric 3x3 eigenvectors (cyclic Jacobi in FP64), ascending eigenvalues,
+// sign fixed so the largest-magnitude component is positive
+void eigen_sym3(const double S[9], double vec[3][3])
+{
+	double A[9], V[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };
+	std::memcpy(A, S, sizeof(A));
+	for (int sweep = 0; sweep < 64; ++sweep)
+	{
+		if (A[1] * A[1] + A[2] * A[2] + A[5] * A[5] < 1e-300)
+			break;
+		for (int p = 0; p < 2; ++p)
+			for (int q = p + 1; q < 3; ++q)
+			{
+				const double apq = A[3 * p + q];
+				if (std::fabs(apq) < 1e-300)
+					continue;
+				const double th = (A[3 * q + q] - A[3 * p + p]) / (2.0 * apq);
+				const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+				const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+				for (int k = 0; k < 3; ++k)
+				{
+					const double kp = A[3 * k + p], kq = A[3 * k + q];
+					A[3 * k + p] = c * kp - s * kq;
+					A[3 * k + q] = s * kp + c * kq;
+				}
+				for (int k = 0; k < 3; ++k)
+				{
+					const double pk = A[3 * p + k], qk = A[3 * q + k];
+					A[3 * p + k] = c * pk - s * qk;
+					A[3 * q + k] = s * pk + c * qk;
+				}
+				for (int k = 0; k < 3; ++k)
+				{
+					const double kp = V[3 * k + p], kq = V[3 * k + q];
+					V[3 * k + p] = c * kp - s * kq;
+					V[3 * k + q] = s * kp + c * kq;
+				}
+			}
+	}
+	int idx[3] = { 0, 1, 2 };
+	std::sort(idx, idx + 3, [&](int a, int b) { return A[4 * a] < A[4 * b]; });
+	for (int i = 0; i < 3; ++i)
+	{
+		double v[3] = { V[idx[i]], V[3 + idx[i]], V[6 + idx[i]] };
+		int	   big = 0;
+		for (int k = 1; k < 3; ++k)
+			if (std::fabs(v[k]) > std::fabs(v[big]))
+				big = k;
+		const double sg = v[big] < 0 ? -1.0 : 1.0;
+		for (int k = 0; k < 3; ++k)
+			vec[i][k] = sg * v[k];
+	}
+}
+
+void push_dir(std::vector<float>& out, const double* d, double sign)
+{
+	if (!(std::isfinite(d[0]) && std::isfinite(d[1]) && std::isfinite(d[2])))
+		return;
+	const double l2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+	if (!(l2 > 1e-12) || out.size() >= 3 * kMaxDirs)
+		return;
+	const double l = std::sqrt(l2);
+	for (int k = 0; k < 3; ++k)
+		out.push_back((float)(sign * d[k] / l));
+}
+
+uint32_t initial_k(uint32_t nb, uint32_t diameter)
+{
+	if (diameter == 0)
+		return 3;
+	const uint32_t ka = (uint32_t)std::ceil(std::sqrt((float)nb));
+	return std::max(3u, std::min(ka, diameter));
+}
+
+uint32_t grow_k(uint32_t cur, uint32_t diam)
+{
+	if (cur >= diam)
+		return cur;
+	return (uint32_t)std::min<uint64_t>(diam, std::max<uint64_t>(1u, (uint64_t)cur * 2u));
+}
+
+struct ComboState
+{
+	bool	   present = false, converged = false, pruned = false, eligible = false;
+	PrpSummary s{};
+	uint32_t   ks = 0, kd = 0;
+};
+
+float response_objective(const PrpSummary& s)
+{
+	const float cov = s.ef_count ? (float)s.covered / (float)s.ef_count : 0.0f;
+	if (!std::isfinite(s.objective) || !std::isfinite(cov) || cov <= 0.0f)
+		return INFINITY;
+	const double o = (double)s.objective / ((double)cov * cov);
+	return std::isfinite(o) && o <= (double)FLT_MAX ? (float)o : INFINITY;
+}
+
+int scalar_cmp(float l, float r)
+{
+	const bool lf = std::isfinite(l), rf = std::isfinite(r);
+	if (lf != rf)
+		return lf ? -1 : 1;
+	if (!lf)
+		return 0;
+	const float tol = 1e-7f + 1e-5f * std::max(std::fabs(l), std::fabs(r));
+	if (l + tol < r)
+		return -1;
+	if (r + tol < l)
+		return 1;
+	return 0;
+}
+
+// prp_response_is_better (intersection_resolver2.h:852-884)
+bool better(const PrpSummary& a, uint32_t ca, const PrpSummary& b, uint32_t cb)
+{
+	int c = scalar_cmp(response_objective(a), response_objective(b));
+	if (c)
+		return c < 0;
+	if ((c = scalar_cmp(a.depth, b.depth)))
+		return c < 0;
+	if ((c = scalar_cmp(a.disp_cost, b.disp_cost)))
+		return c < 0;
+	if (a.hit_count != b.hit_count)
+		return a.hit_count < b.hit_count;
+	return ca < cb;
+}
+
+} // namespace
+
+void PrpResolver::init(Context* c, uint32_t nv_, uint32_t ne_, uint32_t nf_, const uint32_t* edges_,
+	const uint32_t* faces_, const uint32_t* fae_, const uint32_t* eaf_, const float* rest_x_, const float* va_,
+	const float* ea_, const float* fa_, const float* mass_, const PrpParams& p)
+{
+	ctx = c;
+	nv = nv_, ne = ne_, nf = nf_;
+	edges = edges_, faces = faces_, fae = fae_, eaf = eaf_, rest_x = rest_x_;
+	va = va_, ea = ea_, fa = fa_, mass = mass_;
+	params = p;
+	UT_REQUIRE(p.total_dirs >= 1 && p.total_dirs <= kMaxDirs, "prp: total_dirs must be in [1, 64]");
+	UT_REQUIRE(p.batch_contours >= 1, "prp: batch_contours must be positive");
+	const uint32_t mx = std::max({ nv, ne, nf, 2u });
+	prim_bits = 32 - __builtin_clz(mx - 1);
+	UT_REQUIRE(prim_bits <= 26, "prp: mesh too large for 32-bit hit keys");
+	slot_bits = 30 - prim_bits;
+	const hipStream_t	  s = ctx->stream;
+	std::vector<uint32_t> he(2 * (size_t)ne), hf(3 * (size_t)nf);
+	UT_HIP_CHECK(hipMemcpyAsync(he.data(), edges, 8 * (size_t)ne, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipMemcpyAsync(hf.data(), faces, 12 * (size_t)nf, hipMemcpyDeviceToHost, s));
+	h_va.resize(nv);
+	UT_HIP_CHECK(hipMemcpyAsync(h_va.data(), va, 4 * (size_t)nv, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
+	std::vector<uint32_t> src, dst, off, val;
+	for (uint32_t e = 0; e < ne; ++e)
+		src.push_back(he[2 * e]), dst.push_back(he[2 * e + 1]), src.push_back(he[2 * e + 1]), dst.push_back(he[2 * e]);
+	build_csr(nv, src, dst, off, val, true);
+	upload(vv_off, off, s), upload(vv, val, s);
+	src.clear(), dst.clear();
+	for (uint32_t e = 0; e < ne; ++e)
+		src.push_back(he[2 * e]), dst.push_back(e), src.push_back(he[2 * e + 1]), dst.push_back(e);
+	build_csr(nv, src, dst, off, val, true);
+	upload(ve_off, off, s), upload(ve, val, s);
+	src.clear(), dst.clear();
+	for (uint32_t f = 0; f < nf; ++f)
+		for (int k = 0; k < 3; ++k)
+			src.push_back(hf[3 * f + k]), dst.push_back(f);
+	build_csr(nv, src, dst, off, val, true);
+	upload(vf_off, off, s), upload(vf, val, s);
+	normals.reserve(3 * ((size_t)nf + ne + nv));
+	counters.reserve(8);
+	status.reserve(2);
+	spill.reserve((size_t)kRayBlocks * kTravThreads * (kStackTotal - kStackLds));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void PrpResolver::build_contour_tables(const ContourBuilder& ct)
+{
+	const hipStream_t s = ctx->stream;
+	total_pairs = ct.num_total;
+	nc = ct.num_contours;
+	const uint32_t		  N = total_pairs;
+	std::vector<uint32_t> words(12 * (size_t)N), cidx(N), midx(N), adjn(N), adj(12 * (size_t)N);
+	if (N)
+	{
+		UT_HIP_CHECK(hipMemcpyAsync(words.data(), ct.pairs.get(), 48 * (size_t)N, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(cidx.data(), ct.contour_index.get(), 4 * (size_t)N, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(midx.data(), ct.mesh_index.get(), 4 * (size_t)N, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(adjn.data(), ct.adj_n.get(), 4 * (size_t)N, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(adj.data(), ct.adj.get(), 48 * (size_t)N, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipStreamSynchronize(s));
+	}
+	h_pair_words = words;
+	h_contour_of_pair = cidx;
+	h_mesh_of_pair = midx;
+	h_adj = adj;
+	h_adjn = adjn;
+	// pairs of each contour, ascending pair index
+	h_cpair_off.assign(nc + 1, 0);
+	for (uint32_t p = 0; p < N; ++p)
+		h_cpair_off[cidx[p] + 1]++;
+	for (uint32_t c = 0; c < nc; ++c)
+		h_cpair_off[c + 1] += h_cpair_off[c];
+	h_cpairs.resize(N);
+	std::vector<uint32_t> fill(h_cpair_off.begin(), h_cpair_off.end() - 1);
+	for (uint32_t p = 0; p < N; ++p)
+		h_cpairs[fill[cidx[p]]++] = p;
+	std::vector<uint32_t> rec(6 * (size_t)N);
+	for (uint32_t i = 0; i < N; ++i)
+	{
+		const uint32_t p = h_cpairs[i];
+		for (int k = 0; k < 5; ++k)
+			rec[6 * (size_t)i + k] = words[12 * (size_t)p + k];
+		rec[6 * (size_t)i + 5] = midx[p];
+	}
+	// boundary vertices per (contour, role) from every pair; flags from non-loop pairs
+	std::vector<uint32_t> bs, bd, fs, fd;
+	for (uint32_t p = 0; p < N; ++p)
+	{
+		const uint32_t* q = &words[12 * (size_t)p];
+		const uint32_t	m = midx[p], c = cidx[p];
+		for (int k = 0; k < 5; ++k)
+			bs.push_back(2 * c + (k < 2 ? m : m ^ 1u)), bd.push_back(q[k]);
+		if (q[0] == q[1])
+			continue;
+		for (int k = 0; k < 5; ++k)
+			fs.push_back(q[k]), fd.push_back(2 * c + (k < 2 ? m : m ^ 1u));
+	}
+	std::vector<uint32_t> foff, fval;
+	build_csr(2 * nc, bs, bd, h_b_off, h_b_verts, true);
+	build_csr(nv, fs, fd, foff, fval, true);
+	upload(cpair_off, h_cpair_off, s);
+	upload(cpair_rec, rec, s);
+	upload(b_off, h_b_off, s);
+	upload(b_verts, h_b_verts, s);
+	upload(flag_off, foff, s);
+	upload(flags, fval, s);
+	if (fval.empty())
+	{
+		flags.reserve(1);
+	}
+}
+
+void PrpResolver::directions(uint32_t c0, uint32_t n)
+{
+	// global area-weighted centroid (computed once per resolve)
+	const double* g = global_c;
+	for (uint32_t lc = 0; lc < n; ++lc)
+	{
+		const uint32_t c = c0 + lc;
+		const uint32_t p0 = h_cpair_off[c], p1 = h_cpair_off[c + 1], np = p1 - p0;
+		auto		   X = [&](uint32_t v, int k) { return h_x[3 * (size_t)v + k]; };
+		auto		   pos = [&](uint32_t p, float out[3])
+		{
+			const uint32_t* q = &h_pair_words[12 * (size_t)p];
+			float			b0;
+			std::memcpy(&b0, q + 5, 4);
+			for (int k = 0; k < 3; ++k)
+				out[k] = X(q[0], k) * b0 + X(q[1], k) * (1.0f - b0);
+		};
+		double c_[3] = { 0, 0, 0 }, u[3] = { 0, 0, 0 }, sw = 0;
+		for (uint32_t i = p0; i < p1; ++i)
+		{
+			const uint32_t p = h_cpairs[i];
+			float		   pp[3], w = 0.0f;
+			pos(p, pp);
+			for (uint32_t j = 0; j < h_adjn[p]; ++j)
+			{
+				float qq[3];
+				pos(h_adj[12 * (size_t)p + j], qq);
+				const float dx = qq[0] - pp[0], dy = qq[1] - pp[1], dz = qq[2] - pp[2];
+				w += std::sqrt(dx * dx + dy * dy + dz * dz);
+			}
+			w *= 0.5f;
+			for (int k = 0; k < 3; ++k)
+				c_[k] += (double)pp[k] * w, u[k] += pp[k];
+			sw += w;
+		}
+		for (int k = 0; k < 3; ++k)
+			c_[k] = sw == 0.0 ? u[k] / np : c_[k] / sw;
+		double S[9] = { 0 }, mc[2][3] = { { 0 } };
+		for (int r = 0; r < 2; ++r)
+		{
+			double ws = 0;
+			for (uint32_t i = h_b_off[2 * c + r]; i < h_b_off[2 * c + r + 1]; ++i)
+			{
+				const uint32_t v = h_b_verts[i];
+				const double   w = std::max((double)h_va[v], 1e-8);
+				double		   d[3];
+				for (int k = 0; k < 3; ++k)
+					mc[r][k] += (double)X(v, k) * w, d[k] = X(v, k) - c_[k];
+				for (int a = 0; a < 3; ++a)
+					for (int b = 0; b < 3; ++b)
+						S[3 * a + b] += w * d[a] * d[b];
+				ws += w;
+			}
+			for (int k = 0; k < 3; ++k)
+				mc[r][k] = ws > 1e-8 ? mc[r][k] / ws : c_[k];
+		}
+		double E[3][3] = { { 0, 1, 0 }, { 1, 0, 0 }, { 0, 0, 1 } };
+		double tr = S[0] + S[4] + S[8], fro = 0;
+		bool   ok = true;
+		for (int k = 0; k < 9; ++k)
+			fro += S[k] * S[k], ok = ok && std::isfinite(S[k]);
+		if (ok && tr >= 1e-10 && fro >= 1e-20)
+			eigen_sym3(S, E);
+		std::vector<float> d;
+		for (int i = 0; i < 3; ++i)
+			push_dir(d, E[i], 1.0), push_dir(d, E[i], -1.0);
+		for (int r = 0; r < 2; ++r)
+		{
+			double dd[3] = { c_[0] - mc[r][0], c_[1] - mc[r][1], c_[2] - mc[r][2] };
+			const double l2 = dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2];
+			if (!(std::isfinite(l2) && l2 > 1e-12))
+				std::memcpy(dd, E[r + 1], sizeof(dd));
+			push_dir(d, dd, 1.0);
+		}
+		const double dg[3] = { c_[0] - g[0], c_[1] - g[1], c_[2] - g[2] };
+		push_dir(d, dg, 1.0), push_dir(d, dg, -1.0);
+		const uint32_t total = params.total_dirs;
+		for (uint32_t i = 0; total > 10 && d.size() / 3 < total && i < 4 * kMaxDirs; ++i)
+		{
+			const uint32_t m = total > 10 ? total - 10 : 1;
+			const double   zz = 1.0 - (2.0 * i + 1.0) / m;
+			const double   rr = std::sqrt(std::max(0.0, 1.0 - zz * zz));
+			const double   ph = i * 2.399963229728653;
+			const double   f[3] = { rr * std::cos(ph), zz, rr * std::sin(ph) };
+			push_dir(d, f, 1.0);
+		}
+		h_dirs[c] = d;
+	}
+}
+
+void PrpResolver::distances(uint32_t c0, uint32_t n)
+{
+	const hipStream_t s = ctx->stream;
+	const size_t	  V = nv;
+	hop.reserve(2 * (size_t)params.batch_contours * V);
+	geo.reserve(2 * (size_t)params.batch_contours * V);
+	queue.reserve(4 * (size_t)params.batch_contours * V);
+	inq.reserve(2 * (size_t)params.batch_contours * V);
+	own.reserve(2 * (size_t)params.batch_contours * V);
+	diameter.reserve(2 * (size_t)params.batch_contours);
+	k_prp_distances<<<2 * n, kBfsThreads, 0, s>>>(nv, c0, b_off.get(), b_verts.get(), vv_off.get(), vv.get(), rest_x,
+		hop.get(), geo.get(), queue.get(), inq.get(), diameter.get());
+	const uint32_t m = 2 * n * nv;
+	k_prp_fill_u8<<<div_up(m, 256), 256, 0, s>>>(m, own.get(), (uint8_t)kOwnUnknown);
+	UT_LAUNCH_CHECK();
+	h_diam.resize(2 * n);
+	UT_HIP_CHECK(hipMemcpyAsync(h_diam.data(), diameter.get(), 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void PrpResolver::prepare(const float* x, const ContourBuilder& ct, uint32_t c0, uint32_t n)
+{
+	const hipStream_t s = ctx->stream;
+	x_dev = x;
+	build_contour_tables(ct);
+	UT_REQUIRE(c0 + n <= nc, "prp: contour range out of bounds");
+	UT_REQUIRE(n <= params.batch_contours, "prp: batch larger than batch_contours");
+	h_x.resize(3 * (size_t)nv);
+	UT_HIP_CHECK(hipMemcpyAsync(h_x.data(), x, 12 * (size_t)nv, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
+	double gw = 0;
+	global_c[0] = global_c[1] = global_c[2] = 0.0;
+	for (uint32_t v = 0; v < nv; ++v)
+	{
+		const double w = std::max((double)h_va[v], 1e-8);
+		for (int k = 0; k < 3; ++k)
+			global_c[k] += w * h_x[3 * (size_t)v + k];
+		gw += w;
+	}
+	for (int k = 0; k < 3; ++k)
+		global_c[k] /= gw;
+	float* fn = normals.get();
+	float* en = fn + 3 * (size_t)nf;
+	float* vn = en + 3 * (size_t)ne;
+	k_prp_face_normals<<<div_up(nf, 256), 256, 0, s>>>(nf, x, faces, fn);
+	k_prp_edge_vert_normals<<<div_up(ne + nv, 256), 256, 0, s>>>(ne, nv, eaf, vf_off.get(), vf.get(), fa, fn, en, vn);
+	UT_LAUNCH_CHECK();
+	h_dirs.assign(nc, {});
+	batch0 = c0, batch_n = n;
+	if (n == 0)
+		return;
+	distances(c0, n);
+	directions(c0, n);
+}
+
+void PrpResolver::ensure_hits(uint32_t cap)
+{
+	if (cap <= hit_cap)
+		return;
+	hit_cap = cap;
+	hits.reserve(cap), hits_sorted.reserve(cap);
+	for (int i = 0; i < 2; ++i)
+		sort_k[i].reserve(cap), sort_v[i].reserve(cap);
+	sort_h.reserve(radix_sort_hist_size(cap));
+	comp_info.reserve(5 * (size_t)cap);
+	comp_key.reserve(cap);
+	roots.reserve(cap);
+}
+
+uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree, Lbvh& edge_tree,
+	const std::vector<uint32_t>& kr, std::vector<PrpSummary>& out)
+{
+	const hipStream_t s = ctx->stream;
+	const uint32_t	  ns = (uint32_t)sl.size(), n2 = 2 * batch_n, V = nv, E = ne, F = nf;
+	out.assign(ns, PrpSummary{});
+	if (ns == 0)
+		return 0;
+	UT_REQUIRE(ns <= (1u << slot_bits) && ns <= 4096, "prp: too many slots in one evaluation");
+	UT_REQUIRE(kr.size() == n2, "prp: one k per (contour, role)");
+	upload(slots, sl, s);
+	upload(kring, kr, s);
+	cand.reserve((size_t)n2 * (V + E));
+	cand_cnt.reserve(2 * (size_t)n2);
+	target_cnt.reserve(n2);
+	req_off.reserve(3 * ((size_t)ns + 1));
+	slot_range.reserve(2 * (size_t)ns);
+	slot_best.reserve(2 * (size_t)ns);
+	slot_root.reserve(ns);
+	summary.reserve(ns);
+	if (hit_cap == 0)
+		ensure_hits(1u << 20);
+	// ownership and candidates
+	k_prp_own<<<dim3(div_up(V, 256), n2), 256, 0, s>>>(
+		V, batch0, n2, kring.get(), hop.get(), geo.get(), x_dev, b_off.get(), b_verts.get(), params.tau, own.get());
+	k_prp_fill_u32<<<div_up(2 * n2, 256), 256, 0, s>>>(2 * n2, cand_cnt.get(), 0u);
+	k_prp_fill_u32<<<div_up(n2, 256), 256, 0, s>>>(n2, target_cnt.get(), 0u);
+	k_prp_cand_lists<<<dim3(div_up(std::max(V, E), 256), n2), 256, 0, s>>>(
+		V, E, n2, kring.get(), hop.get(), own.get(), edges, cand.get(), cand_cnt.get());
+	k_prp_any_faces<<<dim3(div_up(F, 256), n2), 256, 0, s>>>(
+		F, n2, kring.get(), V, hop.get(), own.get(), faces, target_cnt.get());
+	k_prp_slot_prep<<<ns, 256, 0, s>>>(V, E, ns, slots.get(), cand.get(), cand_cnt.get(), x_dev, edges);
+	k_prp_request_offsets<<<1, 1024, 0, s>>>(ns, slots.get(), cand_cnt.get(), target_cnt.get(), req_off.get());
+	UT_LAUNCH_CHECK();
+	uint32_t nh = 0;
+	for (int attempt = 0; attempt < 8; ++attempt)
+	{
+		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop.get(), own.get(), kring.get(), x_dev, edges,
+			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr };
+		k_prp_fill_u32<<<1, 64, 0, s>>>(4, counters.get(), 0u);
+		k_prp_fill_u32<<<1, 64, 0, s>>>(2, status.get(), 1u);
+		ra.nodes = face_tree.nodes.get();
+		k_prp_ray_vert<0><<<kRayBlocks, kTravThreads, 0, s>>>(ra);
+		k_prp_ray_vert<2><<<kRayBlocks, kTravThreads, 0, s>>>(ra);
+		ra.nodes = edge_tree.nodes.get();
+		k_prp_ray_edge<<<kRayBlocks, kTravThreads, 0, s>>>(ra);
+		UT_LAUNCH_CHECK();
+		uint32_t st[2];
+		UT_HIP_CHECK(hipMemcpyAsync(&nh, counters.get(), 4, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(st, status.get(), 8, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipStreamSynchronize(s));
+		UT_REQUIRE(st[0] == 1u, "prp: ray traversal revisited a node (corrupt tree)");
+		UT_REQUIRE(st[1] == 0u, "prp: ray traversal stack overflow");
+		if (nh <= hit_cap)
+			break;
+		ensure_hits(std::max(nh + nh / 4, 2 * hit_cap));
+	}
+	UT_REQUIRE(nh <= hit_cap, "prp: hit buffer could not be grown");
+	last_raw_hits = nh;
+	k_prp_fill_u32<<<div_up(2 * ns, 256), 256, 0, s>>>(2 * ns, slot_range.get(), 0u);
+	k_prp_fill_u32<<<div_up(ns, 256), 256, 0, s>>>(ns, slot_root.get(), kNone);
+	k_prp_fill_u64<<<div_up(2 * ns, 256), 256, 0, s>>>(2 * ns, slot_best.get(), 0ull);
+	UT_LAUNCH_CHECK();
+	if (nh > 0)
+	{
+		// sort by (slot, type, source primitive)
+		k_prp_keys<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), prim_bits, sort_k[0].get(), sort_v[0].get());
+		RadixSortBuffers rb;
+		rb.keys[0] = sort_k[0].get(), rb.keys[1] = sort_k[1].get();
+		rb.vals[0] = sort_v[0].get(), rb.vals[1] = sort_v[1].get();
+		rb.hist = sort_h.get();
+		const uint32_t slot_need = 32 - __builtin_clz(std::max(ns, 2u) - 1);
+		radix_sort_pairs(rb, nh, (int)(slot_need + 2 + prim_bits), s);
+		const uint32_t* keys = rb.keys[rb.result];
+		const uint32_t* perm = rb.vals[rb.result];
+		k_prp_gather<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), perm, hits_sorted.get(), ns, slot_range.get());
+		uint32_t* parent = comp_info.get();
+		uint32_t* attached = parent + hit_cap;
+		uint32_t* cnt = attached + hit_cap;
+		uint32_t* cov = cnt + hit_cap;
+		k_prp_uf_init<<<div_up(nh, 256), 256, 0, s>>>(nh, parent, attached, cnt, comp_key.get(), cov);
+		HitArgs ha{ nh, V, prim_bits, hits_sorted.get(), keys, parent, vv_off.get(), vv.get(), ve_off.get(), ve.get(),
+			vf_off.get(), vf.get(), edges, faces, fae, eaf };
+		k_prp_hit_union<<<div_up(nh, 256), 256, 0, s>>>(ha);
+		// support hash: at most 6 entries per hit, load factor <= 1/2
+		uint32_t hc = 1;
+		while (hc < 12u * nh)
+			hc <<= 1;
+		if (hc > hash_cap)
+		{
+			hash_cap = hc;
+			hash.reserve(hash_cap);
+		}
+		k_prp_fill_u64<<<div_up(hash_cap, 256), 256, 0, s>>>(hash_cap, hash.get(), kHashEmpty);
+		k_prp_fill_u32<<<1, 64, 0, s>>>(1, counters.get() + 1, 0u);
+		UT_LAUNCH_CHECK();
+		CompArgs ca{ nh, ns, V, batch0, hits_sorted.get(), parent, slots.get(), flag_off.get(), flags.get(), edges,
+			faces, attached, cnt, comp_key.get(), cov, roots.get(), counters.get() + 1, hash.get(), hash_cap,
+			slot_best.get(), slot_root.get(), cpair_off.get(), cpair_rec.get() };
+		k_prp_components<<<div_up(nh, 256), 256, 0, s>>>(ca, hits_sorted.get());
+		k_prp_support<<<div_up(nh, 256), 256, 0, s>>>(ca);
+		k_prp_coverage<<<1024, 256, 0, s>>>(ca);
+		// key stage: slot_best[2j + 1] starts at the largest key
+		k_prp_init_keys<<<div_up(ns, 256), 256, 0, s>>>(ns, slot_best.get());
+		k_prp_select_key<<<div_up(nh, 256), 256, 0, s>>>(ca);
+		k_prp_select_root<<<div_up(nh, 256), 256, 0, s>>>(ca);
+		UT_LAUNCH_CHECK();
+	}
+	float* fn = normals.get();
+	float* en = fn + 3 * (size_t)nf;
+	float* vn = en + 3 * (size_t)ne;
+	EvalArgs ea_{ ns, V, batch0, hits_sorted.get(), slot_range.get(), slot_root.get(),
+		comp_info.get() + 3 * (size_t)hit_cap, slots.get(), cpair_off.get(), edges, faces, hop.get(), va, ea, fa, mass,
+		fn, en, vn, summary.get() };
+	k_prp_eval<<<ns, 256, 0, s>>>(ea_);
+	UT_LAUNCH_CHECK();
+	UT_HIP_CHECK(hipMemcpyAsync(out.data(), summary.get(), sizeof(PrpSummary) * ns, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
+	return nh;
+}
+
+uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& face_tree, Lbvh& edge_tree,
+	std::vector<PrpChoice>& choices, ContactRec* out, uint32_t* out_count, uint32_t cap, PrpStats* stats)
+{
+	using clk = std::chrono::steady_clock;
+	const auto		  t0 = clk::now();
+	const hipStream_t s = ctx->stream;
+	PrpStats		  st;
+	prepare(x, ct, 0, 0);
+	choices.assign(nc, PrpChoice{});
+	const uint32_t active = std::min(nc, params.max_contours);
+	st.contours = active;
+	final_hits.reserve(std::max<size_t>(1024, final_cap));
+	final_contour.reserve(std::max<size_t>(1024, final_cap));
+	if (final_cap < 1024)
+		final_cap = 1024;
+	uint32_t n_final = 0;
+	const auto t1 = clk::now();
+	double	   ms_levels = 0.0, ms_final = 0.0;
+	for (uint32_t c0 = 0; c0 < active; c0 += params.batch_contours)
+	{
+		const uint32_t n = std::min(params.batch_contours, active - c0);
+		batch0 = c0, batch_n = n;
+		const auto tl0 = clk::now();
+		distances(c0, n);
+		directions(c0, n);
+		// adaptive k-ring driver (evaluate_current_runtime_directions_template :1058-1230)
+		std::vector<uint32_t>				 ks(n), kd(n), D0(n), D1(n);
+		std::vector<std::vector<ComboState>> R(n);
+		std::vector<uint8_t>				 live(n, 1);
+		for (uint32_t lc = 0; lc < n; ++lc)
+		{
+			const uint32_t c = c0 + lc;
+			D0[lc] = h_diam[2 * lc], D1[lc] = h_diam[2 * lc + 1];
+			ks[lc] = initial_k(h_b_off[2 * c + 1] - h_b_off[2 * c], D0[lc]);
+			kd[lc] = initial_k(h_b_off[2 * c + 2] - h_b_off[2 * c + 1], D1[lc]);
+			R[lc].assign(h_dirs[c].size() / 3, ComboState{});
+			choices[c].levels = 0;
+		}
+		for (uint32_t level = 0; level < 2 * 32 + 1; ++level)
+		{
+			std::vector<PrpSlot> sl;
+			for (uint32_t lc = 0; lc < n; ++lc)
+				if (live[lc])
+					for (uint32_t ci = 0; ci < R[lc].size(); ++ci)
+						if (!R[lc][ci].converged && !R[lc][ci].pruned)
+						{
+							const float* d = &h_dirs[c0 + lc][3 * ci];
+							sl.push_back(PrpSlot{ lc, ci, ks[lc], kd[lc], { d[0], d[1], d[2] }, 0.0f, 0.0f, 0u });
+						}
+			if (sl.empty())
+				break;
+			std::vector<uint32_t> kr(2 * n);
+			for (uint32_t lc = 0; lc < n; ++lc)
+				kr[2 * lc] = ks[lc], kr[2 * lc + 1] = kd[lc];
+			std::vector<PrpSummary> res;
+			// evaluate in chunks that fit the slot key bits
+			for (size_t b = 0; b < sl.size(); b += 4096)
+			{
+				std::vector<PrpSlot>	part(sl.begin() + b, sl.begin() + std::min(sl.size(), b + 4096));
+				std::vector<PrpSummary> pr;
+				st.hits += eval_slots(part, face_tree, edge_tree, kr, pr);
+				res.insert(res.end(), pr.begin(), pr.end());
+			}
+			st.slots += (uint32_t)sl.size();
+			++st.levels;
+			for (size_t j = 0; j < sl.size(); ++j)
+			{
+				const uint32_t lc = sl[j].contour, ci = sl[j].combo;
+				ComboState&	   r = R[lc][ci];
+				r.present = true;
+				r.s = res[j];
+				r.ks = ks[lc], r.kd = kd[lc];
+				const bool full = ks[lc] >= D0[lc] && kd[lc] >= D1[lc];
+				const bool src_ok = r.s.max_hop_src < ks[lc] || ks[lc] >= D0[lc];
+				const bool dst_ok = r.s.max_hop_dst < kd[lc] || kd[lc] >= D1[lc];
+				r.eligible = r.s.usable && src_ok && dst_ok;
+				r.converged = r.eligible || full;
+			}
+			for (uint32_t lc = 0; lc < n; ++lc)
+			{
+				if (!live[lc])
+					continue;
+				choices[c0 + lc].levels = level + 1;
+				auto& rr = R[lc];
+				bool  has = false;
+				float best = INFINITY;
+				for (auto& r : rr)
+					if (r.eligible && r.s.usable)
+						has = true, best = std::min(best, response_objective(r.s));
+				if (has)
+					for (auto& r : rr)
+						if (!r.converged && r.present && response_objective(r.s) >= best)
+							r.pruned = true;
+				bool pending = false, gs = false, gd = false;
+				for (auto& r : rr)
+				{
+					if (r.converged || r.pruned)
+						continue;
+					pending = true;
+					gs = gs || (ks[lc] < D0[lc] && (!r.s.usable || r.s.max_hop_src >= ks[lc]));
+					gd = gd || (kd[lc] < D1[lc] && (!r.s.usable || r.s.max_hop_dst >= kd[lc]));
+				}
+				if (!pending)
+				{
+					live[lc] = 0;
+					continue;
+				}
+				const uint32_t ns_ = gs ? grow_k(ks[lc], D0[lc]) : ks[lc];
+				const uint32_t nd_ = gd ? grow_k(kd[lc], D1[lc]) : kd[lc];
+				if (ns_ <= ks[lc] && nd_ <= kd[lc])
+				{
+					live[lc] = 0; // cannot grow further (the reference raises here)
+					continue;
+				}
+				ks[lc] = ns_, kd[lc] = nd_;
+			}
+		}
+		// selection (prp_select_response)
+		std::vector<PrpSlot>  chosen;
+		std::vector<uint32_t> kr(2 * n);
+		for (uint32_t lc = 0; lc < n; ++lc)
+		{
+			int sel = -1;
+			for (uint32_t ci = 0; ci < R[lc].size(); ++ci)
+			{
+				const ComboState& r = R[lc][ci];
+				if (r.eligible && r.s.usable && (sel < 0 || better(r.s, ci, R[lc][sel].s, (uint32_t)sel)))
+					sel = (int)ci;
+			}
+			PrpChoice& ch = choices[c0 + lc];
+			kr[2 * lc] = ks[lc], kr[2 * lc + 1] = kd[lc];
+			if (sel < 0)
+				continue;
+			const ComboState& r = R[lc][sel];
+			const float*	  d = &h_dirs[c0 + lc][3 * sel];
+			ch.combo = sel, ch.ks = r.ks, ch.kd = r.kd, ch.hit_count = r.s.hit_count, ch.covered = r.s.covered;
+			ch.ef_count = r.s.ef_count, ch.depth = r.s.depth, ch.objective = r.s.objective, ch.area = r.s.area;
+			ch.disp_cost = r.s.disp_cost;
+			std::memcpy(ch.dir, d, 12);
+			kr[2 * lc] = r.ks, kr[2 * lc + 1] = r.kd;
+			chosen.push_back(PrpSlot{ lc, (uint32_t)sel, r.ks, r.kd, { d[0], d[1], d[2] }, 0.0f, 0.0f, 0u });
+		}
+		const auto tl1 = clk::now();
+		ms_levels += std::chrono::duration<double, std::milli>(tl1 - tl0).count();
+		if (chosen.empty())
+			continue;
+		// final pass: the kept hits of every chosen slot
+		std::vector<PrpSummary> fin;
+		const uint32_t			nh = eval_slots(chosen, face_tree, edge_tree, kr, fin);
+		if (n_final + nh > final_cap)
+		{
+			// grow, keeping what is there
+			const uint32_t		 ncap = std::max(2 * final_cap, n_final + nh + nh / 4);
+			DevBuf<PrpHit>		 nfh;
+			DevBuf<uint32_t>	 nfc;
+			nfh.reserve(ncap), nfc.reserve(ncap);
+			if (n_final)
+			{
+				UT_HIP_CHECK(hipMemcpyAsync(nfh.get(), final_hits.get(), sizeof(PrpHit) * n_final,
+					hipMemcpyDeviceToDevice, s));
+				UT_HIP_CHECK(hipMemcpyAsync(nfc.get(), final_contour.get(), 4 * (size_t)n_final,
+					hipMemcpyDeviceToDevice, s));
+			}
+			UT_HIP_CHECK(hipStreamSynchronize(s));
+			std::swap(final_hits.ptr, nfh.ptr), std::swap(final_hits.cap, nfh.cap);
+			std::swap(final_contour.ptr, nfc.ptr), std::swap(final_contour.cap, nfc.cap);
+			final_cap = ncap;
+		}
+		if (nh)
+		{
+			k_prp_fill_u32<<<1, 64, 0, s>>>(1, counters.get() + 2, n_final);
+			k_prp_keep<<<div_up(nh, 256), 256, 0, s>>>(nh, hits_sorted.get(), slot_root.get(), slots.get(), c0,
+				final_hits.get(), final_contour.get(), counters.get() + 2, final_cap);
+			UT_LAUNCH_CHECK();
+			UT_HIP_CHECK(hipMemcpyAsync(&n_final, counters.get() + 2, 4, hipMemcpyDeviceToHost, s));
+			UT_HIP_CHECK(hipStreamSynchronize(s));
+		}
+		ms_final += std::chrono::duration<double, std::milli>(clk::now() - tl1).count();
+	}
+	// contour suppression (:450-590) and responses
+	const auto tf0 = clk::now();
+	std::vector<uint8_t> sup(std::max(nc, 1u), 0);
+	if (n_final && nc)
+	{
+		adjm.reserve((size_t)nc * nc);
+		const uint32_t m = nc * nc;
+		k_prp_fill_u8<<<div_up(m, 256), 256, 0, s>>>(m, adjm.get(), 0);
+		k_prp_adjacent_contours<<<div_up(n_final, 256), 256, 0, s>>>(n_final, final_hits.get(), final_contour.get(),
+			flag_off.get(), flags.get(), edges, faces, nc, adjm.get());
+		UT_LAUNCH_CHECK();
+		std::vector<uint8_t> A((size_t)nc * nc);
+		UT_HIP_CHECK(hipMemcpyAsync(A.data(), adjm.get(), A.size(), hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipStreamSynchronize(s));
+		std::vector<uint32_t> order;
+		for (uint32_t c = 0; c < active; ++c)
+		{
+			bool any = false;
+			for (uint32_t o = 0; o < nc; ++o)
+				any = any || (o != c && A[(size_t)c * nc + o]);
+			if (any)
+				order.push_back(c);
+		}
+		std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b)
+			{
+				if (choices[a].area != choices[b].area)
+					return choices[a].area > choices[b].area;
+				return a < b;
+			});
+		for (uint32_t o : order)
+		{
+			if (sup[o])
+				continue;
+			for (uint32_t a = 0; a < active; ++a)
+				if (a != o && A[(size_t)o * nc + a])
+					sup[a] = 1;
+		}
+		std::vector<float> cdir(3 * (size_t)nc, 0.0f), cC((size_t)nc, 0.0f);
+		for (uint32_t c = 0; c < nc; ++c)
+		{
+			choices[c].suppressed = sup[c];
+			std::memcpy(&cdir[3 * (size_t)c], choices[c].dir, 12);
+			cC[c] = std::max(-choices[c].depth - 3e-3f, -params.response_depth);
+		}
+		upload(contour_c, cC, s);
+		upload(contour_dir, cdir, s);
+		upload(suppressed, sup, s);
+		k_prp_responses<<<div_up(n_final, 256), 256, 0, s>>>(n_final, final_hits.get(), final_contour.get(),
+			contour_dir.get(), contour_c.get(), suppressed.get(), edges, faces, va, ea, fa, params.kappa, out,
+			out_count, cap);
+		UT_LAUNCH_CHECK();
+	}
+	else
+		for (uint32_t c = 0; c < nc; ++c)
+			choices[c].suppressed = 0;
+	uint32_t nresp = 0;
+	for (uint32_t i = 0; i < n_final; ++i)
+		(void)i;
+	// responses written = kept hits of non-suppressed contours (count on the host)
+	if (n_final)
+	{
+		std::vector<uint32_t> fc(n_final);
+		UT_HIP_CHECK(hipMemcpyAsync(fc.data(), final_contour.get(), 4 * (size_t)n_final, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipStreamSynchronize(s));
+		for (uint32_t c : fc)
+			nresp += sup[c] ? 0u : 1u;
+	}
+	last_final_hits = n_final;
+	st.responses = nresp;
+	st.ms_prepare = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
+	st.ms_levels = (float)ms_levels;
+	st.ms_final = (float)(ms_final + std::chrono::duration<double, std::milli>(clk::now() - tf0).count());
+	if (stats)
+		*stats = st;
+	return nresp;
+}
+
+} // namespace ut

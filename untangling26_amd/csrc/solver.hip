@@ -269,7 +269,8 @@ StepReport NewtonSolver::physics_step_GPU()
 		pcg.contact_count = counts.get() + 3;
 		pcg.contact_cap = cap_contacts;
 		const PcgReport pr =
-			pcg.solve(p.pcg, fp64 ? (const void*)vals64.get() : (const void*)vals32.get(), rhs.get(), dx.get());
+			pcg.solve(p.pcg, fp64 ? (const void*)vals64.get() : (const void*)vals32.get(), rhs.get(), dx.get(),
+				p.pcg.precision == kPcgMixed ? vals64.get() : nullptr);
 		rep.pcg_iterations += pr.iterations;
 		rep.pcg_refinements += pr.refinements;
 		rep.pcg_relative_residual = pr.relative_residual;

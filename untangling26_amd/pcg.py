@@ -39,15 +39,22 @@ class ConjugateGradientSolver:
             pass
 
     def device_solve(self, vals, b, x, precision=MIXED, preconditioner=BLOCK_JACOBI, max_iters=100, rel_tol=1e-6,
-                     inner_tol=1e-2, max_refine=20, regularization=0.0, check_every=8):
-        """Solves A x = b; returns the PcgSolveReport as a dict (also kept in last_solve_report)."""
+                     inner_tol=1e-2, max_refine=20, regularization=0.0, check_every=8, vals64=None):
+        """Solves A x = b; returns the PcgSolveReport as a dict (also kept in last_solve_report).
+        In MIXED mode `vals64` (the FP64 matrix) makes the refinement residual FP64, so the
+        result solves the FP64 system; without it the FP32 matrix is refined against."""
         want = torch.float64 if precision == FP64 else torch.float32
         assert vals.dtype == want and vals.is_contiguous(), f"values must be {want}"
         assert b.dtype == torch.float64 and x.dtype == torch.float64
         p = PcgParams(precision, preconditioner, max_iters, rel_tol, inner_tol, max_refine, regularization,
                       check_every)
         r = PcgReport()
-        rc = lib().ut_pcg_solve(self.handle, C.byref(p), ptr(vals), ptr(b), ptr(x), C.byref(r))
+        if vals64 is not None:
+            assert precision == MIXED and vals64.dtype == torch.float64 and vals64.is_contiguous()
+            rc = lib().ut_pcg_solve_refined(self.handle, C.byref(p), ptr(vals), ptr(vals64), ptr(b), ptr(x),
+                                            C.byref(r))
+        else:
+            rc = lib().ut_pcg_solve(self.handle, C.byref(p), ptr(vals), ptr(b), ptr(x), C.byref(r))
         self.ctx.check(rc)
         rep = {k: getattr(r, k) for k, _ in PcgReport._fields_}
         rep["converged"] = bool(rep["converged"])
