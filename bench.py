@@ -1,14 +1,19 @@
-"""Benchmark of the per-step untangling hot path on one MI355X.
+"""Benchmark of the per-step untangling path on one MI355X.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--config folded_cloth]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--dirs 64] [--no-cpu]
 
-A step is one pass of the per-step collision pipeline over the folded
-256x256-vertex cloth (130,050 triangles), inputs resident in HBM:
-  face + edge LBVH rebuild (global box, Morton, radix sort, Karras),
-  leaf update + refit of both trees,
-  VF and EE broad-phase queries,
-  EF intersection detection (edge-vs-face traversal fused with the EF test).
-Prints one JSON line (see the "roofline" and "cpu_baseline" keys).
+Workload: the folded self-intersecting cloth, 256 x 256 vertices (130,050
+triangles, 2 m side, figure-eight crossing, seed 0), repaired with the
+untangling configuration of untangling26.cli.configure_solver (quasi-static,
+no gravity, quadratic contacts, PRP responses, CCD line search, one Newton
+iteration per step).  A step is one physics_step_gpu call: LBVH rebuild,
+proximity + EF detection, contours, PRP (per-contour ray casting of candidate
+directions, clustering, J(r), responses), contact filtering, assembly, PCG to
+relative residual 1e-6, CCD line search.  Everything is resident in HBM.
+
+The K timed steps start from the same tangled state for both PCG precisions:
+mixed (FP32 SpMV / vectors with FP64 refinement, the reported value) and plain
+FP64; the FP64 rate and the speedup go into `config`.
 """
 from __future__ import annotations
 
@@ -28,11 +33,74 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="folded_cloth")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="cloth resolution (vertices per side)")
+    ap.add_argument("--dirs", type=int, default=64, help="candidate directions per contour")
+    ap.add_argument("--pcg-iters", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     return ap.parse_args()
+
+
+def make_solver(args, precision):
+    import numpy as np
+
+    from untangling26_amd import scenes
+    from untangling26_amd.lcs_py import NewtonSolver
+
+    m = scenes.folded_cloth(0, n=args.n)
+    rest = scenes.folded_cloth_rest(n=args.n)
+    s = NewtonSolver()
+    s.init_device("hip")
+    s.register_world_data(s.create_world_data_from_array("folded_cloth", m.x, m.faces, rest))
+    cfg = s.get_config()
+    # untangling26/cli.py:93-118
+    cfg.use_quasi_static_mode = True
+    cfg.gravity = (0.0, 0.0, 0.0)
+    cfg.contact_energy_type = 0
+    cfg.use_untangling_PRP = True
+    cfg.untangling_response_depth = 1.0
+    cfg.nonlinear_iter_count = 1
+    cfg.use_ccd_linesearch = True
+    cfg.pcg_iter_count = args.pcg_iters
+    cfg.pcg_rel_tol = 1e-6
+    cfg.pcg_precision = precision
+    cfg.prp_directions = args.dirs
+    s.init_solver()
+    return s, m, np.asarray(rest, np.float32)
+
+
+def run_leg(args, precision):
+    import torch
+
+    s, m, rest = make_solver(args, precision)
+    x0 = s.positions().clone()
+    v0 = torch.zeros_like(x0)
+    for _ in range(args.warmup):
+        s.physics_step_gpu()
+    s.set_state(x0, v0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = [s.physics_step_gpu() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return s, m, rest, reps, dt
+
+
+def summarize(reps):
+    import numpy as np
+
+    keys = ("ms_detect", "ms_contours", "ms_prp", "ms_assemble", "ms_pcg", "ms_ccd", "ms_total")
+    out = {k: float(np.mean([r[k] for r in reps])) for k in keys}
+    out["ms_contours"] -= out["ms_prp"]  # the PRP time is inside the contour window
+    out["pcg_iterations"] = float(np.mean([r["pcg_iterations"] for r in reps]))
+    out["pcg_converged_steps"] = int(sum(1 for r in reps if r["pcg_converged"]))
+    out["ef_pairs"] = [int(r["ef_total"]) for r in reps]
+    out["contours"] = [int(r["contours"]) for r in reps]
+    out["prp_responses"] = float(np.mean([r["prp_responses"] for r in reps]))
+    out["prp_levels"] = float(np.mean([r["prp_levels"] for r in reps]))
+    return out
 
 
 def main():
@@ -42,175 +110,82 @@ def main():
               "(the sharding design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
-    from untangling26_amd import scenes
-    from untangling26_amd.lbvh import LBVH, TREE_EDGE, TREE_FACE
-    from untangling26_amd.intersection import ef_detect_fused
-    from untangling26_amd.runtime import Context
+    from untangling26_amd.lcs_py import PCG_FP64, PCG_MIXED
 
-    ctx = Context(0)
-    mesh = scenes.CONFIGS[args.config](0)
-    d_hat_np, th_np = scenes.per_vertex(mesh)
+    s, m, rest, reps, dt = run_leg(args, PCG_MIXED)
+    mixed = summarize(reps)
+    value = args.steps / dt
+    # roofline of the PCG SpMV (the kernel of the PCG iteration loop), measured live
+    pat = s._setup.pattern
+    nnzb = int(len(pat.keys))
+    spmv_ms = s.profile_spmv(100)
+    spmv_bytes = nnzb * (9 * 4 + 4) + m.nv * (12 + 12)
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+    free_step = next((i + 1 for i, r in enumerate(reps) if r["ef_total"] == 0), None)
+    data = s.get_intersection_contour_data()
 
-    def dev(a):
-        t = torch.from_numpy(np.ascontiguousarray(a))
-        if t.dtype == torch.uint32:
-            t = t.view(torch.int32)
-        return t.cuda()
-
-    X, F, E = dev(mesh.x), dev(mesh.faces), dev(mesh.edges)
-    D, T = dev(d_hat_np), dev(th_np)
-    face_tree = LBVH(ctx, TREE_FACE, mesh.nf)
-    edge_tree = LBVH(ctx, TREE_EDGE, mesh.ne)
-    counts = torch.zeros(8, dtype=torch.int32, device="cuda")  # vf, ee, ef broad, ef narrow
-    caps = {"vf": 1 << 20, "ee": 1 << 21, "ef": 1 << 22, "efn": 1 << 18}
-    bufs = {}
-
-    def alloc():
-        bufs["vf"] = torch.empty(2 * caps["vf"], dtype=torch.int32, device="cuda")
-        bufs["ee"] = torch.empty(2 * caps["ee"], dtype=torch.int32, device="cuda")
-        bufs["ef"] = torch.empty(2 * caps["ef"], dtype=torch.int32, device="cuda")
-        bufs["efp"] = torch.empty(12 * caps["efn"], dtype=torch.int32, device="cuda")
-        bufs["efi"] = torch.empty(2 * caps["efn"], dtype=torch.int32, device="cuda")
-
-    alloc()
-
-    def st_build():
-        face_tree.reduce_face_tree_aabb(X, F)
-        face_tree.construct_tree()
-        edge_tree.reduce_edge_tree_aabb(X, E)
-        edge_tree.construct_tree()
-
-    def st_refit():
-        face_tree.update_face_tree_leave_aabb(T, X, X, F)
-        face_tree.refit()
-        edge_tree.update_edge_tree_leave_aabb(T, X, X, E)
-        edge_tree.refit()
-
-    def st_query():
-        counts[0:2].zero_()
-        face_tree.broad_phase_query_from_verts(X, X, counts[0:1], bufs["vf"], D, T)
-        edge_tree.broad_phase_query_from_edges(X, X, E, counts[1:2], bufs["ee"], D, T)
-
-    def st_ef():
-        counts[3:4].zero_()
-        ef_detect_fused(ctx, face_tree, X, E, F, D, T, counts[3:4], bufs["efp"], bufs["efi"])
-
-    def step():
-        st_build()
-        st_refit()
-        st_query()
-        st_ef()
-
-    # size the lists once (the reference's resize-and-retry), then run without host syncs
-    for _ in range(4):
-        step()
-        c = counts.cpu().tolist()
-        need = {"vf": c[0], "ee": c[1], "efn": c[3]}
-        if all(need[k] <= caps[k] for k in need):
-            break
-        for k in need:
-            caps[k] = max(caps[k], 2 * need[k])
-        alloc()
-    c = counts.cpu().tolist()
-    n_vf, n_ee, n_ef = c[0], c[1], c[3]
-    # candidate count of the fused EF kernel (for its byte count): one broad query
-    counts[2:3].zero_()
-    face_tree.broad_phase_query_from_edges_unculled(X, X, E, counts[2:3], bufs["ef"], D, T)
-    n_ef_broad = int(counts[2].item())
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record()
-    for _ in range(args.steps):
-        step()
-    t1.record()
-    torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1) / args.steps
-
-    # per-stage average durations (HIP events on the launch stream)
-    def timed(fn, reps=20):
-        fn()
-        torch.cuda.synchronize()
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(reps):
-            fn()
-        b.record()
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps
-
-    stage_ms = {"build": timed(st_build), "refit": timed(st_refit), "query": timed(st_query), "ef_dcd": timed(st_ef)}
-
-    # Roofline of the fused EF kernel (broad traversal + narrow test): per
-    # candidate pair 5 x 12 B vertex positions + 8 B edge + 12 B face topology
-    # = 80 B, per edge query 8 B edge + 24 B positions + 8 B d_hat/thickness,
-    # plus 56 B per emitted pair (48 B EfPair + 8 B ids); node visits (64 B)
-    # hit in L2 and are not counted.
-    ef_bytes = 80.0 * n_ef_broad + 40.0 * mesh.ne + 56.0 * n_ef
-    ef_ms = stage_ms["ef_dcd"]
-    achieved = ef_bytes / (ef_ms * 1e-3) / 1e9
+    fp64 = None
+    if not args.no_fp64:
+        s64, _, _, reps64, dt64 = run_leg(args, PCG_FP64)
+        fp64 = summarize(reps64)
+        fp64["steps_per_s"] = args.steps / dt64
+        del s64
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(mesh, d_hat_np, th_np)
+        sys.path.insert(0, str(ROOT / "oracle"))
+        from cpu_step import cpu_step_baseline
 
-    out = {
-        "metric": "steps/s (per-step collision pipeline: LBVH rebuild+refit, VF/EE queries, fused EF detection)",
-        "value": 1000.0 / ms,
+        cpu = cpu_step_baseline(m, rest, s, args)
+
+    line = {
+        "metric": "steps/s to intersection-free, 131k-tri fold; mixed-prec vs FP64 PCG speedup",
+        "value": round(value, 4),
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": round(1e3 * dt / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded folded cloth)",
-        "config": {"workload": f"{mesh.name}: {mesh.nf} tris, {mesh.ne} edges, {mesh.nv} verts",
-                   "vf_pairs": n_vf, "ee_pairs": n_ee, "ef_broad_pairs": n_ef_broad, "ef_pairs": n_ef,
-                   "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()}},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_ef_detect"},
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded folded cloth, figure-eight crossing)",
+        "config": {
+            "workload": f"folded_cloth {args.n}x{args.n} ({m.nf} tris) untangling repair step, "
+                        f"{args.dirs} directions/contour, PCG rel. residual 1e-6",
+            "pcg": "mixed FP32 + FP64 refinement",
+            "fp64_steps_per_s": None if fp64 is None else round(fp64["steps_per_s"], 4),
+            "mixed_vs_fp64_speedup": None if fp64 is None else round(value / fp64["steps_per_s"], 3),
+            "steps_to_intersection_free": free_step,
+            "ef_pairs_first_last": [mixed["ef_pairs"][0], mixed["ef_pairs"][-1]],
+            "contours_first_last": [mixed["contours"][0], mixed["contours"][-1]],
+            "final_num_pairs": int(data["num_pairs"]),
+            "stage_ms_mixed": {k: round(v, 3) for k, v in mixed.items() if k.startswith("ms_")},
+            "stage_ms_fp64": None if fp64 is None else {k: round(v, 3) for k, v in fp64.items()
+                                                         if k.startswith("ms_")},
+            "pcg_iterations_mixed": mixed["pcg_iterations"],
+            "pcg_iterations_fp64": None if fp64 is None else fp64["pcg_iterations"],
+            "pcg_converged_steps_mixed": mixed["pcg_converged_steps"],
+            "prp_responses_per_step": mixed["prp_responses"],
+            "prp_levels_per_step": mixed["prp_levels"],
+        },
+        "roofline": {
+            "kernel": "k_pcg_spmv<float> (SELL-64 3x3 BSR SpMV of the PCG iteration)",
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": spmv_bytes,
+            "avg_launch_ms": round(spmv_ms, 5),
+        },
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
-
-
-def cpu_baseline(mesh, d_hat, th):
-    """Reference-algorithm CPU restatement of the same step (oracle, 1 thread)."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle
-
-    def one():
-        ft = oracle.lbvh_build(1, mesh.x, mesh.faces)
-        et = oracle.lbvh_build(2, mesh.x, mesh.edges)
-        fl = oracle.leaf_boxes(1, mesh.x, mesh.x, mesh.faces, th)
-        el = oracle.leaf_boxes(2, mesh.x, mesh.x, mesh.edges, th)
-        qv = oracle.query_boxes(0, mesh.x, mesh.x, None, d_hat, th)
-        qe = oracle.query_boxes(1, mesh.x, mesh.x, mesh.edges, d_hat, th)
-        oracle.lbvh_query(ft, fl, qv)
-        oracle.lbvh_query(et, el, qe, cull=True)
-        ef = oracle.lbvh_query(ft, fl, qe)
-        oracle.ef_dcd(ef, mesh.x, mesh.edges, mesh.faces, d_hat)
-
-    t = time.perf_counter()
-    n = 0
-    while True:
-        one()
-        n += 1
-        if time.perf_counter() - t > 10.0 or n >= 20:
-            break
-    dt = (time.perf_counter() - t) / n
-    return {"value": 1.0 / dt, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full steps of the same pipeline on the same mesh (C oracle, -O2, 1 thread)"}
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -268,8 +268,20 @@ typedef struct ut_assemble_args {
 	double*			vals;
 	float*			vals32;
 	double*			energy;		 /* optional double[4]: inertia, stretch, bending */
+	int				quasi_static;		  /* 1: no inertia gradient / energy (soft_inertia_energy.cpp:68-80) */
+	int				qs_no_inertia_hessian; /* with quasi_static: drop the m / h^2 diagonal as well */
 } ut_assemble_args;
 UT_API int ut_assemble(ut_ctx* ctx, const ut_assemble_args* args);
+
+/* PRP parameters (SceneParams PRP fields, scene_params.h:56-80). */
+typedef struct ut_prp_params {
+	uint32_t max_contours;	 /* untangling_process_contours_count (scene_params.h:67), default 256 */
+	uint32_t total_dirs;	 /* directions per contour: 10 = get_raycasting_direction; more append a Fibonacci set */
+	uint32_t batch_contours; /* contours evaluated together (PRP_cpu_contour_batch_size, scene_params.h:68) */
+	float	 response_depth; /* untangling_response_depth */
+	float	 kappa;			 /* stiffness_untangling */
+	float	 tau;			 /* PRP_intrinsic_tau */
+} ut_prp_params;
 
 /* ---- Newton step: lcs::NewtonSolver (newton_solver.h:283-326) ------------------ */
 
@@ -299,6 +311,7 @@ typedef struct ut_solver_desc {
 	const uint32_t* diag_slot;
 	const uint32_t* face_slots;	 /* [9F] slot of block (a, b) of face vertices a, b */
 	const uint32_t* hinge_slots; /* [16H] */
+	const float*	rest_x;		 /* [3V] rest positions for the PRP rest geodesics; NULL = x0 */
 } ut_solver_desc;
 
 /* SceneParams fields the step uses (scene_params.h:24-157). */
@@ -313,6 +326,9 @@ typedef struct ut_solver_params {
 	int			  use_ccd;
 	int			  use_untangling; /* EF detection + contours each iteration */
 	ut_pcg_params pcg;
+	int			  use_prp;		/* use_untangling_PRP: PRP responses for the contours */
+	int			  quasi_static; /* use_quasi_static_mode with QuasiStaticInertiaMode::HessianOnly */
+	ut_prp_params prp;
 } ut_solver_params;
 
 typedef struct ut_step_report {
@@ -323,6 +339,8 @@ typedef struct ut_step_report {
 	float	 toi;
 	double	 energy[3]; /* inertia, stretch, bending at the last assembly */
 	float	 ms_detect, ms_contours, ms_assemble, ms_pcg, ms_ccd, ms_total;
+	uint32_t prp_responses, prp_contours, prp_levels, prp_hits;
+	float	 ms_prp;
 } ut_step_report;
 
 UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,
@@ -336,20 +354,19 @@ UT_API int ut_solver_get_state(ut_solver* s, float* x, float* v);
 UT_API int ut_solver_set_state(ut_solver* s, const float* x, const float* v);
 /* UT_CT_* output of the contours built in the last step (see ut_contours_get). */
 UT_API int ut_solver_contour_get(ut_solver* s, int what, void* dst);
+/* Measurement hook (no counterpart in the reference): average duration in ms,
+ * from HIP events on the solver's stream, of `reps` launches of one kernel of
+ * the step on the state of the last step.  UT_PROFILE_PCG_SPMV: the PCG
+ * iteration's SELL SpMV on the last assembled matrix (FP32 values unless the
+ * PCG runs in FP64). */
+enum { UT_PROFILE_PCG_SPMV = 0 };
+UT_API int ut_solver_profile(ut_solver* s, int what, uint32_t reps, float* avg_ms);
 
 /* ---- PRP untangling: IntersectionResolver::host_resolve with PRP
  *      (intersection_resolver.cpp:2131-2217 -> intersection_resolver2.cpp:2750-4069) ---- */
 
 typedef struct ut_prp ut_prp;
 
-typedef struct ut_prp_params {
-	uint32_t max_contours;	 /* untangling_process_contours_count (scene_params.h:67), default 256 */
-	uint32_t total_dirs;	 /* directions per contour: 10 = get_raycasting_direction; more append a Fibonacci set */
-	uint32_t batch_contours; /* contours evaluated together (PRP_cpu_contour_batch_size, scene_params.h:68) */
-	float	 response_depth; /* untangling_response_depth */
-	float	 kappa;			 /* stiffness_untangling */
-	float	 tau;			 /* PRP_intrinsic_tau */
-} ut_prp_params;
 
 /* PRPMinContourHitInfo essentials of the chosen direction of one contour. */
 typedef struct ut_prp_choice {
@@ -395,11 +412,12 @@ UT_API int ut_prp_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, con
 UT_API int ut_prp_destroy(ut_prp* p);
 UT_API int ut_prp_set_params(ut_prp* p, const ut_prp_params* params);
 /* host_resolve_intersections_PRP: contours of ct at positions x (device), rays
- * through the LBVHs built at x; response pairs (CollisionPairTemplate, 64 B)
- * appended to out_pairs at device counter out_count.  choices[c] for every
- * contour (choices_cap entries).  Synchronises. */
+ * through the LBVHs built at x; the response pairs (CollisionPairTemplate, 64 B)
+ * are copied to out_pairs (device, out_cap records) and *num_pairs receives
+ * their count; returns 1 (capacity) when out_cap was too small.  choices[c] for
+ * every contour (choices_cap entries).  Synchronises. */
 UT_API int ut_prp_resolve(ut_prp* p, const float* x, ut_contours* ct, ut_lbvh* face_tree, ut_lbvh* edge_tree,
-	void* out_pairs, uint32_t* out_count, uint32_t out_cap, ut_prp_choice* choices, uint32_t choices_cap,
+	void* out_pairs, uint32_t out_cap, uint32_t* num_pairs, ut_prp_choice* choices, uint32_t choices_cap,
 	ut_prp_stats* stats);
 /* Parity hooks: prepare contour batch [c0, c0 + n) (distances, directions),
  * read its directions and distances, evaluate slots and read the sorted hits of

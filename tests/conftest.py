@@ -24,3 +24,8 @@ def ctx():
     c = Context(0)
     yield c
     c.synchronize()
+
+
+def pytest_runtest_logstart(nodeid, location):
+    # name the case before it runs, so that a run cut short still shows what was running
+    print(f"\n[start] {nodeid}", flush=True)

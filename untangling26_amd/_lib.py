@@ -48,10 +48,11 @@ SIGNATURES = {
     "ut_pcg_destroy": (C.c_int, [vp]),
     "ut_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "ut_pcg_solve_refined": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "ut_solver_profile": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
     "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 11 + [C.POINTER(vp)]),
     "ut_prp_destroy": (C.c_int, [vp]),
     "ut_prp_set_params": (C.c_int, [vp, vp]),
-    "ut_prp_resolve": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp]),
+    "ut_prp_resolve": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_uint32, u32p, vp, C.c_uint32, vp]),
     "ut_prp_prepare": (C.c_int, [vp, vp, vp, C.c_uint32, C.c_uint32]),
     "ut_prp_directions": (C.c_int, [vp, C.c_uint32, vp, u32p]),
     "ut_prp_distances": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, u32p]),
@@ -95,7 +96,13 @@ class AssembleArgs(C.Structure):
                 ("x_tilde", vp), ("mass", vp), ("inv_h2", C.c_double), ("faces", vp), ("dm", vp), ("face_area", vp),
                 ("mu_lambda", vp), ("face_slots", vp), ("hinges", vp), ("rest_angle", vp), ("bend_stiff_area", vp),
                 ("hinge_slots", vp), ("contacts", vp), ("contact_count", vp), ("contact_cap", C.c_uint32),
-                ("diag_slot", vp), ("grad", vp), ("vals", vp), ("vals32", vp), ("energy", vp)]
+                ("diag_slot", vp), ("grad", vp), ("vals", vp), ("vals32", vp), ("energy", vp),
+                ("quasi_static", C.c_int), ("qs_no_inertia_hessian", C.c_int)]
+
+
+class PrpParams(C.Structure):
+    _fields_ = [("max_contours", C.c_uint32), ("total_dirs", C.c_uint32), ("batch_contours", C.c_uint32),
+                ("response_depth", C.c_float), ("kappa", C.c_float), ("tau", C.c_float)]
 
 
 class SolverDesc(C.Structure):
@@ -104,13 +111,14 @@ class SolverDesc(C.Structure):
                           "rest_vert_area", "rest_edge_area", "rest_face_area", "dm", "mu_lambda", "hinges",
                           "rest_angle", "bend_stiff_area")] + [
         ("nslices", C.c_uint32), ("nslots", C.c_uint32)] + [
-        (n, vp) for n in ("slice_off", "col", "diag_slot", "face_slots", "hinge_slots")]
+        (n, vp) for n in ("slice_off", "col", "diag_slot", "face_slots", "hinge_slots", "rest_x")]
 
 
 class SolverParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("kappa", C.c_double), ("energy", C.c_int),
                 ("damping", C.c_double), ("nonlinear_iters", C.c_int), ("use_self_collision", C.c_int),
-                ("use_ccd", C.c_int), ("use_untangling", C.c_int), ("pcg", PcgParams)]
+                ("use_ccd", C.c_int), ("use_untangling", C.c_int), ("pcg", PcgParams), ("use_prp", C.c_int),
+                ("quasi_static", C.c_int), ("prp", PrpParams)]
 
 
 class StepReport(C.Structure):
@@ -118,12 +126,9 @@ class StepReport(C.Structure):
                 ("ef_total", C.c_uint32), ("contours", C.c_uint32), ("pcg_iterations", C.c_uint32),
                 ("pcg_refinements", C.c_uint32), ("pcg_relative_residual", C.c_double), ("pcg_converged", C.c_int),
                 ("toi", C.c_float), ("energy", C.c_double * 3), ("ms_detect", C.c_float), ("ms_contours", C.c_float),
-                ("ms_assemble", C.c_float), ("ms_pcg", C.c_float), ("ms_ccd", C.c_float), ("ms_total", C.c_float)]
-
-
-class PrpParams(C.Structure):
-    _fields_ = [("max_contours", C.c_uint32), ("total_dirs", C.c_uint32), ("batch_contours", C.c_uint32),
-                ("response_depth", C.c_float), ("kappa", C.c_float), ("tau", C.c_float)]
+                ("ms_assemble", C.c_float), ("ms_pcg", C.c_float), ("ms_ccd", C.c_float), ("ms_total", C.c_float),
+                ("prp_responses", C.c_uint32), ("prp_contours", C.c_uint32), ("prp_levels", C.c_uint32),
+                ("prp_hits", C.c_uint32), ("ms_prp", C.c_float)]
 
 
 class PrpChoice(C.Structure):

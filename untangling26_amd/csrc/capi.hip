@@ -613,6 +613,18 @@ ut::SolverParams to_params(const ut_solver_params& q)
 	p.use_ccd = q.use_ccd;
 	p.use_untangling = q.use_untangling;
 	p.pcg = to_pcg(q.pcg);
+	p.use_prp = q.use_prp;
+	p.quasi_static = q.quasi_static;
+	p.qs_inertia_hessian = 1;
+	if (q.prp.total_dirs)
+	{
+		p.prp.max_contours = q.prp.max_contours;
+		p.prp.total_dirs = q.prp.total_dirs;
+		p.prp.batch_contours = q.prp.batch_contours ? q.prp.batch_contours : 32;
+		p.prp.response_depth = q.prp.response_depth;
+		p.prp.kappa = q.prp.kappa;
+		p.prp.tau = q.prp.tau;
+	}
 	return p;
 }
 } // namespace
@@ -658,6 +670,7 @@ UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_so
 			d.diag_slot = desc->diag_slot;
 			d.face_slots = desc->face_slots;
 			d.hinge_slots = desc->hinge_slots;
+			d.rest_x = desc->rest_x;
 			h->s.init(&ctx->c, d, to_params(*params), x0, v0);
 		});
 	if (rc != ut::kOk)
@@ -708,6 +721,11 @@ UT_API int ut_solver_step(ut_solver* s, ut_step_report* report)
 		report->ms_pcg = r.ms_pcg;
 		report->ms_ccd = r.ms_ccd;
 		report->ms_total = r.ms_total;
+		report->prp_responses = r.prp_responses;
+		report->prp_contours = r.prp_contours;
+		report->prp_levels = r.prp_levels;
+		report->prp_hits = r.prp_hits;
+		report->ms_prp = r.ms_prp;
 	}
 	return ut::kOk;
 }
@@ -716,7 +734,12 @@ UT_API int ut_solver_set_params(ut_solver* s, const ut_solver_params* params)
 {
 	if (!s || !params)
 		return ut::kInvalid;
-	return guarded(s->s.ctx, [&] { s->s.p = to_params(*params); });
+	return guarded(s->s.ctx,
+		[&]
+		{
+			s->s.p = to_params(*params);
+			s->s.prp.params = s->s.p.prp;
+		});
 }
 
 UT_API int ut_solver_get_state(ut_solver* s, float* x, float* v)
@@ -795,6 +818,8 @@ UT_API int ut_assemble(ut_ctx* ctx, const ut_assemble_args* q)
 			a.x_tilde = q->x_tilde;
 			a.mass = q->mass;
 			a.inv_h2 = q->inv_h2;
+			a.inertia_grad_scale = q->quasi_static ? 0.0 : 1.0;
+			a.inertia_hess_scale = (q->quasi_static && q->qs_no_inertia_hessian) ? 0.0 : 1.0;
 			a.faces = q->faces;
 			a.dm = q->dm;
 			a.face_area = q->face_area;
@@ -892,18 +917,26 @@ UT_API int ut_prp_set_params(ut_prp* p, const ut_prp_params* params)
 }
 
 UT_API int ut_prp_resolve(ut_prp* p, const float* x, ut_contours* ct, ut_lbvh* face_tree, ut_lbvh* edge_tree,
-	void* out_pairs, uint32_t* out_count, uint32_t out_cap, ut_prp_choice* choices, uint32_t choices_cap,
+	void* out_pairs, uint32_t out_cap, uint32_t* num_pairs, ut_prp_choice* choices, uint32_t choices_cap,
 	ut_prp_stats* stats)
 {
-	if (!p || !x || !ct || !face_tree || !edge_tree || (!out_pairs && out_cap) || !out_count)
+	if (!p || !x || !ct || !face_tree || !edge_tree || (!out_pairs && out_cap) || !num_pairs)
 		return ut::kInvalid;
-	return guarded(p->r.ctx,
-		[&]
-		{
-			std::vector<ut::PrpChoice> ch;
-			ut::PrpStats			   st;
-			p->r.resolve(x, ct->b, face_tree->b, edge_tree->b, ch, reinterpret_cast<ut::ContactRec*>(out_pairs),
-				out_count, out_cap, &st);
+	uint32_t nresp = 0;
+	int		 rc = guarded(p->r.ctx,
+		 [&]
+		 {
+			 std::vector<ut::PrpChoice> ch;
+			 ut::PrpStats			   st;
+			 nresp = p->r.resolve(x, ct->b, face_tree->b, edge_tree->b, ch, &st);
+			 *num_pairs = nresp;
+			 const uint32_t n = std::min(nresp, out_cap);
+			 if (n)
+			 {
+				 UT_HIP_CHECK(hipMemcpyAsync(out_pairs, p->r.responses.get(), sizeof(ut::ContactRec) * n,
+					 hipMemcpyDeviceToDevice, p->r.ctx->stream));
+				 UT_HIP_CHECK(hipStreamSynchronize(p->r.ctx->stream));
+			 }
 			if (choices)
 				for (uint32_t c = 0; c < ch.size() && c < choices_cap; ++c)
 				{
@@ -922,6 +955,9 @@ UT_API int ut_prp_resolve(ut_prp* p, const float* x, ut_contours* ct, ut_lbvh* f
 				stats->ms_prepare = st.ms_prepare, stats->ms_levels = st.ms_levels, stats->ms_final = st.ms_final;
 			}
 		});
+	if (rc != ut::kOk)
+		return rc;
+	return nresp > out_cap ? ut::kCapacity : ut::kOk;
 }
 
 UT_API int ut_prp_prepare(ut_prp* p, const float* x, ut_contours* ct, uint32_t c0, uint32_t n)
@@ -1017,5 +1053,18 @@ UT_API int ut_prp_normals(ut_prp* p, float* out)
 			const size_t n = 3 * ((size_t)p->r.nf + p->r.ne + p->r.nv);
 			UT_HIP_CHECK(hipMemcpyAsync(out, p->r.normals.get(), 4 * n, hipMemcpyDeviceToHost, p->r.ctx->stream));
 			UT_HIP_CHECK(hipStreamSynchronize(p->r.ctx->stream));
+		});
+}
+
+UT_API int ut_solver_profile(ut_solver* s, int what, uint32_t reps, float* avg_ms)
+{
+	if (!s || !avg_ms || what != UT_PROFILE_PCG_SPMV)
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			const bool fp64 = s->s.p.pcg.precision == ut::kPcgFp64;
+			*avg_ms = s->s.pcg.profile_spmv(fp64 ? (const void*)s->s.vals64.get() : (const void*)s->s.vals32.get(),
+				!fp64, reps);
 		});
 }

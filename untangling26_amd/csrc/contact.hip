@@ -168,6 +168,25 @@ struct NarrowOp
 {
 	const ContactArgs* a;
 	int				   kind; // 0 VF DCD, 1 EE DCD, 2 VF CCD, 3 EE CCD
+	// fn_filter_broadphase_{vf,ee}_pairs (intersection_resolver.cpp:2427-2535)
+	__device__ __forceinline__ bool ccd_filtered(uint32_t q, uint32_t leaf) const
+	{
+		if (!a->filter.off && !a->filter.invalid)
+			return false;
+		uint32_t s[3], t[3];
+		int		 ns, nt;
+		if (kind == 2)
+		{
+			s[0] = q, ns = 1;
+			t[0] = a->faces[3 * leaf], t[1] = a->faces[3 * leaf + 1], t[2] = a->faces[3 * leaf + 2], nt = 3;
+		}
+		else
+		{
+			s[0] = a->edges[2 * q], s[1] = a->edges[2 * q + 1], ns = 2;
+			t[0] = a->edges[2 * leaf], t[1] = a->edges[2 * leaf + 1], nt = 2;
+		}
+		return pair_filtered(a->filter, s, ns, t, nt);
+	}
 	__device__ __forceinline__ void operator()(bool valid, uint32_t q, uint32_t leaf)
 	{
 		if (kind <= 1)
@@ -181,7 +200,7 @@ struct NarrowOp
 				a->pair_ids[s] = make_uint2(q, leaf);
 			}
 		}
-		else if (valid)
+		else if (valid && !ccd_filtered(q, leaf))
 		{
 			const double t = kind == 2 ? vf_ccd(q, leaf, *a) : ee_ccd(q, leaf, *a);
 			if (t != kLineSearchMaxT)

@@ -1,4 +1,6 @@
 #pragma once
+
+#include "region_view.hpp"
 #include "ut_runtime.hpp"
 #include "ef_dcd.hpp"
 #include "lbvh.hpp"
@@ -45,6 +47,7 @@ struct ContactArgs
 	uint2*			pair_ids = nullptr;
 	uint32_t		max_pairs = 0;
 	float*			toi = nullptr; // CCD: global minimum time of impact (non-negative float)
+	RegionView		filter;		   // CCD: pairs across a contour or at invalid vertices are skipped
 };
 
 // kind: 0 VF DCD, 1 EE DCD, 2 VF CCD, 3 EE CCD.  With `tree` the query is fused

@@ -34,8 +34,9 @@ __global__ void k_zero64(double* p, size_t n)
 
 // E = m / (2 h^2) |x - x_tilde|^2 (soft_inertia_energy.cpp:25-110)
 __global__ void k_asm_inertia(uint32_t nv, const float* __restrict__ x, const double* __restrict__ x_tilde,
-	const float* __restrict__ mass, double inv_h2, const uint32_t* __restrict__ diag_slot, uint32_t nslots,
-	double* __restrict__ grad, double* __restrict__ vals, double* __restrict__ energy_part)
+	const float* __restrict__ mass, double inv_h2, double grad_scale, double hess_scale,
+	const uint32_t* __restrict__ diag_slot, uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals,
+	double* __restrict__ energy_part)
 {
 	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
 	double		   e = 0.0;
@@ -46,13 +47,17 @@ __global__ void k_asm_inertia(uint32_t nv, const float* __restrict__ x, const do
 		for (int a = 0; a < 3; ++a)
 		{
 			d[a] = (double)x[3 * v + a] - x_tilde[3 * v + a];
-			atomicAdd(&grad[3 * v + a], k * d[a]);
+			if (grad_scale != 0.0)
+				atomicAdd(&grad[3 * v + a], grad_scale * k * d[a]);
 		}
-		e = 0.5 * k * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+		e = grad_scale * 0.5 * k * (d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 		const uint32_t s = diag_slot[v];
-		atomicAdd(&vals[0 * (size_t)nslots + s], k);
-		atomicAdd(&vals[4 * (size_t)nslots + s], k);
-		atomicAdd(&vals[8 * (size_t)nslots + s], k);
+		if (hess_scale != 0.0)
+		{
+			atomicAdd(&vals[0 * (size_t)nslots + s], hess_scale * k);
+			atomicAdd(&vals[4 * (size_t)nslots + s], hess_scale * k);
+			atomicAdd(&vals[8 * (size_t)nslots + s], hess_scale * k);
+		}
 	}
 	if (energy_part)
 	{
@@ -212,8 +217,8 @@ void assemble(Context* ctx, const AssembleArgs& a)
 	if (a.energy)
 		k_zero64<<<1, 64, 0, s>>>(a.energy, 4);
 	if (a.mass && a.x_tilde)
-		k_asm_inertia<<<div_up(a.nv, 256), 256, 0, s>>>(a.nv, a.x, a.x_tilde, a.mass, a.inv_h2, a.diag_slot, a.nslots,
-			a.grad, a.vals, a.energy ? a.energy + 0 : nullptr);
+		k_asm_inertia<<<div_up(a.nv, 256), 256, 0, s>>>(a.nv, a.x, a.x_tilde, a.mass, a.inv_h2, a.inertia_grad_scale,
+			a.inertia_hess_scale, a.diag_slot, a.nslots, a.grad, a.vals, a.energy ? a.energy + 0 : nullptr);
 	if (a.nf && a.faces && a.dm)
 		k_asm_stretch<<<div_up(a.nf, 256), 256, 0, s>>>(a.nf, a.x, a.faces, a.dm, a.face_area, a.mu_lambda,
 			a.face_slots, a.nslots, a.grad, a.vals, a.energy ? a.energy + 1 : nullptr);

@@ -13,6 +13,8 @@ struct AssembleArgs
 	const double* x_tilde = nullptr; // double[3V]
 	const float*  mass = nullptr;	 // per vertex
 	double		  inv_h2 = 0.0;
+	double		  inertia_grad_scale = 1.0; // 0 in quasi-static mode (no inertia gradient or energy)
+	double		  inertia_hess_scale = 1.0; // 0 when the quasi-static inertia Hessian is off
 	// BW98 stretch / shear per face
 	const uint32_t* faces = nullptr;
 	const float*	dm = nullptr;		 // float[4F]: d0, d1, d2, d3 with Dm = [[d0, d2], [d1, d3]]

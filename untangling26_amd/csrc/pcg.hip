@@ -728,3 +728,53 @@ void PcgSolver::spmv64(const void* vals, bool vals_fp32, const double* x, double
 }
 
 } // namespace ut
+
+namespace ut
+{
+namespace
+{
+__global__ void k_pcg_zero_flags(uint32_t* f)
+{
+	if (threadIdx.x < 4)
+		f[threadIdx.x] = 0u;
+}
+
+template <typename T>
+float profile_spmv_typed(PcgSolver& s, const T* vals, uint32_t reps)
+{
+	hipStream_t	   st = s.ctx->stream;
+	const uint32_t n = s.n, nb = div_up(n, kPcgThreads);
+	Vecs<T>		   v = carve<T>(s.work.get(), n);
+	double*		   pb = s.partials.get();
+	PcgParts	   parts{ pb, pb + 8 * nb + 8, { pb + nb, pb + 2 * nb }, pb + 3 * nb, pb + 4 * nb };
+	k_pcg_zero_flags<<<1, 64, 0, st>>>(s.flags.get());
+	hipEvent_t a, b;
+	UT_HIP_CHECK(hipEventCreate(&a));
+	UT_HIP_CHECK(hipEventCreate(&b));
+	k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
+		n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get());
+	UT_HIP_CHECK(hipEventRecord(a, st));
+	for (uint32_t i = 0; i < reps; ++i)
+		k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
+			n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get());
+	UT_HIP_CHECK(hipEventRecord(b, st));
+	UT_HIP_CHECK(hipEventSynchronize(b));
+	float ms = 0.0f;
+	UT_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+	(void)hipEventDestroy(a);
+	(void)hipEventDestroy(b);
+	UT_LAUNCH_CHECK();
+	return ms / (float)std::max(reps, 1u);
+}
+} // namespace
+
+float PcgSolver::profile_spmv(const void* vals, bool vals_fp32, uint32_t reps)
+{
+	UT_REQUIRE(vals, "profile: null values");
+	if (n == 0)
+		return 0.0f;
+	if (vals_fp32)
+		return profile_spmv_typed<float>(*this, reinterpret_cast<const float*>(vals), reps);
+	return profile_spmv_typed<double>(*this, reinterpret_cast<const double*>(vals), reps);
+}
+} // namespace ut

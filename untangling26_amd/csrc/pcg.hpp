@@ -87,6 +87,9 @@ struct PcgSolver
 	PcgReport solve(const PcgParams& p, const void* vals, const double* b, double* x, const double* vals64 = nullptr);
 	// y = A x (FP64 vectors; vals FP32 when vals_fp32, else FP64)
 	void spmv64(const void* vals, bool vals_fp32, const double* x, double* y);
+	// Average duration (ms, HIP events on the solver's stream) of `reps`
+	// launches of the PCG iteration's SpMV kernel on vals, over the work vectors.
+	float profile_spmv(const void* vals, bool vals_fp32, uint32_t reps);
 };
 
 } // namespace ut

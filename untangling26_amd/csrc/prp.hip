@@ -1284,12 +1284,17 @@ void build_csr(uint32_t n, const std::vector<uint32_t>& src, const std::vector<u
 	val.resize(w);
 }
 
+// Synchronous upload: the host vector may be reused or freed right after the
+// call, so the copy must have consumed it before returning.
 template <typename T>
 void upload(DevBuf<T>& d, const std::vector<T>& h, hipStream_t s)
 {
 	d.reserve(h.size());
 	if (!h.empty())
+	{
 		UT_HIP_CHECK(hipMemcpyAsync(d.get(), h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+		UT_HIP_CHECK(hipStreamSynchronize(s));
+	}
 }
 
 // symmetric 3x3 eigenvectors (cyclic Jacobi in FP64), ascending eigenvalues,
@@ -1732,7 +1737,8 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop.get(), own.get(), kring.get(), x_dev, edges,
 			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr };
 		k_prp_fill_u32<<<1, 64, 0, s>>>(4, counters.get(), 0u);
-		k_prp_fill_u32<<<1, 64, 0, s>>>(2, status.get(), 1u);
+		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get(), 1u);	  // [0] tree health
+		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get() + 1, 0u); // [1] stack overflows
 		ra.nodes = face_tree.nodes.get();
 		k_prp_ray_vert<0><<<kRayBlocks, kTravThreads, 0, s>>>(ra);
 		k_prp_ray_vert<2><<<kRayBlocks, kTravThreads, 0, s>>>(ra);
@@ -1814,7 +1820,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 }
 
 uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& face_tree, Lbvh& edge_tree,
-	std::vector<PrpChoice>& choices, ContactRec* out, uint32_t* out_count, uint32_t cap, PrpStats* stats)
+	std::vector<PrpChoice>& choices, PrpStats* stats)
 {
 	using clk = std::chrono::steady_clock;
 	const auto		  t0 = clk::now();
@@ -1995,6 +2001,7 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 	// contour suppression (:450-590) and responses
 	const auto tf0 = clk::now();
 	std::vector<uint8_t> sup(std::max(nc, 1u), 0);
+	suppressed.reserve(sup.size());
 	if (n_final && nc)
 	{
 		adjm.reserve((size_t)nc * nc);
@@ -2039,25 +2046,21 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 		upload(contour_c, cC, s);
 		upload(contour_dir, cdir, s);
 		upload(suppressed, sup, s);
+		responses.reserve(n_final);
+		k_prp_fill_u32<<<1, 64, 0, s>>>(1, counters.get() + 3, 0u);
 		k_prp_responses<<<div_up(n_final, 256), 256, 0, s>>>(n_final, final_hits.get(), final_contour.get(),
-			contour_dir.get(), contour_c.get(), suppressed.get(), edges, faces, va, ea, fa, params.kappa, out,
-			out_count, cap);
+			contour_dir.get(), contour_c.get(), suppressed.get(), edges, faces, va, ea, fa, params.kappa,
+			responses.get(), counters.get() + 3, n_final);
 		UT_LAUNCH_CHECK();
 	}
 	else
 		for (uint32_t c = 0; c < nc; ++c)
 			choices[c].suppressed = 0;
 	uint32_t nresp = 0;
-	for (uint32_t i = 0; i < n_final; ++i)
-		(void)i;
-	// responses written = kept hits of non-suppressed contours (count on the host)
 	if (n_final)
 	{
-		std::vector<uint32_t> fc(n_final);
-		UT_HIP_CHECK(hipMemcpyAsync(fc.data(), final_contour.get(), 4 * (size_t)n_final, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(&nresp, counters.get() + 3, 4, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipStreamSynchronize(s));
-		for (uint32_t c : fc)
-			nresp += sup[c] ? 0u : 1u;
 	}
 	last_final_hits = n_final;
 	st.responses = nresp;

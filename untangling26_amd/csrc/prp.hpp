@@ -137,11 +137,11 @@ struct PrpResolver
 		const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, const float* rest_x, const float* vert_area,
 		const float* edge_area, const float* face_area, const float* mass, const PrpParams& p);
 
-	// Resolve the contours of `ct` at positions x: choices[nc] and response
-	// records appended to out (at out_count[0] on the device, capacity cap).
-	// Returns the number of responses written.
+	// Resolve the contours of `ct` at positions x: choices[nc]; the response
+	// records (CollisionPairTemplate) are left in `responses`.  Returns their count.
 	uint32_t resolve(const float* x, const ContourBuilder& ct, Lbvh& face_tree, Lbvh& edge_tree,
-		std::vector<PrpChoice>& choices, ContactRec* out, uint32_t* out_count, uint32_t cap, PrpStats* stats);
+		std::vector<PrpChoice>& choices, PrpStats* stats);
+	DevBuf<ContactRec> responses;
 
 	// Parity hooks.  prepare(): contour tables, normals, and for the batch of
 	// contours [c0, c0 + n) the distances and directions.  eval_slots(): one

@@ -17,17 +17,17 @@ namespace ut
 {
 
 __global__ void k_predict(uint32_t nv, const float* __restrict__ x, const float* __restrict__ v, double h, double gx,
-	double gy, double gz, const uint32_t* __restrict__ vprop, double* __restrict__ x_tilde)
+	double gy, double gz, const uint32_t* __restrict__ vprop, int quasi_static, double* __restrict__ x_tilde)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nv)
 		return;
-	// fn_predict_position (newton_solver.cpp:561-598): v_pred = v + h g, x_tilde = x + h v_pred
+	// fn_predict_position (newton_solver.cpp:561-598): v_pred = h g (+ v unless quasi-static), x_tilde = x + h v_pred
 	const bool fixed = vprop && (vprop[i] & kPropFixed);
 	const double g[3] = { gx, gy, gz };
 	for (int a = 0; a < 3; ++a)
 	{
-		const double vp = (double)v[3 * i + a] + (fixed ? 0.0 : h * g[a]);
+		const double vp = (quasi_static ? 0.0 : (double)v[3 * i + a]) + (fixed ? 0.0 : h * g[a]);
 		x_tilde[3 * i + a] = (double)x[3 * i + a] + (fixed ? 0.0 : h * vp);
 	}
 }
@@ -108,7 +108,21 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 	edge_tree.init(ctx, kTreeEdge, d.ne);
 	pcg.init(ctx, d.nv, d.nslices, d.nslots, d.slice_off, d.col, d.diag_slot);
 	if (d.face_adj_edges && d.edge_adj_faces)
+	{
 		contours.init(ctx, d.nv, d.ne, d.nf, d.edges, d.faces, d.face_adj_edges, d.edge_adj_faces);
+		if (d.rest_vert_area && d.rest_edge_area && d.rest_face_area)
+		{
+			if (!d.rest_x)
+			{
+				rest_copy.reserve(n3);
+				UT_HIP_CHECK(hipMemcpyAsync(rest_copy.get(), x0, 4 * n3, hipMemcpyDeviceToDevice, ctx->stream));
+				d.rest_x = rest_copy.get();
+			}
+			prp.init(ctx, d.nv, d.ne, d.nf, d.edges, d.faces, d.face_adj_edges, d.edge_adj_faces, d.rest_x,
+				d.rest_vert_area, d.rest_edge_area, d.rest_face_area, d.mass, p.prp);
+			regions.init(ctx, d.nv);
+		}
+	}
 	ensure_contact_capacity(std::max<uint32_t>(4096, 4 * d.nv), std::max<uint32_t>(1024, d.ne / 4));
 	UT_HIP_CHECK(hipHostMalloc((void**)&h_counts, 32, hipHostMallocDefault));
 	for (auto& e : ev)
@@ -124,13 +138,26 @@ NewtonSolver::~NewtonSolver()
 			(void)hipEventDestroy(e);
 }
 
-void NewtonSolver::ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef)
+void NewtonSolver::ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef, uint32_t keep)
 {
 	if (need_contacts > cap_contacts)
 	{
+		// grow, preserving the first `keep` records
+		DevBuf<ContactRec> np;
+		DevBuf<uint32_t>   ni;
+		np.reserve(need_contacts);
+		ni.reserve(2 * (size_t)need_contacts);
+		if (keep)
+		{
+			UT_HIP_CHECK(hipMemcpyAsync(np.get(), contact_pairs.get(), sizeof(ContactRec) * keep,
+				hipMemcpyDeviceToDevice, ctx->stream));
+			UT_HIP_CHECK(hipMemcpyAsync(ni.get(), contact_ids.get(), 8 * (size_t)keep, hipMemcpyDeviceToDevice,
+				ctx->stream));
+			UT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+		}
+		std::swap(contact_pairs.ptr, np.ptr), std::swap(contact_pairs.cap, np.cap);
+		std::swap(contact_ids.ptr, ni.ptr), std::swap(contact_ids.cap, ni.cap);
 		cap_contacts = need_contacts;
-		contact_pairs.reserve(cap_contacts);
-		contact_ids.reserve(2 * (size_t)cap_contacts);
 	}
 	if (need_ef > cap_ef)
 	{
@@ -215,9 +242,9 @@ StepReport NewtonSolver::physics_step_GPU()
 		edge_tree.construct_tree();
 	}
 	k_predict<<<g(d.nv), 256, 0, s>>>(d.nv, x.get(), v.get(), h, p.gravity[0], p.gravity[1], p.gravity[2], d.vprop,
-		x_tilde.get());
+		p.quasi_static, x_tilde.get());
 	const bool fp64 = p.pcg.precision == kPcgFp64;
-	float	   t_det = 0, t_ct = 0, t_asm = 0, t_pcg = 0, t_ccd = 0;
+	float	   t_det = 0, t_ct = 0, t_asm = 0, t_pcg = 0, t_ccd = 0, t_prp = 0;
 	for (int it = 0; it < std::max(1, p.nonlinear_iters); ++it)
 	{
 		UT_HIP_CHECK(hipMemcpyAsync(x_iter_start.get(), x.get(), 4 * n3, hipMemcpyDeviceToDevice, s));
@@ -229,11 +256,43 @@ StepReport NewtonSolver::physics_step_GPU()
 			ncontacts = rep.vf_pairs + rep.ee_pairs;
 		}
 		UT_HIP_CHECK(hipEventRecord(ev[2], s));
+		regions.clear();
 		if (p.use_self_collision && p.use_untangling && contours.ctx)
 		{
 			contours.build(rep.ef_pairs, ef_pairs.get(), ef_ids.get());
 			rep.ef_total = contours.num_total;
 			rep.contours = contours.num_contours;
+			if (contours.num_total > 0 && prp.ctx)
+			{
+				// host_resolve -> PRP responses appended after the proximity pairs
+				// (upload_response_pairs_to_gpu, intersection_resolver2.cpp:2223-2294)
+				uint32_t nresp = 0;
+				if (p.use_prp && contours.num_contours > 0)
+				{
+					PrpStats ps;
+					nresp = prp.resolve(x.get(), contours, face_tree, edge_tree, last_choices, &ps);
+					rep.prp_responses += nresp;
+					rep.prp_contours += ps.contours;
+					rep.prp_levels += ps.levels;
+					rep.prp_hits += ps.hits;
+					t_prp += ps.ms_prepare + ps.ms_levels + ps.ms_final;
+				}
+				else
+					prp.last_final_hits = 0;
+				// filter_invalid_pairs (intersection_resolver.cpp:2640-2682) on the proximity pairs
+				regions.build(prp, contours);
+				regions.filter_proximity(contact_pairs.get(), ncontacts, x.get(), d.offset);
+				if (nresp)
+				{
+					ensure_contact_capacity(ncontacts + nresp, cap_ef, ncontacts);
+					UT_HIP_CHECK(hipMemcpyAsync(contact_pairs.get() + ncontacts, prp.responses.get(),
+						sizeof(ContactRec) * nresp, hipMemcpyDeviceToDevice, s));
+					h_counts[4] = ncontacts + nresp;
+					UT_HIP_CHECK(hipMemcpyAsync(counts.get() + 3, h_counts + 4, 4, hipMemcpyHostToDevice, s));
+					UT_HIP_CHECK(hipStreamSynchronize(s));
+					ncontacts += nresp;
+				}
+			}
 		}
 		UT_HIP_CHECK(hipEventRecord(ev[3], s));
 		AssembleArgs a;
@@ -245,6 +304,9 @@ StepReport NewtonSolver::physics_step_GPU()
 		a.x_tilde = x_tilde.get();
 		a.mass = d.mass;
 		a.inv_h2 = 1.0 / (h * h);
+		// quasi-static: no inertia gradient/energy; HessianOnly keeps the diagonal (soft_inertia_energy.cpp:68-80)
+		a.inertia_grad_scale = p.quasi_static ? 0.0 : 1.0;
+		a.inertia_hess_scale = (p.quasi_static && !p.qs_inertia_hessian) ? 0.0 : 1.0;
 		a.faces = d.faces;
 		a.dm = d.dm;
 		a.face_area = d.rest_face_area;
@@ -294,6 +356,7 @@ StepReport NewtonSolver::physics_step_GPU()
 			c.d_hat = d.d_hat;
 			c.offset = d.offset;
 			c.toi = toi.get();
+			c.filter = regions.view();
 			contact_query(ctx, 2, c, &face_tree, nullptr, nullptr, 0);
 			contact_query(ctx, 3, c, &edge_tree, nullptr, nullptr, 0);
 		}
@@ -326,6 +389,7 @@ StepReport NewtonSolver::physics_step_GPU()
 	rep.ms_pcg = t_pcg;
 	rep.ms_ccd = t_ccd;
 	rep.ms_total = total;
+	rep.ms_prp = t_prp;
 	return rep;
 }
 

@@ -7,6 +7,8 @@
 #include "energy.hpp"
 #include "lbvh.hpp"
 #include "pcg.hpp"
+#include "prp.hpp"
+#include "regions.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -22,6 +24,7 @@ struct SolverDesc
 	const uint32_t* face_adj_edges = nullptr;
 	const uint32_t* edge_adj_faces = nullptr;
 	// per-vertex data (device)
+	const float*	rest_x = nullptr; // rest positions (PRP rest geodesics); x0 when null
 	const float*	mass = nullptr;
 	const float*	d_hat = nullptr;
 	const float*	offset = nullptr;
@@ -55,7 +58,11 @@ struct SolverParams
 	int	   use_self_collision = 1;
 	int	   use_ccd = 1;
 	int	   use_untangling = 1; // EF detection + contours every iteration
+	int	   use_prp = 1;		   // PRP responses for the contours (use_untangling_PRP)
+	int	   quasi_static = 0;   // use_quasi_static_mode: x_tilde = x + h^2 g, no inertia gradient
+	int	   qs_inertia_hessian = 1; // QuasiStaticInertiaMode::HessianOnly keeps m / h^2 on the diagonal
 	PcgParams pcg;
+	PrpParams prp;
 };
 
 struct StepReport
@@ -67,6 +74,8 @@ struct StepReport
 	float	 toi = 1.0f;
 	double	 energy[3] = { 0, 0, 0 };
 	float	 ms_detect = 0, ms_contours = 0, ms_assemble = 0, ms_pcg = 0, ms_ccd = 0, ms_total = 0;
+	uint32_t prp_responses = 0, prp_contours = 0, prp_levels = 0, prp_hits = 0;
+	float	 ms_prp = 0;
 };
 
 struct NewtonSolver
@@ -79,9 +88,13 @@ struct NewtonSolver
 	DevBuf<double> x_tilde, grad, rhs, dx, vals64, energy;
 	DevBuf<float>  vals32;
 	DevBuf<float>  toi; // [1]
+	DevBuf<float>  rest_copy;
 	Lbvh		   face_tree, edge_tree;
 	PcgSolver	   pcg;
 	ContourBuilder contours;
+	PrpResolver	   prp;
+	RegionFilter   regions;
+	std::vector<PrpChoice> last_choices;
 
 	// contact lists: VF pairs then EE pairs in one array
 	uint32_t			cap_contacts = 0, cap_ef = 0;
@@ -99,7 +112,7 @@ struct NewtonSolver
 
 private:
 	void detect(StepReport& rep);
-	void ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef);
+	void ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef, uint32_t keep = 0);
 };
 
 } // namespace ut

@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import initializer
-from ._lib import PcgParams, SolverDesc, SolverParams, StepReport, lib, ptr
+from ._lib import PcgParams, PrpParams, SolverDesc, SolverParams, StepReport, lib, ptr
 from .contours import CT_COMP_MIN, CT_CONTOUR_INDEX, CT_MESH_INDEX, CT_PAIR_IDS, CT_PAIRS
 from .runtime import Context
 from .scenes import Mesh
@@ -41,10 +41,18 @@ class SceneParams:
     use_ccd_linesearch: bool = True
     use_untangling: bool = True
     d_hat: float = 1e-3
+    use_untangling_PRP: bool = True
+    use_quasi_static_mode: bool = False
+    untangling_response_depth: float = 3e-3
+    untangling_process_contours_count: int = 256
+    PRP_cpu_contour_batch_size: int = 32
+    PRP_intrinsic_tau: float = 0.25
+    stiffness_untangling: float = 1e8
     # additions of this build
     pcg_precision: int = PCG_MIXED
     pcg_inner_tol: float = 1e-2
     pcg_block_jacobi: bool = True
+    prp_directions: int = 10  # 10 = the reference's direction generator; more appends a Fibonacci set
 
 
 @dataclass
@@ -113,6 +121,10 @@ class NewtonSolver:
         p.use_untangling = int(c.use_untangling)
         p.pcg = PcgParams(c.pcg_precision, 0 if c.pcg_block_jacobi else 1, c.pcg_iter_count, c.pcg_rel_tol,
                           c.pcg_inner_tol, 20, 0.0, 8)
+        p.use_prp = int(c.use_untangling_PRP)
+        p.quasi_static = int(c.use_quasi_static_mode)
+        p.prp = PrpParams(c.untangling_process_contours_count, c.prp_directions, c.PRP_cpu_contour_batch_size,
+                          c.untangling_response_depth, c.stiffness_untangling, c.PRP_intrinsic_tau)
         return p
 
     def init_solver(self):
@@ -147,6 +159,8 @@ class NewtonSolver:
             setattr(d, k, ptr(t))
         d.vprop = None
         d.nslices, d.nslots = pat.nslices, pat.nslots
+        self._dev["rest_x"] = up(np.asarray(rest, np.float32))
+        d.rest_x = ptr(self._dev["rest_x"])
         x0 = up(mesh.x)
         h = C.c_void_p()
         self._ctx.check(lib().ut_solver_create(self._ctx.handle, C.byref(d), C.byref(self._params()), ptr(x0),
@@ -172,6 +186,12 @@ class NewtonSolver:
                              StepReport._fields_}
         return self._last_report
 
+    def profile_spmv(self, reps=50):
+        """Average ms of the PCG SpMV kernel on the last assembled matrix (HIP events)."""
+        ms = C.c_float()
+        self._ctx.check(lib().ut_solver_profile(self._handle, 0, reps, C.byref(ms)))
+        return ms.value
+
     def physics_step_cpu(self):
         raise RuntimeError("physics_step_cpu is not provided: this build runs the step on the GPU only")
 
@@ -183,6 +203,10 @@ class NewtonSolver:
         x = torch.empty((self._setup.nv, 3), dtype=torch.float32, device=self._ctx.device)
         self._ctx.check(lib().ut_solver_get_state(self._handle, ptr(x), None))
         return x
+
+    def set_state(self, x: torch.Tensor, v: torch.Tensor | None = None):
+        self._ctx.check(lib().ut_solver_set_state(self._handle, ptr(x.contiguous()),
+                                                  None if v is None else ptr(v.contiguous())))
 
     def set_positions(self, x: torch.Tensor):
         self._ctx.check(lib().ut_solver_set_state(self._handle, ptr(x.contiguous()), None))

@@ -67,16 +67,21 @@ class PrpResolver:
         except Exception:
             pass
 
-    def resolve(self, x, contours, face_tree, edge_tree, cap=1 << 20):
-        """(choices structured array, response records int32[n, 16] on the device, stats dict)."""
-        out = torch.empty((cap, 16), dtype=torch.int32, device=self.ctx.device)
-        cnt = torch.zeros(1, dtype=torch.int32, device=self.ctx.device)
+    def resolve(self, x, contours, face_tree, edge_tree, cap=1 << 16):
+        """(choices list of dicts, response records int32[n, 16] on the device, stats dict)."""
         nc = max(contours.num_contours, 1)
         ch = (PrpChoice * nc)()
         st = PrpStats()
-        self.ctx.check(lib().ut_prp_resolve(self.handle, ptr(x), contours.handle, face_tree.handle,
-                                            edge_tree.handle, ptr(out), ptr(cnt), cap, ch, nc, C.byref(st)))
-        n = int(cnt.item())
+        n32 = C.c_uint32()
+        while True:
+            out = torch.empty((cap, 16), dtype=torch.int32, device=self.ctx.device)
+            rc = lib().ut_prp_resolve(self.handle, ptr(x), contours.handle, face_tree.handle, edge_tree.handle,
+                                      ptr(out), cap, C.byref(n32), ch, nc, C.byref(st))
+            self.ctx.check(rc)
+            if n32.value <= cap:
+                break
+            cap = n32.value
+        n = n32.value
         choices = [{k: (list(getattr(c, k)) if k == "dir" else getattr(c, k)) for k, _ in PrpChoice._fields_}
                    for c in ch[:contours.num_contours]]
         return choices, out[:min(n, cap)], {k: getattr(st, k) for k, _ in PrpStats._fields_}
