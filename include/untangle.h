@@ -329,6 +329,8 @@ typedef struct ut_solver_params {
 	int			  use_prp;		/* use_untangling_PRP: PRP responses for the contours */
 	int			  quasi_static; /* use_quasi_static_mode with QuasiStaticInertiaMode::HessianOnly */
 	ut_prp_params prp;
+	int			  region_filter; /* filter_invalid_pairs on proximity and CCD pairs; must be 0: the filter
+									is switched off until its kernels are verified on the device */
 } ut_solver_params;
 
 typedef struct ut_step_report {

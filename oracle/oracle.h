@@ -146,6 +146,10 @@ uint32_t or_prp_directions(uint32_t np, const float* points, const float* weight
 	uint32_t total, float* out);
 void	 or_prp_prepare(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adjn,
 		 const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t total_dirs);
+void	 or_prp_prepare_n(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adjn,
+		 const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t total_dirs,
+		 uint32_t n_prepare);
+void	 or_prp_choose(or_prp_mesh* m, uint32_t c, void* out);
 uint32_t or_prp_eval_slot(or_prp_mesh* m, uint32_t c, const float* dir, uint32_t k_src, uint32_t k_dst,
 	void* out_hits, uint32_t cap, void* out_metrics);
 void	 or_prp_get_state(const or_prp_mesh* m, uint32_t c, float* dirs, uint32_t* ndirs, uint32_t* diam, uint32_t* nb);

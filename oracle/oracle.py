@@ -84,6 +84,8 @@ def lib():
                                         C.c_uint32, p]
         L.or_prp_directions.restype = C.c_uint32
         L.or_prp_prepare.argtypes = [C.c_void_p, C.c_uint32, p, p, p, p, p, C.c_uint32, C.c_uint32]
+        L.or_prp_prepare_n.argtypes = [C.c_void_p, C.c_uint32, p, p, p, p, p, C.c_uint32, C.c_uint32, C.c_uint32]
+        L.or_prp_choose.argtypes = [C.c_void_p, C.c_uint32, p]
         L.or_prp_eval_slot.argtypes = [C.c_void_p, C.c_uint32, p, C.c_uint32, C.c_uint32, p, C.c_uint32, p]
         L.or_prp_eval_slot.restype = C.c_uint32
         L.or_prp_get_state.argtypes = [C.c_void_p, C.c_uint32, p, p, p, p]
@@ -467,12 +469,18 @@ class PrpOracle:
         lib().or_prp_distances(self.h, len(b), _p(b), _p(hop), _p(geo))
         return hop, geo
 
-    def prepare(self, pairs, adj, adjn, contour_index, mesh_index, nc, total_dirs=10):
+    def prepare(self, pairs, adj, adjn, contour_index, mesh_index, nc, total_dirs=10, n_prepare=None):
         self._ct = [_c(pairs, np.uint32), _c(adj, np.uint32), _c(adjn, np.uint32), _c(contour_index, np.uint32),
                     _c(mesh_index, np.uint32)]
         self.nc = int(nc)
         N = self._ct[0].size // 12
-        lib().or_prp_prepare(self.h, N, *[_p(a) for a in self._ct], self.nc, total_dirs)
+        n_prepare = self.nc if n_prepare is None else int(n_prepare)
+        lib().or_prp_prepare_n(self.h, N, *[_p(a) for a in self._ct], self.nc, total_dirs, n_prepare)
+
+    def choose(self, c):
+        ch = np.zeros(1, PRP_CHOICE)
+        lib().or_prp_choose(self.h, c, _p(ch))
+        return ch[0]
 
     def state(self, c):
         dirs = np.zeros((64, 3), np.float32)

@@ -526,6 +526,13 @@ static int flag_has(const or_prp_mesh* m, uint32_t v, uint32_t flag)
 void or_prp_prepare(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adjn,
 	const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t total_dirs)
 {
+	or_prp_prepare_n(m, N, pairs, adj, adjn, contour_index, mesh_index, nc, total_dirs, nc);
+}
+
+/* As or_prp_prepare, with distances and directions only for contours < n_prepare. */
+void or_prp_prepare_n(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adjn,
+	const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t total_dirs, uint32_t n_prepare)
+{
 	free_contours(m);
 	m->N = N, m->nc = nc, m->pairs = pairs, m->adj = adj, m->adjn = adjn, m->cidx = contour_index, m->midx = mesh_index;
 	const uint32_t V = m->nv;
@@ -595,7 +602,7 @@ void or_prp_prepare(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uin
 	m->hop = malloc(sizeof(uint32_t) * 2 * (size_t)nc * V + 4);
 	m->geo = malloc(sizeof(float) * 2 * (size_t)nc * V + 4);
 	m->diameter = calloc(2 * nc + 1, 4);
-	for (uint32_t s = 0; s < 2 * nc; ++s)
+	for (uint32_t s = 0; s < 2 * nc && s < 2 * n_prepare; ++s)
 	{
 		uint32_t* hop = m->hop + (size_t)s * V;
 		or_prp_distances(m, m->b_off[s + 1] - m->b_off[s], m->b_verts + m->b_off[s], hop, m->geo + (size_t)s * V);
@@ -621,7 +628,7 @@ void or_prp_prepare(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const uin
 	/* directions */
 	m->dirs = calloc((size_t)nc * kMaxDirs * 3 + 1, sizeof(float));
 	m->ndirs = calloc(nc + 1, 4);
-	for (uint32_t c = 0; c < nc; ++c)
+	for (uint32_t c = 0; c < nc && c < n_prepare; ++c)
 	{
 		const uint32_t n = m->c_off[c + 1] - m->c_off[c];
 		float*		   pts = malloc(sizeof(float) * 3 * (n + 1));
@@ -745,6 +752,191 @@ static int seg_tri(f3 o, f3 e, f3 A, f3 B, f3 C, float* bary, float* t)
 	return or_line_intersection(s, en, a, b, cc, bary, t);
 }
 
+/* Exact-filter acceleration for the brute-force ray tests: targets are binned
+ * by their bounding box in the plane orthogonal to the ray direction, so a
+ * vertex ray only visits targets whose projected box holds the vertex's
+ * projection, and a swept edge only targets whose box meets its own.  The
+ * margin keeps the filter conservative; every visited pair runs the same
+ * test as the brute force. */
+typedef struct
+{
+	double	  e1[3], e2[3], lo[2], cell, eps;
+	uint32_t  nx, ny;
+	uint32_t *off, *items;
+} grid2;
+
+static void proj(const grid2* g, f3 p, double* u, double* v)
+{
+	*u = g->e1[0] * p.x + g->e1[1] * p.y + g->e1[2] * p.z;
+	*v = g->e2[0] * p.x + g->e2[1] * p.y + g->e2[2] * p.z;
+}
+
+static void grid_basis(grid2* g, f3 dir)
+{
+	const double d[3] = { dir.x, dir.y, dir.z };
+	const double a[3] = { fabs(d[1]) < 0.9 ? 0.0 : 1.0, fabs(d[1]) < 0.9 ? 1.0 : 0.0, 0.0 };
+	double		 c[3] = { a[1] * d[2] - a[2] * d[1], a[2] * d[0] - a[0] * d[2], a[0] * d[1] - a[1] * d[0] };
+	double		 l = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+	for (int k = 0; k < 3; ++k)
+		g->e1[k] = c[k] / l;
+	const double* e = g->e1;
+	double		  b[3] = { d[1] * e[2] - d[2] * e[1], d[2] * e[0] - d[0] * e[2], d[0] * e[1] - d[1] * e[0] };
+	l = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+	for (int k = 0; k < 3; ++k)
+		g->e2[k] = b[k] / l;
+}
+
+/* boxes[4 * i] = (ulo, vlo, uhi, vhi) of item ids[i] */
+static void grid_build(grid2* g, uint32_t n, const double* boxes, const uint32_t* ids)
+{
+	double lo[2] = { 1e300, 1e300 }, hi[2] = { -1e300, -1e300 };
+	for (uint32_t i = 0; i < n; ++i)
+		for (int k = 0; k < 2; ++k)
+		{
+			lo[k] = fmin(lo[k], boxes[4 * i + k]);
+			hi[k] = fmax(hi[k], boxes[4 * i + 2 + k]);
+		}
+	const double ext = n ? fmax(hi[0] - lo[0], hi[1] - lo[1]) : 1.0;
+	g->eps = 1e-4 * (ext > 0 ? ext : 1.0) + 1e-6;
+	const uint32_t r = n ? (uint32_t)ceil(sqrt((double)n)) : 1;
+	g->cell = fmax(ext / r, 1e-12);
+	g->lo[0] = lo[0] - g->eps, g->lo[1] = lo[1] - g->eps;
+	g->nx = (uint32_t)fmin((hi[0] - lo[0] + 2 * g->eps) / g->cell + 1, 4096);
+	g->ny = (uint32_t)fmin((hi[1] - lo[1] + 2 * g->eps) / g->cell + 1, 4096);
+	if (!n)
+		g->nx = g->ny = 1;
+	const size_t nc = (size_t)g->nx * g->ny;
+	g->off = calloc(nc + 1, 4);
+	for (int pass = 0; pass < 2; ++pass)
+	{
+		uint32_t* fill = pass ? calloc(nc, 4) : NULL;
+		for (uint32_t i = 0; i < n; ++i)
+		{
+			int x0 = (int)((boxes[4 * i] - g->eps - g->lo[0]) / g->cell), y0 = (int)((boxes[4 * i + 1] - g->eps - g->lo[1]) / g->cell);
+			int x1 = (int)((boxes[4 * i + 2] + g->eps - g->lo[0]) / g->cell), y1 = (int)((boxes[4 * i + 3] + g->eps - g->lo[1]) / g->cell);
+			x0 = x0 < 0 ? 0 : x0, y0 = y0 < 0 ? 0 : y0;
+			x1 = x1 >= (int)g->nx ? (int)g->nx - 1 : x1, y1 = y1 >= (int)g->ny ? (int)g->ny - 1 : y1;
+			for (int y = y0; y <= y1; ++y)
+				for (int x = x0; x <= x1; ++x)
+				{
+					const size_t c = (size_t)y * g->nx + x;
+					if (pass)
+						g->items[g->off[c] + fill[c]++] = ids[i];
+					else
+						g->off[c + 1]++;
+				}
+		}
+		if (!pass)
+		{
+			for (size_t c = 0; c < nc; ++c)
+				g->off[c + 1] += g->off[c];
+			g->items = malloc(sizeof(uint32_t) * (g->off[nc] + 1));
+		}
+		free(fill);
+	}
+}
+
+static void grid_free(grid2* g)
+{
+	free(g->off), free(g->items);
+	g->off = g->items = NULL;
+}
+
+/* cell range of a 2D box (clamped) */
+static void grid_range(const grid2* g, double ulo, double vlo, double uhi, double vhi, int* x0, int* y0, int* x1, int* y1)
+{
+	*x0 = (int)floor((ulo - g->eps - g->lo[0]) / g->cell), *y0 = (int)floor((vlo - g->eps - g->lo[1]) / g->cell);
+	*x1 = (int)floor((uhi + g->eps - g->lo[0]) / g->cell), *y1 = (int)floor((vhi + g->eps - g->lo[1]) / g->cell);
+	*x0 = *x0 < 0 ? 0 : *x0, *y0 = *y0 < 0 ? 0 : *y0;
+	*x1 = *x1 >= (int)g->nx ? (int)g->nx - 1 : *x1, *y1 = *y1 >= (int)g->ny ? (int)g->ny - 1 : *y1;
+}
+
+static void face_grid(const or_prp_mesh* m, grid2* g, f3 dir, const uint8_t* cf)
+{
+	grid_basis(g, dir);
+	uint32_t n = 0;
+	for (uint32_t f = 0; f < m->nf; ++f)
+		n += cf[f] != 0;
+	double*	  boxes = malloc(sizeof(double) * 4 * (n + 1));
+	uint32_t* ids = malloc(sizeof(uint32_t) * (n + 1));
+	n = 0;
+	for (uint32_t f = 0; f < m->nf; ++f)
+	{
+		if (!cf[f])
+			continue;
+		double lo[2] = { 1e300, 1e300 }, hi[2] = { -1e300, -1e300 };
+		for (int k = 0; k < 3; ++k)
+		{
+			double u, v;
+			proj(g, X(m, m->faces[3 * f + k]), &u, &v);
+			lo[0] = fmin(lo[0], u), lo[1] = fmin(lo[1], v), hi[0] = fmax(hi[0], u), hi[1] = fmax(hi[1], v);
+		}
+		boxes[4 * n] = lo[0], boxes[4 * n + 1] = lo[1], boxes[4 * n + 2] = hi[0], boxes[4 * n + 3] = hi[1];
+		ids[n++] = f;
+	}
+	grid_build(g, n, boxes, ids);
+	free(boxes), free(ids);
+}
+
+static void vert_face_rays(const or_prp_mesh* m, f3 dir, int kind, const uint8_t* cv, const uint8_t* cf,
+	or_prp_hit** hits, uint32_t* n, uint32_t* cap)
+{
+	grid2 g;
+	face_grid(m, &g, dir, cf);
+	const f3 cast = kind == 0 ? dir : scl(dir, -1.0f);
+	for (uint32_t v = 0; v < m->nv; ++v)
+	{
+		if (!cv[v])
+			continue;
+		const f3 o = X(m, v), en = add(o, scl(cast, 1.0f));
+		double	 u, w;
+		proj(&g, o, &u, &w);
+		int x0, y0, x1, y1;
+		grid_range(&g, u, w, u, w, &x0, &y0, &x1, &y1);
+		for (int y = y0; y <= y1; ++y)
+			for (int x = x0; x <= x1; ++x)
+			{
+				const size_t c = (size_t)y * g.nx + x;
+				for (uint32_t k = g.off[c]; k < g.off[c + 1]; ++k)
+				{
+					const uint32_t	f = g.items[k];
+					const uint32_t* t = m->faces + 3 * f;
+					if (t[0] == v || t[1] == v || t[2] == v)
+						continue;
+					/* a face spanning several cells is visited once per cell of the query box */
+					int first = 1;
+					if (x1 > x0 || y1 > y0)
+					{
+						/* dedupe: skip unless this is the lowest cell of the query box that holds f */
+						for (int yy = y0; yy <= y && first; ++yy)
+							for (int xx = x0; xx <= x1 && first; ++xx)
+							{
+								if (yy == y && xx >= x)
+									break;
+								const size_t cc = (size_t)yy * g.nx + xx;
+								for (uint32_t q = g.off[cc]; q < g.off[cc + 1]; ++q)
+									if (g.items[q] == f)
+									{
+										first = 0;
+										break;
+									}
+							}
+					}
+					if (!first)
+						continue;
+					float b[3], tt;
+					if (seg_tri(o, en, X(m, t[0]), X(m, t[1]), X(m, t[2]), b, &tt) && tt >= 0.0f && tt <= 1.0f)
+					{
+						or_prp_hit h = { kind == 0 ? 0u : 2u, kind == 0 ? v : f, kind == 0 ? f : v, b[0], b[1], tt * 1.0f,
+							0, 0 };
+						push_hit(hits, n, cap, h);
+					}
+				}
+			}
+	}
+	grid_free(&g);
+}
+
 /* VF / EE / FV rays of one contour and direction (cached_bvh.cpp:131-555,
  * request packing intersection_resolver2.cpp:3066-3124). */
 static uint32_t raycast(const or_prp_mesh* m, f3 dir, const uint8_t* cv0, const uint8_t* ce0, const uint8_t* cf0,
@@ -771,26 +963,33 @@ static uint32_t raycast(const or_prp_mesh* m, f3 dir, const uint8_t* cv0, const 
 	const float dl2 = dot(dir, dir);
 	/* VF: role-0 vertices along +dir against role-1 faces */
 	if (any_f1)
-		for (uint32_t v = 0; v < m->nv; ++v)
-		{
-			if (!cv0[v])
-				continue;
-			const f3 o = X(m, v), en = add(o, scl(dir, 1.0f));
-			for (uint32_t f = 0; f < m->nf; ++f)
-			{
-				const uint32_t* t = m->faces + 3 * f;
-				if (!cf1[f] || t[0] == v || t[1] == v || t[2] == v)
-					continue;
-				float b[3], tt;
-				if (seg_tri(o, en, X(m, t[0]), X(m, t[1]), X(m, t[2]), b, &tt) && tt >= 0.0f && tt <= 1.0f)
-				{
-					or_prp_hit h = { 0, v, f, b[0], b[1], tt * 1.0f, 0, 0 };
-					push_hit(hits, &n, cap, h);
-				}
-			}
-		}
+		vert_face_rays(m, dir, 0, cv0, cf1, hits, &n, cap);
 	/* EE: role-0 edges swept along +dir against role-1 edges */
 	if (any_e1)
+	{
+		grid2 g;
+		grid_basis(&g, dir);
+		uint32_t ntar = 0;
+		for (uint32_t e = 0; e < m->ne; ++e)
+			ntar += ce1[e] != 0;
+		double*	  boxes = malloc(sizeof(double) * 4 * (ntar + 1));
+		uint32_t* ids = malloc(sizeof(uint32_t) * (ntar + 1));
+		ntar = 0;
+		for (uint32_t e = 0; e < m->ne; ++e)
+		{
+			if (!ce1[e])
+				continue;
+			double u0, v0, u1, v1;
+			proj(&g, X(m, m->edges[2 * e]), &u0, &v0);
+			proj(&g, X(m, m->edges[2 * e + 1]), &u1, &v1);
+			boxes[4 * ntar] = fmin(u0, u1), boxes[4 * ntar + 1] = fmin(v0, v1);
+			boxes[4 * ntar + 2] = fmax(u0, u1), boxes[4 * ntar + 3] = fmax(v0, v1);
+			ids[ntar++] = e;
+		}
+		grid_build(&g, ntar, boxes, ids);
+		free(boxes), free(ids);
+		uint32_t* stamp = calloc(m->ne + 1, 4);
+		uint32_t  gen = 0;
 		for (uint32_t e = 0; e < m->ne; ++e)
 		{
 			if (!ce0[e])
@@ -816,55 +1015,54 @@ static uint32_t raycast(const or_prp_mesh* m, f3 dir, const uint8_t* cv0, const 
 				pn = scl(pn, 1.0f / pl);
 				pd = dot(pn, p0);
 			}
-			for (uint32_t g = 0; g < m->ne; ++g)
-			{
-				if (!ce1[g])
-					continue;
-				const uint32_t b0 = m->edges[2 * g], b1 = m->edges[2 * g + 1];
-				if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1)
-					continue;
-				const f3 q0 = X(m, b0), q1 = X(m, b1);
-				if (use_plane)
+			double u0, v0, u1, v1;
+			proj(&g, p0, &u0, &v0);
+			proj(&g, p1, &u1, &v1);
+			int x0, y0, x1, y1;
+			grid_range(&g, fmin(u0, u1), fmin(v0, v1), fmax(u0, u1), fmax(v0, v1), &x0, &y0, &x1, &y1);
+			++gen;
+			for (int y = y0; y <= y1; ++y)
+				for (int x = x0; x <= x1; ++x)
 				{
-					const float d0 = dot(pn, q0) - pd, d1 = dot(pn, q1) - pd;
-					if ((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f))
-						continue;
+					const size_t c = (size_t)y * g.nx + x;
+					for (uint32_t k = g.off[c]; k < g.off[c + 1]; ++k)
+					{
+						const uint32_t gi = g.items[k];
+						if (stamp[gi] == gen)
+							continue;
+						stamp[gi] = gen;
+						const uint32_t b0 = m->edges[2 * gi], b1 = m->edges[2 * gi + 1];
+						if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1)
+							continue;
+						const f3 q0 = X(m, b0), q1 = X(m, b1);
+						if (use_plane)
+						{
+							const float d0 = dot(pn, q0) - pd, d1 = dot(pn, q1) - pd;
+							if ((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f))
+								continue;
+						}
+						const f3	e2 = sub(q1, q0), r = sub(q0, p0), nn = crs(e1, e2);
+						const float det = -dot(dir, nn);
+						if (fabsf(det) < 1e-10f)
+							continue;
+						const float inv = 1.0f / det;
+						const float s_ = -dot(r, crs(e1, e2)) * inv;
+						const float t1 = -dot(dir, crs(r, e2)) * inv;
+						const float t2 = dot(dir, crs(e1, r)) * inv;
+						if (s_ >= 0.0f && s_ <= rmax && t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f)
+						{
+							or_prp_hit h = { 1, e, gi, 1.0f - t1, 1.0f - t2, s_, 0, 0 };
+							push_hit(hits, &n, cap, h);
+						}
+					}
 				}
-				const f3	e2 = sub(q1, q0), r = sub(q0, p0), nn = crs(e1, e2);
-				const float det = -dot(dir, nn);
-				if (fabsf(det) < 1e-10f)
-					continue;
-				const float inv = 1.0f / det;
-				const float s_ = -dot(r, crs(e1, e2)) * inv;
-				const float t1 = -dot(dir, crs(r, e2)) * inv;
-				const float t2 = dot(dir, crs(e1, r)) * inv;
-				if (s_ >= 0.0f && s_ <= rmax && t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f)
-				{
-					or_prp_hit h = { 1, e, g, 1.0f - t1, 1.0f - t2, s_, 0, 0 };
-					push_hit(hits, &n, cap, h);
-				}
-			}
 		}
+		free(stamp);
+		grid_free(&g);
+	}
 	/* FV: role-1 vertices along -dir against role-0 faces */
 	if (any_f0)
-		for (uint32_t v = 0; v < m->nv; ++v)
-		{
-			if (!cv1[v])
-				continue;
-			const f3 o = X(m, v), en = add(o, scl(scl(dir, -1.0f), 1.0f));
-			for (uint32_t f = 0; f < m->nf; ++f)
-			{
-				const uint32_t* t = m->faces + 3 * f;
-				if (!cf0[f] || t[0] == v || t[1] == v || t[2] == v)
-					continue;
-				float b[3], tt;
-				if (seg_tri(o, en, X(m, t[0]), X(m, t[1]), X(m, t[2]), b, &tt) && tt >= 0.0f && tt <= 1.0f)
-				{
-					or_prp_hit h = { 2, f, v, b[0], b[1], tt * 1.0f, 0, 0 };
-					push_hit(hits, &n, cap, h);
-				}
-			}
-		}
+		vert_face_rays(m, dir, 2, cv1, cf0, hits, &n, cap);
 	return n;
 }
 
@@ -1470,7 +1668,7 @@ uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_de
 			{
 				a = fmaxf(0.5f * (m->ea[h->src] + m->ea[h->dst]), 3e-6f);
 				rec[0] = m->edges[2 * h->src], rec[1] = m->edges[2 * h->src + 1];
-				rec[2] = m->edges[2 * h->dst], rec[3] = m->edges[2 * h->dst + 1];
+				rec[2] = m->edges[2 * h->dst], rec[3] = m->edges[2 * h->dst + 1] | 0x80000000u; /* EE type bit, collision_data.h:78 */
 				f[6] = h->b0, f[7] = h->b1;
 			}
 			const float stiff = a * kappa;
@@ -1487,4 +1685,14 @@ uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_de
 		free(lists[c]);
 	free(lists), free(counts), free(adjm), free(hasadj), free(order), free(sup), free(ch);
 	return nr;
+}
+
+/* The adaptive k-ring choice of one contour (the per-contour driver work of
+ * or_prp_resolve); out receives or_prp_choice. */
+void or_prp_choose(or_prp_mesh* m, uint32_t c, void* out)
+{
+	or_prp_choice ch;
+	choose_contour(m, c, &ch);
+	if (out)
+		memcpy(out, &ch, sizeof(ch));
 }

@@ -118,7 +118,7 @@ class SolverParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("kappa", C.c_double), ("energy", C.c_int),
                 ("damping", C.c_double), ("nonlinear_iters", C.c_int), ("use_self_collision", C.c_int),
                 ("use_ccd", C.c_int), ("use_untangling", C.c_int), ("pcg", PcgParams), ("use_prp", C.c_int),
-                ("quasi_static", C.c_int), ("prp", PrpParams)]
+                ("quasi_static", C.c_int), ("prp", PrpParams), ("region_filter", C.c_int)]
 
 
 class StepReport(C.Structure):

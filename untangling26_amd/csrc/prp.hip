@@ -1224,7 +1224,7 @@ __global__ void k_prp_responses(uint32_t n, const PrpHit* __restrict__ h, const 
 	{
 		ar = fmaxf(0.5f * (ea[x.src] + ea[x.dst]), 3e-6f);
 		r.idx[0] = edges[2 * x.src], r.idx[1] = edges[2 * x.src + 1];
-		r.idx[2] = edges[2 * x.dst], r.idx[3] = edges[2 * x.dst + 1];
+		r.idx[2] = edges[2 * x.dst], r.idx[3] = edges[2 * x.dst + 1] | 0x80000000u; // make_ee_pair type bit
 	}
 	const float stiff = ar * kappa;
 	r.normal[0] = nn[0], r.normal[1] = nn[1], r.normal[2] = nn[2];

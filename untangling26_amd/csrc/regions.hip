@@ -77,10 +77,16 @@ __global__ void k_region_copy(uint32_t n, const uint32_t* a, uint32_t* b)
 		b[i] = a[i];
 }
 
+// Record indices carry the collision type in bit 31 (VF: idx[2], EE: idx[3];
+// collision_data.h:36-80): every index is masked before it addresses a vertex.
+constexpr uint32_t kTypeBit = 0x80000000u;
+constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
+
 __device__ __forceinline__ uint32_t rec_sides(const ContactRec& r, uint32_t* a, int* na, uint32_t* b, int* nb)
 {
-	const bool vf = (r.idx[2] & 0x80000000u) != 0u;
-	const uint32_t i0 = r.idx[0], i1 = r.idx[1], i2 = r.idx[2] & 0x7FFFFFFFu, i3 = r.idx[3];
+	const bool	   vf = (r.idx[2] & kTypeBit) != 0u;
+	const uint32_t i0 = r.idx[0] & kIndexMask, i1 = r.idx[1] & kIndexMask, i2 = r.idx[2] & kIndexMask,
+				   i3 = r.idx[3] & kIndexMask;
 	if (vf)
 	{
 		a[0] = i0, *na = 1;
@@ -102,8 +108,9 @@ __global__ void k_mark_invalid(uint32_t n, const ContactRec* __restrict__ pairs,
 	if (i >= n)
 		return;
 	const ContactRec r = pairs[i];
-	const bool		 vf = (r.idx[2] & 0x80000000u) != 0u;
-	const uint32_t	 id[4] = { r.idx[0], r.idx[1], r.idx[2] & 0x7FFFFFFFu, r.idx[3] };
+	const bool		 vf = (r.idx[2] & kTypeBit) != 0u;
+	const uint32_t	 id[4] = { r.idx[0] & kIndexMask, r.idx[1] & kIndexMask, r.idx[2] & kIndexMask,
+		  r.idx[3] & kIndexMask };
 	const float		 b0 = r.bary[0], b1 = r.bary[1];
 	const float		 w[4] = { vf ? 1.0f : b0, vf ? -b0 : 1.0f - b0, -b1, vf ? b0 + b1 - 1.0f : b1 - 1.0f };
 	f3				 d = mk3(0.0f, 0.0f, 0.0f);

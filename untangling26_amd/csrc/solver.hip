@@ -84,6 +84,7 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 	p = params;
 	UT_REQUIRE(d.nv && d.nf && d.ne && d.edges && d.faces && d.mass && d.d_hat && d.offset, "solver: missing mesh data");
 	UT_REQUIRE(d.slice_off && d.col && d.diag_slot && d.face_slots, "solver: missing block pattern");
+	UT_REQUIRE(!p.region_filter, "solver: the contact region filter is switched off (not yet verified on the device)");
 	const size_t n3 = 3 * (size_t)d.nv;
 	x.reserve(n3);
 	v.reserve(n3);
@@ -280,8 +281,11 @@ StepReport NewtonSolver::physics_step_GPU()
 				else
 					prp.last_final_hits = 0;
 				// filter_invalid_pairs (intersection_resolver.cpp:2640-2682) on the proximity pairs
-				regions.build(prp, contours);
-				regions.filter_proximity(contact_pairs.get(), ncontacts, x.get(), d.offset);
+				if (p.region_filter)
+				{
+					regions.build(prp, contours);
+					regions.filter_proximity(contact_pairs.get(), ncontacts, x.get(), d.offset);
+				}
 				if (nresp)
 				{
 					ensure_contact_capacity(ncontacts + nresp, cap_ef, ncontacts);

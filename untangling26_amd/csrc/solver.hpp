@@ -61,6 +61,9 @@ struct SolverParams
 	int	   use_prp = 1;		   // PRP responses for the contours (use_untangling_PRP)
 	int	   quasi_static = 0;   // use_quasi_static_mode: x_tilde = x + h^2 g, no inertia gradient
 	int	   qs_inertia_hessian = 1; // QuasiStaticInertiaMode::HessianOnly keeps m / h^2 on the diagonal
+	// filter_invalid_pairs (intersection_resolver.cpp:2594-2682) on the proximity and CCD
+	// pairs; switched off (init refuses it) until its kernels are verified on the device
+	int		  region_filter = 0;
 	PcgParams pcg;
 	PrpParams prp;
 };

@@ -93,15 +93,22 @@ def two_sheets(seed: int = 0, n: int = 32, side: float = 1.0, depth: float = 0.0
     return Mesh(x, faces, name=f"two_sheets_{n}x{n}")
 
 
-def folded_cloth(seed: int = 0, n: int = 256, side: float = 2.0, amp: float = 0.08, noise: float = 0.005) -> Mesh:
+def folded_cloth(seed: int = 0, n: int = 256, side: float = 2.0, amp: float = 0.08, noise: float = 0.005,
+                 shift: float = 0.25) -> Mesh:
+    """Sheet folded about its midline; the folded half carries the figure-eight
+    height field plus noise.  `shift` (in grid cells) offsets the folded half in
+    the plane: without it the mirrored grid projects exactly onto the lower
+    grid, every crossing lies on an edge or vertex of the other layer and the
+    EF filters (intersection_resolver.cpp:160-181) reject most of them."""
     rng = np.random.default_rng(seed)
     u = np.linspace(0.0, 1.0, n)
     v = np.linspace(0.0, 1.0, n)
     uu, vv = np.meshgrid(u, v, indexing="ij")
     x = np.empty((n, n, 3))
     folded = uu > 0.5
-    x[..., 0] = np.where(folded, (1.0 - uu), uu) * side
-    x[..., 2] = vv * side
+    cell = side / (n - 1)
+    x[..., 0] = np.where(folded, (1.0 - uu) * side + shift * cell, uu * side)
+    x[..., 2] = vv * side + np.where(folded, shift * cell, 0.0)
     up = 2.0 * uu - 1.0  # coordinate along the folded half, 0 at the fold
     h = amp * np.sin(2 * np.pi * up) * np.sin(np.pi * vv) + rng.normal(0.0, noise, size=uu.shape)
     x[..., 1] = np.where(folded, h, 0.0)
