@@ -77,15 +77,31 @@ def run_leg(args, precision):
     s, m, rest = make_solver(args, precision)
     x0 = s.positions().clone()
     v0 = torch.zeros_like(x0)
-    for _ in range(args.warmup):
-        s.physics_step_gpu()
+    for i in range(args.warmup):
+        r = s.physics_step_gpu()
+        log(f"{precision_name(precision)} warmup step {i}: {r['ms_total']:.1f} ms, {r['ef_total']} EF pairs, "
+            f"{r['contours']} contours, {r['prp_responses']} PRP responses, {r['pcg_iterations']} PCG iterations")
     s.set_state(x0, v0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    reps = [s.physics_step_gpu() for _ in range(args.steps)]
+    reps = []
+    for i in range(args.steps):
+        reps.append(s.physics_step_gpu())
+        # the report read-back synchronizes the step; one line per step keeps a long run visibly alive
+        log(f"{precision_name(precision)} step {i}: {reps[-1]['ms_total']:.1f} ms, {reps[-1]['ef_total']} EF pairs")
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return s, m, rest, reps, dt
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def precision_name(p):
+    from untangling26_amd.lcs_py import PCG_FP64
+
+    return "fp64" if p == PCG_FP64 else "mixed"
 
 
 def summarize(reps):
