@@ -17,6 +17,9 @@
  *    crosses this boundary.
  *  - One stream per context; a context is not thread-safe (the reference
  *    drives one luisa::compute::Stream from one thread, solver_interface.h:39-67).
+ *  - A context lives until ut_ctx_destroy has been called AND every object
+ *    created from it has been destroyed, so objects may be destroyed after
+ *    their context, in any order.
  *
  * Reference citations are paths inside ChengzhuUwU/Untangling26.
  */

@@ -1221,7 +1221,10 @@ static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit
 	const uint32_t V = m->nv;
 	uint32_t*	   idx = malloc(sizeof(uint32_t) * (n + 1));
 	uint32_t	   nv = 0;
-	float		   vol = 0, depth = 0, area = 0, cost = 0;
+	/* per-hit terms in FP32 as the reference, sums in FP64: the reference's FP32
+	 * sum depends on its (parallel) hit order, the FP64 sum rounded once does not */
+	double		   vol = 0, area = 0, cost = 0;
+	float		   depth = 0;
 	uint32_t	   hs = 0, hd = 0;
 	for (uint32_t i = 0; i < n; ++i)
 	{
@@ -1238,11 +1241,11 @@ static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit
 		else
 			a = m->fa[h->src], nn = m->fn + 3 * h->src;
 		al = fabsf(dot(mk(nn[0], nn[1], nn[2]), dir));
-		vol += a * al * h->dist;
-		area += a;
+		vol += (double)(a * al * h->dist);
+		area += (double)a;
 		depth = h->dist > depth ? h->dist : depth;
 		const float sh = h->dist + 3e-3f, im = inv_mass(m, h);
-		cost += im > 1e-8f ? sh * sh / im : 0.0f;
+		cost += (double)(im > 1e-8f ? sh * sh / im : 0.0f);
 		uint32_t sv[3], dv[3];
 		const uint32_t ns = hit_src_verts(m, h, sv), nd = hit_dst_verts(m, h, dv);
 		for (uint32_t k = 0; k < ns; ++k)
@@ -1260,10 +1263,10 @@ static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit
 	}
 	out->hit_count = nv;
 	out->depth = depth;
-	out->area = area;
-	out->objective = area * depth * depth;
-	out->volume = vol;
-	out->disp_cost = cost;
+	out->area = (float)area;
+	out->objective = out->area * depth * depth;
+	out->volume = (float)vol;
+	out->disp_cost = (float)cost;
 	out->max_hop_src = hs;
 	out->max_hop_dst = hd;
 	out->ef_count = m->c_off[c + 1] - m->c_off[c];

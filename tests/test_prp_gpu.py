@@ -112,8 +112,9 @@ def test_slot_hits_match_oracle(scene, kmode):
         for k in ("hit_count", "covered", "ef_count", "max_hop_src", "max_hop_dst", "usable"):
             assert s[k] == om[k], (j, k, s[k], om[k])
         assert s["depth"] == om["depth"]
+        # FP32 per-hit terms summed in FP64 on both sides and rounded once: equal up to the last FP32 ulp
         for k in ("area", "volume", "objective", "disp_cost"):
-            assert np.isclose(s[k], om[k], rtol=1e-5, atol=1e-12), (j, k, s[k], om[k])
+            assert np.isclose(s[k], om[k], rtol=1e-6, atol=1e-12), (j, k, s[k], om[k])
         total += len(oh)
     assert total > 0
 

@@ -49,7 +49,7 @@ SIGNATURES = {
     "ut_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "ut_pcg_solve_refined": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "ut_solver_profile": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
-    "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 11 + [C.POINTER(vp)]),
+    "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 10 + [C.POINTER(vp)]),
     "ut_prp_destroy": (C.c_int, [vp]),
     "ut_prp_set_params": (C.c_int, [vp, vp]),
     "ut_prp_resolve": (C.c_int, [vp, vp, vp, vp, vp, vp, C.c_uint32, u32p, vp, C.c_uint32, vp]),

@@ -12,34 +12,79 @@
 #include "prp.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <new>
 #include <vector>
 
+// A context is reference counted: the caller's handle plus one reference per
+// object created from it, so objects may be destroyed after the context in any
+// order (as garbage collectors do at interpreter exit).
 struct ut_ctx
 {
-	ut::Context c;
+	ut::Context		 c;
+	std::atomic<int> refs{ 1 };
 };
+
+namespace
+{
+void ctx_retain(ut_ctx* c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+void ctx_release(ut_ctx* c)
+{
+	if (!c || c->refs.fetch_sub(1, std::memory_order_acq_rel) != 1)
+		return;
+	if (c->c.own_stream && c->c.stream)
+	{
+		(void)hipSetDevice(c->c.device);
+		(void)hipStreamSynchronize(c->c.stream);
+		(void)hipStreamDestroy(c->c.stream);
+	}
+	delete c;
+}
+} // namespace
+
 struct ut_lbvh
 {
 	ut::Lbvh b;
+	ut_ctx*	 owner = nullptr;
 };
 struct ut_contours
 {
 	ut::ContourBuilder b;
+	ut_ctx*			   owner = nullptr;
 };
 struct ut_pcg
 {
 	ut::PcgSolver s;
+	ut_ctx*		  owner = nullptr;
 };
 struct ut_solver
 {
 	ut::NewtonSolver s;
+	ut_ctx*			 owner = nullptr;
 };
 struct ut_prp
 {
 	ut::PrpResolver r;
+	ut_ctx*			owner = nullptr;
 };
+
+namespace
+{
+// finish the object's queued work, free it, then drop its context reference
+template <typename H>
+void destroy_handle(H* h, ut::Context* c)
+{
+	if (c && c->stream)
+	{
+		(void)hipSetDevice(c->device);
+		(void)hipStreamSynchronize(c->stream);
+	}
+	ut_ctx* owner = h->owner;
+	delete h;
+	ctx_release(owner);
+}
+} // namespace
 
 namespace
 {
@@ -140,15 +185,7 @@ UT_API int ut_ctx_create(int device, void* hip_stream, int flags, ut_ctx** out)
 
 UT_API int ut_ctx_destroy(ut_ctx* ctx)
 {
-	if (!ctx)
-		return ut::kOk;
-	if (ctx->c.own_stream && ctx->c.stream)
-	{
-		(void)hipSetDevice(ctx->c.device);
-		(void)hipStreamSynchronize(ctx->c.stream);
-		(void)hipStreamDestroy(ctx->c.stream);
-	}
-	delete ctx;
+	ctx_release(ctx);
 	return ut::kOk;
 }
 
@@ -178,6 +215,8 @@ UT_API int ut_lbvh_create(ut_ctx* ctx, int tree_type, uint32_t num_leaves, ut_lb
 		delete h;
 		return rc;
 	}
+	h->owner = ctx;
+	ctx_retain(ctx);
 	*out = h;
 	return ut::kOk;
 }
@@ -185,11 +224,7 @@ UT_API int ut_lbvh_create(ut_ctx* ctx, int tree_type, uint32_t num_leaves, ut_lb
 UT_API int ut_lbvh_destroy(ut_lbvh* bvh)
 {
 	if (bvh)
-	{
-		if (bvh->b.ctx)
-			(void)hipStreamSynchronize(bvh->b.ctx->stream);
-		delete bvh;
-	}
+		destroy_handle(bvh, bvh->b.ctx);
 	return ut::kOk;
 }
 
@@ -379,6 +414,8 @@ UT_API int ut_contours_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf
 		delete h;
 		return rc;
 	}
+	h->owner = ctx;
+	ctx_retain(ctx);
 	*out = h;
 	return ut::kOk;
 }
@@ -386,11 +423,7 @@ UT_API int ut_contours_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf
 UT_API int ut_contours_destroy(ut_contours* ct)
 {
 	if (ct)
-	{
-		if (ct->b.ctx)
-			(void)hipStreamSynchronize(ct->b.ctx->stream);
-		delete ct;
-	}
+		destroy_handle(ct, ct->b.ctx);
 	return ut::kOk;
 }
 
@@ -468,6 +501,8 @@ UT_API int ut_pcg_create(ut_ctx* ctx, uint32_t n, uint32_t nslices, uint32_t nsl
 		delete h;
 		return rc;
 	}
+	h->owner = ctx;
+	ctx_retain(ctx);
 	*out = h;
 	return ut::kOk;
 }
@@ -475,11 +510,7 @@ UT_API int ut_pcg_create(ut_ctx* ctx, uint32_t n, uint32_t nslices, uint32_t nsl
 UT_API int ut_pcg_destroy(ut_pcg* pcg)
 {
 	if (pcg)
-	{
-		if (pcg->s.ctx)
-			(void)hipStreamSynchronize(pcg->s.ctx->stream);
-		delete pcg;
-	}
+		destroy_handle(pcg, pcg->s.ctx);
 	return ut::kOk;
 }
 
@@ -680,6 +711,8 @@ UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_so
 		delete h;
 		return rc;
 	}
+	h->owner = ctx;
+	ctx_retain(ctx);
 	*out = h;
 	return ut::kOk;
 }
@@ -687,11 +720,7 @@ UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_so
 UT_API int ut_solver_destroy(ut_solver* s)
 {
 	if (s)
-	{
-		if (s->s.ctx)
-			(void)hipStreamSynchronize(s->s.ctx->stream);
-		delete s;
-	}
+		destroy_handle(s, s->s.ctx);
 	return ut::kOk;
 }
 
@@ -889,6 +918,8 @@ UT_API int ut_prp_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, con
 		delete h;
 		return rc;
 	}
+	h->owner = ctx;
+	ctx_retain(ctx);
 	*out = h;
 	return ut::kOk;
 }
@@ -896,11 +927,7 @@ UT_API int ut_prp_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, con
 UT_API int ut_prp_destroy(ut_prp* p)
 {
 	if (p)
-	{
-		if (p->r.ctx)
-			(void)hipStreamSynchronize(p->r.ctx->stream);
-		delete p;
-	}
+		destroy_handle(p, p->r.ctx);
 	return ut::kOk;
 }
 

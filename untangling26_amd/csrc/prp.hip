@@ -1031,7 +1031,9 @@ __global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
 	const f3	   dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
 	const uint32_t* H0 = a.hop_all + (size_t)(2 * sl.contour) * a.V;
 	const uint32_t* H1 = a.hop_all + (size_t)(2 * sl.contour + 1) * a.V;
-	float		   area = 0.0f, vol = 0.0f, cost = 0.0f, depth = 0.0f;
+	// per-hit terms in FP32, sums in FP64 rounded once (independent of the hit order)
+	double		   area = 0.0, vol = 0.0, cost = 0.0;
+	float		   depth = 0.0f;
 	uint32_t	   cnt = 0, hs = 0, hd = 0;
 	if (root != kNone)
 		for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
@@ -1048,8 +1050,8 @@ __global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
 				ar = a.ea[h.src], nn = ld3(a.en, h.src);
 			else
 				ar = a.fa[h.src], nn = ld3(a.fn, h.src);
-			area += ar;
-			vol += ar * fabsf(dot3(nn, dir)) * h.dist;
+			area += (double)ar;
+			vol += (double)(ar * fabsf(dot3(nn, dir)) * h.dist);
 			depth = fmaxf(depth, h.dist);
 			float im;
 			if (h.type == 1)
@@ -1067,7 +1069,7 @@ __global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
 				im = h.type == 0 ? inv_mass_of(a, v) + fpart : fpart + inv_mass_of(a, v);
 			}
 			const float shd = h.dist + 3e-3f;
-			cost += im > 1e-8f ? shd * shd / im : 0.0f;
+			cost += (double)(im > 1e-8f ? shd * shd / im : 0.0f);
 			uint32_t	   sv[3], dv[3];
 			const uint32_t ns = src_verts(a.edges, a.faces, h, sv), nd = dst_verts(a.edges, a.faces, h, dv);
 			for (uint32_t k = 0; k < ns; ++k)
@@ -1087,24 +1089,27 @@ __global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
 		hs = max(hs, (uint32_t)__shfl_xor(hs, o, kWave));
 		hd = max(hd, (uint32_t)__shfl_xor(hd, o, kWave));
 	}
-	__shared__ float	sf[4][4];
+	__shared__ double	sf[3][4];
+	__shared__ float	sdepth[4];
 	__shared__ uint32_t su[3][4];
 	const uint32_t		w = threadIdx.x / kWave;
 	if (__lane_id() == 0)
-		sf[0][w] = area, sf[1][w] = vol, sf[2][w] = cost, sf[3][w] = depth, su[0][w] = cnt, su[1][w] = hs,
+		sf[0][w] = area, sf[1][w] = vol, sf[2][w] = cost, sdepth[w] = depth, su[0][w] = cnt, su[1][w] = hs,
 		su[2][w] = hd;
 	__syncthreads();
 	if (threadIdx.x == 0)
 	{
 		PrpSummary s{};
+		double	   sa = 0.0, sv = 0.0, sc = 0.0;
 		for (uint32_t k = 0; k < blockDim.x / kWave; ++k)
 		{
-			s.area += sf[0][k], s.volume += sf[1][k], s.disp_cost += sf[2][k];
-			s.depth = fmaxf(s.depth, sf[3][k]);
+			sa += sf[0][k], sv += sf[1][k], sc += sf[2][k];
+			s.depth = fmaxf(s.depth, sdepth[k]);
 			s.hit_count += su[0][k];
 			s.max_hop_src = max(s.max_hop_src, su[1][k]);
 			s.max_hop_dst = max(s.max_hop_dst, su[2][k]);
 		}
+		s.area = (float)sa, s.volume = (float)sv, s.disp_cost = (float)sc;
 		s.raw_hits = hi - lo;
 		s.ef_count = a.cpair_off[a.c0 + sl.contour + 1] - a.cpair_off[a.c0 + sl.contour];
 		s.covered = root != kNone ? a.cov[root] : 0u;
