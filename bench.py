@@ -116,6 +116,9 @@ def summarize(reps):
     out["contours"] = [int(r["contours"]) for r in reps]
     out["prp_responses"] = float(np.mean([r["prp_responses"] for r in reps]))
     out["prp_levels"] = float(np.mean([r["prp_levels"] for r in reps]))
+    out["prp_slots"] = float(np.mean([r["prp_slots"] for r in reps]))
+    out["prp_rays"] = {k: float(np.mean([r[f"prp_rays_{k}"] for r in reps])) for k in ("vf", "ee", "fv")}
+    out["prp_hits"] = float(np.mean([r["prp_hits"] for r in reps]))
     return out
 
 
@@ -187,6 +190,9 @@ def main():
             "pcg_converged_steps_mixed": mixed["pcg_converged_steps"],
             "prp_responses_per_step": mixed["prp_responses"],
             "prp_levels_per_step": mixed["prp_levels"],
+            "prp_slots_per_step": mixed["prp_slots"],
+            "prp_rays_per_step": mixed["prp_rays"],
+            "prp_raw_hits_per_step": mixed["prp_hits"],
         },
         "roofline": {
             "kernel": "k_pcg_spmv<float> (SELL-64 3x3 BSR SpMV of the PCG iteration)",

@@ -346,6 +346,8 @@ typedef struct ut_step_report {
 	float	 ms_detect, ms_contours, ms_assemble, ms_pcg, ms_ccd, ms_total;
 	uint32_t prp_responses, prp_contours, prp_levels, prp_hits;
 	float	 ms_prp;
+	uint32_t prp_slots;
+	uint64_t prp_rays_vf, prp_rays_ee, prp_rays_fv;
 } ut_step_report;
 
 UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,
@@ -384,6 +386,7 @@ typedef struct ut_prp_choice {
 typedef struct ut_prp_stats {
 	uint32_t contours, levels, slots, hits, responses;
 	float	 ms_prepare, ms_levels, ms_final;
+	uint64_t rays_vf, rays_ee, rays_fv; /* ray requests cast, per kind */
 } ut_prp_stats;
 
 /* One (contour, direction) evaluation at hop radii (ks, kd); contour is the

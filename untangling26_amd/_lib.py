@@ -128,7 +128,8 @@ class StepReport(C.Structure):
                 ("toi", C.c_float), ("energy", C.c_double * 3), ("ms_detect", C.c_float), ("ms_contours", C.c_float),
                 ("ms_assemble", C.c_float), ("ms_pcg", C.c_float), ("ms_ccd", C.c_float), ("ms_total", C.c_float),
                 ("prp_responses", C.c_uint32), ("prp_contours", C.c_uint32), ("prp_levels", C.c_uint32),
-                ("prp_hits", C.c_uint32), ("ms_prp", C.c_float)]
+                ("prp_hits", C.c_uint32), ("ms_prp", C.c_float), ("prp_slots", C.c_uint32),
+                ("prp_rays_vf", C.c_uint64), ("prp_rays_ee", C.c_uint64), ("prp_rays_fv", C.c_uint64)]
 
 
 class PrpChoice(C.Structure):
@@ -141,7 +142,7 @@ class PrpChoice(C.Structure):
 class PrpStats(C.Structure):
     _fields_ = [("contours", C.c_uint32), ("levels", C.c_uint32), ("slots", C.c_uint32), ("hits", C.c_uint32),
                 ("responses", C.c_uint32), ("ms_prepare", C.c_float), ("ms_levels", C.c_float),
-                ("ms_final", C.c_float)]
+                ("ms_final", C.c_float), ("rays_vf", C.c_uint64), ("rays_ee", C.c_uint64), ("rays_fv", C.c_uint64)]
 
 
 def lib():

@@ -757,6 +757,8 @@ UT_API int ut_solver_step(ut_solver* s, ut_step_report* report)
 		report->prp_levels = r.prp_levels;
 		report->prp_hits = r.prp_hits;
 		report->ms_prp = r.ms_prp;
+		report->prp_slots = r.prp_slots;
+		report->prp_rays_vf = r.prp_rays[0], report->prp_rays_ee = r.prp_rays[1], report->prp_rays_fv = r.prp_rays[2];
 	}
 	return ut::kOk;
 }
@@ -982,6 +984,7 @@ UT_API int ut_prp_resolve(ut_prp* p, const float* x, ut_contours* ct, ut_lbvh* f
 				stats->contours = st.contours, stats->levels = st.levels, stats->slots = st.slots;
 				stats->hits = st.hits, stats->responses = st.responses;
 				stats->ms_prepare = st.ms_prepare, stats->ms_levels = st.ms_levels, stats->ms_final = st.ms_final;
+				stats->rays_vf = st.rays[0], stats->rays_ee = st.rays[1], stats->rays_fv = st.rays[2];
 			}
 		});
 	if (rc != ut::kOk)

@@ -273,7 +273,7 @@ __global__ void k_prp_cand_lists(uint32_t V, uint32_t E, uint32_t nslots2, const
 // three ray kinds.  One workgroup per slot; offsets by a single-workgroup scan.
 __global__ void k_prp_slot_prep(uint32_t V, uint32_t E, uint32_t nslots, PrpSlot* __restrict__ slots,
 	const uint32_t* __restrict__ cand, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ x,
-	const uint32_t* __restrict__ edges)
+	const uint32_t* __restrict__ edges, float* __restrict__ tbox)
 {
 	const uint32_t j = blockIdx.x;
 	if (j >= nslots)
@@ -283,28 +283,43 @@ __global__ void k_prp_slot_prep(uint32_t V, uint32_t E, uint32_t nslots, PrpSlot
 	const uint32_t n = cand_cnt[2 * s1 + 1];
 	const uint32_t* C = cand + (size_t)s1 * (V + E) + V;
 	const f3		dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
-	float			lo = FLT_MAX, hi = -FLT_MAX;
+	// [0] projection min, [1] projection max, [2..4] box min, [5..7] box max of the endpoints
+	float r[8] = { FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX };
 	for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
 	{
 		const uint32_t e = C[i];
-		const float	   p0 = dot3(ld3(x, edges[2 * e]), dir), p1 = dot3(ld3(x, edges[2 * e + 1]), dir);
-		lo = fminf(lo, fminf(p0, p1));
-		hi = fmaxf(hi, fmaxf(p0, p1));
+		const f3	   q0 = ld3(x, edges[2 * e]), q1 = ld3(x, edges[2 * e + 1]);
+		const float	   p0 = dot3(q0, dir), p1 = dot3(q1, dir);
+		r[0] = fminf(r[0], fminf(p0, p1));
+		r[1] = fmaxf(r[1], fmaxf(p0, p1));
+		r[2] = fminf(r[2], fminf(q0.x, q1.x)), r[3] = fminf(r[3], fminf(q0.y, q1.y));
+		r[4] = fminf(r[4], fminf(q0.z, q1.z));
+		r[5] = fmaxf(r[5], fmaxf(q0.x, q1.x)), r[6] = fmaxf(r[6], fmaxf(q0.y, q1.y));
+		r[7] = fmaxf(r[7], fmaxf(q0.z, q1.z));
 	}
-	lo = wave_min(lo);
-	hi = wave_max(hi);
-	__shared__ float slo[16], shi[16];
+	__shared__ float sr[8][16];
 	const uint32_t	 w = threadIdx.x / kWave;
-	if (__lane_id() == 0)
-		slo[w] = lo, shi[w] = hi;
-	__syncthreads();
-	if (threadIdx.x == 0)
+	for (int k = 0; k < 8; ++k)
 	{
-		float a = slo[0], b = shi[0];
-		for (uint32_t k = 1; k < blockDim.x / kWave; ++k)
-			a = fminf(a, slo[k]), b = fmaxf(b, shi[k]);
-		slots[j].ee_pmin = n ? a : 0.0f;
-		slots[j].ee_pmax = n ? b : 0.0f;
+		const bool mn = k == 0 || (k >= 2 && k <= 4);
+		const float v = mn ? wave_min(r[k]) : wave_max(r[k]);
+		if (__lane_id() == 0)
+			sr[k][w] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x < 8)
+	{
+		const int  k = threadIdx.x;
+		const bool mn = k == 0 || (k >= 2 && k <= 4);
+		float	   a = sr[k][0];
+		for (uint32_t q = 1; q < blockDim.x / kWave; ++q)
+			a = mn ? fminf(a, sr[k][q]) : fmaxf(a, sr[k][q]);
+		if (k == 0)
+			slots[j].ee_pmin = n ? a : 0.0f;
+		else if (k == 1)
+			slots[j].ee_pmax = n ? a : 0.0f;
+		else
+			tbox[6 * (size_t)j + (k - 2)] = a; // empty candidate set: an inverted box that meets no node
 	}
 }
 
@@ -397,6 +412,7 @@ struct RayArgs
 	uint32_t*		status;
 	uint32_t*		spill;
 	const BvhNode*	nodes;
+	const float*	tbox; // [6 * nslots] candidate target edge boxes
 };
 
 __device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots, uint32_t r)
@@ -523,7 +539,9 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 	{
 		const uint32_t r = base + threadIdx.x;
 		bool		   active = r < total;
-		box3		   sw = box_empty();
+		box3		   sw = box_empty(), cb = box_empty();
+		f3			   pn = mk3(0.0f, 0.0f, 0.0f);
+		float		   pd = 0.0f;
 		if (active)
 		{
 			const uint32_t slot = slot_of(off, a.nslots, r);
@@ -542,11 +560,36 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 				active = false;
 			sw = box_add(box_add(box_add(box_of(p0), p1), p0 + dir * rmax), p1 + dir * rmax);
 			sw = box_inflate(sw, 1e-5f);
+			const float* tb = a.tbox + 6 * (size_t)slot;
+			cb = box3{ mk3(tb[0], tb[1], tb[2]), mk3(tb[3], tb[4], tb[5]) };
+			// the leaf's plane prune (same normal): a kept target edge has a point within 1e-6 of the plane
+			pn = cross3(p1 - p0, dir);
+			const float pl = sqrtf(dot3(pn, pn));
+			if (pl >= 1e-10f)
+			{
+				pn = pn * (1.0f / pl);
+				pd = dot3(pn, p0);
+			}
+			else
+				pn = mk3(0.0f, 0.0f, 0.0f); // no plane prune at the leaf either
 		}
-		auto test = [&](const box3& b)
+		// Node culling.  A node can hold a kept target edge only if its box meets (1) the swept box,
+		// (2) the box of the slot's candidate target edges (every kept edge is a candidate and lies in
+		// it) and (3) the slab |n.x - pd| <= 1e-6 that the leaf's plane prune requires of one point of
+		// the edge.  (2) and (3) hold for the leaf box of any kept edge and so for its ancestors;
+		// kEeNodeMargin covers the FP32 rounding of the node and leaf dot products (~1e-6 here).
+		constexpr float kEeNodeMargin = 1e-4f;
+		auto			test = [&](const box3& b)
 		{
-			return (sw.lo.x <= b.hi.x) & (sw.hi.x >= b.lo.x) & (sw.lo.y <= b.hi.y) & (sw.hi.y >= b.lo.y)
-				& (sw.lo.z <= b.hi.z) & (sw.hi.z >= b.lo.z);
+			const bool in_sw = (sw.lo.x <= b.hi.x) & (sw.hi.x >= b.lo.x) & (sw.lo.y <= b.hi.y)
+				& (sw.hi.y >= b.lo.y) & (sw.lo.z <= b.hi.z) & (sw.hi.z >= b.lo.z);
+			const bool in_cb = (cb.lo.x <= b.hi.x) & (cb.hi.x >= b.lo.x) & (cb.lo.y <= b.hi.y)
+				& (cb.hi.y >= b.lo.y) & (cb.lo.z <= b.hi.z) & (cb.hi.z >= b.lo.z);
+			const f3	c = (b.lo + b.hi) * 0.5f, h = (b.hi - b.lo) * 0.5f;
+			const float dc = dot3(pn, c) - pd;
+			const float rr = fabsf(pn.x) * h.x + fabsf(pn.y) * h.y + fabsf(pn.z) * h.z;
+			const bool	in_slab = (dc - rr <= 1e-6f + kEeNodeMargin) & (dc + rr >= -1e-6f - kEeNodeMargin);
+			return in_sw & in_cb & in_slab;
 		};
 		auto narrow = [&](bool valid, uint32_t rr, uint32_t g)
 		{
@@ -621,6 +664,7 @@ struct HitArgs
 	uint32_t		n, V, prim_bits;
 	const PrpHit*	h;		 // sorted by (slot, type, src)
 	const uint32_t* keys;	 // sorted keys
+	const uint32_t* range;	 // [2 * slots] hit range of each slot
 	uint32_t*		parent;	 // union-find
 	const uint32_t *vv_off, *vv, *ve_off, *ve, *vf_off, *vf;
 	const uint32_t *edges, *faces, *fae, *eaf;
@@ -645,15 +689,21 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* k, uint32_t 
 	return lo;
 }
 
+// Lock-free union-find: roots are linked by CAS (larger index under smaller);
+// find halves the path.  A halving store only ever writes to a non-root
+// element (never a CAS target) and writes one of its ancestors, so every
+// parent chain stays valid under concurrent unions.
 __device__ __forceinline__ uint32_t uf_find(uint32_t* p, uint32_t x)
 {
-	uint32_t r = x;
 	for (;;)
 	{
-		const uint32_t q = __hip_atomic_load(&p[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		if (q == r)
-			return r;
-		r = q;
+		const uint32_t q = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (q == x)
+			return x;
+		const uint32_t g = __hip_atomic_load(&p[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (g != q)
+			__hip_atomic_store(&p[x], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		x = g;
 	}
 }
 
@@ -714,8 +764,10 @@ template <typename Pred>
 __device__ __forceinline__ void probe(const HitArgs& a, uint32_t i, uint32_t slot, uint32_t type, uint32_t prim,
 	const Pred& pred)
 {
+	// keys carry the slot in their top bits: search only the slot's own range
 	const uint32_t key = hit_key(slot, type, prim, a.prim_bits);
-	for (uint32_t j = lower_bound_u32(a.keys, a.n, key); j < a.n && a.keys[j] == key; ++j)
+	const uint32_t lo = a.range[2 * slot], hi = a.range[2 * slot + 1];
+	for (uint32_t j = lo + lower_bound_u32(a.keys + lo, hi - lo, key); j < hi && a.keys[j] == key; ++j)
 		if (j != i && pred(a.h[j]))
 			uf_union(a.parent, i, j);
 }
@@ -837,6 +889,14 @@ __device__ __forceinline__ uint64_t canonical_key(const PrpHit& h)
 	return ((uint64_t)h.type << 62) | ((uint64_t)(h.src & 0x3fffffffu) << 31) | (uint64_t)(h.dst & 0x7fffffffu);
 }
 
+// after all unions: every element points at its root
+__global__ void k_prp_uf_compress(uint32_t n, uint32_t* parent)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		parent[i] = uf_find(parent, i);
+}
+
 __global__ void k_prp_uf_init(uint32_t n, uint32_t* parent, uint32_t* attached, uint32_t* count, uint64_t* key,
 	uint32_t* cov)
 {
@@ -850,24 +910,51 @@ __global__ void k_prp_uf_init(uint32_t n, uint32_t* parent, uint32_t* attached, 
 __global__ void k_prp_components(CompArgs a, PrpHit* hs)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= a.n)
-		return;
-	const uint32_t r = uf_find(const_cast<uint32_t*>(a.parent), i);
-	PrpHit		   h = hs[i];
-	h.root = r;
-	hs[i] = h;
-	const uint32_t c = a.c0 + a.slots[h.slot].contour;
-	uint32_t	   sv[3], dv[3];
-	const uint32_t ns = src_verts(a.edges, a.faces, h, sv), nd = dst_verts(a.edges, a.faces, h, dv);
-	bool		   adj = true;
-	for (uint32_t k = 0; k < ns; ++k)
-		adj = adj && has_flag(a.flag_off, a.flags, sv[k], 2 * c);
-	for (uint32_t k = 0; k < nd; ++k)
-		adj = adj && has_flag(a.flag_off, a.flags, dv[k], 2 * c + 1);
-	if (adj)
-		atomicOr(&a.attached[r], 1u);
-	atomicAdd(&a.count[r], 1u);
-	atomicMin((unsigned long long*)&a.key[r], (unsigned long long)canonical_key(h));
+	const bool	   valid = i < a.n;
+	uint32_t	   r = 0;
+	bool		   adj = false;
+	uint64_t	   key = ~0ull;
+	if (valid)
+	{
+		r = uf_find(const_cast<uint32_t*>(a.parent), i);
+		PrpHit h = hs[i];
+		h.root = r;
+		hs[i] = h;
+		const uint32_t c = a.c0 + a.slots[h.slot].contour;
+		uint32_t	   sv[3], dv[3];
+		const uint32_t ns = src_verts(a.edges, a.faces, h, sv), nd = dst_verts(a.edges, a.faces, h, dv);
+		adj = true;
+		for (uint32_t k = 0; k < ns; ++k)
+			adj = adj && has_flag(a.flag_off, a.flags, sv[k], 2 * c);
+		for (uint32_t k = 0; k < nd; ++k)
+			adj = adj && has_flag(a.flag_off, a.flags, dv[k], 2 * c + 1);
+		key = canonical_key(h);
+	}
+	// one set of atomics per distinct root in the wave (hits are sorted, so a wave
+	// mostly holds one or two clusters, which would otherwise serialise on one address)
+	uint64_t pending = __ballot(valid);
+	while (pending)
+	{
+		const int	   leader = __ffsll((unsigned long long)pending) - 1;
+		const uint32_t lr = __shfl(r, leader, kWave);
+		const bool	   mine = ((pending >> __lane_id()) & 1ull) && r == lr;
+		const uint64_t m = __ballot(mine);
+		const bool	   any_adj = __ballot(mine && adj) != 0ull;
+		uint64_t	   k = mine ? key : ~0ull;
+		for (int o = 32; o > 0; o >>= 1)
+		{
+			const uint64_t t = __shfl_xor(k, o, kWave);
+			k = t < k ? t : k;
+		}
+		if ((int)__lane_id() == leader)
+		{
+			if (any_adj)
+				atomicOr(&a.attached[lr], 1u);
+			atomicAdd(&a.count[lr], (uint32_t)__popcll(m));
+			atomicMin((unsigned long long*)&a.key[lr], (unsigned long long)k);
+		}
+		pending &= ~m;
+	}
 }
 
 __device__ __forceinline__ uint64_t hmix(uint64_t k)
@@ -1733,14 +1820,16 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		V, E, n2, kring.get(), hop.get(), own.get(), edges, cand.get(), cand_cnt.get());
 	k_prp_any_faces<<<dim3(div_up(F, 256), n2), 256, 0, s>>>(
 		F, n2, kring.get(), V, hop.get(), own.get(), faces, target_cnt.get());
-	k_prp_slot_prep<<<ns, 256, 0, s>>>(V, E, ns, slots.get(), cand.get(), cand_cnt.get(), x_dev, edges);
+	slot_tbox.reserve(6 * (size_t)ns);
+	k_prp_slot_prep<<<ns, 256, 0, s>>>(V, E, ns, slots.get(), cand.get(), cand_cnt.get(), x_dev, edges,
+		slot_tbox.get());
 	k_prp_request_offsets<<<1, 1024, 0, s>>>(ns, slots.get(), cand_cnt.get(), target_cnt.get(), req_off.get());
 	UT_LAUNCH_CHECK();
 	uint32_t nh = 0;
 	for (int attempt = 0; attempt < 8; ++attempt)
 	{
 		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop.get(), own.get(), kring.get(), x_dev, edges,
-			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr };
+			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr, slot_tbox.get() };
 		k_prp_fill_u32<<<1, 64, 0, s>>>(4, counters.get(), 0u);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get(), 1u);	  // [0] tree health
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get() + 1, 0u); // [1] stack overflows
@@ -1750,10 +1839,15 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		ra.nodes = edge_tree.nodes.get();
 		k_prp_ray_edge<<<kRayBlocks, kTravThreads, 0, s>>>(ra);
 		UT_LAUNCH_CHECK();
-		uint32_t st[2];
+		uint32_t st[2], tot[3];
 		UT_HIP_CHECK(hipMemcpyAsync(&nh, counters.get(), 4, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(st, status.get(), 8, hipMemcpyDeviceToHost, s));
+		for (int t = 0; t < 3; ++t)
+			UT_HIP_CHECK(hipMemcpyAsync(&tot[t], req_off.get() + (size_t)t * (ns + 1) + ns, 4, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipStreamSynchronize(s));
+		if (attempt == 0)
+			for (int t = 0; t < 3; ++t)
+				ray_count[t] += tot[t];
 		UT_REQUIRE(st[0] == 1u, "prp: ray traversal revisited a node (corrupt tree)");
 		UT_REQUIRE(st[1] == 0u, "prp: ray traversal stack overflow");
 		if (nh <= hit_cap)
@@ -1784,9 +1878,10 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		uint32_t* cnt = attached + hit_cap;
 		uint32_t* cov = cnt + hit_cap;
 		k_prp_uf_init<<<div_up(nh, 256), 256, 0, s>>>(nh, parent, attached, cnt, comp_key.get(), cov);
-		HitArgs ha{ nh, V, prim_bits, hits_sorted.get(), keys, parent, vv_off.get(), vv.get(), ve_off.get(), ve.get(),
-			vf_off.get(), vf.get(), edges, faces, fae, eaf };
+		HitArgs ha{ nh, V, prim_bits, hits_sorted.get(), keys, slot_range.get(), parent, vv_off.get(), vv.get(),
+			ve_off.get(), ve.get(), vf_off.get(), vf.get(), edges, faces, fae, eaf };
 		k_prp_hit_union<<<div_up(nh, 256), 256, 0, s>>>(ha);
+		k_prp_uf_compress<<<div_up(nh, 256), 256, 0, s>>>(nh, parent);
 		// support hash: at most 6 entries per hit, load factor <= 1/2
 		uint32_t hc = 1;
 		while (hc < 12u * nh)
@@ -1831,6 +1926,7 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 	const auto		  t0 = clk::now();
 	const hipStream_t s = ctx->stream;
 	PrpStats		  st;
+	ray_count[0] = ray_count[1] = ray_count[2] = 0;
 	prepare(x, ct, 0, 0);
 	choices.assign(nc, PrpChoice{});
 	const uint32_t active = std::min(nc, params.max_contours);
@@ -2069,6 +2165,8 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 	}
 	last_final_hits = n_final;
 	st.responses = nresp;
+	for (int t = 0; t < 3; ++t)
+		st.rays[t] = ray_count[t];
 	st.ms_prepare = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
 	st.ms_levels = (float)ms_levels;
 	st.ms_final = (float)(ms_final + std::chrono::duration<double, std::milli>(clk::now() - tf0).count());

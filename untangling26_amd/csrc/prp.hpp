@@ -81,6 +81,7 @@ struct PrpStats
 {
 	uint32_t contours = 0, levels = 0, slots = 0, hits = 0, responses = 0;
 	float	 ms_prepare = 0, ms_levels = 0, ms_final = 0;
+	uint64_t rays[3] = { 0, 0, 0 }; // VF, EE, FV ray requests cast
 };
 
 struct PrpResolver
@@ -117,7 +118,9 @@ struct PrpResolver
 	DevBuf<uint32_t> kring;					 // [2 * batch] current k of each (contour, role)
 	DevBuf<uint32_t> cand_off, cand;		 // per (contour, role): candidate vertex and edge lists
 	DevBuf<uint32_t> cand_cnt;				 // [2 * batch][2]
+	uint64_t		 ray_count[3] = { 0, 0, 0 }; // ray requests cast since the last resolve began
 	DevBuf<PrpSlot>	 slots;
+	DevBuf<float>	 slot_tbox;				 // [6 * slots] box of the candidate target edges (EE culling)
 	DevBuf<PrpHit>	 hits, hits_sorted;
 	DevBuf<uint32_t> counters;				 // [0] hits, [1] attached roots, [2] final hits, [3] overflow
 	DevBuf<uint32_t> sort_k[2], sort_v[2], sort_h;

@@ -276,6 +276,9 @@ StepReport NewtonSolver::physics_step_GPU()
 					rep.prp_contours += ps.contours;
 					rep.prp_levels += ps.levels;
 					rep.prp_hits += ps.hits;
+					rep.prp_slots += ps.slots;
+					for (int k = 0; k < 3; ++k)
+						rep.prp_rays[k] += ps.rays[k];
 					t_prp += ps.ms_prepare + ps.ms_levels + ps.ms_final;
 				}
 				else

@@ -79,6 +79,8 @@ struct StepReport
 	float	 ms_detect = 0, ms_contours = 0, ms_assemble = 0, ms_pcg = 0, ms_ccd = 0, ms_total = 0;
 	uint32_t prp_responses = 0, prp_contours = 0, prp_levels = 0, prp_hits = 0;
 	float	 ms_prp = 0;
+	uint32_t prp_slots = 0;
+	uint64_t prp_rays[3] = { 0, 0, 0 };
 };
 
 struct NewtonSolver
