@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--pcg-iters", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one plain device copy of this many MiB before the legs (counter calibration)")
     return ap.parse_args()
 
 
@@ -119,7 +121,38 @@ def summarize(reps):
     out["prp_slots"] = float(np.mean([r["prp_slots"] for r in reps]))
     out["prp_rays"] = {k: float(np.mean([r[f"prp_rays_{k}"] for r in reps])) for k in ("vf", "ee", "fv")}
     out["prp_hits"] = float(np.mean([r["prp_hits"] for r in reps]))
+    out["ee"] = {k: float(np.sum([r[f"prp_ee_{k}"] for r in reps]))
+                 for k in ("launches", "requests", "visits", "leaf_tests", "hits")}
+    out["ee"]["ms"] = float(np.sum([r["ms_prp_ee"] for r in reps]))
     return out
+
+
+# Algorithmic bytes of the PRP EE sweep kernel (k_prp_ray_edge) per unit of work:
+# a request reads its slot id, candidate index and source edge (16 B); a node visit
+# reads one 64 B BvhNode (both child boxes and references); a leaf test reads the
+# target edge's ids, endpoint positions, hops and ownership (48 B); a hit writes a
+# 32 B PrpHit.
+EE_BYTES = {"requests": 16, "visits": 64, "leaf_tests": 48, "hits": 32}
+
+
+def measured_traffic(args):
+    """HBM bytes per k_prp_ray_edge launch from the committed counter passes of this
+    workload (FETCH_SIZE + WRITE_SIZE, calibrated on a 256 MiB copy), or None."""
+    p = ROOT / "profiles" / "k_prp_ray_edge_folded_cloth_256.traffic.json"
+    if not p.exists():
+        return None
+    t = json.loads(p.read_text())
+    if t.get("n") != args.n or t.get("dirs") != args.dirs:
+        return None
+    return t
+
+
+def ee_roofline(ee):
+    launches = max(ee["launches"], 1.0)
+    per_launch = sum(EE_BYTES[k] * ee[k] for k in EE_BYTES) / launches
+    avg_ms = ee["ms"] / launches
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    return per_launch, avg_ms, achieved
 
 
 def main():
@@ -133,10 +166,23 @@ def main():
 
     from untangling26_amd.lcs_py import PCG_FP64, PCG_MIXED
 
+    if args.copy_probe_mb:
+        # a copy of known size: its FETCH_SIZE / WRITE_SIZE counters calibrate the kernel's
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     s, m, rest, reps, dt = run_leg(args, PCG_MIXED)
     mixed = summarize(reps)
     value = args.steps / dt
-    # roofline of the PCG SpMV (the kernel of the PCG iteration loop), measured live
+    # roofline of the dominant kernel, the PRP EE sweep: HIP-event time of its launches in
+    # the timed steps and the counters of the same launches
+    ee_bytes, ee_ms, ee_achieved = ee_roofline(mixed["ee"])
+    traffic = measured_traffic(args)
+    # the PCG SpMV (the kernel of the PCG iteration loop), measured live on the last matrix
     pat = s._setup.pattern
     nnzb = int(len(pat.keys))
     spmv_ms = s.profile_spmv(100)
@@ -193,17 +239,27 @@ def main():
             "prp_slots_per_step": mixed["prp_slots"],
             "prp_rays_per_step": mixed["prp_rays"],
             "prp_raw_hits_per_step": mixed["prp_hits"],
+            "prp_ee_kernel": {"launches": mixed["ee"]["launches"], "avg_launch_ms": round(ee_ms, 4),
+                              "requests_per_launch": mixed["ee"]["requests"] / max(mixed["ee"]["launches"], 1),
+                              "visits_per_launch": mixed["ee"]["visits"] / max(mixed["ee"]["launches"], 1),
+                              "leaf_tests_per_launch": mixed["ee"]["leaf_tests"] / max(mixed["ee"]["launches"], 1),
+                              "hits_per_launch": mixed["ee"]["hits"] / max(mixed["ee"]["launches"], 1),
+                              "bytes_per_unit": EE_BYTES},
+            "roofline_pcg_spmv": {"kernel": "k_pcg_spmv<float>", "bound": "hbm", "achieved": round(achieved, 2),
+                                  "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                  "bytes_per_launch": spmv_bytes, "avg_launch_ms": round(spmv_ms, 5)},
         },
         "roofline": {
-            "kernel": "k_pcg_spmv<float> (SELL-64 3x3 BSR SpMV of the PCG iteration)",
+            "kernel": "k_prp_ray_edge (PRP EE sweep: LBVH traversal + edge-edge tests)",
             "bound": "hbm",
-            "achieved": round(achieved, 2),
+            "achieved": round(ee_achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_per_launch": spmv_bytes,
-            "avg_launch_ms": round(spmv_ms, 5),
+            "frac": round(ee_achieved / HBM_PEAK_GBS, 4),
+            "traffic": None if traffic is None else traffic["traffic_bytes_per_launch"],
+            "traffic_source": None if traffic is None else traffic["source"],
+            "bytes_per_launch": round(ee_bytes),
+            "avg_launch_ms": round(ee_ms, 4),
         },
         "cpu_baseline": cpu,
     }

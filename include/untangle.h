@@ -348,6 +348,11 @@ typedef struct ut_step_report {
 	float	 ms_prp;
 	uint32_t prp_slots;
 	uint64_t prp_rays_vf, prp_rays_ee, prp_rays_fv;
+	/* launch profile of the PRP EE sweep kernel: launches, HIP-event ms, and the
+	   requests, node visits, leaf tests and hits its algorithmic bytes count */
+	uint32_t prp_ee_launches;
+	float	 ms_prp_ee;
+	uint64_t prp_ee_requests, prp_ee_visits, prp_ee_leaf_tests, prp_ee_hits;
 } ut_step_report;
 
 UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,

@@ -759,6 +759,9 @@ UT_API int ut_solver_step(ut_solver* s, ut_step_report* report)
 		report->ms_prp = r.ms_prp;
 		report->prp_slots = r.prp_slots;
 		report->prp_rays_vf = r.prp_rays[0], report->prp_rays_ee = r.prp_rays[1], report->prp_rays_fv = r.prp_rays[2];
+		report->prp_ee_launches = r.prp_ee.launches, report->ms_prp_ee = r.prp_ee.ms;
+		report->prp_ee_requests = r.prp_ee.requests, report->prp_ee_visits = r.prp_ee.visits;
+		report->prp_ee_leaf_tests = r.prp_ee.leaf_tests, report->prp_ee_hits = r.prp_ee.hits;
 	}
 	return ut::kOk;
 }

@@ -53,6 +53,8 @@ __device__ __forceinline__ box3 query_box(int kind,
 // Runs the traversal for query q (valid when active).  narrow(valid, q, leaf)
 // is called by all 64 lanes together.  test(box) decides whether a child box
 // is entered; spill_slot selects this lane's region of the global stack.
+// When stat is given, stat[0] += node visits and stat[1] += leaf tests of the
+// wave (one atomic each per wave).
 template <typename NodeTest, typename Narrow>
 __device__ __forceinline__ void traverse_staged_test(TravShared& sh,
 	bool							 active,
@@ -64,12 +66,14 @@ __device__ __forceinline__ void traverse_staged_test(TravShared& sh,
 	uint32_t* __restrict__ status,
 	uint32_t* __restrict__ spill,
 	uint32_t max_visits,
-	Narrow&	 narrow)
+	Narrow&	 narrow,
+	unsigned long long* stat = nullptr)
 {
 	const uint32_t w = threadIdx.x / kWave;
 	const uint32_t lane = __lane_id();
 	uint2*		   stage = sh.stage[w];
 	uint32_t	   pending = 0; // wave-uniform
+	uint32_t	   tests = 0;	// wave-uniform: leaf tests run
 	const uint64_t lt = lanemask_lt();
 
 	auto run_batch = [&](uint32_t take)
@@ -78,6 +82,7 @@ __device__ __forceinline__ void traverse_staged_test(TravShared& sh,
 		const bool	   valid = lane < take;
 		const uint2	   c = valid ? stage[pending - take + lane] : make_uint2(0u, 0u);
 		pending -= take;
+		tests += take;
 		narrow(valid, c.x, c.y);
 	};
 
@@ -157,6 +162,17 @@ __device__ __forceinline__ void traverse_staged_test(TravShared& sh,
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 	if (pending > 0)
 		run_batch(pending);
+	if (stat)
+	{
+		uint32_t v = visits;
+		for (int o = 32; o > 0; o >>= 1)
+			v += __shfl_xor(v, o, kWave);
+		if (lane == 0)
+		{
+			atomicAdd(&stat[0], (unsigned long long)v);
+			atomicAdd(&stat[1], (unsigned long long)tests);
+		}
+	}
 }
 
 // Box-overlap traversal of a query box (broad phase); the query index is its spill slot.

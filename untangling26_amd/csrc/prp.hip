@@ -29,6 +29,7 @@ constexpr uint32_t kHopInf = 0xFFFFFFFFu;
 constexpr uint32_t kOwnUnknown = 0xFFu;
 constexpr int	   kBfsThreads = 1024;
 constexpr int	   kRayBlocks = 1024; // persistent ray grid (spill stack per lane)
+static_assert(kRayBlocks % kNumXcd == 0, "the ray grid is split evenly over the XCDs");
 constexpr uint32_t kMaxDirs = 64;
 constexpr uint64_t kHashEmpty = ~0ull;
 
@@ -413,6 +414,7 @@ struct RayArgs
 	uint32_t*		spill;
 	const BvhNode*	nodes;
 	const float*	tbox; // [6 * nslots] candidate target edge boxes
+	unsigned long long* tstat; // [3 kinds][4]: node visits, leaf tests, hits, unused
 };
 
 __device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots, uint32_t r)
@@ -429,11 +431,14 @@ __device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots
 	return lo;
 }
 
-__device__ __forceinline__ void emit_hit(const RayArgs& a, bool ok, const PrpHit& h)
+__device__ __forceinline__ void emit_hit(const RayArgs& a, bool ok, const PrpHit& h, int kind)
 {
 	const uint32_t i = wave_append(a.count, ok);
 	if (ok && i < a.cap)
 		a.hits[i] = h;
+	const uint64_t m = __ballot(ok);
+	if (m && __lane_id() == 0)
+		atomicAdd(&a.tstat[4 * kind + 2], (unsigned long long)__popcll(m));
 }
 
 // Per-lane request state shared by the three ray kinds.
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 	const uint32_t		  total = a.off[Kind * (a.nslots + 1) + a.nslots];
 	const uint32_t*		  off = a.off + Kind * (a.nslots + 1);
 	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
-	for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x)
+	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
 		const uint32_t r = base + threadIdx.x;
 		const bool	   active = r < total;
@@ -521,9 +526,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 					}
 				}
 			}
-			emit_hit(a, ok, h);
+			emit_hit(a, ok, h, Kind);
 		};
-		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow);
+		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow,
+			a.tstat + 4 * Kind);
 	}
 }
 
@@ -535,7 +541,7 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 	const uint32_t		  total = a.off[1 * (a.nslots + 1) + a.nslots];
 	const uint32_t*		  off = a.off + (a.nslots + 1);
 	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
-	for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x)
+	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
 		const uint32_t r = base + threadIdx.x;
 		bool		   active = r < total;
@@ -652,9 +658,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 					}
 				}
 			}
-			emit_hit(a, ok, h);
+			emit_hit(a, ok, h, 1);
 		};
-		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow);
+		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow,
+			a.tstat + 4);
 	}
 }
 
@@ -1560,7 +1567,18 @@ void PrpResolver::init(Context* c, uint32_t nv_, uint32_t ne_, uint32_t nf_, con
 	counters.reserve(8);
 	status.reserve(2);
 	spill.reserve((size_t)kRayBlocks * kTravThreads * (kStackTotal - kStackLds));
+	tstat.reserve(12);
+	if (!ee_ev[0])
+		for (auto& e : ee_ev)
+			UT_HIP_CHECK(hipEventCreate(&e));
 	UT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+PrpResolver::~PrpResolver()
+{
+	for (auto& e : ee_ev)
+		if (e)
+			(void)hipEventDestroy(e);
 }
 
 void PrpResolver::build_contour_tables(const ContourBuilder& ct)
@@ -1810,7 +1828,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	slot_root.reserve(ns);
 	summary.reserve(ns);
 	if (hit_cap == 0)
-		ensure_hits(1u << 20);
+		ensure_hits(1u << 23); // 8M hits (256 MB): the 256^2 fold needs ~5M per evaluation
 	// ownership and candidates
 	k_prp_own<<<dim3(div_up(V, 256), n2), 256, 0, s>>>(
 		V, batch0, n2, kring.get(), hop.get(), geo.get(), x_dev, b_off.get(), b_verts.get(), params.tau, own.get());
@@ -1829,7 +1847,9 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	for (int attempt = 0; attempt < 8; ++attempt)
 	{
 		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop.get(), own.get(), kring.get(), x_dev, edges,
-			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr, slot_tbox.get() };
+			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr, slot_tbox.get(),
+			tstat.get() };
+		k_prp_fill_u64<<<1, 64, 0, s>>>(12, reinterpret_cast<uint64_t*>(tstat.get()), 0ull);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(4, counters.get(), 0u);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get(), 1u);	  // [0] tree health
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get() + 1, 0u); // [1] stack overflows
@@ -1837,17 +1857,31 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		k_prp_ray_vert<0><<<kRayBlocks, kTravThreads, 0, s>>>(ra);
 		k_prp_ray_vert<2><<<kRayBlocks, kTravThreads, 0, s>>>(ra);
 		ra.nodes = edge_tree.nodes.get();
+		UT_HIP_CHECK(hipEventRecord(ee_ev[0], s));
 		k_prp_ray_edge<<<kRayBlocks, kTravThreads, 0, s>>>(ra);
+		UT_HIP_CHECK(hipEventRecord(ee_ev[1], s));
 		UT_LAUNCH_CHECK();
-		uint32_t st[2], tot[3];
+		uint32_t		   st[2], tot[3];
+		unsigned long long ts[12];
+		UT_HIP_CHECK(hipMemcpyAsync(ts, tstat.get(), sizeof(ts), hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(&nh, counters.get(), 4, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(st, status.get(), 8, hipMemcpyDeviceToHost, s));
 		for (int t = 0; t < 3; ++t)
 			UT_HIP_CHECK(hipMemcpyAsync(&tot[t], req_off.get() + (size_t)t * (ns + 1) + ns, 4, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipStreamSynchronize(s));
 		if (attempt == 0)
+		{
 			for (int t = 0; t < 3; ++t)
 				ray_count[t] += tot[t];
+			float ms = 0.0f;
+			UT_HIP_CHECK(hipEventElapsedTime(&ms, ee_ev[0], ee_ev[1]));
+			ee_prof.launches += 1;
+			ee_prof.ms += ms;
+			ee_prof.requests += tot[1];
+			ee_prof.visits += ts[4];
+			ee_prof.leaf_tests += ts[5];
+			ee_prof.hits += ts[6];
+		}
 		UT_REQUIRE(st[0] == 1u, "prp: ray traversal revisited a node (corrupt tree)");
 		UT_REQUIRE(st[1] == 0u, "prp: ray traversal stack overflow");
 		if (nh <= hit_cap)
@@ -1927,6 +1961,7 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 	const hipStream_t s = ctx->stream;
 	PrpStats		  st;
 	ray_count[0] = ray_count[1] = ray_count[2] = 0;
+	ee_prof = PrpKernelProfile{};
 	prepare(x, ct, 0, 0);
 	choices.assign(nc, PrpChoice{});
 	const uint32_t active = std::min(nc, params.max_contours);
@@ -2167,6 +2202,7 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 	st.responses = nresp;
 	for (int t = 0; t < 3; ++t)
 		st.rays[t] = ray_count[t];
+	st.ee = ee_prof;
 	st.ms_prepare = (float)std::chrono::duration<double, std::milli>(t1 - t0).count();
 	st.ms_levels = (float)ms_levels;
 	st.ms_final = (float)(ms_final + std::chrono::duration<double, std::milli>(clk::now() - tf0).count());

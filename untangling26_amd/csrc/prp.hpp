@@ -77,15 +77,26 @@ struct PrpChoice
 	float	 depth = 0, objective = 0, area = 0, disp_cost = 0;
 };
 
+// Launch profile of the EE sweep kernel (the PRP's dominant kernel): HIP-event
+// time and the counters its algorithmic bytes are computed from.
+struct PrpKernelProfile
+{
+	uint32_t launches = 0;
+	float	 ms = 0;
+	uint64_t requests = 0, visits = 0, leaf_tests = 0, hits = 0;
+};
+
 struct PrpStats
 {
 	uint32_t contours = 0, levels = 0, slots = 0, hits = 0, responses = 0;
 	float	 ms_prepare = 0, ms_levels = 0, ms_final = 0;
 	uint64_t rays[3] = { 0, 0, 0 }; // VF, EE, FV ray requests cast
+	PrpKernelProfile ee;
 };
 
 struct PrpResolver
 {
+	~PrpResolver();
 	Context* ctx = nullptr;
 	uint32_t nv = 0, ne = 0, nf = 0;
 	// caller's device arrays (static topology and rest data)
@@ -119,6 +130,9 @@ struct PrpResolver
 	DevBuf<uint32_t> cand_off, cand;		 // per (contour, role): candidate vertex and edge lists
 	DevBuf<uint32_t> cand_cnt;				 // [2 * batch][2]
 	uint64_t		 ray_count[3] = { 0, 0, 0 }; // ray requests cast since the last resolve began
+	PrpKernelProfile ee_prof;
+	hipEvent_t		 ee_ev[2] = { nullptr, nullptr };
+	DevBuf<unsigned long long> tstat; // [3 kinds][4] traversal counters of one evaluation
 	DevBuf<PrpSlot>	 slots;
 	DevBuf<float>	 slot_tbox;				 // [6 * slots] box of the candidate target edges (EE culling)
 	DevBuf<PrpHit>	 hits, hits_sorted;

@@ -81,6 +81,7 @@ struct StepReport
 	float	 ms_prp = 0;
 	uint32_t prp_slots = 0;
 	uint64_t prp_rays[3] = { 0, 0, 0 };
+	PrpKernelProfile prp_ee;
 };
 
 struct NewtonSolver

@@ -15,6 +15,20 @@ namespace ut
 
 constexpr int kWave = 64;
 
+// MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs
+// (workgroup i runs on XCD i % 8), and each XCD has its own L2.
+constexpr uint32_t kNumXcd = 8;
+
+// Logical workgroup id of a persistent grid (size a multiple of kNumXcd) that
+// numbers the workgroups of one XCD consecutively.  A grid-stride loop over
+// logical ids keeps its load balance (every round of gridDim.x tiles is shared
+// by all XCDs), while the tiles one XCD works on at the same time are
+// neighbours in the work order and so share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block()
+{
+	return (blockIdx.x % kNumXcd) * (gridDim.x / kNumXcd) + blockIdx.x / kNumXcd;
+}
+
 struct f3
 {
 	float x, y, z;
