@@ -160,6 +160,12 @@ int		 or_prp_select(uint32_t n, const float* objective, const uint32_t* covered,
 		 const float* depth, const float* cost, const uint32_t* hits, const uint32_t* eligible);
 uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa,
 	void* choices_out, uint32_t* recs, uint32_t cap);
+/* or_prp_resolve plus meta[2i] = contour and meta[2i + 1] = hit type (0 VF, 1 EE,
+ * 2 FV) of response i: the region semantics of host_apply_untangling_constraint
+ * (intersection_resolver2.cpp:2339-2345) need the kind, which the VF-layout FV
+ * record does not carry. */
+uint32_t or_prp_resolve_meta(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa,
+	void* choices_out, uint32_t* recs, uint32_t* meta, uint32_t cap);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,8 @@ def lib():
         L.or_prp_hits_adjacent.argtypes = [C.c_void_p, p, p]
         L.or_prp_resolve.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_float, p, p, C.c_uint32]
         L.or_prp_resolve.restype = C.c_uint32
+        L.or_prp_resolve_meta.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_float, p, p, p, C.c_uint32]
+        L.or_prp_resolve_meta.restype = C.c_uint32
         L.or_line_intersection.argtypes = [p, p, p, p, p, p, p]
         L.or_line_intersection.restype = C.c_int
         L.or_ef_dcd.argtypes = [p, C.c_size_t, p, p, p, p, p, p, p, C.c_size_t]
@@ -521,3 +523,111 @@ class PrpOracle:
         recs = np.zeros((max(n, 1), 16), np.uint32)
         lib().or_prp_resolve(self.h, max_contours, response_depth, kappa, _p(ch), _p(recs), n)
         return ch[:self.nc], recs[:n]
+
+    def resolve_meta(self, max_contours=256, response_depth=1.0, kappa=1e8):
+        """(choices, records uint32[n, 16], meta uint32[n, 2] = (contour, hit type 0 VF / 1 EE / 2 FV))."""
+        ch = np.zeros(max(self.nc, 1), PRP_CHOICE)
+        n = lib().or_prp_resolve_meta(self.h, max_contours, response_depth, kappa, _p(ch), None, None, 0)
+        recs = np.zeros((max(n, 1), 16), np.uint32)
+        meta = np.zeros((max(n, 1), 2), np.uint32)
+        lib().or_prp_resolve_meta(self.h, max_contours, response_depth, kappa, _p(ch), _p(recs), _p(meta), n)
+        return ch[:self.nc], recs[:n], meta[:n]
+
+
+# ---------------------------------------------------------------- contact region filter
+# IntersectionResolver::filter_invalid_pairs / filter_broad_phase_pairs
+# (intersection_resolver.cpp:2233-2535, 2594-2682) with the region sets of
+# host_apply_untangling_constraint (intersection_resolver2.cpp:2296-2426).
+
+TYPE_BIT = np.uint32(0x80000000)
+INDEX_MASK = np.uint32(0x7FFFFFFF)
+
+
+def region_sets(nv, ef_pairs, contour_index, mesh_index, responses=None, meta=None):
+    """Per-vertex sorted region lists.  EF pair p (edge verts idx[0:2], face verts
+    idx[2:5]): edge verts get 2c + m, face verts 2c + m ^ 1 (:2354-2368).  PRP
+    response i of contour c: VF -> vertex 2c, face verts 2c ^ 1; FV -> vertex 2c ^ 1,
+    face verts 2c; EE -> source edge 2c, target edge 2c ^ 1 (:2339-2345).  Edges and
+    faces take the union of their vertices' regions (:2386-2418)."""
+    sets = [set() for _ in range(nv)]
+    ef = np.asarray(ef_pairs, np.uint32).reshape(-1, 5)
+    for p in range(len(ef)):
+        er = 2 * int(contour_index[p]) + int(mesh_index[p])
+        for k in range(5):
+            sets[int(ef[p, k])].add(er if k < 2 else er ^ 1)
+    if responses is not None and len(responses):
+        R = np.asarray(responses, np.uint32).reshape(-1, 16)
+        M = np.asarray(meta, np.uint32).reshape(-1, 2)
+        for i in range(len(R)):
+            r = 2 * int(M[i, 0])
+            t = int(M[i, 1])
+            info = {0: (r, r ^ 1, r ^ 1, r ^ 1), 2: (r ^ 1, r, r, r), 1: (r, r, r ^ 1, r ^ 1)}[t]
+            for k in range(4):
+                sets[int(R[i, k] & INDEX_MASK)].add(info[k])
+    return [sorted(s) for s in sets]
+
+
+def _regions_match(regions, a, b):
+    ra = set()
+    for v in a:
+        ra.update(regions[v])
+    for v in b:
+        for q in regions[v]:
+            if (q ^ 1) in ra:
+                return True
+    return False
+
+
+def _record_sides(rec):
+    idx = [int(rec[k] & INDEX_MASK) for k in range(4)]
+    if rec[2] & TYPE_BIT:
+        return idx, [idx[0]], idx[1:]
+    return idx, idx[:2], idx[2:]
+
+
+def filter_proximity(records, x, offset, regions):
+    """(records with k1 = k2 = 0 for the disabled pairs, vert_is_invalid uint8[V]).
+    A vertex is invalid when it is part of a proximity pair closer than the pair's
+    thickness (fn_identify_invalid_verts_from_proximity_pairs, FP32 as the kernel);
+    a pair is disabled when it touches an invalid vertex or its two sides lie in
+    regions 2c + r and 2c + (r ^ 1) of one contour."""
+    R = np.array(records, np.uint32).reshape(-1, 16)
+    x = np.asarray(x, np.float32).reshape(-1, 3)
+    off = np.asarray(offset, np.float32)
+    invalid = np.zeros(len(x), np.uint8)
+    for rec in R:
+        idx, _, _ = _record_sides(rec)
+        b0, b1 = rec[10:12].view(np.float32)
+        one = np.float32(1)
+        if rec[2] & TYPE_BIT:
+            w = (one, -b0, -b1, b0 + b1 - one)
+        else:
+            w = (b0, one - b0, -b1, b1 - one)
+        d = np.zeros(3, np.float32)
+        for k in range(4):
+            d = d + x[idx[k]] * np.float32(w[k])
+        t = off[idx[0]] + off[idx[2]]
+        if np.float32(np.dot(d, d)) < t * t + np.float32(1e-10):
+            invalid[idx] = 1
+    out = R.copy()
+    for i, rec in enumerate(R):
+        idx, a, b = _record_sides(rec)
+        if invalid[idx].any() or _regions_match(regions, a, b):
+            out[i, 8] = 0  # k1
+            out[i, 9] = 0  # k2
+    return out, invalid
+
+
+def ccd_pair_mask(kind, broad, edges, faces, regions, invalid):
+    """True for the swept broad-phase pairs CCD keeps (fn_filter_broadphase_{vf,ee}_pairs):
+    kind 2 = (vertex, face), kind 3 = (edge, edge)."""
+    B = np.asarray(broad, np.uint32).reshape(-1, 2)
+    keep = np.ones(len(B), bool)
+    for i, (q, l) in enumerate(B):
+        if kind == 2:
+            a, b = [int(q)], [int(v) for v in faces[l]]
+        else:
+            a, b = [int(v) for v in edges[q]], [int(v) for v in edges[l]]
+        if invalid[a + b].any() or _regions_match(regions, a, b):
+            keep[i] = False
+    return keep

@@ -1578,6 +1578,12 @@ static void adj_contours(const or_prp_mesh* m, uint32_t c, const or_prp_hit* h, 
 uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa, void* choices_out,
 	uint32_t* recs, uint32_t cap)
 {
+	return or_prp_resolve_meta(m, max_contours, response_depth, kappa, choices_out, recs, NULL, cap);
+}
+
+uint32_t or_prp_resolve_meta(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa,
+	void* choices_out, uint32_t* recs, uint32_t* meta, uint32_t cap)
+{
 	const uint32_t nc = m->nc, active = nc < max_contours ? nc : max_contours;
 	or_prp_choice* ch = calloc(nc + 1, sizeof(or_prp_choice));
 	for (uint32_t c = 0; c < nc; ++c)
@@ -1678,7 +1684,11 @@ uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_de
 			f[0] = nn[0], f[1] = nn[1], f[2] = nn[2], f[3] = a, f[4] = stiff * C, f[5] = stiff;
 			memcpy(rec + 4, f, 48);
 			if (nr < cap)
+			{
 				memcpy(recs + 16 * (size_t)nr, rec, 64);
+				if (meta)
+					meta[2 * (size_t)nr] = c, meta[2 * (size_t)nr + 1] = h->type;
+			}
 			++nr;
 		}
 	}

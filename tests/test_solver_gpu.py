@@ -42,6 +42,7 @@ def _solver(name="two_sheets", precision=PCG_MIXED, n=None, untangle=False, iter
         cfg.use_untangling_PRP = True
         cfg.untangling_response_depth = 1.0
         cfg.nonlinear_iter_count = 1
+        cfg.use_region_filter = True
     else:
         cfg.use_untangling_PRP = False
     s.init_solver()
@@ -105,4 +106,5 @@ def test_untangling_steps_reduce_intersections(ctx):
         if data["num_pairs"] == 0:
             break
     assert responses[0] > 0
-    assert counts[-1] < counts[0], counts
+    # the oracle pipeline separates these sheets in one step (tests/test_untangle_cpu.py)
+    assert counts[-1] == 0 and len(counts) <= 3, counts

@@ -53,6 +53,10 @@ class SceneParams:
     pcg_inner_tol: float = 1e-2
     pcg_block_jacobi: bool = True
     prp_directions: int = 10  # 10 = the reference's direction generator; more appends a Fibonacci set
+    # filter_invalid_pairs / filter_broad_phase_pairs on proximity and CCD pairs.  The library
+    # refuses it (status -1) until its kernels are verified on the device; without it the CCD
+    # line search blocks the repair motion through the contours.
+    use_region_filter: bool = False
 
 
 @dataclass
@@ -125,6 +129,7 @@ class NewtonSolver:
         p.quasi_static = int(c.use_quasi_static_mode)
         p.prp = PrpParams(c.untangling_process_contours_count, c.prp_directions, c.PRP_cpu_contour_batch_size,
                           c.untangling_response_depth, c.stiffness_untangling, c.PRP_intrinsic_tau)
+        p.region_filter = int(c.use_region_filter)
         return p
 
     def init_solver(self):
