@@ -353,6 +353,7 @@ typedef struct ut_step_report {
 	uint32_t prp_ee_launches;
 	float	 ms_prp_ee;
 	uint64_t prp_ee_requests, prp_ee_visits, prp_ee_leaf_tests, prp_ee_hits;
+	uint64_t prp_ee_cand_tests; /* leaf tests whose target passed the candidate check */
 } ut_step_report;
 
 UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,

@@ -762,6 +762,7 @@ UT_API int ut_solver_step(ut_solver* s, ut_step_report* report)
 		report->prp_ee_launches = r.prp_ee.launches, report->ms_prp_ee = r.prp_ee.ms;
 		report->prp_ee_requests = r.prp_ee.requests, report->prp_ee_visits = r.prp_ee.visits;
 		report->prp_ee_leaf_tests = r.prp_ee.leaf_tests, report->prp_ee_hits = r.prp_ee.hits;
+		report->prp_ee_cand_tests = r.prp_ee.cand_tests;
 	}
 	return ut::kOk;
 }
@@ -1030,12 +1031,12 @@ UT_API int ut_prp_distances(ut_prp* p, uint32_t local_contour, uint32_t role, ui
 			const size_t   off = (size_t)(2 * local_contour + role) * r.nv;
 			const hipStream_t s = r.ctx->stream;
 			if (hop)
-				UT_HIP_CHECK(hipMemcpyAsync(hop, r.hop.get() + off, 4 * (size_t)r.nv, hipMemcpyDeviceToHost, s));
+				UT_HIP_CHECK(hipMemcpyAsync(hop, r.hop_b + off, 4 * (size_t)r.nv, hipMemcpyDeviceToHost, s));
 			if (geo)
-				UT_HIP_CHECK(hipMemcpyAsync(geo, r.geo.get() + off, 4 * (size_t)r.nv, hipMemcpyDeviceToHost, s));
+				UT_HIP_CHECK(hipMemcpyAsync(geo, r.geo_b + off, 4 * (size_t)r.nv, hipMemcpyDeviceToHost, s));
 			UT_HIP_CHECK(hipStreamSynchronize(s));
 			if (diameter)
-				*diameter = r.h_diam[2 * local_contour + role];
+				*diameter = r.diam_b[2 * local_contour + role];
 		});
 }
 

@@ -431,14 +431,20 @@ __device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots
 	return lo;
 }
 
-__device__ __forceinline__ void emit_hit(const RayArgs& a, bool ok, const PrpHit& h, int kind)
+// all 64 lanes call this; `acc` (meaningful in lane 0) counts the wave's hits
+__device__ __forceinline__ void emit_hit(const RayArgs& a, bool ok, const PrpHit& h, unsigned long long& acc)
 {
 	const uint32_t i = wave_append(a.count, ok);
 	if (ok && i < a.cap)
 		a.hits[i] = h;
-	const uint64_t m = __ballot(ok);
-	if (m && __lane_id() == 0)
-		atomicAdd(&a.tstat[4 * kind + 2], (unsigned long long)__popcll(m));
+	acc += (unsigned long long)__popcll(__ballot(ok));
+}
+
+// one atomic per wave for the counters a traversal accumulated in lane 0
+__device__ __forceinline__ void flush_stat(unsigned long long* p, unsigned long long v)
+{
+	if (__lane_id() == 0 && v)
+		atomicAdd(p, v);
 }
 
 // Per-lane request state shared by the three ray kinds.
@@ -460,9 +466,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
 	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
-		const uint32_t r = base + threadIdx.x;
-		const bool	   active = r < total;
-		RayReq		   q{};
+		const uint32_t	   r = base + threadIdx.x;
+		const bool		   active = r < total;
+		unsigned long long hit_acc = 0; // lane 0: hits of this wave's traversal
+		RayReq			   q{};
 		if (active)
 		{
 			q.slot = slot_of(off, a.nslots, r);
@@ -526,10 +533,11 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 					}
 				}
 			}
-			emit_hit(a, ok, h, Kind);
+			emit_hit(a, ok, h, hit_acc);
 		};
 		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow,
 			a.tstat + 4 * Kind);
+		flush_stat(&a.tstat[4 * Kind + 2], hit_acc);
 	}
 }
 
@@ -543,9 +551,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
 	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
-		const uint32_t r = base + threadIdx.x;
-		bool		   active = r < total;
-		box3		   sw = box_empty(), cb = box_empty();
+		const uint32_t	   r = base + threadIdx.x;
+		bool			   active = r < total;
+		unsigned long long hit_acc = 0, cand_acc = 0; // lane 0: hits and candidate leaf tests of the wave
+		box3			   sw = box_empty(), cb = box_empty();
 		f3			   pn = mk3(0.0f, 0.0f, 0.0f);
 		float		   pd = 0.0f;
 		if (active)
@@ -611,7 +620,9 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 				const uint32_t	b0 = a.edges[2 * g], b1 = a.edges[2 * g + 1];
 				const uint32_t* H = a.hop_all + (size_t)s1 * a.V;
 				const uint8_t*	O = a.own_all + (size_t)s1 * a.V;
-				if (a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1 && cand_edge(H, O, a.kring[s1], b0, b1))
+				const bool cand = a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1 && cand_edge(H, O, a.kring[s1], b0, b1);
+				cand_acc += (unsigned long long)__popcll(__ballot(cand)); // lane 0 is valid whenever narrow runs
+				if (cand)
 				{
 					const f3	dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
 					const f3	p0 = ld3(a.x, a0), p1 = ld3(a.x, a1), e1 = p1 - p0;
@@ -658,10 +669,12 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 					}
 				}
 			}
-			emit_hit(a, ok, h, 1);
+			emit_hit(a, ok, h, hit_acc);
 		};
 		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow,
 			a.tstat + 4);
+		flush_stat(&a.tstat[4 + 2], hit_acc);
+		flush_stat(&a.tstat[4 + 3], cand_acc);
 	}
 }
 
@@ -671,7 +684,11 @@ struct HitArgs
 	uint32_t		n, V, prim_bits;
 	const PrpHit*	h;		 // sorted by (slot, type, src)
 	const uint32_t* keys;	 // sorted keys
+	const uint32_t* dst;	 // target primitive of each sorted hit
 	const uint32_t* range;	 // [2 * slots] hit range of each slot
+	const uint64_t* table;	 // bucket key -> first index (generation | key | index), or null
+	uint32_t		table_cap;
+	uint64_t		gen;
 	uint32_t*		parent;	 // union-find
 	const uint32_t *vv_off, *vv, *ve_off, *ve, *vf_off, *vf;
 	const uint32_t *edges, *faces, *fae, *eaf;
@@ -766,58 +783,165 @@ __device__ __forceinline__ bool edges_adjacent(const HitArgs& a, uint32_t e1, ui
 	return false;
 }
 
-// visit every hit of bucket (slot, type, prim) and union with i when pred(j) holds
-template <typename Pred>
-__device__ __forceinline__ void probe(const HitArgs& a, uint32_t i, uint32_t slot, uint32_t type, uint32_t prim,
-	const Pred& pred)
+__device__ __forceinline__ uint64_t mix64(uint64_t k)
 {
-	// keys carry the slot in their top bits: search only the slot's own range
-	const uint32_t key = hit_key(slot, type, prim, a.prim_bits);
-	const uint32_t lo = a.range[2 * slot], hi = a.range[2 * slot + 1];
-	for (uint32_t j = lo + lower_bound_u32(a.keys + lo, hi - lo, key); j < hi && a.keys[j] == key; ++j)
-		if (j != i && pred(a.h[j]))
-			uf_union(a.parent, i, j);
+	k ^= k >> 33;
+	k *= 0xff51afd7ed558ccdull;
+	k ^= k >> 33;
+	k *= 0xc4ceb9fe1a85ec53ull;
+	k ^= k >> 33;
+	return k;
 }
 
-// find_adjacent_hits (intersection_resolver_helper.cpp:771-902) as bucket probes
+// Bucket table: entry = generation (8 bits) | key (32 bits) | first index (24 bits);
+// entries of other generations are empty.  One insert per bucket start.
+__global__ void k_prp_bucket_insert(uint32_t n, const uint32_t* __restrict__ keys, uint64_t* table, uint32_t cap,
+	uint64_t gen)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n || (i > 0 && keys[i] == keys[i - 1]))
+		return;
+	const uint64_t e = (gen << 56) | ((uint64_t)keys[i] << 24) | i;
+	for (uint32_t s = (uint32_t)mix64(keys[i]) & (cap - 1), c = 0; c < cap; s = (s + 1) & (cap - 1), ++c)
+	{
+		uint64_t cur = __hip_atomic_load(&table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		while ((cur >> 56) != gen)
+		{
+			const uint64_t prev =
+				atomicCAS((unsigned long long*)&table[s], (unsigned long long)cur, (unsigned long long)e);
+			if (prev == cur)
+				return;
+			cur = prev;
+		}
+	}
+}
+
+__device__ __forceinline__ uint32_t bucket_first(const HitArgs& a, uint32_t key)
+{
+	for (uint32_t s = (uint32_t)mix64(key) & (a.table_cap - 1), c = 0; c < a.table_cap;
+		s = (s + 1) & (a.table_cap - 1), ++c)
+	{
+		const uint64_t e = a.table[s];
+		if ((e >> 56) != a.gen)
+			return kNone;
+		if ((uint32_t)(e >> 24) == key)
+			return (uint32_t)(e & 0xFFFFFFu);
+	}
+	return kNone;
+}
+
+// Index of the hit (slot, type, source prim) -> dst d, or kNone.  Hits are
+// sorted by (key, dst) and a (source, dst) pair occurs at most once per slot:
+// the bucket is found through the table (or a search of the slot's range), then
+// d by galloping from the bucket start, so large buckets cost O(log size).
+__device__ __forceinline__ uint32_t find_hit(const HitArgs& a, uint32_t slot, uint32_t type, uint32_t prim, uint32_t d)
+{
+	const uint32_t key = hit_key(slot, type, prim, a.prim_bits);
+	const uint32_t hi = a.range[2 * slot + 1];
+	uint32_t	   lo;
+	if (a.table)
+	{
+		lo = bucket_first(a, key);
+		if (lo == kNone)
+			return kNone;
+	}
+	else
+	{
+		const uint32_t l0 = a.range[2 * slot];
+		lo = l0 + lower_bound_u32(a.keys + l0, hi - l0, key);
+		if (lo >= hi || a.keys[lo] != key)
+			return kNone;
+	}
+	auto	 less = [&](uint32_t m) { return a.keys[m] == key && a.dst[m] < d; };
+	uint32_t top = hi;
+	for (uint32_t step = 1;; step <<= 1)
+	{
+		const uint32_t m = lo + step - 1;
+		if (m >= hi || !less(m))
+		{
+			top = m < hi ? m : hi;
+			break;
+		}
+		lo = m + 1;
+	}
+	while (lo < top)
+	{
+		const uint32_t mid = (lo + top) >> 1;
+		if (less(mid))
+			lo = mid + 1;
+		else
+			top = mid;
+	}
+	return (lo < hi && a.keys[lo] == key && a.dst[lo] == d) ? lo : kNone;
+}
+
+// find_adjacent_hits (intersection_resolver_helper.cpp:771-902): every rule
+// relates hit i to hits of a neighbouring source primitive whose target lies in
+// a small set fixed by i's target, so each rule is a few exact lookups.  The
+// relation is symmetric (VF-FV, VF-EE and FV-EE are one condition seen from
+// either side, and the same-type rules mirror), so each adjacent pair is looked
+// up from one side only: cross-type pairs from the VF / FV hit, same-type pairs
+// from the hit whose varying primitive is smaller.
 __global__ void k_prp_hit_union(HitArgs a)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= a.n)
 		return;
-	const PrpHit L = a.h[i];
+	const PrpHit   L = a.h[i];
 	const uint32_t s = L.slot;
+	auto		   link = [&](uint32_t type, uint32_t prim, uint32_t d)
+	{
+		const uint32_t j = find_hit(a, s, type, prim, d);
+		if (j != kNone && j != i)
+			uf_union(a.parent, i, j);
+	};
 	if (L.type == 0) // VF(v, f)
 	{
 		const uint32_t v = L.src, f = L.dst;
+		// VF(w, f), w adjacent to v (w > v: the pair is symmetric)
 		for (uint32_t k = a.vv_off[v]; k < a.vv_off[v + 1]; ++k)
-			probe(a, i, s, 0, a.vv[k], [&](const PrpHit& R) { return R.dst == f; });
+			if (a.vv[k] > v)
+				link(0, a.vv[k], f);
+		// FV(f', u), f' a face of v, u a vertex of f
 		for (uint32_t k = a.vf_off[v]; k < a.vf_off[v + 1]; ++k)
-			probe(a, i, s, 2, a.vf[k], [&](const PrpHit& R) { return face_has_vert(a.faces, f, R.dst); });
+			for (int q = 0; q < 3; ++q)
+				link(2, a.vf[k], a.faces[3 * f + q]);
+		// EE(e, e'), e an edge of v, e' an edge of f
 		for (uint32_t k = a.ve_off[v]; k < a.ve_off[v + 1]; ++k)
-			probe(a, i, s, 1, a.ve[k], [&](const PrpHit& R) { return face_has_edge(a.eaf, f, R.dst); });
+			for (int q = 0; q < 3; ++q)
+				link(1, a.ve[k], a.fae[3 * f + q]);
 	}
 	else if (L.type == 2) // FV(f, v)
 	{
 		const uint32_t f = L.src, v = L.dst;
+		// (VF(u, f'), u a vertex of f, f' a face of v: found from the VF side)
+		// FV(f, u), u adjacent to v (u > v)
+		for (uint32_t q = a.vv_off[v]; q < a.vv_off[v + 1]; ++q)
+			if (a.vv[q] > v)
+				link(2, f, a.vv[q]);
+		// EE(e, e'), e an edge of f, e' an edge of v
 		for (int k = 0; k < 3; ++k)
-			probe(a, i, s, 0, a.faces[3 * f + k], [&](const PrpHit& R) { return face_has_vert(a.faces, R.dst, v); });
-		probe(a, i, s, 2, f, [&](const PrpHit& R) { return verts_adjacent(a, v, R.dst); });
-		for (int k = 0; k < 3; ++k)
-			probe(a, i, s, 1, a.fae[3 * f + k], [&](const PrpHit& R) { return edge_has_vert(a.edges, R.dst, v); });
+			for (uint32_t q = a.ve_off[v]; q < a.ve_off[v + 1]; ++q)
+				link(1, a.fae[3 * f + k], a.ve[q]);
 	}
 	else // EE(e1, e2)
 	{
 		const uint32_t e1 = L.src, e2 = L.dst;
-		for (int k = 0; k < 2; ++k)
-			probe(a, i, s, 0, a.edges[2 * e1 + k], [&](const PrpHit& R) { return face_has_edge(a.eaf, R.dst, e2); });
-		for (int k = 0; k < 2; ++k)
+		// (VF and FV partners are found from their side)
+		// EE(e1, e'), e' another edge of a face of e2 (e' > e2)
+		for (int q = 0; q < 2; ++q)
 		{
-			const uint32_t f = a.eaf[2 * e1 + k];
-			if (f != kNone)
-				probe(a, i, s, 2, f, [&](const PrpHit& R) { return edge_has_vert(a.edges, e2, R.dst); });
+			const uint32_t g = a.eaf[2 * e2 + q];
+			if (g == kNone)
+				continue;
+			for (int j = 0; j < 3; ++j)
+			{
+				const uint32_t ae = a.fae[3 * g + j];
+				if (ae > e2)
+					link(1, e1, ae);
+			}
 		}
-		probe(a, i, s, 1, e1, [&](const PrpHit& R) { return edges_adjacent(a, e2, R.dst); });
+		// EE(e', e2), e' another edge of a face of e1 (e' > e1)
 		for (int k = 0; k < 2; ++k)
 		{
 			const uint32_t f = a.eaf[2 * e1 + k];
@@ -826,8 +950,8 @@ __global__ void k_prp_hit_union(HitArgs a)
 			for (int j = 0; j < 3; ++j)
 			{
 				const uint32_t ae = a.fae[3 * f + j];
-				if (ae != e1)
-					probe(a, i, s, 1, ae, [&](const PrpHit& R) { return R.dst == e2; });
+				if (ae > e1)
+					link(1, ae, e2);
 			}
 		}
 	}
@@ -849,6 +973,7 @@ struct CompArgs
 	uint32_t*		nroots;
 	uint64_t*		hash;
 	uint32_t		hash_cap;
+	uint64_t		gen; // support-key generation of this evaluation
 	uint64_t*		slot_best; // [2 * nslots]: (cov << 32 | count), then key
 	uint32_t*		slot_root; // [nslots]
 	const uint32_t *cpair_off, *cpair_rec;
@@ -974,14 +1099,29 @@ __device__ __forceinline__ uint64_t hmix(uint64_t k)
 	return k;
 }
 
+// Open-addressing set of 64-bit keys whose top 8 bits are the evaluation's
+// generation: an entry of another generation (or the ~0 fill) counts as empty,
+// so the table is cleared only when the generation counter wraps.
+__device__ __forceinline__ uint64_t key_gen(uint64_t k) { return k >> 56; }
+
 __device__ __forceinline__ void hash_insert(uint64_t* t, uint32_t cap, uint64_t k)
 {
+	const uint64_t gen = key_gen(k);
 	for (uint32_t i = (uint32_t)hmix(k) & (cap - 1), n = 0; n < cap; i = (i + 1) & (cap - 1), ++n)
 	{
-		const uint64_t old = atomicCAS((unsigned long long*)&t[i], (unsigned long long)kHashEmpty,
-			(unsigned long long)k);
-		if (old == kHashEmpty || old == k)
-			return;
+		uint64_t cur = __hip_atomic_load(&t[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		for (;;)
+		{
+			if (cur == k)
+				return;
+			if (key_gen(cur) == gen)
+				break; // another key of this evaluation: probe on
+			const uint64_t prev =
+				atomicCAS((unsigned long long*)&t[i], (unsigned long long)cur, (unsigned long long)k);
+			if (prev == cur)
+				return;
+			cur = prev; // lost the race: look at what was written
+		}
 	}
 }
 
@@ -992,15 +1132,16 @@ __device__ __forceinline__ bool hash_has(const uint64_t* t, uint32_t cap, uint64
 		const uint64_t v = t[i];
 		if (v == k)
 			return true;
-		if (v == kHashEmpty)
+		if (key_gen(v) != key_gen(k))
 			return false;
 	}
 	return false;
 }
 
-__device__ __forceinline__ uint64_t support_key(uint32_t root, uint32_t v, uint32_t side)
+// generation (8 bits) | root hit (29 bits) | vertex (26 bits) | side (1 bit)
+__device__ __forceinline__ uint64_t support_key(uint64_t gen, uint32_t root, uint32_t v, uint32_t side)
 {
-	return ((uint64_t)root << 32) | ((uint64_t)v << 1) | side;
+	return (gen << 56) | ((uint64_t)root << 27) | ((uint64_t)v << 1) | side;
 }
 
 // support vertices of every hit in an attached cluster: (root, vertex, 0 = source / 1 = target)
@@ -1017,9 +1158,9 @@ __global__ void k_prp_support(CompArgs a)
 	uint32_t	   sv[3], dv[3];
 	const uint32_t ns = src_verts(a.edges, a.faces, h, sv), nd = dst_verts(a.edges, a.faces, h, dv);
 	for (uint32_t k = 0; k < ns; ++k)
-		hash_insert(a.hash, a.hash_cap, support_key(h.root, sv[k], 0));
+		hash_insert(a.hash, a.hash_cap, support_key(a.gen, h.root, sv[k], 0));
 	for (uint32_t k = 0; k < nd; ++k)
-		hash_insert(a.hash, a.hash_cap, support_key(h.root, dv[k], 1));
+		hash_insert(a.hash, a.hash_cap, support_key(a.gen, h.root, dv[k], 1));
 }
 
 // EF-anchor coverage of each attached cluster (component_anchor_coverage :3587-3621):
@@ -1041,11 +1182,11 @@ __global__ void k_prp_coverage(CompArgs a)
 			const uint32_t* q = a.cpair_rec + 6 * p;
 			const uint32_t	m = q[5];
 			const uint32_t	es = m == 0 ? 0u : 1u, fs = es ^ 1u;
-			const bool		e_ok = hash_has(a.hash, a.hash_cap, support_key(r, q[0], es))
-				|| hash_has(a.hash, a.hash_cap, support_key(r, q[1], es));
-			const bool f_ok = hash_has(a.hash, a.hash_cap, support_key(r, q[2], fs))
-				|| hash_has(a.hash, a.hash_cap, support_key(r, q[3], fs))
-				|| hash_has(a.hash, a.hash_cap, support_key(r, q[4], fs));
+			const bool		e_ok = hash_has(a.hash, a.hash_cap, support_key(a.gen, r, q[0], es))
+				|| hash_has(a.hash, a.hash_cap, support_key(a.gen, r, q[1], es));
+			const bool f_ok = hash_has(a.hash, a.hash_cap, support_key(a.gen, r, q[2], fs))
+				|| hash_has(a.hash, a.hash_cap, support_key(a.gen, r, q[3], fs))
+				|| hash_has(a.hash, a.hash_cap, support_key(a.gen, r, q[4], fs));
 			cnt += (e_ok && f_ok) ? 1u : 0u;
 		}
 		__shared__ uint32_t part[16];
@@ -1215,24 +1356,37 @@ __global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
 	}
 }
 
-__global__ void k_prp_keys(uint32_t n, const PrpHit* __restrict__ h, uint32_t pb, uint32_t* __restrict__ keys,
+// two-pass LSD order (slot, type, src, dst): first by dst, then stably by the bucket key
+__global__ void k_prp_dst_keys(uint32_t n, const PrpHit* __restrict__ h, uint32_t* __restrict__ keys,
 	uint32_t* __restrict__ vals)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	keys[i] = hit_key(h[i].slot, h[i].type, h[i].src, pb);
+	keys[i] = h[i].dst;
 	vals[i] = i;
 }
 
+__global__ void k_prp_keys(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ order, uint32_t pb,
+	uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const uint32_t j = order[i];
+	keys[i] = hit_key(h[j].slot, h[j].type, h[j].src, pb);
+	vals[i] = j;
+}
+
 __global__ void k_prp_gather(uint32_t n, const PrpHit* __restrict__ in, const uint32_t* __restrict__ perm,
-	PrpHit* __restrict__ out, uint32_t nslots, uint32_t* __restrict__ range)
+	PrpHit* __restrict__ out, uint32_t* __restrict__ dst, uint32_t nslots, uint32_t* __restrict__ range)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	const PrpHit h = in[perm[i]];
 	out[i] = h;
+	dst[i] = h.dst;
 	const uint32_t prev = i > 0 ? in[perm[i - 1]].slot : kNone;
 	const uint32_t next = i + 1 < n ? in[perm[i + 1]].slot : kNone;
 	if (prev != h.slot)
@@ -1742,16 +1896,15 @@ void PrpResolver::distances(uint32_t c0, uint32_t n)
 {
 	const hipStream_t s = ctx->stream;
 	const size_t	  V = nv;
-	hop.reserve(2 * (size_t)params.batch_contours * V);
-	geo.reserve(2 * (size_t)params.batch_contours * V);
-	queue.reserve(4 * (size_t)params.batch_contours * V);
-	inq.reserve(2 * (size_t)params.batch_contours * V);
+	const size_t	  nn = std::max<size_t>(n, params.batch_contours);
+	hop.reserve(2 * nn * V);
+	geo.reserve(2 * nn * V);
+	queue.reserve(4 * nn * V);
+	inq.reserve(2 * nn * V);
 	own.reserve(2 * (size_t)params.batch_contours * V);
-	diameter.reserve(2 * (size_t)params.batch_contours);
+	diameter.reserve(2 * nn);
 	k_prp_distances<<<2 * n, kBfsThreads, 0, s>>>(nv, c0, b_off.get(), b_verts.get(), vv_off.get(), vv.get(), rest_x,
 		hop.get(), geo.get(), queue.get(), inq.get(), diameter.get());
-	const uint32_t m = 2 * n * nv;
-	k_prp_fill_u8<<<div_up(m, 256), 256, 0, s>>>(m, own.get(), (uint8_t)kOwnUnknown);
 	UT_LAUNCH_CHECK();
 	h_diam.resize(2 * n);
 	UT_HIP_CHECK(hipMemcpyAsync(h_diam.data(), diameter.get(), 8 * (size_t)n, hipMemcpyDeviceToHost, s));
@@ -1790,7 +1943,21 @@ void PrpResolver::prepare(const float* x, const ContourBuilder& ct, uint32_t c0,
 	if (n == 0)
 		return;
 	distances(c0, n);
+	begin_batch(0);
 	directions(c0, n);
+}
+
+// The batch's (contour, role) rows start at contour `first` of the distance
+// arrays; ownership is recomputed for the batch.
+void PrpResolver::begin_batch(uint32_t first)
+{
+	const hipStream_t s = ctx->stream;
+	hop_b = hop.get() + 2 * (size_t)first * nv;
+	geo_b = geo.get() + 2 * (size_t)first * nv;
+	diam_b = h_diam.data() + 2 * (size_t)first;
+	const uint32_t m = 2 * batch_n * nv;
+	k_prp_fill_u8<<<div_up(m, 256), 256, 0, s>>>(m, own.get(), (uint8_t)kOwnUnknown);
+	UT_LAUNCH_CHECK();
 }
 
 void PrpResolver::ensure_hits(uint32_t cap)
@@ -1798,7 +1965,7 @@ void PrpResolver::ensure_hits(uint32_t cap)
 	if (cap <= hit_cap)
 		return;
 	hit_cap = cap;
-	hits.reserve(cap), hits_sorted.reserve(cap);
+	hits.reserve(cap), hits_sorted.reserve(cap), hit_dst.reserve(cap);
 	for (int i = 0; i < 2; ++i)
 		sort_k[i].reserve(cap), sort_v[i].reserve(cap);
 	sort_h.reserve(radix_sort_hist_size(cap));
@@ -1831,13 +1998,13 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		ensure_hits(1u << 23); // 8M hits (256 MB): the 256^2 fold needs ~5M per evaluation
 	// ownership and candidates
 	k_prp_own<<<dim3(div_up(V, 256), n2), 256, 0, s>>>(
-		V, batch0, n2, kring.get(), hop.get(), geo.get(), x_dev, b_off.get(), b_verts.get(), params.tau, own.get());
+		V, batch0, n2, kring.get(), hop_b, geo_b, x_dev, b_off.get(), b_verts.get(), params.tau, own.get());
 	k_prp_fill_u32<<<div_up(2 * n2, 256), 256, 0, s>>>(2 * n2, cand_cnt.get(), 0u);
 	k_prp_fill_u32<<<div_up(n2, 256), 256, 0, s>>>(n2, target_cnt.get(), 0u);
 	k_prp_cand_lists<<<dim3(div_up(std::max(V, E), 256), n2), 256, 0, s>>>(
-		V, E, n2, kring.get(), hop.get(), own.get(), edges, cand.get(), cand_cnt.get());
+		V, E, n2, kring.get(), hop_b, own.get(), edges, cand.get(), cand_cnt.get());
 	k_prp_any_faces<<<dim3(div_up(F, 256), n2), 256, 0, s>>>(
-		F, n2, kring.get(), V, hop.get(), own.get(), faces, target_cnt.get());
+		F, n2, kring.get(), V, hop_b, own.get(), faces, target_cnt.get());
 	slot_tbox.reserve(6 * (size_t)ns);
 	k_prp_slot_prep<<<ns, 256, 0, s>>>(V, E, ns, slots.get(), cand.get(), cand_cnt.get(), x_dev, edges,
 		slot_tbox.get());
@@ -1846,7 +2013,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	uint32_t nh = 0;
 	for (int attempt = 0; attempt < 8; ++attempt)
 	{
-		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop.get(), own.get(), kring.get(), x_dev, edges,
+		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop_b, own.get(), kring.get(), x_dev, edges,
 			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr, slot_tbox.get(),
 			tstat.get() };
 		k_prp_fill_u64<<<1, 64, 0, s>>>(12, reinterpret_cast<uint64_t*>(tstat.get()), 0ull);
@@ -1881,6 +2048,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 			ee_prof.visits += ts[4];
 			ee_prof.leaf_tests += ts[5];
 			ee_prof.hits += ts[6];
+			ee_prof.cand_tests += ts[7];
 		}
 		UT_REQUIRE(st[0] == 1u, "prp: ray traversal revisited a node (corrupt tree)");
 		UT_REQUIRE(st[1] == 0u, "prp: ray traversal stack overflow");
@@ -1896,40 +2064,73 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	UT_LAUNCH_CHECK();
 	if (nh > 0)
 	{
-		// sort by (slot, type, source primitive)
-		k_prp_keys<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), prim_bits, sort_k[0].get(), sort_v[0].get());
+		// sort by (slot, type, source primitive, target primitive): by dst, then stably by the key
+		k_prp_dst_keys<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), sort_k[0].get(), sort_v[0].get());
+		RadixSortBuffers r1;
+		r1.keys[0] = sort_k[0].get(), r1.keys[1] = sort_k[1].get();
+		r1.vals[0] = sort_v[0].get(), r1.vals[1] = sort_v[1].get();
+		r1.hist = sort_h.get();
+		radix_sort_pairs(r1, nh, (int)prim_bits, s);
+		const int a1 = r1.result, b1 = a1 ^ 1;
+		k_prp_keys<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), sort_v[a1].get(), prim_bits, sort_k[b1].get(),
+			sort_v[b1].get());
 		RadixSortBuffers rb;
-		rb.keys[0] = sort_k[0].get(), rb.keys[1] = sort_k[1].get();
-		rb.vals[0] = sort_v[0].get(), rb.vals[1] = sort_v[1].get();
+		rb.keys[0] = sort_k[b1].get(), rb.keys[1] = sort_k[a1].get();
+		rb.vals[0] = sort_v[b1].get(), rb.vals[1] = sort_v[a1].get();
 		rb.hist = sort_h.get();
 		const uint32_t slot_need = 32 - __builtin_clz(std::max(ns, 2u) - 1);
 		radix_sort_pairs(rb, nh, (int)(slot_need + 2 + prim_bits), s);
 		const uint32_t* keys = rb.keys[rb.result];
 		const uint32_t* perm = rb.vals[rb.result];
-		k_prp_gather<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), perm, hits_sorted.get(), ns, slot_range.get());
+		k_prp_gather<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), perm, hits_sorted.get(), hit_dst.get(), ns,
+			slot_range.get());
 		uint32_t* parent = comp_info.get();
 		uint32_t* attached = parent + hit_cap;
 		uint32_t* cnt = attached + hit_cap;
 		uint32_t* cov = cnt + hit_cap;
 		k_prp_uf_init<<<div_up(nh, 256), 256, 0, s>>>(nh, parent, attached, cnt, comp_key.get(), cov);
-		HitArgs ha{ nh, V, prim_bits, hits_sorted.get(), keys, slot_range.get(), parent, vv_off.get(), vv.get(),
-			ve_off.get(), ve.get(), vf_off.get(), vf.get(), edges, faces, fae, eaf };
+		// bucket table (2 entries per hit, generation-tagged; binary search beyond 2^24 hits)
+		const uint64_t* table = nullptr;
+		uint32_t		tcap = 1;
+		uint64_t		tgen = 0;
+		if (nh < (1u << 24))
+		{
+			while (tcap < 2u * nh)
+				tcap <<= 1;
+			if (tcap > bucket_cap || bucket_gen >= 255u)
+			{
+				bucket_cap = std::max(tcap, bucket_cap);
+				buckets.reserve(bucket_cap);
+				k_prp_fill_u64<<<div_up(bucket_cap, 256), 256, 0, s>>>(bucket_cap, buckets.get(), kHashEmpty);
+				bucket_gen = 0;
+			}
+			tcap = bucket_cap;
+			tgen = bucket_gen++;
+			k_prp_bucket_insert<<<div_up(nh, 256), 256, 0, s>>>(nh, keys, buckets.get(), tcap, tgen);
+			table = buckets.get();
+		}
+		HitArgs ha{ nh, V, prim_bits, hits_sorted.get(), keys, hit_dst.get(), slot_range.get(), table, tcap, tgen, parent,
+			vv_off.get(), vv.get(), ve_off.get(), ve.get(), vf_off.get(), vf.get(), edges, faces, fae, eaf };
 		k_prp_hit_union<<<div_up(nh, 256), 256, 0, s>>>(ha);
 		k_prp_uf_compress<<<div_up(nh, 256), 256, 0, s>>>(nh, parent);
 		// support hash: at most 6 entries per hit, load factor <= 1/2
 		uint32_t hc = 1;
 		while (hc < 12u * nh)
 			hc <<= 1;
-		if (hc > hash_cap)
+		if (hc > hash_cap || hash_gen >= 255u)
 		{
-			hash_cap = hc;
+			// new table or wrapped generation: clear to ~0 (generation 255, never used)
+			hash_cap = std::max(hc, hash_cap);
 			hash.reserve(hash_cap);
+			k_prp_fill_u64<<<div_up(hash_cap, 256), 256, 0, s>>>(hash_cap, hash.get(), kHashEmpty);
+			hash_gen = 0;
 		}
-		k_prp_fill_u64<<<div_up(hash_cap, 256), 256, 0, s>>>(hash_cap, hash.get(), kHashEmpty);
+		const uint64_t gen = hash_gen++;
+		UT_REQUIRE(nh < (1u << 29), "prp: too many hits for the support keys");
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, counters.get() + 1, 0u);
 		UT_LAUNCH_CHECK();
 		CompArgs ca{ nh, ns, V, batch0, hits_sorted.get(), parent, slots.get(), flag_off.get(), flags.get(), edges,
-			faces, attached, cnt, comp_key.get(), cov, roots.get(), counters.get() + 1, hash.get(), hash_cap,
+			faces, attached, cnt, comp_key.get(), cov, roots.get(), counters.get() + 1, hash.get(), hash_cap, gen,
 			slot_best.get(), slot_root.get(), cpair_off.get(), cpair_rec.get() };
 		k_prp_components<<<div_up(nh, 256), 256, 0, s>>>(ca, hits_sorted.get());
 		k_prp_support<<<div_up(nh, 256), 256, 0, s>>>(ca);
@@ -1944,7 +2145,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	float* en = fn + 3 * (size_t)nf;
 	float* vn = en + 3 * (size_t)ne;
 	EvalArgs ea_{ ns, V, batch0, hits_sorted.get(), slot_range.get(), slot_root.get(),
-		comp_info.get() + 3 * (size_t)hit_cap, slots.get(), cpair_off.get(), edges, faces, hop.get(), va, ea, fa, mass,
+		comp_info.get() + 3 * (size_t)hit_cap, slots.get(), cpair_off.get(), edges, faces, hop_b, va, ea, fa, mass,
 		fn, en, vn, summary.get() };
 	k_prp_eval<<<ns, 256, 0, s>>>(ea_);
 	UT_LAUNCH_CHECK();
@@ -1971,6 +2172,10 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 	if (final_cap < 1024)
 		final_cap = 1024;
 	uint32_t n_final = 0;
+	// hop and rest-geodesic distances of every processed contour in one launch
+	// (2 * active workgroups); the batches below read their rows of it
+	if (active)
+		distances(0, active);
 	const auto t1 = clk::now();
 	double	   ms_levels = 0.0, ms_final = 0.0;
 	for (uint32_t c0 = 0; c0 < active; c0 += params.batch_contours)
@@ -1978,7 +2183,7 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 		const uint32_t n = std::min(params.batch_contours, active - c0);
 		batch0 = c0, batch_n = n;
 		const auto tl0 = clk::now();
-		distances(c0, n);
+		begin_batch(c0);
 		directions(c0, n);
 		// adaptive k-ring driver (evaluate_current_runtime_directions_template :1058-1230)
 		std::vector<uint32_t>				 ks(n), kd(n), D0(n), D1(n);
@@ -1987,7 +2192,7 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 		for (uint32_t lc = 0; lc < n; ++lc)
 		{
 			const uint32_t c = c0 + lc;
-			D0[lc] = h_diam[2 * lc], D1[lc] = h_diam[2 * lc + 1];
+			D0[lc] = diam_b[2 * lc], D1[lc] = diam_b[2 * lc + 1];
 			ks[lc] = initial_k(h_b_off[2 * c + 1] - h_b_off[2 * c], D0[lc]);
 			kd[lc] = initial_k(h_b_off[2 * c + 2] - h_b_off[2 * c + 1], D1[lc]);
 			R[lc].assign(h_dirs[c].size() / 3, ComboState{});

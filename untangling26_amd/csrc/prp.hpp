@@ -84,6 +84,7 @@ struct PrpKernelProfile
 	uint32_t launches = 0;
 	float	 ms = 0;
 	uint64_t requests = 0, visits = 0, leaf_tests = 0, hits = 0;
+	uint64_t cand_tests = 0; // leaf tests whose target is a candidate (the rest are culled by hop/ownership)
 };
 
 struct PrpStats
@@ -124,7 +125,11 @@ struct PrpResolver
 	DevBuf<uint32_t> hop;					 // [2 * batch][V]
 	DevBuf<float>	 geo, curd;				 // [2 * batch][V]
 	DevBuf<uint32_t> queue, inq;			 // BFS / Bellman-Ford work lists [2 * batch][2V], [2 * batch][V]
-	DevBuf<uint32_t> diameter;				 // [2 * batch]
+	DevBuf<uint32_t> diameter;				 // [2 * contours]
+	// rows of the current batch in hop / geo / h_diam (distances may hold more contours)
+	const uint32_t*	 hop_b = nullptr;
+	const float*	 geo_b = nullptr;
+	const uint32_t*	 diam_b = nullptr;
 	DevBuf<uint8_t>	 own;					 // [2 * batch][V] ownership (valid where hop <= k)
 	DevBuf<uint32_t> kring;					 // [2 * batch] current k of each (contour, role)
 	DevBuf<uint32_t> cand_off, cand;		 // per (contour, role): candidate vertex and edge lists
@@ -136,12 +141,16 @@ struct PrpResolver
 	DevBuf<PrpSlot>	 slots;
 	DevBuf<float>	 slot_tbox;				 // [6 * slots] box of the candidate target edges (EE culling)
 	DevBuf<PrpHit>	 hits, hits_sorted;
+	DevBuf<uint32_t> hit_dst; // target primitive of each sorted hit
 	DevBuf<uint32_t> counters;				 // [0] hits, [1] attached roots, [2] final hits, [3] overflow
 	DevBuf<uint32_t> sort_k[2], sort_v[2], sort_h;
 	DevBuf<uint32_t> slot_range;			 // [2 * slots] sorted hit range per slot
 	DevBuf<uint32_t> comp_info;				 // per hit (as root): [attached, count, key lo, key hi, cov]
 	DevBuf<uint32_t> roots;					 // attached roots list
-	DevBuf<uint64_t> hash;					 // (root, vertex, side) set
+	DevBuf<uint64_t> hash;					 // (generation, root, vertex, side) set
+	uint32_t		 hash_gen = 255;		 // generation of the next evaluation (255: clear first)
+	DevBuf<uint64_t> buckets;				 // hit bucket key -> first sorted index (generation-tagged)
+	uint32_t		 bucket_cap = 0, bucket_gen = 255;
 	DevBuf<uint64_t> slot_best;				 // [2 * slots]
 	DevBuf<PrpSummary> summary;
 	DevBuf<uint8_t>	 adjm;					 // contour adjacency [nc * nc]
@@ -191,6 +200,7 @@ struct PrpResolver
 private:
 	void build_contour_tables(const ContourBuilder& ct);
 	void distances(uint32_t c0, uint32_t n);
+	void begin_batch(uint32_t first);
 	void directions(uint32_t c0, uint32_t n);
 	void ensure_hits(uint32_t cap);
 };
