@@ -130,9 +130,10 @@ def summarize(reps):
 # Algorithmic bytes of the PRP EE sweep kernel (k_prp_ray_edge) per unit of work:
 # a request reads its slot id, candidate index and source edge (16 B); a node visit
 # reads one 64 B BvhNode (both child boxes and references); a leaf test reads the
-# target edge's ids, endpoint positions, hops and ownership (48 B); a hit writes a
-# 32 B PrpHit.
-EE_BYTES = {"requests": 16, "visits": 64, "leaf_tests": 48, "hits": 32}
+# target edge's vertex ids, hops and ownership for the candidate check (18 B); a
+# candidate's edge-edge test reads the source and target endpoint positions (48 B);
+# a hit writes a 32 B PrpHit.
+EE_BYTES = {"requests": 16, "visits": 64, "leaf_tests": 18, "cand_tests": 48, "hits": 32}
 
 
 def measured_traffic(args):

@@ -353,7 +353,7 @@ typedef struct ut_step_report {
 	uint32_t prp_ee_launches;
 	float	 ms_prp_ee;
 	uint64_t prp_ee_requests, prp_ee_visits, prp_ee_leaf_tests, prp_ee_hits;
-	uint64_t prp_ee_cand_tests; /* leaf tests whose target passed the candidate check */
+	uint64_t prp_ee_cand_tests; /* leaves whose target passed the candidate check (edge-edge tests run) */
 } ut_step_report;
 
 UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,

@@ -21,11 +21,15 @@ constexpr int kStackLds = 32;
 constexpr int kStackTotal = 128;
 constexpr int kStageCap = 192; // < 64 pending + up to 128 new per step
 
-struct TravShared
+// LDS part of the traversal stacks (LdsDepth entries per lane; deeper entries
+// spill to the global stack) and the per-wave candidate stage.
+template <int LdsDepth>
+struct TravSharedT
 {
-	uint32_t stack[kStackLds * kTravThreads];
+	uint32_t stack[LdsDepth * kTravThreads];
 	uint2	 stage[kTravWaves][kStageCap];
 };
+using TravShared = TravSharedT<kStackLds>;
 
 __device__ __forceinline__ box3 query_box(int kind,
 	uint32_t									q,
@@ -55,8 +59,8 @@ __device__ __forceinline__ box3 query_box(int kind,
 // is entered; spill_slot selects this lane's region of the global stack.
 // When stat is given, stat[0] += node visits and stat[1] += leaf tests of the
 // wave (one atomic each per wave).
-template <typename NodeTest, typename Narrow>
-__device__ __forceinline__ void traverse_staged_test(TravShared& sh,
+template <int LdsDepth, typename NodeTest, typename Narrow>
+__device__ __forceinline__ void traverse_staged_test(TravSharedT<LdsDepth>& sh,
 	bool							 active,
 	uint32_t						 q,
 	uint32_t						 spill_slot,
@@ -116,10 +120,10 @@ __device__ __forceinline__ void traverse_staged_test(TravShared& sh,
 			const bool t0 = h0 && !l0, t1 = h1 && !l1;
 			if (t0 && t1)
 			{
-				if (sp < kStackLds)
+				if (sp < LdsDepth)
 					sh.stack[sp * kTravThreads + threadIdx.x] = refs.y;
 				else if (sp < kStackTotal)
-					spill[(size_t)spill_slot * (kStackTotal - kStackLds) + (sp - kStackLds)] = refs.y;
+					spill[(size_t)spill_slot * (kStackTotal - LdsDepth) + (sp - LdsDepth)] = refs.y;
 				if (sp < kStackTotal)
 				{
 					++sp;
@@ -138,8 +142,8 @@ __device__ __forceinline__ void traverse_staged_test(TravShared& sh,
 			else if (sp > 0)
 			{
 				--sp;
-				node = sp < kStackLds ? sh.stack[sp * kTravThreads + threadIdx.x]
-									  : spill[(size_t)spill_slot * (kStackTotal - kStackLds) + (sp - kStackLds)];
+				node = sp < LdsDepth ? sh.stack[sp * kTravThreads + threadIdx.x]
+									 : spill[(size_t)spill_slot * (kStackTotal - LdsDepth) + (sp - LdsDepth)];
 			}
 			else
 				go = false;

@@ -28,7 +28,11 @@ namespace
 constexpr uint32_t kHopInf = 0xFFFFFFFFu;
 constexpr uint32_t kOwnUnknown = 0xFFu;
 constexpr int	   kBfsThreads = 1024;
-constexpr int	   kRayBlocks = 1024; // persistent ray grid (spill stack per lane)
+// Persistent ray grid: 6 workgroups per CU on 256 CUs (the ray kernels use ~80
+// VGPRs, so 6 waves per SIMD), each lane with its own spill stack.  A 16-entry
+// LDS stack keeps 6 workgroups within the 160 KB of LDS per CU.
+constexpr int	   kRayBlocks = 1536;
+constexpr int	   kRayStackLds = 16;
 static_assert(kRayBlocks % kNumXcd == 0, "the ray grid is split evenly over the XCDs");
 constexpr uint32_t kMaxDirs = 64;
 constexpr uint64_t kHashEmpty = ~0ull;
@@ -414,7 +418,7 @@ struct RayArgs
 	uint32_t*		spill;
 	const BvhNode*	nodes;
 	const float*	tbox; // [6 * nslots] candidate target edge boxes
-	unsigned long long* tstat; // [3 kinds][4]: node visits, leaf tests, hits, unused
+	unsigned long long* tstat; // [3 kinds][4]: node visits, leaf tests, hits, candidate leaf tests
 };
 
 __device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots, uint32_t r)
@@ -460,10 +464,13 @@ struct RayReq
 template <int Kind>
 __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 {
-	__shared__ TravShared sh;
-	const uint32_t		  total = a.off[Kind * (a.nslots + 1) + a.nslots];
-	const uint32_t*		  off = a.off + Kind * (a.nslots + 1);
-	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ TravSharedT<kRayStackLds> sh;
+	// request of each lane (slot, source vertex), read by the leaf tests of the lane's
+	// wave: a staged candidate of request r belongs to lane r - base
+	__shared__ uint32_t req_slot[kTravThreads], req_src[kTravThreads];
+	const uint32_t		total = a.off[Kind * (a.nslots + 1) + a.nslots];
+	const uint32_t*		off = a.off + Kind * (a.nslots + 1);
+	const uint32_t		spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
 	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
 		const uint32_t	   r = base + threadIdx.x;
@@ -480,7 +487,13 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 			q.dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
 			q.o = ld3(a.x, q.src);
 			q.tmax = 1.0f;
+			req_slot[threadIdx.x] = q.slot;
+			req_src[threadIdx.x] = q.src;
 		}
+		// the wave's own lanes wrote the entries its leaf tests read
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 		const f3 cast = Kind == 0 ? q.dir : -q.dir;
 		const f3 inv = mk3(1.0f / cast.x, 1.0f / cast.y, 1.0f / cast.z);
 		auto	 test = [&](const box3& b)
@@ -503,11 +516,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 			PrpHit h{};
 			if (valid)
 			{
-				const uint32_t slot = slot_of(off, a.nslots, rr);
+				const uint32_t slot = req_slot[rr - base];
 				const PrpSlot& sl = a.slots[slot];
 				const uint32_t s_dst = 2 * sl.contour + (Kind == 0 ? 1u : 0u);
-				const uint32_t s_src = s_dst ^ 1u;
-				const uint32_t v = a.cand[(size_t)s_src * (a.V + a.E) + (rr - off[slot])];
+				const uint32_t v = req_src[rr - base];
 				const uint32_t t0 = a.faces[3 * f], t1 = a.faces[3 * f + 1], t2 = a.faces[3 * f + 2];
 				const uint32_t* H = a.hop_all + (size_t)s_dst * a.V;
 				const uint8_t*	O = a.own_all + (size_t)s_dst * a.V;
@@ -545,7 +557,7 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 // (plane test, |det| >= 1e-10, Cramer solve with s in [0, ray_max], t1, t2 in [0, 1], shared vertices skipped).
 __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 {
-	__shared__ TravShared sh;
+	__shared__ TravSharedT<kRayStackLds> sh;
 	const uint32_t		  total = a.off[1 * (a.nslots + 1) + a.nslots];
 	const uint32_t*		  off = a.off + (a.nslots + 1);
 	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1720,7 +1732,7 @@ void PrpResolver::init(Context* c, uint32_t nv_, uint32_t ne_, uint32_t nf_, con
 	normals.reserve(3 * ((size_t)nf + ne + nv));
 	counters.reserve(8);
 	status.reserve(2);
-	spill.reserve((size_t)kRayBlocks * kTravThreads * (kStackTotal - kStackLds));
+	spill.reserve((size_t)kRayBlocks * kTravThreads * (kStackTotal - kRayStackLds));
 	tstat.reserve(12);
 	if (!ee_ev[0])
 		for (auto& e : ee_ev)
@@ -2046,9 +2058,9 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 			ee_prof.ms += ms;
 			ee_prof.requests += tot[1];
 			ee_prof.visits += ts[4];
-			ee_prof.leaf_tests += ts[5];
+			ee_prof.leaf_tests += ts[5]; // leaves reached by the sweep (each staged and tested)
 			ee_prof.hits += ts[6];
-			ee_prof.cand_tests += ts[7];
+			ee_prof.cand_tests += ts[7]; // of which candidates of role 1: the edge-edge test runs
 		}
 		UT_REQUIRE(st[0] == 1u, "prp: ray traversal revisited a node (corrupt tree)");
 		UT_REQUIRE(st[1] == 0u, "prp: ray traversal stack overflow");

@@ -83,8 +83,9 @@ struct PrpKernelProfile
 {
 	uint32_t launches = 0;
 	float	 ms = 0;
-	uint64_t requests = 0, visits = 0, leaf_tests = 0, hits = 0;
-	uint64_t cand_tests = 0; // leaf tests whose target is a candidate (the rest are culled by hop/ownership)
+	uint64_t requests = 0, visits = 0, hits = 0;
+	uint64_t leaf_tests = 0; // leaves the traversal reached (each checked against the candidate set)
+	uint64_t cand_tests = 0; // of which candidates: the edge-edge test runs on these
 };
 
 struct PrpStats
