@@ -203,6 +203,12 @@ UT_API int ut_pcg_solve_refined(ut_pcg* pcg, const ut_pcg_params* params, const 
 	const double* b, double* x, ut_pcg_report* report);
 /* y = A x (FP64 vectors and accumulation; vals FP32 when vals_fp32 != 0). */
 UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const double* x, double* y);
+/* Contact pairs (CollisionPairTemplate list, device count) whose off-diagonal
+   blocks w_a w_b k2 n n^T are applied matrix-free by ut_pcg_solve* and
+   ut_bsr_spmv; their diagonal blocks belong in the block values (ut_assemble
+   puts them there).  pairs = NULL detaches.  Replaces the contact triplets the
+   reference's SpMV callback sums (newton_solver.cpp:1517-1555). */
+UT_API int ut_pcg_set_contacts(ut_pcg* pcg, const void* pairs, const uint32_t* count, uint32_t cap);
 
 /* ---- proximity DCD and CCD: NarrowPhasesDetector (narrow_phase.h:25-346) ---- */
 
@@ -240,6 +246,16 @@ UT_API int ut_contact_query(ut_ctx* ctx, int kind, const ut_contact_args* args, 
 	const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap);
 /* Sets *toi = 1 (line_search_max_t) on the context stream (fn_reset_toi). */
 UT_API int ut_toi_reset(ut_ctx* ctx, float* toi);
+/* Contact part of the incremental potential at x: for each record the FP64
+   closest-point distance d is recomputed from x, then kappa * area * b(d - th,
+   d_hat) (UT_ENERGY_BARRIER) or kappa * area * (d - th - d_hat)^2 / 2
+   (UT_ENERGY_QUADRATIC), summed over the records with k2 != 0 (disabled pairs
+   carry no energy).  The energies whose derivatives narrow_phase.cpp:1091-1112
+   assembles; no counterpart entry point in the reference (its line search sums
+   them inside the solver).  out: device double[257], out[0] receives the sum,
+   out[1..256] is scratch. */
+UT_API int ut_contact_energy(ut_ctx* ctx, const void* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const float* d_hat, const float* offset, double kappa, int energy, double* out);
 
 /* ---- assembly of the incremental potential ------------------------------------------
  * NewtonSolver's material assembly (newton_solver.cpp:2641-2714 with

@@ -631,3 +631,37 @@ def ccd_pair_mask(kind, broad, edges, faces, regions, invalid):
         if invalid[a + b].any() or _regions_match(regions, a, b):
             keep[i] = False
     return keep
+
+
+# ---------------------------------------------------------------- contact energy
+def contact_energy(records, x, d_hat, offset, kappa=1e8, energy=1):
+    """Sum over contact records of kappa * area * b(d - th, d_hat) (energy 1, barrier) or
+    kappa * area * (d - th - d_hat)^2 / 2 (energy 0), with d the FP64 closest-point
+    distance recomputed from x (float64 [V, 3]) and th, d_hat from the two vertices the
+    DCD used (VF: vertex and first face vertex; EE: first vertex of each edge); records
+    with k2 = 0 carry no energy (narrow_phase.cpp:1091-1112, cipc_kernel.hpp:15-36)."""
+    R = np.asarray(records, np.uint32).reshape(-1, 16)
+    X = np.asarray(x, np.float64).reshape(-1, 3)
+    total = 0.0
+    for rec in R:
+        i = [int(v & INDEX_MASK) for v in rec[:4]]
+        area, k2 = rec[7:8].view(np.float32)[0], rec[9:10].view(np.float32)[0]
+        if k2 == 0.0:
+            continue
+        if rec[2] & TYPE_BIT:
+            b = pt_coeff(X[i[0]], X[i[1]], X[i[2]], X[i[3]])
+            diff = b[0] * (X[i[0]] - X[i[1]]) + b[1] * (X[i[0]] - X[i[2]]) + b[2] * (X[i[0]] - X[i[3]])
+            ka, kb = i[0], i[1]
+        else:
+            c = ee_coeff(X[i[0]], X[i[1]], X[i[2]], X[i[3]])
+            diff = (c[0] * X[i[0]] + c[1] * X[i[1]]) - (c[2] * X[i[2]] + c[3] * X[i[3]])
+            ka, kb = i[0], i[2]
+        d = float(np.sqrt(diff @ diff))
+        dh = (float(d_hat[ka]) + float(d_hat[kb])) * 0.5
+        th = float(offset[ka]) + float(offset[kb])
+        if energy == 0:
+            C_ = (d - th) - dh
+            total += 0.5 * float(area) * kappa * C_ * C_
+        else:
+            total += float(area) * kappa * barrier(d - th, dh)
+    return total

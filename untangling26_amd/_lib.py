@@ -48,6 +48,8 @@ SIGNATURES = {
     "ut_pcg_destroy": (C.c_int, [vp]),
     "ut_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, vp]),
     "ut_pcg_solve_refined": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
+    "ut_pcg_set_contacts": (C.c_int, [vp, vp, vp, C.c_uint32]),
+    "ut_contact_energy": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp]),
     "ut_solver_profile": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
     "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 10 + [C.POINTER(vp)]),
     "ut_prp_destroy": (C.c_int, [vp]),

@@ -569,7 +569,31 @@ UT_API int ut_bsr_spmv(ut_pcg* pcg, const void* vals, int vals_fp32, const doubl
 	return guarded(pcg->s.ctx, [&] { pcg->s.spmv64(vals, vals_fp32 != 0, x, y); });
 }
 
+UT_API int ut_pcg_set_contacts(ut_pcg* pcg, const void* pairs, const uint32_t* count, uint32_t cap)
+{
+	if (!pcg || (pairs && !count))
+		return ut::kInvalid;
+	pcg->s.contacts = reinterpret_cast<const ut::ContactRec*>(pairs);
+	pcg->s.contact_count = pairs ? count : nullptr;
+	pcg->s.contact_cap = pairs ? cap : 0u;
+	return ut::kOk;
+}
+
 // ---- contacts ----------------------------------------------------------------
+UT_API int ut_contact_energy(ut_ctx* ctx, const void* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const float* d_hat, const float* offset, double kappa, int energy, double* out)
+{
+	if (!ctx || !pairs || !count || !x || !d_hat || !offset || !out)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			UT_REQUIRE(energy == ut::kEnergyQuadratic || energy == ut::kEnergyBarrier, "contact energy: unknown energy");
+			ut::contact_energy(&ctx->c, reinterpret_cast<const ut::ContactRec*>(pairs), count, cap, x, d_hat, offset,
+				kappa, energy, out, out + 1);
+		});
+}
+
 UT_API int ut_contact_query(ut_ctx* ctx, int kind, const ut_contact_args* args, ut_lbvh* tree,
 	const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap)
 {

@@ -281,4 +281,96 @@ void toi_reset(Context* ctx, float* toi)
 	UT_LAUNCH_CHECK();
 }
 
+namespace
+{
+constexpr int kEnergyThreads = 256;
+
+// Contact part of the incremental potential at x: for every proximity record
+// the closest-point distance is recomputed in FP64 from x (pt / ee coefficients,
+// as the DCD that produced the record), then kappa * area * b(d - th, d_hat)
+// for the barrier or kappa * area * C^2 / 2, C = d - th - d_hat, for the
+// quadratic energy (narrow_phase.cpp:1091-1112 integrates to these).
+__global__ __launch_bounds__(kEnergyThreads) void k_contact_energy(const ContactRec* __restrict__ pairs,
+	const uint32_t* __restrict__ count, uint32_t cap, const float* __restrict__ x, const float* __restrict__ d_hat,
+	const float* __restrict__ offset, double kappa, int energy, double* __restrict__ part)
+{
+	const uint32_t n = min(*count, cap);
+	double		   acc = 0.0;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const ContactRec r = pairs[i];
+		const uint32_t	 i0 = r.idx[0] & 0x7FFFFFFFu, i1 = r.idx[1] & 0x7FFFFFFFu, i2 = r.idx[2] & 0x7FFFFFFFu,
+					   i3 = r.idx[3] & 0x7FFFFFFFu;
+		const bool		 vf = (r.idx[2] & kTypeMask) != 0u;
+		v3<double>		 diff;
+		uint32_t		 ka, kb; // the two vertices whose d_hat / offset the DCD used
+		if (vf)
+		{
+			const v3<double> p = load_v3<double>(x, i0);
+			const v3<double> t0 = load_v3<double>(x, i1), t1 = load_v3<double>(x, i2), t2 = load_v3<double>(x, i3);
+			const v3<double> b = pt_coeff(p, t0, t1, t2);
+			diff = b.x * (p - t0) + b.y * (p - t1) + b.z * (p - t2);
+			ka = i0, kb = i1;
+		}
+		else
+		{
+			const v3<double> a0 = load_v3<double>(x, i0), a1 = load_v3<double>(x, i1);
+			const v3<double> b0 = load_v3<double>(x, i2), b1 = load_v3<double>(x, i3);
+			const c4<double> c = ee_coeff(a0, a1, b0, b1);
+			diff = (c.a * a0 + c.b * a1) - (c.c * b0 + c.d * b1);
+			ka = i0, kb = i2;
+		}
+		const double d = sqrt(dot(diff, diff));
+		const double dh = ((double)d_hat[ka] + (double)d_hat[kb]) * 0.5;
+		const double th = (double)offset[ka] + (double)offset[kb];
+		const double area = (double)r.area;
+		double		 e;
+		if (energy == kEnergyQuadratic)
+		{
+			const double C = (d - th) - dh;
+			e = 0.5 * area * kappa * C * C;
+		}
+		else
+			e = area * kappa * barrier(d - th, dh);
+		if (r.k2 != 0.0f) // disabled pairs (k1 = k2 = 0) carry no energy
+			acc += e;
+	}
+	__shared__ double s[kEnergyThreads / kWave];
+	for (int o = 32; o > 0; o >>= 1)
+		acc += __shfl_xor(acc, o, kWave);
+	if (__lane_id() == 0)
+		s[threadIdx.x / kWave] = acc;
+	__syncthreads();
+	if (threadIdx.x == 0)
+	{
+		double t = 0.0;
+		for (int k = 0; k < kEnergyThreads / kWave; ++k)
+			t += s[k];
+		part[blockIdx.x] = t;
+	}
+}
+
+__global__ void k_sum_parts(uint32_t n, const double* __restrict__ part, double* __restrict__ out)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		double t = 0.0;
+		for (uint32_t k = 0; k < n; ++k)
+			t += part[k];
+		*out = t;
+	}
+}
+} // namespace
+
+void contact_energy(Context* ctx, const ContactRec* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const float* d_hat, const float* offset, double kappa, int energy, double* out, double* scratch)
+{
+	UT_REQUIRE(pairs && count && x && d_hat && offset && out && scratch, "contact energy: null input");
+	constexpr uint32_t kBlocks = 256;
+	k_contact_energy<<<kBlocks, kEnergyThreads, 0, ctx->stream>>>(pairs, count, cap, x, d_hat, offset, kappa, energy,
+		scratch);
+	k_sum_parts<<<1, 64, 0, ctx->stream>>>(kBlocks, scratch, out);
+	UT_LAUNCH_CHECK();
+}
+
 } // namespace ut

@@ -56,5 +56,9 @@ struct ContactArgs
 void contact_query(Context* ctx, int kind, const ContactArgs& a, Lbvh* tree, const uint32_t* broad,
 	const uint32_t* broad_count, uint32_t broad_cap);
 void toi_reset(Context* ctx, float* toi);
+// Sum of the contact energies of the records at x (FP64 distances recomputed from
+// x); *out is a device double, scratch holds 256 doubles.
+void contact_energy(Context* ctx, const ContactRec* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const float* d_hat, const float* offset, double kappa, int energy, double* out, double* scratch);
 
 } // namespace ut
