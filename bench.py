@@ -65,6 +65,7 @@ def make_solver(args, precision):
     cfg.untangling_response_depth = 1.0
     cfg.nonlinear_iter_count = 1
     cfg.use_ccd_linesearch = True
+    cfg.ignore_near_zero_dist_pairs = False
     cfg.pcg_iter_count = args.pcg_iters
     cfg.pcg_rel_tol = 1e-6
     cfg.pcg_precision = precision

@@ -121,6 +121,19 @@ UT_API int ut_ef_detect(ut_lbvh* face_tree, const float* x, const uint32_t* edge
 	const uint32_t* faces, const uint32_t* vprop, const float* d_hat, const float* thickness, uint32_t* count,
 	void* pairs, uint32_t* pair_ids, uint32_t max_pairs);
 
+/* fn_add_ef_from_proximity_pairs (intersection_resolver.cpp:199-363), run by
+ * ef_dcd_intersection_query (:404-421) when ignore_near_zero_dist_pairs is
+ * false: every VF / EE proximity pair (recs: CollisionPairTemplate records,
+ * rec_ids: (vertex, face) or (edge, edge), *rec_count of them, at most
+ * rec_cap read) whose weighted point difference has |d|^2 < 1e-10 appends EF
+ * pairs to (pairs, pair_ids), incrementing *count as ut_ef_detect does.
+ * vert_adj_off uint32[V+1] / vert_adj_edges: the edges around each vertex
+ * (sa_vert_adj_edges_csr); edge_adj_faces as for ut_contours_create. */
+UT_API int ut_ef_from_proximity(ut_ctx* ctx, const void* recs, const uint32_t* rec_ids, const uint32_t* rec_count,
+	uint32_t rec_cap, const float* x, const uint32_t* edges, const uint32_t* faces, const uint32_t* vert_adj_off,
+	const uint32_t* vert_adj_edges, const uint32_t* edge_adj_faces, uint32_t* count, void* pairs,
+	uint32_t* pair_ids, uint32_t max_pairs);
+
 /* ---- contours: IntersectionResolver::host_preprocess_ef_pairs
  *      (intersection_resolver.cpp:683-1096) + intersection_contour_construction_from_ext_adjacent
  *      (intersection_resolver2.cpp:254-309, contour_flood_filling_template
@@ -131,10 +144,11 @@ enum {
 	UT_CT_PAIRS = 1,		 /* EfPair[total] (12 words each) */
 	UT_CT_CONTOUR_INDEX = 2, /* uint32[total] ef_pair_contour_index */
 	UT_CT_MESH_INDEX = 3,	 /* uint32[total] ef_pair_mesh_index (parity) */
-	UT_CT_ADJ = 4,			 /* uint32[12*total] ef_pair_adj_pairs_ext, 12 slots per pair */
-	UT_CT_ADJ_N = 5,		 /* uint32[total] used slots */
+	UT_CT_ADJ = 4,			 /* uint32[num_adj] ef_pair_adj_pairs_ext, neighbours packed by pair */
+	UT_CT_ADJ_N = 5,		 /* uint32[total] neighbour count of each pair */
 	UT_CT_INCONSISTENT = 6,	 /* uint32[total] 1 at the smallest pair of an odd-cycle component */
-	UT_CT_COMP_MIN = 7		 /* uint32[total] smallest pair index of the pair's contour */
+	UT_CT_COMP_MIN = 7,		 /* uint32[total] smallest pair index of the pair's contour */
+	UT_CT_ADJ_OFF = 8		 /* uint32[total+1] start of each pair's neighbours in UT_CT_ADJ */
 };
 /* Mesh topology (device arrays kept by reference): face_adj_edges uint32[3F]
  * (edge between face[j] and face[j+1], init_mesh_data.cpp:419-483) and
@@ -144,11 +158,17 @@ UT_API int ut_contours_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf
 	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, ut_contours** out);
 UT_API int ut_contours_destroy(ut_contours* ct);
 /* num_pairs EF pairs (EfPair records + (eid, fid)) in any order.  Sorts them
- * canonically, adds loop-vertex pairs, builds the EF adjacency and the
- * contours.  Synchronises; *num_total = EF pairs including loop pairs (what
- * narrow_phase_collision_count[1] holds after preprocessing), *num_contours. */
+ * canonically ((eid, fid), then payload bytes), drops degenerate pairs with no
+ * non-degenerate neighbour when ignore_near_zero_dist_pairs is 0
+ * (intersection_resolver.cpp:841-897), adds loop-vertex pairs, builds the EF
+ * adjacency and the contours.  Synchronises; *num_total = EF pairs including
+ * loop pairs (what narrow_phase_collision_count[1] holds after
+ * preprocessing), *num_contours. */
 UT_API int ut_contours_build(ut_contours* ct, uint32_t num_pairs, const void* pairs, const uint32_t* pair_ids,
-	uint32_t* num_total, uint32_t* num_contours);
+	int ignore_near_zero_dist_pairs, uint32_t* num_total, uint32_t* num_contours);
+/* Counts of the last build: out4 = {pairs given, pairs after the degenerate
+ * filter, total with loop pairs, adjacency entries}. */
+UT_API int ut_contours_counts(ut_contours* ct, uint32_t* out4);
 /* Device-to-device copy of one UT_CT_* output of the last build. */
 UT_API int ut_contours_get(ut_contours* ct, int what, void* dst);
 /* Components with parity of an undirected CSR graph with 0/1 flip weights
@@ -350,6 +370,8 @@ typedef struct ut_solver_params {
 	ut_prp_params prp;
 	int			  region_filter; /* filter_invalid_pairs on proximity and CCD pairs; must be 0: the filter
 									is switched off until its kernels are verified on the device */
+	int			  ignore_near_zero_dist_pairs; /* SceneParams default 1; 0 (cli.py:107) adds EF pairs
+											  from zero-distance proximity pairs and the degenerate filter */
 } ut_solver_params;
 
 typedef struct ut_step_report {

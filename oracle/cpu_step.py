@@ -38,25 +38,31 @@ def cpu_step_baseline(mesh, rest, solver, args, prp_budget_s: float = 8.0, pcg_i
     edges, faces = st.edges, st.faces
     d_hat, off = scenes.per_vertex(mesh, d_hat=1e-3, thickness=0.0)
     times = {}
+    vae = oracle.vert_adj_edges(mesh.nv, edges)
 
     def detect():
         qb = oracle.query_boxes(0, x, x, None, d_hat, off)
         lb = oracle.leaf_boxes(1, x, x, faces, off)
         vf = oracle.overlap_pairs(qb, lb)
-        rv, _ = oracle.dcd_list(0, vf, x, edges, faces, d_hat, off, st.rest_vert_area, st.rest_edge_area,
-                                st.rest_face_area, 1e8, 0)
+        rv, iv = oracle.dcd_list(0, vf, x, edges, faces, d_hat, off, st.rest_vert_area, st.rest_edge_area,
+                                 st.rest_face_area, 1e8, 0)
         qe = oracle.query_boxes(1, x, x, edges, d_hat, off)
         le = oracle.leaf_boxes(2, x, x, edges, off)
         ee = oracle.overlap_pairs(qe, le, cull=True)
-        re, _ = oracle.dcd_list(1, ee, x, edges, faces, d_hat, off, st.rest_vert_area, st.rest_edge_area,
-                                st.rest_face_area, 1e8, 0)
+        re, ie = oracle.dcd_list(1, ee, x, edges, faces, d_hat, off, st.rest_vert_area, st.rest_edge_area,
+                                 st.rest_face_area, 1e8, 0)
         pairs, ids = oracle.ef_detect(x, edges, faces, d_hat, off)
-        return np.concatenate([rv, re]) if len(rv) + len(re) else np.zeros((0, 16), np.uint32), pairs, ids
+        recs = np.concatenate([rv, re]).reshape(-1, 16)
+        # zero-distance proximity pairs -> EF pairs (ignore_near_zero_dist_pairs = False)
+        zp, zi = oracle.ef_from_proximity(recs, np.concatenate([iv, ie]).reshape(-1, 2), x, edges, faces, *vae,
+                                          st.edge_adj_faces)
+        return recs, np.concatenate([pairs, zp]), np.concatenate([ids, zi])
 
     times["detect"], (recs, pairs, ids) = _t(detect)
 
     def contours():
-        oid, opr, adj, adjn = oracle.ef_preprocess(ids, pairs, edges, faces, st.face_adj_edges, st.edge_adj_faces)
+        oid, opr, adj, adjn = oracle.ef_preprocess(ids, pairs, edges, faces, st.face_adj_edges, st.edge_adj_faces,
+                                                   filter_degenerate=True)
         nc, ci, mi = oracle.contours_dfs(opr, adj, adjn)
         return opr, adj, adjn, ci, mi, nc
 

@@ -77,11 +77,21 @@ size_t or_lbvh_query(uint32_t n, const uint32_t* children, const uint32_t* objec
 /* init_mesh_data.cpp:419-500: face_adj_edges uint32[3F], edge_adj_faces uint32[2E]. */
 int or_mesh_topology(uint32_t ne, uint32_t nf, const uint32_t* edges, const uint32_t* faces, uint32_t* face_adj_edges,
 	uint32_t* edge_adj_faces);
-/* host_preprocess_ef_pairs (intersection_resolver.cpp:683-1096) for DCD pairs. */
+/* host_preprocess_ef_pairs (intersection_resolver.cpp:683-1096): canonical
+ * order, optional degenerate filter, loop-vertex pairs, packed adjacency
+ * (*adj malloc'ed, release with or_free).  Returns the total pair count. */
 long or_ef_preprocess(uint32_t P, const uint32_t* in_ids, const uint32_t* in_pairs, const uint32_t* edges,
-	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, uint32_t* out_ids,
-	uint32_t* out_pairs, uint32_t* adj, uint32_t* adj_n);
-/* contour_flood_filling_template (intersection_resolver_helper.cpp:518-575). */
+	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, int filter_degenerate,
+	uint32_t* out_ids, uint32_t* out_pairs, uint32_t** adj, uint32_t* adj_n, uint32_t* kept);
+void or_free(void* p);
+/* is_degenerate_ef_pair (intersection_resolver.cpp:561-569) of one EfPair (12 words). */
+int or_ef_degenerate(const uint32_t* pair);
+/* fn_add_ef_from_proximity_pairs (intersection_resolver.cpp:199-363): EF pairs
+ * from zero-distance VF / EE proximity pairs.  Returns the number generated. */
+size_t or_ef_from_proximity(size_t n, const uint32_t* recs, const uint32_t* ids, const float* x, const uint32_t* edges,
+	const uint32_t* faces, const uint32_t* vae_off, const uint32_t* vae, const uint32_t* edge_adj_faces,
+	uint32_t* out_pairs, uint32_t* out_ids, size_t cap);
+/* contour_flood_filling_template (intersection_resolver_helper.cpp:518-575), packed adjacency. */
 uint32_t or_contours_dfs(uint32_t N, const uint32_t* pairs, const uint32_t* adj, const uint32_t* adj_n,
 	uint32_t* contour_index, uint32_t* mesh_index);
 /* cpu_reference of UnitTest/test_parallel_contour_construction.cpp:155-198. */

@@ -44,8 +44,13 @@ def lib():
         L.or_lbvh_query.restype = C.c_size_t
         L.or_mesh_topology.argtypes = [C.c_uint32, C.c_uint32, p, p, p, p]
         L.or_mesh_topology.restype = C.c_int
-        L.or_ef_preprocess.argtypes = [C.c_uint32, p, p, p, p, p, p, p, p, p, p]
+        L.or_ef_preprocess.argtypes = [C.c_uint32, p, p, p, p, p, p, C.c_int, p, p, C.POINTER(p), p, p]
         L.or_ef_preprocess.restype = C.c_long
+        L.or_free.argtypes = [p]
+        L.or_ef_degenerate.argtypes = [p]
+        L.or_ef_degenerate.restype = C.c_int
+        L.or_ef_from_proximity.argtypes = [C.c_size_t, p, p, p, p, p, p, p, p, p, p, C.c_size_t]
+        L.or_ef_from_proximity.restype = C.c_size_t
         L.or_contours_dfs.argtypes = [C.c_uint32, p, p, p, p, p]
         L.or_contours_dfs.restype = C.c_uint32
         L.or_graph_bfs.argtypes = [C.c_uint32, p, p, p, p, p, p]
@@ -242,21 +247,65 @@ def mesh_topology(edges, faces):
     return fae, eaf
 
 
-def ef_preprocess(ids, pairs, edges, faces, fae, eaf):
-    """Canonical EF list with loop pairs appended, and the 12-slot adjacency."""
+def ef_preprocess(ids, pairs, edges, faces, fae, eaf, filter_degenerate=False, with_kept=False):
+    """Canonical EF list with loop pairs appended, and the packed adjacency:
+    neighbours of pair i are adj[sum(adjn[:i]) : sum(adjn[:i + 1])].
+    filter_degenerate is the reference's not ignore_near_zero_dist_pairs."""
     ids = _c(ids, np.uint32).reshape(-1, 2)
     pairs = _c(pairs, np.uint32).reshape(-1, 12)
     P = ids.shape[0]
     cap = 3 * max(P, 1)
     oid = np.zeros((cap, 2), np.uint32)
     opr = np.zeros((cap, 12), np.uint32)
-    adj = np.zeros((cap, 12), np.uint32)
     adjn = np.zeros(cap, np.uint32)
+    kept = C.c_uint32()
+    ap = C.c_void_p()
     n = lib().or_ef_preprocess(P, _p(ids), _p(pairs), _p(_c(edges, np.uint32)), _p(_c(faces, np.uint32)),
-                               _p(_c(fae, np.uint32)), _p(_c(eaf, np.uint32)), _p(oid), _p(opr), _p(adj), _p(adjn))
-    if n < 0:
-        raise RuntimeError("EF adjacency exceeded 12 entries")
-    return oid[:n], opr[:n], adj[:n], adjn[:n]
+                               _p(_c(fae, np.uint32)), _p(_c(eaf, np.uint32)), int(bool(filter_degenerate)), _p(oid),
+                               _p(opr), C.byref(ap), _p(adjn), C.byref(kept))
+    m = int(adjn[:n].sum(dtype=np.int64))
+    adj = np.ctypeslib.as_array(C.cast(ap, C.POINTER(C.c_uint32)), shape=(max(m, 1),))[:m].copy()
+    lib().or_free(ap)
+    out = (oid[:n], opr[:n], adj, adjn[:n])
+    return out + (int(kept.value),) if with_kept else out
+
+
+def adj_rows(adj, adjn):
+    """Packed adjacency as a list of rows."""
+    off = np.concatenate([[0], np.cumsum(np.asarray(adjn, np.int64))])
+    return [adj[off[i]:off[i + 1]] for i in range(len(adjn))]
+
+
+def ef_degenerate(pair):
+    return bool(lib().or_ef_degenerate(_p(_c(pair, np.uint32))))
+
+
+def vert_adj_edges(nv, edges):
+    """Vertex -> edge CSR (offsets [V+1], edges ascending per vertex)."""
+    e = np.asarray(edges, np.int64).reshape(-1, 2)
+    v = e.ravel()
+    eid = np.repeat(np.arange(len(e)), 2)
+    order = np.lexsort((eid, v))
+    off = np.zeros(nv + 1, np.uint32)
+    off[1:] = np.cumsum(np.bincount(v, minlength=nv))
+    return off, eid[order].astype(np.uint32)
+
+
+def ef_from_proximity(recs, ids, x, edges, faces, vae_off, vae, eaf):
+    """EF pairs from zero-distance proximity pairs: (pairs[n, 12], ids[n, 2]) in generation order."""
+    recs = _c(recs, np.uint32).reshape(-1, 16)
+    ids = _c(ids, np.uint32).reshape(-1, 2)
+    n = recs.shape[0]
+    cap = 64
+    while True:
+        pr = np.zeros((cap, 12), np.uint32)
+        pid = np.zeros((cap, 2), np.uint32)
+        k = lib().or_ef_from_proximity(n, _p(recs), _p(ids), _p(_c(x, np.float32)), _p(_c(edges, np.uint32)),
+                                       _p(_c(faces, np.uint32)), _p(_c(vae_off, np.uint32)), _p(_c(vae, np.uint32)),
+                                       _p(_c(eaf, np.uint32)), _p(pr), _p(pid), cap)
+        if k <= cap:
+            return pr[:k], pid[:k]
+        cap = int(k)
 
 
 def contours_dfs(pairs, adj, adjn):

@@ -79,8 +79,9 @@ struct or_prp_mesh
 	/* contour state */
 	uint32_t		N, nc;
 	const uint32_t* pairs; /* 12 words per pair */
-	const uint32_t* adj;
+	const uint32_t* adj;  /* packed: neighbours of p at adj[adj_off[p] ..] */
 	const uint32_t* adjn;
+	size_t*			adj_off;
 	const uint32_t* cidx;
 	const uint32_t* midx;
 	uint32_t *		c_off, *c_pairs;	   /* pairs of contour c, ascending */
@@ -167,8 +168,9 @@ static void free_contours(or_prp_mesh* m)
 {
 	free(m->c_off), free(m->c_pairs), free(m->b_off), free(m->b_verts), free(m->flag_off), free(m->flags);
 	free(m->hop), free(m->geo), free(m->diameter), free(m->dirs), free(m->ndirs);
-	free(m->fn), free(m->en), free(m->vn);
+	free(m->fn), free(m->en), free(m->vn), free(m->adj_off);
 	m->fn = m->en = m->vn = NULL;
+	m->adj_off = NULL;
 	m->c_off = m->c_pairs = m->b_off = m->b_verts = m->flag_off = m->flags = m->hop = m->diameter = m->ndirs = NULL;
 	m->geo = m->dirs = NULL;
 	m->nc = 0;
@@ -535,6 +537,10 @@ void or_prp_prepare_n(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const u
 {
 	free_contours(m);
 	m->N = N, m->nc = nc, m->pairs = pairs, m->adj = adj, m->adjn = adjn, m->cidx = contour_index, m->midx = mesh_index;
+	m->adj_off = malloc(sizeof(size_t) * (N + 1));
+	m->adj_off[0] = 0;
+	for (uint32_t p = 0; p < N; ++p)
+		m->adj_off[p + 1] = m->adj_off[p] + adjn[p];
 	const uint32_t V = m->nv;
 	m->c_off = calloc(nc + 1, 4);
 	m->c_pairs = malloc(4 * (N + 1));
@@ -640,7 +646,7 @@ void or_prp_prepare_n(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const u
 			pts[3 * i] = pp.x, pts[3 * i + 1] = pp.y, pts[3 * i + 2] = pp.z;
 			float s = 0.0f;
 			for (uint32_t j = 0; j < adjn[p]; ++j)
-				s += len(sub(pair_pos(m, adj[12 * p + j]), pp));
+				s += len(sub(pair_pos(m, adj[m->adj_off[p] + j]), pp));
 			wts[i] = 0.5f * s;
 		}
 		m->ndirs[c] = or_prp_directions(n, pts, wts, m->b_off[2 * c + 1] - m->b_off[2 * c], m->b_verts + m->b_off[2 * c],

@@ -225,7 +225,7 @@ def test_ef_preprocess_and_dfs_consistent(name):
     assert np.array_equal(oid[:P], ids)  # canonical order = sorted (eid, fid)
     # adjacency is symmetric
     N = len(oid)
-    nb = [set(adj[i, :adjn[i]].tolist()) for i in range(N)]
+    nb = [set(r.tolist()) for r in oracle.adj_rows(adj, adjn)]
     for i in range(N):
         for j in nb[i]:
             assert i in nb[j]

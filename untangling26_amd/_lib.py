@@ -41,7 +41,10 @@ SIGNATURES = {
     "ut_ef_detect": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, C.c_uint32]),
     "ut_contours_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, vp, C.POINTER(vp)]),
     "ut_contours_destroy": (C.c_int, [vp]),
-    "ut_contours_build": (C.c_int, [vp, C.c_uint32, vp, vp, u32p, u32p]),
+    "ut_ef_from_proximity": (C.c_int, [vp, vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                       C.c_uint32]),
+    "ut_contours_build": (C.c_int, [vp, C.c_uint32, vp, vp, C.c_int, u32p, u32p]),
+    "ut_contours_counts": (C.c_int, [vp, u32p]),
     "ut_contours_get": (C.c_int, [vp, C.c_int, vp]),
     "ut_graph_components": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp]),
     "ut_pcg_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
@@ -120,7 +123,8 @@ class SolverParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3), ("kappa", C.c_double), ("energy", C.c_int),
                 ("damping", C.c_double), ("nonlinear_iters", C.c_int), ("use_self_collision", C.c_int),
                 ("use_ccd", C.c_int), ("use_untangling", C.c_int), ("pcg", PcgParams), ("use_prp", C.c_int),
-                ("quasi_static", C.c_int), ("prp", PrpParams), ("region_filter", C.c_int)]
+                ("quasi_static", C.c_int), ("prp", PrpParams), ("region_filter", C.c_int),
+                ("ignore_near_zero_dist_pairs", C.c_int)]
 
 
 class StepReport(C.Structure):

@@ -15,8 +15,8 @@ import torch
 from ._lib import lib, ptr
 from .runtime import Context
 
-CT_PAIR_IDS, CT_PAIRS, CT_CONTOUR_INDEX, CT_MESH_INDEX, CT_ADJ, CT_ADJ_N, CT_INCONSISTENT, CT_COMP_MIN = range(8)
-MAX_ADJ = 12
+(CT_PAIR_IDS, CT_PAIRS, CT_CONTOUR_INDEX, CT_MESH_INDEX, CT_ADJ, CT_ADJ_N, CT_INCONSISTENT, CT_COMP_MIN,
+ CT_ADJ_OFF) = range(9)
 
 
 class ContourBuilder:
@@ -38,18 +38,27 @@ class ContourBuilder:
         except Exception:
             pass
 
-    def build(self, num_pairs: int, pairs, pair_ids):
+    def build(self, num_pairs: int, pairs, pair_ids, ignore_near_zero_dist_pairs: bool = True):
+        """ignore_near_zero_dist_pairs=False (the CLI's setting) drops degenerate
+        pairs that have no non-degenerate neighbour."""
         nt, nc = C.c_uint32(), C.c_uint32()
-        self.ctx.check(lib().ut_contours_build(self.handle, num_pairs, ptr(pairs), ptr(pair_ids), C.byref(nt),
-                                               C.byref(nc)))
+        self.ctx.check(lib().ut_contours_build(self.handle, num_pairs, ptr(pairs), ptr(pair_ids),
+                                               int(bool(ignore_near_zero_dist_pairs)), C.byref(nt), C.byref(nc)))
         self.num_total, self.num_contours = nt.value, nc.value
         return self.num_total, self.num_contours
 
+    def counts(self):
+        """(pairs given, pairs after the degenerate filter, total with loop pairs, adjacency entries)."""
+        out = (C.c_uint32 * 4)()
+        self.ctx.check(lib().ut_contours_counts(self.handle, out))
+        return tuple(out)
+
     def get(self, what: int):
         n = self.num_total
-        shape = {CT_PAIR_IDS: (n, 2), CT_PAIRS: (n, 12), CT_ADJ: (n, MAX_ADJ)}.get(what, (n,))
+        shape = {CT_PAIR_IDS: (n, 2), CT_PAIRS: (n, 12), CT_ADJ: (self.counts()[3],),
+                 CT_ADJ_OFF: (n + 1 if n else 0,)}.get(what, (n,))
         out = torch.empty(shape, dtype=torch.int32, device=self.ctx.device)
-        if n:
+        if out.numel():
             self.ctx.check(lib().ut_contours_get(self.handle, what, ptr(out)))
         return out
 

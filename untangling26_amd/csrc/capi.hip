@@ -398,6 +398,35 @@ UT_API int ut_ef_detect(ut_lbvh* face_tree, const float* x, const uint32_t* edge
 		});
 }
 
+UT_API int ut_ef_from_proximity(ut_ctx* ctx, const void* recs, const uint32_t* rec_ids, const uint32_t* rec_count,
+	uint32_t rec_cap, const float* x, const uint32_t* edges, const uint32_t* faces, const uint32_t* vert_adj_off,
+	const uint32_t* vert_adj_edges, const uint32_t* edge_adj_faces, uint32_t* count, void* pairs,
+	uint32_t* pair_ids, uint32_t max_pairs)
+{
+	if (!ctx)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			ut::EfFromProximityArgs a;
+			a.recs = reinterpret_cast<const uint32_t*>(recs);
+			a.rec_ids = rec_ids;
+			a.rec_count = rec_count;
+			a.rec_cap = rec_cap;
+			a.x = x;
+			a.edges = edges;
+			a.faces = faces;
+			a.vert_adj_off = vert_adj_off;
+			a.vert_adj_edges = vert_adj_edges;
+			a.edge_adj_faces = edge_adj_faces;
+			a.count = count;
+			a.pairs = reinterpret_cast<uint32_t*>(pairs);
+			a.pair_ids = pair_ids;
+			a.max_pairs = max_pairs;
+			ut::ef_from_proximity(&ctx->c, a);
+		});
+}
+
 // ---- contours ---------------------------------------------------------------
 UT_API int ut_contours_create(ut_ctx* ctx, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges,
 	const uint32_t* faces, const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces, ut_contours** out)
@@ -428,14 +457,15 @@ UT_API int ut_contours_destroy(ut_contours* ct)
 }
 
 UT_API int ut_contours_build(ut_contours* ct, uint32_t num_pairs, const void* pairs, const uint32_t* pair_ids,
-	uint32_t* num_total, uint32_t* num_contours)
+	int ignore_near_zero_dist_pairs, uint32_t* num_total, uint32_t* num_contours)
 {
 	if (!ct)
 		return ut::kInvalid;
 	return guarded(ct->b.ctx,
 		[&]
 		{
-			ct->b.build(num_pairs, reinterpret_cast<const ut::EfPairRec*>(pairs), pair_ids);
+			ct->b.build(num_pairs, reinterpret_cast<const ut::EfPairRec*>(pairs), pair_ids,
+				ignore_near_zero_dist_pairs == 0);
 			if (num_total)
 				*num_total = ct->b.num_total;
 			if (num_contours)
@@ -443,33 +473,51 @@ UT_API int ut_contours_build(ut_contours* ct, uint32_t num_pairs, const void* pa
 		});
 }
 
+// One UT_CT_* output of a contour build: device source and byte count.
+static void contour_output(const ut::ContourBuilder& b, int what, const void** src, size_t* bytes)
+{
+	const size_t n = b.num_total;
+	switch (what)
+	{
+	case UT_CT_PAIR_IDS: *src = b.ids.get(), *bytes = 8 * n; break;
+	case UT_CT_PAIRS: *src = b.pairs.get(), *bytes = sizeof(ut::EfPairRec) * n; break;
+	case UT_CT_CONTOUR_INDEX: *src = b.contour_index.get(), *bytes = 4 * n; break;
+	case UT_CT_MESH_INDEX: *src = b.mesh_index.get(), *bytes = 4 * n; break;
+	case UT_CT_ADJ: *src = b.adj.get(), *bytes = 4 * (size_t)b.num_adj; break;
+	case UT_CT_ADJ_N: *src = b.adj_n.get(), *bytes = 4 * n; break;
+	case UT_CT_INCONSISTENT: *src = b.comp_inconsistent.get(), *bytes = 4 * n; break;
+	case UT_CT_COMP_MIN: *src = b.comp_min.get(), *bytes = 4 * n; break;
+	case UT_CT_ADJ_OFF: *src = b.adj_off.get(), *bytes = n ? 4 * (n + 1) : 0; break;
+	default: throw ut::Error(ut::kInvalid, "unknown contour output");
+	}
+}
+
+static void copy_contour_output(const ut::ContourBuilder& b, int what, void* dst)
+{
+	const void* src = nullptr;
+	size_t		bytes = 0;
+	contour_output(b, what, &src, &bytes);
+	UT_REQUIRE(dst || bytes == 0, "null destination");
+	if (bytes)
+		UT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
+}
+
 UT_API int ut_contours_get(ut_contours* ct, int what, void* dst)
 {
 	if (!ct)
 		return ut::kInvalid;
-	return guarded(ct->b.ctx,
-		[&]
-		{
-			auto&		 b = ct->b;
-			const size_t n = b.num_total;
-			const void*	 src = nullptr;
-			size_t		 bytes = 0;
-			switch (what)
-			{
-			case UT_CT_PAIR_IDS: src = b.ids.get(), bytes = 8 * n; break;
-			case UT_CT_PAIRS: src = b.pairs.get(), bytes = sizeof(ut::EfPairRec) * n; break;
-			case UT_CT_CONTOUR_INDEX: src = b.contour_index.get(), bytes = 4 * n; break;
-			case UT_CT_MESH_INDEX: src = b.mesh_index.get(), bytes = 4 * n; break;
-			case UT_CT_ADJ: src = b.adj.get(), bytes = 4 * ut::kMaxEfAdj * n; break;
-			case UT_CT_ADJ_N: src = b.adj_n.get(), bytes = 4 * n; break;
-			case UT_CT_INCONSISTENT: src = b.comp_inconsistent.get(), bytes = 4 * n; break;
-			case UT_CT_COMP_MIN: src = b.comp_min.get(), bytes = 4 * n; break;
-			default: throw ut::Error(ut::kInvalid, "unknown contour output");
-			}
-			UT_REQUIRE(dst || bytes == 0, "null destination");
-			if (bytes)
-				UT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
-		});
+	return guarded(ct->b.ctx, [&] { copy_contour_output(ct->b, what, dst); });
+}
+
+UT_API int ut_contours_counts(ut_contours* ct, uint32_t* out4)
+{
+	if (!ct || !out4)
+		return ut::kInvalid;
+	out4[0] = ct->b.num_input;
+	out4[1] = ct->b.num_pairs;
+	out4[2] = ct->b.num_total;
+	out4[3] = ct->b.num_adj;
+	return ut::kOk;
 }
 
 UT_API int ut_graph_components(ut_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
@@ -673,6 +721,7 @@ ut::SolverParams to_params(const ut_solver_params& q)
 	p.qs_inertia_hessian = 1;
 	UT_REQUIRE(q.region_filter == 0, "solver: the contact region filter is switched off (not yet verified on the device)");
 	p.region_filter = q.region_filter;
+	p.ignore_near_zero_dist_pairs = q.ignore_near_zero_dist_pairs;
 	if (q.prp.total_dirs)
 	{
 		p.prp.max_contours = q.prp.max_contours;
@@ -840,26 +889,8 @@ UT_API int ut_solver_contour_get(ut_solver* s, int what, void* dst)
 	return guarded(s->s.ctx,
 		[&]
 		{
-			auto&		 b = s->s.contours;
-			UT_REQUIRE(b.ctx, "solver has no contour builder (topology not given)");
-			const size_t n = b.num_total;
-			const void*	 src = nullptr;
-			size_t		 bytes = 0;
-			switch (what)
-			{
-			case UT_CT_PAIR_IDS: src = b.ids.get(), bytes = 8 * n; break;
-			case UT_CT_PAIRS: src = b.pairs.get(), bytes = sizeof(ut::EfPairRec) * n; break;
-			case UT_CT_CONTOUR_INDEX: src = b.contour_index.get(), bytes = 4 * n; break;
-			case UT_CT_MESH_INDEX: src = b.mesh_index.get(), bytes = 4 * n; break;
-			case UT_CT_ADJ: src = b.adj.get(), bytes = 4 * ut::kMaxEfAdj * n; break;
-			case UT_CT_ADJ_N: src = b.adj_n.get(), bytes = 4 * n; break;
-			case UT_CT_INCONSISTENT: src = b.comp_inconsistent.get(), bytes = 4 * n; break;
-			case UT_CT_COMP_MIN: src = b.comp_min.get(), bytes = 4 * n; break;
-			default: throw ut::Error(ut::kInvalid, "unknown contour output");
-			}
-			UT_REQUIRE(dst || bytes == 0, "null destination");
-			if (bytes)
-				UT_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.ctx->stream));
+			UT_REQUIRE(s->s.contours.ctx, "solver has no contour builder (topology not given)");
+			copy_contour_output(s->s.contours, what, dst);
 		});
 }
 

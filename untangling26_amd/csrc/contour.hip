@@ -156,7 +156,7 @@ __global__ void k_ct_key_eid(uint32_t P, const uint32_t* __restrict__ ids, const
 	k[i] = ids[2 * perm[i]];
 }
 
-// Gather into canonical order (sorted by edge, then face; intersection_resolver.cpp:695-726).
+// Gather into (edge, face) order (intersection_resolver.cpp:695-726).
 __global__ void k_ct_gather(uint32_t P, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ in_ids,
 	const EfPairRec* __restrict__ in_pairs, uint32_t* __restrict__ ids, EfPairRec* __restrict__ pairs,
 	uint64_t* __restrict__ keys)
@@ -170,6 +170,48 @@ __global__ void k_ct_gather(uint32_t P, const uint32_t* __restrict__ perm, const
 	ids[2 * i + 1] = f;
 	pairs[i] = in_pairs[j];
 	keys[i] = ((uint64_t)e << 32) | f;
+}
+
+// memcmp order of two 48-byte records: the first differing byte decides, as
+// unsigned bytes in memory order (little-endian words: lowest byte first).
+__device__ __forceinline__ int payload_cmp(const EfPairRec& a, const EfPairRec& b)
+{
+	const uint32_t* x = reinterpret_cast<const uint32_t*>(&a);
+	const uint32_t* y = reinterpret_cast<const uint32_t*>(&b);
+	for (int w = 0; w < 12; ++w)
+	{
+		const uint32_t d = x[w] ^ y[w];
+		if (d)
+		{
+			const int sh = __builtin_ctz(d) & ~7;
+			return ((x[w] >> sh) & 0xFFu) < ((y[w] >> sh) & 0xFFu) ? -1 : 1;
+		}
+	}
+	return 0;
+}
+
+// Pairs with equal (edge, face) keys (e.g. a DCD pair and pairs made from
+// zero-distance proximity pairs) ordered by payload bytes, the tie-break of
+// intersection_resolver.cpp:712-716.  One thread per run of equal keys.
+__global__ void k_ct_tie_order(uint32_t P, const uint64_t* __restrict__ keys, EfPairRec* __restrict__ pairs)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P || (i > 0 && keys[i - 1] == keys[i]) || i + 1 >= P || keys[i + 1] != keys[i])
+		return;
+	uint32_t end = i + 1;
+	while (end < P && keys[end] == keys[i])
+		++end;
+	for (uint32_t a = i + 1; a < end; ++a) // insertion sort; runs are a few pairs long
+	{
+		const EfPairRec r = pairs[a];
+		uint32_t		b = a;
+		while (b > i && payload_cmp(pairs[b - 1], r) > 0)
+		{
+			pairs[b] = pairs[b - 1];
+			--b;
+		}
+		pairs[b] = r;
+	}
 }
 
 __device__ __forceinline__ bool face_has(const uint32_t* faces, uint32_t f, uint32_t v)
@@ -199,6 +241,115 @@ __device__ __forceinline__ uint32_t loop_vertex(uint32_t e, uint32_t f, uint32_t
 	return kNone;
 }
 
+__device__ __forceinline__ uint32_t lower_key(const uint64_t* __restrict__ keys, uint32_t P, uint64_t k)
+{
+	uint32_t lo = 0, hi = P;
+	while (lo < hi)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (keys[mid] < k)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+// compute_adj_pairs (intersection_resolver.cpp:503-558) of pair p among the P
+// sorted pairs, in the reference's order: every pair (all of a key's pairs)
+// on another edge of a face around e with the same intersected face, then
+// every pair of an edge of the intersected face cutting a face around e.
+// visit(q) returns false to stop.
+template <class Visit>
+__device__ __forceinline__ void for_each_adjacent(uint32_t p, uint32_t P, const uint32_t* __restrict__ ids,
+	const uint64_t* __restrict__ keys, const uint32_t* __restrict__ face_adj_edges,
+	const uint32_t* __restrict__ edge_adj_faces, Visit&& visit)
+{
+	const uint32_t e = ids[2 * p], f = ids[2 * p + 1];
+	const uint32_t T[2] = { edge_adj_faces[2 * e], edge_adj_faces[2 * e + 1] };
+	auto		   key_run = [&](uint32_t ke, uint32_t kf) -> bool
+	{
+		const uint64_t k = ((uint64_t)ke << 32) | kf;
+		for (uint32_t q = lower_key(keys, P, k); q < P && keys[q] == k; ++q)
+			if (q != p && !visit(q))
+				return false;
+		return true;
+	};
+	for (int k = 0; k < 2; ++k)
+	{
+		if (T[k] == kNone)
+			continue;
+		const uint32_t a = face_adj_edges[3 * T[k]], b = face_adj_edges[3 * T[k] + 1], c = face_adj_edges[3 * T[k] + 2];
+		const uint32_t o0 = (a == e) ? b : a;
+		const uint32_t o1 = (a == e || b == e) ? c : b;
+		if (!key_run(o0, f) || !key_run(o1, f))
+			return;
+	}
+	for (int ii = 0; ii < 3; ++ii)
+	{
+		const uint32_t fe = face_adj_edges[3 * f + ii];
+		if (fe == e)
+			continue;
+		for (int k = 0; k < 2; ++k)
+			if (T[k] != kNone && !key_run(fe, T[k]))
+				return;
+	}
+}
+
+// is_degenerate_ef_pair (intersection_resolver.cpp:561-569).
+__device__ __forceinline__ bool ef_degenerate(const EfPairRec& r)
+{
+	const float e1 = 1.0f - r.bary[0];
+	const float f2 = 1.0f - r.bary[1] - r.bary[2];
+	const float emin = e1 < r.bary[0] ? e1 : r.bary[0];
+	float		fmin = r.bary[1];
+	if (r.bary[2] < fmin)
+		fmin = r.bary[2];
+	if (f2 < fmin)
+		fmin = f2;
+	return emin < 1e-4f || fmin < 1e-4f;
+}
+
+// intersection_resolver.cpp:841-870: a degenerate pair stays when one of its
+// adjacent pairs is not degenerate.
+__global__ void k_ct_keep(uint32_t P, const uint32_t* __restrict__ ids, const uint64_t* __restrict__ keys,
+	const EfPairRec* __restrict__ pairs, const uint32_t* __restrict__ face_adj_edges,
+	const uint32_t* __restrict__ edge_adj_faces, uint32_t* __restrict__ keep)
+{
+	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= P)
+		return;
+	bool k = !ef_degenerate(pairs[p]);
+	if (!k)
+		for_each_adjacent(p, P, ids, keys, face_adj_edges, edge_adj_faces,
+			[&](uint32_t q)
+			{
+				k = !ef_degenerate(pairs[q]);
+				return !k;
+			});
+	keep[p] = k ? 1u : 0u;
+}
+
+// Kept pairs, in order, into the second block of P slots of ids / pairs.
+__global__ void k_ct_compact(uint32_t P, const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+	uint32_t* __restrict__ ids, EfPairRec* __restrict__ pairs)
+{
+	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= P || !keep[p])
+		return;
+	const uint32_t w = P + pos[p];
+	ids[2 * w] = ids[2 * p];
+	ids[2 * w + 1] = ids[2 * p + 1];
+	pairs[w] = pairs[p];
+}
+
+__global__ void k_ct_keys(uint32_t P, const uint32_t* __restrict__ ids, uint64_t* __restrict__ keys)
+{
+	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p < P)
+		keys[p] = ((uint64_t)ids[2 * p] << 32) | ids[2 * p + 1];
+}
+
 __global__ void k_ct_loop_count(uint32_t P, const uint32_t* __restrict__ ids, const uint32_t* __restrict__ edges,
 	const uint32_t* __restrict__ faces, const uint32_t* __restrict__ face_adj_edges,
 	const uint32_t* __restrict__ edge_adj_faces, uint32_t* __restrict__ cnt)
@@ -218,16 +369,39 @@ __global__ void k_ct_loop_count(uint32_t P, const uint32_t* __restrict__ ids, co
 	cnt[p] = c;
 }
 
+// Neighbour counts of the P pairs (compute_adj_pairs plus their loop pairs).
+__global__ void k_ct_adj_count(uint32_t P, const uint32_t* __restrict__ ids, const uint64_t* __restrict__ keys,
+	const uint32_t* __restrict__ face_adj_edges, const uint32_t* __restrict__ edge_adj_faces,
+	const uint32_t* __restrict__ loop_cnt, uint32_t* __restrict__ adj_n)
+{
+	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= P)
+		return;
+	uint32_t n = loop_cnt[p];
+	for_each_adjacent(p, P, ids, keys, face_adj_edges, edge_adj_faces,
+		[&](uint32_t)
+		{
+			++n;
+			return true;
+		});
+	adj_n[p] = n;
+}
+
+// Loop pairs (intersection_resolver.cpp:1009-1018): pair P + r, r = loop_off[p] + j,
+// edge bary (1, 0), face bary (1, 0, 0), all five indices the loop vertex; its
+// only neighbour is its source pair, stored after the P pairs' neighbours.
 __global__ void k_ct_loop_emit(uint32_t P, uint32_t* __restrict__ ids, EfPairRec* __restrict__ pairs,
 	const uint32_t* __restrict__ edges, const uint32_t* __restrict__ faces,
 	const uint32_t* __restrict__ face_adj_edges, const uint32_t* __restrict__ edge_adj_faces,
-	const uint32_t* __restrict__ off, uint32_t* __restrict__ adj, uint32_t* __restrict__ adj_n)
+	const uint32_t* __restrict__ loop_off, const uint32_t* __restrict__ adj_total, uint32_t* __restrict__ adj,
+	uint32_t* __restrict__ adj_off, uint32_t* __restrict__ adj_n)
 {
 	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= P)
 		return;
 	const uint32_t e = ids[2 * p], f = ids[2 * p + 1];
-	uint32_t	   idx = P + off[p];
+	const uint32_t base = *adj_total;
+	uint32_t	   r = loop_off[p];
 	for (int k = 0; k < 2; ++k)
 	{
 		const uint32_t T = edge_adj_faces[2 * e + k];
@@ -235,95 +409,50 @@ __global__ void k_ct_loop_emit(uint32_t P, uint32_t* __restrict__ ids, EfPairRec
 		const uint32_t le = T != kNone ? loop_vertex(e, f, T, edges, faces, face_adj_edges, &v) : kNone;
 		if (le == kNone)
 			continue;
+		const uint32_t idx = P + r;
 		ids[2 * idx] = le;
 		ids[2 * idx + 1] = f;
-		EfPairRec r;
+		EfPairRec rec;
 		for (int q = 0; q < 5; ++q)
-			r.idx[q] = v;
-		r.bary[0] = 1.0f; // edge bary (1, 0), face bary (1, 0, 0)
-		r.bary[1] = 1.0f;
-		r.bary[2] = 0.0f;
-		r.grad[0] = r.grad[1] = r.grad[2] = 0.0f;
-		r.stiff = 0.0f;
-		pairs[idx] = r;
-		adj[kMaxEfAdj * idx] = p;
+			rec.idx[q] = v;
+		rec.bary[0] = 1.0f;
+		rec.bary[1] = 1.0f;
+		rec.bary[2] = 0.0f;
+		rec.grad[0] = rec.grad[1] = rec.grad[2] = 0.0f;
+		rec.stiff = 0.0f;
+		pairs[idx] = rec;
+		adj[base + r] = p;
+		adj_off[idx] = base + r;
 		adj_n[idx] = 1;
-		++idx;
+		++r;
 	}
 }
 
-__device__ __forceinline__ int find_pair(const uint64_t* __restrict__ keys, uint32_t P, uint32_t e, uint32_t f)
+__global__ void k_ct_adj_tail(uint32_t P, const uint32_t* __restrict__ loops, const uint32_t* __restrict__ adj_total,
+	uint32_t* __restrict__ adj_off)
 {
-	const uint64_t k = ((uint64_t)e << 32) | f;
-	uint32_t	   lo = 0, hi = P;
-	while (lo < hi)
-	{
-		const uint32_t mid = (lo + hi) >> 1;
-		if (keys[mid] < k)
-			lo = mid + 1;
-		else
-			hi = mid;
-	}
-	return (lo < P && keys[lo] == k) ? (int)lo : -1;
+	if (threadIdx.x == 0)
+		adj_off[P + *loops] = *adj_total + *loops;
 }
 
-// compute_adj_pairs (intersection_resolver.cpp:503-558) plus the loop pairs
-// appended to their source pair (:1009-1018), in the reference's order.
-__global__ void k_ct_adjacency(uint32_t P, const uint32_t* __restrict__ ids, const uint64_t* __restrict__ keys,
+__global__ void k_ct_adj_fill(uint32_t P, const uint32_t* __restrict__ ids, const uint64_t* __restrict__ keys,
 	const uint32_t* __restrict__ face_adj_edges, const uint32_t* __restrict__ edge_adj_faces,
-	const uint32_t* __restrict__ loop_cnt, const uint32_t* __restrict__ loop_off, uint32_t* __restrict__ adj,
-	uint32_t* __restrict__ adj_n, uint32_t* __restrict__ status)
+	const uint32_t* __restrict__ loop_cnt, const uint32_t* __restrict__ loop_off, const uint32_t* __restrict__ adj_off,
+	uint32_t* __restrict__ adj)
 {
 	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= P)
 		return;
-	const uint32_t e = ids[2 * p], f = ids[2 * p + 1];
-	const uint32_t T[2] = { edge_adj_faces[2 * e], edge_adj_faces[2 * e + 1] };
-	uint32_t	   n = 0;
-	auto		   push = [&](uint32_t q)
-	{
-		if (n < kMaxEfAdj)
-			adj[kMaxEfAdj * p + n] = q;
-		++n;
-	};
-	// via the other edges of the faces around e, same intersected face
-	for (int k = 0; k < 2; ++k)
-	{
-		if (T[k] == kNone)
-			continue;
-		const uint32_t a = face_adj_edges[3 * T[k]], b = face_adj_edges[3 * T[k] + 1], c = face_adj_edges[3 * T[k] + 2];
-		const uint32_t o0 = (a == e) ? b : a;
-		const uint32_t o1 = (a == e || b == e) ? c : b;
-		const int	   q0 = find_pair(keys, P, o0, f);
-		if (q0 >= 0 && (uint32_t)q0 != p)
-			push((uint32_t)q0);
-		const int q1 = find_pair(keys, P, o1, f);
-		if (q1 >= 0 && (uint32_t)q1 != p)
-			push((uint32_t)q1);
-	}
-	// via an edge of the intersected face cutting a face around e
-	for (int ii = 0; ii < 3; ++ii)
-	{
-		const uint32_t fe = face_adj_edges[3 * f + ii];
-		if (fe == e)
-			continue;
-		for (int k = 0; k < 2; ++k)
+	uint32_t* out = adj + adj_off[p];
+	uint32_t  n = 0;
+	for_each_adjacent(p, P, ids, keys, face_adj_edges, edge_adj_faces,
+		[&](uint32_t q)
 		{
-			if (T[k] == kNone)
-				continue;
-			const int q = find_pair(keys, P, fe, T[k]);
-			if (q >= 0 && (uint32_t)q != p)
-				push((uint32_t)q);
-		}
-	}
+			out[n++] = q;
+			return true;
+		});
 	for (uint32_t j = 0; j < loop_cnt[p]; ++j)
-		push(P + loop_off[p] + j);
-	if (n > kMaxEfAdj)
-	{
-		atomicAdd(&status[0], 1u);
-		n = kMaxEfAdj;
-	}
-	adj_n[p] = n;
+		out[n++] = P + loop_off[p] + j;
 }
 
 __device__ __forceinline__ uint32_t flip_weight(const EfPairRec& a, const EfPairRec& b)
@@ -332,15 +461,15 @@ __device__ __forceinline__ uint32_t flip_weight(const EfPairRec& a, const EfPair
 }
 
 __global__ void k_ct_union(uint32_t N, const EfPairRec* __restrict__ pairs, const uint32_t* __restrict__ adj,
-	const uint32_t* __restrict__ adj_n, uint32_t* parent)
+	const uint32_t* __restrict__ adj_off, uint32_t* parent)
 {
 	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= N)
 		return;
 	const EfPairRec rp = pairs[p];
-	for (uint32_t s = 0; s < adj_n[p]; ++s)
+	for (uint32_t s = adj_off[p]; s < adj_off[p + 1]; ++s)
 	{
-		const uint32_t q = adj[kMaxEfAdj * p + s];
+		const uint32_t q = adj[s];
 		const uint32_t w = flip_weight(rp, pairs[q]);
 		uf_union(parent, 2 * p, 2 * q + w);
 		uf_union(parent, 2 * p + 1, 2 * q + (1u - w));
@@ -384,7 +513,7 @@ void ContourBuilder::reserve(uint32_t p)
 	if (p <= cap)
 		return;
 	cap = p;
-	const size_t total = 3 * (size_t)p; // up to two loop pairs per DCD pair
+	const size_t total = 3 * (size_t)p; // up to two loop pairs per pair
 	for (int k = 0; k < 2; ++k)
 	{
 		sort_k[k].reserve(p);
@@ -397,7 +526,8 @@ void ContourBuilder::reserve(uint32_t p)
 	pairs.reserve(total);
 	loop_cnt.reserve(p);
 	loop_off.reserve(p);
-	adj.reserve(kMaxEfAdj * total);
+	adj.reserve(8 * total);
+	adj_off.reserve(total + 1);
 	adj_n.reserve(total);
 	uf.reserve(2 * total);
 	comp_min.reserve(total);
@@ -408,18 +538,20 @@ void ContourBuilder::reserve(uint32_t p)
 	comp_inconsistent.reserve(total);
 }
 
-void ContourBuilder::build(uint32_t P, const EfPairRec* in_pairs, const uint32_t* in_ids)
+void ContourBuilder::build(uint32_t P, const EfPairRec* in_pairs, const uint32_t* in_ids, bool filter_degenerate)
 {
+	num_input = P;
 	num_pairs = P;
 	num_total = 0;
+	num_adj = 0;
 	num_contours = 0;
 	if (P == 0)
 		return;
 	UT_REQUIRE(in_pairs && in_ids, "contour build: null input");
 	reserve(P);
 	hipStream_t s = ctx->stream;
-	const uint32_t g = div_up(P, 256);
-	// canonical order: stable sort by face, then by edge
+	uint32_t	g = div_up(P, 256);
+	// canonical order: stable sort by face, then by edge, then payload within equal keys
 	RadixSortBuffers rs;
 	for (int k = 0; k < 2; ++k)
 	{
@@ -438,40 +570,70 @@ void ContourBuilder::build(uint32_t P, const EfPairRec* in_pairs, const uint32_t
 	radix_sort_pairs(rs, P, bits_for(ne), s);
 	const uint32_t* perm = rs.vals[rs.result];
 	k_ct_gather<<<g, 256, 0, s>>>(P, perm, in_ids, in_pairs, ids.get(), pairs.get(), keys.get());
-	// loop-vertex pairs
-	uint32_t* d_status = scalars.get();
+	k_ct_tie_order<<<g, 256, 0, s>>>(P, keys.get(), pairs.get());
+	UT_LAUNCH_CHECK();
 	uint32_t* d_loops = scalars.get() + 1;
 	uint32_t* d_contours = scalars.get() + 2;
-	k_fill_u32<<<1, 64, 0, s>>>(d_status, 1, 0u);
+	uint32_t* d_adj = scalars.get() + 3;
+	uint32_t* d_kept = scalars.get() + 4;
+	if (filter_degenerate)
+	{
+		// the sort buffers are free once the pairs are gathered
+		uint32_t* keep = sort_k[0].get();
+		uint32_t* pos = sort_k[1].get();
+		k_ct_keep<<<g, 256, 0, s>>>(P, ids.get(), keys.get(), pairs.get(), face_adj_edges, edge_adj_faces, keep);
+		exclusive_scan(keep, pos, P, scan_tmp.get(), d_kept, s);
+		UT_LAUNCH_CHECK();
+		uint32_t K = 0;
+		UT_HIP_CHECK(hipMemcpyAsync(&K, d_kept, 4, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipStreamSynchronize(s));
+		// no pair kept: all pairs stay (intersection_resolver.cpp:874-877)
+		if (K > 0 && K < P)
+		{
+			k_ct_compact<<<g, 256, 0, s>>>(P, keep, pos, ids.get(), pairs.get());
+			UT_HIP_CHECK(hipMemcpyAsync(ids.get(), ids.get() + 2 * (size_t)P, 8 * (size_t)K, hipMemcpyDeviceToDevice, s));
+			UT_HIP_CHECK(hipMemcpyAsync(pairs.get(), pairs.get() + P, sizeof(EfPairRec) * K, hipMemcpyDeviceToDevice, s));
+			P = K;
+			g = div_up(P, 256);
+			k_ct_keys<<<g, 256, 0, s>>>(P, ids.get(), keys.get());
+			UT_LAUNCH_CHECK();
+		}
+		num_pairs = P;
+	}
+	// loop-vertex pairs and the packed adjacency
 	k_ct_loop_count<<<g, 256, 0, s>>>(P, ids.get(), edges, faces, face_adj_edges, edge_adj_faces, loop_cnt.get());
 	exclusive_scan(loop_cnt.get(), loop_off.get(), P, scan_tmp.get(), d_loops, s);
-	k_ct_loop_emit<<<g, 256, 0, s>>>(P, ids.get(), pairs.get(), edges, faces, face_adj_edges, edge_adj_faces,
-		loop_off.get(), adj.get(), adj_n.get());
-	k_ct_adjacency<<<g, 256, 0, s>>>(P, ids.get(), keys.get(), face_adj_edges, edge_adj_faces, loop_cnt.get(),
-		loop_off.get(), adj.get(), adj_n.get(), d_status);
+	k_ct_adj_count<<<g, 256, 0, s>>>(P, ids.get(), keys.get(), face_adj_edges, edge_adj_faces, loop_cnt.get(),
+		adj_n.get());
+	exclusive_scan(adj_n.get(), adj_off.get(), P, scan_tmp.get(), d_adj, s);
+	k_ct_adj_tail<<<1, 64, 0, s>>>(P, d_loops, d_adj, adj_off.get());
 	UT_LAUNCH_CHECK();
-	uint32_t L = 0;
-	UT_HIP_CHECK(hipMemcpyAsync(&L, d_loops, 4, hipMemcpyDeviceToHost, s));
+	uint32_t sc[5] = { 0, 0, 0, 0, 0 };
+	UT_HIP_CHECK(hipMemcpyAsync(sc, scalars.get(), 20, hipMemcpyDeviceToHost, s));
 	UT_HIP_CHECK(hipStreamSynchronize(s));
-	const uint32_t N = P + L;
+	const uint32_t L = sc[1], N = P + L;
 	num_total = N;
-	const uint32_t gn = div_up(N, 256), g2 = div_up(2 * N, 256);
+	num_adj = sc[3] + L;
+	adj.reserve(num_adj);
+	k_ct_loop_emit<<<g, 256, 0, s>>>(P, ids.get(), pairs.get(), edges, faces, face_adj_edges, edge_adj_faces,
+		loop_off.get(), d_adj, adj.get(), adj_off.get(), adj_n.get());
+	k_ct_adj_fill<<<g, 256, 0, s>>>(P, ids.get(), keys.get(), face_adj_edges, edge_adj_faces, loop_cnt.get(),
+		loop_off.get(), adj_off.get(), adj.get());
 	// components + parity
+	const uint32_t gn = div_up(N, 256), g2 = div_up(2 * N, 256);
 	k_uf_init<<<g2, 256, 0, s>>>(uf.get(), 2 * N);
 	k_fill_u32<<<gn, 256, 0, s>>>(comp_inconsistent.get(), N, 0u);
-	k_ct_union<<<gn, 256, 0, s>>>(N, pairs.get(), adj.get(), adj_n.get(), uf.get());
+	k_ct_union<<<gn, 256, 0, s>>>(N, pairs.get(), adj.get(), adj_off.get(), uf.get());
 	k_uf_compress<<<g2, 256, 0, s>>>(uf.get(), 2 * N);
 	k_uf_labels<<<gn, 256, 0, s>>>(uf.get(), N, comp_min.get(), mesh_index.get(), is_root.get(),
 		comp_inconsistent.get());
 	exclusive_scan(is_root.get(), rank.get(), N, scan_tmp.get(), d_contours, s);
 	k_ct_contour_index<<<gn, 256, 0, s>>>(N, comp_min.get(), rank.get(), contour_index.get());
 	UT_LAUNCH_CHECK();
-	uint32_t sc[3] = { 0, 0, 0 };
-	UT_HIP_CHECK(hipMemcpyAsync(sc, scalars.get(), 12, hipMemcpyDeviceToHost, s));
+	uint32_t nc = 0;
+	UT_HIP_CHECK(hipMemcpyAsync(&nc, d_contours, 4, hipMemcpyDeviceToHost, s));
 	UT_HIP_CHECK(hipStreamSynchronize(s));
-	num_contours = sc[2];
-	if (sc[0] != 0)
-		throw Error(kInternal, "EF pair adjacency exceeded " + std::to_string(kMaxEfAdj) + " entries");
+	num_contours = nc;
 }
 
 } // namespace ut

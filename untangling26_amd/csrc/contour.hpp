@@ -13,8 +13,6 @@
 namespace ut
 {
 
-constexpr int kMaxEfAdj = 12; // 4 (shared face) + 6 (edge of the face) + 2 loop pairs
-
 struct ContourBuilder
 {
 	Context*		ctx = nullptr;
@@ -24,19 +22,22 @@ struct ContourBuilder
 	const uint32_t* face_adj_edges = nullptr; // [3F]  edge between face[j], face[j+1]
 	const uint32_t* edge_adj_faces = nullptr; // [2E]  ascending, kNone when absent
 
-	uint32_t num_pairs = 0;	   // DCD pairs (before loop pairs)
+	uint32_t num_input = 0;	   // EF pairs handed to build()
+	uint32_t num_pairs = 0;	   // after the degenerate filter (before loop pairs)
 	uint32_t num_total = 0;	   // + loop pairs
+	uint32_t num_adj = 0;	   // adjacency entries
 	uint32_t num_contours = 0;
 	uint32_t cap = 0;
 
 	DevBuf<uint32_t>  sort_k[2], sort_v[2], sort_h, scan_tmp, scalars;
-	DevBuf<uint64_t>  keys;		  // sorted (eid << 32 | fid) of the DCD pairs
+	DevBuf<uint64_t>  keys;		  // sorted (eid << 32 | fid) of the non-loop pairs
 	DevBuf<uint32_t>  ids;		  // [2 * total] (eid, fid), sorted, loop pairs appended
 	DevBuf<EfPairRec> pairs;	  // [total]
 	DevBuf<uint32_t>  loop_cnt;	  // [P]
 	DevBuf<uint32_t>  loop_off;	  // [P]
-	DevBuf<uint32_t>  adj;		  // [kMaxEfAdj * total]
-	DevBuf<uint32_t>  adj_n;	  // [total]
+	DevBuf<uint32_t>  adj;		  // [num_adj] neighbours, packed by pair
+	DevBuf<uint32_t>  adj_off;	  // [total + 1] start of each pair's neighbours
+	DevBuf<uint32_t>  adj_n;	  // [total] neighbour counts
 	DevBuf<uint32_t>  uf;		  // [2 * total] union-find parents of the parity-doubled graph
 	DevBuf<uint32_t>  comp_min;	  // [total] smallest pair index of the component
 	DevBuf<uint32_t>  mesh_index; // [total] parity relative to that pair
@@ -47,8 +48,9 @@ struct ContourBuilder
 
 	void init(Context* c, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges, const uint32_t* faces,
 		const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces);
-	// num_pairs DCD pairs in any order; synchronises to learn the loop-pair and contour counts.
-	void build(uint32_t num_pairs, const EfPairRec* in_pairs, const uint32_t* in_ids);
+	// num_pairs EF pairs in any order; filter_degenerate is the reference's
+	// !ignore_near_zero_dist_pairs.  Synchronises to learn the counts.
+	void build(uint32_t num_pairs, const EfPairRec* in_pairs, const uint32_t* in_ids, bool filter_degenerate);
 	void reserve(uint32_t p);
 };
 

@@ -10,6 +10,8 @@
 #include "lbvh_traverse.hpp"
 #include "seg_tri.hpp"
 
+#include <algorithm>
+
 namespace ut
 {
 
@@ -201,6 +203,110 @@ void ef_dcd(Context* ctx, const EfDcdArgs& a)
 		reinterpret_cast<EfPairRec*>(a.pairs),
 		reinterpret_cast<uint2*>(a.pair_ids),
 		a.max_pairs);
+	UT_LAUNCH_CHECK();
+}
+
+// A proximity pair whose weighted point difference vanishes (|d|^2 < 1e-10,
+// the thickness term is 0 in the reference) turns into EF pairs with zero
+// gradient and stiffness.  VF (v, f): one per edge around v, edge bary (1, 0)
+// when v is the edge's first vertex else (0, 1), face bary |w1|, |w2|.  EE
+// (e1, e2): for each face around e2 without a vertex of e1, (e1, face) with
+// e1's bary and the face corners on e2 weighted by e1's bary; then for each
+// face around e1 without a vertex of e2, (e2, face) with e2's bary and the
+// corners on e1 weighted by e1's bary (the reference weights both with e1).
+__global__ void k_ef_from_proximity(EfFromProximityArgs a)
+{
+	const uint32_t n = min(*a.rec_count, a.rec_cap);
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const uint32_t* r = a.recs + 16 * (size_t)i;
+		const uint32_t	r2 = r[2], r3 = r[3];
+		const bool		is_vf = (r2 & 0x80000000u) != 0, is_ee = !is_vf && (r3 & 0x80000000u) != 0;
+		if (!is_vf && !is_ee)
+			continue;
+		const uint32_t q[4] = { r[0] & 0x7FFFFFFFu, r[1] & 0x7FFFFFFFu, r2 & 0x7FFFFFFFu, r3 & 0x7FFFFFFFu };
+		const float	   b0 = __uint_as_float(r[10]), b1 = __uint_as_float(r[11]);
+		float		   w[4];
+		if (is_vf)
+			w[0] = 1.0f, w[1] = -b0, w[2] = -b1, w[3] = b0 + b1 - 1.0f;
+		else
+			w[0] = b0, w[1] = 1.0f - b0, w[2] = -b1, w[3] = b1 - 1.0f;
+		float d[3];
+		for (int c = 0; c < 3; ++c)
+			d[c] = w[0] * a.x[3 * q[0] + c] + w[1] * a.x[3 * q[1] + c] + w[2] * a.x[3 * q[2] + c]
+				+ w[3] * a.x[3 * q[3] + c];
+		const float d2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+		if (d2 >= 1e-10f)
+			continue;
+		const uint2 id = reinterpret_cast<const uint2*>(a.rec_ids)[i];
+		auto		emit = [&](uint32_t ev0, uint32_t ev1, uint32_t fid, float eb, float fb0, float fb1, uint32_t eid)
+		{
+			const uint32_t s = atomicAdd(a.count, 1u);
+			if (s >= a.max_pairs)
+				return;
+			EfPairRec o;
+			o.idx[0] = ev0, o.idx[1] = ev1;
+			o.idx[2] = a.faces[3 * fid], o.idx[3] = a.faces[3 * fid + 1], o.idx[4] = a.faces[3 * fid + 2];
+			o.bary[0] = eb, o.bary[1] = fb0, o.bary[2] = fb1;
+			o.grad[0] = o.grad[1] = o.grad[2] = 0.0f;
+			o.stiff = 0.0f;
+			reinterpret_cast<EfPairRec*>(a.pairs)[s] = o;
+			reinterpret_cast<uint2*>(a.pair_ids)[s] = make_uint2(eid, fid);
+		};
+		if (is_vf)
+		{
+			const uint32_t v = q[0], f = id.y;
+			const float	   fb0 = fabsf(w[1]), fb1 = fabsf(w[2]);
+			for (uint32_t k = a.vert_adj_off[v]; k < a.vert_adj_off[v + 1]; ++k)
+			{
+				const uint32_t ae = a.vert_adj_edges[k];
+				const uint32_t ev0 = a.edges[2 * ae], ev1 = a.edges[2 * ae + 1];
+				emit(ev0, ev1, f, ev0 == v ? 1.0f : 0.0f, fb0, fb1, ae);
+			}
+			continue;
+		}
+		const uint32_t e1 = id.x, e2 = id.y;
+		const uint32_t a0 = a.edges[2 * e1], a1 = a.edges[2 * e1 + 1], c0 = a.edges[2 * e2], c1 = a.edges[2 * e2 + 1];
+		const float	   eb1[2] = { fabsf(w[0]), fabsf(w[1]) }, eb2x = fabsf(w[2]);
+		for (int side = 0; side < 2; ++side)
+		{
+			const uint32_t fe = side ? e1 : e2, ee = side ? e2 : e1;
+			const uint32_t s0 = side ? a0 : c0, s1 = side ? a1 : c1; // corners the face shares
+			const uint32_t x0 = side ? c0 : a0, x1 = side ? c1 : a1; // must not be in the face
+			for (int k = 0; k < 2; ++k)
+			{
+				const uint32_t af = a.edge_adj_faces[2 * fe + k];
+				if (af == kNone)
+					continue;
+				const uint32_t F[3] = { a.faces[3 * af], a.faces[3 * af + 1], a.faces[3 * af + 2] };
+				if (F[0] == x0 || F[1] == x0 || F[2] == x0 || F[0] == x1 || F[1] == x1 || F[2] == x1)
+					continue;
+				float fb[3] = { 0.0f, 0.0f, 0.0f };
+				int	  matched = 0;
+				for (int j = 0; j < 3; ++j)
+				{
+					if (F[j] == s0)
+						fb[j] = eb1[0], ++matched;
+					else if (F[j] == s1)
+						fb[j] = eb1[1], ++matched;
+				}
+				if (matched == 2)
+					emit(a.edges[2 * ee], a.edges[2 * ee + 1], af, side ? eb2x : eb1[0], fb[0], fb[1], ee);
+			}
+		}
+	}
+}
+
+void ef_from_proximity(Context* ctx, const EfFromProximityArgs& a)
+{
+	UT_REQUIRE(a.recs && a.rec_ids && a.rec_count && a.x && a.edges && a.faces && a.vert_adj_off && a.vert_adj_edges
+			&& a.edge_adj_faces && a.count,
+		"ef_from_proximity: null input");
+	UT_REQUIRE(a.max_pairs == 0 || (a.pairs && a.pair_ids), "ef_from_proximity: null output");
+	if (a.rec_cap == 0)
+		return;
+	const uint32_t blocks = std::min<uint32_t>(div_up(a.rec_cap, 256), 2048);
+	k_ef_from_proximity<<<blocks, 256, 0, ctx->stream>>>(a);
 	UT_LAUNCH_CHECK();
 }
 

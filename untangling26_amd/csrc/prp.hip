@@ -1753,14 +1753,16 @@ void PrpResolver::build_contour_tables(const ContourBuilder& ct)
 	total_pairs = ct.num_total;
 	nc = ct.num_contours;
 	const uint32_t		  N = total_pairs;
-	std::vector<uint32_t> words(12 * (size_t)N), cidx(N), midx(N), adjn(N), adj(12 * (size_t)N);
+	std::vector<uint32_t> words(12 * (size_t)N), cidx(N), midx(N), adjn(N), adj(ct.num_adj), adj_off(N + 1);
 	if (N)
 	{
 		UT_HIP_CHECK(hipMemcpyAsync(words.data(), ct.pairs.get(), 48 * (size_t)N, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(cidx.data(), ct.contour_index.get(), 4 * (size_t)N, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(midx.data(), ct.mesh_index.get(), 4 * (size_t)N, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(adjn.data(), ct.adj_n.get(), 4 * (size_t)N, hipMemcpyDeviceToHost, s));
-		UT_HIP_CHECK(hipMemcpyAsync(adj.data(), ct.adj.get(), 48 * (size_t)N, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(adj_off.data(), ct.adj_off.get(), 4 * (size_t)(N + 1), hipMemcpyDeviceToHost, s));
+		if (ct.num_adj)
+			UT_HIP_CHECK(hipMemcpyAsync(adj.data(), ct.adj.get(), 4 * (size_t)ct.num_adj, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipStreamSynchronize(s));
 	}
 	h_pair_words = words;
@@ -1768,6 +1770,7 @@ void PrpResolver::build_contour_tables(const ContourBuilder& ct)
 	h_mesh_of_pair = midx;
 	h_adj = adj;
 	h_adjn = adjn;
+	h_adj_off = adj_off;
 	// pairs of each contour, ascending pair index
 	h_cpair_off.assign(nc + 1, 0);
 	for (uint32_t p = 0; p < N; ++p)
@@ -1840,7 +1843,7 @@ void PrpResolver::directions(uint32_t c0, uint32_t n)
 			for (uint32_t j = 0; j < h_adjn[p]; ++j)
 			{
 				float qq[3];
-				pos(h_adj[12 * (size_t)p + j], qq);
+				pos(h_adj[h_adj_off[p] + j], qq);
 				const float dx = qq[0] - pp[0], dy = qq[1] - pp[1], dz = qq[2] - pp[2];
 				w += std::sqrt(dx * dx + dy * dy + dz * dz);
 			}

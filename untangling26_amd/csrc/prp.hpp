@@ -182,7 +182,7 @@ struct PrpResolver
 	// host-side contour data of the last prepare
 	uint32_t						nc = 0, batch0 = 0, batch_n = 0, total_pairs = 0;
 	std::vector<uint32_t>			h_cpair_off, h_cpairs, h_pair_words, h_b_off, h_b_verts, h_diam;
-	std::vector<uint32_t>			h_contour_of_pair, h_mesh_of_pair, h_adj, h_adjn;
+	std::vector<uint32_t>			h_contour_of_pair, h_mesh_of_pair, h_adj, h_adjn, h_adj_off;
 	std::vector<std::vector<float>> h_dirs;
 	std::vector<float>				h_va;
 	double							global_c[3] = { 0, 0, 0 };

@@ -111,6 +111,8 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 	if (d.face_adj_edges && d.edge_adj_faces)
 	{
 		contours.init(ctx, d.nv, d.ne, d.nf, d.edges, d.faces, d.face_adj_edges, d.edge_adj_faces);
+		if (!p.ignore_near_zero_dist_pairs)
+			build_vert_adj_edges();
 		if (d.rest_vert_area && d.rest_edge_area && d.rest_face_area)
 		{
 			if (!d.rest_x)
@@ -128,6 +130,34 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 	UT_HIP_CHECK(hipHostMalloc((void**)&h_counts, 32, hipHostMallocDefault));
 	for (auto& e : ev)
 		UT_HIP_CHECK(hipEventCreate(&e));
+}
+
+void NewtonSolver::build_vert_adj_edges()
+{
+	std::vector<uint32_t> e(2 * (size_t)d.ne), off(d.nv + 1, 0), col(2 * (size_t)d.ne);
+	if (d.ne)
+	{
+		UT_HIP_CHECK(hipMemcpyAsync(e.data(), d.edges, 8 * (size_t)d.ne, hipMemcpyDeviceToHost, ctx->stream));
+		UT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+	}
+	for (uint32_t v : e)
+	{
+		UT_REQUIRE(v < d.nv, "solver: edge vertex out of range");
+		off[v + 1]++;
+	}
+	for (uint32_t v = 0; v < d.nv; ++v)
+		off[v + 1] += off[v];
+	std::vector<uint32_t> fill(off.begin(), off.end() - 1);
+	for (uint32_t k = 0; k < d.ne; ++k) // ascending edge ids per vertex
+		for (int j = 0; j < 2; ++j)
+			col[fill[e[2 * k + j]]++] = k;
+	vert_adj_off.reserve(d.nv + 1);
+	vert_adj_edges.reserve(col.size());
+	UT_HIP_CHECK(hipMemcpyAsync(vert_adj_off.get(), off.data(), 4 * off.size(), hipMemcpyHostToDevice, ctx->stream));
+	if (!col.empty())
+		UT_HIP_CHECK(
+			hipMemcpyAsync(vert_adj_edges.get(), col.data(), 4 * col.size(), hipMemcpyHostToDevice, ctx->stream));
+	UT_HIP_CHECK(hipStreamSynchronize(ctx->stream)); // the host vectors go out of scope
 }
 
 NewtonSolver::~NewtonSolver()
@@ -211,6 +241,26 @@ void NewtonSolver::detect(StepReport& rep)
 			e.pair_ids = ef_ids.get();
 			e.max_pairs = cap_ef;
 			ef_detect(ctx, face_tree, e, d.ne, d.offset);
+			if (!p.ignore_near_zero_dist_pairs && contours.ctx)
+			{
+				// fn_add_ef_from_proximity_pairs over the VF + EE proximity list
+				EfFromProximityArgs z;
+				z.recs = reinterpret_cast<const uint32_t*>(contact_pairs.get());
+				z.rec_ids = contact_ids.get();
+				z.rec_count = counts.get() + 3;
+				z.rec_cap = cap_contacts;
+				z.x = x.get();
+				z.edges = d.edges;
+				z.faces = d.faces;
+				z.vert_adj_off = vert_adj_off.get();
+				z.vert_adj_edges = vert_adj_edges.get();
+				z.edge_adj_faces = d.edge_adj_faces;
+				z.count = counts.get() + 2;
+				z.pairs = reinterpret_cast<uint32_t*>(ef_pairs.get());
+				z.pair_ids = ef_ids.get();
+				z.max_pairs = cap_ef;
+				ef_from_proximity(ctx, z);
+			}
 		}
 		UT_HIP_CHECK(hipMemcpyAsync(h_counts, counts.get(), 16, hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipStreamSynchronize(s));
@@ -260,7 +310,7 @@ StepReport NewtonSolver::physics_step_GPU()
 		regions.clear();
 		if (p.use_self_collision && p.use_untangling && contours.ctx)
 		{
-			contours.build(rep.ef_pairs, ef_pairs.get(), ef_ids.get());
+			contours.build(rep.ef_pairs, ef_pairs.get(), ef_ids.get(), !p.ignore_near_zero_dist_pairs);
 			rep.ef_total = contours.num_total;
 			rep.contours = contours.num_contours;
 			if (contours.num_total > 0 && prp.ctx)

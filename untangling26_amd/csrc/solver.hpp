@@ -64,6 +64,9 @@ struct SolverParams
 	// filter_invalid_pairs (intersection_resolver.cpp:2594-2682) on the proximity and CCD
 	// pairs; switched off (init refuses it) until its kernels are verified on the device
 	int		  region_filter = 0;
+	// scene_params.h:55; 0 (the CLI's setting) turns zero-distance proximity pairs into EF
+	// pairs (intersection_resolver.cpp:404-421) and filters degenerate EF pairs (:841-897)
+	int		  ignore_near_zero_dist_pairs = 1;
 	PcgParams pcg;
 	PrpParams prp;
 };
@@ -109,6 +112,7 @@ struct NewtonSolver
 	DevBuf<EfPairRec>	ef_pairs;
 	DevBuf<uint32_t>	ef_ids;
 	DevBuf<uint32_t>	counts; // [0] VF, [1] EE, [2] EF, [3] total contacts
+	DevBuf<uint32_t>	vert_adj_off, vert_adj_edges; // edges around each vertex (ascending)
 	uint32_t*			h_counts = nullptr;
 	hipEvent_t			ev[8] = {};
 
@@ -118,6 +122,7 @@ struct NewtonSolver
 
 private:
 	void detect(StepReport& rep);
+	void build_vert_adj_edges(); // sa_vert_adj_edges_csr (init_mesh_data.cpp), once at set-up
 	void ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef, uint32_t keep = 0);
 };
 

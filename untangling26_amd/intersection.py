@@ -33,6 +33,17 @@ def ef_detect_fused(ctx: Context, face_tree: LBVH, x, edges, faces, d_hat, thick
                                  ptr(d_hat), ptr(thickness), ptr(count), ptr(pairs), ptr(pair_ids), cap))
 
 
+def add_ef_from_proximity_pairs(ctx: Context, recs, rec_ids, rec_count, x, edges, faces, vert_adj_off,
+                                vert_adj_edges, edge_adj_faces, count, pairs, pair_ids):
+    """fn_add_ef_from_proximity_pairs (intersection_resolver.cpp:199-363): appends
+    EF pairs made from the zero-distance VF / EE proximity pairs (ut_ef_from_proximity)."""
+    rec_cap = rec_ids.numel() // 2
+    cap = pair_ids.numel() // 2
+    ctx.check(lib().ut_ef_from_proximity(ctx.handle, ptr(recs), ptr(rec_ids), ptr(rec_count), rec_cap, ptr(x),
+                                         ptr(edges), ptr(faces), ptr(vert_adj_off), ptr(vert_adj_edges),
+                                         ptr(edge_adj_faces), ptr(count), ptr(pairs), ptr(pair_ids), cap))
+
+
 def decode_ef_pairs(pairs: torch.Tensor, n: int):
     """Splits EfPair words into (indices uint32[n,5], baries float[n,3])."""
     w = pairs[: n * EF_PAIR_WORDS].view(n, EF_PAIR_WORDS)

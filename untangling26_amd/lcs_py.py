@@ -57,6 +57,9 @@ class SceneParams:
     # refuses it (status -1) until its kernels are verified on the device; without it the CCD
     # line search blocks the repair motion through the contours.
     use_region_filter: bool = False
+    # scene_params.h:55 (the CLI sets it False, cli.py:107): False adds EF pairs from
+    # zero-distance proximity pairs and drops isolated degenerate EF pairs
+    ignore_near_zero_dist_pairs: bool = True
 
 
 @dataclass
@@ -130,6 +133,7 @@ class NewtonSolver:
         p.prp = PrpParams(c.untangling_process_contours_count, c.prp_directions, c.PRP_cpu_contour_batch_size,
                           c.untangling_response_depth, c.stiffness_untangling, c.PRP_intrinsic_tau)
         p.region_filter = int(c.use_region_filter)
+        p.ignore_near_zero_dist_pairs = int(c.ignore_near_zero_dist_pairs)
         return p
 
     def init_solver(self):

@@ -1,8 +1,9 @@
 """Static mesh adjacency used by the per-step path (host side, computed once).
 
 Restates the parts of Initializer/init_mesh_data.cpp:419-500 the hot path
-reads: face_adj_edges (the edge between face[j] and face[(j+1)%3]) and
-edge_adj_faces (the faces around each edge, ascending, 0xFFFFFFFF if absent).
+reads: face_adj_edges (the edge between face[j] and face[(j+1)%3]),
+edge_adj_faces (the faces around each edge, ascending, 0xFFFFFFFF if absent)
+and the edges around each vertex (sa_vert_adj_edges_csr, ascending).
 Meshes with an edge shared by more than two faces are rejected.
 """
 from __future__ import annotations
@@ -60,3 +61,14 @@ def rest_areas(x_rest: np.ndarray, edges: np.ndarray, faces: np.ndarray, edge_ad
         ea[m] += fa[eaf[m, k]] / 3.0
     return (np.maximum(va, min_area).astype(np.float32), np.maximum(ea, min_area).astype(np.float32),
             fa.astype(np.float32))
+
+
+def vert_adj_edges(num_verts: int, edges: np.ndarray):
+    """Edges around each vertex as CSR: (offsets uint32[V+1], edge ids ascending per vertex)."""
+    e = np.asarray(edges, np.int64).reshape(-1, 2)
+    v = e.ravel()
+    eid = np.repeat(np.arange(len(e), dtype=np.int64), 2)
+    order = np.lexsort((eid, v))
+    off = np.zeros(num_verts + 1, np.uint32)
+    off[1:] = np.cumsum(np.bincount(v, minlength=num_verts))
+    return off, eid[order].astype(np.uint32)
