@@ -118,9 +118,10 @@ def test_ef_contours_match_oracle(ctx, name):
         _check_build(cb, N, nc, in_ids, in_pairs, m, fae, eaf, filter_degenerate=not ignore)
 
 
-def _rows(ids, pairs):
-    """(eid, fid, payload) rows in lexicographic order, for multiset comparison."""
-    r = np.concatenate([np.asarray(ids, np.uint32).reshape(-1, 2), np.asarray(pairs, np.uint32).reshape(-1, 12)], 1)
+def _rows(ids, pairs, words=12):
+    """(id pair, record words) rows in lexicographic order, for multiset comparison."""
+    r = np.concatenate([np.asarray(ids, np.uint32).reshape(-1, 2), np.asarray(pairs, np.uint32).reshape(-1, words)],
+                       1)
     return r[np.lexsort(r.T[::-1])] if len(r) else r
 
 
@@ -189,7 +190,7 @@ def test_zero_distance_contours_match_oracle(ctx, mixed):
     narrow.ee_dcd_query_repulsion(x, rc, grec, gid, et)
     nr = int(rc.item())
     assert nvf_gpu == nvf and nr == len(recs)
-    g, w = _rows(u32(gid[: 2 * nr]), u32(grec[: 16 * nr])), _rows(ids, recs)
+    g, w = _rows(u32(gid[: 2 * nr]), u32(grec[: 16 * nr]), 16), _rows(ids, recs, 16)
     exact = [c for c in range(18) if c not in (10, 11)]  # record words 8, 9 (k1, k2) carry log/pow rounding
     assert np.array_equal(g[:, exact], w[:, exact])
     assert np.allclose(g[:, 10:12].view(np.float32), w[:, 10:12].view(np.float32), rtol=1e-5, atol=0)
