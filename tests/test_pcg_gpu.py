@@ -149,3 +149,46 @@ def test_pcg_regularization(ctx):
                               precision=FP64, max_iters=2000, rel_tol=1e-10, regularization=lam)
     assert convo and rep["converged"]
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+@pytest.mark.parametrize("energy", [0, 1])
+def test_pcg_with_contacts_matches_oracle(ctx, energy):
+    """Newton system of two parallel sheets with 3.5k contacts inside d_hat: the
+    assembled blocks plus the matrix-free contact off-diagonal blocks.  The
+    iteration applies the contact blocks through per-vertex contact lists (no
+    atomics).  The FP64 iterate after a fixed 60 iterations equals the oracle's
+    within 1e-9 (longer runs drift apart by summation order on this stiffer
+    system: both converge, in iteration counts a few apart); the converged
+    FP64 and mixed solutions satisfy the system, checked with the atomic form
+    of the contact product."""
+    from autograd_cases import ip_case
+
+    m, st, x_tilde, recs = ip_case(energy=energy)
+    g, v, _ = oracle.assemble(st, m.x, x_tilde, 3600.0, recs)
+    assert len(recs) > 3000
+    solver = ConjugateGradientSolver(ctx, st.pattern)
+    pairs = torch.from_numpy(recs.view(np.int32).ravel()).cuda()
+    cnt = torch.tensor([len(recs)], dtype=torch.int32, device="cuda")
+    solver.set_contacts(pairs, cnt, len(recs) + 100)
+    B = torch.from_numpy(-g).cuda()
+    V64 = torch.from_numpy(np.ascontiguousarray(v)).cuda().reshape(-1)
+    xo, ito, _, _, _ = oracle.pcg_sell(st.pattern, v, -g, recs, max_iters=60, rel_tol=1e-300)
+    x = torch.zeros_like(B)
+    rep = solver.device_solve(V64, B, x, precision=FP64, max_iters=60, rel_tol=1e-300, check_every=4)
+    assert rep["iterations"] == ito == 60
+    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+    _, ito, _, convo, _ = oracle.pcg_sell(st.pattern, v, -g, recs, max_iters=3000, rel_tol=1e-10)
+    assert convo
+    x.zero_()
+    rep = solver.device_solve(V64, B, x, precision=FP64, max_iters=3000, rel_tol=1e-10, check_every=4)
+    assert rep["converged"] and rep["relative_residual"] <= 1e-10, rep
+    assert abs(rep["iterations"] - ito) <= 0.05 * ito, (rep["iterations"], ito)
+    y = torch.zeros_like(B)
+    solver.spmv(V64, x, y)
+    assert (B - y).norm().item() <= 2e-10 * B.norm().item()
+    xm = torch.zeros_like(B)
+    rm = solver.device_solve(V64.float(), B, xm, precision=MIXED, max_iters=3000, rel_tol=1e-8, vals64=V64)
+    assert rm["converged"] and rm["relative_residual"] <= 1e-8, rm
+    solver.spmv(V64, xm, y)
+    assert (B - y).norm().item() <= 2e-8 * B.norm().item()
+    solver.set_contacts(None)

@@ -8,6 +8,7 @@
 // synchronisations per iteration (precond_cg.cpp:810-816, 837-841).
 #include "ut_common.hpp"
 #include "pcg.hpp"
+#include "radix_sort.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -168,6 +169,75 @@ __global__ __launch_bounds__(kPcgThreads) void k_contact_offdiag(const ContactRe
 		block_partial(acc, part);
 }
 
+// Same product as k_contact_offdiag, scattered into the per-vertex entry lists:
+// contrib[pos[4c + a]] = k2 w_a (S - w_a s_a) n for every corner a of contact c.
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_contact_contrib(const ContactRec* __restrict__ pairs, uint32_t nc,
+	const T* __restrict__ p, const uint32_t* __restrict__ pos, T* __restrict__ contrib,
+	const uint32_t* __restrict__ flags)
+{
+	if (group_stopped(flags))
+		return;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nc)
+		return;
+	const ContactRec c = pairs[i];
+	double			 w[4], sv[4];
+	pair_weights(c, w);
+	const double nx = c.normal[0], ny = c.normal[1], nz = c.normal[2], k2 = c.k2;
+	double		 S = 0.0;
+	for (int a = 0; a < 4; ++a)
+	{
+		const uint32_t v = c.idx[a] & 0x7FFFFFFFu;
+		sv[a] = nx * (double)p[3 * v] + ny * (double)p[3 * v + 1] + nz * (double)p[3 * v + 2];
+		S += w[a] * sv[a];
+	}
+	for (int a = 0; a < 4; ++a)
+	{
+		const double   cf = k2 * w[a] * (S - w[a] * sv[a]);
+		const uint32_t o = 3 * pos[4 * i + a];
+		contrib[o] = (T)(cf * nx);
+		contrib[o + 1] = (T)(cf * ny);
+		contrib[o + 2] = (T)(cf * nz);
+	}
+}
+
+__global__ void k_ct_entry_keys(const ContactRec* __restrict__ pairs, uint32_t nc, uint32_t* __restrict__ keys,
+	uint32_t* __restrict__ vals)
+{
+	const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= 4 * nc)
+		return;
+	keys[e] = pairs[e >> 2].idx[e & 3] & 0x7FFFFFFFu;
+	vals[e] = e;
+}
+
+__global__ void k_ct_entry_pos(uint32_t m, const uint32_t* __restrict__ sorted_vals, uint32_t* __restrict__ pos)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m)
+		pos[sorted_vals[i]] = i;
+}
+
+// off[v] = first sorted entry with vertex >= v, for v = 0 .. n.
+__global__ void k_ct_entry_off(uint32_t n, uint32_t m, const uint32_t* __restrict__ sorted_keys,
+	uint32_t* __restrict__ off)
+{
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v > n)
+		return;
+	uint32_t lo = 0, hi = m;
+	while (lo < hi)
+	{
+		const uint32_t mid = (lo + hi) >> 1;
+		if (sorted_keys[mid] < v)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	off[v] = lo;
+}
+
 // ---- preconditioner -----------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_precond(uint32_t n, const T* __restrict__ vals, uint32_t nslots,
@@ -292,7 +362,8 @@ __global__ void k_pcg_check0(uint32_t nb, PcgParts parts, uint32_t* flags)
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint32_t* __restrict__ slice_off,
 	const uint32_t* __restrict__ col, const T* __restrict__ vals, uint32_t nslots, const T* __restrict__ p,
-	T* __restrict__ q, double lambda, const T* __restrict__ scale, PcgParts parts, const uint32_t* __restrict__ flags)
+	T* __restrict__ q, double lambda, const T* __restrict__ scale, PcgParts parts, const uint32_t* __restrict__ flags,
+	const uint32_t* __restrict__ ct_off, const T* __restrict__ contrib)
 {
 	if (group_stopped(flags))
 		return;
@@ -309,6 +380,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint
 			a.y += ls * pr.y;
 			a.z += ls * pr.z;
 		}
+		if (ct_off) // contact off-diagonal blocks, this vertex's entries in contact order
+			for (uint32_t e = ct_off[row], end = ct_off[row + 1]; e < end; ++e)
+			{
+				a.x += contrib[3 * e];
+				a.y += contrib[3 * e + 1];
+				a.z += contrib[3 * e + 2];
+			}
 		stv(q, row, a);
 		pq = dotd(pr, a);
 	}
@@ -507,6 +585,46 @@ PcgSolver::~PcgSolver()
 		(void)hipHostFree(h_norm);
 }
 
+void PcgSolver::prepare_contacts()
+{
+	ct_entries = 0;
+	if (!contacts || !contact_count || !contact_cap)
+		return;
+	hipStream_t st = ctx->stream;
+	uint32_t	nc = 0;
+	UT_HIP_CHECK(hipMemcpyAsync(&nc, contact_count, 4, hipMemcpyDeviceToHost, st));
+	UT_HIP_CHECK(hipStreamSynchronize(st));
+	nc = std::min(nc, contact_cap);
+	if (nc == 0)
+		return;
+	const uint32_t m = 4 * nc;
+	for (int k = 0; k < 2; ++k)
+	{
+		ct_keys[k].reserve(m);
+		ct_vals[k].reserve(m);
+	}
+	ct_hist.reserve(radix_sort_hist_size(m));
+	ct_pos.reserve(m);
+	ct_off.reserve((size_t)n + 1);
+	ct_contrib.reserve(3 * (size_t)m);
+	k_ct_entry_keys<<<div_up(m, 256), 256, 0, st>>>(contacts, nc, ct_keys[0].get(), ct_vals[0].get());
+	RadixSortBuffers rs;
+	for (int k = 0; k < 2; ++k)
+	{
+		rs.keys[k] = ct_keys[k].get();
+		rs.vals[k] = ct_vals[k].get();
+	}
+	rs.hist = ct_hist.get();
+	int bits = 8; // keys are vertex ids < n
+	while (bits < 32 && ((n - 1) >> bits) != 0u)
+		bits += 8;
+	radix_sort_pairs(rs, m, bits, st);
+	k_ct_entry_pos<<<div_up(m, 256), 256, 0, st>>>(m, rs.vals[rs.result], ct_pos.get());
+	k_ct_entry_off<<<div_up(n + 1, 256), 256, 0, st>>>(n, m, rs.keys[rs.result], ct_off.get());
+	UT_LAUNCH_CHECK();
+	ct_entries = m;
+}
+
 namespace
 {
 template <typename T>
@@ -551,13 +669,15 @@ uint32_t run_iterations(PcgSolver& s, const Vecs<T>& v, const T* vals, PcgParts 
 	while (it < budget)
 	{
 		const uint32_t chunk = std::min(check_every, budget - it);
+		const uint32_t nce = s.ct_entries / 4;
+		T*			   contrib = reinterpret_cast<T*>(s.ct_contrib.get());
 		for (uint32_t k = 0; k < chunk; ++k, ++it)
 		{
-			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
-				s.n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, lambda, v.scale, parts, s.flags.get());
-			if (s.contacts)
-				k_contact_offdiag<T><<<kContactBlocks, kPcgThreads, 0, st>>>(
-					s.contacts, s.contact_count, s.contact_cap, v.p, v.q, 1.0, parts.pqc, s.flags.get());
+			if (nce)
+				k_contact_contrib<T><<<div_up(nce, kPcgThreads), kPcgThreads, 0, st>>>(
+					s.contacts, nce, v.p, s.ct_pos.get(), contrib, s.flags.get());
+			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(s.n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, lambda,
+				v.scale, parts, s.flags.get(), nce ? s.ct_off.get() : nullptr, contrib);
 			k_pcg_update<T><<<nb, kPcgThreads, 0, st>>>(
 				it, s.n, nb, v.minv, v.x, v.r, v.z, v.p, v.q, parts, s.flags.get());
 			k_pcg_dir<T><<<nb, kPcgThreads, 0, st>>>(it, s.n, nb, tol2, v.z, v.p, parts, s.flags.get());
@@ -582,8 +702,9 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 	PcgParts	   parts{ pb, pb + 8 * nb + 8, { pb + nb, pb + 2 * nb }, pb + 3 * nb, pb + 4 * nb };
 	double*		   res_part = pb + 5 * nb;
 	double*		   d_norm = pb + 6 * nb;
-	if (!s.contacts)
-		UT_HIP_CHECK(hipMemsetAsync(parts.pqc, 0, sizeof(double) * kContactBlocks, st));
+	// the contact part of p^T A p is in the SpMV's own partials
+	UT_HIP_CHECK(hipMemsetAsync(parts.pqc, 0, sizeof(double) * kContactBlocks, st));
+	s.prepare_contacts();
 	PcgReport	   rep;
 	const bool	   mixed = prm.precision == kPcgMixed;
 
@@ -752,11 +873,11 @@ float profile_spmv_typed(PcgSolver& s, const T* vals, uint32_t reps)
 	UT_HIP_CHECK(hipEventCreate(&a));
 	UT_HIP_CHECK(hipEventCreate(&b));
 	k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
-		n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get());
+		n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get(), nullptr, (const T*)nullptr);
 	UT_HIP_CHECK(hipEventRecord(a, st));
 	for (uint32_t i = 0; i < reps; ++i)
 		k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
-			n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get());
+			n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get(), nullptr, (const T*)nullptr);
 	UT_HIP_CHECK(hipEventRecord(b, st));
 	UT_HIP_CHECK(hipEventSynchronize(b));
 	float ms = 0.0f;

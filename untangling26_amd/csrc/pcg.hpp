@@ -73,6 +73,17 @@ struct PcgSolver
 	const uint32_t*	  contact_count = nullptr;
 	uint32_t		  contact_cap = 0;
 
+	// Per-vertex contact lists of the current solve: entry e = 4 c + a is corner a
+	// of contact c; entries sorted by vertex (stable, so contact order within a
+	// vertex), ct_off[v] .. ct_off[v+1] the entries of vertex v, ct_pos[e] the
+	// sorted position of entry e.  The iteration writes each entry's force
+	// w_a k2 (S - w_a s_a) n into ct_contrib[ct_pos[e]] and the SpMV sums a
+	// vertex's entries in order: no atomics, fixed summation order.
+	uint32_t		 ct_entries = 0;
+	DevBuf<uint32_t> ct_keys[2], ct_vals[2], ct_hist, ct_off, ct_pos;
+	DevBuf<double>	 ct_contrib; // [3 * entries], FP32 or FP64 by the solve's precision
+	void			 prepare_contacts();
+
 	DevBuf<uint8_t> work; // vectors (sized for FP64)
 	DevBuf<double>	partials;
 	DevBuf<uint32_t> flags; // [0] done, [1] iterations, [2] non-finite

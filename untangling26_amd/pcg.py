@@ -61,5 +61,12 @@ class ConjugateGradientSolver:
         self.last_solve_report = rep
         return rep
 
+    def set_contacts(self, pairs=None, count=None, cap: int = 0):
+        """Contact records (int32 words, 16 per record) whose off-diagonal blocks
+        w_a w_b k2 n n^T every product with A applies; None clears them."""
+        self._contacts = (pairs, count)  # the library keeps the pointers
+        self.ctx.check(lib().ut_pcg_set_contacts(self.handle, ptr(pairs), ptr(count), int(cap) if pairs is not None
+                                                 else 0))
+
     def spmv(self, vals, x, y):
         self.ctx.check(lib().ut_bsr_spmv(self.handle, ptr(vals), int(vals.dtype == torch.float32), ptr(x), ptr(y)))
