@@ -567,8 +567,8 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 		bool			   active = r < total;
 		unsigned long long hit_acc = 0, cand_acc = 0; // lane 0: hits and candidate leaf tests of the wave
 		box3			   sw = box_empty(), cb = box_empty();
-		f3			   pn = mk3(0.0f, 0.0f, 0.0f);
-		float		   pd = 0.0f;
+		f3			   pn = mk3(0.0f, 0.0f, 0.0f), wn = mk3(0.0f, 0.0f, 0.0f);
+		float		   pd = 0.0f, wlo = -1e30f, whi = 1e30f;
 		if (active)
 		{
 			const uint32_t slot = slot_of(off, a.nslots, r);
@@ -596,6 +596,19 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 			{
 				pn = pn * (1.0f / pl);
 				pd = dot3(pn, p0);
+				// in-plane axis across the sweep: w . dir = 0, so the swept parallelogram
+				// p0 + t e1 + s dir covers only [min, max] of w . p0, w . p1 along it
+				wn = cross3(pn, dir);
+				const float wl = sqrtf(dot3(wn, wn));
+				if (wl >= 1e-10f)
+				{
+					wn = wn * (1.0f / wl);
+					const float w0 = dot3(wn, p0), w1 = dot3(wn, p1);
+					wlo = fminf(w0, w1);
+					whi = fmaxf(w0, w1);
+				}
+				else
+					wn = mk3(0.0f, 0.0f, 0.0f);
 			}
 			else
 				pn = mk3(0.0f, 0.0f, 0.0f); // no plane prune at the leaf either
@@ -603,8 +616,11 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 		// Node culling.  A node can hold a kept target edge only if its box meets (1) the swept box,
 		// (2) the box of the slot's candidate target edges (every kept edge is a candidate and lies in
 		// it) and (3) the slab |n.x - pd| <= 1e-6 that the leaf's plane prune requires of one point of
-		// the edge.  (2) and (3) hold for the leaf box of any kept edge and so for its ancestors;
-		// kEeNodeMargin covers the FP32 rounding of the node and leaf dot products (~1e-6 here).
+		// the edge.  (4) A kept edge crosses that plane at X = p0 + t e1 + s dir with t in [0, 1],
+		// so w . X lies in [min, max] of w . p0, w . p1 for the in-plane axis w across the sweep:
+		// the node box must meet that slab as well.  (2)-(4) hold for the leaf box of any kept
+		// edge and so for its ancestors; kEeNodeMargin covers the FP32 rounding of the node and
+		// leaf dot products and of the intersection parameters (~1e-6 here).
 		constexpr float kEeNodeMargin = 1e-4f;
 		auto			test = [&](const box3& b)
 		{
@@ -616,7 +632,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 			const float dc = dot3(pn, c) - pd;
 			const float rr = fabsf(pn.x) * h.x + fabsf(pn.y) * h.y + fabsf(pn.z) * h.z;
 			const bool	in_slab = (dc - rr <= 1e-6f + kEeNodeMargin) & (dc + rr >= -1e-6f - kEeNodeMargin);
-			return in_sw & in_cb & in_slab;
+			const float wc = dot3(wn, c);
+			const float wr = fabsf(wn.x) * h.x + fabsf(wn.y) * h.y + fabsf(wn.z) * h.z;
+			const bool	in_beam = (wc + wr >= wlo - kEeNodeMargin) & (wc - wr <= whi + kEeNodeMargin);
+			return in_sw & in_cb & in_slab & in_beam;
 		};
 		auto narrow = [&](bool valid, uint32_t rr, uint32_t g)
 		{
