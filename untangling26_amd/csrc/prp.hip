@@ -861,28 +861,25 @@ __device__ __forceinline__ uint32_t bucket_first(const HitArgs& a, uint32_t key)
 	return kNone;
 }
 
-// Index of the hit (slot, type, source prim) -> dst d, or kNone.  Hits are
-// sorted by (key, dst) and a (source, dst) pair occurs at most once per slot:
-// the bucket is found through the table (or a search of the slot's range), then
-// d by galloping from the bucket start, so large buckets cost O(log size).
-__device__ __forceinline__ uint32_t find_hit(const HitArgs& a, uint32_t slot, uint32_t type, uint32_t prim, uint32_t d)
+// First sorted index of the bucket (slot, type, source prim), or kNone: through
+// the table, or a search of the slot's range.
+__device__ __forceinline__ uint32_t bucket_start(const HitArgs& a, uint32_t slot, uint32_t key)
 {
-	const uint32_t key = hit_key(slot, type, prim, a.prim_bits);
-	const uint32_t hi = a.range[2 * slot + 1];
-	uint32_t	   lo;
 	if (a.table)
-	{
-		lo = bucket_first(a, key);
-		if (lo == kNone)
-			return kNone;
-	}
-	else
-	{
-		const uint32_t l0 = a.range[2 * slot];
-		lo = l0 + lower_bound_u32(a.keys + l0, hi - l0, key);
-		if (lo >= hi || a.keys[lo] != key)
-			return kNone;
-	}
+		return bucket_first(a, key);
+	const uint32_t l0 = a.range[2 * slot], hi = a.range[2 * slot + 1];
+	const uint32_t lo = l0 + lower_bound_u32(a.keys + l0, hi - l0, key);
+	return (lo < hi && a.keys[lo] == key) ? lo : kNone;
+}
+
+// Index of the hit of bucket `key` (starting at or before `lo`) with dst d, or
+// kNone.  Hits are sorted by (key, dst) and a (source, dst) pair occurs at most
+// once per slot: d is found by galloping from lo, so large buckets cost
+// O(log size).  *next gets the first index with dst >= d, where a search for a
+// larger d of the same bucket may start.
+__device__ __forceinline__ uint32_t gallop_dst(const HitArgs& a, uint32_t key, uint32_t lo, uint32_t hi, uint32_t d,
+	uint32_t* next)
+{
 	auto	 less = [&](uint32_t m) { return a.keys[m] == key && a.dst[m] < d; };
 	uint32_t top = hi;
 	for (uint32_t step = 1;; step <<= 1)
@@ -903,6 +900,7 @@ __device__ __forceinline__ uint32_t find_hit(const HitArgs& a, uint32_t slot, ui
 		else
 			top = mid;
 	}
+	*next = lo;
 	return (lo < hi && a.keys[lo] == key && a.dst[lo] == d) ? lo : kNone;
 }
 
@@ -919,12 +917,26 @@ __global__ void k_prp_hit_union(HitArgs a)
 	if (i >= a.n)
 		return;
 	const PrpHit   L = a.h[i];
-	const uint32_t s = L.slot;
-	auto		   link = [&](uint32_t type, uint32_t prim, uint32_t d)
+	const uint32_t s = L.slot, hi = a.range[2 * s + 1];
+	// the hits (type, prim) -> d for the n targets d[0] < d[1] < ... : one bucket
+	// lookup, then galloping searches that resume where the previous one ended
+	auto link = [&](uint32_t type, uint32_t prim, const uint32_t* d, int n)
 	{
-		const uint32_t j = find_hit(a, s, type, prim, d);
-		if (j != kNone && j != i)
-			uf_union(a.parent, i, j);
+		const uint32_t key = hit_key(s, type, prim, a.prim_bits);
+		uint32_t	   lo = bucket_start(a, s, key);
+		if (lo == kNone)
+			return;
+		for (int q = 0; q < n; ++q)
+		{
+			const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
+			if (j != kNone && j != i)
+				uf_union(a.parent, i, j);
+		}
+	};
+	auto sort3 = [](uint32_t x, uint32_t y, uint32_t z, uint32_t* o)
+	{
+		const uint32_t lo = min(x, min(y, z)), hi3 = max(x, max(y, z));
+		o[0] = lo, o[1] = x + y + z - lo - hi3, o[2] = hi3; // the three values are distinct
 	};
 	if (L.type == 0) // VF(v, f)
 	{
@@ -932,34 +944,39 @@ __global__ void k_prp_hit_union(HitArgs a)
 		// VF(w, f), w adjacent to v (w > v: the pair is symmetric)
 		for (uint32_t k = a.vv_off[v]; k < a.vv_off[v + 1]; ++k)
 			if (a.vv[k] > v)
-				link(0, a.vv[k], f);
+				link(0, a.vv[k], &f, 1);
 		// FV(f', u), f' a face of v, u a vertex of f
+		uint32_t fv[3], fe[3];
+		sort3(a.faces[3 * f], a.faces[3 * f + 1], a.faces[3 * f + 2], fv);
 		for (uint32_t k = a.vf_off[v]; k < a.vf_off[v + 1]; ++k)
-			for (int q = 0; q < 3; ++q)
-				link(2, a.vf[k], a.faces[3 * f + q]);
+			link(2, a.vf[k], fv, 3);
 		// EE(e, e'), e an edge of v, e' an edge of f
+		sort3(a.fae[3 * f], a.fae[3 * f + 1], a.fae[3 * f + 2], fe);
 		for (uint32_t k = a.ve_off[v]; k < a.ve_off[v + 1]; ++k)
-			for (int q = 0; q < 3; ++q)
-				link(1, a.ve[k], a.fae[3 * f + q]);
+			link(1, a.ve[k], fe, 3);
 	}
 	else if (L.type == 2) // FV(f, v)
 	{
 		const uint32_t f = L.src, v = L.dst;
 		// (VF(u, f'), u a vertex of f, f' a face of v: found from the VF side)
-		// FV(f, u), u adjacent to v (u > v)
-		for (uint32_t q = a.vv_off[v]; q < a.vv_off[v + 1]; ++q)
-			if (a.vv[q] > v)
-				link(2, f, a.vv[q]);
-		// EE(e, e'), e an edge of f, e' an edge of v
+		// FV(f, u), u adjacent to v (u > v): the neighbours of v are listed ascending
+		const uint32_t q0 = a.vv_off[v], q1 = a.vv_off[v + 1];
+		uint32_t	   q = q0;
+		while (q < q1 && a.vv[q] <= v)
+			++q;
+		if (q < q1)
+			link(2, f, a.vv + q, (int)(q1 - q));
+		// EE(e, e'), e an edge of f, e' an edge of v (listed ascending)
 		for (int k = 0; k < 3; ++k)
-			for (uint32_t q = a.ve_off[v]; q < a.ve_off[v + 1]; ++q)
-				link(1, a.fae[3 * f + k], a.ve[q]);
+			link(1, a.fae[3 * f + k], a.ve + a.ve_off[v], (int)(a.ve_off[v + 1] - a.ve_off[v]));
 	}
 	else // EE(e1, e2)
 	{
 		const uint32_t e1 = L.src, e2 = L.dst;
 		// (VF and FV partners are found from their side)
 		// EE(e1, e'), e' another edge of a face of e2 (e' > e2)
+		uint32_t t[4];
+		int		 nt = 0;
 		for (int q = 0; q < 2; ++q)
 		{
 			const uint32_t g = a.eaf[2 * e2 + q];
@@ -969,9 +986,17 @@ __global__ void k_prp_hit_union(HitArgs a)
 			{
 				const uint32_t ae = a.fae[3 * g + j];
 				if (ae > e2)
-					link(1, e1, ae);
+					t[nt++] = ae;
 			}
 		}
+		for (int x = 1; x < nt; ++x) // ascending (at most 4)
+			for (int y = x; y > 0 && t[y - 1] > t[y]; --y)
+			{
+				const uint32_t tmp = t[y - 1];
+				t[y - 1] = t[y];
+				t[y] = tmp;
+			}
+		link(1, e1, t, nt);
 		// EE(e', e2), e' another edge of a face of e1 (e' > e1)
 		for (int k = 0; k < 2; ++k)
 		{
@@ -982,7 +1007,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 			{
 				const uint32_t ae = a.fae[3 * f + j];
 				if (ae > e1)
-					link(1, ae, e2);
+					link(1, ae, &e2, 1);
 			}
 		}
 	}
