@@ -1311,9 +1311,19 @@ struct EvalArgs
 
 __device__ __forceinline__ float inv_mass_of(const EvalArgs& a, uint32_t v) { return 1.0f / fmaxf(a.mass[v], 1e-8f); }
 
-__global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
+// Partial sums of one slot over one of kEvalSplit interleaved shares of its
+// hits (so a slot with many hits is spread over several workgroups).
+struct EvalPart
 {
-	const uint32_t j = blockIdx.x;
+	double	 area, vol, cost;
+	float	 depth;
+	uint32_t cnt, hs, hd, pad;
+};
+constexpr uint32_t kEvalSplit = 16;
+
+__global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a, EvalPart* __restrict__ part)
+{
+	const uint32_t j = blockIdx.x, b = blockIdx.y;
 	if (j >= a.nslots)
 		return;
 	const uint32_t lo = a.range[2 * j], hi = a.range[2 * j + 1];
@@ -1322,12 +1332,12 @@ __global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
 	const f3	   dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
 	const uint32_t* H0 = a.hop_all + (size_t)(2 * sl.contour) * a.V;
 	const uint32_t* H1 = a.hop_all + (size_t)(2 * sl.contour + 1) * a.V;
-	// per-hit terms in FP32, sums in FP64 rounded once (independent of the hit order)
+	// per-hit terms in FP32, sums in FP64 in a fixed order, rounded once
 	double		   area = 0.0, vol = 0.0, cost = 0.0;
 	float		   depth = 0.0f;
 	uint32_t	   cnt = 0, hs = 0, hd = 0;
 	if (root != kNone)
-		for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x)
+		for (uint32_t i = lo + b * blockDim.x + threadIdx.x; i < hi; i += kEvalSplit * blockDim.x)
 		{
 			const PrpHit h = a.h[i];
 			if (h.root != root)
@@ -1390,26 +1400,47 @@ __global__ __launch_bounds__(256) void k_prp_eval(EvalArgs a)
 	__syncthreads();
 	if (threadIdx.x == 0)
 	{
-		PrpSummary s{};
-		double	   sa = 0.0, sv = 0.0, sc = 0.0;
+		EvalPart p{};
 		for (uint32_t k = 0; k < blockDim.x / kWave; ++k)
 		{
-			sa += sf[0][k], sv += sf[1][k], sc += sf[2][k];
-			s.depth = fmaxf(s.depth, sdepth[k]);
-			s.hit_count += su[0][k];
-			s.max_hop_src = max(s.max_hop_src, su[1][k]);
-			s.max_hop_dst = max(s.max_hop_dst, su[2][k]);
+			p.area += sf[0][k], p.vol += sf[1][k], p.cost += sf[2][k];
+			p.depth = fmaxf(p.depth, sdepth[k]);
+			p.cnt += su[0][k];
+			p.hs = max(p.hs, su[1][k]);
+			p.hd = max(p.hd, su[2][k]);
 		}
-		s.area = (float)sa, s.volume = (float)sv, s.disp_cost = (float)sc;
-		s.raw_hits = hi - lo;
-		s.ef_count = a.cpair_off[a.c0 + sl.contour + 1] - a.cpair_off[a.c0 + sl.contour];
-		s.covered = root != kNone ? a.cov[root] : 0u;
-		s.objective = s.area * s.depth * s.depth;
-		s.root = root;
-		const float cv = s.ef_count ? (float)s.covered / (float)s.ef_count : 0.0f;
-		s.usable = s.hit_count > 0 && s.covered > 0 && isfinite(s.objective) && isfinite(cv) && cv > 0.0f;
-		a.out[j] = s;
+		part[(size_t)j * kEvalSplit + b] = p;
 	}
+}
+
+__global__ void k_prp_eval_final(EvalArgs a, const EvalPart* __restrict__ part)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= a.nslots)
+		return;
+	const uint32_t lo = a.range[2 * j], hi = a.range[2 * j + 1];
+	const uint32_t root = a.slot_root[j];
+	const PrpSlot  sl = a.slots[j];
+	PrpSummary	   s{};
+	double		   sa = 0.0, sv = 0.0, sc = 0.0;
+	for (uint32_t b = 0; b < kEvalSplit; ++b)
+	{
+		const EvalPart& p = part[(size_t)j * kEvalSplit + b];
+		sa += p.area, sv += p.vol, sc += p.cost;
+		s.depth = fmaxf(s.depth, p.depth);
+		s.hit_count += p.cnt;
+		s.max_hop_src = max(s.max_hop_src, p.hs);
+		s.max_hop_dst = max(s.max_hop_dst, p.hd);
+	}
+	s.area = (float)sa, s.volume = (float)sv, s.disp_cost = (float)sc;
+	s.raw_hits = hi - lo;
+	s.ef_count = a.cpair_off[a.c0 + sl.contour + 1] - a.cpair_off[a.c0 + sl.contour];
+	s.covered = root != kNone ? a.cov[root] : 0u;
+	s.objective = s.area * s.depth * s.depth;
+	s.root = root;
+	const float cv = s.ef_count ? (float)s.covered / (float)s.ef_count : 0.0f;
+	s.usable = s.hit_count > 0 && s.covered > 0 && isfinite(s.objective) && isfinite(cv) && cv > 0.0f;
+	a.out[j] = s;
 }
 
 // two-pass LSD order (slot, type, src, dst): first by dst, then stably by the bucket key
@@ -2206,7 +2237,10 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	EvalArgs ea_{ ns, V, batch0, hits_sorted.get(), slot_range.get(), slot_root.get(),
 		comp_info.get() + 3 * (size_t)hit_cap, slots.get(), cpair_off.get(), edges, faces, hop_b, va, ea, fa, mass,
 		fn, en, vn, summary.get() };
-	k_prp_eval<<<ns, 256, 0, s>>>(ea_);
+	eval_part.reserve(sizeof(EvalPart) * (size_t)ns * kEvalSplit);
+	EvalPart* ep = reinterpret_cast<EvalPart*>(eval_part.get());
+	k_prp_eval<<<dim3(ns, kEvalSplit), 256, 0, s>>>(ea_, ep);
+	k_prp_eval_final<<<div_up(ns, 64), 64, 0, s>>>(ea_, ep);
 	UT_LAUNCH_CHECK();
 	UT_HIP_CHECK(hipMemcpyAsync(out.data(), summary.get(), sizeof(PrpSummary) * ns, hipMemcpyDeviceToHost, s));
 	UT_HIP_CHECK(hipStreamSynchronize(s));
