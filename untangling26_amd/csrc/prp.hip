@@ -274,8 +274,10 @@ __global__ void k_prp_cand_lists(uint32_t V, uint32_t E, uint32_t nslots2, const
 }
 
 // Per slot: projections of the candidate target edges (role 1) on the slot
-// direction (intersection_resolver2.h:572-595) and the request offsets of the
-// three ray kinds.  One workgroup per slot; offsets by a single-workgroup scan.
+// direction (intersection_resolver2.h:572-595), and the boxes of the candidate
+// edges of both roles: tbox[12 j .. +5] role 1, [12 j + 6 .. +11] role 0.  Every
+// candidate face has two candidate edges (at its owned vertex) covering its
+// corners, so these boxes also hold the candidate faces of the role.
 __global__ void k_prp_slot_prep(uint32_t V, uint32_t E, uint32_t nslots, PrpSlot* __restrict__ slots,
 	const uint32_t* __restrict__ cand, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ x,
 	const uint32_t* __restrict__ edges, float* __restrict__ tbox)
@@ -284,47 +286,56 @@ __global__ void k_prp_slot_prep(uint32_t V, uint32_t E, uint32_t nslots, PrpSlot
 	if (j >= nslots)
 		return;
 	PrpSlot		   sl = slots[j];
-	const uint32_t s1 = 2 * sl.contour + 1;
-	const uint32_t n = cand_cnt[2 * s1 + 1];
-	const uint32_t* C = cand + (size_t)s1 * (V + E) + V;
-	const f3		dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
-	// [0] projection min, [1] projection max, [2..4] box min, [5..7] box max of the endpoints
-	float r[8] = { FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX };
-	for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+	const f3	   dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+	// [0] projection min, [1] projection max, [2..4] box min, [5..7] box max of the
+	// role-1 endpoints; [8..10] box min, [11..13] box max of the role-0 endpoints
+	float r[14] = { FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX, FLT_MAX, FLT_MAX,
+		FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX };
+	for (uint32_t role = 0; role < 2; ++role)
 	{
-		const uint32_t e = C[i];
-		const f3	   q0 = ld3(x, edges[2 * e]), q1 = ld3(x, edges[2 * e + 1]);
-		const float	   p0 = dot3(q0, dir), p1 = dot3(q1, dir);
-		r[0] = fminf(r[0], fminf(p0, p1));
-		r[1] = fmaxf(r[1], fmaxf(p0, p1));
-		r[2] = fminf(r[2], fminf(q0.x, q1.x)), r[3] = fminf(r[3], fminf(q0.y, q1.y));
-		r[4] = fminf(r[4], fminf(q0.z, q1.z));
-		r[5] = fmaxf(r[5], fmaxf(q0.x, q1.x)), r[6] = fmaxf(r[6], fmaxf(q0.y, q1.y));
-		r[7] = fmaxf(r[7], fmaxf(q0.z, q1.z));
+		const uint32_t	s_ = 2 * sl.contour + role;
+		const uint32_t	n = cand_cnt[2 * s_ + 1];
+		const uint32_t* C = cand + (size_t)s_ * (V + E) + V;
+		float*			b = role ? r + 2 : r + 8;
+		for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+		{
+			const uint32_t e = C[i];
+			const f3	   q0 = ld3(x, edges[2 * e]), q1 = ld3(x, edges[2 * e + 1]);
+			if (role)
+			{
+				const float p0 = dot3(q0, dir), p1 = dot3(q1, dir);
+				r[0] = fminf(r[0], fminf(p0, p1));
+				r[1] = fmaxf(r[1], fmaxf(p0, p1));
+			}
+			b[0] = fminf(b[0], fminf(q0.x, q1.x)), b[1] = fminf(b[1], fminf(q0.y, q1.y));
+			b[2] = fminf(b[2], fminf(q0.z, q1.z));
+			b[3] = fmaxf(b[3], fmaxf(q0.x, q1.x)), b[4] = fmaxf(b[4], fmaxf(q0.y, q1.y));
+			b[5] = fmaxf(b[5], fmaxf(q0.z, q1.z));
+		}
 	}
-	__shared__ float sr[8][16];
+	auto is_min = [](int k) { return k == 0 || (k >= 2 && k <= 4) || (k >= 8 && k <= 10); };
+	__shared__ float sr[14][16];
 	const uint32_t	 w = threadIdx.x / kWave;
-	for (int k = 0; k < 8; ++k)
+	for (int k = 0; k < 14; ++k)
 	{
-		const bool mn = k == 0 || (k >= 2 && k <= 4);
-		const float v = mn ? wave_min(r[k]) : wave_max(r[k]);
+		const float v = is_min(k) ? wave_min(r[k]) : wave_max(r[k]);
 		if (__lane_id() == 0)
 			sr[k][w] = v;
 	}
 	__syncthreads();
-	if (threadIdx.x < 8)
+	if (threadIdx.x < 14)
 	{
-		const int  k = threadIdx.x;
-		const bool mn = k == 0 || (k >= 2 && k <= 4);
-		float	   a = sr[k][0];
+		const int k = threadIdx.x;
+		float	  a = sr[k][0];
 		for (uint32_t q = 1; q < blockDim.x / kWave; ++q)
-			a = mn ? fminf(a, sr[k][q]) : fmaxf(a, sr[k][q]);
+			a = is_min(k) ? fminf(a, sr[k][q]) : fmaxf(a, sr[k][q]);
+		const uint32_t n1 = cand_cnt[2 * (2 * sl.contour + 1) + 1];
 		if (k == 0)
-			slots[j].ee_pmin = n ? a : 0.0f;
+			slots[j].ee_pmin = n1 ? a : 0.0f;
 		else if (k == 1)
-			slots[j].ee_pmax = n ? a : 0.0f;
+			slots[j].ee_pmax = n1 ? a : 0.0f;
 		else
-			tbox[6 * (size_t)j + (k - 2)] = a; // empty candidate set: an inverted box that meets no node
+			tbox[12 * (size_t)j + (k - 2)] = a; // empty candidate set: an inverted box that meets no node
 	}
 }
 
@@ -417,7 +428,7 @@ struct RayArgs
 	uint32_t*		status;
 	uint32_t*		spill;
 	const BvhNode*	nodes;
-	const float*	tbox; // [6 * nslots] candidate target edge boxes
+	const float*	tbox; // [12 * nslots] candidate edge boxes of role 1, then role 0
 	unsigned long long* tstat; // [3 kinds][4]: node visits, leaf tests, hits, candidate leaf tests
 };
 
@@ -477,6 +488,7 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 		const bool		   active = r < total;
 		unsigned long long hit_acc = 0; // lane 0: hits of this wave's traversal
 		RayReq			   q{};
+		box3			   cb = box_empty();
 		if (active)
 		{
 			q.slot = slot_of(off, a.nslots, r);
@@ -489,6 +501,9 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 			q.tmax = 1.0f;
 			req_slot[threadIdx.x] = q.slot;
 			req_src[threadIdx.x] = q.src;
+			// the candidate target faces (role 1 for VF rays, role 0 for FV rays) lie in this box
+			const float* tb = a.tbox + 12 * (size_t)q.slot + (Kind == 0 ? 0 : 6);
+			cb = box3{ mk3(tb[0], tb[1], tb[2]), mk3(tb[3], tb[4], tb[5]) };
 		}
 		// the wave's own lanes wrote the entries its leaf tests read
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -507,7 +522,9 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 			tmx = fminf(tmx, fmaxf(t1y, t2y));
 			tmin = fmaxf(tmin, fminf(t1z, t2z));
 			tmx = fminf(tmx, fmaxf(t1z, t2z));
-			return tmx + 1e-5f >= fmaxf(tmin, 0.0f) && tmin <= q.tmax + 1e-5f;
+			const bool in_cb = (cb.lo.x <= b.hi.x) & (cb.hi.x >= b.lo.x) & (cb.lo.y <= b.hi.y)
+				& (cb.hi.y >= b.lo.y) & (cb.lo.z <= b.hi.z) & (cb.hi.z >= b.lo.z);
+			return in_cb && tmx + 1e-5f >= fmaxf(tmin, 0.0f) && tmin <= q.tmax + 1e-5f;
 		};
 		auto narrow = [&](bool valid, uint32_t rr, uint32_t f)
 		{
@@ -587,7 +604,7 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 				active = false;
 			sw = box_add(box_add(box_add(box_of(p0), p1), p0 + dir * rmax), p1 + dir * rmax);
 			sw = box_inflate(sw, 1e-5f);
-			const float* tb = a.tbox + 6 * (size_t)slot;
+			const float* tb = a.tbox + 12 * (size_t)slot;
 			cb = box3{ mk3(tb[0], tb[1], tb[2]), mk3(tb[3], tb[4], tb[5]) };
 			// the leaf's plane prune (same normal): a kept target edge has a point within 1e-6 of the plane
 			pn = cross3(p1 - p0, dir);
@@ -2095,7 +2112,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		V, E, n2, kring.get(), hop_b, own.get(), edges, cand.get(), cand_cnt.get());
 	k_prp_any_faces<<<dim3(div_up(F, 256), n2), 256, 0, s>>>(
 		F, n2, kring.get(), V, hop_b, own.get(), faces, target_cnt.get());
-	slot_tbox.reserve(6 * (size_t)ns);
+	slot_tbox.reserve(12 * (size_t)ns);
 	k_prp_slot_prep<<<ns, 256, 0, s>>>(V, E, ns, slots.get(), cand.get(), cand_cnt.get(), x_dev, edges,
 		slot_tbox.get());
 	k_prp_request_offsets<<<1, 1024, 0, s>>>(ns, slots.get(), cand_cnt.get(), target_cnt.get(), req_off.get());
