@@ -141,7 +141,7 @@ __device__ __forceinline__ double vf_ccd(uint32_t vid, uint32_t fid, const Conta
 	v3<double> dx[4] = { load_v3<double>(a.x_end, vid) - x0[0], load_v3<double>(a.x_end, f0) - x0[1],
 		load_v3<double>(a.x_end, f1) - x0[2], load_v3<double>(a.x_end, f2) - x0[3] };
 	const double th = (double)a.offset[vid] + (double)a.offset[f0];
-	return toi_postprocess(accd_toi(false, x0, dx, th));
+	return toi_postprocess(accd_toi(false, x0, dx, th, a.toi));
 }
 
 __device__ __forceinline__ double ee_ccd(uint32_t left, uint32_t right, const ContactArgs& a)
@@ -155,7 +155,7 @@ __device__ __forceinline__ double ee_ccd(uint32_t left, uint32_t right, const Co
 	v3<double> dx[4] = { load_v3<double>(a.x_end, l0) - x0[0], load_v3<double>(a.x_end, l1) - x0[1],
 		load_v3<double>(a.x_end, r0) - x0[2], load_v3<double>(a.x_end, r1) - x0[3] };
 	const double th = (double)a.offset[l0] + (double)a.offset[r1]; // narrow_phase.cpp:656 uses right_edge[1]
-	return toi_postprocess(accd_toi(true, x0, dx, th));
+	return toi_postprocess(accd_toi(true, x0, dx, th, a.toi));
 }
 
 // Non-negative floats order like their bit patterns.

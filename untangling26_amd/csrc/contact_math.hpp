@@ -248,8 +248,15 @@ UT_HD T ccd_dist2(bool is_ee, const v3<T>* x)
 
 // ccd_helper (accd.hpp:48-110): conservative advancement.  Returns the
 // sentinel 0.001 when the iteration cap is hit, as the reference does.
+//
+// gmin (device only, may be null): the running minimum over the pairs of the
+// post-processed TOI (toi_postprocess).  The advanced toi only grows, and the
+// post-processed value of a pair whose toi has reached r is at least
+// max(r, 3e-3) (0.5 below 3e-3, 1 for the sentinel), so once that bound
+// reaches gmin the pair cannot lower the minimum and returns max_t, which the
+// post-processing keeps at 1.  The minimum is unchanged, only the work is cut.
 template <typename T>
-UT_HD T accd_toi(bool is_ee, v3<T>* x0, v3<T>* dx, T offset)
+UT_HD T accd_toi(bool is_ee, v3<T>* x0, v3<T>* dx, T offset, const float* gmin = nullptr)
 {
 	centerize(x0);
 	centerize(dx);
@@ -266,6 +273,14 @@ UT_HD T accd_toi(bool is_ee, v3<T>* x0, v3<T>* dx, T offset)
 	{
 		if (iter > kCcdMaxIter)
 			return (T)0.001;
+#if defined(__HIP_DEVICE_COMPILE__)
+		if (gmin && (iter & 15) == 15)
+		{
+			const float g = __hip_atomic_load(gmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if ((T)g <= (toi > (T)3e-3 ? toi : (T)3e-3))
+				return max_t;
+		}
+#endif
 		v3<T> xt[4];
 		for (int k = 0; k < 4; ++k)
 			xt[k] = x0[k] + toi * dx[k];
