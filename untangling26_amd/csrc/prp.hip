@@ -251,9 +251,11 @@ __device__ __forceinline__ bool cand_face(const uint32_t* H, const uint8_t* O, u
 }
 
 // cand lists: per slot s, vertices at [s * (V + E)], edges at [s * (V + E) + V]
+// Also the candidate edges as a bitmap, ecand[s * ewords + e / 64] bit e % 64
+// (grid rows of 64 lanes are aligned to 64 edges, one word per wave).
 __global__ void k_prp_cand_lists(uint32_t V, uint32_t E, uint32_t nslots2, const uint32_t* __restrict__ kring,
 	const uint32_t* __restrict__ hop_all, const uint8_t* __restrict__ own_all, const uint32_t* __restrict__ edges,
-	uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_cnt)
+	uint32_t* __restrict__ cand, uint32_t* __restrict__ cand_cnt, uint64_t* __restrict__ ecand, uint32_t ewords)
 {
 	const uint32_t s = blockIdx.y;
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -271,6 +273,9 @@ __global__ void k_prp_cand_lists(uint32_t V, uint32_t E, uint32_t nslots2, const
 	const uint32_t pe = wave_append(&cand_cnt[2 * s + 1], ein);
 	if (ein)
 		C[V + pe] = i;
+	const uint64_t em = __ballot(ein);
+	if (__lane_id() == 0 && (i >> 6) < ewords)
+		ecand[(size_t)s * ewords + (i >> 6)] = em;
 }
 
 // Per slot: projections of the candidate target edges (role 1) on the slot
@@ -393,9 +398,11 @@ __global__ void k_prp_request_offsets(uint32_t nslots, const PrpSlot* __restrict
 }
 
 // number of candidate faces of each (contour, role): needed only as "any"
+// Whether (contour, role) s has candidate faces, and the candidate faces as a
+// bitmap fcand[s * fwords + f / 64] bit f % 64.
 __global__ void k_prp_any_faces(uint32_t F, uint32_t nslots2, const uint32_t* __restrict__ kring, uint32_t V,
 	const uint32_t* __restrict__ hop_all, const uint8_t* __restrict__ own_all, const uint32_t* __restrict__ faces,
-	uint32_t* __restrict__ target_cnt)
+	uint32_t* __restrict__ target_cnt, uint64_t* __restrict__ fcand, uint32_t fwords)
 {
 	const uint32_t s = blockIdx.y;
 	const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -404,8 +411,14 @@ __global__ void k_prp_any_faces(uint32_t F, uint32_t nslots2, const uint32_t* __
 	const uint32_t* H = hop_all + (size_t)s * V;
 	const uint8_t*	O = own_all + (size_t)s * V;
 	const bool		in = f < F && cand_face(H, O, kring[s], faces[3 * f], faces[3 * f + 1], faces[3 * f + 2]);
-	if (__ballot(in) != 0ull && __lane_id() == 0)
-		atomicOr(&target_cnt[s], 1u);
+	const uint64_t	m = __ballot(in);
+	if (__lane_id() == 0)
+	{
+		if (m != 0ull)
+			atomicOr(&target_cnt[s], 1u);
+		if ((f >> 6) < fwords)
+			fcand[(size_t)s * fwords + (f >> 6)] = m;
+	}
 }
 
 
@@ -429,6 +442,9 @@ struct RayArgs
 	uint32_t*		spill;
 	const BvhNode*	nodes;
 	const float*	tbox; // [12 * nslots] candidate edge boxes of role 1, then role 0
+	const uint64_t* ecand; // candidate edge bitmaps per (contour, role), ewords each
+	const uint64_t* fcand; // candidate face bitmaps per (contour, role), fwords each
+	uint32_t		ewords, fwords;
 	unsigned long long* tstat; // [3 kinds][4]: node visits, leaf tests, hits, candidate leaf tests
 };
 
@@ -537,10 +553,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 				const PrpSlot& sl = a.slots[slot];
 				const uint32_t s_dst = 2 * sl.contour + (Kind == 0 ? 1u : 0u);
 				const uint32_t v = req_src[rr - base];
-				const uint32_t t0 = a.faces[3 * f], t1 = a.faces[3 * f + 1], t2 = a.faces[3 * f + 2];
-				const uint32_t* H = a.hop_all + (size_t)s_dst * a.V;
-				const uint8_t*	O = a.own_all + (size_t)s_dst * a.V;
-				if (v != t0 && v != t1 && v != t2 && cand_face(H, O, a.kring[s_dst], t0, t1, t2))
+				const bool		is_cand = (a.fcand[(size_t)s_dst * a.fwords + (f >> 6)] >> (f & 63)) & 1ull;
+				const uint32_t t0 = is_cand ? a.faces[3 * f] : v, t1 = is_cand ? a.faces[3 * f + 1] : v,
+							   t2 = is_cand ? a.faces[3 * f + 2] : v;
+				if (is_cand && v != t0 && v != t1 && v != t2)
 				{
 					const f3 d = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
 					const f3 o = ld3(a.x, v);
@@ -664,11 +680,14 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 				const PrpSlot&	sl = a.slots[slot];
 				const uint32_t	s1 = 2 * sl.contour + 1;
 				const uint32_t	e = a.cand[(size_t)(2 * sl.contour) * (a.V + a.E) + a.V + (rr - off[slot])];
-				const uint32_t	a0 = a.edges[2 * e], a1 = a.edges[2 * e + 1];
-				const uint32_t	b0 = a.edges[2 * g], b1 = a.edges[2 * g + 1];
-				const uint32_t* H = a.hop_all + (size_t)s1 * a.V;
-				const uint8_t*	O = a.own_all + (size_t)s1 * a.V;
-				const bool cand = a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1 && cand_edge(H, O, a.kring[s1], b0, b1);
+				const bool		is_cand = (a.ecand[(size_t)s1 * a.ewords + (g >> 6)] >> (g & 63)) & 1ull;
+				uint32_t		a0 = 0, a1 = 0, b0 = 1, b1 = 1;
+				if (is_cand)
+				{
+					a0 = a.edges[2 * e], a1 = a.edges[2 * e + 1];
+					b0 = a.edges[2 * g], b1 = a.edges[2 * g + 1];
+				}
+				const bool cand = is_cand && a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1;
 				cand_acc += (unsigned long long)__popcll(__ballot(cand)); // lane 0 is valid whenever narrow runs
 				if (cand)
 				{
@@ -2108,10 +2127,13 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		V, batch0, n2, kring.get(), hop_b, geo_b, x_dev, b_off.get(), b_verts.get(), params.tau, own.get());
 	k_prp_fill_u32<<<div_up(2 * n2, 256), 256, 0, s>>>(2 * n2, cand_cnt.get(), 0u);
 	k_prp_fill_u32<<<div_up(n2, 256), 256, 0, s>>>(n2, target_cnt.get(), 0u);
+	const uint32_t ewords = div_up(E, 64), fwords = div_up(F, 64);
+	ecand.reserve((size_t)n2 * ewords);
+	fcand.reserve((size_t)n2 * fwords);
 	k_prp_cand_lists<<<dim3(div_up(std::max(V, E), 256), n2), 256, 0, s>>>(
-		V, E, n2, kring.get(), hop_b, own.get(), edges, cand.get(), cand_cnt.get());
+		V, E, n2, kring.get(), hop_b, own.get(), edges, cand.get(), cand_cnt.get(), ecand.get(), ewords);
 	k_prp_any_faces<<<dim3(div_up(F, 256), n2), 256, 0, s>>>(
-		F, n2, kring.get(), V, hop_b, own.get(), faces, target_cnt.get());
+		F, n2, kring.get(), V, hop_b, own.get(), faces, target_cnt.get(), fcand.get(), fwords);
 	slot_tbox.reserve(12 * (size_t)ns);
 	k_prp_slot_prep<<<ns, 256, 0, s>>>(V, E, ns, slots.get(), cand.get(), cand_cnt.get(), x_dev, edges,
 		slot_tbox.get());
@@ -2121,8 +2143,8 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	for (int attempt = 0; attempt < 8; ++attempt)
 	{
 		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop_b, own.get(), kring.get(), x_dev, edges,
-			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr, slot_tbox.get(),
-			tstat.get() };
+			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr, slot_tbox.get(), ecand.get(),
+			fcand.get(), ewords, fwords, tstat.get() };
 		k_prp_fill_u64<<<1, 64, 0, s>>>(12, reinterpret_cast<uint64_t*>(tstat.get()), 0ull);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(4, counters.get(), 0u);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get(), 1u);	  // [0] tree health

@@ -154,6 +154,7 @@ struct PrpResolver
 	uint32_t		 bucket_cap = 0, bucket_gen = 255;
 	DevBuf<uint64_t> slot_best;				 // [2 * slots]
 	DevBuf<PrpSummary> summary;
+	DevBuf<uint64_t>   ecand, fcand; // candidate edge / face bitmaps per (contour, role)
 	DevBuf<uint8_t>	   eval_part; // per (slot, share) partial sums of the evaluation
 	DevBuf<uint8_t>	 adjm;					 // contour adjacency [nc * nc]
 	DevBuf<float>	 contour_c;				 // response constraint per contour
