@@ -58,6 +58,23 @@ static f3	 scl(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 static float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 static f3	 crs(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 static float len(f3 a) { return sqrtf(dot(a, a)); }
+/* Beam condition of an EE sweep hit (same FP32 operations as the GPU): the
+ * target endpoints' w . q range must meet [min, max] of w . p0, w . p1 widened
+ * by 1e-4, w = normalize(pn x dir) the in-plane axis across the sweep.  A
+ * crossing inside the swept strip satisfies it; it removes only hits whose
+ * Cramer solve is ill-conditioned (target nearly in the sweep plane) and lands
+ * more than the margin off the strip. */
+static int ee_in_beam(f3 pn, f3 dir, f3 p0, f3 p1, f3 q0, f3 q1)
+{
+	f3			wn = crs(pn, dir);
+	const float wl = sqrtf(dot(wn, wn));
+	if (wl < 1e-10f)
+		return 1;
+	wn = scl(wn, 1.0f / wl);
+	const float w0 = dot(wn, p0), w1 = dot(wn, p1), u0 = dot(wn, q0), u1 = dot(wn, q1);
+	return fmaxf(u0, u1) >= fminf(w0, w1) - 1e-4f && fminf(u0, u1) <= fmaxf(w0, w1) + 1e-4f;
+}
+
 static f3	 safe_norm(f3 v)
 {
 	const float l2 = dot(v, v);
@@ -1045,6 +1062,8 @@ static uint32_t raycast(const or_prp_mesh* m, f3 dir, const uint8_t* cv0, const 
 						{
 							const float d0 = dot(pn, q0) - pd, d1 = dot(pn, q1) - pd;
 							if ((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f))
+								continue;
+							if (!ee_in_beam(pn, dir, p0, p1, q0, q1))
 								continue;
 						}
 						const f3	e2 = sub(q1, q0), r = sub(q0, p0), nn = crs(e1, e2);

@@ -588,6 +588,26 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 
 // EE (kind 1): source edge swept along dir by ray_max (h:597-613) against candidate edges of role 1
 // (plane test, |det| >= 1e-10, Cramer solve with s in [0, ray_max], t1, t2 in [0, 1], shared vertices skipped).
+// Beam condition of an EE sweep hit: with the sweep plane's unit normal pn and
+// w = normalize(pn x dir) the in-plane axis across the sweep, the target
+// endpoints' w . q range must meet [min, max] of w . p0, w . p1 widened by
+// kEeBeamMargin.  A crossing at X = p0 + t e1 + s dir (t in [0, 1]) with the
+// target segment has w . X in that range, so the condition only removes hits
+// whose FP32 Cramer solve is so ill-conditioned (target nearly in the sweep
+// plane) that the computed crossing lies more than the margin off the swept
+// strip.  The oracle applies the same test.
+constexpr float kEeBeamMargin = 1e-4f;
+__device__ __forceinline__ bool ee_in_beam(f3 pn, f3 dir, f3 p0, f3 p1, f3 q0, f3 q1)
+{
+	f3			wn = cross3(pn, dir);
+	const float wl = sqrtf(dot3(wn, wn));
+	if (wl < 1e-10f)
+		return true;
+	wn = wn * (1.0f / wl);
+	const float w0 = dot3(wn, p0), w1 = dot3(wn, p1), u0 = dot3(wn, q0), u1 = dot3(wn, q1);
+	return fmaxf(u0, u1) >= fminf(w0, w1) - kEeBeamMargin && fminf(u0, u1) <= fmaxf(w0, w1) + kEeBeamMargin;
+}
+
 __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 {
 	__shared__ TravSharedT<kRayStackLds> sh;
@@ -649,11 +669,11 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 		// Node culling.  A node can hold a kept target edge only if its box meets (1) the swept box,
 		// (2) the box of the slot's candidate target edges (every kept edge is a candidate and lies in
 		// it) and (3) the slab |n.x - pd| <= 1e-6 that the leaf's plane prune requires of one point of
-		// the edge.  (4) A kept edge crosses that plane at X = p0 + t e1 + s dir with t in [0, 1],
-		// so w . X lies in [min, max] of w . p0, w . p1 for the in-plane axis w across the sweep:
-		// the node box must meet that slab as well.  (2)-(4) hold for the leaf box of any kept
-		// edge and so for its ancestors; kEeNodeMargin covers the FP32 rounding of the node and
-		// leaf dot products and of the intersection parameters (~1e-6 here).
+		// the edge.  (4) The leaf keeps only an edge whose endpoints' w . q range meets
+		// [min, max] of w . p0, w . p1 widened by kEeBeamMargin, w the in-plane axis across the
+		// sweep (ee_in_beam; a crossing at X = p0 + t e1 + s dir with t in [0, 1] has w . X in
+		// that range).  (2)-(4) hold for the leaf box of any kept edge and so for its ancestors;
+		// kEeNodeMargin covers the FP32 rounding of the node and leaf dot products (~1e-6 here).
 		constexpr float kEeNodeMargin = 1e-4f;
 		auto			test = [&](const box3& b)
 		{
@@ -667,7 +687,8 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 			const bool	in_slab = (dc - rr <= 1e-6f + kEeNodeMargin) & (dc + rr >= -1e-6f - kEeNodeMargin);
 			const float wc = dot3(wn, c);
 			const float wr = fabsf(wn.x) * h.x + fabsf(wn.y) * h.y + fabsf(wn.z) * h.z;
-			const bool	in_beam = (wc + wr >= wlo - kEeNodeMargin) & (wc - wr <= whi + kEeNodeMargin);
+			const bool	in_beam =
+				(wc + wr >= wlo - kEeBeamMargin - kEeNodeMargin) & (wc - wr <= whi + kEeBeamMargin + kEeNodeMargin);
 			return in_sw & in_cb & in_slab & in_beam;
 		};
 		auto narrow = [&](bool valid, uint32_t rr, uint32_t g)
@@ -714,7 +735,8 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
 					if (use_plane)
 					{
 						const float d0 = dot3(pn, q0) - pd, d1 = dot3(pn, q1) - pd;
-						keep = !((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f));
+						keep = !((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f))
+							&& ee_in_beam(pn, dir, p0, p1, q0, q1);
 					}
 					if (keep)
 					{
