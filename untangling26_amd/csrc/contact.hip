@@ -14,6 +14,8 @@
 #include "contact_math.hpp"
 #include "lbvh_traverse.hpp"
 
+#include <algorithm>
+
 namespace ut
 {
 
@@ -224,6 +226,57 @@ __global__ __launch_bounds__(kTravThreads) void k_contact_fused(int kind, Contac
 	traverse_staged(sh, active, q, qb, !vf, nodes, status, spill, max_visits, op);
 }
 
+// CCD broad phase: the swept-box traversal lists the (query, leaf) pairs that
+// survive the region filter; the ACCD runs in its own kernel over that list, so
+// the traversal is not held to the FP64 ACCD's register budget (occupancy).
+struct CcdEmit
+{
+	const NarrowOp* op;
+	uint32_t*		count;
+	uint2*			list;
+	uint32_t		cap;
+	__device__ __forceinline__ void operator()(bool valid, uint32_t q, uint32_t leaf)
+	{
+		const bool	   keep = valid && !op->ccd_filtered(q, leaf);
+		const uint32_t s = wave_append(count, keep);
+		if (keep && s < cap)
+			list[s] = make_uint2(q, leaf);
+	}
+};
+
+constexpr int kCcdStackLds = 16;
+
+__global__ __launch_bounds__(kTravThreads) void k_ccd_broad(int kind, ContactArgs a, const BvhNode* __restrict__ nodes,
+	uint32_t* __restrict__ status, uint32_t* __restrict__ spill, uint32_t max_visits, uint32_t* __restrict__ count,
+	uint2* __restrict__ list, uint32_t cap)
+{
+	__shared__ TravSharedT<kCcdStackLds> sh;
+	const bool		vf = kind == 2;
+	const uint32_t	nq = vf ? a.num_verts : a.num_edges;
+	const uint32_t	q = blockIdx.x * kTravThreads + threadIdx.x;
+	const bool		active = q < nq;
+	const box3		qb = active ? query_box(vf ? kQueryVerts : kQueryEdges, q, a.x, a.x_end, a.edges, a.d_hat, a.offset)
+									: box_empty();
+	const NarrowOp	op{ &a, kind };
+	CcdEmit			emit{ &op, count, list, cap };
+	auto			test = [&](const box3& b) { return box_overlap(b, qb); };
+	traverse_staged_test(sh, active, q, q, test, !vf, nodes, status, spill, max_visits, emit);
+}
+
+// ACCD over the listed pairs; the running minimum TOI lets pairs stop early.
+__global__ __launch_bounds__(256) void k_ccd_pairs(int kind, ContactArgs a, const uint2* __restrict__ list,
+	const uint32_t* __restrict__ count, uint32_t cap)
+{
+	const uint32_t n = min(*count, cap);
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		const uint2	 p = list[i];
+		const double t = kind == 2 ? vf_ccd(p.x, p.y, a) : ee_ccd(p.x, p.y, a);
+		if (t != kLineSearchMaxT)
+			toi_min(a.toi, t);
+	}
+}
+
 __global__ __launch_bounds__(256) void k_contact_list(int kind, ContactArgs a, const uint2* __restrict__ broad,
 	const uint32_t* __restrict__ broad_count, uint32_t broad_cap)
 {
@@ -262,9 +315,37 @@ void contact_query(Context* ctx, int kind, const ContactArgs& a, Lbvh* tree, con
 		const uint32_t nq = vf ? a.num_verts : a.num_edges;
 		if (nq == 0)
 			return;
-		tree->spill.reserve((size_t)nq * (kStackTotal - kStackLds));
-		k_contact_fused<<<div_up(nq, kTravThreads), kTravThreads, 0, ctx->stream>>>(
-			kind, a, tree->nodes.get(), tree->status.get(), tree->spill.get(), tree->num_leaves + 1);
+		// per-query spill region: the CCD listing keeps fewer stack entries in LDS
+		tree->spill.reserve((size_t)nq * (kStackTotal - std::min(kStackLds, kCcdStackLds)));
+		if (kind <= 1)
+			k_contact_fused<<<div_up(nq, kTravThreads), kTravThreads, 0, ctx->stream>>>(
+				kind, a, tree->nodes.get(), tree->status.get(), tree->spill.get(), tree->num_leaves + 1);
+		else
+		{
+			// list the swept pairs (grow the list and list again on overflow), then the ACCD
+			tree->pair_count.reserve(1);
+			if (tree->pair_buf.cap < (1u << 20))
+				tree->pair_buf.reserve(1u << 20);
+			uint32_t n = 0;
+			for (int attempt = 0; attempt < 8; ++attempt)
+			{
+				const uint32_t cap = (uint32_t)tree->pair_buf.cap;
+				UT_HIP_CHECK(hipMemsetAsync(tree->pair_count.get(), 0, 4, ctx->stream));
+				k_ccd_broad<<<div_up(nq, kTravThreads), kTravThreads, 0, ctx->stream>>>(kind, a, tree->nodes.get(),
+					tree->status.get(), tree->spill.get(), tree->num_leaves + 1, tree->pair_count.get(),
+					tree->pair_buf.get(), cap);
+				UT_LAUNCH_CHECK();
+				UT_HIP_CHECK(hipMemcpyAsync(&n, tree->pair_count.get(), 4, hipMemcpyDeviceToHost, ctx->stream));
+				UT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+				if (n <= cap)
+					break;
+				tree->pair_buf.reserve((size_t)n + n / 4);
+			}
+			UT_REQUIRE(n <= tree->pair_buf.cap, "CCD: swept pair list could not be grown");
+			if (n)
+				k_ccd_pairs<<<std::min<uint32_t>(div_up(n, 256), 4096), 256, 0, ctx->stream>>>(
+					kind, a, tree->pair_buf.get(), tree->pair_count.get(), (uint32_t)tree->pair_buf.cap);
+		}
 	}
 	else
 	{

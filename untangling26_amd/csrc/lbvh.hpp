@@ -68,6 +68,8 @@ struct Lbvh
 	DevBuf<float>	 root_box;			// [6]
 	DevBuf<uint32_t> status;			// [2] : [0] health (1 ok), [1] truncated queries
 	DevBuf<uint32_t> spill;				// traversal stack entries beyond the LDS part
+	DevBuf<uint2>	 pair_buf;			// swept pairs of a CCD query (see contact_query)
+	DevBuf<uint32_t> pair_count;		// [1]
 
 	void init(Context* c, int tree_type, uint32_t n);
 	// lbvh.cpp:1003-1021 (reduce_{face,edge}_tree_aabb) + fn_compute_mortons

@@ -608,7 +608,7 @@ __device__ __forceinline__ bool ee_in_beam(f3 pn, f3 dir, f3 p0, f3 p1, f3 q0, f
 	return fmaxf(u0, u1) >= fminf(w0, w1) - kEeBeamMargin && fminf(u0, u1) <= fmaxf(w0, w1) + kEeBeamMargin;
 }
 
-__global__ __launch_bounds__(kTravThreads) void k_prp_ray_edge(RayArgs a)
+__global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 {
 	__shared__ TravSharedT<kRayStackLds> sh;
 	const uint32_t		  total = a.off[1 * (a.nslots + 1) + a.nslots];
