@@ -976,25 +976,42 @@ __global__ void k_prp_hit_union(HitArgs a)
 		return;
 	const PrpHit   L = a.h[i];
 	const uint32_t s = L.slot, hi = a.range[2 * s + 1];
-	// the hits (type, prim) -> d for the n targets d[0] < d[1] < ... : one bucket
-	// lookup, then galloping searches that resume where the previous one ended
-	auto link = [&](uint32_t type, uint32_t prim, const uint32_t* d, int n)
+	// the hits (type, prim) -> d for targets in ascending order: one bucket lookup,
+	// then galloping searches that resume where the previous one ended
+	auto link_list = [&](uint32_t type, uint32_t prim, const uint32_t* __restrict__ d, uint32_t n)
 	{
 		const uint32_t key = hit_key(s, type, prim, a.prim_bits);
 		uint32_t	   lo = bucket_start(a, s, key);
 		if (lo == kNone)
 			return;
-		for (int q = 0; q < n; ++q)
+		for (uint32_t q = 0; q < n; ++q)
 		{
 			const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
 			if (j != kNone && j != i)
 				uf_union(a.parent, i, j);
 		}
 	};
-	auto sort3 = [](uint32_t x, uint32_t y, uint32_t z, uint32_t* o)
+	// the same for up to four targets held in registers (kNone entries, last, are skipped)
+	auto link4 = [&](uint32_t type, uint32_t prim, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3)
 	{
-		const uint32_t lo = min(x, min(y, z)), hi3 = max(x, max(y, z));
-		o[0] = lo, o[1] = x + y + z - lo - hi3, o[2] = hi3; // the three values are distinct
+		const uint32_t key = hit_key(s, type, prim, a.prim_bits);
+		uint32_t	   lo = bucket_start(a, s, key);
+		if (lo == kNone)
+			return;
+		const uint32_t d[4] = { d0, d1, d2, d3 };
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			if (d[q] != kNone)
+			{
+				const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
+				if (j != kNone && j != i)
+					uf_union(a.parent, i, j);
+			}
+	};
+	auto cswap = [](uint32_t& x, uint32_t& y)
+	{
+		const uint32_t lo = min(x, y), hi2 = max(x, y);
+		x = lo, y = hi2;
 	};
 	if (L.type == 0) // VF(v, f)
 	{
@@ -1002,16 +1019,17 @@ __global__ void k_prp_hit_union(HitArgs a)
 		// VF(w, f), w adjacent to v (w > v: the pair is symmetric)
 		for (uint32_t k = a.vv_off[v]; k < a.vv_off[v + 1]; ++k)
 			if (a.vv[k] > v)
-				link(0, a.vv[k], &f, 1);
+				link4(0, a.vv[k], f, kNone, kNone, kNone);
 		// FV(f', u), f' a face of v, u a vertex of f
-		uint32_t fv[3], fe[3];
-		sort3(a.faces[3 * f], a.faces[3 * f + 1], a.faces[3 * f + 2], fv);
+		uint32_t u0 = a.faces[3 * f], u1 = a.faces[3 * f + 1], u2 = a.faces[3 * f + 2];
+		cswap(u0, u1), cswap(u1, u2), cswap(u0, u1);
 		for (uint32_t k = a.vf_off[v]; k < a.vf_off[v + 1]; ++k)
-			link(2, a.vf[k], fv, 3);
+			link4(2, a.vf[k], u0, u1, u2, kNone);
 		// EE(e, e'), e an edge of v, e' an edge of f
-		sort3(a.fae[3 * f], a.fae[3 * f + 1], a.fae[3 * f + 2], fe);
+		uint32_t g0 = a.fae[3 * f], g1 = a.fae[3 * f + 1], g2 = a.fae[3 * f + 2];
+		cswap(g0, g1), cswap(g1, g2), cswap(g0, g1);
 		for (uint32_t k = a.ve_off[v]; k < a.ve_off[v + 1]; ++k)
-			link(1, a.ve[k], fe, 3);
+			link4(1, a.ve[k], g0, g1, g2, kNone);
 	}
 	else if (L.type == 2) // FV(f, v)
 	{
@@ -1023,38 +1041,31 @@ __global__ void k_prp_hit_union(HitArgs a)
 		while (q < q1 && a.vv[q] <= v)
 			++q;
 		if (q < q1)
-			link(2, f, a.vv + q, (int)(q1 - q));
+			link_list(2, f, a.vv + q, q1 - q);
 		// EE(e, e'), e an edge of f, e' an edge of v (listed ascending)
 		for (int k = 0; k < 3; ++k)
-			link(1, a.fae[3 * f + k], a.ve + a.ve_off[v], (int)(a.ve_off[v + 1] - a.ve_off[v]));
+			link_list(1, a.fae[3 * f + k], a.ve + a.ve_off[v], a.ve_off[v + 1] - a.ve_off[v]);
 	}
 	else // EE(e1, e2)
 	{
 		const uint32_t e1 = L.src, e2 = L.dst;
 		// (VF and FV partners are found from their side)
-		// EE(e1, e'), e' another edge of a face of e2 (e' > e2)
-		uint32_t t[4];
-		int		 nt = 0;
-		for (int q = 0; q < 2; ++q)
+		// EE(e1, e'), e' another edge of a face of e2 (e' > e2), ascending
+		uint32_t t[4] = { kNone, kNone, kNone, kNone };
+#pragma unroll
+		for (int q2 = 0; q2 < 2; ++q2)
 		{
-			const uint32_t g = a.eaf[2 * e2 + q];
+			const uint32_t g = a.eaf[2 * e2 + q2];
 			if (g == kNone)
 				continue;
-			for (int j = 0; j < 3; ++j)
-			{
-				const uint32_t ae = a.fae[3 * g + j];
-				if (ae > e2)
-					t[nt++] = ae;
-			}
+			const uint32_t x0 = a.fae[3 * g], x1 = a.fae[3 * g + 1], x2 = a.fae[3 * g + 2];
+			// the two edges of g other than e2
+			const uint32_t o0 = x0 == e2 ? x1 : x0, o1 = (x0 == e2 || x1 == e2) ? x2 : x1;
+			t[2 * q2] = o0 > e2 ? o0 : kNone;
+			t[2 * q2 + 1] = o1 > e2 ? o1 : kNone;
 		}
-		for (int x = 1; x < nt; ++x) // ascending (at most 4)
-			for (int y = x; y > 0 && t[y - 1] > t[y]; --y)
-			{
-				const uint32_t tmp = t[y - 1];
-				t[y - 1] = t[y];
-				t[y] = tmp;
-			}
-		link(1, e1, t, nt);
+		cswap(t[0], t[1]), cswap(t[2], t[3]), cswap(t[0], t[2]), cswap(t[1], t[3]), cswap(t[1], t[2]);
+		link4(1, e1, t[0], t[1], t[2], t[3]);
 		// EE(e', e2), e' another edge of a face of e1 (e' > e1)
 		for (int k = 0; k < 2; ++k)
 		{
@@ -1065,7 +1076,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 			{
 				const uint32_t ae = a.fae[3 * f + j];
 				if (ae > e1)
-					link(1, ae, &e2, 1);
+					link4(1, ae, e2, kNone, kNone, kNone);
 			}
 		}
 	}
