@@ -54,11 +54,17 @@ __device__ __forceinline__ box3 query_box(int kind,
 	return qb;
 }
 
+// Wave totals of a traversal (wave-uniform after the call).
+struct TravCount
+{
+	unsigned long long visits = 0, tests = 0;
+};
+
 // Runs the traversal for query q (valid when active).  narrow(valid, q, leaf)
 // is called by all 64 lanes together.  test(box) decides whether a child box
 // is entered; spill_slot selects this lane's region of the global stack.
-// When stat is given, stat[0] += node visits and stat[1] += leaf tests of the
-// wave (one atomic each per wave).
+// When count is given, the wave's node visits and leaf tests are added to it
+// (registers; the caller flushes them once).
 template <int LdsDepth, typename NodeTest, typename Narrow>
 __device__ __forceinline__ void traverse_staged_test(TravSharedT<LdsDepth>& sh,
 	bool							 active,
@@ -71,7 +77,7 @@ __device__ __forceinline__ void traverse_staged_test(TravSharedT<LdsDepth>& sh,
 	uint32_t* __restrict__ spill,
 	uint32_t max_visits,
 	Narrow&	 narrow,
-	unsigned long long* stat = nullptr)
+	TravCount* count = nullptr)
 {
 	const uint32_t w = threadIdx.x / kWave;
 	const uint32_t lane = __lane_id();
@@ -166,16 +172,13 @@ __device__ __forceinline__ void traverse_staged_test(TravSharedT<LdsDepth>& sh,
 	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 	if (pending > 0)
 		run_batch(pending);
-	if (stat)
+	if (count)
 	{
 		uint32_t v = visits;
 		for (int o = 32; o > 0; o >>= 1)
 			v += __shfl_xor(v, o, kWave);
-		if (lane == 0)
-		{
-			atomicAdd(&stat[0], (unsigned long long)v);
-			atomicAdd(&stat[1], (unsigned long long)tests);
-		}
+		count->visits += v;
+		count->tests += tests;
 	}
 }
 

@@ -423,6 +423,8 @@ __global__ void k_prp_any_faces(uint32_t F, uint32_t nslots2, const uint32_t* __
 
 
 // ------------------------------------------------------------------ rays (cached_bvh.cpp:131-555)
+constexpr uint32_t kHitLists = 64;
+constexpr uint32_t kListStride = 16;
 struct RayArgs
 {
 	uint32_t		V, E, nslots;
@@ -435,9 +437,9 @@ struct RayArgs
 	const float*	x;
 	const uint32_t* edges;
 	const uint32_t* faces;
-	PrpHit*			hits;
-	uint32_t*		count;
-	uint32_t		cap;
+	PrpHit*			hits;		// kHitLists sub-lists of sub_cap entries
+	uint32_t*		list_count; // counter of sub-list k at [kListStride * k]
+	uint32_t		sub_cap;
 	uint32_t*		status;
 	uint32_t*		spill;
 	const BvhNode*	nodes;
@@ -463,11 +465,14 @@ __device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots
 }
 
 // all 64 lanes call this; `acc` (meaningful in lane 0) counts the wave's hits
+// Appends to the sub-list of the calling wave (one of kHitLists, by global wave
+// index): a single shared counter would serialise every wave's append.
 __device__ __forceinline__ void emit_hit(const RayArgs& a, bool ok, const PrpHit& h, unsigned long long& acc)
 {
-	const uint32_t i = wave_append(a.count, ok);
-	if (ok && i < a.cap)
-		a.hits[i] = h;
+	const uint32_t k = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kHitLists;
+	const uint32_t i = wave_append(&a.list_count[kListStride * k], ok);
+	if (ok && i < a.sub_cap)
+		a.hits[(size_t)k * a.sub_cap + i] = h;
 	acc += (unsigned long long)__popcll(__ballot(ok));
 }
 
@@ -498,11 +503,13 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 	const uint32_t		total = a.off[Kind * (a.nslots + 1) + a.nslots];
 	const uint32_t*		off = a.off + Kind * (a.nslots + 1);
 	const uint32_t		spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
+	// wave totals over the whole persistent loop, flushed once
+	TravCount		   tc;
+	unsigned long long hit_acc = 0; // lane 0: hits of this wave
 	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
 		const uint32_t	   r = base + threadIdx.x;
 		const bool		   active = r < total;
-		unsigned long long hit_acc = 0; // lane 0: hits of this wave's traversal
 		RayReq			   q{};
 		box3			   cb = box_empty();
 		if (active)
@@ -581,9 +588,11 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 			emit_hit(a, ok, h, hit_acc);
 		};
 		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow,
-			a.tstat + 4 * Kind);
-		flush_stat(&a.tstat[4 * Kind + 2], hit_acc);
+			&tc);
 	}
+	flush_stat(&a.tstat[4 * Kind], tc.visits);
+	flush_stat(&a.tstat[4 * Kind + 1], tc.tests);
+	flush_stat(&a.tstat[4 * Kind + 2], hit_acc);
 }
 
 // EE (kind 1): source edge swept along dir by ray_max (h:597-613) against candidate edges of role 1
@@ -614,11 +623,13 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 	const uint32_t		  total = a.off[1 * (a.nslots + 1) + a.nslots];
 	const uint32_t*		  off = a.off + (a.nslots + 1);
 	const uint32_t		  spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
+	// wave totals over the whole persistent loop, flushed once
+	TravCount		   tc;
+	unsigned long long hit_acc = 0, cand_acc = 0; // lane 0: hits and candidate leaf tests of the wave
 	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
 		const uint32_t	   r = base + threadIdx.x;
 		bool			   active = r < total;
-		unsigned long long hit_acc = 0, cand_acc = 0; // lane 0: hits and candidate leaf tests of the wave
 		box3			   sw = box_empty(), cb = box_empty();
 		f3			   pn = mk3(0.0f, 0.0f, 0.0f), wn = mk3(0.0f, 0.0f, 0.0f);
 		float		   pd = 0.0f, wlo = -1e30f, whi = 1e30f;
@@ -761,10 +772,12 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 			emit_hit(a, ok, h, hit_acc);
 		};
 		traverse_staged_test(sh, active, r, spill_slot, test, false, a.nodes, a.status, a.spill, 1u << 22, narrow,
-			a.tstat + 4);
-		flush_stat(&a.tstat[4 + 2], hit_acc);
-		flush_stat(&a.tstat[4 + 3], cand_acc);
+			&tc);
 	}
+	flush_stat(&a.tstat[4], tc.visits);
+	flush_stat(&a.tstat[4 + 1], tc.tests);
+	flush_stat(&a.tstat[4 + 2], hit_acc);
+	flush_stat(&a.tstat[4 + 3], cand_acc);
 }
 
 // ------------------------------------------------------------------ hit graph
@@ -1512,15 +1525,29 @@ __global__ void k_prp_eval_final(EvalArgs a, const EvalPart* __restrict__ part)
 	a.out[j] = s;
 }
 
-// two-pass LSD order (slot, type, src, dst): first by dst, then stably by the bucket key
-__global__ void k_prp_dst_keys(uint32_t n, const PrpHit* __restrict__ h, uint32_t* __restrict__ keys,
-	uint32_t* __restrict__ vals)
+// two-pass LSD order (slot, type, src, dst): first by dst, then stably by the
+// bucket key.  The raw hits sit in kHitLists sub-lists; sub-list k's entries
+// take positions off_k .. off_k + count_k - 1 of the sort, and the values are
+// the hits' indices in the sub-list storage.
+__global__ void k_prp_dst_keys(const uint32_t* __restrict__ list_count, uint32_t sub_cap,
+	const PrpHit* __restrict__ h, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
+	const uint32_t k = blockIdx.y;
+	__shared__ uint32_t s_off;
+	if (threadIdx.x == 0)
+	{
+		uint32_t o = 0;
+		for (uint32_t j = 0; j < k; ++j)
+			o += list_count[kListStride * j];
+		s_off = o;
+	}
+	__syncthreads();
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
+	if (i >= list_count[kListStride * k])
 		return;
-	keys[i] = h[i].dst;
-	vals[i] = i;
+	const uint32_t src = k * sub_cap + i;
+	keys[s_off + i] = h[src].dst;
+	vals[s_off + i] = src;
 }
 
 __global__ void k_prp_keys(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ order, uint32_t pb,
@@ -2121,6 +2148,7 @@ void PrpResolver::begin_batch(uint32_t first)
 
 void PrpResolver::ensure_hits(uint32_t cap)
 {
+	cap = div_up(cap, kHitLists) * kHitLists; // whole sub-lists
 	if (cap <= hit_cap)
 		return;
 	hit_cap = cap;
@@ -2172,14 +2200,17 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		slot_tbox.get());
 	k_prp_request_offsets<<<1, 1024, 0, s>>>(ns, slots.get(), cand_cnt.get(), target_cnt.get(), req_off.get());
 	UT_LAUNCH_CHECK();
-	uint32_t nh = 0;
+	uint32_t			  nh = 0, max_list = 0;
+	std::vector<uint32_t> lc(kHitLists * kListStride);
+	list_count.reserve(kHitLists * kListStride);
 	for (int attempt = 0; attempt < 8; ++attempt)
 	{
 		RayArgs ra{ V, E, ns, slots.get(), req_off.get(), cand.get(), hop_b, own.get(), kring.get(), x_dev, edges,
-			faces, hits.get(), counters.get(), hit_cap, status.get(), spill.get(), nullptr, slot_tbox.get(), ecand.get(),
+			faces, hits.get(), list_count.get(), hit_cap / kHitLists, status.get(), spill.get(), nullptr, slot_tbox.get(),
+			ecand.get(),
 			fcand.get(), ewords, fwords, tstat.get() };
 		k_prp_fill_u64<<<1, 64, 0, s>>>(12, reinterpret_cast<uint64_t*>(tstat.get()), 0ull);
-		k_prp_fill_u32<<<1, 64, 0, s>>>(4, counters.get(), 0u);
+		k_prp_fill_u32<<<div_up(kHitLists * kListStride, 256), 256, 0, s>>>(kHitLists * kListStride, list_count.get(), 0u);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get(), 1u);	  // [0] tree health
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get() + 1, 0u); // [1] stack overflows
 		ra.nodes = face_tree.nodes.get();
@@ -2193,7 +2224,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		uint32_t		   st[2], tot[3];
 		unsigned long long ts[12];
 		UT_HIP_CHECK(hipMemcpyAsync(ts, tstat.get(), sizeof(ts), hipMemcpyDeviceToHost, s));
-		UT_HIP_CHECK(hipMemcpyAsync(&nh, counters.get(), 4, hipMemcpyDeviceToHost, s));
+		UT_HIP_CHECK(hipMemcpyAsync(lc.data(), list_count.get(), 4 * lc.size(), hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(st, status.get(), 8, hipMemcpyDeviceToHost, s));
 		for (int t = 0; t < 3; ++t)
 			UT_HIP_CHECK(hipMemcpyAsync(&tot[t], req_off.get() + (size_t)t * (ns + 1) + ns, 4, hipMemcpyDeviceToHost, s));
@@ -2214,11 +2245,14 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		}
 		UT_REQUIRE(st[0] == 1u, "prp: ray traversal revisited a node (corrupt tree)");
 		UT_REQUIRE(st[1] == 0u, "prp: ray traversal stack overflow");
-		if (nh <= hit_cap)
+		nh = 0, max_list = 0;
+		for (uint32_t k = 0; k < kHitLists; ++k)
+			nh += lc[kListStride * k], max_list = std::max(max_list, lc[kListStride * k]);
+		if (max_list <= hit_cap / kHitLists)
 			break;
-		ensure_hits(std::max(nh + nh / 4, 2 * hit_cap));
+		ensure_hits(std::max(2 * hit_cap, (max_list + max_list / 4) * kHitLists));
 	}
-	UT_REQUIRE(nh <= hit_cap, "prp: hit buffer could not be grown");
+	UT_REQUIRE(max_list <= hit_cap / kHitLists, "prp: hit buffer could not be grown");
 	last_raw_hits = nh;
 	k_prp_fill_u32<<<div_up(2 * ns, 256), 256, 0, s>>>(2 * ns, slot_range.get(), 0u);
 	k_prp_fill_u32<<<div_up(ns, 256), 256, 0, s>>>(ns, slot_root.get(), kNone);
@@ -2227,7 +2261,8 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 	if (nh > 0)
 	{
 		// sort by (slot, type, source primitive, target primitive): by dst, then stably by the key
-		k_prp_dst_keys<<<div_up(nh, 256), 256, 0, s>>>(nh, hits.get(), sort_k[0].get(), sort_v[0].get());
+		k_prp_dst_keys<<<dim3(div_up(max_list, 256), kHitLists), 256, 0, s>>>(list_count.get(), hit_cap / kHitLists,
+			hits.get(), sort_k[0].get(), sort_v[0].get());
 		RadixSortBuffers r1;
 		r1.keys[0] = sort_k[0].get(), r1.keys[1] = sort_k[1].get();
 		r1.vals[0] = sort_v[0].get(), r1.vals[1] = sort_v[1].get();

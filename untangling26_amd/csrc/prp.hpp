@@ -141,7 +141,11 @@ struct PrpResolver
 	DevBuf<unsigned long long> tstat; // [3 kinds][4] traversal counters of one evaluation
 	DevBuf<PrpSlot>	 slots;
 	DevBuf<float>	 slot_tbox;				 // [6 * slots] box of the candidate target edges (EE culling)
+	// raw hits of an evaluation: kHitLists sub-lists of hit_cap / kHitLists entries,
+	// each with its own counter (list_count[16 k], 64 B apart), so that appends
+	// from different waves do not contend for one counter
 	DevBuf<PrpHit>	 hits, hits_sorted;
+	DevBuf<uint32_t> list_count;
 	DevBuf<uint32_t> hit_dst; // target primitive of each sorted hit
 	DevBuf<uint32_t> counters;				 // [0] hits, [1] attached roots, [2] final hits, [3] overflow
 	DevBuf<uint32_t> sort_k[2], sort_v[2], sort_h;
