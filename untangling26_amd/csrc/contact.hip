@@ -229,18 +229,23 @@ __global__ __launch_bounds__(kTravThreads) void k_contact_fused(int kind, Contac
 // CCD broad phase: the swept-box traversal lists the (query, leaf) pairs that
 // survive the region filter; the ACCD runs in its own kernel over that list, so
 // the traversal is not held to the FP64 ACCD's register budget (occupancy).
+// The list is kCcdLists sub-lists of `cap` pairs, one counter each (64 B
+// apart): a single shared counter would serialise the waves' appends.
+constexpr uint32_t kCcdLists = 64;
+constexpr uint32_t kCcdListStride = 16;
 struct CcdEmit
 {
 	const NarrowOp* op;
-	uint32_t*		count;
-	uint2*			list;
+	uint32_t*		count; // [kCcdLists * kCcdListStride]
+	uint2*			list;  // [kCcdLists * cap]
 	uint32_t		cap;
 	__device__ __forceinline__ void operator()(bool valid, uint32_t q, uint32_t leaf)
 	{
+		const uint32_t k = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kCcdLists;
 		const bool	   keep = valid && !op->ccd_filtered(q, leaf);
-		const uint32_t s = wave_append(count, keep);
+		const uint32_t s = wave_append(&count[kCcdListStride * k], keep);
 		if (keep && s < cap)
-			list[s] = make_uint2(q, leaf);
+			list[(size_t)k * cap + s] = make_uint2(q, leaf);
 	}
 };
 
@@ -263,14 +268,16 @@ __global__ __launch_bounds__(kTravThreads) void k_ccd_broad(int kind, ContactArg
 	traverse_staged_test(sh, active, q, q, test, !vf, nodes, status, spill, max_visits, emit);
 }
 
-// ACCD over the listed pairs; the running minimum TOI lets pairs stop early.
+// ACCD over the listed pairs (blockIdx.y = sub-list); the running minimum TOI
+// lets pairs stop early.
 __global__ __launch_bounds__(256) void k_ccd_pairs(int kind, ContactArgs a, const uint2* __restrict__ list,
 	const uint32_t* __restrict__ count, uint32_t cap)
 {
-	const uint32_t n = min(*count, cap);
+	const uint32_t k = blockIdx.y;
+	const uint32_t n = min(count[kCcdListStride * k], cap);
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
 	{
-		const uint2	 p = list[i];
+		const uint2	 p = list[(size_t)k * cap + i];
 		const double t = kind == 2 ? vf_ccd(p.x, p.y, a) : ee_ccd(p.x, p.y, a);
 		if (t != kLineSearchMaxT)
 			toi_min(a.toi, t);
@@ -322,29 +329,34 @@ void contact_query(Context* ctx, int kind, const ContactArgs& a, Lbvh* tree, con
 				kind, a, tree->nodes.get(), tree->status.get(), tree->spill.get(), tree->num_leaves + 1);
 		else
 		{
-			// list the swept pairs (grow the list and list again on overflow), then the ACCD
-			tree->pair_count.reserve(1);
-			if (tree->pair_buf.cap < (1u << 20))
-				tree->pair_buf.reserve(1u << 20);
-			uint32_t n = 0;
+			// list the swept pairs (grow the lists and list again on overflow), then the ACCD
+			tree->pair_count.reserve(kCcdLists * kCcdListStride);
+			if (tree->pair_buf.cap < (1u << 21))
+				tree->pair_buf.reserve(1u << 21);
+			std::vector<uint32_t> cnt(kCcdLists * kCcdListStride);
+			uint32_t			  cap = 0, mx = 0;
 			for (int attempt = 0; attempt < 8; ++attempt)
 			{
-				const uint32_t cap = (uint32_t)tree->pair_buf.cap;
-				UT_HIP_CHECK(hipMemsetAsync(tree->pair_count.get(), 0, 4, ctx->stream));
+				cap = (uint32_t)(tree->pair_buf.cap / kCcdLists);
+				UT_HIP_CHECK(hipMemsetAsync(tree->pair_count.get(), 0, 4 * cnt.size(), ctx->stream));
 				k_ccd_broad<<<div_up(nq, kTravThreads), kTravThreads, 0, ctx->stream>>>(kind, a, tree->nodes.get(),
 					tree->status.get(), tree->spill.get(), tree->num_leaves + 1, tree->pair_count.get(),
 					tree->pair_buf.get(), cap);
 				UT_LAUNCH_CHECK();
-				UT_HIP_CHECK(hipMemcpyAsync(&n, tree->pair_count.get(), 4, hipMemcpyDeviceToHost, ctx->stream));
+				UT_HIP_CHECK(
+					hipMemcpyAsync(cnt.data(), tree->pair_count.get(), 4 * cnt.size(), hipMemcpyDeviceToHost, ctx->stream));
 				UT_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-				if (n <= cap)
+				mx = 0;
+				for (uint32_t k = 0; k < kCcdLists; ++k)
+					mx = std::max(mx, cnt[kCcdListStride * k]);
+				if (mx <= cap)
 					break;
-				tree->pair_buf.reserve((size_t)n + n / 4);
+				tree->pair_buf.reserve((size_t)(mx + mx / 4) * kCcdLists);
 			}
-			UT_REQUIRE(n <= tree->pair_buf.cap, "CCD: swept pair list could not be grown");
-			if (n)
-				k_ccd_pairs<<<std::min<uint32_t>(div_up(n, 256), 4096), 256, 0, ctx->stream>>>(
-					kind, a, tree->pair_buf.get(), tree->pair_count.get(), (uint32_t)tree->pair_buf.cap);
+			UT_REQUIRE(mx <= cap, "CCD: swept pair lists could not be grown");
+			if (mx)
+				k_ccd_pairs<<<dim3(std::min<uint32_t>(div_up(mx, 256), 64), kCcdLists), 256, 0, ctx->stream>>>(
+					kind, a, tree->pair_buf.get(), tree->pair_count.get(), cap);
 		}
 	}
 	else
