@@ -169,6 +169,30 @@ __global__ __launch_bounds__(kPcgThreads) void k_contact_offdiag(const ContactRe
 		block_partial(acc, part);
 }
 
+// Entry contributions are stored 4-wide (x, y, z, 0) so that each is one
+// aligned vector store / load.
+__device__ __forceinline__ void st_contrib(float* c, uint32_t o, double x, double y, double z)
+{
+	*reinterpret_cast<float4*>(c + 4 * (size_t)o) = make_float4((float)x, (float)y, (float)z, 0.0f);
+}
+__device__ __forceinline__ void st_contrib(double* c, uint32_t o, double x, double y, double z)
+{
+	double2* d = reinterpret_cast<double2*>(c + 4 * (size_t)o);
+	d[0] = make_double2(x, y);
+	d[1] = make_double2(z, 0.0);
+}
+__device__ __forceinline__ V3<float> ld_contrib(const float* c, uint32_t o)
+{
+	const float4 v = *reinterpret_cast<const float4*>(c + 4 * (size_t)o);
+	return V3<float>{ v.x, v.y, v.z };
+}
+__device__ __forceinline__ V3<double> ld_contrib(const double* c, uint32_t o)
+{
+	const double2* d = reinterpret_cast<const double2*>(c + 4 * (size_t)o);
+	const double2  a = d[0], b = d[1];
+	return V3<double>{ a.x, a.y, b.x };
+}
+
 // Same product as k_contact_offdiag, scattered into the per-vertex entry lists:
 // contrib[pos[4c + a]] = k2 w_a (S - w_a s_a) n for every corner a of contact c.
 template <typename T>
@@ -194,11 +218,8 @@ __global__ __launch_bounds__(kPcgThreads) void k_contact_contrib(const ContactRe
 	}
 	for (int a = 0; a < 4; ++a)
 	{
-		const double   cf = k2 * w[a] * (S - w[a] * sv[a]);
-		const uint32_t o = 3 * pos[4 * i + a];
-		contrib[o] = (T)(cf * nx);
-		contrib[o + 1] = (T)(cf * ny);
-		contrib[o + 2] = (T)(cf * nz);
+		const double cf = k2 * w[a] * (S - w[a] * sv[a]);
+		st_contrib(contrib, pos[4 * i + a], (double)(T)(cf * nx), (double)(T)(cf * ny), (double)(T)(cf * nz));
 	}
 }
 
@@ -383,9 +404,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint
 		if (ct_off) // contact off-diagonal blocks, this vertex's entries in contact order
 			for (uint32_t e = ct_off[row], end = ct_off[row + 1]; e < end; ++e)
 			{
-				a.x += contrib[3 * e];
-				a.y += contrib[3 * e + 1];
-				a.z += contrib[3 * e + 2];
+				const V3<T> c = ld_contrib(contrib, e);
+				a.x += c.x;
+				a.y += c.y;
+				a.z += c.z;
 			}
 		stv(q, row, a);
 		pq = dotd(pr, a);
@@ -606,7 +628,7 @@ void PcgSolver::prepare_contacts()
 	ct_hist.reserve(radix_sort_hist_size(m));
 	ct_pos.reserve(m);
 	ct_off.reserve((size_t)n + 1);
-	ct_contrib.reserve(3 * (size_t)m);
+	ct_contrib.reserve(4 * (size_t)m);
 	k_ct_entry_keys<<<div_up(m, 256), 256, 0, st>>>(contacts, nc, ct_keys[0].get(), ct_vals[0].get());
 	RadixSortBuffers rs;
 	for (int k = 0; k < 2; ++k)

@@ -81,7 +81,7 @@ struct PcgSolver
 	// vertex's entries in order: no atomics, fixed summation order.
 	uint32_t		 ct_entries = 0;
 	DevBuf<uint32_t> ct_keys[2], ct_vals[2], ct_hist, ct_off, ct_pos;
-	DevBuf<double>	 ct_contrib; // [3 * entries], FP32 or FP64 by the solve's precision
+	DevBuf<double>	 ct_contrib; // [4 * entries] (x, y, z, 0), FP32 or FP64 by the solve's precision
 	void			 prepare_contacts();
 
 	DevBuf<uint8_t> work; // vectors (sized for FP64)
