@@ -833,6 +833,41 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t* p, uint32_t x)
 	}
 }
 
+// Union for the hit graph: finds read parents with cheap (workgroup-scope,
+// possibly stale) loads, which still return ancestors, since parents only move
+// up; the link is a device-scope CAS, and a failed CAS returns the element's
+// current parent, from which the walk resumes.  Each failure replaces the
+// larger of the two candidates by a smaller index, so the loop ends.
+__device__ __forceinline__ uint32_t uf_find_cached(const uint32_t* p, uint32_t x)
+{
+	for (;;)
+	{
+		const uint32_t q = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		if (q == x)
+			return x;
+		x = q;
+	}
+}
+
+__device__ __forceinline__ void uf_union_cached(uint32_t* p, uint32_t a, uint32_t b)
+{
+	a = uf_find_cached(p, a);
+	b = uf_find_cached(p, b);
+	while (a != b)
+	{
+		const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
+		const uint32_t old = atomicCAS(&p[hi], hi, lo);
+		if (old == hi)
+			return;
+		// hi was linked meanwhile: continue from its current parent (< hi)
+		const uint32_t up = uf_find_cached(p, old);
+		if (a == hi)
+			a = up;
+		else
+			b = up;
+	}
+}
+
 __device__ __forceinline__ void uf_union(uint32_t* p, uint32_t a, uint32_t b)
 {
 	for (;;)
@@ -1001,7 +1036,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 		{
 			const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
 			if (j != kNone && j != i)
-				uf_union(a.parent, i, j);
+				uf_union_cached(a.parent, i, j);
 		}
 	};
 	// the same for up to four targets held in registers (kNone entries, last, are skipped)
@@ -1018,7 +1053,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 			{
 				const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
 				if (j != kNone && j != i)
-					uf_union(a.parent, i, j);
+					uf_union_cached(a.parent, i, j);
 			}
 	};
 	auto cswap = [](uint32_t& x, uint32_t& y)
