@@ -1195,11 +1195,13 @@ __device__ __forceinline__ uint64_t canonical_key(const PrpHit& h)
 }
 
 // after all unions: every element points at its root
+// After the unions no root changes, so a walk over cached (possibly already
+// shortened) parents reaches the same root.
 __global__ void k_prp_uf_compress(uint32_t n, uint32_t* parent)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n)
-		parent[i] = uf_find(parent, i);
+		parent[i] = uf_find_cached(parent, i);
 }
 
 __global__ void k_prp_uf_init(uint32_t n, uint32_t* parent, uint32_t* attached, uint32_t* count, uint64_t* key,
