@@ -67,6 +67,58 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 	return r;
 }
 
+// K sums of partial arrays (cnt[k] entries each) and the stop flag, read by the
+// first wave in one pass so that the loads of all arrays are in flight together;
+// each sum has the order of sum_partials, so every workgroup gets identical
+// values.  One call per kernel (its LDS slots are not re-fenced).
+template <int K>
+struct GroupSums
+{
+	double v[K];
+	bool   stop;
+};
+template <int K>
+__device__ __forceinline__ GroupSums<K> group_sums(const double* const (&part)[K], const uint32_t (&cnt)[K],
+	const uint32_t* flags)
+{
+	__shared__ double	s_val[K];
+	__shared__ uint32_t s_stop;
+	if (threadIdx.x < kWave)
+	{
+		if (threadIdx.x == 0)
+			s_stop = __hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		double	 acc[K];
+		uint32_t m = 0;
+#pragma unroll
+		for (int k = 0; k < K; ++k)
+		{
+			acc[k] = 0.0;
+			m = max(m, cnt[k]);
+		}
+		for (uint32_t i = threadIdx.x; i < m; i += kWave)
+		{
+#pragma unroll
+			for (int k = 0; k < K; ++k)
+				if (i < cnt[k])
+					acc[k] += part[k][i];
+		}
+#pragma unroll
+		for (int k = 0; k < K; ++k)
+		{
+			const double s = wave_sum(acc[k]);
+			if (threadIdx.x == 0)
+				s_val[k] = s;
+		}
+	}
+	__syncthreads();
+	GroupSums<K> r;
+#pragma unroll
+	for (int k = 0; k < K; ++k)
+		r.v[k] = s_val[k];
+	r.stop = s_stop != 0u;
+	return r;
+}
+
 template <typename T>
 struct V3
 {
@@ -420,10 +472,11 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(uint32_t it, uint32_
 	const T* __restrict__ minv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
 	const T* __restrict__ q, PcgParts parts, uint32_t* __restrict__ flags)
 {
-	if (group_stopped(flags))
+	const GroupSums<3> gs = group_sums<3>({ parts.pq, parts.pqc, parts.rz[(it + 1) & 1] }, { nb, kContactBlocks, nb }, flags);
+	if (gs.stop)
 		return;
-	const double pq = sum_partials(parts.pq, nb) + sum_partials(parts.pqc, kContactBlocks);
-	const double rz = sum_partials(parts.rz[(it + 1) & 1], nb);
+	const double pq = gs.v[0] + gs.v[1];
+	const double rz = gs.v[2];
 	const double alpha = rz / pq;
 	const bool	 ok = isfinite(alpha) && pq > 0.0;
 	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
@@ -455,16 +508,35 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(uint32_t it, uint32_
 	block_partial(rr, parts.rr);
 }
 
+// Next search direction of one vertex: z + beta p (p kept when beta is not finite).
+template <typename T>
+__device__ __forceinline__ V3<T> next_dir(const T* __restrict__ z, const T* __restrict__ p, uint32_t v, T b, bool ok)
+{
+	V3<T> pv = ldv(p, v);
+	if (ok)
+	{
+		const V3<T> zv = ldv(z, v);
+		pv.x = zv.x + b * pv.x;
+		pv.y = zv.y + b * pv.y;
+		pv.z = zv.z + b * pv.z;
+	}
+	return pv;
+}
+
+// pn = z + beta p for every row, and the contact products of pn (the entries the
+// next SpMV gathers, as k_contact_contrib computes them) for contacts i < nc:
+// each contact forms pn at its four vertices itself, bit-identical to the rows'
+// values, so no launch is needed between the two.  p and pn are distinct buffers.
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(uint32_t it, uint32_t n, uint32_t nb, double tol2,
-	const T* __restrict__ z, T* __restrict__ p, PcgParts parts, uint32_t* __restrict__ flags)
+	const T* __restrict__ z, const T* __restrict__ p, T* __restrict__ pn, PcgParts parts, uint32_t* __restrict__ flags,
+	const ContactRec* __restrict__ pairs, uint32_t nc, const uint32_t* __restrict__ pos, T* __restrict__ contrib)
 {
-	if (group_stopped(flags))
+	const GroupSums<4> gs = group_sums<4>({ parts.rz[it & 1], parts.rz[(it + 1) & 1], parts.rr, parts.bb },
+		{ nb, nb, nb, nb }, flags);
+	if (gs.stop)
 		return;
-	const double rzn = sum_partials(parts.rz[it & 1], nb);
-	const double rzo = sum_partials(parts.rz[(it + 1) & 1], nb);
-	const double rr = sum_partials(parts.rr, nb);
-	const double bb = sum_partials(parts.bb, nb);
+	const double rzn = gs.v[0], rzo = gs.v[1], rr = gs.v[2], bb = gs.v[3];
 	if (blockIdx.x == 0 && threadIdx.x == 0)
 	{
 		flags[1] = it + 1;
@@ -476,17 +548,30 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(uint32_t it, uint32_t n
 		else if (rr <= tol2 * bb)
 			flags[0] = 1u;
 	}
-	const double beta = rzn / rzo;
-	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-	if (row < n && isfinite(beta))
+	const double   beta = rzn / rzo;
+	const bool	   ok = isfinite(beta);
+	const T		   b = (T)beta;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		stv(pn, i, next_dir(z, p, i, b, ok));
+	if (i < nc)
 	{
-		const T		b = (T)beta;
-		const V3<T> zv = ldv(z, row);
-		V3<T>		pv = ldv(p, row);
-		pv.x = zv.x + b * pv.x;
-		pv.y = zv.y + b * pv.y;
-		pv.z = zv.z + b * pv.z;
-		stv(p, row, pv);
+		const ContactRec c = pairs[i];
+		double			 w[4], sv[4];
+		pair_weights(c, w);
+		const double nx = c.normal[0], ny = c.normal[1], nz = c.normal[2], k2 = c.k2;
+		double		 S = 0.0;
+		for (int a = 0; a < 4; ++a)
+		{
+			const V3<T> pv = next_dir(z, p, c.idx[a] & 0x7FFFFFFFu, b, ok);
+			sv[a] = nx * (double)pv.x + ny * (double)pv.y + nz * (double)pv.z;
+			S += w[a] * sv[a];
+		}
+		for (int a = 0; a < 4; ++a)
+		{
+			const double cf = k2 * w[a] * (S - w[a] * sv[a]);
+			st_contrib(contrib, pos[4 * i + a], (double)(T)(cf * nx), (double)(T)(cf * ny), (double)(T)(cf * nz));
+		}
 	}
 }
 
@@ -588,9 +673,9 @@ void PcgSolver::init(Context* c, uint32_t n_, uint32_t nslices_, uint32_t nslots
 	col = cl;
 	diag_slot = ds;
 	const uint32_t nb = div_up(n, kPcgThreads);
-	// x, r, z, p, q (3n each), minv (9n), scale (n), r64 (3n), scale64 (n), d64 spare (3n) in doubles
-	// + 256 B of alignment slack per carved array (10 arrays)
-	work.reserve(sizeof(double) * ((size_t)5 * 3 * n + 9 * n + n + 3 * n + n + 3 * n) + 10 * 256);
+	// x, r, z, p, p2, q (3n each), minv (9n), scale (n), r64 (3n), scale64 (n), d64 spare (3n) in doubles
+	// + 256 B of alignment slack per carved array (11 arrays)
+	work.reserve(sizeof(double) * ((size_t)6 * 3 * n + 9 * n + n + 3 * n + n + 3 * n) + 11 * 256);
 	partials.reserve(8 * (size_t)nb + 8 + kContactBlocks);
 	flags.reserve(4);
 	if (!h_flags)
@@ -652,7 +737,7 @@ namespace
 template <typename T>
 struct Vecs
 {
-	T *x, *r, *z, *p, *q, *minv, *scale;
+	T *x, *r, *z, *p, *p2, *q, *minv, *scale;
 	double *r64, *scale64, *spare;
 };
 template <typename T>
@@ -670,6 +755,7 @@ Vecs<T> carve(uint8_t* base, uint32_t n)
 	v.r = (T*)take(3 * (size_t)n, sizeof(T));
 	v.z = (T*)take(3 * (size_t)n, sizeof(T));
 	v.p = (T*)take(3 * (size_t)n, sizeof(T));
+	v.p2 = (T*)take(3 * (size_t)n, sizeof(T));
 	v.q = (T*)take(3 * (size_t)n, sizeof(T));
 	v.minv = (T*)take(9 * (size_t)n, sizeof(T));
 	v.scale = (T*)take(n, sizeof(T));
@@ -693,16 +779,20 @@ uint32_t run_iterations(PcgSolver& s, const Vecs<T>& v, const T* vals, PcgParts 
 		const uint32_t chunk = std::min(check_every, budget - it);
 		const uint32_t nce = s.ct_entries / 4;
 		T*			   contrib = reinterpret_cast<T*>(s.ct_contrib.get());
+		if (it == 0 && nce) // products of the first direction; later ones come with k_pcg_dir
+			k_contact_contrib<T><<<div_up(nce, kPcgThreads), kPcgThreads, 0, st>>>(
+				s.contacts, nce, v.p, s.ct_pos.get(), contrib, s.flags.get());
 		for (uint32_t k = 0; k < chunk; ++k, ++it)
 		{
-			if (nce)
-				k_contact_contrib<T><<<div_up(nce, kPcgThreads), kPcgThreads, 0, st>>>(
-					s.contacts, nce, v.p, s.ct_pos.get(), contrib, s.flags.get());
-			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(s.n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, lambda,
+			// iteration it uses direction buffer it & 1 and writes the next one to the other
+			T* const p = (it & 1) ? v.p2 : v.p;
+			T* const pn = (it & 1) ? v.p : v.p2;
+			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(s.n, s.slice_off, s.col, vals, s.nslots, p, v.q, lambda,
 				v.scale, parts, s.flags.get(), nce ? s.ct_off.get() : nullptr, contrib);
 			k_pcg_update<T><<<nb, kPcgThreads, 0, st>>>(
-				it, s.n, nb, v.minv, v.x, v.r, v.z, v.p, v.q, parts, s.flags.get());
-			k_pcg_dir<T><<<nb, kPcgThreads, 0, st>>>(it, s.n, nb, tol2, v.z, v.p, parts, s.flags.get());
+				it, s.n, nb, v.minv, v.x, v.r, v.z, p, v.q, parts, s.flags.get());
+			k_pcg_dir<T><<<std::max(nb, div_up(nce, kPcgThreads)), kPcgThreads, 0, st>>>(
+				it, s.n, nb, tol2, v.z, p, pn, parts, s.flags.get(), s.contacts, nce, s.ct_pos.get(), contrib);
 		}
 		UT_LAUNCH_CHECK();
 		UT_HIP_CHECK(hipMemcpyAsync(s.h_flags, s.flags.get(), 12, hipMemcpyDeviceToHost, st));
