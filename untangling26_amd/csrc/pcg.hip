@@ -147,21 +147,42 @@ template <typename TV, typename TX, typename TA>
 __device__ __forceinline__ V3<TA> row_spmv(uint32_t row, const uint32_t* __restrict__ slice_off,
 	const uint32_t* __restrict__ col, const TV* __restrict__ vals, uint32_t nslots, const TX* __restrict__ x)
 {
+	// Slots in batches of B: the batch's column indices, then its x and block
+	// values, are all loaded before any is used, so a row costs two memory round
+	// trips per batch rather than two per slot.  Slots past the width reload the
+	// last one and are skipped in the sums; the width is the same for the 64 rows
+	// of a slice (one wave), so that guard is uniform.
+	constexpr int  B = sizeof(TA) == 4 ? 8 : 4;
 	const uint32_t s = row / kSlice, lane = row % kSlice;
 	const uint32_t off = slice_off[s];
 	const uint32_t w = (slice_off[s + 1] - off) / kSlice;
 	TA			   a0 = 0, a1 = 0, a2 = 0;
-	for (uint32_t k = 0; k < w; ++k)
+	for (uint32_t k0 = 0; k0 < w; k0 += B)
 	{
-		const uint32_t slot = off + k * kSlice + lane;
-		const uint32_t c = col[slot];
-		const TA	   x0 = (TA)x[3 * c], x1 = (TA)x[3 * c + 1], x2 = (TA)x[3 * c + 2];
-		a0 += (TA)vals[0 * (size_t)nslots + slot] * x0 + (TA)vals[1 * (size_t)nslots + slot] * x1
-			+ (TA)vals[2 * (size_t)nslots + slot] * x2;
-		a1 += (TA)vals[3 * (size_t)nslots + slot] * x0 + (TA)vals[4 * (size_t)nslots + slot] * x1
-			+ (TA)vals[5 * (size_t)nslots + slot] * x2;
-		a2 += (TA)vals[6 * (size_t)nslots + slot] * x0 + (TA)vals[7 * (size_t)nslots + slot] * x1
-			+ (TA)vals[8 * (size_t)nslots + slot] * x2;
+		uint32_t slot[B], c[B];
+#pragma unroll
+		for (int j = 0; j < B; ++j)
+		{
+			slot[j] = off + min(k0 + j, w - 1) * kSlice + lane;
+			c[j] = col[slot[j]];
+		}
+		TA xv[B][3], m[B][9];
+#pragma unroll
+		for (int j = 0; j < B; ++j)
+		{
+			for (int d = 0; d < 3; ++d)
+				xv[j][d] = (TA)x[3 * c[j] + d];
+			for (int e = 0; e < 9; ++e)
+				m[j][e] = (TA)vals[e * (size_t)nslots + slot[j]];
+		}
+#pragma unroll
+		for (int j = 0; j < B; ++j)
+			if (k0 + j < w)
+			{
+				a0 += m[j][0] * xv[j][0] + m[j][1] * xv[j][1] + m[j][2] * xv[j][2];
+				a1 += m[j][3] * xv[j][0] + m[j][4] * xv[j][1] + m[j][5] * xv[j][2];
+				a2 += m[j][6] * xv[j][0] + m[j][7] * xv[j][1] + m[j][8] * xv[j][2];
+			}
 	}
 	return V3<TA>{ a0, a1, a2 };
 }
@@ -438,14 +459,16 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint
 	T* __restrict__ q, double lambda, const T* __restrict__ scale, PcgParts parts, const uint32_t* __restrict__ flags,
 	const uint32_t* __restrict__ ct_off, const T* __restrict__ contrib)
 {
-	if (group_stopped(flags))
-		return;
+	// flags[0] has no writer while this kernel runs (stream order), so every
+	// thread reads the same value; the read overlaps the row's loads.
+	const uint32_t stop = __hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
 	double		   pq = 0.0;
+	V3<T>		   a{ 0, 0, 0 }, pr{ 0, 0, 0 };
 	if (row < n)
 	{
-		V3<T>		a = row_spmv<T, T, T>(row, slice_off, col, vals, nslots, p);
-		const V3<T> pr = ldv(p, row);
+		a = row_spmv<T, T, T>(row, slice_off, col, vals, nslots, p);
+		pr = ldv(p, row);
 		if (lambda != 0.0)
 		{
 			const T ls = (T)lambda * scale[row];
@@ -453,14 +476,27 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint
 			a.y += ls * pr.y;
 			a.z += ls * pr.z;
 		}
-		if (ct_off) // contact off-diagonal blocks, this vertex's entries in contact order
-			for (uint32_t e = ct_off[row], end = ct_off[row + 1]; e < end; ++e)
+		if (ct_off) // contact off-diagonal blocks, this vertex's entries in contact order, 4 loads in flight
+			for (uint32_t e = ct_off[row], end = ct_off[row + 1]; e < end; e += 4)
 			{
-				const V3<T> c = ld_contrib(contrib, e);
-				a.x += c.x;
-				a.y += c.y;
-				a.z += c.z;
+				V3<T> c[4];
+#pragma unroll
+				for (int j = 0; j < 4; ++j)
+					c[j] = ld_contrib(contrib, min(e + j, end - 1));
+#pragma unroll
+				for (int j = 0; j < 4; ++j)
+					if (e + j < end)
+					{
+						a.x += c[j].x;
+						a.y += c[j].y;
+						a.z += c[j].z;
+					}
 			}
+	}
+	if (stop)
+		return;
+	if (row < n)
+	{
 		stv(q, row, a);
 		pq = dotd(pr, a);
 	}
@@ -472,6 +508,19 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(uint32_t it, uint32_
 	const T* __restrict__ minv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
 	const T* __restrict__ q, PcgParts parts, uint32_t* __restrict__ flags)
 {
+	// the row's loads are issued before the partial sums are waited for
+	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+	V3<T>		   pv{ 0, 0, 0 }, qv{ 0, 0, 0 }, xv{ 0, 0, 0 }, rv{ 0, 0, 0 };
+	T			   mi[9] = {};
+	if (row < n)
+	{
+		pv = ldv(p, row);
+		qv = ldv(q, row);
+		xv = ldv(x, row);
+		rv = ldv(r, row);
+		for (int k = 0; k < 9; ++k)
+			mi[k] = minv[k * (size_t)n + row];
+	}
 	const GroupSums<3> gs = group_sums<3>({ parts.pq, parts.pqc, parts.rz[(it + 1) & 1] }, { nb, kContactBlocks, nb }, flags);
 	if (gs.stop)
 		return;
@@ -479,20 +528,18 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(uint32_t it, uint32_
 	const double rz = gs.v[2];
 	const double alpha = rz / pq;
 	const bool	 ok = isfinite(alpha) && pq > 0.0;
-	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
-	double		   rzn = 0.0, rr = 0.0;
+	double		 rzn = 0.0, rr = 0.0;
 	if (ok && row < n)
 	{
-		const T		a = (T)alpha;
-		const V3<T> pv = ldv(p, row), qv = ldv(q, row);
-		V3<T>		xv = ldv(x, row), rv = ldv(r, row);
+		const T a = (T)alpha;
 		xv.x += a * pv.x;
 		xv.y += a * pv.y;
 		xv.z += a * pv.z;
 		rv.x -= a * qv.x;
 		rv.y -= a * qv.y;
 		rv.z -= a * qv.z;
-		const V3<T> zv = apply_minv(minv, n, row, rv);
+		const V3<T> zv{ mi[0] * rv.x + mi[1] * rv.y + mi[2] * rv.z, mi[3] * rv.x + mi[4] * rv.y + mi[5] * rv.z,
+			mi[6] * rv.x + mi[7] * rv.y + mi[8] * rv.z };
 		stv(x, row, xv);
 		stv(r, row, rv);
 		stv(z, row, zv);
@@ -510,12 +557,10 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_update(uint32_t it, uint32_
 
 // Next search direction of one vertex: z + beta p (p kept when beta is not finite).
 template <typename T>
-__device__ __forceinline__ V3<T> next_dir(const T* __restrict__ z, const T* __restrict__ p, uint32_t v, T b, bool ok)
+__device__ __forceinline__ V3<T> next_dir(V3<T> zv, V3<T> pv, T b, bool ok)
 {
-	V3<T> pv = ldv(p, v);
 	if (ok)
 	{
-		const V3<T> zv = ldv(z, v);
 		pv.x = zv.x + b * pv.x;
 		pv.y = zv.y + b * pv.y;
 		pv.z = zv.z + b * pv.z;
@@ -532,6 +577,28 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(uint32_t it, uint32_t n
 	const T* __restrict__ z, const T* __restrict__ p, T* __restrict__ pn, PcgParts parts, uint32_t* __restrict__ flags,
 	const ContactRec* __restrict__ pairs, uint32_t nc, const uint32_t* __restrict__ pos, T* __restrict__ contrib)
 {
+	// the row's and the contact's loads are issued before the partial sums are waited for
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	V3<T>		   zr{ 0, 0, 0 }, pr{ 0, 0, 0 };
+	if (i < n)
+	{
+		zr = ldv(z, i);
+		pr = ldv(p, i);
+	}
+	ContactRec c{};
+	V3<T>	   zc[4], pc[4];
+	uint32_t   at[4] = {};
+	if (i < nc)
+	{
+		c = pairs[i];
+		for (int a = 0; a < 4; ++a)
+		{
+			const uint32_t v = c.idx[a] & 0x7FFFFFFFu;
+			zc[a] = ldv(z, v);
+			pc[a] = ldv(p, v);
+			at[a] = pos[4 * i + a];
+		}
+	}
 	const GroupSums<4> gs = group_sums<4>({ parts.rz[it & 1], parts.rz[(it + 1) & 1], parts.rr, parts.bb },
 		{ nb, nb, nb, nb }, flags);
 	if (gs.stop)
@@ -548,29 +615,27 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(uint32_t it, uint32_t n
 		else if (rr <= tol2 * bb)
 			flags[0] = 1u;
 	}
-	const double   beta = rzn / rzo;
-	const bool	   ok = isfinite(beta);
-	const T		   b = (T)beta;
-	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	const double beta = rzn / rzo;
+	const bool	 ok = isfinite(beta);
+	const T		 b = (T)beta;
 	if (i < n)
-		stv(pn, i, next_dir(z, p, i, b, ok));
+		stv(pn, i, next_dir(zr, pr, b, ok));
 	if (i < nc)
 	{
-		const ContactRec c = pairs[i];
-		double			 w[4], sv[4];
+		double w[4], sv[4];
 		pair_weights(c, w);
 		const double nx = c.normal[0], ny = c.normal[1], nz = c.normal[2], k2 = c.k2;
 		double		 S = 0.0;
 		for (int a = 0; a < 4; ++a)
 		{
-			const V3<T> pv = next_dir(z, p, c.idx[a] & 0x7FFFFFFFu, b, ok);
+			const V3<T> pv = next_dir(zc[a], pc[a], b, ok);
 			sv[a] = nx * (double)pv.x + ny * (double)pv.y + nz * (double)pv.z;
 			S += w[a] * sv[a];
 		}
 		for (int a = 0; a < 4; ++a)
 		{
 			const double cf = k2 * w[a] * (S - w[a] * sv[a]);
-			st_contrib(contrib, pos[4 * i + a], (double)(T)(cf * nx), (double)(T)(cf * ny), (double)(T)(cf * nz));
+			st_contrib(contrib, at[a], (double)(T)(cf * nx), (double)(T)(cf * ny), (double)(T)(cf * nz));
 		}
 	}
 }
