@@ -124,7 +124,9 @@ __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint3
 			for (uint32_t k = vv_off[u]; k < vv_off[u + 1]; ++k)
 			{
 				const uint32_t w = vv[k];
-				if (atomicCAS(&H[w], kHopInf, level + 1) == kHopInf)
+				// a plain read skips the atomic for vertices already reached (a stale
+				// read can only show kHopInf, which costs the atomic and nothing else)
+				if (H[w] == kHopInf && atomicCAS(&H[w], kHopInf, level + 1) == kHopInf)
 					nxt[atomicAdd(&nnext, 1u)] = w;
 			}
 		}
@@ -162,7 +164,12 @@ __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint3
 				const f3	   d = xu - ld3(rest_x, w);
 				const float	   nd = du + sqrtf(dot3(d, d));
 				const uint32_t bits = __float_as_uint(nd);
-				const uint32_t old = atomicMin(reinterpret_cast<uint32_t*>(&G[w]), bits);
+				uint32_t* const gw = reinterpret_cast<uint32_t*>(&G[w]);
+				// values only decrease, so a plain read that is not above nd (stale or
+				// not) proves the atomic would change nothing
+				if (bits >= *gw)
+					continue;
+				const uint32_t old = atomicMin(gw, bits);
 				if (bits < old && atomicExch(&I[w], gen) != gen)
 					nxt[atomicAdd(&nnext, 1u)] = w;
 			}
