@@ -160,10 +160,16 @@ __device__ __forceinline__ double ee_ccd(uint32_t left, uint32_t right, const Co
 	return toi_postprocess(accd_toi(true, x0, dx, th, a.toi));
 }
 
-// Non-negative floats order like their bit patterns.
+// Non-negative floats order like their bit patterns.  The minimum only
+// decreases, so a relaxed read that is not above t (stale or not) proves the
+// atomic would change nothing: atomics on one word serialise, and most pairs
+// with a time of impact do not lower the running minimum.
 __device__ __forceinline__ void toi_min(float* toi, double t)
 {
-	atomicMin(reinterpret_cast<uint32_t*>(toi), __float_as_uint((float)t));
+	uint32_t* const w = reinterpret_cast<uint32_t*>(toi);
+	const uint32_t	b = __float_as_uint((float)t);
+	if (b < __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+		atomicMin(w, b);
 }
 
 struct NarrowOp
