@@ -856,7 +856,10 @@ __device__ __forceinline__ uint32_t uf_find_cached(const uint32_t* p, uint32_t x
 	}
 }
 
-__device__ __forceinline__ void uf_union_cached(uint32_t* p, uint32_t a, uint32_t b)
+// Returns the root the two sets share after the union (an ancestor of a and of
+// b); a caller that unions one element with many passes it back as `a`, so
+// each later find starts near the root instead of at the element.
+__device__ __forceinline__ uint32_t uf_union_cached(uint32_t* p, uint32_t a, uint32_t b)
 {
 	a = uf_find_cached(p, a);
 	b = uf_find_cached(p, b);
@@ -865,7 +868,7 @@ __device__ __forceinline__ void uf_union_cached(uint32_t* p, uint32_t a, uint32_
 		const uint32_t hi = a > b ? a : b, lo = a > b ? b : a;
 		const uint32_t old = atomicCAS(&p[hi], hi, lo);
 		if (old == hi)
-			return;
+			return lo;
 		// hi was linked meanwhile: continue from its current parent (< hi)
 		const uint32_t up = uf_find_cached(p, old);
 		if (a == hi)
@@ -873,6 +876,7 @@ __device__ __forceinline__ void uf_union_cached(uint32_t* p, uint32_t a, uint32_
 		else
 			b = up;
 	}
+	return a;
 }
 
 __device__ __forceinline__ void uf_union(uint32_t* p, uint32_t a, uint32_t b)
@@ -1031,6 +1035,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 		return;
 	const PrpHit   L = a.h[i];
 	const uint32_t s = L.slot, hi = a.range[2 * s + 1];
+	uint32_t	   ri = i; // an ancestor of i: where the next union's find of i starts
 	// the hits (type, prim) -> d for targets in ascending order: one bucket lookup,
 	// then galloping searches that resume where the previous one ended
 	auto link_list = [&](uint32_t type, uint32_t prim, const uint32_t* __restrict__ d, uint32_t n)
@@ -1043,7 +1048,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 		{
 			const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
 			if (j != kNone && j != i)
-				uf_union_cached(a.parent, i, j);
+				ri = uf_union_cached(a.parent, ri, j);
 		}
 	};
 	// the same for up to four targets held in registers (kNone entries, last, are skipped)
@@ -1060,7 +1065,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 			{
 				const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
 				if (j != kNone && j != i)
-					uf_union_cached(a.parent, i, j);
+					ri = uf_union_cached(a.parent, ri, j);
 			}
 	};
 	auto cswap = [](uint32_t& x, uint32_t& y)
