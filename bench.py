@@ -228,6 +228,9 @@ def main():
             "pcg": "mixed FP32 + FP64 refinement",
             "fp64_steps_per_s": None if fp64 is None else round(fp64["steps_per_s"], 4),
             "mixed_vs_fp64_speedup": None if fp64 is None else round(value / fp64["steps_per_s"], 3),
+            # the PCG solve alone (per-step PCG time, FP64 over mixed): the rest of the
+            # step is the same work in both legs and varies by a few ms between runs
+            "pcg_speedup_mixed_vs_fp64": None if fp64 is None else round(fp64["ms_pcg"] / max(mixed["ms_pcg"], 1e-9), 3),
             "steps_to_intersection_free": free_step,
             "ef_pairs_first_last": [mixed["ef_pairs"][0], mixed["ef_pairs"][-1]],
             "contours_first_last": [mixed["contours"][0], mixed["contours"][-1]],
