@@ -160,8 +160,8 @@ def ee_roofline(ee):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the path has no multi-GPU implementation "
+              "(it would shard by Morton ranges, contours and DOF stripes)", file=sys.stderr)
         sys.exit(2)
 
     import torch
