@@ -368,8 +368,9 @@ typedef struct ut_solver_params {
 	int			  use_prp;		/* use_untangling_PRP: PRP responses for the contours */
 	int			  quasi_static; /* use_quasi_static_mode with QuasiStaticInertiaMode::HessianOnly */
 	ut_prp_params prp;
-	int			  region_filter; /* filter_invalid_pairs on proximity and CCD pairs; must be 0: the filter
-									is switched off until its kernels are verified on the device */
+	int			  region_filter; /* with use_untangling: filter_invalid_pairs on the proximity pairs and
+									filter_broad_phase_pairs on the CCD pairs (newton_solver.cpp:1308-1311,
+									:2513-2517 run both whenever use_untangling is set) */
 	int			  ignore_near_zero_dist_pairs; /* SceneParams default 1; 0 (cli.py:107) adds EF pairs
 											  from zero-distance proximity pairs and the degenerate filter */
 } ut_solver_params;

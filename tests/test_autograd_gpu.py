@@ -80,3 +80,28 @@ def test_autograd_function(ctx):
     for b in range(3):
         gb = ip.gradient(xb[b].detach()) * wts[b]
         assert torch.allclose(xb.grad[b].double(), gb, rtol=1e-6, atol=1e-6 * gb.abs().max().item())
+
+
+def test_in_place_update_refreshes_contacts(ctx):
+    """An in-place update of x between two evaluations (as an optimiser step does)
+    must re-detect the contacts: the result equals a fresh potential's at the new x."""
+    m, st, x_tilde, recs = ip_case(energy=BARRIER)
+    ip = IncrementalPotential(ctx, st, x_tilde=x_tilde, inv_h2=3600.0, kappa=1e8, energy=BARRIER)
+    x = torch.from_numpy(m.x.copy()).cuda()
+    ip.gradient(x)
+    # move the upper sheet (the second half, 0.6 mm above in y) towards the lower one by
+    # 15% of d_hat: every contact's distance, normal and k1 change
+    upper = torch.arange(m.nv, device=x.device) >= m.nv // 2
+    with torch.no_grad():
+        x[upper, 1] -= 1.5e-4
+    g = ip.gradient(x)
+    e = ip.energies(x)
+    fresh = IncrementalPotential(ctx, st, x_tilde=x_tilde, inv_h2=3600.0, kappa=1e8, energy=BARRIER)
+    xc = x.clone()
+    g_want, e_want = fresh.gradient(xc), fresh.energies(xc)
+    # FP64 atomics in the assembly sum in any order: equal to rounding, while a stale
+    # contact list (k1 and normals of the old positions) is off by far more
+    assert torch.allclose(g, g_want, rtol=1e-10, atol=1e-10 * g_want.abs().max().item())
+    assert torch.allclose(e, e_want, rtol=1e-12)
+    g_old = ip.gradient(torch.from_numpy(m.x.copy()).cuda())
+    assert (g_old - g_want).abs().max() > 1e-3 * g_want.abs().max()

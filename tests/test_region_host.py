@@ -1,0 +1,209 @@
+"""The contact region filter's per-record code, run on the CPU.
+
+regions.hip (k_region_hits, k_region_ef, k_mark_invalid, k_disable_pairs) and
+the CCD broad-phase filter of contact.hip call the host/device functions of
+csrc/region_view.hpp once per record.  tests/host/region_host.cpp runs those
+same functions over array views that check every index against its array
+before the access, on the proximity records, PRP hits, EF pairs and swept
+pairs of real scenes from the oracle pipeline:
+
+* with the masked decode no index leaves its array, and the region lists,
+  invalid vertices, disabled records and CCD keep masks equal the oracle's
+  (filter_invalid_pairs / filter_broad_phase_pairs,
+  intersection_resolver.cpp:2233-2682; host_apply_untangling_constraint,
+  intersection_resolver2.cpp:2296-2426);
+* the decode the filter once had (only idx[2] masked, so bit 31 of an EE
+  record's idx[3] stays on its vertex index; planted only in the host harness)
+  sends the EE records' indices out of every per-vertex array, and the check
+  counts those accesses;
+* the same runs pass under AddressSanitizer + UBSan (a child process with the
+  sanitizer runtime preloaded)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+for _p in (ROOT, ROOT / "oracle", ROOT / "tests"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+TYPE_BIT = np.uint32(0x80000000)
+MASK = np.uint32(0x7FFFFFFF)
+SCENES = ("two_sheets", "folded_small")
+
+
+def _load(asan=False):
+    sys.path.insert(0, str(ROOT / "tests" / "host"))
+    import build as host_build  # tests/host/build.py
+
+    libs = host_build.build()
+    lib = C.CDLL(str(libs["librh_asan.so" if asan else "librh.so"]))
+    lib.rh_violations.restype = C.c_long
+    lib.rh_region_lists.restype = C.c_long
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _scene(name):
+    """Oracle pipeline on a scene: proximity records, EF contours, PRP responses
+    (with the kept hits they come from), region sets, and the static swept pairs."""
+    import oracle
+    from prp_cases import prp_fixture
+
+    f = prp_fixture(name)
+    m, st = f["mesh"], f["setup"]
+    x = np.ascontiguousarray(m.x, np.float32)
+    E, F = st.edges, st.faces
+    d_hat, off = scenes_per_vertex(m)
+    qv = oracle.query_boxes(0, x, x, None, d_hat, off)
+    bvf = oracle.overlap_pairs(qv, oracle.leaf_boxes(1, x, x, F, off))
+    vf, _ = oracle.dcd_list(0, bvf, x, E, F, d_hat, off, st.rest_vert_area, st.rest_edge_area,
+                            st.rest_face_area, 1e8, 0)
+    qe = oracle.query_boxes(1, x, x, E, d_hat, off)
+    bee = oracle.overlap_pairs(qe, oracle.leaf_boxes(2, x, x, E, off), cull=True)
+    ee, _ = oracle.dcd_list(1, bee, x, E, F, d_hat, off, st.rest_vert_area, st.rest_edge_area,
+                            st.rest_face_area, 1e8, 0)
+    prox = np.ascontiguousarray(np.concatenate([vf, ee]), np.uint32)
+    _, resp, meta = f["oracle"].resolve_meta(256, 1.0, 1e8)
+    ef5 = np.ascontiguousarray(f["pairs"][:, :5], np.uint32)
+    ci = np.ascontiguousarray(f["contour_index"], np.uint32)
+    mi = np.ascontiguousarray(f["mesh_index"], np.uint32)
+    regions = oracle.region_sets(st.nv, ef5, ci, mi, resp, meta)
+    hits, hc = _hits_from_responses(resp, meta, E, F)
+    return dict(nv=st.nv, x=x, off=off, E=np.ascontiguousarray(E, np.uint32), F=np.ascontiguousarray(F, np.uint32),
+                prox=prox, resp=resp, meta=meta, ef5=ef5, ci=ci, mi=mi, nc=int(f["nc"]), regions=regions,
+                hits=hits, hc=hc, bvf=np.ascontiguousarray(bvf, np.uint32), bee=np.ascontiguousarray(bee, np.uint32))
+
+
+def scenes_per_vertex(m):
+    from untangling26_amd import scenes
+
+    d, th = scenes.per_vertex(m, d_hat=1e-3, thickness=0.0)
+    return np.ascontiguousarray(d, np.float32), np.ascontiguousarray(th, np.float32)
+
+
+def _hits_from_responses(resp, meta, E, F):
+    """The kept hits (slot, type, src, dst, ...) behind the response records:
+    VF = (vertex, face), FV = (face, vertex), EE = (edge, edge); primitives found
+    from the records' vertex ids."""
+    fid = {tuple(sorted(map(int, f))): i for i, f in enumerate(F)}
+    eid = {tuple(sorted(map(int, e))): i for i, e in enumerate(E)}
+    hits = np.zeros((len(resp), 8), np.uint32)
+    hc = np.zeros(len(resp), np.uint32)
+    for i, (r, (c, t)) in enumerate(zip(resp, meta)):
+        idx = [int(v & MASK) for v in r[:4]]
+        if t == 0:
+            src, dst = idx[0], fid[tuple(sorted(idx[1:]))]
+        elif t == 2:
+            src, dst = fid[tuple(sorted(idx[1:]))], idx[0]
+        else:
+            src, dst = eid[tuple(sorted(idx[:2]))], eid[tuple(sorted(idx[2:]))]
+        hits[i, 1:4] = (t, src, dst)
+        hc[i] = c
+    return hits, hc
+
+
+def _csr(nv, regions):
+    off = np.zeros(nv + 1, np.uint32)
+    off[1:] = np.cumsum([len(r) for r in regions])
+    vals = np.array([q for r in regions for q in r], np.uint32)
+    return off, (vals if len(vals) else np.zeros(1, np.uint32))
+
+
+def run_checks(lib, s, unmasked=False):
+    """All per-record passes on scene s; returns (violations, results)."""
+    lib.rh_reset()
+    nv = s["nv"]
+    # region lists (RegionFilter::build)
+    sup = np.zeros(max(s["nc"], 1), np.uint8)
+    cap = 5 * len(s["ef5"]) + 6 * len(s["hits"]) + 1
+    off = np.zeros(nv + 1, np.uint32)
+    vals = np.zeros(cap, np.uint32)
+    total = lib.rh_region_lists(nv, len(s["hits"]), _p(s["hits"]), _p(s["hc"]), _p(sup), len(sup), len(s["ef5"]),
+                                _p(s["ef5"]), _p(s["ci"]), _p(s["mi"]), _p(s["E"]), len(s["E"]), _p(s["F"]),
+                                len(s["F"]), _p(off), _p(vals), C.c_size_t(cap))
+    lists = [sorted(set(vals[off[v]:off[v + 1]].tolist())) for v in range(nv)]
+    # filter_invalid_pairs on the proximity records
+    roff, rvals = _csr(nv, s["regions"])
+    recs = s["prox"].copy()
+    invalid = np.zeros(nv, np.uint32)
+    lib.rh_filter_proximity(nv, _p(s["x"]), _p(s["off"]), len(recs), _p(recs), _p(roff), _p(rvals),
+                            C.c_size_t(len(rvals)), _p(invalid), int(unmasked))
+    # CCD broad-phase filter
+    keep = {}
+    for kind, b in ((2, s["bvf"]), (3, s["bee"])):
+        k = np.zeros(len(b), np.uint8)
+        lib.rh_ccd_keep(kind, len(b), _p(b), _p(s["E"]), len(s["E"]), _p(s["F"]), len(s["F"]), nv, _p(roff),
+                        _p(rvals), C.c_size_t(len(rvals)), _p(invalid), _p(k))
+        keep[kind] = k.astype(bool)
+    return lib.rh_violations(), dict(total=total, lists=lists, recs=recs, invalid=invalid, keep=keep)
+
+
+@pytest.fixture(scope="module", params=SCENES)
+def scene(request):
+    return _scene(request.param)
+
+
+def test_scene_has_both_record_types(scene):
+    p = scene["prox"]
+    assert np.any(p[:, 2] & TYPE_BIT) and np.any(p[:, 3] & TYPE_BIT)
+    assert len(scene["resp"]) > 0 and len(scene["ef5"]) > 0
+
+
+def test_masked_decode_stays_in_bounds_and_matches_oracle(scene):
+    import oracle
+
+    viol, r = run_checks(_load(), scene)
+    assert viol == 0
+    assert r["total"] == 5 * len(scene["ef5"]) + sum(
+        {0: 4, 1: 4, 2: 4}[int(t)] for t in scene["hits"][:, 1])
+    assert r["lists"] == scene["regions"]
+    want, inv = oracle.filter_proximity(scene["prox"], scene["x"], scene["off"], scene["regions"])
+    assert np.array_equal(r["invalid"].astype(np.uint8), inv)
+    assert np.array_equal(r["recs"], want)
+    for kind, b in ((2, scene["bvf"]), (3, scene["bee"])):
+        m = oracle.ccd_pair_mask(kind, b, scene["E"], scene["F"], scene["regions"], inv)
+        assert np.array_equal(r["keep"][kind], m)
+    # the filter does something on these scenes
+    assert (r["recs"][:, 9] == 0).sum() > (scene["prox"][:, 9] == 0).sum()
+
+
+def test_unmasked_decode_leaves_the_arrays(scene):
+    viol, _ = run_checks(_load(), scene, unmasked=True)
+    n_ee = int(np.count_nonzero(scene["prox"][:, 3] & TYPE_BIT))
+    assert n_ee > 0 and viol >= n_ee
+
+
+def test_under_address_sanitizer():
+    pytest.importorskip("ctypes")
+    sys.path.insert(0, str(ROOT / "tests" / "host"))
+    import build as host_build
+
+    host_build.build()
+    env = dict(os.environ, LD_PRELOAD=host_build.asan_runtime(),
+               ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([sys.executable, __file__, "--asan-child"], capture_output=True, text=True, env=env,
+                       timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
+    assert "asan child ok" in r.stdout
+
+
+if __name__ == "__main__" and sys.argv[1:] == ["--asan-child"]:
+    lib = _load(asan=True)
+    for name in SCENES:
+        s = _scene(name)
+        viol, res = run_checks(lib, s)
+        assert viol == 0, (name, viol)
+        assert res["lists"] == s["regions"], name
+    print("asan child ok", flush=True)

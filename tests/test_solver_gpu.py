@@ -11,13 +11,7 @@ import oracle
 from untangling26_amd import scenes, topology
 from untangling26_amd.lcs_py import PCG_FP64, PCG_MIXED, NewtonSolver
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason=(
-        "solver steps faulted the GPU: the contact region filter used idx[3] of EE records (type bit 31 set) as "
-        "a vertex index; the filter now masks the bit and is switched off, and these steps stay off the device "
-        "until that fix has been verified there")),
-]
+pytestmark = pytest.mark.gpu
 
 
 def _solver(name="two_sheets", precision=PCG_MIXED, n=None, untangle=False, iters=2000):

@@ -105,7 +105,9 @@ class IncrementalPotential:
             if n <= self.cap:
                 break
             self._reserve(2 * n)
-        self._x = x32
+        # a copy: x32 may share storage with the caller's tensor, which an in-place
+        # update (an optimiser step) would change under the cache
+        self._x = x32.clone()
         return n
 
     def _assemble(self, x32, want_vals):

@@ -719,7 +719,6 @@ ut::SolverParams to_params(const ut_solver_params& q)
 	p.use_prp = q.use_prp;
 	p.quasi_static = q.quasi_static;
 	p.qs_inertia_hessian = 1;
-	UT_REQUIRE(q.region_filter == 0, "solver: the contact region filter is switched off (not yet verified on the device)");
 	p.region_filter = q.region_filter;
 	p.ignore_near_zero_dist_pairs = q.ignore_near_zero_dist_pairs;
 	if (q.prp.total_dirs)

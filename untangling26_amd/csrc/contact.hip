@@ -183,16 +183,7 @@ struct NarrowOp
 			return false;
 		uint32_t s[3], t[3];
 		int		 ns, nt;
-		if (kind == 2)
-		{
-			s[0] = q, ns = 1;
-			t[0] = a->faces[3 * leaf], t[1] = a->faces[3 * leaf + 1], t[2] = a->faces[3 * leaf + 2], nt = 3;
-		}
-		else
-		{
-			s[0] = a->edges[2 * q], s[1] = a->edges[2 * q + 1], ns = 2;
-			t[0] = a->edges[2 * leaf], t[1] = a->edges[2 * leaf + 1], nt = 2;
-		}
+		ccd_pair_sides(kind, q, leaf, a->edges, a->faces, s, &ns, t, &nt);
 		return pair_filtered(a->filter, s, ns, t, nt);
 	}
 	__device__ __forceinline__ void operator()(bool valid, uint32_t q, uint32_t leaf)

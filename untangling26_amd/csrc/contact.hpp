@@ -1,5 +1,6 @@
 #pragma once
 
+#include "contact_rec.hpp"
 #include "region_view.hpp"
 #include "ut_runtime.hpp"
 #include "ef_dcd.hpp"
@@ -13,19 +14,6 @@ enum ContactEnergy : int
 	kEnergyQuadratic = 0, // contact_energy_type 0
 	kEnergyBarrier = 1,	  // contact_energy_type 1 (log barrier, the default)
 };
-
-// CollisionPair::CollisionPairTemplate (collision_data.h:48-150): 64 B.
-struct ContactRec
-{
-	uint32_t idx[4];	// VF: v, f0, f1 | type bit, f2;  EE: a0, a1, b0, b1 | type bit
-	float	 normal[3]; // vec1.xyz
-	float	 area;		// vec1.w
-	float	 k1, k2;	// vec2.xy
-	float	 bary[2];	// vec2.zw: VF face bary x, y;  EE edge-a bary x, edge-b bary x
-	float	 dv[3];		// vec3.xyz (friction)
-	float	 lambda;	// vec3.w
-};
-static_assert(sizeof(ContactRec) == 64, "CollisionPairTemplate layout");
 
 struct ContactArgs
 {

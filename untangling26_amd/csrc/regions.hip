@@ -10,19 +10,6 @@ namespace ut
 namespace
 {
 
-__device__ __forceinline__ uint32_t hit_verts(const uint32_t* edges, const uint32_t* faces, uint32_t type,
-	uint32_t prim, bool src, uint32_t* o)
-{
-	// VF: source vertex / target face; EE: edges; FV: source face / target vertex
-	const bool vert = (type == 0 && src) || (type == 2 && !src);
-	if (vert)
-		return o[0] = prim, 1u;
-	if (type == 1)
-		return o[0] = edges[2 * prim], o[1] = edges[2 * prim + 1], 2u;
-	o[0] = faces[3 * prim], o[1] = faces[3 * prim + 1], o[2] = faces[3 * prim + 2];
-	return 3u;
-}
-
 // pass 0 counts, pass 1 writes (cursor = running offsets)
 __global__ void k_region_hits(int pass, uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ hc,
 	const uint8_t* __restrict__ suppressed, const uint32_t* __restrict__ edges, const uint32_t* __restrict__ faces,
@@ -36,7 +23,7 @@ __global__ void k_region_hits(int pass, uint32_t n, const PrpHit* __restrict__ h
 	uint32_t	   v[3];
 	for (int side = 0; side < 2; ++side)
 	{
-		const uint32_t nvv = hit_verts(edges, faces, x.type, side == 0 ? x.src : x.dst, side == 0, v);
+		const uint32_t nvv = hit_side_verts(edges, faces, x.type, side == 0 ? x.src : x.dst, side == 0, v);
 		for (uint32_t k = 0; k < nvv; ++k)
 		{
 			const uint32_t slot = atomicAdd(&cnt[v[k]], 1u);
@@ -59,7 +46,7 @@ __global__ void k_region_ef(int pass, uint32_t n, const EfPairRec* __restrict__ 
 		const uint32_t v = pairs[i].idx[k];
 		const uint32_t slot = atomicAdd(&cnt[v], 1u);
 		if (pass)
-			vals[slot] = k < 2 ? er : er ^ 1u;
+			vals[slot] = ef_vertex_region(k, er);
 	}
 }
 
@@ -77,29 +64,6 @@ __global__ void k_region_copy(uint32_t n, const uint32_t* a, uint32_t* b)
 		b[i] = a[i];
 }
 
-// Record indices carry the collision type in bit 31 (VF: idx[2], EE: idx[3];
-// collision_data.h:36-80): every index is masked before it addresses a vertex.
-constexpr uint32_t kTypeBit = 0x80000000u;
-constexpr uint32_t kIndexMask = 0x7FFFFFFFu;
-
-__device__ __forceinline__ uint32_t rec_sides(const ContactRec& r, uint32_t* a, int* na, uint32_t* b, int* nb)
-{
-	const bool	   vf = (r.idx[2] & kTypeBit) != 0u;
-	const uint32_t i0 = r.idx[0] & kIndexMask, i1 = r.idx[1] & kIndexMask, i2 = r.idx[2] & kIndexMask,
-				   i3 = r.idx[3] & kIndexMask;
-	if (vf)
-	{
-		a[0] = i0, *na = 1;
-		b[0] = i1, b[1] = i2, b[2] = i3, *nb = 3;
-	}
-	else
-	{
-		a[0] = i0, a[1] = i1, *na = 2;
-		b[0] = i2, b[1] = i3, *nb = 2;
-	}
-	return vf ? 1u : 0u;
-}
-
 // fn_identify_invalid_verts_from_proximity_pairs (intersection_resolver.cpp:2353-2374)
 __global__ void k_mark_invalid(uint32_t n, const ContactRec* __restrict__ pairs, const float* __restrict__ x,
 	const float* __restrict__ offset, uint32_t* __restrict__ invalid)
@@ -107,19 +71,10 @@ __global__ void k_mark_invalid(uint32_t n, const ContactRec* __restrict__ pairs,
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	const ContactRec r = pairs[i];
-	const bool		 vf = (r.idx[2] & kTypeBit) != 0u;
-	const uint32_t	 id[4] = { r.idx[0] & kIndexMask, r.idx[1] & kIndexMask, r.idx[2] & kIndexMask,
-		  r.idx[3] & kIndexMask };
-	const float		 b0 = r.bary[0], b1 = r.bary[1];
-	const float		 w[4] = { vf ? 1.0f : b0, vf ? -b0 : 1.0f - b0, -b1, vf ? b0 + b1 - 1.0f : b1 - 1.0f };
-	f3				 d = mk3(0.0f, 0.0f, 0.0f);
-	for (int k = 0; k < 4; ++k)
-		d = d + ld3(x, id[k]) * w[k];
-	const float t = offset[id[0]] + offset[id[2]];
-	if (dot3(d, d) < t * t + 1e-10f)
+	const RecordSides r = decode_record(pairs[i]);
+	if (record_too_close(r, x, offset))
 		for (int k = 0; k < 4; ++k)
-			invalid[id[k]] = 1u;
+			invalid[r.id[k]] = 1u;
 }
 
 // fn_filter_invalid_proximity_pairs_from_invalid_verts / _from_region_info (:2376-2417)
@@ -128,10 +83,8 @@ __global__ void k_disable_pairs(uint32_t n, ContactRec* __restrict__ pairs, Regi
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	uint32_t a[3], b[3];
-	int		 na, nb;
-	rec_sides(pairs[i], a, &na, b, &nb);
-	if (pair_filtered(rv, a, na, b, nb))
+	const RecordSides r = decode_record(pairs[i]);
+	if (pair_filtered(rv, r.a, r.na, r.b, r.nb))
 	{
 		pairs[i].k1 = 0.0f;
 		pairs[i].k2 = 0.0f;

@@ -84,7 +84,6 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 	p = params;
 	UT_REQUIRE(d.nv && d.nf && d.ne && d.edges && d.faces && d.mass && d.d_hat && d.offset, "solver: missing mesh data");
 	UT_REQUIRE(d.slice_off && d.col && d.diag_slot && d.face_slots, "solver: missing block pattern");
-	UT_REQUIRE(!p.region_filter, "solver: the contact region filter is switched off (not yet verified on the device)");
 	const size_t n3 = 3 * (size_t)d.nv;
 	x.reserve(n3);
 	v.reserve(n3);

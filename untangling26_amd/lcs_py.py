@@ -53,10 +53,10 @@ class SceneParams:
     pcg_inner_tol: float = 1e-2
     pcg_block_jacobi: bool = True
     prp_directions: int = 10  # 10 = the reference's direction generator; more appends a Fibonacci set
-    # filter_invalid_pairs / filter_broad_phase_pairs on proximity and CCD pairs.  The library
-    # refuses it (status -1) until its kernels are verified on the device; without it the CCD
-    # line search blocks the repair motion through the contours.
-    use_region_filter: bool = False
+    # filter_invalid_pairs / filter_broad_phase_pairs on proximity and CCD pairs; the reference
+    # runs both whenever use_untangling is set (newton_solver.cpp:1308-1311, :2513-2517).
+    # Without them the CCD line search blocks the repair motion through the contours.
+    use_region_filter: bool = True
     # scene_params.h:55 (the CLI sets it False, cli.py:107): False adds EF pairs from
     # zero-distance proximity pairs and drops isolated degenerate EF pairs
     ignore_near_zero_dist_pairs: bool = True
