@@ -176,6 +176,16 @@ UT_API int ut_contours_get(ut_contours* ct, int what, void* dst);
  * uint32[2n]; outputs uint32[n]. */
 UT_API int ut_graph_components(ut_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
 	const uint32_t* flip_w, uint32_t* scratch, uint32_t* comp_min, uint32_t* parity, uint32_t* inconsistent);
+/* ut_graph_components without the scratch argument, allocating nothing: no
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * union-find parents live in a static device array of 2 * 2^20 uint32 per GPU
+ * (8 MiB, held until the process ends); n above 2^20 is rejected (status -1).
+ * Calls on one GPU share that array, so calls that overlap on one GPU (other
+ * streams, other threads) race; the array keeps the last call's contents and
+ * every call re-initialises what it uses.  Same results as ut_graph_components. */
+UT_API int ut_graph_components_static(ut_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+	const uint32_t* flip_w, uint32_t* comp_min, uint32_t* parity, uint32_t* inconsistent);
 
 /* ---- PCG: ConjugateGradientSolver (Solver/LinearSolver/precond_cg.h:471-543) ---- */
 

@@ -74,3 +74,17 @@ def graph_components(ctx: Context, row_ptr, col_idx, flip_w):
     ctx.check(lib().ut_graph_components(ctx.handle, n, ptr(row_ptr), ptr(col_idx), ptr(flip_w), ptr(scratch),
                                         ptr(cm), ptr(par), ptr(inc)))
     return cm[:n], par[:n], inc[:n]
+
+
+def graph_components_static(ctx: Context, row_ptr, col_idx, flip_w):
+    """graph_components through ut_graph_components_static: the union-find
+    parents live in the library's static device array (no workspace, no
+    allocation inside the library; n <= 2^20)."""
+    n = row_ptr.shape[0] - 1
+    dev = ctx.device
+    cm = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    par = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    inc = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+    ctx.check(lib().ut_graph_components_static(ctx.handle, n, ptr(row_ptr), ptr(col_idx), ptr(flip_w), ptr(cm),
+                                               ptr(par), ptr(inc)))
+    return cm[:n], par[:n], inc[:n]

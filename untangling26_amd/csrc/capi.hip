@@ -1,6 +1,7 @@
 // extern "C" boundary of libuntangle_hip.  Each entry point converts C++
 // exceptions into status codes and records the message on the context.
 #include "../../include/untangle.h"
+#include "graph_static.hpp"
 #include "ut_common.hpp"
 #include "ut_runtime.hpp"
 #include "lbvh.hpp"
@@ -166,6 +167,7 @@ UT_API int ut_ctx_create(int device, void* hip_stream, int flags, ut_ctx** out)
 			UT_REQUIRE(device >= 0 && device < n, "no such HIP device");
 			h->c.device = device;
 			UT_HIP_CHECK(hipSetDevice(device));
+			ut::graph_static_setup(device);
 			if (flags & UT_CTX_OWN_STREAM)
 			{
 				UT_HIP_CHECK(hipStreamCreateWithFlags(&h->c.stream, hipStreamNonBlocking));
@@ -530,6 +532,23 @@ UT_API int ut_graph_components(ut_ctx* ctx, uint32_t n, const uint32_t* row_ptr,
 		{
 			UT_REQUIRE(n == 0 || (row_ptr && scratch && comp_min && parity && inconsistent), "null pointer");
 			ut::graph_components(&ctx->c, n, row_ptr, col_idx, flip_w, scratch, comp_min, parity, inconsistent, nullptr);
+		});
+}
+
+UT_API int ut_graph_components_static(ut_ctx* ctx, uint32_t n, const uint32_t* row_ptr, const uint32_t* col_idx,
+	const uint32_t* flip_w, uint32_t* comp_min, uint32_t* parity, uint32_t* inconsistent)
+{
+	if (!ctx)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			UT_REQUIRE(n <= ut::kGraphStaticMaxVertices,
+				"ut_graph_components_static: n = " + std::to_string(n) + " exceeds the static scratch limit of "
+					+ std::to_string(ut::kGraphStaticMaxVertices) + " vertices");
+			UT_REQUIRE(n == 0 || (row_ptr && comp_min && parity && inconsistent), "null pointer");
+			uint32_t* const uf = ut::graph_static_scratch(ctx->c.device);
+			ut::graph_components(&ctx->c, n, row_ptr, col_idx, flip_w, uf, comp_min, parity, inconsistent, nullptr);
 		});
 }
 
