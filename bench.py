@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="cloth resolution (vertices per side)")
     ap.add_argument("--dirs", type=int, default=64, help="candidate directions per contour")
     ap.add_argument("--pcg-iters", type=int, default=1000)
+    ap.add_argument("--max-repair", type=int, default=200, help="step cap of the repair episode (cli.py --max-iters)")
+    ap.add_argument("--probe-every", type=int, default=3,
+                    help="re-solve every n-th repair step's system at both precisions (0: never)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -74,27 +77,107 @@ def make_solver(args, precision):
     return s, m, np.asarray(rest, np.float32)
 
 
-def run_leg(args, precision):
+def run_leg(args, precision, probe_every=0):
+    """Warm-up, the K timed repair steps, then one whole repair episode.
+
+    Timed region: exactly K physics_step_gpu calls from the tangled state.  A step
+    whose report shows no EF pair and no contour at its start has found the fold
+    intersection-free (the CLI's stopping test, cli.py:134-148): the repair is
+    complete, so the state is reset to the tangled one and the next step starts a
+    new repair.  Every timed step is therefore a repair step.
+
+    Episode (after the timed region): from the tangled state, step until a step
+    reports zero EF pairs and zero contours (at most --max-repair steps), each step
+    timed on the host around its synchronising report read-back.  With
+    probe_every > 0, every probe_every-th step's system is solved again at FP64 and
+    mixed precision (ut_solver_pcg_resolve; outside the step times)."""
     import torch
+
+    from untangling26_amd.lcs_py import PCG_FP64, PCG_MIXED
 
     s, m, rest = make_solver(args, precision)
     x0 = s.positions().clone()
     v0 = torch.zeros_like(x0)
+    name = precision_name(precision)
     for i in range(args.warmup):
         r = s.physics_step_gpu()
-        log(f"{precision_name(precision)} warmup step {i}: {r['ms_total']:.1f} ms, {r['ef_total']} EF pairs, "
+        log(f"{name} warmup step {i}: {r['ms_total']:.1f} ms, {r['ef_total']} EF pairs, "
             f"{r['contours']} contours, {r['prp_responses']} PRP responses, {r['pcg_iterations']} PCG iterations")
     s.set_state(x0, v0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    reps = []
+    reps, restarts = [], 0
     for i in range(args.steps):
         reps.append(s.physics_step_gpu())
+        r = reps[-1]
         # the report read-back synchronizes the step; one line per step keeps a long run visibly alive
-        log(f"{precision_name(precision)} step {i}: {reps[-1]['ms_total']:.1f} ms, {reps[-1]['ef_total']} EF pairs")
+        log(f"{name} step {i}: {r['ms_total']:.1f} ms, {r['ef_total']} EF pairs, {r['contours']} contours, "
+            f"PCG {r['pcg_iterations']} it, residual {r['pcg_relative_residual']:.2e}")
+        if r["ef_total"] == 0 and r["contours"] == 0:
+            s.set_state(x0, v0)
+            restarts += 1
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return s, m, rest, reps, dt
+
+    s.set_state(x0, v0)
+    torch.cuda.synchronize()
+    episode, probes = [], []
+    for i in range(args.max_repair):
+        ts = time.perf_counter()
+        r = s.physics_step_gpu()
+        sec = time.perf_counter() - ts
+        episode.append((r, sec))
+        done = r["ef_total"] == 0 and r["contours"] == 0
+        log(f"{name} repair {i}: {1e3 * sec:.1f} ms, {r['ef_total']} EF pairs, {r['contours']} contours, "
+            f"PCG {r['pcg_iterations']} it, residual {r['pcg_relative_residual']:.2e}")
+        if probe_every and not done and i % probe_every == 0:
+            p64, pmx = s.pcg_resolve(PCG_FP64), s.pcg_resolve(PCG_MIXED)
+            probes.append({"step": i, "fp64": p64, "mixed": pmx})
+            log(f"  PCG probe: fp64 {p64['ms']:.2f} ms {p64['iterations']} it {p64['relative_residual']:.2e} | "
+                f"mixed {pmx['ms']:.2f} ms {pmx['iterations']} it {pmx['relative_residual']:.2e}")
+        if done:
+            break
+    return dict(solver=s, mesh=m, rest=rest, reps=reps, dt=dt, restarts=restarts, episode=episode, probes=probes)
+
+
+def episode_summary(ep, max_repair):
+    """Repair episode: the number of update steps before a step found the fold
+    intersection-free (that last call only validates), their time, and the
+    per-step record of the trajectory."""
+    done = bool(ep) and ep[-1][0]["ef_total"] == 0 and ep[-1][0]["contours"] == 0
+    upd = ep[:-1] if done else ep
+    secs = sum(t for _, t in upd)
+    return {
+        "reached": done,
+        "steps_to_intersection_free": len(upd) if done else None,
+        "seconds_to_intersection_free": round(secs, 4) if done else None,
+        "steps_per_s": round(len(upd) / secs, 4) if upd and secs > 0 else None,
+        "step_cap": max_repair,
+        "ef_pairs": [int(r["ef_total"]) for r, _ in ep],
+        "contours": [int(r["contours"]) for r, _ in ep],
+        "pcg_iterations": [int(r["pcg_iterations"]) for r, _ in upd],
+        "pcg_converged_steps": int(sum(1 for r, _ in upd if r["pcg_converged"])),
+        "pcg_relative_residual_max": max((float(r["pcg_relative_residual"]) for r, _ in upd), default=None),
+        "ms_per_step": [round(1e3 * t, 2) for _, t in ep],
+    }
+
+
+def probe_summary(probes):
+    """Mixed vs FP64 PCG on identical systems, counted where both reached the tolerance."""
+    both = [p for p in probes if p["fp64"]["converged"] and p["mixed"]["converged"]]
+    if not both:
+        return {"systems": len(probes), "both_converged": 0}
+    f = sum(p["fp64"]["ms"] for p in both)
+    x = sum(p["mixed"]["ms"] for p in both)
+    return {
+        "systems": len(probes), "both_converged": len(both),
+        "fp64_ms": round(f, 3), "mixed_ms": round(x, 3), "speedup": round(f / x, 3),
+        "fp64_iterations_mean": round(sum(p["fp64"]["iterations"] for p in both) / len(both), 1),
+        "mixed_iterations_mean": round(sum(p["mixed"]["iterations"] for p in both) / len(both), 1),
+        "fp64_residual_max": max(p["fp64"]["relative_residual"] for p in both),
+        "mixed_residual_max": max(p["mixed"]["relative_residual"] for p in both),
+        "steps": [p["step"] for p in probes],
+    }
 
 
 def log(msg):
@@ -128,13 +211,11 @@ def summarize(reps):
     return out
 
 
-# Algorithmic bytes of the PRP EE sweep kernel (k_prp_ray_edge) per unit of work:
-# a request reads its slot id, candidate index and source edge (16 B); a node visit
-# reads one 64 B BvhNode (both child boxes and references); a leaf test reads the
-# target edge's vertex ids, hops and ownership for the candidate check (18 B); a
-# candidate's edge-edge test reads the source and target endpoint positions (48 B);
-# a hit writes a 32 B PrpHit.
-EE_BYTES = {"requests": 16, "visits": 64, "leaf_tests": 18, "cand_tests": 48, "hits": 32}
+# Algorithmic bytes of a ray-cast kernel per unit of work (compulsory HBM reads and
+# writes, no cache reuse credit): a request 16 B, a BVH node visit 40 B (a box and a
+# child reference), a leaf test 48 B (the target's 36 B of positions + 12 B of
+# indices), a hit 32 B.  The PRP EE sweep (k_prp_ray_edge) counts these units itself.
+EE_BYTES = {"requests": 16, "visits": 40, "leaf_tests": 48, "hits": 32}
 
 
 def measured_traffic(args):
@@ -177,8 +258,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    s, m, rest, reps, dt = run_leg(args, PCG_MIXED)
+    leg = run_leg(args, PCG_MIXED, probe_every=args.probe_every)
+    s, m, rest, reps, dt = leg["solver"], leg["mesh"], leg["rest"], leg["reps"], leg["dt"]
     mixed = summarize(reps)
+    mixed_ep = episode_summary(leg["episode"], args.max_repair)
+    probes = probe_summary(leg["probes"])
     value = args.steps / dt
     # roofline of the dominant kernel, the PRP EE sweep: HIP-event time of its launches in
     # the timed steps and the counters of the same launches
@@ -190,15 +274,15 @@ def main():
     spmv_ms = s.profile_spmv(100)
     spmv_bytes = nnzb * (9 * 4 + 4) + m.nv * (12 + 12)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    free_step = next((i + 1 for i, r in enumerate(reps) if r["ef_total"] == 0), None)
     data = s.get_intersection_contour_data()
 
-    fp64 = None
+    fp64 = fp64_ep = None
     if not args.no_fp64:
-        s64, _, _, reps64, dt64 = run_leg(args, PCG_FP64)
-        fp64 = summarize(reps64)
-        fp64["steps_per_s"] = args.steps / dt64
-        del s64
+        leg64 = run_leg(args, PCG_FP64)
+        fp64 = summarize(leg64["reps"])
+        fp64["steps_per_s"] = args.steps / leg64["dt"]
+        fp64_ep = episode_summary(leg64["episode"], args.max_repair)
+        del leg64
 
     cpu = None
     if not args.no_cpu:
@@ -225,13 +309,25 @@ def main():
         "config": {
             "workload": f"folded_cloth {args.n}x{args.n} ({m.nf} tris) untangling repair step, "
                         f"{args.dirs} directions/contour, PCG rel. residual 1e-6",
+            "generator": "scenes.folded_cloth(seed 0): 2 m sheet folded at its midline, the folded half "
+                         "offset by a quarter cell in-plane (coincident layers make the EF predicates "
+                         "degenerate), z += 0.08 sin(2 pi u) sin(pi v) + N(0, 0.005)",
             "pcg": "mixed FP32 + FP64 refinement",
+            "timed_steps": "K repair steps from the tangled state; a step that finds the fold "
+                           "intersection-free restarts the repair from the tangled state",
+            "timed_restarts": leg["restarts"],
+            "repair_mixed": mixed_ep,
+            "repair_fp64": fp64_ep,
+            "steps_to_intersection_free": mixed_ep["steps_to_intersection_free"],
+            "seconds_to_intersection_free_mixed": mixed_ep["seconds_to_intersection_free"],
+            "seconds_to_intersection_free_fp64": None if fp64_ep is None else fp64_ep["seconds_to_intersection_free"],
+            "mixed_vs_fp64_speedup_to_intersection_free": (
+                round(fp64_ep["seconds_to_intersection_free"] / mixed_ep["seconds_to_intersection_free"], 3)
+                if fp64_ep and fp64_ep["reached"] and mixed_ep["reached"] else None),
+            "pcg_mixed_vs_fp64_equal_residual": probes,
             "fp64_steps_per_s": None if fp64 is None else round(fp64["steps_per_s"], 4),
             "mixed_vs_fp64_speedup": None if fp64 is None else round(value / fp64["steps_per_s"], 3),
-            # the PCG solve alone (per-step PCG time, FP64 over mixed): the rest of the
-            # step is the same work in both legs and varies by a few ms between runs
             "pcg_speedup_mixed_vs_fp64": None if fp64 is None else round(fp64["ms_pcg"] / max(mixed["ms_pcg"], 1e-9), 3),
-            "steps_to_intersection_free": free_step,
             "ef_pairs_first_last": [mixed["ef_pairs"][0], mixed["ef_pairs"][-1]],
             "contours_first_last": [mixed["contours"][0], mixed["contours"][-1]],
             "final_num_pairs": int(data["num_pairs"]),
@@ -261,6 +357,8 @@ def main():
         "roofline": {
             "kernel": "k_prp_ray_edge (PRP EE sweep: LBVH traversal + edge-edge tests)",
             "bound": "hbm",
+            "limiter": "latency of dependent BVH node fetches (most node reads hit L2; measured HBM "
+                       "traffic is below the algorithmic bytes)",
             "achieved": round(ee_achieved, 2),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -423,6 +423,14 @@ UT_API int ut_solver_contour_get(ut_solver* s, int what, void* dst);
  * PCG runs in FP64). */
 enum { UT_PROFILE_PCG_SPMV = 0 };
 UT_API int ut_solver_profile(ut_solver* s, int what, uint32_t reps, float* avg_ms);
+/* Measurement hook (no counterpart in the reference): solves the last Newton
+ * iteration's system again -- the same assembled matrix, contact blocks and
+ * right-hand side -- at `precision` (UT_PCG_FP64, or UT_PCG_MIXED when the
+ * solver itself runs mixed) into a probe vector, leaving the solver's state
+ * unchanged.  *ms (may be null) receives the solve's duration from HIP events
+ * on the solver's stream.  Returns 0, or 2 when the solve did not reach the
+ * tolerance.  Lets a caller compare the two precisions on identical systems. */
+UT_API int ut_solver_pcg_resolve(ut_solver* s, int precision, ut_pcg_report* report, float* ms);
 
 /* ---- PRP untangling: IntersectionResolver::host_resolve with PRP
  *      (intersection_resolver.cpp:2131-2217 -> intersection_resolver2.cpp:2750-4069) ---- */

@@ -89,17 +89,20 @@ extern "C"
 {
 
 long rh_violations() { return g_violations; }
+uint32_t rh_dedup(uint32_t* v, uint32_t n) { return ut::region_dedup(v, n); }
 void rh_reset() { g_violations = 0; }
 
 // Region lists per vertex (the two passes of RegionFilter::build): kept PRP
 // hits (rows of 8 u32: slot, type, src, dst, b0, b1, dist, root) of contours
 // hit_contour[i], skipped when suppressed[contour]; then every EF pair (rows
 // of 5 vertex ids) with region 2 * contour_index + mesh_index.  off[nv + 1]
-// and vals[cap] are written; returns the number of entries (may exceed cap).
+// and vals[cap] are written, then each vertex's list is reduced in place by
+// region_dedup to its first kept[v] entries; returns the number of entries
+// before the reduction (may exceed cap).
 long rh_region_lists(uint32_t nv, uint32_t nh, const uint32_t* hits, const uint32_t* hit_contour,
 	const uint8_t* suppressed, uint32_t ncontours, uint32_t nef, const uint32_t* ef, const uint32_t* contour_index,
 	const uint32_t* mesh_index, const uint32_t* edges, uint32_t ne, const uint32_t* faces, uint32_t nf, uint32_t* off,
-	uint32_t* vals, size_t cap)
+	uint32_t* vals, size_t cap, uint32_t* kept)
 {
 	const U32					 E{ edges, 2 * (size_t)ne }, F{ faces, 3 * (size_t)nf };
 	const U32					 HC{ hit_contour, nh };
@@ -154,6 +157,9 @@ long rh_region_lists(uint32_t nv, uint32_t nh, const uint32_t* hits, const uint3
 			total = run;
 		}
 	}
+	if ((size_t)total <= cap)
+		for (uint32_t v = 0; v < nv; ++v)
+			kept[v] = ut::region_dedup(vals + off[v], off[v + 1] - off[v]);
 	return total;
 }
 

@@ -129,10 +129,12 @@ def run_checks(lib, s, unmasked=False):
     cap = 5 * len(s["ef5"]) + 6 * len(s["hits"]) + 1
     off = np.zeros(nv + 1, np.uint32)
     vals = np.zeros(cap, np.uint32)
+    kept = np.zeros(nv, np.uint32)
     total = lib.rh_region_lists(nv, len(s["hits"]), _p(s["hits"]), _p(s["hc"]), _p(sup), len(sup), len(s["ef5"]),
                                 _p(s["ef5"]), _p(s["ci"]), _p(s["mi"]), _p(s["E"]), len(s["E"]), _p(s["F"]),
-                                len(s["F"]), _p(off), _p(vals), C.c_size_t(cap))
-    lists = [sorted(set(vals[off[v]:off[v + 1]].tolist())) for v in range(nv)]
+                                len(s["F"]), _p(off), _p(vals), C.c_size_t(cap), _p(kept))
+    raw = [vals[off[v]:off[v] + kept[v]].tolist() for v in range(nv)]
+    lists = [sorted(set(r)) for r in raw]
     # filter_invalid_pairs on the proximity records
     roff, rvals = _csr(nv, s["regions"])
     recs = s["prox"].copy()
@@ -146,7 +148,7 @@ def run_checks(lib, s, unmasked=False):
         lib.rh_ccd_keep(kind, len(b), _p(b), _p(s["E"]), len(s["E"]), _p(s["F"]), len(s["F"]), nv, _p(roff),
                         _p(rvals), C.c_size_t(len(rvals)), _p(invalid), _p(k))
         keep[kind] = k.astype(bool)
-    return lib.rh_violations(), dict(total=total, lists=lists, recs=recs, invalid=invalid, keep=keep)
+    return lib.rh_violations(), dict(total=total, lists=lists, raw=raw, recs=recs, invalid=invalid, keep=keep)
 
 
 @pytest.fixture(scope="module", params=SCENES)
@@ -168,6 +170,9 @@ def test_masked_decode_stays_in_bounds_and_matches_oracle(scene):
     assert r["total"] == 5 * len(scene["ef5"]) + sum(
         {0: 4, 1: 4, 2: 4}[int(t)] for t in scene["hits"][:, 1])
     assert r["lists"] == scene["regions"]
+    # duplicates removed: a vertex with at most 8 distinct regions lists each once
+    assert all(len(q) == len(set(q)) for q in r["raw"] if len(set(q)) <= 8)
+    assert sum(len(q) for q in r["raw"]) < r["total"]
     want, inv = oracle.filter_proximity(scene["prox"], scene["x"], scene["off"], scene["regions"])
     assert np.array_equal(r["invalid"].astype(np.uint8), inv)
     assert np.array_equal(r["recs"], want)
@@ -182,6 +187,25 @@ def test_unmasked_decode_leaves_the_arrays(scene):
     viol, _ = run_checks(_load(), scene, unmasked=True)
     n_ee = int(np.count_nonzero(scene["prox"][:, 3] & TYPE_BIT))
     assert n_ee > 0 and viol >= n_ee
+
+
+def test_region_dedup_keeps_every_region():
+    lib = _load()
+    lib.rh_dedup.restype = C.c_uint32
+    rng = np.random.default_rng(3)
+    for distinct in (1, 2, 8, 9, 20):
+        ids = rng.choice(4096, distinct, replace=False).astype(np.uint32)
+        v = np.ascontiguousarray(rng.choice(ids, 200), np.uint32)
+        v[:distinct] = ids  # every id present
+        k = lib.rh_dedup(_p(v), len(v))
+        kept = v[:k].tolist()
+        assert set(kept) == set(ids.tolist())
+        # the first 8 distinct regions appear once; beyond them entries may repeat
+        first = list(dict.fromkeys(kept))[:8]
+        assert all(kept.count(r) == 1 for r in first)
+        if distinct <= 8:
+            assert k == distinct
+    assert lib.rh_dedup(_p(np.zeros(1, np.uint32)), 0) == 0
 
 
 def test_under_address_sanitizer():

@@ -47,6 +47,7 @@ SIGNATURES = {
     "ut_contours_counts": (C.c_int, [vp, u32p]),
     "ut_contours_get": (C.c_int, [vp, C.c_int, vp]),
     "ut_graph_components": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp]),
+    "ut_solver_pcg_resolve": (C.c_int, [vp, C.c_int, vp, vp]),
     "ut_graph_components_static": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]),
     "ut_pcg_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     "ut_pcg_destroy": (C.c_int, [vp]),

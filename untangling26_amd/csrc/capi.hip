@@ -1177,3 +1177,23 @@ UT_API int ut_solver_profile(ut_solver* s, int what, uint32_t reps, float* avg_m
 				!fp64, reps);
 		});
 }
+
+UT_API int ut_solver_pcg_resolve(ut_solver* s, int precision, ut_pcg_report* report, float* ms)
+{
+	if (!s || !report)
+		return ut::kInvalid;
+	int conv = 1;
+	const int rc = guarded(s->s.ctx,
+		[&]
+		{
+			const ut::PcgReport r = s->s.resolve_last_system(precision, ms);
+			report->relative_residual = r.relative_residual;
+			report->infinity_norm = r.infinity_norm;
+			report->iterations = r.iterations;
+			report->refinements = r.refinements;
+			report->converged = r.converged;
+			report->failure = r.failure;
+			conv = r.converged;
+		});
+	return rc != ut::kOk ? rc : (conv ? ut::kOk : ut::kNotConverged);
+}

@@ -154,4 +154,31 @@ UT_HD uint32_t hit_side_verts(const U& edges, const U& faces, uint32_t type, uin
 // er = 2 * contour + mesh index.
 UT_HD uint32_t ef_vertex_region(int k, uint32_t er) { return k < 2 ? er : er ^ 1u; }
 
+// Duplicate removal in one vertex's region list v[0, n), in place: a vertex
+// collects one entry per EF pair and per PRP hit that touches it, mostly
+// copies of one or two regions.  The first kRegionSet distinct regions are
+// kept once each (checked against a register set); entries beyond them are
+// kept as they come.  The kept list has the same set of regions.  Returns the
+// kept count.
+constexpr int kRegionSet = 8;
+UT_HD uint32_t region_dedup(uint32_t* v, uint32_t n)
+{
+	uint32_t set[kRegionSet];
+	int		 k = 0;
+	uint32_t w = 0;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		const uint32_t r = v[i];
+		bool		   seen = false;
+		for (int j = 0; j < k; ++j)
+			seen = seen || set[j] == r;
+		if (seen)
+			continue;
+		if (k < kRegionSet)
+			set[k++] = r;
+		v[w++] = r;
+	}
+	return w;
+}
+
 } // namespace ut

@@ -64,6 +64,26 @@ __global__ void k_region_copy(uint32_t n, const uint32_t* a, uint32_t* b)
 		b[i] = a[i];
 }
 
+// duplicate removal per vertex, then the kept counts
+__global__ void k_region_dedup(uint32_t nv, const uint32_t* __restrict__ off, uint32_t* __restrict__ vals,
+	uint32_t* __restrict__ kept)
+{
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v < nv)
+		kept[v] = region_dedup(vals + off[v], off[v + 1] - off[v]);
+}
+
+__global__ void k_region_compact(uint32_t nv, const uint32_t* __restrict__ off, const uint32_t* __restrict__ vals,
+	const uint32_t* __restrict__ off2, uint32_t* __restrict__ vals2)
+{
+	const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v >= nv)
+		return;
+	const uint32_t b = off[v], n = off2[v + 1] - off2[v], o = off2[v];
+	for (uint32_t i = 0; i < n; ++i)
+		vals2[o + i] = vals[b + i];
+}
+
 // fn_identify_invalid_verts_from_proximity_pairs (intersection_resolver.cpp:2353-2374)
 __global__ void k_mark_invalid(uint32_t n, const ContactRec* __restrict__ pairs, const float* __restrict__ x,
 	const float* __restrict__ offset, uint32_t* __restrict__ invalid)
@@ -98,6 +118,7 @@ void RegionFilter::init(Context* c, uint32_t nv_)
 	ctx = c;
 	nv = nv_;
 	off.reserve(nv + 1);
+	off2.reserve(nv + 1);
 	cnt.reserve(nv + 1);
 	invalid.reserve(nv);
 	scan_tmp.reserve(scan_tmp_size(nv + 1));
@@ -127,6 +148,12 @@ void RegionFilter::build(const PrpResolver& prp, const ContourBuilder& ct)
 			prp.suppressed.get(), prp.edges, prp.faces, cnt.get(), regions.get());
 	k_region_ef<<<div_up(ne, 256), 256, 0, s>>>(1, ne, ct.pairs.get(), ct.contour_index.get(), ct.mesh_index.get(),
 		cnt.get(), regions.get());
+	// one entry per distinct region: the filters' match loops run over these lists
+	// for every proximity and swept pair
+	k_region_dedup<<<div_up(nv, 256), 256, 0, s>>>(nv, off.get(), regions.get(), cnt.get());
+	exclusive_scan(cnt.get(), off2.get(), nv + 1, scan_tmp.get(), nullptr, s);
+	regions2.reserve(std::max(total, 1u));
+	k_region_compact<<<div_up(nv, 256), 256, 0, s>>>(nv, off.get(), regions.get(), off2.get(), regions2.get());
 	UT_LAUNCH_CHECK();
 	active = true;
 }
@@ -148,8 +175,8 @@ RegionView RegionFilter::view() const
 	RegionView v;
 	if (active)
 	{
-		v.off = off.get();
-		v.regions = regions.get();
+		v.off = off2.get();
+		v.regions = regions2.get();
 		v.invalid = invalid.get();
 	}
 	return v;

@@ -24,6 +24,7 @@ struct RegionFilter
 	Context*		 ctx = nullptr;
 	uint32_t		 nv = 0;
 	DevBuf<uint32_t> off, regions, cnt, invalid, scan_tmp;
+	DevBuf<uint32_t> off2, regions2; // the lists without duplicates (what the filters read)
 	bool			 active = false;
 
 	void init(Context* c, uint32_t nv);

@@ -397,6 +397,7 @@ StepReport NewtonSolver::physics_step_GPU()
 		rep.pcg_refinements += pr.refinements;
 		rep.pcg_relative_residual = pr.relative_residual;
 		rep.pcg_converged = pr.converged;
+		has_system = true;
 		UT_HIP_CHECK(hipEventRecord(ev[5], s));
 		// device_ccd_line_search (newton_solver.cpp:1446-1483)
 		k_trial<<<g(n3), 256, 0, s>>>(n3, x_iter_start.get(), dx.get(), x_end.get());
@@ -451,6 +452,26 @@ StepReport NewtonSolver::physics_step_GPU()
 	rep.ms_total = total;
 	rep.ms_prp = t_prp;
 	return rep;
+}
+
+PcgReport NewtonSolver::resolve_last_system(int precision, float* ms)
+{
+	UT_REQUIRE(has_system, "solver: no system assembled yet (run a step first)");
+	UT_REQUIRE(precision == kPcgFp64 || precision == kPcgMixed, "solver: probe precision must be FP64 or mixed");
+	UT_REQUIRE(precision == kPcgFp64 || p.pcg.precision != kPcgFp64,
+		"solver: the FP32 matrix copy exists only when the solver runs the mixed or FP32 PCG");
+	const hipStream_t s = ctx->stream;
+	dx_probe.reserve(3 * (size_t)d.nv);
+	PcgParams pp = p.pcg;
+	pp.precision = precision;
+	UT_HIP_CHECK(hipEventRecord(ev[0], s));
+	const PcgReport r = pcg.solve(pp, precision == kPcgFp64 ? (const void*)vals64.get() : (const void*)vals32.get(),
+		rhs.get(), dx_probe.get(), precision == kPcgMixed ? vals64.get() : nullptr);
+	UT_HIP_CHECK(hipEventRecord(ev[1], s));
+	UT_HIP_CHECK(hipEventSynchronize(ev[1]));
+	if (ms)
+		UT_HIP_CHECK(hipEventElapsedTime(ms, ev[0], ev[1]));
+	return r;
 }
 
 } // namespace ut

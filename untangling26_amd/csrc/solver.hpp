@@ -61,9 +61,9 @@ struct SolverParams
 	int	   use_prp = 1;		   // PRP responses for the contours (use_untangling_PRP)
 	int	   quasi_static = 0;   // use_quasi_static_mode: x_tilde = x + h^2 g, no inertia gradient
 	int	   qs_inertia_hessian = 1; // QuasiStaticInertiaMode::HessianOnly keeps m / h^2 on the diagonal
-	// filter_invalid_pairs (intersection_resolver.cpp:2594-2682) on the proximity and CCD
-	// pairs; switched off (init refuses it) until its kernels are verified on the device
-	int		  region_filter = 0;
+	// filter_invalid_pairs / filter_broad_phase_pairs (intersection_resolver.cpp:2594-2682)
+	// on the proximity and CCD pairs; the reference runs both whenever use_untangling is set
+	int		  region_filter = 1;
 	// scene_params.h:55; 0 (the CLI's setting) turns zero-distance proximity pairs into EF
 	// pairs (intersection_resolver.cpp:404-421) and filters degenerate EF pairs (:841-897)
 	int		  ignore_near_zero_dist_pairs = 1;
@@ -119,6 +119,13 @@ struct NewtonSolver
 	void init(Context* c, const SolverDesc& desc, const SolverParams& params, const float* x0, const float* v0);
 	~NewtonSolver();
 	StepReport physics_step_GPU();
+	// Solves the last Newton iteration's system again (same matrix, contacts and
+	// right-hand side) at `precision` into a probe vector, leaving the solver's
+	// state alone; *ms is the solve's duration (HIP events on the stream).
+	PcgReport resolve_last_system(int precision, float* ms);
+
+	DevBuf<double> dx_probe;
+	bool		   has_system = false;
 
 private:
 	void detect(StepReport& rep);

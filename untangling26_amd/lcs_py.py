@@ -201,6 +201,19 @@ class NewtonSolver:
         self._ctx.check(lib().ut_solver_profile(self._handle, 0, reps, C.byref(ms)))
         return ms.value
 
+    def pcg_resolve(self, precision):
+        """Solve the last step's system again at `precision` (PCG_FP64, or PCG_MIXED when
+        the solver runs mixed); the solver state is unchanged.  Returns the solve's report
+        (relative_residual, iterations, refinements, converged, failure) and its ms."""
+        from ._lib import PcgReport
+
+        r = PcgReport()
+        ms = C.c_float()
+        self._ctx.check(lib().ut_solver_pcg_resolve(self._handle, int(precision), C.byref(r), C.byref(ms)))
+        out = {k: getattr(r, k) for k, _ in PcgReport._fields_}
+        out["ms"] = ms.value
+        return out
+
     def physics_step_cpu(self):
         raise RuntimeError("physics_step_cpu is not provided: this build runs the step on the GPU only")
 
