@@ -170,6 +170,21 @@ int		 or_prp_select(uint32_t n, const float* objective, const uint32_t* covered,
 		 const float* depth, const float* cost, const uint32_t* hits, const uint32_t* eligible);
 uint32_t or_prp_resolve(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa,
 	void* choices_out, uint32_t* recs, uint32_t cap);
+/* Known-answer access to single steps of the PRP selection (see oracle_prp.c):
+ * the k-ring driver with a caller-supplied evaluator, the contour suppression
+ * and the penetration summary of a hit list. */
+typedef void (*or_prp_eval_fn)(void* user, uint32_t combo, uint32_t ks, uint32_t kd, void* metrics);
+int		 or_prp_drive(uint32_t nd, uint32_t ks0, uint32_t kd0, uint32_t Ds, uint32_t Dd, or_prp_eval_fn eval, void* user,
+		 void* metrics_out, uint32_t* flags, uint32_t* radii, uint32_t* levels, uint32_t* no_response);
+uint32_t or_prp_suppress(uint32_t nc, const uint8_t* adjm, const float* area, uint32_t nactive, const uint32_t* active,
+	uint8_t* sup);
+void	 or_prp_penetration(uint32_t n, const void* hits, const float* face_area, const float* edge_area,
+		 const float* vert_area, const float* mass, const float* face_normal, const float* edge_normal,
+		 const float* vert_normal, const uint32_t* edges, const uint32_t* faces, const float* dir, void* out_metrics);
+int		 or_prp_hit_attached(const or_prp_mesh* m, uint32_t c, const void* hit);
+uint32_t or_prp_vertex_flags(const or_prp_mesh* m, uint32_t v, uint32_t* out, uint32_t cap);
+int		 or_prp_pick_component(uint32_t ncomp, const uint8_t* attached, const uint32_t* cov, const uint32_t* cnt,
+		 const uint64_t* key);
 /* or_prp_resolve plus meta[2i] = contour and meta[2i + 1] = hit type (0 VF, 1 EE,
  * 2 FV) of response i: the region semantics of host_apply_untangling_constraint
  * (intersection_resolver2.cpp:2339-2345) need the kind, which the VF-layout FV

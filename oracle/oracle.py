@@ -100,6 +100,17 @@ def lib():
         L.or_prp_geo.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
         L.or_prp_geo.restype = C.c_void_p
         L.or_prp_select.argtypes = [C.c_uint32, p, p, p, p, p, p, p]
+        L.or_prp_drive.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, p, p, p, p, p, p, p]
+        L.or_prp_drive.restype = C.c_int
+        L.or_prp_suppress.argtypes = [C.c_uint32, p, p, C.c_uint32, p, p]
+        L.or_prp_suppress.restype = C.c_uint32
+        L.or_prp_penetration.argtypes = [C.c_uint32, p, p, p, p, p, p, p, p, p, p, p, p]
+        L.or_prp_hit_attached.argtypes = [p, C.c_uint32, p]
+        L.or_prp_hit_attached.restype = C.c_int
+        L.or_prp_vertex_flags.argtypes = [p, C.c_uint32, p, C.c_uint32]
+        L.or_prp_vertex_flags.restype = C.c_uint32
+        L.or_prp_pick_component.argtypes = [C.c_uint32, p, p, p, p]
+        L.or_prp_pick_component.restype = C.c_int
         L.or_prp_hits_adjacent.argtypes = [C.c_void_p, p, p]
         L.or_prp_resolve.argtypes = [C.c_void_p, C.c_uint32, C.c_float, C.c_float, p, p, C.c_uint32]
         L.or_prp_resolve.restype = C.c_uint32
@@ -488,6 +499,71 @@ def prp_select(objective, covered, ef_count, depth, cost, hits, eligible):
                                _p(_c(hits, np.uint32)), _p(_c(eligible, np.uint32)))
 
 
+PRP_EVAL_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p)
+
+
+def prp_drive(nd, ks0, kd0, diam_src, diam_dst, evaluate):
+    """The oracle's adaptive k-ring driver with a Python evaluator:
+    evaluate(combo, ks, kd) -> dict of PRP_METRICS fields (missing ones zero).
+    Returns dict(selected, levels, no_response, metrics[nd], eligible, converged,
+    pruned, evaluated, radii[nd, 2])."""
+    def cb(_user, combo, ks, kd, out):
+        m = np.zeros(1, PRP_METRICS)
+        for k, v in evaluate(int(combo), int(ks), int(kd)).items():
+            m[k] = v
+        C.memmove(out, m.ctypes.data, PRP_METRICS.itemsize)
+
+    fn = PRP_EVAL_FN(cb)
+    met = np.zeros(max(nd, 1), PRP_METRICS)
+    flags = np.zeros(max(nd, 1), np.uint32)
+    radii = np.zeros((max(nd, 1), 2), np.uint32)
+    levels = np.zeros(1, np.uint32)
+    nr = np.zeros(1, np.uint32)
+    sel = lib().or_prp_drive(nd, ks0, kd0, diam_src, diam_dst, C.cast(fn, C.c_void_p), None, _p(met), _p(flags),
+                             _p(radii), _p(levels), _p(nr))
+    f = flags[:nd]
+    return dict(selected=int(sel), levels=int(levels[0]), no_response=bool(nr[0]), metrics=met[:nd],
+                eligible=(f & 1) != 0, converged=(f & 2) != 0, pruned=(f & 4) != 0, evaluated=(f & 8) != 0,
+                radii=radii[:nd])
+
+
+def prp_suppress(adjacency, area, active):
+    """Contours suppressed by the oracle's suppression pass: adjacency[c] = contours
+    that contour c's kept hits reach; area[c]; active contour ids."""
+    nc = len(adjacency)
+    adjm = np.zeros((max(nc, 1), max(nc, 1)), np.uint8)
+    for c, row in enumerate(adjacency):
+        for o in row:
+            adjm[c, o] = 1
+    sup = np.zeros(max(nc, 1), np.uint8)
+    act = _c(active, np.uint32)
+    lib().or_prp_suppress(nc, _p(adjm), _p(_c(area, np.float32)), len(act), _p(act), _p(sup))
+    return [int(c) for c in np.nonzero(sup[:nc])[0]]
+
+
+def prp_penetration(hits, face_area, edge_area, vert_area, mass, face_normal, edge_normal, vert_normal, edges,
+                    faces, direction):
+    """The oracle's penetration summary (volume, depth, objective, area, displacement
+    cost) of the valid hits (PRP_HIT records) with the given per-primitive data."""
+    h = np.ascontiguousarray(np.asarray(hits, PRP_HIT))
+    arr = [_c(a if len(a) else np.zeros(1), np.float32) for a in (face_area, edge_area, vert_area, mass)]
+    nrm = [_c(np.asarray(a, np.float32).reshape(-1, 3) if len(a) else np.zeros((1, 3)), np.float32)
+           for a in (face_normal, edge_normal, vert_normal)]
+    top = [_c(a if len(a) else np.zeros((1, 3)), np.uint32) for a in (edges, faces)]
+    out = np.zeros(1, PRP_METRICS)
+    lib().or_prp_penetration(len(h), _p(h), *[_p(a) for a in arr], *[_p(a) for a in nrm], *[_p(a) for a in top],
+                             _p(_c(direction, np.float32)), _p(out))
+    return out[0]
+
+
+def prp_pick_component(attached, cov, cnt, key):
+    """The oracle's kept-component rule over per-component data; index or -1."""
+    n = len(attached)
+    a = _c(attached, np.uint8)
+    return lib().or_prp_pick_component(n, _p(a if n else np.zeros(1, np.uint8)), _p(_c(cov, np.uint32)),
+                                       _p(_c(cnt, np.uint32)), _p(_c(key, np.uint64)))
+
+
 class PrpOracle:
     """CPU restatement of the PRP path on one mesh (oracle_prp.c)."""
 
@@ -560,6 +636,15 @@ class PrpOracle:
         hits = np.zeros(max(n, 1), PRP_HIT)
         lib().or_prp_eval_slot(self.h, c, _p(d), k_src, k_dst, _p(hits), n, _p(met))
         return hits[:n], met[0]
+
+    def hit_attached(self, c, hit):
+        h = np.ascontiguousarray(np.asarray([hit], PRP_HIT))
+        return bool(lib().or_prp_hit_attached(self.h, c, _p(h)))
+
+    def vertex_flags(self, v):
+        out = np.zeros(64, np.uint32)
+        n = lib().or_prp_vertex_flags(self.h, v, _p(out), 64)
+        return out[:min(n, 64)].tolist()
 
     def hits_adjacent(self, a, b):
         a = np.ascontiguousarray(np.asarray([a], PRP_HIT))

@@ -29,13 +29,21 @@
  * largest-magnitude component is positive (the reference takes Eigen's sign:
  * parity unpinned for the sign, the direction set is sign-symmetric).
  *
- * Pinning (tests/test_oracle_prp.py): the ownership and comparator cases of
- * UnitTest/test_prp_selection.cpp:74-290 (known answers); direction slots
+ * Pinning (tests/test_oracle_prp.py, tests/test_oracle_prp_rules.py): the
+ * ownership and comparator cases of UnitTest/test_prp_selection.cpp:74-290, the
+ * suppression cases (:426-444), the five scripted-evaluator cases of the k-ring
+ * driver (:446-516, through or_prp_drive) and the penetration-summary case
+ * (:530-555, through or_prp_penetration) as known answers; direction slots
  * against numpy.linalg.eigh on the fixture of test_prp_selection.cpp:201-270;
  * hop distances against the closed form on grids; rest geodesics against
  * scipy.sparse.csgraph.dijkstra; ray hits against closed-form rays on
- * axis-aligned sheets; components against scipy connected_components of the
- * same adjacency; J(r) against its closed form.
+ * axis-aligned sheets; hit adjacency against a restatement of the reference's
+ * bucketed find_adjacent_hits lookup over dense hit sets plus a positive and a
+ * negative hand-built case per rule; attachment and the kept-component order
+ * against hand-built cases; components against scipy connected_components of
+ * the same adjacency; J(r) against its closed form.  Parity unpinned: directions
+ * beyond the reference's 10 (a Fibonacci-sphere extension of this build, for
+ * the 16- and 64-direction configurations) and the eigenvector sign.
  */
 #include "oracle.h"
 
@@ -1176,6 +1184,42 @@ static int hit_adj_contour(const or_prp_mesh* m, uint32_t c, const or_prp_hit* h
 	return 1;
 }
 
+/* is_hit_adj_contour (intersection_resolver2.cpp:378-412) with the source on
+ * role 0 (every ray of the reference is cast with src_mesh_idx 0, cached_bvh.cpp:271-277,
+ * :513-514): all source vertices flagged 2c and all target vertices 2c + 1. */
+int or_prp_hit_attached(const or_prp_mesh* m, uint32_t c, const void* hit)
+{
+	return hit_adj_contour(m, c, (const or_prp_hit*)hit);
+}
+
+/* The boundary flags (2c + role) of vertex v, ascending; returns their count. */
+uint32_t or_prp_vertex_flags(const or_prp_mesh* m, uint32_t v, uint32_t* out, uint32_t cap)
+{
+	const uint32_t n = m->flag_off[v + 1] - m->flag_off[v];
+	for (uint32_t i = 0; i < n && i < cap; ++i)
+		out[i] = m->flags[m->flag_off[v] + i];
+	return n;
+}
+
+/* The kept component of a slot (intersection_resolver2.cpp:3610-3650): among the
+ * components with a hit attached to the contour, the largest EF-anchor coverage,
+ * then the most hits, then the smallest canonical key (the minimum hit key of the
+ * component).  Returns its index, or -1 when no component is attached. */
+int or_prp_pick_component(uint32_t ncomp, const uint8_t* attached, const uint32_t* cov, const uint32_t* cnt,
+	const uint64_t* key)
+{
+	int sel = -1;
+	for (uint32_t i = 0; i < ncomp; ++i)
+	{
+		if (!attached[i])
+			continue;
+		if (sel < 0 || cov[i] > cov[sel] || (cov[i] == cov[sel] && cnt[i] > cnt[sel])
+			|| (cov[i] == cov[sel] && cnt[i] == cnt[sel] && key[i] < key[sel]))
+			sel = (int)i;
+	}
+	return sel;
+}
+
 /* covered EF pairs of contour c by the support of hits[idx[0..n)] */
 static uint32_t coverage(const or_prp_mesh* m, uint32_t c, const or_prp_hit* hits, const uint32_t* idx, uint32_t n,
 	uint8_t* src_set, uint8_t* dst_set)
@@ -1218,7 +1262,17 @@ static uint64_t hit_key(const or_prp_hit* h)
 	return ((uint64_t)h->type << 62) | ((uint64_t)(h->src & 0x3fffffffu) << 31) | (uint64_t)(h->dst & 0x7fffffffu);
 }
 
-static float inv_mass(const or_prp_mesh* m, const or_prp_hit* h)
+/* The arrays prp_eval_penetration_v2 reads (intersection_resolver2.cpp:1883-1896):
+ * rest areas, masses and normals per vertex / edge / face, and the topology the
+ * effective inverse mass needs. */
+typedef struct
+{
+	const float *	va, *ea, *fa, *mass, *vn, *en, *fn;
+	const uint32_t *edges, *faces;
+} pen_arrays;
+
+/* compute_effective_inverse_mass_weight (intersection_resolver_helper.cpp:611-645) */
+static float inv_mass(const pen_arrays* m, const or_prp_hit* h)
 {
 #define IM(v) (1.0f / fmaxf(m->mass[v], 1e-8f))
 	if (h->type == 0)
@@ -1239,6 +1293,65 @@ static float inv_mass(const or_prp_mesh* m, const or_prp_hit* h)
 #undef IM
 }
 
+/* prp_eval_penetration_v2 (intersection_resolver2.cpp:1883-1954) over the hits
+ * idx[0, n): per hit the source primitive's rest area A and |n . r| of its
+ * normal; volume = sum A |n . r| d, depth = max d, area = sum A, objective =
+ * area * depth^2, displacement cost = sum (d + 3e-3)^2 / w_eff (hits with
+ * w_eff <= 1e-8 add nothing).  Per-hit terms in FP32 as the reference, sums in
+ * FP64: the reference's FP32 sum depends on its (parallel) hit order, the FP64
+ * sum rounded once does not. */
+static void penetration(const pen_arrays* m, f3 dir, const or_prp_hit* hits, const uint32_t* idx, uint32_t n,
+	or_prp_metrics* out)
+{
+	double vol = 0, area = 0, cost = 0;
+	float  depth = 0;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		const or_prp_hit* h = hits + idx[i];
+		float			  a;
+		const float*	  nn;
+		if (h->type == 0)
+			a = m->va[h->src], nn = m->vn + 3 * h->src;
+		else if (h->type == 1)
+			a = m->ea[h->src], nn = m->en + 3 * h->src;
+		else
+			a = m->fa[h->src], nn = m->fn + 3 * h->src;
+		const float al = fabsf(dot(mk(nn[0], nn[1], nn[2]), dir));
+		vol += (double)(a * al * h->dist);
+		area += (double)a;
+		depth = h->dist > depth ? h->dist : depth;
+		const float sh = h->dist + 3e-3f, im = inv_mass(m, h);
+		cost += (double)(im > 1e-8f ? sh * sh / im : 0.0f);
+	}
+	out->hit_count = n;
+	out->depth = depth;
+	out->area = (float)area;
+	out->objective = out->area * depth * depth;
+	out->volume = (float)vol;
+	out->disp_cost = (float)cost;
+}
+
+/* Known-answer access to prp_eval_penetration_v2: the valid hits of hits[0, n)
+ * with caller-given areas, masses, normals and topology; writes or_prp_metrics
+ * (hit_count, depth, volume, objective, area, disp_cost; the rest zero). */
+void or_prp_penetration(uint32_t n, const void* hits_in, const float* face_area, const float* edge_area,
+	const float* vert_area, const float* mass, const float* face_normal, const float* edge_normal,
+	const float* vert_normal, const uint32_t* edges, const uint32_t* faces, const float* dir, void* out_metrics)
+{
+	const or_prp_hit* hits = hits_in;
+	const pen_arrays  pa = { vert_area, edge_area, face_area, mass, vert_normal, edge_normal, face_normal, edges, faces };
+	uint32_t*		  idx = malloc(sizeof(uint32_t) * (n + 1));
+	uint32_t		  nv = 0;
+	for (uint32_t i = 0; i < n; ++i)
+		if (hits[i].valid)
+			idx[nv++] = i;
+	or_prp_metrics met;
+	memset(&met, 0, sizeof(met));
+	penetration(&pa, mk(dir[0], dir[1], dir[2]), hits, idx, nv, &met);
+	memcpy(out_metrics, &met, sizeof(met));
+	free(idx);
+}
+
 /* prp_eval_penetration_v2 + coverage + frontier hops over the valid hits */
 static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit* hits, uint32_t n,
 	or_prp_metrics* out, uint8_t* src_set, uint8_t* dst_set)
@@ -1246,10 +1359,6 @@ static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit
 	const uint32_t V = m->nv;
 	uint32_t*	   idx = malloc(sizeof(uint32_t) * (n + 1));
 	uint32_t	   nv = 0;
-	/* per-hit terms in FP32 as the reference, sums in FP64: the reference's FP32
-	 * sum depends on its (parallel) hit order, the FP64 sum rounded once does not */
-	double		   vol = 0, area = 0, cost = 0;
-	float		   depth = 0;
 	uint32_t	   hs = 0, hd = 0;
 	for (uint32_t i = 0; i < n; ++i)
 	{
@@ -1257,21 +1366,7 @@ static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit
 		if (!h->valid)
 			continue;
 		idx[nv++] = i;
-		float		   a, al;
-		const float*   nn;
-		if (h->type == 0)
-			a = m->va[h->src], nn = m->vn + 3 * h->src;
-		else if (h->type == 1)
-			a = m->ea[h->src], nn = m->en + 3 * h->src;
-		else
-			a = m->fa[h->src], nn = m->fn + 3 * h->src;
-		al = fabsf(dot(mk(nn[0], nn[1], nn[2]), dir));
-		vol += (double)(a * al * h->dist);
-		area += (double)a;
-		depth = h->dist > depth ? h->dist : depth;
-		const float sh = h->dist + 3e-3f, im = inv_mass(m, h);
-		cost += (double)(im > 1e-8f ? sh * sh / im : 0.0f);
-		uint32_t sv[3], dv[3];
+		uint32_t	   sv[3], dv[3];
 		const uint32_t ns = hit_src_verts(m, h, sv), nd = hit_dst_verts(m, h, dv);
 		for (uint32_t k = 0; k < ns; ++k)
 		{
@@ -1286,12 +1381,8 @@ static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit
 				hd = hh;
 		}
 	}
-	out->hit_count = nv;
-	out->depth = depth;
-	out->area = (float)area;
-	out->objective = out->area * depth * depth;
-	out->volume = (float)vol;
-	out->disp_cost = (float)cost;
+	const pen_arrays pa = { m->va, m->ea, m->fa, m->mass, m->vn, m->en, m->fn, m->edges, m->faces };
+	penetration(&pa, dir, hits, idx, nv, out);
 	out->max_hop_src = hs;
 	out->max_hop_dst = hd;
 	out->ef_count = m->c_off[c + 1] - m->c_off[c];
@@ -1332,30 +1423,35 @@ static uint32_t eval_slot(const or_prp_mesh* m, uint32_t c, f3 dir, uint32_t ks,
 	for (uint32_t i = 0; i < n; ++i)
 		if (hit_adj_contour(m, c, H + i))
 			attached[uf_find(par, i)] = 1;
-	int		 found = 0;
-	uint32_t best_root = 0, best_cov = 0, best_hits = 0;
-	uint64_t best_key = ~0ull;
+	/* per component (by root): attachment, coverage, hit count, canonical key */
+	uint32_t* roots = malloc(sizeof(uint32_t) * (n + 1));
+	uint32_t* cov = malloc(sizeof(uint32_t) * (n + 1));
+	uint32_t* cnt = malloc(sizeof(uint32_t) * (n + 1));
+	uint64_t* key = malloc(sizeof(uint64_t) * (n + 1));
+	uint8_t*  att = malloc(n + 1);
+	uint32_t  nc = 0;
 	for (uint32_t r = 0; r < n; ++r)
 	{
-		if (uf_find(par, r) != r || !attached[r])
+		if (uf_find(par, r) != r)
 			continue;
-		uint32_t cnt = 0;
-		uint64_t key = ~0ull;
+		uint32_t k = 0;
+		uint64_t kk = ~0ull;
 		for (uint32_t i = 0; i < n; ++i)
 			if (uf_find(par, i) == r)
 			{
-				idx[cnt++] = i;
-				const uint64_t k = hit_key(H + i);
-				key = k < key ? k : key;
+				idx[k++] = i;
+				const uint64_t hk = hit_key(H + i);
+				kk = hk < kk ? hk : kk;
 			}
-		const uint32_t cov = coverage(m, c, H, idx, cnt, src_set, dst_set);
-		if (!found || cov > best_cov || (cov == best_cov && cnt > best_hits)
-			|| (cov == best_cov && cnt == best_hits && key < best_key))
-			found = 1, best_root = r, best_cov = cov, best_hits = cnt, best_key = key;
+		roots[nc] = r, cnt[nc] = k, key[nc] = kk, att[nc] = attached[r];
+		cov[nc] = attached[r] ? coverage(m, c, H, idx, k, src_set, dst_set) : 0;
+		++nc;
 	}
-	if (found)
+	const int sel = or_prp_pick_component(nc, att, cov, cnt, key);
+	if (sel >= 0)
 		for (uint32_t i = 0; i < n; ++i)
-			H[i].valid = uf_find(par, i) == best_root;
+			H[i].valid = uf_find(par, i) == roots[sel];
+	free(roots), free(cov), free(cnt), free(key), free(att);
 	memset(met, 0, sizeof(*met));
 	eval_hits(m, c, dir, H, n, met, src_set, dst_set);
 	met->raw_hits = n;
@@ -1496,27 +1592,37 @@ static uint32_t grow(uint32_t cur, uint32_t diam)
 	return (uint32_t)(n < diam ? n : diam);
 }
 
-static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
+/* The adaptive k-ring driver of one contour (evaluate_current_runtime_directions_template,
+ * intersection_resolver2.cpp:1058-1230) over nd directions, from radii (ks, kd) with
+ * role diameters (Ds, Dd).  eval(user, combo, ks, kd, &metrics) evaluates one
+ * direction at the current radii.  Per level every direction that has neither
+ * converged nor been pruned is evaluated; a result is eligible when usable and
+ * detached (each role's farthest support hop below its radius, or the radius at
+ * the role's diameter); a direction converges when eligible or when both radii
+ * reached the diameters; unconverged directions whose response objective is not
+ * below the best eligible one are pruned; a role grows (doubling, capped at its
+ * diameter) when some pending direction's support touches its frontier or is not
+ * usable.  R[nd] receives each direction's last result; returns the levels run,
+ * *no_response = 1 when it finished without an eligible response. */
+typedef or_prp_eval_fn prp_eval_fn; /* metrics: or_prp_metrics */
+
+static uint32_t drive(uint32_t nd, uint32_t ks, uint32_t kd, uint32_t Ds, uint32_t Dd, prp_eval_fn eval, void* user,
+	combo_result* R, int* no_response)
 {
-	const uint32_t nd = m->ndirs[c];
-	const uint32_t Ds = m->diameter[2 * c], Dd = m->diameter[2 * c + 1];
-	uint32_t	   ks = initial_k(m->b_off[2 * c + 1] - m->b_off[2 * c], Ds);
-	uint32_t	   kd = initial_k(m->b_off[2 * c + 2] - m->b_off[2 * c + 1], Dd);
-	combo_result*  R = calloc(nd + 1, sizeof(combo_result));
-	or_prp_hit*	   hits = NULL;
-	uint32_t	   cap = 0, level = 0;
-	memset(out, 0, sizeof(*out));
-	out->combo = -1;
-	for (;; ++level)
+	enum { kMaxLevels = 2 * 32 + 1 };
+	memset(R, 0, sizeof(combo_result) * (nd + 1));
+	uint32_t level = 0;
+	int		 has = 0;
+	for (; level < kMaxLevels; ++level)
 	{
 		for (uint32_t ci = 0; ci < nd; ++ci)
 		{
 			combo_result* r = R + ci;
 			if (r->converged || r->pruned)
 				continue;
-			const float*   d = m->dirs + ((size_t)c * kMaxDirs + ci) * 3;
 			or_prp_metrics met;
-			eval_slot(m, c, mk(d[0], d[1], d[2]), ks, kd, &hits, &cap, &met);
+			memset(&met, 0, sizeof(met));
+			eval(user, ci, ks, kd, &met);
 			memset(r, 0, sizeof(*r));
 			r->combo = ci, r->hit_count = met.hit_count, r->covered = met.covered, r->ef_count = met.ef_count;
 			r->depth = met.depth, r->objective = met.objective, r->area = met.area, r->disp_cost = met.disp_cost;
@@ -1529,7 +1635,7 @@ static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
 			r->pruned = 0;
 			r->hs = met.max_hop_src, r->hd = met.max_hop_dst;
 		}
-		int	  has = 0;
+		has = 0;
 		float best_obj = INFINITY;
 		for (uint32_t ci = 0; ci < nd; ++ci)
 			if (R[ci].eligible && R[ci].usable)
@@ -1559,11 +1665,76 @@ static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
 			break; /* the reference raises here */
 		ks = ns, kd = ndd;
 	}
+	*no_response = !has;
+	return level + 1;
+}
+
+static int select_combo(const combo_result* R, uint32_t nd)
+{
 	int sel = -1;
 	for (uint32_t ci = 0; ci < nd; ++ci)
 		if (R[ci].eligible && R[ci].usable && (sel < 0 || better(R + ci, R + sel)))
 			sel = (int)ci;
-	out->levels = level + 1;
+	return sel;
+}
+
+/* Known-answer access to the driver: eval is a C callback (scripted metrics in
+ * the tests).  Writes per direction its last metrics (or_prp_metrics[nd]),
+ * flags[nd] (bit 0 eligible, 1 converged, 2 pruned, 3 evaluated) and the radii
+ * (ks, kd) of that evaluation; *levels and *no_response as drive(); returns the
+ * selected direction (prp_select_response) or -1. */
+int or_prp_drive(uint32_t nd, uint32_t ks0, uint32_t kd0, uint32_t Ds, uint32_t Dd, prp_eval_fn eval, void* user,
+	void* metrics_out, uint32_t* flags, uint32_t* radii, uint32_t* levels, uint32_t* no_response)
+{
+	combo_result* R = calloc(nd + 1, sizeof(combo_result));
+	int			  nr = 0;
+	*levels = drive(nd, ks0, kd0, Ds, Dd, eval, user, R, &nr);
+	*no_response = (uint32_t)nr;
+	or_prp_metrics* M = metrics_out;
+	for (uint32_t ci = 0; ci < nd; ++ci)
+	{
+		const combo_result* r = R + ci;
+		memset(M + ci, 0, sizeof(*M));
+		M[ci].hit_count = r->hit_count, M[ci].covered = r->covered, M[ci].ef_count = r->ef_count;
+		M[ci].depth = r->depth, M[ci].volume = r->volume, M[ci].objective = r->objective, M[ci].area = r->area;
+		M[ci].disp_cost = r->disp_cost, M[ci].max_hop_src = r->hs, M[ci].max_hop_dst = r->hd;
+		M[ci].usable = r->usable;
+		flags[ci] = (r->eligible ? 1u : 0u) | (r->converged ? 2u : 0u) | (r->pruned ? 4u : 0u) | (r->present ? 8u : 0u);
+		radii[2 * ci] = r->ks, radii[2 * ci + 1] = r->kd;
+	}
+	const int sel = select_combo(R, nd);
+	free(R);
+	return sel;
+}
+
+typedef struct
+{
+	or_prp_mesh* m;
+	uint32_t	 c;
+	or_prp_hit*	 hits;
+	uint32_t	 cap;
+} slot_eval_ctx;
+
+static void slot_eval(void* user, uint32_t ci, uint32_t ks, uint32_t kd, void* met)
+{
+	slot_eval_ctx* e = user;
+	const float*   d = e->m->dirs + ((size_t)e->c * kMaxDirs + ci) * 3;
+	eval_slot(e->m, e->c, mk(d[0], d[1], d[2]), ks, kd, &e->hits, &e->cap, met);
+}
+
+static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
+{
+	const uint32_t nd = m->ndirs[c];
+	const uint32_t Ds = m->diameter[2 * c], Dd = m->diameter[2 * c + 1];
+	const uint32_t ks = initial_k(m->b_off[2 * c + 1] - m->b_off[2 * c], Ds);
+	const uint32_t kd = initial_k(m->b_off[2 * c + 2] - m->b_off[2 * c + 1], Dd);
+	combo_result*  R = calloc(nd + 1, sizeof(combo_result));
+	slot_eval_ctx  e = { m, c, NULL, 0 };
+	int			   nr = 0;
+	memset(out, 0, sizeof(*out));
+	out->combo = -1;
+	out->levels = drive(nd, ks, kd, Ds, Dd, slot_eval, &e, R, &nr);
+	const int sel = select_combo(R, nd);
 	if (sel >= 0)
 	{
 		const combo_result* r = R + sel;
@@ -1573,7 +1744,7 @@ static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
 		out->disp_cost = r->disp_cost;
 		memcpy(out->dir, d, 12);
 	}
-	free(R), free(hits);
+	free(R), free(e.hits);
 }
 
 /* get_adj_contours (intersection_resolver2.cpp:414-448): other contours whose
@@ -1596,6 +1767,54 @@ static void adj_contours(const or_prp_mesh* m, uint32_t c, const or_prp_hit* h, 
 		if (ok)
 			mark[fs / 2] = 1;
 	}
+}
+
+/* prp_select_pre_finalize_contours_to_suppress (intersection_resolver2.cpp:514-582):
+ * adjm[o * nc + a] != 0 when contour o's kept hits reach contour a.  The active
+ * contours with an adjacency, by area descending then index ascending, each
+ * suppress the active contours they reach unless suppressed themselves.
+ * sup[nc] receives 1 for every suppressed contour; returns their count. */
+uint32_t or_prp_suppress(uint32_t nc, const uint8_t* adjm, const float* area, uint32_t nactive, const uint32_t* active,
+	uint8_t* sup)
+{
+	uint8_t*  is_active = calloc(nc + 1, 1);
+	uint32_t* order = malloc(4 * (nc + 1));
+	uint32_t  no = 0, ns = 0;
+	for (uint32_t i = 0; i < nactive; ++i)
+		is_active[active[i]] = 1;
+	for (uint32_t i = 0; i < nactive; ++i)
+	{
+		const uint32_t c = active[i];
+		int			   any = 0;
+		for (uint32_t o = 0; o < nc && !any; ++o)
+			any = adjm[(size_t)c * nc + o] != 0;
+		if (any)
+			order[no++] = c;
+	}
+	for (uint32_t i = 0; i < no; ++i) /* area descending, index ascending */
+		for (uint32_t j = i + 1; j < no; ++j)
+		{
+			const float ai = area[order[i]], aj = area[order[j]];
+			if (aj > ai || (aj == ai && order[j] < order[i]))
+			{
+				uint32_t t = order[i];
+				order[i] = order[j], order[j] = t;
+			}
+		}
+	memset(sup, 0, nc);
+	for (uint32_t i = 0; i < no; ++i)
+	{
+		const uint32_t o = order[i];
+		if (sup[o])
+			continue;
+		for (uint32_t a = 0; a < nc; ++a)
+			if (a != o && is_active[a] && adjm[(size_t)o * nc + a])
+				sup[a] = 1;
+	}
+	for (uint32_t c = 0; c < nc; ++c)
+		ns += sup[c];
+	free(is_active), free(order);
+	return ns;
 }
 
 /* Full PRP of the prepared contours: choices[nc]; response records (16 words
@@ -1632,38 +1851,16 @@ uint32_t or_prp_resolve_meta(or_prp_mesh* m, uint32_t max_contours, float respon
 	}
 	/* suppression of adjacent contours (:450-590) */
 	uint8_t* adjm = calloc((size_t)nc * nc + 1, 1);
-	uint8_t* hasadj = calloc(nc + 1, 1);
 	for (uint32_t c = 0; c < active; ++c)
 		for (uint32_t i = 0; i < counts[c]; ++i)
 			adj_contours(m, c, lists[c] + i, adjm + (size_t)c * nc);
-	for (uint32_t c = 0; c < active; ++c)
-		for (uint32_t o = 0; o < nc; ++o)
-			hasadj[c] |= adjm[(size_t)c * nc + o] && o != c;
-	uint32_t* order = malloc(4 * (nc + 1));
-	uint32_t  no = 0;
-	for (uint32_t c = 0; c < active; ++c)
-		if (hasadj[c])
-			order[no++] = c;
-	for (uint32_t i = 0; i < no; ++i) /* area descending, index ascending */
-		for (uint32_t j = i + 1; j < no; ++j)
-		{
-			const float ai = ch[order[i]].area, aj = ch[order[j]].area;
-			if (aj > ai || (aj == ai && order[j] < order[i]))
-			{
-				uint32_t t = order[i];
-				order[i] = order[j], order[j] = t;
-			}
-		}
+	uint32_t* act = calloc(nc + 1, 4);
+	float*	  area = calloc(nc + 1, 4);
+	for (uint32_t c = 0; c < nc; ++c)
+		act[c] = c, area[c] = ch[c].area;
 	uint8_t* sup = calloc(nc + 1, 1);
-	for (uint32_t i = 0; i < no; ++i)
-	{
-		const uint32_t o = order[i];
-		if (sup[o])
-			continue;
-		for (uint32_t a = 0; a < active; ++a)
-			if (a != o && adjm[(size_t)o * nc + a])
-				sup[a] = 1;
-	}
+	or_prp_suppress(nc, adjm, area, active, act, sup);
+	free(act), free(area);
 	/* responses */
 	uint32_t nr = 0;
 	for (uint32_t c = 0; c < active; ++c)
@@ -1721,7 +1918,7 @@ uint32_t or_prp_resolve_meta(or_prp_mesh* m, uint32_t max_contours, float respon
 		memcpy(choices_out, ch, sizeof(or_prp_choice) * nc);
 	for (uint32_t c = 0; c < nc; ++c)
 		free(lists[c]);
-	free(lists), free(counts), free(adjm), free(hasadj), free(order), free(sup), free(ch);
+	free(lists), free(counts), free(adjm), free(sup), free(ch);
 	return nr;
 }
 

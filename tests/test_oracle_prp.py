@@ -237,3 +237,124 @@ def test_resolve_responses_two_sheets():
     C = fl[:, 4] / fl[:, 5]
     assert np.all([np.isclose(C, c, rtol=1e-5).any() for c in Cs])
     assert (C < 0).all()
+
+
+# ---- the adaptive k-ring driver (evaluate_current_runtime_directions_template,
+# intersection_resolver2.cpp:1058-1230), pinned by the five scripted-evaluator cases of
+# UnitTest/test_prp_selection.cpp:446-516 (one direction, radii starting at 2, role
+# diameters 8 unless a case changes them).  The reference's make_response (:43-61) is
+# one hit, one covered anchor, depth 1, displacement cost 1; coverage = covered / anchors.
+def _response(objective=1.0, covered=1, anchors=1, hop_src=0, hop_dst=0):
+    return dict(hit_count=1, covered=covered, ef_count=anchors, objective=objective, depth=1.0, disp_cost=1.0,
+                max_hop_src=hop_src, max_hop_dst=hop_dst, usable=int(covered > 0 and anchors > 0))
+
+
+def _drive(evaluate, diam_src=8, diam_dst=8):
+    calls = []
+
+    def ev(combo, ks, kd):
+        calls.append((ks, kd))
+        return evaluate(len(calls) - 1, ks, kd)
+
+    r = oracle.prp_drive(1, 2, 2, diam_src, diam_dst, ev)
+    return r, calls
+
+
+def test_driver_default_expansion_is_frontier_detachment():
+    # :458-469 support at hop 2 touches both frontiers at k = 2, detaches at k = 4: two
+    # evaluations, then eligible although coverage was full from the start
+    r, calls = _drive(lambda i, ks, kd: _response(hop_src=2, hop_dst=2))
+    assert len(calls) == 2 and calls == [(2, 2), (4, 4)]
+    assert r["eligible"][0] and r["selected"] == 0 and not r["no_response"]
+
+
+def test_driver_expands_only_the_touching_role():
+    # :470-488 source support touches its frontier (hop 2 at k 2), target detached (hop 1):
+    # only the source radius grows; the detached partial-coverage response is eligible
+    r, calls = _drive(lambda i, ks, kd: _response(covered=1, anchors=2, hop_src=2 if i == 0 else 3, hop_dst=1))
+    assert calls == [(2, 2), (4, 2)]
+    assert r["eligible"][0] and r["selected"] == 0
+    # a wrong rule that grows both roles whenever one touches would give (4, 4)
+
+
+def test_driver_empty_rays_terminate_at_the_mesh_bound():
+    # :489-497 no hits at any radius: 2 -> 4 -> 8 (the diameter), three evaluations,
+    # finished without a response; zero anchors are not coverage
+    r, calls = _drive(lambda i, ks, kd: dict())
+    assert calls == [(2, 2), (4, 4), (8, 8)]
+    assert r["no_response"] and r["selected"] == -1 and not r["eligible"][0] and r["converged"][0]
+
+
+def test_driver_exhausted_role_does_not_shrink():
+    # :498-505 source diameter 0: the source radius stays 2 while the target expands
+    r, calls = _drive(lambda i, ks, kd: dict(), diam_src=0)
+    assert calls == [(2, 2), (2, 4), (2, 8)]
+    assert all(ks == 2 for ks, _ in calls) and r["no_response"]
+
+
+def test_driver_exhausted_and_detached_roles_finish_at_once():
+    # :506-516 source diameter 2 (exhausted at k 2), target detached: one evaluation, eligible
+    r, calls = _drive(lambda i, ks, kd: _response(covered=1, anchors=2, hop_src=2, hop_dst=1), diam_src=2)
+    assert calls == [(2, 2)] and r["eligible"][0]
+
+
+def test_driver_prunes_directions_that_cannot_win():
+    # two directions: 0 is eligible at once with objective 1 (coverage 1); 1 touches its
+    # frontier with objective 4 (>= the best) and is pruned instead of expanded; a third
+    # touching direction with objective 0.25 keeps expanding until it detaches
+    def ev(combo, ks, kd):
+        if combo == 0:
+            return _response(objective=1.0)
+        if combo == 1:
+            return _response(objective=4.0, hop_src=ks, hop_dst=ks)
+        return _response(objective=0.25, hop_src=3, hop_dst=3)
+
+    seen = []
+    r = oracle.prp_drive(3, 2, 2, 8, 8, lambda c, ks, kd: (seen.append((c, ks, kd)), ev(c, ks, kd))[1])
+    assert r["pruned"][1] and not r["converged"][1]
+    assert (1, 4, 4) not in seen and (2, 4, 4) in seen
+    assert r["eligible"][2] and r["selected"] == 2 and tuple(r["radii"][2]) == (4, 4)
+
+
+# ---- contour suppression (prp_select_pre_finalize_contours_to_suppress,
+# intersection_resolver2.cpp:514-582), the three cases of test_prp_selection.cpp:426-444
+def test_suppression_one_way_adjacency_keeps_its_owner():
+    assert oracle.prp_suppress([[1], []], [1.0, 10.0], [0, 1]) == [1]
+
+
+def test_suppression_mutual_adjacency_keeps_the_larger_area():
+    assert oracle.prp_suppress([[1], [0]], [1.0, 2.0], [0, 1]) == [0]
+
+
+def test_suppression_equal_areas_keep_the_smaller_index():
+    assert oracle.prp_suppress([[1], [0]], [2.0, 2.0], [0, 1]) == [1]
+
+
+def test_suppression_skips_suppressed_owners_and_inactive_contours():
+    # 2 (largest) suppresses 1; 1 would suppress 0 but is itself suppressed; 3 is not active
+    assert oracle.prp_suppress([[], [0], [1, 3], []], [1.0, 5.0, 9.0, 0.0], [0, 1, 2]) == [1]
+
+
+# ---- penetration summary (prp_eval_penetration_v2, intersection_resolver2.cpp:1883-1954),
+# the aligned / opposing case of test_prp_selection.cpp:530-555: one VF hit from vertex 0
+# onto face (1, 2, 3) at barycentrics (0.2, 0.3), direction and vertex normals (0, 1, 0),
+# distance 0.1; vertex areas 3, face area 2, masses 1
+def _pen(face_normal):
+    h = np.zeros(1, oracle.PRP_HIT)
+    h["type"], h["src"], h["dst"], h["b0"], h["b1"], h["dist"], h["valid"] = 0, 0, 0, 0.2, 0.3, 0.1, 1
+    up = [0.0, 1.0, 0.0]
+    return oracle.prp_penetration(h, [2.0], [], [3.0] * 4, [1.0] * 4, [face_normal], [], [up] * 4, [],
+                                  [[1, 2, 3]], up)
+
+
+def test_penetration_summary_known_answers():
+    aligned, opposing = _pen([0.0, 1.0, 0.0]), _pen([0.0, -1.0, 0.0])
+    # volume = A |n . r| d = 3 * 1 * 0.1; objective = A depth^2 = 3 * 0.01 (no normal factor)
+    assert abs(aligned["volume"] - 0.3) < 1e-6 and abs(aligned["objective"] - 0.03) < 1e-6
+    assert aligned["depth"] == np.float32(0.1) and aligned["area"] == 3.0 and aligned["hit_count"] == 1
+    # the target's normal (flip metadata) does not change the displacement cost
+    assert abs(aligned["disp_cost"] - opposing["disp_cost"]) < 1e-6
+    # compute_effective_inverse_mass_weight (intersection_resolver_helper.cpp:611-626) for a VF
+    # hit: 1/m_v + sum b_i^2 / m_i with barycentrics (0.2, 0.3, 0.5) -> 1.38;
+    # cost = (d + 3e-3)^2 / 1.38
+    assert abs(aligned["disp_cost"] - 0.103 ** 2 / 1.38) < 1e-7
