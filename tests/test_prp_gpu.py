@@ -3,7 +3,14 @@ normals, per-contour hop / rest-geodesic distances, directions, the raw and
 kept ray-hit sets of (contour, direction, k) slots (bit-exact as sets), the
 J(r) summaries, and the full resolve (chosen directions, k-rings, response
 records).  The oracle gets the GPU's directions (an FP64 eigen-decomposition
-on both sides; they agree to 1e-6) so that hit sets can be compared exactly."""
+on both sides; they agree to 1e-6) so that hit sets can be compared exactly.
+
+Direction counts: 10 (the reference's generator), 16 (the two-sheet
+configuration) and 64 (the folded-cloth configuration); directions beyond the
+tenth are this build's Fibonacci-sphere extension, restated by the oracle
+(their parity against the reference is unpinned: it has no such directions).
+Scenes: two crossing 32 x 32 sheets, a 48 x 48 fold (29 contours), and a batch
+of four contours of the 256 x 256 fold (130,050 triangles) at radius 8."""
 import numpy as np
 import pytest
 import torch
@@ -52,10 +59,14 @@ def _hitset(h):
             for a in h}
 
 
-@pytest.fixture(scope="module", params=["two_sheets", "folded_small"])
+SCENES = [("two_sheets", 10), ("two_sheets", 16), ("two_sheets", 64), ("folded_small", 10), ("folded_small", 64)]
+
+
+@pytest.fixture(scope="module", params=SCENES, ids=[f"{n}-{d}dirs" for n, d in SCENES])
 def scene(request, ctx):
-    f = prp_fixture(request.param)
-    return GpuScene(ctx, f)
+    name, dirs = request.param
+    f = prp_fixture(name, total_dirs=dirs)
+    return GpuScene(ctx, f, total_dirs=dirs)
 
 
 def test_contours_and_normals(scene):
@@ -119,9 +130,10 @@ def test_slot_hits_match_oracle(scene, kmode):
     assert total > 0
 
 
-def test_resolve_matches_oracle(ctx):
-    f = prp_fixture("two_sheets")
-    g = GpuScene(ctx, f, max_contours=256, response_depth=1.0, kappa=1e8)
+@pytest.mark.parametrize("name,dirs", SCENES, ids=[f"{n}-{d}dirs" for n, d in SCENES])
+def test_resolve_matches_oracle(ctx, name, dirs):
+    f = prp_fixture(name, total_dirs=dirs)
+    g = GpuScene(ctx, f, max_contours=256, response_depth=1.0, kappa=1e8, total_dirs=dirs)
     choices, recs, stats = g.prp.resolve(g.x, g.ct, g.det.face_tree, g.etree)
     o = f["oracle"]
     for c in range(f["nc"]):
@@ -145,3 +157,54 @@ def test_resolve_matches_oracle(ctx):
     assert np.array_equal(A[:, :4], B[:, :4])
     fa, fb = A[:, 4:12].view(np.float32), B[:, 4:12].view(np.float32)
     assert np.allclose(fa, fb, rtol=1e-5, atol=1e-7)
+
+
+def test_folded_256_contour_batch(ctx):
+    """Four contours of the 256 x 256 fold (the folded-cloth configuration's mesh,
+    64 directions) at radius 8: raw and kept hit sets and summaries of all 256 slots."""
+    import oracle as orc
+    from untangling26_amd import initializer
+
+    m = scenes.folded_cloth(0, n=256)
+    rest = scenes.folded_cloth_rest(n=256)
+    st = initializer.setup_cloth(m, x_rest=rest)
+    d, th = scenes.per_vertex(m)
+    pairs, ids = orc.ef_detect(m.x, m.edges, m.faces, d, th)
+    oid, opr, adj, adjn = orc.ef_preprocess(ids, pairs, m.edges, m.faces, st.face_adj_edges, st.edge_adj_faces)
+    nc, ci, mi = orc.contours_dfs(opr, adj, adjn)
+    o = orc.PrpOracle(m.x, rest, st.edges, st.faces, st.face_adj_edges, st.edge_adj_faces, st.rest_vert_area,
+                      st.rest_edge_area, st.rest_face_area, st.mass)
+    n = 4
+    o.prepare(opr, adj, adjn, ci, mi, nc, 64, n_prepare=n)
+    f = dict(mesh=m, setup=st, rest=np.asarray(rest, np.float32), ids=oid, pairs=opr, nc=nc, oracle=o)
+    g = GpuScene(ctx, f, total_dirs=64)
+    assert g.ct.num_total == len(oid) and g.ct.num_contours == nc
+    g.prp.prepare(g.x, g.ct, 0, n)
+    slots, kr = [], np.full(2 * n, 8, np.uint32)
+    for c in range(n):
+        dirs = g.prp.directions(c)
+        odirs, odiam, _ = o.state(c)
+        assert dirs.shape == (64, 3) and np.abs(dirs - odirs).max() <= 1e-5
+        for role in (0, 1):
+            hop, geo, diam = g.prp.distances(c, role)
+            assert np.array_equal(hop, o.hop(c, role)) and diam == odiam[role]
+        o.set_dirs(c, dirs)
+        for ci_ in range(64):
+            slots.append((c, ci_, 8, 8, dirs[ci_], 0.0, 0.0, 0))
+    slots = np.array(slots, PRP_SLOT)
+    summ, hits = g.prp.eval(g.det.face_tree, g.etree, slots, kr)
+    total = 0
+    for j, sl in enumerate(slots):
+        oh, om = o.eval_slot(int(sl["contour"]), sl["dir"], 8, 8)
+        gh = hits[hits["slot"] == j]
+        assert _hitset(gh) == _hitset(oh), (j, len(gh), len(oh))
+        s = summ[j]
+        kept = gh[gh["root"] == s["root"]] if s["hit_count"] else gh[:0]
+        assert _hitset(kept) == _hitset(oh[oh["valid"] != 0]), j
+        for k in ("hit_count", "covered", "ef_count", "max_hop_src", "max_hop_dst", "usable"):
+            assert s[k] == om[k], (j, k, s[k], om[k])
+        assert s["depth"] == om["depth"]
+        for k in ("area", "volume", "objective", "disp_cost"):
+            assert np.isclose(s[k], om[k], rtol=1e-6, atol=1e-12), (j, k, s[k], om[k])
+        total += len(oh)
+    assert total > 100000
