@@ -61,6 +61,7 @@ def make_solver(args, precision):
     s.register_world_data(s.create_world_data_from_array("folded_cloth", m.x, m.faces, rest))
     cfg = s.get_config()
     # untangling26/cli.py:93-118
+    cfg.use_untangling = True
     cfg.use_quasi_static_mode = True
     cfg.gravity = (0.0, 0.0, 0.0)
     cfg.contact_energy_type = 0
@@ -330,7 +331,7 @@ def main():
             "pcg_speedup_mixed_vs_fp64": None if fp64 is None else round(fp64["ms_pcg"] / max(mixed["ms_pcg"], 1e-9), 3),
             "ef_pairs_first_last": [mixed["ef_pairs"][0], mixed["ef_pairs"][-1]],
             "contours_first_last": [mixed["contours"][0], mixed["contours"][-1]],
-            "final_num_pairs": int(data["num_pairs"]),
+            "final_num_pairs": int(data["num_pairs"][0]),
             "stage_ms_mixed": {k: round(v, 3) for k, v in mixed.items() if k.startswith("ms_")},
             "stage_ms_fp64": None if fp64 is None else {k: round(v, 3) for k, v in fp64.items()
                                                          if k.startswith("ms_")},

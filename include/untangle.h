@@ -148,7 +148,10 @@ enum {
 	UT_CT_ADJ_N = 5,		 /* uint32[total] neighbour count of each pair */
 	UT_CT_INCONSISTENT = 6,	 /* uint32[total] 1 at the smallest pair of an odd-cycle component */
 	UT_CT_COMP_MIN = 7,		 /* uint32[total] smallest pair index of the pair's contour */
-	UT_CT_ADJ_OFF = 8		 /* uint32[total+1] start of each pair's neighbours in UT_CT_ADJ */
+	UT_CT_ADJ_OFF = 8,		 /* uint32[total+1] start of each pair's neighbours in UT_CT_ADJ */
+	UT_CT_CONTOUR_TYPE = 9,	 /* uint32[contours] classify_contour_type (intersection_resolver2.cpp:311-375):
+							    0 undefined, 1 closed, 2 eight, 3 LL, 4 BLI, 5 cross, 6 BIBI, 7 BBII */
+	UT_CT_MESH_CLOSED = 10	 /* uint32[2*contours] mesh_is_closed of roles 0 and 1 (:2823-2841) */
 };
 /* Mesh topology (device arrays kept by reference): face_adj_edges uint32[3F]
  * (edge between face[j] and face[j+1], init_mesh_data.cpp:419-483) and
@@ -423,6 +426,16 @@ UT_API int ut_solver_contour_get(ut_solver* s, int what, void* dst);
  * PCG runs in FP64). */
 enum { UT_PROFILE_PCG_SPMV = 0 };
 UT_API int ut_solver_profile(ut_solver* s, int what, uint32_t reps, float* avg_ms);
+/* Positions of the solver (device float[3V]): which 0 = current, 1 = start of
+ * the last Newton iteration (the positions the last detection, contours and
+ * PRP saw; sa_x_iter_start), 2 = start of the last step (sa_x_step_start). */
+UT_API int ut_solver_get_positions(ut_solver* s, int which, float* x);
+/* The PRP response pairs of the last Newton iteration (target_point_template_pairs
+ * and target_point_template_pairs_indices, intersection_resolver2.cpp:2105-2221):
+ * *count = their number; the first min(count, cap) records (64 B each) and index
+ * quadruples (source, target, packed region 2 * 256 * contour, kind 0 VF / 1 FV /
+ * 2 EE) are copied to the device arrays records and meta. */
+UT_API int ut_solver_responses(ut_solver* s, void* records, uint32_t* meta, uint32_t cap, uint32_t* count);
 /* Measurement hook (no counterpart in the reference): solves the last Newton
  * iteration's system again -- the same assembled matrix, contact blocks and
  * right-hand side -- at `precision` (UT_PCG_FP64, or UT_PCG_MIXED when the

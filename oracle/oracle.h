@@ -140,6 +140,9 @@ void or_assemble(uint32_t nv, uint32_t nf, uint32_t nh, const float* x, const do
 uint32_t or_pcg_contacts(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const uint32_t* contacts,
 	uint32_t ncontacts, const double* b, double* x, int precond, uint32_t max_iters, double rel_tol, double* out);
 
+void or_classify_contours(uint32_t N, const uint32_t* pairs, const uint8_t* is_boundary, const uint32_t* adj_n,
+	const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t* type, uint32_t* closed);
+
 /* PRP untangling path (oracle_prp.c).  Opaque mesh/contour state. */
 typedef struct or_prp_mesh or_prp_mesh;
 or_prp_mesh* or_prp_mesh_create(uint32_t nv, uint32_t ne, uint32_t nf, const float* x, const float* x_rest,

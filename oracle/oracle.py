@@ -105,6 +105,7 @@ def lib():
         L.or_prp_suppress.argtypes = [C.c_uint32, p, p, C.c_uint32, p, p]
         L.or_prp_suppress.restype = C.c_uint32
         L.or_prp_penetration.argtypes = [C.c_uint32, p, p, p, p, p, p, p, p, p, p, p, p]
+        L.or_classify_contours.argtypes = [C.c_uint32, p, p, p, p, p, C.c_uint32, p, p]
         L.or_prp_hit_attached.argtypes = [p, C.c_uint32, p]
         L.or_prp_hit_attached.restype = C.c_int
         L.or_prp_vertex_flags.argtypes = [p, C.c_uint32, p, C.c_uint32]
@@ -326,6 +327,26 @@ def contours_dfs(pairs, adj, adjn):
     mi = np.zeros(max(N, 1), np.uint32)
     nc = lib().or_contours_dfs(N, _p(pairs), _p(_c(adj, np.uint32)), _p(_c(adjn, np.uint32)), _p(ci), _p(mi))
     return int(nc), ci[:N], mi[:N]
+
+
+def classify_contours(pairs, is_boundary, adjn, contour_index, mesh_index, nc):
+    """(ContourType per contour, mesh_is_closed [nc, 2]) of classify_contour_type."""
+    pairs = _c(pairs, np.uint32).reshape(-1, 12)
+    N = pairs.shape[0]
+    t = np.zeros(max(nc, 1), np.uint32)
+    cl = np.zeros((max(nc, 1), 2), np.uint32)
+    lib().or_classify_contours(N, _p(pairs if N else np.zeros((1, 12), np.uint32)),
+                               _p(_c(is_boundary, np.uint8) if N else np.zeros(1, np.uint8)),
+                               _p(_c(adjn, np.uint32) if N else np.zeros(1, np.uint32)),
+                               _p(_c(contour_index, np.uint32) if N else np.zeros(1, np.uint32)),
+                               _p(_c(mesh_index, np.uint32) if N else np.zeros(1, np.uint32)), nc, _p(t), _p(cl))
+    return t[:nc], cl[:nc]
+
+
+def pair_is_boundary(ids, eaf):
+    """1 where the pair's edge (ids[:, 0]) has at most one adjacent face."""
+    e = np.asarray(ids, np.uint32).reshape(-1, 2)[:, 0]
+    return (np.asarray(eaf, np.uint32).reshape(-1, 2)[e, 1] == 0xFFFFFFFF).astype(np.uint8)
 
 
 def graph_bfs(row_ptr, col, w):

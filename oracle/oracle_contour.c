@@ -571,3 +571,53 @@ int or_graph_bfs(uint32_t n, const uint32_t* row_ptr, const uint32_t* col, const
 	free(visited);
 	return any;
 }
+
+/* classify_contour_type (intersection_resolver2.cpp:311-375) and the mesh_is_closed
+ * roles (:2823-2841), contour by contour over its pairs in index order.  pairs:
+ * EfPair records (12 words, edge vertices in words 0 and 1: a loop-vertex pair has
+ * them equal, collision_data.h:199); is_boundary[p]: the pair's edge has at most
+ * one adjacent face (:2814-2818); adj_n[p]: its EF-adjacency degree.  type[nc]
+ * receives the ContourType (0 undefined, 1 closed, 2 eight, 3 LL, 4 BLI, 5 cross,
+ * 6 BIBI, 7 BBII), closed[2 nc] the roles' mesh_is_closed. */
+void or_classify_contours(uint32_t N, const uint32_t* pairs, const uint8_t* is_boundary, const uint32_t* adj_n,
+	const uint32_t* contour_index, const uint32_t* mesh_index, uint32_t nc, uint32_t* type, uint32_t* closed)
+{
+	for (uint32_t c = 0; c < nc; ++c)
+	{
+		uint32_t nb = 0, nl = 0, front = 0, back = 0;
+		int		 is_closed = 1;
+		int		 has_role[2] = { 0, 0 }, nonunit[2] = { 1, 1 };
+		for (uint32_t p = 0; p < N; ++p)
+		{
+			if (contour_index[p] != c)
+				continue;
+			const int loop = pairs[12 * (size_t)p] == pairs[12 * (size_t)p + 1];
+			if (loop)
+				++nl;
+			if (is_boundary[p])
+			{
+				if (nb == 0)
+					front = p;
+				back = p;
+				++nb;
+				is_closed = 0;
+			}
+			if (!loop && adj_n[p] == 1)
+				is_closed = 0;
+			const uint32_t r = mesh_index[p] & 1u;
+			has_role[r] = 1;
+			if (!loop && adj_n[p] == 1)
+				nonunit[r] = 0;
+		}
+		uint32_t t = 0;
+		if (nb == 0)
+			t = !is_closed ? 0 : nl == 0 ? 1 : nl == 1 ? 2 : nl == 2 ? 3 : 0;
+		else if (nl == 0)
+			t = nb == 2 ? (mesh_index[front] == mesh_index[back] ? 7 : 6) : 0;
+		else if (nl == 1)
+			t = nb == 1 ? 4 : nb == 2 ? 5 : 0;
+		type[c] = t;
+		closed[2 * c] = has_role[0] && nonunit[0];
+		closed[2 * c + 1] = has_role[1] && nonunit[1];
+	}
+}

@@ -10,7 +10,7 @@ import oracle
 from contour_cases import (SMALL_FIXED_COMPONENTS, SMALL_FIXED_EDGES, SMALL_FIXED_N, chains, components_from_min,
                            grid, make_undirected_csr, random_sparse)
 from untangling26_amd import scenes, topology
-from untangling26_amd.contours import (CT_ADJ, CT_ADJ_N, CT_ADJ_OFF, CT_CONTOUR_INDEX, CT_INCONSISTENT,
+from untangling26_amd.contours import (CT_ADJ, CT_CONTOUR_TYPE, CT_MESH_CLOSED, CT_ADJ_N, CT_ADJ_OFF, CT_CONTOUR_INDEX, CT_INCONSISTENT,
                                        CT_MESH_INDEX, CT_PAIRS, CT_PAIR_IDS, ContourBuilder, graph_components)
 from untangling26_amd.intersection import EfDetector, add_ef_from_proximity_pairs
 from untangling26_amd.lbvh import LBVH, TREE_EDGE, TREE_FACE
@@ -98,6 +98,12 @@ def _check_build(cb, N, nc, in_ids, in_pairs, m, fae, eaf, filter_degenerate=Fal
         first.setdefault(int(oci[i]), i)
     consistent = np.array([inc[first[int(c)]] == 0 for c in oci], bool)
     assert np.array_equal(mi[consistent], omi[consistent])
+    # classify_contour_type / mesh_is_closed on the contours whose parity is defined
+    ot, ocl = oracle.classify_contours(opr, oracle.pair_is_boundary(oid, eaf), oadjn, oci, omi, onc)
+    gt, gcl = u32(cb.get(CT_CONTOUR_TYPE)), u32(cb.get(CT_MESH_CLOSED)).reshape(-1, 2)
+    ok = np.array([inc[first[c]] == 0 for c in range(onc)], bool)
+    assert gt.shape == (onc,) and np.array_equal(gt[ok], ot[ok])
+    assert np.array_equal(gcl[ok], ocl[ok])
     return kept
 
 

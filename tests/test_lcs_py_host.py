@@ -1,0 +1,89 @@
+"""Host logic of the lcs_py mirror, on the CPU (no device calls): SceneParams
+defaults and the Float3 gravity the CLI assigns component-wise (cli.py:100), the
+features this build rejects, WorldData auto-scaling (world_data.cpp:336-383) and
+world transform (Core/affine_position.h:40-100), and the .state text format
+(solver_interface.cpp:772-803, :969-1063)."""
+import numpy as np
+import pytest
+
+from untangling26_amd import lcs_py
+
+
+def test_scene_params_defaults_and_float3_gravity():
+    c = lcs_py.SceneParams()
+    # scene_params.h defaults
+    assert c.use_untangling is False and c.use_untangling_PRP is False and c.ignore_near_zero_dist_pairs is True
+    assert c.pcg_iter_count == 100 and c.contact_energy_type == 1 and c.nonlinear_iter_count == 1
+    assert c.untangling_response_depth == 3e-3 and c.untangling_process_contours_count == 256
+    assert c.num_substep == 1 and tuple(c.gravity) == (0.0, -9.8, 0.0)
+    # cli.py:100 assigns the components of the bound float3
+    c.gravity.x = c.gravity.y = c.gravity.z = 0.0
+    assert tuple(c.gravity) == (0.0, 0.0, 0.0)
+    c.gravity = (1.0, 2.0, 3.0)
+    assert isinstance(c.gravity, lcs_py.Float3) and c.gravity.z == 3.0
+    assert c.get_substep_dt() == c.implicit_dt
+
+
+@pytest.mark.parametrize("field,value", [("use_floor", True), ("use_untangling_ICM", True),
+                                         ("use_untangling_GIA", True), ("num_substep", 2),
+                                         ("PRP_direction_optimization_iterations", 3),
+                                         ("use_energy_linesearch", True)])
+def test_unsupported_features_are_named(field, value):
+    c = lcs_py.SceneParams()
+    assert c.unsupported() == []
+    setattr(c, field, value)
+    assert c.unsupported()
+
+
+def test_unsupported_materials_raise():
+    w = lcs_py.WorldData("a", np.zeros((3, 3)), [[0, 1, 2]])
+    with pytest.raises(RuntimeError):
+        w.set_physics_material_rigid(thickness=1e-3, stiffness=1e4)
+    w.set_physics_material_cloth(thickness=2e-3, d_hat=5e-4)
+    assert w.material.thickness == 2e-3 and w.material.d_hat == 5e-4
+
+
+def test_auto_scaling_rule():
+    x = np.array([[0, 0, 0], [3, 0, 0], [0, 1, 0]], np.float32)
+    w = lcs_py.WorldData("a", x, [[0, 1, 2]])
+    w.set_auto_scaling(True)
+    # span 3 m > 1.5 m: scaled about the box centre (1.5, 0.5, 0) to a 0.5 m span
+    want = np.array([1.5, 0.5, 0.0]) + (x - np.array([1.5, 0.5, 0.0])) * (0.5 / 3.0)
+    assert np.allclose(w.vertices, want, atol=1e-6)
+    w.set_auto_scaling(False)
+    assert np.array_equal(w.vertices, x)
+    small = lcs_py.WorldData("b", x * 0.4, [[0, 1, 2]]).set_auto_scaling(True)
+    assert np.array_equal(small.vertices, x * np.float32(0.4))  # span 1.2 m: unchanged
+
+
+def test_world_transform_convention():
+    w = lcs_py.WorldData("a", np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [1.0, 0.0, 0.0]]), [[0, 1, 2]])
+    # rotate(r) = X(r.x) Y(r.y) Z(r.z) with X(a) rows (1,0,0), (0,cos,sin), (0,-sin,cos):
+    # a quarter turn about x sends +y to -z and +z to +y
+    w.set_rotation(np.pi / 2, 0.0, 0.0)
+    p = w.world_positions()
+    assert np.allclose(p[0], [0, 0, -1], atol=1e-6) and np.allclose(p[1], [0, 1, 0], atol=1e-6)
+    # Z(a) rows (cos,-sin,0), (sin,cos,0): a quarter turn about z sends +x to +y
+    w.set_rotation(0.0, 0.0, np.pi / 2)
+    assert np.allclose(w.world_positions()[2], [0, 1, 0], atol=1e-6)
+    # scale, then rotation, then translation
+    w.set_rotation(0.0, 0.0, 0.0).set_scale(2.0).set_translation(1.0, 2.0, 3.0)
+    assert np.allclose(w.world_positions()[2], [3, 2, 3])
+
+
+def test_state_text_format_and_round_trip():
+    x = np.array([[0.5, 1e-7, 123456789.0], [1.0 / 3.0, -2.0, 0.0]])
+    v = np.array([[0.0, 0.25, -1e-9], [7.0, 8.0, 9.0]])
+    t = lcs_py.state_text(x, v)
+    # C++ ostream defaults: %g with 6 significant digits
+    assert t.splitlines() == ["o State", "v 0.5 1e-07 1.23457e+08", "v 0.333333 -2 0", "o Velocity",
+                              "v 0 0.25 -1e-09", "v 7 8 9"]
+    q, qv = lcs_py.parse_state_text(t + "\n# a comment line\n")
+    assert np.allclose(q, x, rtol=1e-5) and np.allclose(qv, v, rtol=1e-5)
+
+
+def test_read_obj(tmp_path):
+    p = tmp_path / "m.obj"
+    p.write_text("# c\no m\nv 0 0 0\nv 1 0 0\nv 0 1 0\nv 1 1 0\nvt 0 0\nf 1/1 2/1 3/1 4/1\n")
+    v, f = lcs_py.read_obj(p)
+    assert v.shape == (4, 3) and f.tolist() == [[0, 1, 2], [0, 2, 3]]

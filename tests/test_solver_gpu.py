@@ -28,6 +28,7 @@ def _solver(name="two_sheets", precision=PCG_MIXED, n=None, untangle=False, iter
     s.init_device("hip")
     s.register_world_data(s.create_world_data_from_array(name, m.x, m.faces, rest))
     cfg = s.get_config()
+    cfg.use_untangling = True  # EF detection and contours every step
     cfg.pcg_precision = precision
     cfg.pcg_iter_count = iters
     if untangle:
@@ -67,9 +68,9 @@ def test_step_runs_and_matches_oracle_contours(ctx, name):
     oid, opr, oadj, oadjn = oracle.ef_preprocess(ids, pairs, m.edges, m.faces, fae, eaf)
     nc, ci, mi = oracle.contours_dfs(opr, oadj, oadjn)
     data = s.get_intersection_contour_data()
-    assert data["num_pairs"] == len(oid) and data["num_contours"] == nc
+    assert data["num_pairs"][0] == len(oid) and data["num_contours"][0] == nc
     assert np.array_equal(data["ef_indices"], oid)
-    assert np.array_equal(data["contour_index"], ci)
+    assert np.array_equal(data["ef_pair_contour_index"], ci)
     verts, tris = s.get_sim_result()
     assert verts[0].shape == (m.nv, 3) and np.array_equal(tris[0], m.faces)
 
@@ -97,10 +98,10 @@ def test_untangling_steps_reduce_intersections(ctx):
     for _ in range(12):
         rep = s.physics_step_gpu()
         data = s.get_intersection_contour_data()
-        counts.append(data["num_pairs"])
+        counts.append(int(data["num_pairs"][0]))
         responses.append(rep["prp_responses"])
         assert np.isfinite(s.positions().cpu().numpy()).all()
-        if data["num_pairs"] == 0:
+        if counts[-1] == 0:
             break
     assert responses[0] > 0
     # the oracle pipeline separates these sheets in one step (tests/test_untangle_cpu.py)

@@ -48,6 +48,8 @@ SIGNATURES = {
     "ut_contours_get": (C.c_int, [vp, C.c_int, vp]),
     "ut_graph_components": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp, vp]),
     "ut_solver_pcg_resolve": (C.c_int, [vp, C.c_int, vp, vp]),
+    "ut_solver_get_positions": (C.c_int, [vp, C.c_int, vp]),
+    "ut_solver_responses": (C.c_int, [vp, vp, vp, C.c_uint32, vp]),
     "ut_graph_components_static": (C.c_int, [vp, C.c_uint32, vp, vp, vp, vp, vp, vp]),
     "ut_pcg_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, vp, vp, vp, C.POINTER(vp)]),
     "ut_pcg_destroy": (C.c_int, [vp]),

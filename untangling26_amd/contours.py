@@ -16,7 +16,10 @@ from ._lib import lib, ptr
 from .runtime import Context
 
 (CT_PAIR_IDS, CT_PAIRS, CT_CONTOUR_INDEX, CT_MESH_INDEX, CT_ADJ, CT_ADJ_N, CT_INCONSISTENT, CT_COMP_MIN,
- CT_ADJ_OFF) = range(9)
+ CT_ADJ_OFF, CT_CONTOUR_TYPE, CT_MESH_CLOSED) = range(11)
+
+# ContourType (intersection_resolver2.h:27-37)
+CONTOUR_TYPES = ("Undifined", "Closed", "Eight", "LL", "BLI", "Cross", "BIBI", "BBII")
 
 
 class ContourBuilder:
@@ -56,7 +59,8 @@ class ContourBuilder:
     def get(self, what: int):
         n = self.num_total
         shape = {CT_PAIR_IDS: (n, 2), CT_PAIRS: (n, 12), CT_ADJ: (self.counts()[3],),
-                 CT_ADJ_OFF: (n + 1 if n else 0,)}.get(what, (n,))
+                 CT_ADJ_OFF: (n + 1 if n else 0,), CT_CONTOUR_TYPE: (self.num_contours,),
+                 CT_MESH_CLOSED: (self.num_contours, 2)}.get(what, (n,))
         out = torch.empty(shape, dtype=torch.int32, device=self.ctx.device)
         if out.numel():
             self.ctx.check(lib().ut_contours_get(self.handle, what, ptr(out)))

@@ -490,6 +490,8 @@ static void contour_output(const ut::ContourBuilder& b, int what, const void** s
 	case UT_CT_INCONSISTENT: *src = b.comp_inconsistent.get(), *bytes = 4 * n; break;
 	case UT_CT_COMP_MIN: *src = b.comp_min.get(), *bytes = 4 * n; break;
 	case UT_CT_ADJ_OFF: *src = b.adj_off.get(), *bytes = n ? 4 * (n + 1) : 0; break;
+	case UT_CT_CONTOUR_TYPE: *src = b.contour_type.get(), *bytes = 4 * (size_t)b.num_contours; break;
+	case UT_CT_MESH_CLOSED: *src = b.mesh_closed.get(), *bytes = 8 * (size_t)b.num_contours; break;
 	default: throw ut::Error(ut::kInvalid, "unknown contour output");
 	}
 }
@@ -865,8 +867,7 @@ UT_API int ut_solver_set_params(ut_solver* s, const ut_solver_params* params)
 	return guarded(s->s.ctx,
 		[&]
 		{
-			s->s.p = to_params(*params);
-			s->s.prp.params = s->s.p.prp;
+			s->s.set_params(to_params(*params));
 		});
 }
 
@@ -1175,6 +1176,39 @@ UT_API int ut_solver_profile(ut_solver* s, int what, uint32_t reps, float* avg_m
 			const bool fp64 = s->s.p.pcg.precision == ut::kPcgFp64;
 			*avg_ms = s->s.pcg.profile_spmv(fp64 ? (const void*)s->s.vals64.get() : (const void*)s->s.vals32.get(),
 				!fp64, reps);
+		});
+}
+
+UT_API int ut_solver_get_positions(ut_solver* s, int which, float* x)
+{
+	if (!s || !x || which < 0 || which > 2)
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			const float* src = which == 0 ? s->s.x.get() : which == 1 ? s->s.x_iter_start.get() : s->s.x_step_start.get();
+			UT_HIP_CHECK(
+				hipMemcpyAsync(x, src, 12 * (size_t)s->s.d.nv, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+		});
+}
+
+UT_API int ut_solver_responses(ut_solver* s, void* records, uint32_t* meta, uint32_t cap, uint32_t* count)
+{
+	if (!s || !count)
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			const uint32_t n = s->s.last_responses;
+			*count = n;
+			const uint32_t k = std::min(n, cap);
+			if (k == 0)
+				return;
+			UT_REQUIRE(records && meta, "null destination");
+			const hipStream_t st = s->s.ctx->stream;
+			UT_HIP_CHECK(hipMemcpyAsync(records, s->s.prp.responses.get(), sizeof(ut::ContactRec) * k,
+				hipMemcpyDeviceToDevice, st));
+			UT_HIP_CHECK(hipMemcpyAsync(meta, s->s.prp.response_meta.get(), 16 * (size_t)k, hipMemcpyDeviceToDevice, st));
 		});
 }
 

@@ -484,6 +484,80 @@ __global__ void k_ct_contour_index(uint32_t N, const uint32_t* __restrict__ comp
 		out[p] = rank[comp_min[p]];
 }
 
+// ---- contour classification ---------------------------------------------------
+// classify_contour_type and the mesh_is_closed roles (intersection_resolver2.cpp:
+// 311-375, :2811-2845) need, per contour, only counts: pairs on a boundary edge
+// (at most one adjacent face), loop-vertex pairs (edge vertices equal), non-loop
+// pairs with a single EF neighbour, per role the pairs and such degree-one pairs,
+// and the meshes of the boundary pairs.  Counts are order-free, so one pass with
+// atomics gives the reference's classification.
+enum : uint32_t
+{
+	kCsBoundary = 0,
+	kCsLoop = 1,
+	kCsDeg1 = 2,
+	kCsBoundaryMeshes = 3, // bit m: a boundary pair of mesh m
+	kCsRolePairs = 4,	   // + role
+	kCsRoleDeg1 = 6,	   // + role
+	kCsWords = 8
+};
+
+__global__ void k_ct_classify_count(uint32_t N, const uint32_t* __restrict__ ids, const EfPairRec* __restrict__ pairs,
+	const uint32_t* __restrict__ contour_index, const uint32_t* __restrict__ mesh_index,
+	const uint32_t* __restrict__ adj_n, const uint32_t* __restrict__ edge_adj_faces, uint32_t* __restrict__ stats)
+{
+	const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= N)
+		return;
+	uint32_t* const st = stats + (size_t)kCsWords * contour_index[p];
+	const uint32_t	e = ids[2 * p], m = mesh_index[p] & 1u;
+	const bool		boundary = edge_adj_faces[2 * e + 1] == kNone;
+	const bool		loop = pairs[p].idx[0] == pairs[p].idx[1];
+	const bool		deg1 = !loop && adj_n[p] == 1u;
+	if (boundary)
+	{
+		atomicAdd(&st[kCsBoundary], 1u);
+		atomicOr(&st[kCsBoundaryMeshes], 1u << m);
+	}
+	if (loop)
+		atomicAdd(&st[kCsLoop], 1u);
+	if (deg1)
+	{
+		atomicAdd(&st[kCsDeg1], 1u);
+		atomicAdd(&st[kCsRoleDeg1 + m], 1u);
+	}
+	atomicAdd(&st[kCsRolePairs + m], 1u);
+}
+
+// ContourType (intersection_resolver2.h:27-37): 0 undefined, 1 closed, 2 eight,
+// 3 LL, 4 BLI, 5 cross, 6 BIBI, 7 BBII.
+__device__ __forceinline__ uint32_t contour_type(const uint32_t* st)
+{
+	const uint32_t nb = st[kCsBoundary], nl = st[kCsLoop];
+	const bool	   closed = nb == 0u && st[kCsDeg1] == 0u;
+	if (nb == 0u)
+		return !closed ? 0u : nl == 0u ? 1u : nl == 1u ? 2u : nl == 2u ? 3u : 0u;
+	if (nl == 0u)
+		// two boundary pairs: on one mesh (BBII) or on both (BIBI)
+		return nb == 2u ? (st[kCsBoundaryMeshes] == 3u ? 6u : 7u) : 0u;
+	if (nl == 1u)
+		return nb == 1u ? 4u : nb == 2u ? 5u : 0u;
+	return 0u;
+}
+
+__global__ void k_ct_classify(uint32_t nc, const uint32_t* __restrict__ stats, uint32_t* __restrict__ type,
+	uint32_t* __restrict__ mesh_closed)
+{
+	const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= nc)
+		return;
+	const uint32_t* st = stats + (size_t)kCsWords * c;
+	type[c] = contour_type(st);
+	// a role is closed when it owns a pair and none of its non-loop pairs has degree one
+	for (uint32_t r = 0; r < 2; ++r)
+		mesh_closed[2 * c + r] = (st[kCsRolePairs + r] > 0u && st[kCsRoleDeg1 + r] == 0u) ? 1u : 0u;
+}
+
 static int bits_for(uint32_t n)
 {
 	int b = 0;
@@ -634,6 +708,23 @@ void ContourBuilder::build(uint32_t P, const EfPairRec* in_pairs, const uint32_t
 	UT_HIP_CHECK(hipMemcpyAsync(&nc, d_contours, 4, hipMemcpyDeviceToHost, s));
 	UT_HIP_CHECK(hipStreamSynchronize(s));
 	num_contours = nc;
+	// classification
+	if (nc)
+	{
+		if (nc > cap_contours)
+		{
+			cap_contours = std::max(nc, 2 * cap_contours);
+			contour_stats.reserve((size_t)kCsWords * cap_contours);
+			contour_type.reserve(cap_contours);
+			mesh_closed.reserve(2 * (size_t)cap_contours);
+		}
+		const uint32_t m = kCsWords * nc;
+		k_fill_u32<<<div_up(m, 256), 256, 0, s>>>(contour_stats.get(), m, 0u);
+		k_ct_classify_count<<<gn, 256, 0, s>>>(N, ids.get(), pairs.get(), contour_index.get(), mesh_index.get(),
+			adj_n.get(), edge_adj_faces, contour_stats.get());
+		k_ct_classify<<<div_up(nc, 256), 256, 0, s>>>(nc, contour_stats.get(), contour_type.get(), mesh_closed.get());
+		UT_LAUNCH_CHECK();
+	}
 }
 
 } // namespace ut

@@ -45,6 +45,11 @@ struct ContourBuilder
 	DevBuf<uint32_t>  rank;		  // [total]
 	DevBuf<uint32_t>  contour_index;	// [total]
 	DevBuf<uint32_t>  comp_inconsistent; // [total] indexed by component min pair
+	// classify_contour_type / mesh_is_closed (intersection_resolver2.cpp:311-375, :2811-2845)
+	uint32_t		  cap_contours = 0;
+	DevBuf<uint32_t>  contour_stats; // [8 * contours] per-contour counts
+	DevBuf<uint32_t>  contour_type;	 // [contours] ContourType
+	DevBuf<uint32_t>  mesh_closed;	 // [2 * contours] role 0, role 1 closed
 
 	void init(Context* c, uint32_t nv, uint32_t ne, uint32_t nf, const uint32_t* edges, const uint32_t* faces,
 		const uint32_t* face_adj_edges, const uint32_t* edge_adj_faces);

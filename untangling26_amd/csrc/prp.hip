@@ -1680,7 +1680,7 @@ __global__ void k_prp_responses(uint32_t n, const PrpHit* __restrict__ h, const 
 	const float* __restrict__ cdir, const float* __restrict__ cC, const uint8_t* __restrict__ suppressed,
 	const uint32_t* __restrict__ edges, const uint32_t* __restrict__ faces, const float* __restrict__ va,
 	const float* __restrict__ ea, const float* __restrict__ fa, float kappa, ContactRec* __restrict__ out,
-	uint32_t* __restrict__ out_count, uint32_t cap)
+	uint4* __restrict__ out_meta, uint32_t* __restrict__ out_count, uint32_t cap)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	const bool	   live = i < n && !suppressed[hc[i]];
@@ -1718,6 +1718,13 @@ __global__ void k_prp_responses(uint32_t n, const PrpHit* __restrict__ h, const 
 	r.k2 = stiff;
 	r.bary[0] = x.b0, r.bary[1] = x.b1;
 	out[slot] = r;
+	// target_point_template_pairs_indices (:2159, :2178, :2196): VF (vertex, face, region, 0),
+	// FV (vertex, face, region, 1), EE (source edge, target edge, region, 2); region =
+	// prp_pack_response_region_label(c, 0, label) = 2 * 256 * c with display label 0
+	const uint32_t region = 2u * 256u * c;
+	out_meta[slot] = x.type == 0 ? make_uint4(x.src, x.dst, region, 0u)
+		: x.type == 2			 ? make_uint4(x.dst, x.src, region, 1u)
+								 : make_uint4(x.src, x.dst, region, 2u);
 }
 
 // copy the kept hits of the chosen slots (root == slot root) to the final list
@@ -2637,10 +2644,11 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 		upload(contour_dir, cdir, s);
 		upload(suppressed, sup, s);
 		responses.reserve(n_final);
+		response_meta.reserve(n_final);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, counters.get() + 3, 0u);
 		k_prp_responses<<<div_up(n_final, 256), 256, 0, s>>>(n_final, final_hits.get(), final_contour.get(),
 			contour_dir.get(), contour_c.get(), suppressed.get(), edges, faces, va, ea, fa, params.kappa,
-			responses.get(), counters.get() + 3, n_final);
+			responses.get(), response_meta.get(), counters.get() + 3, n_final);
 		UT_LAUNCH_CHECK();
 	}
 	else

@@ -201,6 +201,7 @@ struct PrpResolver
 	DevBuf<PrpHit>	 final_hits;
 	DevBuf<uint32_t> final_contour;
 	DevBuf<float>	 contour_dir;
+	DevBuf<uint4>	 response_meta; // per response: (source, target, packed region, kind)
 	DevBuf<uint8_t>	 suppressed;
 	uint32_t		 final_cap = 0;
 

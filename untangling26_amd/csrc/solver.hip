@@ -307,6 +307,7 @@ StepReport NewtonSolver::physics_step_GPU()
 		}
 		UT_HIP_CHECK(hipEventRecord(ev[2], s));
 		regions.clear();
+		last_responses = 0;
 		if (p.use_self_collision && p.use_untangling && contours.ctx)
 		{
 			contours.build(rep.ef_pairs, ef_pairs.get(), ef_ids.get(), !p.ignore_near_zero_dist_pairs);
@@ -322,6 +323,7 @@ StepReport NewtonSolver::physics_step_GPU()
 					PrpStats ps;
 					nresp = prp.resolve(x.get(), contours, face_tree, edge_tree, last_choices, &ps);
 					rep.prp_responses += nresp;
+					last_responses = nresp;
 					rep.prp_contours += ps.contours;
 					rep.prp_levels += ps.levels;
 					rep.prp_hits += ps.hits;
@@ -452,6 +454,17 @@ StepReport NewtonSolver::physics_step_GPU()
 	rep.ms_total = total;
 	rep.ms_prp = t_prp;
 	return rep;
+}
+
+void NewtonSolver::set_params(const SolverParams& np)
+{
+	UT_REQUIRE(np.prp.total_dirs >= 1 && np.prp.total_dirs <= 64, "solver: PRP directions must be in [1, 64]");
+	UT_REQUIRE(np.prp.batch_contours >= 1, "solver: PRP batch_contours must be positive");
+	UT_REQUIRE(np.pcg.check_every >= 1, "solver: PCG check_every must be >= 1");
+	if (!np.ignore_near_zero_dist_pairs && contours.ctx && !vert_adj_off.get())
+		build_vert_adj_edges();
+	p = np;
+	prp.params = p.prp;
 }
 
 PcgReport NewtonSolver::resolve_last_system(int precision, float* ms)

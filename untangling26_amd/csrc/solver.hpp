@@ -123,6 +123,13 @@ struct NewtonSolver
 	// right-hand side) at `precision` into a probe vector, leaving the solver's
 	// state alone; *ms is the solve's duration (HIP events on the stream).
 	PcgReport resolve_last_system(int precision, float* ms);
+	// New parameters between steps: validated, and the state a parameter needs is
+	// built on demand (the vertex-edge adjacency of the zero-distance EF pairs when
+	// ignore_near_zero_dist_pairs is turned off after set-up, as the reference's
+	// CLI does: cli.py:93-118 configures the solver after init_solver).
+	void set_params(const SolverParams& params);
+
+	uint32_t last_responses = 0; // PRP response records of the last Newton iteration
 
 	DevBuf<double> dx_probe;
 	bool		   has_system = false;
