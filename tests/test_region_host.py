@@ -41,7 +41,7 @@ SCENES = ("two_sheets", "folded_small")
 
 def _load(asan=False):
     sys.path.insert(0, str(ROOT / "tests" / "host"))
-    import build as host_build  # tests/host/build.py
+    import build_host as host_build  # tests/host/build_host.py
 
     libs = host_build.build()
     lib = C.CDLL(str(libs["librh_asan.so" if asan else "librh.so"]))
@@ -211,7 +211,7 @@ def test_region_dedup_keeps_every_region():
 def test_under_address_sanitizer():
     pytest.importorskip("ctypes")
     sys.path.insert(0, str(ROOT / "tests" / "host"))
-    import build as host_build
+    import build_host as host_build
 
     host_build.build()
     env = dict(os.environ, LD_PRELOAD=host_build.asan_runtime(),
