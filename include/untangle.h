@@ -290,6 +290,46 @@ UT_API int ut_toi_reset(ut_ctx* ctx, float* toi);
 UT_API int ut_contact_energy(ut_ctx* ctx, const void* pairs, const uint32_t* count, uint32_t cap, const float* x,
 	const float* d_hat, const float* offset, double kappa, int energy, double* out);
 
+/* Lagged IPC friction of contact records (Friction::ipc_barrier, friction_kernel.hpp:137-301).
+ * ut_friction_init is fn_process_collision_pair_friction's first Newton iteration
+ * (narrow_phase.cpp:1479-1491): mu_lambda = -mu k1 with mu the mean of the per-vertex
+ * coefficients of the record's vertices 0 and 2, zero relative motion; in place on
+ * records [0, min(*count, cap)).  ut_friction_lagged is its later iterations
+ * (:1492-1506): relative motion sum_k w_k (x_k - x_start_k) with the record's weights,
+ * repulsion switched off (disable_repulsion_part); out may equal pairs.  Positions are
+ * FP32 (x, x_start) or FP64 (x64, x_start64). */
+UT_API int ut_friction_init(ut_ctx* ctx, void* pairs, const uint32_t* count, uint32_t cap, const float* friction_mu);
+UT_API int ut_friction_lagged(ut_ctx* ctx, const void* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const double* x64, const float* x_start, const double* x_start64, void* out);
+
+/* Contact part of the incremental potential at x (no reference counterpart: the
+ * reference evaluates these terms inside its Newton step, fn_compute_repulsion_energy,
+ * narrow_phase.cpp:3432-3530, but exposes no gradient).  UT_TERM_BARRIER: the
+ * repulsion of every record with k2 != 0, its closest-point distance re-solved in
+ * FP64 at x, and its gradient kappa A b'(d - th) w_k n with the re-solved weights;
+ * UT_TERM_FRICTION: the lagged friction of every record with a friction multiplier
+ * and its gradient.  out: double[2 + 512] (barrier energy, friction energy, scratch);
+ * grad (optional, double[3V]) is added to. */
+#define UT_TERM_BARRIER 1
+#define UT_TERM_FRICTION 2
+typedef struct ut_potential_args {
+	const void*		pairs;
+	const uint32_t* count;
+	uint32_t		cap;
+	const float*	x;		   /* FP32 positions [3V], or */
+	const double*	x64;	   /* FP64 positions [3V] (then x_start64 as well) */
+	const float*	x_start;   /* lagged positions of the friction term */
+	const double*	x_start64;
+	const float*	d_hat;	/* [V] */
+	const float*	offset; /* [V] */
+	double			kappa;
+	int				energy; /* UT_ENERGY_* */
+	int				terms;	/* UT_TERM_* */
+	double*			grad;
+	double*			out;
+} ut_potential_args;
+UT_API int ut_contact_potential(ut_ctx* ctx, const ut_potential_args* args);
+
 /* ---- assembly of the incremental potential ------------------------------------------
  * NewtonSolver's material assembly (newton_solver.cpp:2641-2714 with
  * Energies/{soft_inertia_energy,stretch_face_energy,bending_energy_kernel}.cpp)
@@ -322,6 +362,7 @@ typedef struct ut_assemble_args {
 	double*			energy;		 /* optional double[4]: inertia, stretch, bending */
 	int				quasi_static;		  /* 1: no inertia gradient / energy (soft_inertia_energy.cpp:68-80) */
 	int				qs_no_inertia_hessian; /* with quasi_static: drop the m / h^2 diagonal as well */
+	const double*	x64;		 /* optional FP64 positions [3V]; when set they replace x */
 } ut_assemble_args;
 UT_API int ut_assemble(ut_ctx* ctx, const ut_assemble_args* args);
 
@@ -364,6 +405,8 @@ typedef struct ut_solver_desc {
 	const uint32_t* face_slots;	 /* [9F] slot of block (a, b) of face vertices a, b */
 	const uint32_t* hinge_slots; /* [16H] */
 	const float*	rest_x;		 /* [3V] rest positions for the PRP rest geodesics; NULL = x0 */
+	const float*	friction_mu; /* [V] friction coefficient per vertex (sa_contact_active_verts_friction_coeff,
+									the material's friction_mu, default 0.5); NULL: no friction */
 } ut_solver_desc;
 
 /* SceneParams fields the step uses (scene_params.h:24-157). */

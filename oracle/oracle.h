@@ -195,6 +195,24 @@ int		 or_prp_pick_component(uint32_t ncomp, const uint8_t* attached, const uint3
 uint32_t or_prp_resolve_meta(or_prp_mesh* m, uint32_t max_contours, float response_depth, float kappa,
 	void* choices_out, uint32_t* recs, uint32_t* meta, uint32_t cap);
 
+/* ---- lagged IPC friction and the contact potential (oracle_friction.c) ---- */
+/* friction_kernel.hpp:137-301: energy; grad[3] and hess[9] (optional) with respect to
+ * the relative displacement rel of a pair with normal n and multiplier mu_lambda. */
+double or_friction_eval(double mu_lambda, const double* n, const double* rel, double E, double* grad, double* hess);
+/* CollisionPairTemplate weights (collision_data.h:131-136) of a 16-word record. */
+void or_record_weights(const uint32_t* rec, double* w);
+/* The record's 3x3 block k2 n n^T + friction Hessian (K[9], optional) and its point
+ * gradient k1 n + friction gradient (g[3], optional). */
+void or_pair_block(const uint32_t* rec, double* K, double* g);
+/* narrow_phase.cpp:1479-1491: mu_lambda = -mu k1, dv = 0 (in place). */
+void or_friction_init(uint32_t* recs, uint32_t n, const float* mu);
+/* narrow_phase.cpp:1492-1506: dv = sum w (x - x_start), k1 = k2 = 0 (in place). */
+void or_friction_lagged(uint32_t* recs, uint32_t n, const double* x, const double* x_start);
+/* Barrier (terms & 1) and friction (terms & 2) energies out[2] at FP64 x; grad[3V]
+ * (optional) is added to. */
+void or_contact_potential(const uint32_t* recs, uint32_t n, const double* x, const double* x_start, const float* d_hat,
+	const float* offset, double kappa, int energy, int terms, double* grad, double* out);
+
 #ifdef __cplusplus
 }
 #endif

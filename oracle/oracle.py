@@ -61,6 +61,13 @@ def lib():
         L.or_ee_coeff.argtypes = [p, p, p, p, p]
         L.or_barrier.argtypes = [C.c_double, C.c_double, C.c_int]
         L.or_barrier.restype = C.c_double
+        L.or_friction_eval.argtypes = [C.c_double, p, p, C.c_double, p, p]
+        L.or_friction_eval.restype = C.c_double
+        L.or_record_weights.argtypes = [p, p]
+        L.or_pair_block.argtypes = [p, p, p]
+        L.or_friction_init.argtypes = [p, C.c_uint32, p]
+        L.or_friction_lagged.argtypes = [p, C.c_uint32, p, p]
+        L.or_contact_potential.argtypes = [p, C.c_uint32, p, p, p, p, C.c_double, C.c_int, C.c_int, p, p]
         L.or_accd.argtypes = [C.c_int, p, p, C.c_double]
         L.or_accd.restype = C.c_double
         L.or_toi_postprocess.argtypes = [C.c_double]
@@ -820,3 +827,61 @@ def contact_energy(records, x, d_hat, offset, kappa=1e8, energy=1):
         else:
             total += float(area) * kappa * barrier(d - th, dh)
     return total
+
+
+# ---- lagged IPC friction and the contact potential (oracle_friction.c) ----------
+FRICTION_EPS = 1e-4  # Friction::ando_barrier::friction_eps
+
+
+def friction_eval(mu_lambda, n, rel, E=FRICTION_EPS):
+    """(energy, grad[3], hess[3, 3]) of the lagged friction of one pair with respect to its
+    relative displacement rel (friction_kernel.hpp:137-301)."""
+    g = np.zeros(3)
+    h = np.zeros(9)
+    e = lib().or_friction_eval(float(mu_lambda), _p(_c(n, np.float64)), _p(_c(rel, np.float64)), float(E), _p(g),
+                               _p(h))
+    return e, g, h.reshape(3, 3)
+
+
+def pair_block(record):
+    """(K[3, 3], g[3]): the record's block k2 n n^T + friction Hessian and its point gradient."""
+    r = _c(np.asarray(record, np.uint32).reshape(16), np.uint32)
+    K = np.zeros(9)
+    g = np.zeros(3)
+    lib().or_pair_block(_p(r), _p(K), _p(g))
+    return K.reshape(3, 3), g
+
+
+def record_weights(record):
+    r = _c(np.asarray(record, np.uint32).reshape(16), np.uint32)
+    w = np.zeros(4)
+    lib().or_record_weights(_p(r), _p(w))
+    return w
+
+
+def friction_init(records, mu):
+    """First-iteration friction values (mu_lambda = -mu k1, dv = 0) of a copy of the records."""
+    R = np.array(np.asarray(records, np.uint32).reshape(-1, 16), np.uint32, order="C")
+    if len(R):
+        lib().or_friction_init(_p(R), len(R), _p(_c(mu, np.float32)))
+    return R
+
+
+def friction_lagged(records, x, x_start):
+    """Later-iteration friction records (dv = sum w (x - x_start), repulsion off), a copy."""
+    R = np.array(np.asarray(records, np.uint32).reshape(-1, 16), np.uint32, order="C")
+    if len(R):
+        lib().or_friction_lagged(_p(R), len(R), _p(_c(x, np.float64)), _p(_c(x_start, np.float64)))
+    return R
+
+
+def contact_potential(records, x, d_hat, offset, kappa=1e8, energy=1, terms=3, x_start=None):
+    """(barrier energy, friction energy, gradient[V, 3]) at FP64 x."""
+    R = _c(np.asarray(records, np.uint32).reshape(-1, 16), np.uint32)
+    X = _c(x, np.float64)
+    g = np.zeros(X.size)
+    out = np.zeros(2)
+    xs = None if x_start is None else _c(x_start, np.float64)
+    lib().or_contact_potential(_p(R), len(R), _p(X), _p(xs), _p(_c(d_hat, np.float32)),
+                               _p(_c(offset, np.float32)), float(kappa), int(energy), int(terms), _p(g), _p(out))
+    return out[0], out[1], g.reshape(-1, 3)

@@ -238,23 +238,19 @@ void or_assemble(uint32_t nv, uint32_t nf, uint32_t nh, const float* x, const do
 	}
 	for (uint32_t i = 0; i < ncontacts; ++i)
 	{
+		/* gradient w_a (k1 n + friction gradient), diagonal blocks w_a^2 (k2 n n^T + friction
+		 * Hessian): fn_perPair_assemble_gradient_hessian (narrow_phase.cpp:2630-2700) */
 		const uint32_t* r = contacts + 16 * i;
-		float			f[12];
-		memcpy(f, r + 4, 48);
-		const double n[3] = { f[0], f[1], f[2] }, k1 = f[4], k2 = f[5], b0 = f[6], b1 = f[7];
-		double		 w[4];
-		if (r[2] & 0x80000000u)
-			w[0] = 1.0, w[1] = -b0, w[2] = -b1, w[3] = b0 + b1 - 1.0;
-		else
-			w[0] = b0, w[1] = 1.0 - b0, w[2] = -b1, w[3] = b1 - 1.0;
+		double			w[4], K[9], g[3];
+		or_record_weights(r, w);
+		or_pair_block(r, K, g);
 		for (int a = 0; a < 4; ++a)
 		{
 			const uint32_t v = r[a] & 0x7FFFFFFFu;
 			for (int k = 0; k < 3; ++k)
-				grad[3 * v + k] += w[a] * k1 * n[k];
-			for (int p = 0; p < 3; ++p)
-				for (int q = 0; q < 3; ++q)
-					vals[(size_t)(3 * p + q) * nslots + diag_slot[v]] += w[a] * w[a] * k2 * n[p] * n[q];
+				grad[3 * v + k] += w[a] * g[k];
+			for (int c = 0; c < 9; ++c)
+				vals[(size_t)c * nslots + diag_slot[v]] += w[a] * w[a] * K[c];
 		}
 	}
 	if (energy)

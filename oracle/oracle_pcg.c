@@ -17,32 +17,30 @@
 static const uint32_t* g_contacts = NULL;
 static uint32_t		   g_ncontacts = 0;
 
-/* y += off-diagonal contact blocks w_i w_j k2 n n^T (i != j) times x */
+/* y += off-diagonal contact blocks w_i w_j K (i != j) times x, K the pair's block
+ * (k2 n n^T + lagged friction Hessian, or_pair_block) */
 static void contact_mv(const double* x, double* y)
 {
 	for (uint32_t c = 0; c < g_ncontacts; ++c)
 	{
 		const uint32_t* r = g_contacts + 16 * c;
-		float			f[12];
-		memcpy(f, r + 4, 48);
-		const double n[3] = { f[0], f[1], f[2] }, k2 = f[5], b0 = f[6], b1 = f[7];
-		double		 w[4], s[4], S = 0;
-		uint32_t	 v[4];
-		if (r[2] & 0x80000000u)
-			w[0] = 1.0, w[1] = -b0, w[2] = -b1, w[3] = b0 + b1 - 1.0;
-		else
-			w[0] = b0, w[1] = 1.0 - b0, w[2] = -b1, w[3] = b1 - 1.0;
+		double			w[4], K[9], P[3] = { 0, 0, 0 };
+		uint32_t		v[4];
+		or_record_weights(r, w);
+		or_pair_block(r, K, NULL);
 		for (int a = 0; a < 4; ++a)
 		{
 			v[a] = r[a] & 0x7FFFFFFFu;
-			s[a] = n[0] * x[3 * v[a]] + n[1] * x[3 * v[a] + 1] + n[2] * x[3 * v[a] + 2];
-			S += w[a] * s[a];
+			for (int k = 0; k < 3; ++k)
+				P[k] += w[a] * x[3 * v[a] + k];
 		}
 		for (int a = 0; a < 4; ++a)
 		{
-			const double cf = k2 * w[a] * (S - w[a] * s[a]);
+			double q[3];
 			for (int k = 0; k < 3; ++k)
-				y[3 * v[a] + k] += cf * n[k];
+				q[k] = P[k] - w[a] * x[3 * v[a] + k];
+			for (int k = 0; k < 3; ++k)
+				y[3 * v[a] + k] += w[a] * (K[3 * k] * q[0] + K[3 * k + 1] * q[1] + K[3 * k + 2] * q[2]);
 		}
 	}
 }

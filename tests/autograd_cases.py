@@ -35,3 +35,17 @@ def ip_case(energy=1, seed=0):
                             st.rest_face_area, 1e8, energy)
     recs = np.concatenate([vf, ee])
     return m, st, x_tilde, recs
+
+
+def records_at(st, x, energy=1):
+    """The oracle's VF + EE proximity records of setup st at FP32 positions x."""
+    x = np.asarray(x, np.float32)
+    qv = oracle.query_boxes(0, x, x, None, st.d_hat, st.offset)
+    vf, _ = oracle.dcd_list(0, oracle.overlap_pairs(qv, oracle.leaf_boxes(1, x, x, st.faces, st.offset)), x,
+                            st.edges, st.faces, st.d_hat, st.offset, st.rest_vert_area, st.rest_edge_area,
+                            st.rest_face_area, 1e8, energy)
+    qe = oracle.query_boxes(1, x, x, st.edges, st.d_hat, st.offset)
+    ee, _ = oracle.dcd_list(1, oracle.overlap_pairs(qe, oracle.leaf_boxes(2, x, x, st.edges, st.offset), cull=True),
+                            x, st.edges, st.faces, st.d_hat, st.offset, st.rest_vert_area, st.rest_edge_area,
+                            st.rest_face_area, 1e8, energy)
+    return np.concatenate([vf, ee])

@@ -57,6 +57,9 @@ SIGNATURES = {
     "ut_pcg_solve_refined": (C.c_int, [vp, vp, vp, vp, vp, vp, vp]),
     "ut_pcg_set_contacts": (C.c_int, [vp, vp, vp, C.c_uint32]),
     "ut_contact_energy": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, C.c_double, C.c_int, vp]),
+    "ut_friction_init": (C.c_int, [vp, vp, vp, C.c_uint32, vp]),
+    "ut_friction_lagged": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp]),
+    "ut_contact_potential": (C.c_int, [vp, vp]),
     "ut_solver_profile": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
     "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 10 + [C.POINTER(vp)]),
     "ut_prp_destroy": (C.c_int, [vp]),
@@ -106,7 +109,7 @@ class AssembleArgs(C.Structure):
                 ("mu_lambda", vp), ("face_slots", vp), ("hinges", vp), ("rest_angle", vp), ("bend_stiff_area", vp),
                 ("hinge_slots", vp), ("contacts", vp), ("contact_count", vp), ("contact_cap", C.c_uint32),
                 ("diag_slot", vp), ("grad", vp), ("vals", vp), ("vals32", vp), ("energy", vp),
-                ("quasi_static", C.c_int), ("qs_no_inertia_hessian", C.c_int)]
+                ("quasi_static", C.c_int), ("qs_no_inertia_hessian", C.c_int), ("x64", vp)]
 
 
 class PrpParams(C.Structure):
@@ -120,7 +123,16 @@ class SolverDesc(C.Structure):
                           "rest_vert_area", "rest_edge_area", "rest_face_area", "dm", "mu_lambda", "hinges",
                           "rest_angle", "bend_stiff_area")] + [
         ("nslices", C.c_uint32), ("nslots", C.c_uint32)] + [
-        (n, vp) for n in ("slice_off", "col", "diag_slot", "face_slots", "hinge_slots", "rest_x")]
+        (n, vp) for n in ("slice_off", "col", "diag_slot", "face_slots", "hinge_slots", "rest_x", "friction_mu")]
+
+
+class PotentialArgs(C.Structure):
+    _fields_ = [("pairs", vp), ("count", vp), ("cap", C.c_uint32), ("x", vp), ("x64", vp), ("x_start", vp),
+                ("x_start64", vp), ("d_hat", vp), ("offset", vp), ("kappa", C.c_double), ("energy", C.c_int),
+                ("terms", C.c_int), ("grad", vp), ("out", vp)]
+
+
+TERM_BARRIER, TERM_FRICTION = 1, 2
 
 
 class SolverParams(C.Structure):

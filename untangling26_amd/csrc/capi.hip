@@ -663,6 +663,57 @@ UT_API int ut_contact_energy(ut_ctx* ctx, const void* pairs, const uint32_t* cou
 		});
 }
 
+UT_API int ut_friction_init(ut_ctx* ctx, void* pairs, const uint32_t* count, uint32_t cap, const float* friction_mu)
+{
+	if (!ctx || !pairs || !count || !friction_mu)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&] { ut::friction_init(&ctx->c, reinterpret_cast<ut::ContactRec*>(pairs), count, cap, friction_mu); });
+}
+
+UT_API int ut_friction_lagged(ut_ctx* ctx, const void* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const double* x64, const float* x_start, const double* x_start64, void* out)
+{
+	if (!ctx || !pairs || !count || !out || (x64 ? !x_start64 : (!x || !x_start)))
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			ut::friction_lagged(&ctx->c, reinterpret_cast<const ut::ContactRec*>(pairs), count, cap, x, x64, x_start,
+				x_start64, reinterpret_cast<ut::ContactRec*>(out));
+		});
+}
+
+UT_API int ut_contact_potential(ut_ctx* ctx, const ut_potential_args* q)
+{
+	if (!ctx || !q || !q->pairs || !q->count || !q->out || (!q->x && !q->x64))
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			UT_REQUIRE(q->energy == ut::kEnergyQuadratic || q->energy == ut::kEnergyBarrier,
+				"contact potential: unknown energy");
+			UT_REQUIRE(q->terms > 0 && q->terms <= 3, "contact potential: terms must be UT_TERM_BARRIER and/or "
+													  "UT_TERM_FRICTION");
+			ut::PotentialArgs a;
+			a.pairs = reinterpret_cast<const ut::ContactRec*>(q->pairs);
+			a.count = q->count;
+			a.cap = q->cap;
+			a.x = q->x64 ? nullptr : q->x;
+			a.x64 = q->x64;
+			a.x_start = q->x_start;
+			a.x_start64 = q->x_start64;
+			a.d_hat = q->d_hat;
+			a.offset = q->offset;
+			a.kappa = q->kappa;
+			a.energy = q->energy;
+			a.terms = q->terms;
+			a.grad = q->grad;
+			a.out = q->out;
+			ut::contact_potential(&ctx->c, a);
+		});
+}
+
 UT_API int ut_contact_query(ut_ctx* ctx, int kind, const ut_contact_args* args, ut_lbvh* tree,
 	const uint32_t* broad, const uint32_t* broad_count, uint32_t broad_cap)
 {
@@ -797,6 +848,7 @@ UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_so
 			d.face_slots = desc->face_slots;
 			d.hinge_slots = desc->hinge_slots;
 			d.rest_x = desc->rest_x;
+			d.friction_mu = desc->friction_mu;
 			h->s.init(&ctx->c, d, to_params(*params), x0, v0);
 		});
 	if (rc != ut::kOk)
@@ -926,6 +978,7 @@ UT_API int ut_assemble(ut_ctx* ctx, const ut_assemble_args* q)
 			a.nh = q->nh;
 			a.nslots = q->nslots;
 			a.x = q->x;
+			a.x64 = q->x64;
 			a.x_tilde = q->x_tilde;
 			a.mass = q->mass;
 			a.inv_h2 = q->inv_h2;

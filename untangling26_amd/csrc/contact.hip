@@ -12,6 +12,7 @@
 #include "ut_common.hpp"
 #include "contact.hpp"
 #include "contact_math.hpp"
+#include "friction_math.hpp"
 #include "lbvh_traverse.hpp"
 
 #include <algorithm>
@@ -375,68 +376,174 @@ namespace
 {
 constexpr int kEnergyThreads = 256;
 
-// Contact part of the incremental potential at x: for every proximity record
-// the closest-point distance is recomputed in FP64 from x (pt / ee coefficients,
-// as the DCD that produced the record), then kappa * area * b(d - th, d_hat)
-// for the barrier or kappa * area * C^2 / 2, C = d - th - d_hat, for the
-// quadratic energy (narrow_phase.cpp:1091-1112 integrates to these).
-__global__ __launch_bounds__(kEnergyThreads) void k_contact_energy(const ContactRec* __restrict__ pairs,
-	const uint32_t* __restrict__ count, uint32_t cap, const float* __restrict__ x, const float* __restrict__ d_hat,
-	const float* __restrict__ offset, double kappa, int energy, double* __restrict__ part)
+// Contact part of the incremental potential at x (FP32 or FP64 positions P).
+//
+// Barrier term: for every proximity record the closest-point distance is
+// recomputed in FP64 from x (pt / ee coefficients, as the DCD that produced the
+// record), then kappa * area * b(d - th, d_hat) for the barrier or
+// kappa * area * C^2 / 2, C = d - th - d_hat, for the quadratic energy
+// (narrow_phase.cpp:1091-1112 integrates to these); its gradient
+// kappa * area * b'(d - th) w_k n with the re-solved weights w and normal n
+// (the closest point is optimal, so d's derivative is w_k n).  Records with
+// k2 = 0 (disabled, or lagged friction pairs) carry no barrier energy.
+//
+// Friction term: for every record with a friction multiplier, the lagged IPC
+// friction of the tangential displacement sum_k w_k (x_k - x_start_k) with the
+// record's weights and normal (fn_compute_repulsion_energy's friction part,
+// narrow_phase.cpp:3485-3512, and its gradient, friction_math.hpp).
+template <typename P>
+__global__ __launch_bounds__(kEnergyThreads) void k_contact_potential(const ContactRec* __restrict__ pairs,
+	const uint32_t* __restrict__ count, uint32_t cap, const P* __restrict__ x, const P* __restrict__ x_start,
+	const float* __restrict__ d_hat, const float* __restrict__ offset, double kappa, int energy, int terms,
+	double* __restrict__ grad, double* __restrict__ part)
 {
 	const uint32_t n = min(*count, cap);
-	double		   acc = 0.0;
+	double		   acc_b = 0.0, acc_f = 0.0;
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
 	{
 		const ContactRec r = pairs[i];
-		const uint32_t	 i0 = r.idx[0] & 0x7FFFFFFFu, i1 = r.idx[1] & 0x7FFFFFFFu, i2 = r.idx[2] & 0x7FFFFFFFu,
-					   i3 = r.idx[3] & 0x7FFFFFFFu;
-		const bool		 vf = (r.idx[2] & kTypeMask) != 0u;
-		v3<double>		 diff;
-		uint32_t		 ka, kb; // the two vertices whose d_hat / offset the DCD used
-		if (vf)
+		uint32_t		 v[4];
+		v3<double>		 p[4];
+		for (int k = 0; k < 4; ++k)
 		{
-			const v3<double> p = load_v3<double>(x, i0);
-			const v3<double> t0 = load_v3<double>(x, i1), t1 = load_v3<double>(x, i2), t2 = load_v3<double>(x, i3);
-			const v3<double> b = pt_coeff(p, t0, t1, t2);
-			diff = b.x * (p - t0) + b.y * (p - t1) + b.z * (p - t2);
-			ka = i0, kb = i1;
+			v[k] = r.idx[k] & kIndexMask;
+			p[k] = load_v3<double>(x, v[k]);
 		}
-		else
+		const bool vf = (r.idx[2] & kTypeBit) != 0u;
+		if ((terms & kTermBarrier) && r.k2 != 0.0f)
 		{
-			const v3<double> a0 = load_v3<double>(x, i0), a1 = load_v3<double>(x, i1);
-			const v3<double> b0 = load_v3<double>(x, i2), b1 = load_v3<double>(x, i3);
-			const c4<double> c = ee_coeff(a0, a1, b0, b1);
-			diff = (c.a * a0 + c.b * a1) - (c.c * b0 + c.d * b1);
-			ka = i0, kb = i2;
+			double	 w[4];
+			uint32_t ka, kb; // the two vertices whose d_hat / offset the DCD used
+			if (vf)
+			{
+				const v3<double> b = pt_coeff(p[0], p[1], p[2], p[3]);
+				w[0] = b.x + b.y + b.z, w[1] = -b.x, w[2] = -b.y, w[3] = -b.z;
+				ka = v[0], kb = v[1];
+			}
+			else
+			{
+				const c4<double> c = ee_coeff(p[0], p[1], p[2], p[3]);
+				w[0] = c.a, w[1] = c.b, w[2] = -c.c, w[3] = -c.d;
+				ka = v[0], kb = v[2];
+			}
+			v3<double> diff{ 0, 0, 0 };
+			if (vf)
+			{
+				const v3<double> b{ -w[1], -w[2], -w[3] };
+				diff = b.x * (p[0] - p[1]) + b.y * (p[0] - p[2]) + b.z * (p[0] - p[3]);
+			}
+			else
+				diff = (w[0] * p[0] + w[1] * p[1]) - ((-w[2]) * p[2] + (-w[3]) * p[3]);
+			const double d = sqrt(dot(diff, diff));
+			const double dh = ((double)d_hat[ka] + (double)d_hat[kb]) * 0.5;
+			const double th = (double)offset[ka] + (double)offset[kb];
+			const double stiff = (double)r.area * kappa;
+			double		 e, de;
+			if (energy == kEnergyQuadratic)
+			{
+				const double C = (d - th) - dh;
+				e = 0.5 * stiff * C * C;
+				de = stiff * C;
+			}
+			else
+			{
+				e = stiff * barrier(d - th, dh);
+				de = stiff * barrier_d1(d - th, dh);
+			}
+			acc_b += e;
+			if (grad && d > 0.0)
+			{
+				const v3<double> nn = (1.0 / d) * diff;
+				for (int k = 0; k < 4; ++k)
+				{
+					const v3<double> g = (de * w[k]) * nn;
+					atomicAdd(&grad[3 * v[k] + 0], g.x);
+					atomicAdd(&grad[3 * v[k] + 1], g.y);
+					atomicAdd(&grad[3 * v[k] + 2], g.z);
+				}
+			}
 		}
-		const double d = sqrt(dot(diff, diff));
-		const double dh = ((double)d_hat[ka] + (double)d_hat[kb]) * 0.5;
-		const double th = (double)offset[ka] + (double)offset[kb];
-		const double area = (double)r.area;
-		double		 e;
-		if (energy == kEnergyQuadratic)
+		if ((terms & kTermFriction) && r.lambda != 0.0f && x_start)
 		{
-			const double C = (d - th) - dh;
-			e = 0.5 * area * kappa * C * C;
+			double w[4];
+			record_weights(r, w);
+			v3<double> rel{ 0, 0, 0 };
+			for (int k = 0; k < 4; ++k)
+				rel = rel + w[k] * (p[k] - load_v3<double>(x_start, v[k]));
+			const v3<double>   nn{ (double)r.normal[0], (double)r.normal[1], (double)r.normal[2] };
+			const FrictionEval f = friction_eval((double)r.lambda, nn, rel, kFrictionEps);
+			acc_f += f.energy;
+			if (grad)
+				for (int k = 0; k < 4; ++k)
+				{
+					atomicAdd(&grad[3 * v[k] + 0], w[k] * f.grad.x);
+					atomicAdd(&grad[3 * v[k] + 1], w[k] * f.grad.y);
+					atomicAdd(&grad[3 * v[k] + 2], w[k] * f.grad.z);
+				}
 		}
-		else
-			e = area * kappa * barrier(d - th, dh);
-		if (r.k2 != 0.0f) // disabled pairs (k1 = k2 = 0) carry no energy
-			acc += e;
 	}
-	__shared__ double s[kEnergyThreads / kWave];
+	__shared__ double s[2][kEnergyThreads / kWave];
 	for (int o = 32; o > 0; o >>= 1)
-		acc += __shfl_xor(acc, o, kWave);
+	{
+		acc_b += __shfl_xor(acc_b, o, kWave);
+		acc_f += __shfl_xor(acc_f, o, kWave);
+	}
 	if (__lane_id() == 0)
-		s[threadIdx.x / kWave] = acc;
+	{
+		s[0][threadIdx.x / kWave] = acc_b;
+		s[1][threadIdx.x / kWave] = acc_f;
+	}
 	__syncthreads();
 	if (threadIdx.x == 0)
 	{
-		double t = 0.0;
+		double tb = 0.0, tf = 0.0;
 		for (int k = 0; k < kEnergyThreads / kWave; ++k)
-			t += s[k];
-		part[blockIdx.x] = t;
+			tb += s[0][k], tf += s[1][k];
+		part[blockIdx.x] = tb;
+		part[gridDim.x + blockIdx.x] = tf;
+	}
+}
+
+// fn_process_collision_pair_friction (narrow_phase.cpp:1455-1521), first Newton
+// iteration: mu_lambda = -mu k1 with mu the mean of the coefficients of the
+// record's vertices 0 and 2 (get_friction_mu, :357-363), zero relative motion.
+__global__ void k_friction_init(ContactRec* __restrict__ pairs, const uint32_t* __restrict__ count, uint32_t cap,
+	const float* __restrict__ mu)
+{
+	const uint32_t n = min(*count, cap);
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		ContactRec& r = pairs[i];
+		const float m = 0.5f * (mu[r.idx[0] & kIndexMask] + mu[r.idx[2] & kIndexMask]);
+		r.lambda = -m * r.k1;
+		r.dv[0] = r.dv[1] = r.dv[2] = 0.0f;
+	}
+}
+
+// The same kernel in later iterations: the first iteration's records keep their
+// multiplier and weights, their relative motion since the step start is
+// sum_k w_k (x_k - x_start_k), and their repulsion is switched off
+// (disable_repulsion_part); written to out (which may be the input).
+template <typename P>
+__global__ void k_friction_lagged(const ContactRec* pairs, const uint32_t* __restrict__ count,
+	uint32_t cap, const P* __restrict__ x, const P* __restrict__ x_start, ContactRec* out)
+{
+	const uint32_t n = min(*count, cap);
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		ContactRec r = pairs[i];
+		double	   w[4];
+		record_weights(r, w);
+		double rel[3] = { 0, 0, 0 };
+		for (int k = 0; k < 4; ++k)
+		{
+			const uint32_t v = r.idx[k] & kIndexMask;
+			for (int c = 0; c < 3; ++c)
+				rel[c] += w[k] * ((double)x[3 * v + c] - (double)x_start[3 * v + c]);
+		}
+		r.dv[0] = (float)rel[0], r.dv[1] = (float)rel[1], r.dv[2] = (float)rel[2];
+		r.k1 = r.k2 = 0.0f;
+		out[i] = r;
 	}
 }
 
@@ -457,9 +564,47 @@ void contact_energy(Context* ctx, const ContactRec* pairs, const uint32_t* count
 {
 	UT_REQUIRE(pairs && count && x && d_hat && offset && out && scratch, "contact energy: null input");
 	constexpr uint32_t kBlocks = 256;
-	k_contact_energy<<<kBlocks, kEnergyThreads, 0, ctx->stream>>>(pairs, count, cap, x, d_hat, offset, kappa, energy,
-		scratch);
+	k_contact_potential<float><<<kBlocks, kEnergyThreads, 0, ctx->stream>>>(pairs, count, cap, x, nullptr, d_hat,
+		offset, kappa, energy, kTermBarrier, nullptr, scratch);
 	k_sum_parts<<<1, 64, 0, ctx->stream>>>(kBlocks, scratch, out);
+	UT_LAUNCH_CHECK();
+}
+
+void contact_potential(Context* ctx, const PotentialArgs& a)
+{
+	UT_REQUIRE(a.pairs && a.count && (a.x || a.x64) && a.out, "contact potential: null input");
+	UT_REQUIRE(!(a.terms & kTermBarrier) || (a.d_hat && a.offset), "contact potential: barrier needs d_hat, offset");
+	UT_REQUIRE(!(a.terms & kTermFriction) || (a.x64 ? a.x_start64 != nullptr : a.x_start != nullptr),
+		"contact potential: friction needs the lagged positions of the same precision");
+	constexpr uint32_t kBlocks = 256;
+	double*			   part = a.out + 2;
+	if (a.x64)
+		k_contact_potential<double><<<kBlocks, kEnergyThreads, 0, ctx->stream>>>(a.pairs, a.count, a.cap, a.x64,
+			a.x_start64, a.d_hat, a.offset, a.kappa, a.energy, a.terms, a.grad, part);
+	else
+		k_contact_potential<float><<<kBlocks, kEnergyThreads, 0, ctx->stream>>>(a.pairs, a.count, a.cap, a.x,
+			a.x_start, a.d_hat, a.offset, a.kappa, a.energy, a.terms, a.grad, part);
+	k_sum_parts<<<1, 64, 0, ctx->stream>>>(kBlocks, part, a.out);
+	k_sum_parts<<<1, 64, 0, ctx->stream>>>(kBlocks, part + kBlocks, a.out + 1);
+	UT_LAUNCH_CHECK();
+}
+
+void friction_init(Context* ctx, ContactRec* pairs, const uint32_t* count, uint32_t cap, const float* mu)
+{
+	UT_REQUIRE(pairs && count && mu, "friction init: null input");
+	k_friction_init<<<256, 256, 0, ctx->stream>>>(pairs, count, cap, mu);
+	UT_LAUNCH_CHECK();
+}
+
+void friction_lagged(Context* ctx, const ContactRec* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const double* x64, const float* x_start, const double* x_start64, ContactRec* out)
+{
+	UT_REQUIRE(pairs && count && out, "friction lagged: null input");
+	UT_REQUIRE(x64 ? x_start64 != nullptr : (x && x_start), "friction lagged: positions of one precision");
+	if (x64)
+		k_friction_lagged<double><<<256, 256, 0, ctx->stream>>>(pairs, count, cap, x64, x_start64, out);
+	else
+		k_friction_lagged<float><<<256, 256, 0, ctx->stream>>>(pairs, count, cap, x, x_start, out);
 	UT_LAUNCH_CHECK();
 }
 

@@ -44,6 +44,37 @@ struct ContactArgs
 void contact_query(Context* ctx, int kind, const ContactArgs& a, Lbvh* tree, const uint32_t* broad,
 	const uint32_t* broad_count, uint32_t broad_cap);
 void toi_reset(Context* ctx, float* toi);
+enum PotentialTerms : int
+{
+	kTermBarrier = 1,  // repulsion (barrier or quadratic) re-solved at x
+	kTermFriction = 2, // lagged friction of the records with a friction multiplier
+};
+
+struct PotentialArgs
+{
+	const ContactRec* pairs = nullptr;
+	const uint32_t*	  count = nullptr;
+	uint32_t		  cap = 0;
+	const float*	  x = nullptr;	 // positions (FP32) ...
+	const double*	  x64 = nullptr; // ... or FP64 (then every position array is FP64)
+	const float*	  x_start = nullptr;
+	const double*	  x_start64 = nullptr; // lagged positions of the friction term
+	const float*	  d_hat = nullptr;
+	const float*	  offset = nullptr;
+	double			  kappa = 1e8;
+	int				  energy = kEnergyBarrier;
+	int				  terms = kTermBarrier;
+	double*			  grad = nullptr; // optional double[3V]: += dE/dx
+	double*			  out = nullptr;  // double[2 + 512]: barrier, friction energy, then scratch
+};
+
+// Contact potential at x: barrier and / or friction energies, optionally their gradient.
+void contact_potential(Context* ctx, const PotentialArgs& a);
+// First-iteration friction values of the records (mu_lambda = -mu k1, dv = 0), in place.
+void friction_init(Context* ctx, ContactRec* pairs, const uint32_t* count, uint32_t cap, const float* mu);
+// Later iterations: relative motion since x_start, repulsion off; out may alias pairs.
+void friction_lagged(Context* ctx, const ContactRec* pairs, const uint32_t* count, uint32_t cap, const float* x,
+	const double* x64, const float* x_start, const double* x_start64, ContactRec* out);
 // Sum of the contact energies of the records at x (FP64 distances recomputed from
 // x); *out is a device double, scratch holds 256 doubles.
 void contact_energy(Context* ctx, const ContactRec* pairs, const uint32_t* count, uint32_t cap, const float* x,

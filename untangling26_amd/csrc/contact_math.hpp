@@ -53,6 +53,11 @@ UT_HD v3<T> load_v3(const float* __restrict__ p, uint32_t i)
 {
 	return v3<T>{ (T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2] };
 }
+template <typename T>
+UT_HD v3<T> load_v3(const double* __restrict__ p, uint32_t i)
+{
+	return v3<T>{ (T)p[3 * i], (T)p[3 * i + 1], (T)p[3 * i + 2] };
+}
 
 // 2x2 symmetric solve (distance.hpp:36-54): false when the determinant is 0.
 template <typename T>

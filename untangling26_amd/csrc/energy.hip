@@ -3,10 +3,16 @@
 // Material terms (inertia, BW98 stretch/shear, dihedral bending) follow
 // Energies/{soft_inertia_energy,stretch_face_energy,bending_energy_kernel}.cpp;
 // contact terms follow fn_perPair_assemble_gradient_hessian
-// (narrow_phase.cpp:2619-2746): gradient k1 w_i n, diagonal blocks
-// w_i^2 k2 n n^T.  The contacts' off-diagonal blocks (w_i w_j k2 n n^T) are not
-// stored: the PCG applies them matrix-free (pcg.hip), so the block pattern
-// stays the static cloth pattern.
+// (narrow_phase.cpp:2619-2746): gradient w_i (k1 n + friction gradient),
+// diagonal blocks w_i^2 (k2 n n^T + friction Hessian) with the lagged friction
+// values the record carries (friction_math.hpp).  The contacts' off-diagonal
+// blocks (w_i w_j times the same 3x3 block) are not stored: the PCG applies
+// them matrix-free (pcg.hip), so the block pattern stays the static cloth
+// pattern.
+//
+// Positions are FP32 (the solver's storage) or FP64 (the differentiable
+// potential's finite-difference checks): the kernels are templated on the
+// position type P.
 //
 // Everything is evaluated in FP64 and accumulated with FP64 atomics into the
 // gradient (double[3V]) and the planar block values (double[9][nslots]); the
@@ -14,6 +20,7 @@
 #include "ut_common.hpp"
 #include "energy.hpp"
 #include "energy_math.hpp"
+#include "friction_math.hpp"
 
 namespace ut
 {
@@ -33,7 +40,8 @@ __global__ void k_zero64(double* p, size_t n)
 }
 
 // E = m / (2 h^2) |x - x_tilde|^2 (soft_inertia_energy.cpp:25-110)
-__global__ void k_asm_inertia(uint32_t nv, const float* __restrict__ x, const double* __restrict__ x_tilde,
+template <typename P>
+__global__ void k_asm_inertia(uint32_t nv, const P* __restrict__ x, const double* __restrict__ x_tilde,
 	const float* __restrict__ mass, double inv_h2, double grad_scale, double hess_scale,
 	const uint32_t* __restrict__ diag_slot, uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals,
 	double* __restrict__ energy_part)
@@ -67,7 +75,8 @@ __global__ void k_asm_inertia(uint32_t nv, const float* __restrict__ x, const do
 	}
 }
 
-__global__ void k_asm_stretch(uint32_t nf, const float* __restrict__ x, const uint32_t* __restrict__ faces,
+template <typename P>
+__global__ void k_asm_stretch(uint32_t nf, const P* __restrict__ x, const uint32_t* __restrict__ faces,
 	const float* __restrict__ dm, const float* __restrict__ area, const float* __restrict__ mu_lambda,
 	const uint32_t* __restrict__ face_slots, uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals,
 	double* __restrict__ energy_part)
@@ -116,7 +125,8 @@ __global__ void k_asm_stretch(uint32_t nf, const float* __restrict__ x, const ui
 }
 
 // E = 1/2 k A (theta - theta0)^2, Gauss-Newton Hessian k A grad grad^T
-__global__ void k_asm_bending(uint32_t nh, const float* __restrict__ x, const uint32_t* __restrict__ hinges,
+template <typename P>
+__global__ void k_asm_bending(uint32_t nh, const P* __restrict__ x, const uint32_t* __restrict__ hinges,
 	const float* __restrict__ rest_angle, const float* __restrict__ stiff_area, const uint32_t* __restrict__ hinge_slots,
 	uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals, double* __restrict__ energy_part)
 {
@@ -156,27 +166,6 @@ __global__ void k_asm_bending(uint32_t nh, const float* __restrict__ x, const ui
 	}
 }
 
-// CollisionPairTemplate weights (collision_data.h:131-136)
-__device__ __forceinline__ void contact_weights(const ContactRec& c, double w[4])
-{
-	const bool vf = (c.idx[2] & 0x80000000u) != 0u;
-	const double b0 = c.bary[0], b1 = c.bary[1];
-	if (vf)
-	{
-		w[0] = 1.0;
-		w[1] = -b0;
-		w[2] = -b1;
-		w[3] = b0 + b1 - 1.0;
-	}
-	else
-	{
-		w[0] = b0;
-		w[1] = 1.0 - b0;
-		w[2] = -b1;
-		w[3] = b1 - 1.0;
-	}
-}
-
 __global__ void k_asm_contact(const ContactRec* __restrict__ pairs, const uint32_t* __restrict__ count, uint32_t cap,
 	const uint32_t* __restrict__ diag_slot, uint32_t nslots, double* __restrict__ grad, double* __restrict__ vals)
 {
@@ -185,17 +174,17 @@ __global__ void k_asm_contact(const ContactRec* __restrict__ pairs, const uint32
 	{
 		const ContactRec c = pairs[i];
 		double			 w[4];
-		contact_weights(c, w);
-		const double nn[3] = { c.normal[0], c.normal[1], c.normal[2] };
-		const double k1 = c.k1, k2 = c.k2;
+		record_weights(c, w);
+		const v3<double> g = pair_gradient(c);
+		const Sym3		 K = pair_block(c);
 		for (int a = 0; a < 4; ++a)
 		{
-			const uint32_t v = c.idx[a] & 0x7FFFFFFFu;
-			add3(grad, v, V3m(w[a] * k1 * nn[0], w[a] * k1 * nn[1], w[a] * k1 * nn[2]));
+			const uint32_t v = c.idx[a] & kIndexMask;
+			add3(grad, v, w[a] * g);
 			const uint32_t s = diag_slot[v];
 			for (int r = 0; r < 3; ++r)
 				for (int q = 0; q < 3; ++q)
-					atomicAdd(&vals[(size_t)(3 * r + q) * nslots + s], w[a] * w[a] * k2 * nn[r] * nn[q]);
+					atomicAdd(&vals[(size_t)(3 * r + q) * nslots + s], w[a] * w[a] * sym_at(K, r, q));
 		}
 	}
 }
@@ -207,24 +196,33 @@ __global__ void k_round_to32(size_t n, const double* __restrict__ in, float* __r
 		out[i] = (float)in[i];
 }
 
+template <typename P>
+static void material_terms(hipStream_t s, const AssembleArgs& a, const P* x)
+{
+	if (a.mass && a.x_tilde)
+		k_asm_inertia<P><<<div_up(a.nv, 256), 256, 0, s>>>(a.nv, x, a.x_tilde, a.mass, a.inv_h2, a.inertia_grad_scale,
+			a.inertia_hess_scale, a.diag_slot, a.nslots, a.grad, a.vals, a.energy ? a.energy + 0 : nullptr);
+	if (a.nf && a.faces && a.dm)
+		k_asm_stretch<P><<<div_up(a.nf, 256), 256, 0, s>>>(a.nf, x, a.faces, a.dm, a.face_area, a.mu_lambda,
+			a.face_slots, a.nslots, a.grad, a.vals, a.energy ? a.energy + 1 : nullptr);
+	if (a.nh && a.hinges)
+		k_asm_bending<P><<<div_up(a.nh, 256), 256, 0, s>>>(a.nh, x, a.hinges, a.rest_angle, a.bend_stiff_area,
+			a.hinge_slots, a.nslots, a.grad, a.vals, a.energy ? a.energy + 2 : nullptr);
+}
+
 void assemble(Context* ctx, const AssembleArgs& a)
 {
-	UT_REQUIRE(a.x && a.grad && a.vals && a.diag_slot, "assemble: null input");
+	UT_REQUIRE((a.x || a.x64) && a.grad && a.vals && a.diag_slot, "assemble: null input");
 	hipStream_t	   s = ctx->stream;
 	const size_t   nv3 = 3 * (size_t)a.nv, nval = 9 * (size_t)a.nslots;
 	k_zero64<<<div_up((uint32_t)nv3, 256), 256, 0, s>>>(a.grad, nv3);
 	k_zero64<<<(uint32_t)((nval + 255) / 256), 256, 0, s>>>(a.vals, nval);
 	if (a.energy)
 		k_zero64<<<1, 64, 0, s>>>(a.energy, 4);
-	if (a.mass && a.x_tilde)
-		k_asm_inertia<<<div_up(a.nv, 256), 256, 0, s>>>(a.nv, a.x, a.x_tilde, a.mass, a.inv_h2, a.inertia_grad_scale,
-			a.inertia_hess_scale, a.diag_slot, a.nslots, a.grad, a.vals, a.energy ? a.energy + 0 : nullptr);
-	if (a.nf && a.faces && a.dm)
-		k_asm_stretch<<<div_up(a.nf, 256), 256, 0, s>>>(a.nf, a.x, a.faces, a.dm, a.face_area, a.mu_lambda,
-			a.face_slots, a.nslots, a.grad, a.vals, a.energy ? a.energy + 1 : nullptr);
-	if (a.nh && a.hinges)
-		k_asm_bending<<<div_up(a.nh, 256), 256, 0, s>>>(a.nh, a.x, a.hinges, a.rest_angle, a.bend_stiff_area,
-			a.hinge_slots, a.nslots, a.grad, a.vals, a.energy ? a.energy + 2 : nullptr);
+	if (a.x64)
+		material_terms(s, a, a.x64);
+	else
+		material_terms(s, a, a.x);
 	if (a.contacts && a.contact_count && a.contact_cap)
 		k_asm_contact<<<1024, 256, 0, s>>>(a.contacts, a.contact_count, a.contact_cap, a.diag_slot, a.nslots, a.grad,
 			a.vals);

@@ -8,7 +8,8 @@ namespace ut
 struct AssembleArgs
 {
 	uint32_t nv = 0, nf = 0, nh = 0, nslots = 0;
-	const float* x = nullptr; // current positions float[3V]
+	const float*  x = nullptr;	 // current positions float[3V]
+	const double* x64 = nullptr; // FP64 positions double[3V]; when set they replace x
 	// inertia: m / h^2 (x - x_tilde)
 	const double* x_tilde = nullptr; // double[3V]
 	const float*  mass = nullptr;	 // per vertex

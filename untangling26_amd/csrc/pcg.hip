@@ -9,6 +9,7 @@
 #include "ut_common.hpp"
 #include "pcg.hpp"
 #include "radix_sort.hpp"
+#include "friction_math.hpp"
 
 #include <cmath>
 #include <cstring>
@@ -190,21 +191,33 @@ __device__ __forceinline__ V3<TA> row_spmv(uint32_t row, const uint32_t* __restr
 // ---- matrix-free contact off-diagonal blocks ------------------------------------
 constexpr int kContactBlocks = 256;
 
-__device__ __forceinline__ void pair_weights(const ContactRec& c, double w[4])
+// Off-diagonal blocks of one contact applied to p: with the pair's 3x3 block K
+// (k2 n n^T plus the lagged friction Hessian, friction_math.hpp) and weights w,
+// block (a, b) = w_a w_b K, so vertex a receives w_a K (P - w_a p_a) with
+// P = sum_b w_b p_b; the contact's part of p^T A_off p is
+// P^T K P - sum_a w_a^2 p_a^T K p_a.
+template <typename T>
+__device__ __forceinline__ double contact_offdiag(const ContactRec& c, const V3<T> pv[4], v3<double> out[4])
 {
-	const double b0 = c.bary[0], b1 = c.bary[1];
-	if (c.idx[2] & 0x80000000u) // VF (collision_data.h:133)
+	double w[4];
+	record_weights(c, w);
+	const Sym3 K = pair_block(c);
+	v3<double> P{ 0, 0, 0 }, q[4];
+	for (int a = 0; a < 4; ++a)
 	{
-		w[0] = 1.0, w[1] = -b0, w[2] = -b1, w[3] = b0 + b1 - 1.0;
+		q[a] = v3<double>{ (double)pv[a].x, (double)pv[a].y, (double)pv[a].z };
+		P = P + w[a] * q[a];
 	}
-	else // EE (collision_data.h:134)
+	double part = dot(P, sym_mul(K, P));
+	for (int a = 0; a < 4; ++a)
 	{
-		w[0] = b0, w[1] = 1.0 - b0, w[2] = -b1, w[3] = b1 - 1.0;
+		const v3<double> r = P - w[a] * q[a];
+		out[a] = w[a] * sym_mul(K, r);
+		part -= w[a] * w[a] * dot(q[a], sym_mul(K, q[a]));
 	}
+	return part;
 }
 
-// q_i += sign * k2 w_i (S - w_i s_i) n with s_j = n . p_j, S = sum_j w_j s_j;
-// partial of p^T A_off p = k2 (S^2 - sum_j w_j^2 s_j^2).
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_contact_offdiag(const ContactRec* __restrict__ pairs,
 	const uint32_t* __restrict__ count, uint32_t cap, const T* __restrict__ p, T* __restrict__ q, double sign,
@@ -217,26 +230,21 @@ __global__ __launch_bounds__(kPcgThreads) void k_contact_offdiag(const ContactRe
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
 	{
 		const ContactRec c = pairs[i];
-		double			 w[4], sv[4];
-		pair_weights(c, w);
-		const double nx = c.normal[0], ny = c.normal[1], nz = c.normal[2], k2 = c.k2;
-		double		 S = 0.0, diag = 0.0;
-		uint32_t	 v[4];
+		uint32_t		 v[4];
+		V3<T>			 pv[4];
 		for (int a = 0; a < 4; ++a)
 		{
-			v[a] = c.idx[a] & 0x7FFFFFFFu;
-			sv[a] = nx * (double)p[3 * v[a]] + ny * (double)p[3 * v[a] + 1] + nz * (double)p[3 * v[a] + 2];
-			S += w[a] * sv[a];
-			diag += w[a] * w[a] * sv[a] * sv[a];
+			v[a] = c.idx[a] & kIndexMask;
+			pv[a] = V3<T>{ p[3 * v[a]], p[3 * v[a] + 1], p[3 * v[a] + 2] };
 		}
+		v3<double> o[4];
+		acc += contact_offdiag(c, pv, o);
 		for (int a = 0; a < 4; ++a)
 		{
-			const double cf = sign * k2 * w[a] * (S - w[a] * sv[a]);
-			atomicAdd(&q[3 * v[a]], (T)(cf * nx));
-			atomicAdd(&q[3 * v[a] + 1], (T)(cf * ny));
-			atomicAdd(&q[3 * v[a] + 2], (T)(cf * nz));
+			atomicAdd(&q[3 * v[a]], (T)(sign * o[a].x));
+			atomicAdd(&q[3 * v[a] + 1], (T)(sign * o[a].y));
+			atomicAdd(&q[3 * v[a] + 2], (T)(sign * o[a].z));
 		}
-		acc += k2 * (S * S - diag);
 	}
 	if (part)
 		block_partial(acc, part);
@@ -267,7 +275,7 @@ __device__ __forceinline__ V3<double> ld_contrib(const double* c, uint32_t o)
 }
 
 // Same product as k_contact_offdiag, scattered into the per-vertex entry lists:
-// contrib[pos[4c + a]] = k2 w_a (S - w_a s_a) n for every corner a of contact c.
+// contrib[pos[4c + a]] = w_a K (P - w_a p_a) for every corner a of contact c.
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_contact_contrib(const ContactRec* __restrict__ pairs, uint32_t nc,
 	const T* __restrict__ p, const uint32_t* __restrict__ pos, T* __restrict__ contrib,
@@ -279,21 +287,16 @@ __global__ __launch_bounds__(kPcgThreads) void k_contact_contrib(const ContactRe
 	if (i >= nc)
 		return;
 	const ContactRec c = pairs[i];
-	double			 w[4], sv[4];
-	pair_weights(c, w);
-	const double nx = c.normal[0], ny = c.normal[1], nz = c.normal[2], k2 = c.k2;
-	double		 S = 0.0;
+	V3<T>			 pv[4];
 	for (int a = 0; a < 4; ++a)
 	{
-		const uint32_t v = c.idx[a] & 0x7FFFFFFFu;
-		sv[a] = nx * (double)p[3 * v] + ny * (double)p[3 * v + 1] + nz * (double)p[3 * v + 2];
-		S += w[a] * sv[a];
+		const uint32_t v = c.idx[a] & kIndexMask;
+		pv[a] = V3<T>{ p[3 * v], p[3 * v + 1], p[3 * v + 2] };
 	}
+	v3<double> o[4];
+	contact_offdiag(c, pv, o);
 	for (int a = 0; a < 4; ++a)
-	{
-		const double cf = k2 * w[a] * (S - w[a] * sv[a]);
-		st_contrib(contrib, pos[4 * i + a], (double)(T)(cf * nx), (double)(T)(cf * ny), (double)(T)(cf * nz));
-	}
+		st_contrib(contrib, pos[4 * i + a], (double)(T)o[a].x, (double)(T)o[a].y, (double)(T)o[a].z);
 }
 
 __global__ void k_ct_entry_keys(const ContactRec* __restrict__ pairs, uint32_t nc, uint32_t* __restrict__ keys,
@@ -622,21 +625,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(uint32_t it, uint32_t n
 		stv(pn, i, next_dir(zr, pr, b, ok));
 	if (i < nc)
 	{
-		double w[4], sv[4];
-		pair_weights(c, w);
-		const double nx = c.normal[0], ny = c.normal[1], nz = c.normal[2], k2 = c.k2;
-		double		 S = 0.0;
+		V3<T> pv[4];
 		for (int a = 0; a < 4; ++a)
-		{
-			const V3<T> pv = next_dir(zc[a], pc[a], b, ok);
-			sv[a] = nx * (double)pv.x + ny * (double)pv.y + nz * (double)pv.z;
-			S += w[a] * sv[a];
-		}
+			pv[a] = next_dir(zc[a], pc[a], b, ok);
+		v3<double> o[4];
+		contact_offdiag(c, pv, o);
 		for (int a = 0; a < 4; ++a)
-		{
-			const double cf = k2 * w[a] * (S - w[a] * sv[a]);
-			st_contrib(contrib, at[a], (double)(T)(cf * nx), (double)(T)(cf * ny), (double)(T)(cf * nz));
-		}
+			st_contrib(contrib, at[a], (double)(T)o[a].x, (double)(T)o[a].y, (double)(T)o[a].z);
 	}
 }
 

@@ -356,6 +356,39 @@ StepReport NewtonSolver::physics_step_GPU()
 				}
 			}
 		}
+		// fn_process_collision_pair_friction (narrow_phase.cpp:1455-1521, run before every
+		// assembly): the first iteration's proximity records get mu_lambda = -mu k1 (after the
+		// region filter, so disabled records get none) and are kept for the later iterations,
+		// which append them with their relative motion since the step start and no repulsion
+		// (the reference keeps them at the front of the list, reset_narrowphase_count).
+		const uint32_t nprox = rep.vf_pairs + rep.ee_pairs;
+		if (d.friction_mu && it == 0)
+		{
+			nfric = 0;
+			if (p.use_self_collision && nprox > 0)
+			{
+				h_counts[5] = nprox;
+				UT_HIP_CHECK(hipMemcpyAsync(counts.get() + 5, h_counts + 5, 4, hipMemcpyHostToDevice, s));
+				friction_init(ctx, contact_pairs.get(), counts.get() + 5, nprox, d.friction_mu);
+				if (p.nonlinear_iters > 1)
+				{
+					fric_pairs.reserve(nprox);
+					UT_HIP_CHECK(hipMemcpyAsync(fric_pairs.get(), contact_pairs.get(), sizeof(ContactRec) * nprox,
+						hipMemcpyDeviceToDevice, s));
+					nfric = nprox;
+				}
+			}
+		}
+		else if (d.friction_mu && nfric > 0)
+		{
+			ensure_contact_capacity(ncontacts + nfric, cap_ef, ncontacts);
+			friction_lagged(ctx, fric_pairs.get(), counts.get() + 5, nfric, x.get(), nullptr, x_step_start.get(),
+				nullptr, contact_pairs.get() + ncontacts);
+			ncontacts += nfric;
+			h_counts[4] = ncontacts;
+			UT_HIP_CHECK(hipMemcpyAsync(counts.get() + 3, h_counts + 4, 4, hipMemcpyHostToDevice, s));
+			UT_HIP_CHECK(hipStreamSynchronize(s));
+		}
 		UT_HIP_CHECK(hipEventRecord(ev[3], s));
 		AssembleArgs a;
 		a.nv = d.nv;

@@ -32,6 +32,8 @@ struct SolverDesc
 	const float*	rest_vert_area = nullptr;
 	const float*	rest_edge_area = nullptr;
 	const float*	rest_face_area = nullptr;
+	// per-vertex friction coefficient (sa_contact_active_verts_friction_coeff); null: no friction
+	const float* friction_mu = nullptr;
 	// material (device)
 	const float*	dm = nullptr;
 	const float*	mu_lambda = nullptr;
@@ -111,7 +113,9 @@ struct NewtonSolver
 	DevBuf<uint32_t>	contact_ids;
 	DevBuf<EfPairRec>	ef_pairs;
 	DevBuf<uint32_t>	ef_ids;
-	DevBuf<uint32_t>	counts; // [0] VF, [1] EE, [2] EF, [3] total contacts
+	DevBuf<uint32_t>	counts; // [0] VF, [1] EE, [2] EF, [3] total contacts, [5] friction pairs
+	DevBuf<ContactRec>	fric_pairs; // the first iteration's proximity records (lagged friction)
+	uint32_t			nfric = 0;
 	DevBuf<uint32_t>	vert_adj_off, vert_adj_edges; // edges around each vertex (ascending)
 	uint32_t*			h_counts = nullptr;
 	hipEvent_t			ev[8] = {};

@@ -29,6 +29,7 @@ class ClothMaterial:
     density: float = 1e3
     d_hat: float = 1e-3
     contact_offset: float = 0.0
+    friction_mu: float = 0.5  # MaterialBase::default_friction_mu (physical_material.h:66)
 
 
 def convert_prop(young: float, poisson: float):

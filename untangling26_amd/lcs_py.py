@@ -443,6 +443,10 @@ class NewtonSolver:
         d.nslices, d.nslots = pat.nslices, pat.nslots
         self._dev["rest_x"] = up(np.asarray(rest, np.float32))
         d.rest_x = ptr(self._dev["rest_x"])
+        # sa_contact_active_verts_friction_coeff: the material's friction_mu per vertex
+        # (init_collision_data.cpp:188); every contact step carries lagged friction
+        self._dev["friction_mu"] = up(np.full(st.nv, mat.friction_mu, np.float32))
+        d.friction_mu = ptr(self._dev["friction_mu"])
         x0 = up(mesh.x)
         self._x0 = x0.clone()
         self._material_x = np.concatenate([w.vertices[:, [0, 2]] for w in self._worlds]).astype(np.float32)

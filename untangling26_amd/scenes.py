@@ -9,6 +9,9 @@ so cloth sheets lie in the x-z plane and "vertical" means y.
                  half is displaced by 0.08 sin(2 pi u) sin(pi v) + N(0, 0.005)
   nested_layers  8 stacked 128x128-vertex sheets, 4 mm apart, vertex jitter
                  N(0, 6 mm) and a per-layer in-plane shift U(-0.05, 0.05) m
+  draped_cloth   a 64x64-vertex 1 m sheet draped over a level-5 icosphere of
+                 radius 0.3 m, plus seeded states perturbed by N(0, 1 mm) with
+                 every cloth-sphere gap kept in (0.3, 0.9) mm
 """
 from __future__ import annotations
 
@@ -163,3 +166,83 @@ CONFIGS = {
     "folded_cloth": folded_cloth,
     "nested_layers": nested_layers,
 }
+
+
+def icosphere(level: int = 5, radius: float = 0.3) -> tuple[np.ndarray, np.ndarray]:
+    """Icosahedron subdivided `level` times (each triangle into four, new vertices
+    projected onto the sphere): 10 4^level + 2 vertices, 20 4^level faces."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    verts = [np.asarray(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = [tuple(x) for x in f]
+    for _ in range(level):
+        mid = {}
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = verts[a] + verts[b]
+                verts.append(p / np.linalg.norm(p))
+                mid[k] = len(verts) - 1
+            return mid[k]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return (np.asarray(verts) * radius).astype(np.float64), np.asarray(faces, np.uint32)
+
+
+def _drape(n: int, side: float, radius: float, cap_angle: float) -> np.ndarray:
+    """The n x n grid wrapped over the sphere (centre at the origin, y up) keeping arc
+    length from the sheet's centre: within `cap_angle` of the pole the sheet lies on
+    the sphere, beyond it continues along the tangent at the cap's rim."""
+    g = grid_sheet(n, n, side, side) - np.array([side / 2, 0.0, side / 2])
+    rho = np.hypot(g[:, 0], g[:, 2])
+    phi = np.arctan2(g[:, 2], g[:, 0])
+    d = np.stack([np.cos(phi), np.zeros_like(phi), np.sin(phi)], 1)
+    theta = np.minimum(rho / radius, cap_angle)
+    on = np.stack([np.sin(theta), np.cos(theta), np.sin(theta)], 1)
+    p = radius * np.stack([on[:, 0] * d[:, 0], on[:, 1], on[:, 2] * d[:, 2]], 1)
+    extra = np.maximum(rho - radius * cap_angle, 0.0)
+    tangent = np.stack([np.cos(cap_angle) * d[:, 0], -np.sin(cap_angle) * np.ones_like(phi), np.cos(cap_angle) * d[:, 2]], 1)
+    return p + extra[:, None] * tangent
+
+
+def _gap_states(base: np.ndarray, radius: float, rng, sigma: float, gap_lo: float, gap_hi: float) -> np.ndarray:
+    """base perturbed by N(0, sigma) per coordinate; every vertex that comes within
+    1 mm of the sphere is moved radially to a gap drawn from U(gap_lo, gap_hi)."""
+    x = base + rng.normal(0.0, sigma, base.shape)
+    r = np.linalg.norm(x, axis=1)
+    near = r < radius + 1e-3
+    gap = rng.uniform(gap_lo, gap_hi, int(near.sum()))
+    x[near] *= ((radius + gap) / r[near])[:, None]
+    return x
+
+
+def draped_cloth(seed: int = 0, n: int = 64, side: float = 1.0, radius: float = 0.3, level: int = 5,
+                 cap_angle: float = np.pi / 3, states: int = 0, sigma: float = 1e-3, gap=(3e-4, 9e-4)):
+    """The differentiation workload: cloth (vertices 0 .. n^2 - 1) over a fixed
+    icosphere (the rest).  Returns (mesh, rest, cloth_vertex_count, states[B, V, 3]):
+    the base drape has every cloth vertex near the sphere at a gap U(gap) and the
+    states perturb the cloth by N(0, sigma) with the same gap rule, so every
+    contact distance stays inside (1e-6, d_hat = 1e-3) (a cloth edge of 1.6 cm sags
+    at most 0.11 mm below its end points over the sphere and a sphere facet lies
+    at most 0.04 mm inside it)."""
+    rng = np.random.default_rng(seed)
+    sv, sf = icosphere(level, radius)
+    cloth = _gap_states(_drape(n, side, radius, cap_angle), radius, rng, 0.0, *gap)
+    x = np.concatenate([cloth, sv], 0)
+    faces = np.concatenate([grid_faces(n, n, 0), sf + n * n], 0)
+    rest = np.concatenate([grid_sheet(n, n, side, side) - np.array([side / 2, 0.0, side / 2]), sv], 0)
+    st = np.empty((states, len(x), 3), np.float64)
+    for b in range(states):
+        st[b, :n * n] = _gap_states(cloth, radius, rng, sigma, *gap)
+        st[b, n * n:] = sv
+    mesh = Mesh(x, faces, name=f"draped_cloth_{n}x{n}_ico{level}")
+    return mesh, rest.astype(np.float32), n * n, st
