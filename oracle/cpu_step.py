@@ -1,22 +1,27 @@
 """CPU baseline of one untangling step, timed with the oracle (test and
 measurement infrastructure only: the product path never imports this).
 
-The oracle restates the reference's algorithms in plain single-threaded C;
-this module times it on a bounded sample of the benchmark's workload and
-scales each part to one step:
+The oracle restates the reference's algorithms in C; this module runs ONE whole
+step of the benchmark's workload on the host's cores and times each part (wall
+clock), with nothing sampled or scaled:
   detection   VF/EE proximity DCD (grid broad phase + narrow tests) and the EF
-              intersection pass on the full mesh, timed once;
-  contours    EF preprocessing + contour DFS on the full mesh;
-  PRP         distances + directions + adaptive k-ring choice of the first
-              contours until a time budget is spent, scaled to the number of
-              contours a step processes (at most 256);
-  assembly    the incremental potential with the detected contacts;
-  PCG         FP64 iterations (with the contact off-diagonal terms) timed on a
-              fixed count and scaled to the GPU's mean iteration count;
-  CCD         the additive CCD over the swept broad-phase pairs of one step.
+              intersection pass, plus EF pairs from zero-distance proximity pairs
+              (serial);
+  contours    EF preprocessing + contour DFS (serial);
+  PRP         distances and direction sets of every processed contour (at most
+              256; OpenMP over the searches), then the adaptive k-ring choice of
+              every processed contour (OpenMP over each level's directions);
+  assembly    the incremental potential with the detected contacts and their
+              first-iteration friction (serial);
+  PCG         FP64 block-Jacobi PCG to relative residual 1e-6 (at most the
+              benchmark's iteration cap), with the matrix-free contact blocks
+              (OpenMP products and dots);
+  CCD         the additive CCD over the swept broad-phase pairs of the GPU's
+              step (OpenMP over pairs).
 """
 from __future__ import annotations
 
+import os
 import time
 
 import numpy as np
@@ -30,9 +35,26 @@ def _t(fn):
     return time.perf_counter() - t, out
 
 
-def cpu_step_baseline(mesh, rest, solver, args, prp_budget_s: float = 8.0, pcg_iters: int = 30):
+def host_threads(cap: int = 16) -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(cap, n))
+
+
+def cpu_step_baseline(mesh, rest, solver, args, threads: int | None = None):
     from untangling26_amd import scenes
 
+    T = host_threads() if threads is None else int(threads)
+    oracle.set_threads(T)
+    try:
+        return _step(mesh, rest, solver, args, T, scenes)
+    finally:
+        oracle.set_threads(1)
+
+
+def _step(mesh, rest, solver, args, T, scenes):
     st = solver._setup
     x = np.ascontiguousarray(mesh.x, np.float32)
     edges, faces = st.edges, st.faces
@@ -68,28 +90,22 @@ def cpu_step_baseline(mesh, rest, solver, args, prp_budget_s: float = 8.0, pcg_i
 
     times["contours"], (opr, adj, adjn, ci, mi, nc) = _t(contours)
 
-    # PRP: the first contours until the budget is spent
     o = oracle.PrpOracle(x, rest, edges, faces, st.face_adj_edges, st.edge_adj_faces, st.rest_vert_area,
                          st.rest_edge_area, st.rest_face_area, st.mass)
     active = min(nc, 256)
-    done, t_prp = 0, 0.0
-    while done < active and t_prp < prp_budget_s:
-        n = min(active, max(1, 2 * done) if done else 1)
-        tp, _ = _t(lambda: o.prepare(opr, adj, adjn, ci, mi, nc, args.dirs, n_prepare=n))
-        tc = 0.0
-        for c in range(done, n):
-            dt, _ = _t(lambda: o.choose(c))
-            tc += dt
-        # prepare is recomputed for all first n contours each round: charge its share
-        t_prp += tp * (n - done) / n + tc
-        done = n
-    times["prp"] = t_prp / max(done, 1) * active
+
+    def prp():
+        # OpenMP inside: the searches of the preparation, and each level's direction
+        # evaluations of a contour (contours one after another, as the reference's driver)
+        o.prepare(opr, adj, adjn, ci, mi, nc, args.dirs, n_prepare=active)
+        return [o.choose(c) for c in range(active)]
+
+    times["prp"], choices = _t(prp)
 
     x_tilde = x.astype(np.float64).ravel()
-    times["assemble"], (g, vals, _) = _t(lambda: oracle.assemble(st, x, x_tilde, 3600.0, recs))
-    tp, _ = _t(lambda: oracle.pcg_sell(st.pattern, vals, -g, recs, max_iters=pcg_iters, rel_tol=1e-300))
-    gpu_iters = max(1.0, getattr(solver, "_last_report", {}).get("pcg_iterations", pcg_iters) or pcg_iters)
-    times["pcg"] = tp / pcg_iters * gpu_iters
+    fr = oracle.friction_init(recs, np.full(mesh.nv, 0.5, np.float32))
+    times["assemble"], (g, vals, _) = _t(lambda: oracle.assemble(st, x, x_tilde, 3600.0, fr))
+    times["pcg"], sol = _t(lambda: oracle.pcg_sell(st.pattern, vals, -g, fr, max_iters=args.pcg_iters, rel_tol=1e-6))
 
     xe = solver.positions().cpu().numpy().astype(np.float32)
 
@@ -107,11 +123,11 @@ def cpu_step_baseline(mesh, rest, solver, args, prp_budget_s: float = 8.0, pcg_i
     return {
         "value": round(1.0 / total, 5),
         "unit": "steps/s",
-        "cores": 1,
+        "cores": T,
         "kind": "port",
-        "sample": (f"oracle restatement (single-threaded C) of one step on the same {mesh.nf}-triangle state: "
-                   f"detection, contours ({nc} contours), assembly, CCD timed in full; PRP timed on the first "
-                   f"{done} contours and scaled to {active}; PCG timed for {pcg_iters} iterations and scaled to "
-                   f"{gpu_iters:.0f}"),
+        "sample": (f"one whole step of the oracle restatement (C, OpenMP on {T} host cores) on the "
+                   f"same {mesh.nf}-triangle tangled state: detection, contours ({nc} contours), PRP of all {active} "
+                   f"processed contours, assembly with friction, FP64 PCG to 1e-6 ({sol[1]} iterations), CCD; "
+                   f"nothing sampled or scaled"),
         "seconds": {k: round(v, 3) for k, v in times.items()},
     }

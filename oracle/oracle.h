@@ -20,6 +20,12 @@
 extern "C" {
 #endif
 
+/* Threads of the parallel loops (PCG products and dots, CCD minimum, PRP distance
+ * searches and direction sets); 1 (the default) runs everything serially, which
+ * the tests use.  Only the CPU baseline timing raises it. */
+extern int or_nthreads;
+void	   or_set_threads(int n);
+
 /* lbvh.cpp:84-121 make_morton32 on normalised points. */
 void or_morton30(const float* p, uint32_t n, uint32_t* out);
 
@@ -185,6 +191,9 @@ void	 or_prp_penetration(uint32_t n, const void* hits, const float* face_area, c
 		 const float* vert_area, const float* mass, const float* face_normal, const float* edge_normal,
 		 const float* vert_normal, const uint32_t* edges, const uint32_t* faces, const float* dir, void* out_metrics);
 int		 or_prp_hit_attached(const or_prp_mesh* m, uint32_t c, const void* hit);
+/* Components of a hit list: bucketed search (brute = 0) or all pairs (brute != 0);
+ * root[i] = smallest hit index of hit i's component. */
+void or_prp_hit_components(const or_prp_mesh* m, const void* hits, uint32_t n, int brute, uint32_t* root);
 uint32_t or_prp_vertex_flags(const or_prp_mesh* m, uint32_t v, uint32_t* out, uint32_t cap);
 int		 or_prp_pick_component(uint32_t ncomp, const uint8_t* attached, const uint32_t* cov, const uint32_t* cnt,
 		 const uint64_t* key);

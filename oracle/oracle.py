@@ -61,6 +61,8 @@ def lib():
         L.or_ee_coeff.argtypes = [p, p, p, p, p]
         L.or_barrier.argtypes = [C.c_double, C.c_double, C.c_int]
         L.or_barrier.restype = C.c_double
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_prp_hit_components.argtypes = [p, p, C.c_uint32, C.c_int, p]
         L.or_friction_eval.argtypes = [C.c_double, p, p, C.c_double, p, p]
         L.or_friction_eval.restype = C.c_double
         L.or_record_weights.argtypes = [p, p]
@@ -665,6 +667,13 @@ class PrpOracle:
         lib().or_prp_eval_slot(self.h, c, _p(d), k_src, k_dst, _p(hits), n, _p(met))
         return hits[:n], met[0]
 
+    def hit_components(self, hits, brute=False):
+        """Component root (smallest member index) of every hit of a slot's hit list."""
+        h = np.ascontiguousarray(np.asarray(hits, PRP_HIT))
+        root = np.zeros(max(len(h), 1), np.uint32)
+        lib().or_prp_hit_components(self.h, _p(h), len(h), int(bool(brute)), _p(root))
+        return root[:len(h)]
+
     def hit_attached(self, c, hit):
         h = np.ascontiguousarray(np.asarray([hit], PRP_HIT))
         return bool(lib().or_prp_hit_attached(self.h, c, _p(h)))
@@ -885,3 +894,8 @@ def contact_potential(records, x, d_hat, offset, kappa=1e8, energy=1, terms=3, x
     lib().or_contact_potential(_p(R), len(R), _p(X), _p(xs), _p(_c(d_hat, np.float32)),
                                _p(_c(offset, np.float32)), float(kappa), int(energy), int(terms), _p(g), _p(out))
     return out[0], out[1], g.reshape(-1, 3)
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (1: serial, the default the tests use)."""
+    lib().or_set_threads(int(n))

@@ -371,6 +371,8 @@ double or_ccd_list(int kind, const uint32_t* broad, size_t nb, const float* xs, 
 	const uint32_t* edges, const uint32_t* faces, const uint32_t* vprop, const float* off)
 {
 	double m = 1.0;
+	/* the minimum does not depend on the order, so threads (or_set_threads) change nothing */
+#pragma omp parallel for reduction(min : m) schedule(dynamic, 4096) num_threads(or_nthreads) if (or_nthreads > 1)
 	for (size_t i = 0; i < nb; ++i)
 	{
 		double t = kind == 2 ? or_vf_ccd(broad[2 * i], broad[2 * i + 1], xs, xe, faces, vprop, off)

@@ -13,40 +13,69 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
+
+int or_nthreads = 1;
+void or_set_threads(int n) { or_nthreads = n < 1 ? 1 : n; }
 
 static const uint32_t* g_contacts = NULL;
 static uint32_t		   g_ncontacts = 0;
 
-/* y += off-diagonal contact blocks w_i w_j K (i != j) times x, K the pair's block
- * (k2 n n^T + lagged friction Hessian, or_pair_block) */
-static void contact_mv(const double* x, double* y)
+/* y += w_a K (P - w_a x_a) for contact c at each of its vertices a */
+static void contact_apply(const uint32_t* r, const double* x, double* y)
 {
-	for (uint32_t c = 0; c < g_ncontacts; ++c)
+	double	 w[4], K[9], P[3] = { 0, 0, 0 };
+	uint32_t v[4];
+	or_record_weights(r, w);
+	or_pair_block(r, K, NULL);
+	for (int a = 0; a < 4; ++a)
 	{
-		const uint32_t* r = g_contacts + 16 * c;
-		double			w[4], K[9], P[3] = { 0, 0, 0 };
-		uint32_t		v[4];
-		or_record_weights(r, w);
-		or_pair_block(r, K, NULL);
-		for (int a = 0; a < 4; ++a)
-		{
-			v[a] = r[a] & 0x7FFFFFFFu;
-			for (int k = 0; k < 3; ++k)
-				P[k] += w[a] * x[3 * v[a] + k];
-		}
-		for (int a = 0; a < 4; ++a)
-		{
-			double q[3];
-			for (int k = 0; k < 3; ++k)
-				q[k] = P[k] - w[a] * x[3 * v[a] + k];
-			for (int k = 0; k < 3; ++k)
-				y[3 * v[a] + k] += w[a] * (K[3 * k] * q[0] + K[3 * k + 1] * q[1] + K[3 * k + 2] * q[2]);
-		}
+		v[a] = r[a] & 0x7FFFFFFFu;
+		for (int k = 0; k < 3; ++k)
+			P[k] += w[a] * x[3 * v[a] + k];
 	}
+	for (int a = 0; a < 4; ++a)
+	{
+		double q[3];
+		for (int k = 0; k < 3; ++k)
+			q[k] = P[k] - w[a] * x[3 * v[a] + k];
+		for (int k = 0; k < 3; ++k)
+			y[3 * v[a] + k] += w[a] * (K[3 * k] * q[0] + K[3 * k + 1] * q[1] + K[3 * k + 2] * q[2]);
+	}
+}
+
+/* y += off-diagonal contact blocks w_i w_j K (i != j) times x, K the pair's block
+ * (k2 n n^T + lagged friction Hessian, or_pair_block).  With several threads each
+ * sums its share of the contacts into a private vector (then added in thread order). */
+static void contact_mv(uint32_t n, const double* x, double* y)
+{
+	if (or_nthreads <= 1 || g_ncontacts < 4096)
+	{
+		for (uint32_t c = 0; c < g_ncontacts; ++c)
+			contact_apply(g_contacts + 16 * c, x, y);
+		return;
+	}
+	const size_t m = 3 * (size_t)n;
+	const int	 T = or_nthreads;
+	double*		 buf = (double*)calloc((size_t)T * m, sizeof(double));
+#pragma omp parallel num_threads(T)
+	{
+		const int t = omp_get_thread_num();
+		double*	  yb = buf + (size_t)t * m;
+#pragma omp for schedule(static)
+		for (uint32_t c = 0; c < g_ncontacts; ++c)
+			contact_apply(g_contacts + 16 * c, x, yb);
+#pragma omp for schedule(static)
+		for (size_t k = 0; k < m; ++k)
+			for (int u = 0; u < T; ++u)
+				y[k] += buf[(size_t)u * m + k];
+	}
+	free(buf);
 }
 
 static void bsr_mv(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* bl, const double* x, double* y)
 {
+#pragma omp parallel for schedule(static) num_threads(or_nthreads) if (or_nthreads > 1)
 	for (uint32_t i = 0; i < n; ++i)
 	{
 		double a[3] = { 0, 0, 0 };
@@ -59,12 +88,20 @@ static void bsr_mv(uint32_t n, const uint32_t* rp, const uint32_t* ci, const dou
 		}
 		y[3 * i] = a[0], y[3 * i + 1] = a[1], y[3 * i + 2] = a[2];
 	}
-	contact_mv(x, y);
+	contact_mv(n, x, y);
 }
 
+/* serial sum, or an OpenMP reduction with several threads */
 static double dotn(const double* a, const double* b, size_t m)
 {
 	double s = 0;
+	if (or_nthreads <= 1)
+	{
+		for (size_t i = 0; i < m; ++i)
+			s += a[i] * b[i];
+		return s;
+	}
+#pragma omp parallel for reduction(+ : s) schedule(static) num_threads(or_nthreads)
 	for (size_t i = 0; i < m; ++i)
 		s += a[i] * b[i];
 	return s;
@@ -155,11 +192,13 @@ uint32_t or_pcg(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double
 			out[2] = 2;
 			break;
 		}
+#pragma omp parallel for schedule(static) num_threads(or_nthreads) if (or_nthreads > 1)
 		for (size_t k = 0; k < m; ++k)
 		{
 			x[k] += alpha * p[k];
 			r[k] -= alpha * q[k];
 		}
+#pragma omp parallel for schedule(static) num_threads(or_nthreads) if (or_nthreads > 1)
 		for (uint32_t i = 0; i < n; ++i)
 			for (int a = 0; a < 3; ++a)
 				z[3 * i + a] = Minv[9 * i + 3 * a] * r[3 * i] + Minv[9 * i + 3 * a + 1] * r[3 * i + 1]
@@ -170,6 +209,7 @@ uint32_t or_pcg(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double
 			break;
 		const double beta = rzn / rz;
 		rz = rzn;
+#pragma omp parallel for schedule(static) num_threads(or_nthreads) if (or_nthreads > 1)
 		for (size_t k = 0; k < m; ++k)
 			p[k] = z[k] + beta * p[k];
 	}

@@ -633,7 +633,10 @@ void or_prp_prepare_n(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const u
 	m->hop = malloc(sizeof(uint32_t) * 2 * (size_t)nc * V + 4);
 	m->geo = malloc(sizeof(float) * 2 * (size_t)nc * V + 4);
 	m->diameter = calloc(2 * nc + 1, 4);
-	for (uint32_t s = 0; s < 2 * nc && s < 2 * n_prepare; ++s)
+	const uint32_t nsearch = 2 * nc < 2 * n_prepare ? 2 * nc : 2 * n_prepare;
+	/* independent searches: threads (or_set_threads) change nothing */
+#pragma omp parallel for schedule(dynamic) num_threads(or_nthreads) if (or_nthreads > 1)
+	for (uint32_t s = 0; s < nsearch; ++s)
 	{
 		uint32_t* hop = m->hop + (size_t)s * V;
 		or_prp_distances(m, m->b_off[s + 1] - m->b_off[s], m->b_verts + m->b_off[s], hop, m->geo + (size_t)s * V);
@@ -659,7 +662,9 @@ void or_prp_prepare_n(or_prp_mesh* m, uint32_t N, const uint32_t* pairs, const u
 	/* directions */
 	m->dirs = calloc((size_t)nc * kMaxDirs * 3 + 1, sizeof(float));
 	m->ndirs = calloc(nc + 1, 4);
-	for (uint32_t c = 0; c < nc && c < n_prepare; ++c)
+	const uint32_t ndir = nc < n_prepare ? nc : n_prepare;
+#pragma omp parallel for schedule(dynamic) num_threads(or_nthreads) if (or_nthreads > 1)
+	for (uint32_t c = 0; c < ndir; ++c)
 	{
 		const uint32_t n = m->c_off[c + 1] - m->c_off[c];
 		float*		   pts = malloc(sizeof(float) * 3 * (n + 1));
@@ -1392,6 +1397,80 @@ static void eval_hits(const or_prp_mesh* m, uint32_t c, f3 dir, const or_prp_hit
 	free(idx);
 }
 
+/* Union-find of the hit adjacency graph (par[i] -> root, the smallest index of
+ * the component).  Every rule of or_prp_hits_adjacent relates hits whose source
+ * vertex sets share a vertex, except VF-VF, whose source vertices are 1-ring
+ * neighbours; so the candidates of hit i are the hits listed under its source
+ * vertices (and, for a VF hit, under their 1-ring), each tested with the exact
+ * predicate -- the same edges as testing all pairs (brute != 0 does that), in
+ * far less time on large slots.  The partition does not depend on the order in
+ * which edges are found. */
+static void hit_components(const or_prp_mesh* m, const or_prp_hit* H, uint32_t n, int brute, uint32_t* par)
+{
+	for (uint32_t i = 0; i < n; ++i)
+		par[i] = i;
+	if (brute)
+	{
+		for (uint32_t i = 0; i < n; ++i)
+			for (uint32_t j = i + 1; j < n; ++j)
+				if (or_prp_hits_adjacent(m, H + i, H + j))
+				{
+					const uint32_t a = uf_find(par, i), b = uf_find(par, j);
+					if (a != b)
+						par[a > b ? a : b] = a < b ? a : b;
+				}
+		return;
+	}
+	/* buckets: source vertex -> hits */
+	uint32_t* s = calloc(3 * (size_t)n + 1, sizeof(uint32_t));
+	uint32_t* d = calloc(3 * (size_t)n + 1, sizeof(uint32_t));
+	uint32_t  nb = 0;
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		uint32_t	   sv[3];
+		const uint32_t k = hit_src_verts(m, H + i, sv);
+		for (uint32_t a = 0; a < k; ++a)
+			s[nb] = sv[a], d[nb++] = i;
+	}
+	uint32_t *off, *lst;
+	build_csr(m->nv, nb, s, d, &off, &lst);
+	free(s), free(d);
+	for (uint32_t i = 0; i < n; ++i)
+	{
+		uint32_t	   sv[3];
+		const uint32_t k = hit_src_verts(m, H + i, sv);
+		for (uint32_t a = 0; a < k; ++a)
+		{
+			const uint32_t v = sv[a];
+			const uint32_t r0 = H[i].type == 0 ? m->vv_off[v] : 0, r1 = H[i].type == 0 ? m->vv_off[v + 1] : 0;
+			for (uint32_t r = r0; r <= r1; ++r)
+			{
+				const uint32_t u = r == r1 ? v : m->vv[r]; /* the 1-ring (VF only), then v itself */
+				for (uint32_t t = off[u]; t < off[u + 1]; ++t)
+				{
+					const uint32_t j = lst[t];
+					if (j <= i || !or_prp_hits_adjacent(m, H + i, H + j))
+						continue;
+					const uint32_t ra = uf_find(par, i), rb = uf_find(par, j);
+					if (ra != rb)
+						par[ra > rb ? ra : rb] = ra < rb ? ra : rb;
+				}
+			}
+		}
+	}
+	free(off), free(lst);
+}
+
+/* Components of a hit list by the bucketed search (brute = 0) or by testing all
+ * pairs (brute != 0): root[i] = smallest hit index of i's component. */
+void or_prp_hit_components(const or_prp_mesh* m, const void* hits, uint32_t n, int brute, uint32_t* root)
+{
+	const or_prp_hit* H = (const or_prp_hit*)hits;
+	hit_components(m, H, n, brute, root);
+	for (uint32_t i = 0; i < n; ++i)
+		root[i] = uf_find(root, i);
+}
+
 /* Evaluate contour c along dir at (k_src, k_dst): raw hits (all of them,
  * with .valid set on the kept cluster) and the metrics of the kept hits. */
 static uint32_t eval_slot(const or_prp_mesh* m, uint32_t c, f3 dir, uint32_t ks, uint32_t kd, or_prp_hit** hits,
@@ -1405,19 +1484,11 @@ static uint32_t eval_slot(const or_prp_mesh* m, uint32_t c, f3 dir, uint32_t ks,
 	candidates(m, c, 1, kd, cv1, ce1, cf1);
 	const uint32_t n = raycast(m, dir, cv0, ce0, cf0, cv1, ce1, cf1, hits, cap);
 	or_prp_hit*	   H = *hits;
-	/* components of the hit adjacency graph (all-pairs test: the bucketed
-	 * search of the reference visits exactly the pairs the predicates accept) */
+	/* components of the hit adjacency graph */
 	uint32_t* par = malloc(sizeof(uint32_t) * (n + 1));
 	for (uint32_t i = 0; i < n; ++i)
-		par[i] = i, H[i].valid = 0;
-	for (uint32_t i = 0; i < n; ++i)
-		for (uint32_t j = i + 1; j < n; ++j)
-			if (or_prp_hits_adjacent(m, H + i, H + j))
-			{
-				const uint32_t a = uf_find(par, i), b = uf_find(par, j);
-				if (a != b)
-					par[a > b ? a : b] = a < b ? a : b;
-			}
+		H[i].valid = 0;
+	hit_components(m, H, n, 0, par);
 	uint8_t*  attached = calloc(n + 1, 1);
 	uint32_t* idx = malloc(sizeof(uint32_t) * (n + 1));
 	for (uint32_t i = 0; i < n; ++i)
@@ -1607,7 +1678,7 @@ static uint32_t grow(uint32_t cur, uint32_t diam)
 typedef or_prp_eval_fn prp_eval_fn; /* metrics: or_prp_metrics */
 
 static uint32_t drive(uint32_t nd, uint32_t ks, uint32_t kd, uint32_t Ds, uint32_t Dd, prp_eval_fn eval, void* user,
-	combo_result* R, int* no_response)
+	combo_result* R, int* no_response, int threads)
 {
 	enum { kMaxLevels = 2 * 32 + 1 };
 	memset(R, 0, sizeof(combo_result) * (nd + 1));
@@ -1615,6 +1686,8 @@ static uint32_t drive(uint32_t nd, uint32_t ks, uint32_t kd, uint32_t Ds, uint32
 	int		 has = 0;
 	for (; level < kMaxLevels; ++level)
 	{
+		/* a level's evaluations are independent (threads > 1 only with a thread-safe eval) */
+#pragma omp parallel for schedule(dynamic) num_threads(threads) if (threads > 1)
 		for (uint32_t ci = 0; ci < nd; ++ci)
 		{
 			combo_result* r = R + ci;
@@ -1688,7 +1761,7 @@ int or_prp_drive(uint32_t nd, uint32_t ks0, uint32_t kd0, uint32_t Ds, uint32_t 
 {
 	combo_result* R = calloc(nd + 1, sizeof(combo_result));
 	int			  nr = 0;
-	*levels = drive(nd, ks0, kd0, Ds, Dd, eval, user, R, &nr);
+	*levels = drive(nd, ks0, kd0, Ds, Dd, eval, user, R, &nr, 1);
 	*no_response = (uint32_t)nr;
 	or_prp_metrics* M = metrics_out;
 	for (uint32_t ci = 0; ci < nd; ++ci)
@@ -1715,11 +1788,15 @@ typedef struct
 	uint32_t	 cap;
 } slot_eval_ctx;
 
+/* one evaluation with a hit buffer of its own, so evaluations may run in parallel */
 static void slot_eval(void* user, uint32_t ci, uint32_t ks, uint32_t kd, void* met)
 {
 	slot_eval_ctx* e = user;
 	const float*   d = e->m->dirs + ((size_t)e->c * kMaxDirs + ci) * 3;
-	eval_slot(e->m, e->c, mk(d[0], d[1], d[2]), ks, kd, &e->hits, &e->cap, met);
+	or_prp_hit*	   hits = NULL;
+	uint32_t	   cap = 0;
+	eval_slot(e->m, e->c, mk(d[0], d[1], d[2]), ks, kd, &hits, &cap, met);
+	free(hits);
 }
 
 static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
@@ -1733,7 +1810,7 @@ static void choose_contour(or_prp_mesh* m, uint32_t c, or_prp_choice* out)
 	int			   nr = 0;
 	memset(out, 0, sizeof(*out));
 	out->combo = -1;
-	out->levels = drive(nd, ks, kd, Ds, Dd, slot_eval, &e, R, &nr);
+	out->levels = drive(nd, ks, kd, Ds, Dd, slot_eval, &e, R, &nr, or_nthreads);
 	const int sel = select_combo(R, nd);
 	if (sel >= 0)
 	{

@@ -139,6 +139,23 @@ def _check_hit_geometry(f, hits, direction):
         assert 0.0 <= h["dist"] <= 1.0
 
 
+def test_bucketed_components_equal_all_pairs_on_large_slots():
+    """Slots with thousands of hits (the 48x48 fold at wide k-rings): the bucketed
+    search of hit_components finds the partition of the all-pairs test."""
+    f = prp_fixture("folded_small")
+    o = f["oracle"]
+    seen = 0
+    for c in range(min(f["nc"], 4)):
+        dirs, _, _ = o.state(c)
+        for d in dirs[:3]:
+            hits, _ = o.eval_slot(c, d, 12, 12)
+            if len(hits) < 50:
+                continue
+            assert np.array_equal(o.hit_components(hits), o.hit_components(hits, brute=True))
+            seen += 1
+    assert seen >= 2
+
+
 @pytest.mark.parametrize("name", ["two_sheets", "folded_small"])
 def test_slot_hits_geometry_components_and_metrics(name):
     from scipy.sparse import coo_matrix
@@ -171,6 +188,11 @@ def test_slot_hits_geometry_components_and_metrics(name):
                     assert o.hits_adjacent(hits[j], hits[i])
                     rows.append(i), cols.append(j)
         _, lab = connected_components(coo_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n)), directed=False)
+        # the oracle's bucketed component search = the all-pairs partition
+        root = o.hit_components(hits)
+        assert np.array_equal(root, o.hit_components(hits, brute=True))
+        for r in np.unique(root):
+            assert len(set(lab[root == r])) == 1 and (lab == lab[root == r][0]).sum() == (root == r).sum()
         valid = hits["valid"] != 0
         if valid.any():
             assert len(set(lab[valid])) == 1 and (lab == lab[valid][0]).sum() == valid.sum()
