@@ -376,6 +376,55 @@ typedef struct ut_prp_params {
 	float	 tau;			 /* PRP_intrinsic_tau */
 } ut_prp_params;
 
+/* ---- affine bodies (ABD) ------------------------------------------------------ */
+
+/* Rigid bodies as 12-DOF affine bodies: q_b = (p, a1, a2, a3), x_v = p + [a1 a2 a3] xbar_v
+ * (fn_apply_q_to_x_template, newton_solver.cpp:1882-1907).  Device arrays, kept by reference. */
+typedef struct ut_abd_desc {
+	uint32_t		nb, nv;
+	const uint32_t* vert_off; /* [nb + 1] vertices of body b: [vert_off[b], vert_off[b + 1]) */
+	const uint32_t* body;	  /* [nv] body of each vertex */
+	const float*	xbar;	  /* [3 nv] model position of each vertex */
+	const double*	mass4;	  /* [16 nb] compressed 4x4 mass matrix (init_sim_data.cpp:960-1024) */
+	const double*	volume;	  /* [nb] rest volume (abd_volume) */
+	const double*	kappa;	  /* [nb] orthogonality stiffness (RigidMaterial::stiffness) */
+} ut_abd_desc;
+
+/* fn_apply_q_to_x_template: x (FP32 and / or FP64, [3 nv]) from q ([12 nb], FP64). */
+UT_API int ut_abd_q_to_x(ut_ctx* ctx, const ut_abd_desc* desc, const double* q, float* x32, double* x64);
+
+/* q-space gradient and 12x12 blocks of AbdInertiaEnergy (abd_inertia_energy.cpp:25-140),
+ * AbdOrthoEnergy (abd_ortho_energy.cpp:25-140) and the contacts scattered onto the
+ * bodies (add_to_rigid_body, narrow_phase.cpp:2679-2698): grad [12 nb] and blocks
+ * [144 nb] (row-major per body, optional) are overwritten; energy [2] (optional):
+ * inertia, orthogonality. */
+typedef struct ut_abd_assemble_args {
+	const double*	q;
+	const double*	q_tilde; /* NULL: no inertia */
+	double			inv_h2;
+	int				quasi_static;		   /* 1: no inertia gradient / energy */
+	int				qs_no_inertia_hessian; /* with quasi_static: no inertia Hessian either */
+	const void*		contacts;
+	const uint32_t* contact_count;
+	uint32_t		contact_cap;
+	double*			grad;
+	double*			blocks;
+	double*			energy;
+} ut_abd_assemble_args;
+UT_API int ut_abd_assemble(ut_ctx* ctx, const ut_abd_desc* desc, const ut_abd_assemble_args* args);
+/* y = A p with the blocks and the contacts between bodies (matrix-free), FP64. */
+UT_API int ut_abd_apply(ut_ctx* ctx, const ut_abd_desc* desc, const double* blocks, const void* contacts,
+	const uint32_t* contact_count, uint32_t contact_cap, const double* p, double* y);
+/* Block-Jacobi (12x12) PCG of the q-space system in one workgroup (at most 1600
+ * bodies); precision UT_PCG_FP64 or UT_PCG_MIXED (FP32 inner solves, FP64
+ * refinement).  Synchronises; returns 0, or 2 when not converged. */
+typedef struct ut_abd_pcg ut_abd_pcg;
+UT_API int ut_abd_pcg_create(ut_ctx* ctx, uint32_t nb, ut_abd_pcg** out);
+UT_API int ut_abd_pcg_destroy(ut_abd_pcg* pcg);
+UT_API int ut_abd_pcg_solve(ut_abd_pcg* pcg, const ut_abd_desc* desc, const double* blocks, const void* contacts,
+	const uint32_t* contact_count, uint32_t contact_cap, const double* b, double* x, int precision,
+	uint32_t max_iters, double rel_tol, ut_pcg_report* report);
+
 /* ---- Newton step: lcs::NewtonSolver (newton_solver.h:283-326) ------------------ */
 
 /* Static mesh data prepared by the initializer (Solver/Initializer/*); all

@@ -899,3 +899,94 @@ def contact_potential(records, x, d_hat, offset, kappa=1e8, energy=1, terms=3, x
 def set_threads(n):
     """Threads of the oracle's parallel loops (1: serial, the default the tests use)."""
     lib().or_set_threads(int(n))
+
+
+# ---- affine bodies (numpy restatement) ------------------------------------------------
+# q holds 12 values per body, (p, a1, a2, a3); x_v = p + [a1 a2 a3] xbar_v
+# (fn_apply_q_to_x_template, newton_solver.cpp:1882-1907).
+
+def abd_q_to_x(body, xbar, q):
+    Q = np.asarray(q, np.float64).reshape(-1, 4, 3)[np.asarray(body, np.int64)]
+    xb = np.asarray(xbar, np.float64)
+    return Q[:, 0] + xb[:, 0:1] * Q[:, 1] + xb[:, 1:2] * Q[:, 2] + xb[:, 2:3] * Q[:, 3]
+
+
+def abd_bodies(q, q_tilde, mass4, volume, kappa, inv_h2, grad_scale=1.0, hess_scale=1.0):
+    """AbdInertiaEnergy (abd_inertia_energy.cpp:25-140) and AbdOrthoEnergy
+    (abd_ortho_energy.cpp:25-140) per body: (E_inertia, E_ortho, grad [nb, 12],
+    blocks [nb, 12, 12])."""
+    Q = np.asarray(q, np.float64).reshape(-1, 4, 3)
+    nb = Q.shape[0]
+    M4 = np.asarray(mass4, np.float64).reshape(nb, 4, 4)
+    g = np.zeros((nb, 12))
+    H = np.zeros((nb, 12, 12))
+    e_in = e_or = 0.0
+    I3 = np.eye(3)
+    for b in range(nb):
+        if q_tilde is not None:
+            d = Q[b] - np.asarray(q_tilde, np.float64).reshape(-1, 4, 3)[b]
+            G = inv_h2 * M4[b]
+            e_in += grad_scale * 0.5 * float(np.sum(G * (d @ d.T)))
+            g[b] += grad_scale * (G @ d).ravel()
+            H[b] += hess_scale * np.kron(G, I3)
+        A = Q[b, 1:]  # rows = the columns a_i of A
+        s = kappa[b] * volume[b]
+        D = A @ A.T
+        e_or += s * float(np.sum((D - I3) ** 2))
+        g[b, 3:] += 4.0 * s * ((D - I3) @ A).ravel()
+        for i in range(3):
+            for j in range(3):
+                if i == j:
+                    h = np.outer(A[i], A[i]) + (D[i, i] - 1.0) * I3 + A.T @ A
+                else:
+                    h = np.outer(A[j], A[i]) + D[i, j] * I3
+                H[b, 3 + 3 * i:6 + 3 * i, 3 + 3 * j:6 + 3 * j] += 4.0 * s * h
+    return e_in, e_or, g, H
+
+
+def abd_record_sides(rec, body, xbar):
+    """The two sides of a record (VF: vertex | face, EE: edge | edge) as (body, W) with
+    W = sum over the side's vertices of w_k (1, xbar_k) (add_to_rigid_body,
+    narrow_phase.cpp:2679-2698)."""
+    w = record_weights(rec)
+    na = 1 if rec[2] & TYPE_BIT else 2
+    sides = []
+    for ks in (range(0, na), range(na, 4)):
+        W = np.zeros(4)
+        b = None
+        for k in ks:
+            v = int(rec[k] & INDEX_MASK)
+            b = int(body[v])
+            W += w[k] * np.concatenate([[1.0], np.asarray(xbar[v], np.float64)])
+        sides.append((b, W))
+    return sides
+
+
+def abd_contacts(records, body, xbar, nb):
+    """Contact gradient [nb, 12] and own-body blocks [nb, 12, 12] of the records."""
+    g = np.zeros((nb, 12))
+    H = np.zeros((nb, 12, 12))
+    for rec in np.asarray(records, np.uint32).reshape(-1, 16):
+        (ba, Wa), (bb, Wb) = abd_record_sides(rec, body, xbar)
+        K, gp = pair_block(rec)
+        for b, W in ([(ba, Wa + Wb)] if ba == bb else [(ba, Wa), (bb, Wb)]):
+            g[b] += np.kron(W, gp)
+            H[b] += np.kron(np.outer(W, W), K)
+    return g, H
+
+
+def abd_apply(blocks, records, body, xbar, p):
+    """y = A p: per-body blocks plus the records' couplings between two bodies."""
+    nb = blocks.shape[0]
+    P = np.asarray(p, np.float64).reshape(nb, 12)
+    y = np.einsum("bij,bj->bi", blocks, P)
+    for rec in np.asarray(records, np.uint32).reshape(-1, 16):
+        (ba, Wa), (bb, Wb) = abd_record_sides(rec, body, xbar)
+        if ba == bb:
+            continue
+        K, _ = pair_block(rec)
+        ua = P[ba].reshape(4, 3).T @ Wa
+        ub = P[bb].reshape(4, 3).T @ Wb
+        y[ba] += np.kron(Wa, K @ ub)
+        y[bb] += np.kron(Wb, K @ ua)
+    return y.ravel()

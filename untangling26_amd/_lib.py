@@ -60,6 +60,12 @@ SIGNATURES = {
     "ut_friction_init": (C.c_int, [vp, vp, vp, C.c_uint32, vp]),
     "ut_friction_lagged": (C.c_int, [vp, vp, vp, C.c_uint32, vp, vp, vp, vp, vp]),
     "ut_contact_potential": (C.c_int, [vp, vp]),
+    "ut_abd_q_to_x": (C.c_int, [vp, vp, vp, vp, vp]),
+    "ut_abd_assemble": (C.c_int, [vp, vp, vp]),
+    "ut_abd_apply": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint32, vp, vp]),
+    "ut_abd_pcg_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
+    "ut_abd_pcg_destroy": (C.c_int, [vp]),
+    "ut_abd_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_int, C.c_uint32, C.c_double, vp]),
     "ut_solver_profile": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
     "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 10 + [C.POINTER(vp)]),
     "ut_prp_destroy": (C.c_int, [vp]),
@@ -133,6 +139,17 @@ class PotentialArgs(C.Structure):
 
 
 TERM_BARRIER, TERM_FRICTION = 1, 2
+
+
+class AbdDesc(C.Structure):
+    _fields_ = [("nb", C.c_uint32), ("nv", C.c_uint32), ("vert_off", vp), ("body", vp), ("xbar", vp),
+                ("mass4", vp), ("volume", vp), ("kappa", vp)]
+
+
+class AbdAssembleArgs(C.Structure):
+    _fields_ = [("q", vp), ("q_tilde", vp), ("inv_h2", C.c_double), ("quasi_static", C.c_int),
+                ("qs_no_inertia_hessian", C.c_int), ("contacts", vp), ("contact_count", vp),
+                ("contact_cap", C.c_uint32), ("grad", vp), ("blocks", vp), ("energy", vp)]
 
 
 class SolverParams(C.Structure):

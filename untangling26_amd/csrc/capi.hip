@@ -11,6 +11,7 @@
 #include "contact.hpp"
 #include "solver.hpp"
 #include "prp.hpp"
+#include "abd.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -68,6 +69,11 @@ struct ut_prp
 {
 	ut::PrpResolver r;
 	ut_ctx*			owner = nullptr;
+};
+struct ut_abd_pcg
+{
+	ut::AbdPcg s;
+	ut_ctx*	   owner = nullptr;
 };
 
 namespace
@@ -712,6 +718,120 @@ UT_API int ut_contact_potential(ut_ctx* ctx, const ut_potential_args* q)
 			a.out = q->out;
 			ut::contact_potential(&ctx->c, a);
 		});
+}
+
+// ---- affine bodies -------------------------------------------------------------
+namespace
+{
+ut::AbdDesc abd_desc(const ut_abd_desc* q)
+{
+	UT_REQUIRE(q && q->nb && q->nv && q->body && q->xbar && q->mass4 && q->volume && q->kappa,
+		"abd: incomplete body description");
+	ut::AbdDesc d;
+	d.nb = q->nb, d.nv = q->nv, d.vert_off = q->vert_off, d.body = q->body, d.xbar = q->xbar;
+	d.mass4 = q->mass4, d.volume = q->volume, d.kappa = q->kappa;
+	return d;
+}
+} // namespace
+
+UT_API int ut_abd_q_to_x(ut_ctx* ctx, const ut_abd_desc* desc, const double* q, float* x32, double* x64)
+{
+	if (!ctx || !desc || !q || (!x32 && !x64))
+		return ut::kInvalid;
+	return guarded(&ctx->c, [&] { ut::abd_q_to_x(&ctx->c, abd_desc(desc), q, x32, x64); });
+}
+
+UT_API int ut_abd_assemble(ut_ctx* ctx, const ut_abd_desc* desc, const ut_abd_assemble_args* q)
+{
+	if (!ctx || !desc || !q || !q->q || !q->grad)
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			ut::AbdAssembleArgs a;
+			a.q = q->q;
+			a.q_tilde = q->q_tilde;
+			a.inv_h2 = q->inv_h2;
+			a.inertia_grad_scale = q->quasi_static ? 0.0 : 1.0;
+			a.inertia_hess_scale = (q->quasi_static && q->qs_no_inertia_hessian) ? 0.0 : 1.0;
+			a.contacts = reinterpret_cast<const ut::ContactRec*>(q->contacts);
+			a.contact_count = q->contact_count;
+			a.contact_cap = q->contacts ? q->contact_cap : 0u;
+			a.grad = q->grad;
+			a.blocks = q->blocks;
+			a.energy = q->energy;
+			ut::abd_assemble(&ctx->c, abd_desc(desc), a);
+		});
+}
+
+UT_API int ut_abd_apply(ut_ctx* ctx, const ut_abd_desc* desc, const double* blocks, const void* contacts,
+	const uint32_t* contact_count, uint32_t contact_cap, const double* p, double* y)
+{
+	if (!ctx || !desc || !blocks || !p || !y || (contacts && !contact_count))
+		return ut::kInvalid;
+	return guarded(&ctx->c,
+		[&]
+		{
+			ut::abd_apply(&ctx->c, abd_desc(desc), blocks, reinterpret_cast<const ut::ContactRec*>(contacts),
+				contacts ? contact_count : nullptr, contacts ? contact_cap : 0u, p, y);
+		});
+}
+
+UT_API int ut_abd_pcg_create(ut_ctx* ctx, uint32_t nb, ut_abd_pcg** out)
+{
+	if (!ctx || !out || nb == 0)
+		return ut::kInvalid;
+	*out = nullptr;
+	ut_abd_pcg* h = new (std::nothrow) ut_abd_pcg;
+	if (!h)
+		return ut::kInternal;
+	const int rc = guarded(&ctx->c, [&] { h->s.init(&ctx->c, nb); });
+	if (rc != ut::kOk)
+	{
+		delete h;
+		return rc;
+	}
+	h->owner = ctx;
+	ctx_retain(ctx);
+	*out = h;
+	return ut::kOk;
+}
+
+UT_API int ut_abd_pcg_destroy(ut_abd_pcg* pcg)
+{
+	if (!pcg)
+		return ut::kInvalid;
+	destroy_handle(pcg, pcg->s.ctx);
+	return ut::kOk;
+}
+
+UT_API int ut_abd_pcg_solve(ut_abd_pcg* pcg, const ut_abd_desc* desc, const double* blocks, const void* contacts,
+	const uint32_t* contact_count, uint32_t contact_cap, const double* b, double* x, int precision,
+	uint32_t max_iters, double rel_tol, ut_pcg_report* report)
+{
+	if (!pcg || !desc || !blocks || !b || !x || (contacts && !contact_count))
+		return ut::kInvalid;
+	ut::AbdPcgReport r;
+	const int		 rc = guarded(pcg->s.ctx,
+				[&]
+				{
+			UT_REQUIRE(precision == ut::kPcgFp64 || precision == ut::kPcgMixed,
+				"abd pcg: precision must be UT_PCG_FP64 or UT_PCG_MIXED");
+			r = pcg->s.solve(abd_desc(desc), blocks, reinterpret_cast<const ut::ContactRec*>(contacts),
+				contacts ? contact_count : nullptr, contacts ? contact_cap : 0u, b, x, precision, max_iters, rel_tol);
+				});
+	if (rc != ut::kOk)
+		return rc;
+	if (report)
+	{
+		std::memset(report, 0, sizeof(*report));
+		report->relative_residual = r.relative_residual;
+		report->iterations = r.iterations;
+		report->refinements = r.refinements;
+		report->converged = r.converged;
+		report->failure = r.failure;
+	}
+	return r.converged ? ut::kOk : ut::kNotConverged;
 }
 
 UT_API int ut_contact_query(ut_ctx* ctx, int kind, const ut_contact_args* args, ut_lbvh* tree,

@@ -128,3 +128,104 @@ def setup_cloth(mesh: Mesh, mat: ClothMaterial | None = None, x_rest: np.ndarray
         rest_face_area=fa, dm=dm_inv(xr, faces), mu_lambda=mu_lambda, hinges=hinges, rest_angle=rest_angle,
         bend_stiff_area=bend, pattern=pattern, face_slots=face_slots.astype(np.uint32),
         hinge_slots=hinge_slots.astype(np.uint32))
+
+
+# ---- affine bodies -------------------------------------------------------------------
+@dataclass
+class RigidMaterial:
+    """RigidMaterial fields used by the affine-body path: orthogonality stiffness
+    (RigidMaterial::stiffness), density, contact d_hat and offset."""
+    stiffness: float = 1e5
+    density: float = 1e3
+    d_hat: float = 1e-3
+    contact_offset: float = 0.0
+    thickness: float = 1e-3  # lumped vertex mass of the PRP displacement cost
+
+
+def dyadic_mass(xbar: np.ndarray, faces: np.ndarray, density: float):
+    """Mass, first moment and second moment (dyadic) of the solid bounded by a closed,
+    outward-oriented triangle mesh (compute_trimesh_dyadic_mass, init_sim_data.cpp:988-997):
+    sums over the tetrahedra (0, a, b, c) with det = a . (b x c):
+    M = rho sum det / 6, MI = rho sum det (a + b + c) / 24,
+    I = rho sum det (a a^T + b b^T + c c^T + s s^T) / 120, s = a + b + c."""
+    x = np.asarray(xbar, np.float64)
+    f = np.asarray(faces, np.int64)
+    a, b, c = x[f[:, 0]], x[f[:, 1]], x[f[:, 2]]
+    det = np.einsum("ij,ij->i", a, np.cross(b, c))
+    s = a + b + c
+    M = density * det.sum() / 6.0
+    MI = density * (det[:, None] * s).sum(0) / 24.0
+    outer = lambda u: u[:, :, None] * u[:, None, :]
+    I = density * (det[:, None, None] * (outer(a) + outer(b) + outer(c) + outer(s))).sum(0) / 120.0
+    return M, MI, I
+
+
+def mass4(M, MI, I, eps=1.1920929e-7):
+    """The compressed 4x4 mass matrix [[M, MI^T], [MI, I]] with its diagonal clamped
+    to Epsilon (init_sim_data.cpp:1000-1021)."""
+    m = np.zeros((4, 4))
+    m[0, 0] = M
+    m[0, 1:] = m[1:, 0] = MI
+    m[1:, 1:] = I
+    d = np.maximum(np.diag(m), eps)
+    np.fill_diagonal(m, d)
+    return m
+
+
+@dataclass
+class BodySetup:
+    nb: int
+    nv: int
+    vert_off: np.ndarray  # uint32 [nb + 1]
+    body: np.ndarray  # uint32 [nv]
+    xbar: np.ndarray  # float32 [nv, 3]
+    mass4: np.ndarray  # float64 [nb, 4, 4]
+    volume: np.ndarray  # float64 [nb]
+    kappa: np.ndarray  # float64 [nb]
+    q0: np.ndarray  # float64 [nb, 12]
+    # collision data of the vertex mesh
+    x: np.ndarray
+    edges: np.ndarray
+    faces: np.ndarray
+    face_adj_edges: np.ndarray
+    edge_adj_faces: np.ndarray
+    mass: np.ndarray
+    d_hat: np.ndarray
+    offset: np.ndarray
+    vprop: np.ndarray
+    rest_vert_area: np.ndarray
+    rest_edge_area: np.ndarray
+    rest_face_area: np.ndarray
+
+
+def setup_bodies(scene, mat: RigidMaterial | None = None) -> BodySetup:
+    """Affine-body data of a scenes.RigidScene: per body the compressed mass matrix of
+    its solid (surface integration), its volume and orthogonality stiffness; per vertex
+    its body, model position and the rigid property word (kPropRigid | body << 8: the
+    DCD skips pairs inside one body)."""
+    mat = mat or RigidMaterial()
+    m = scene.mesh
+    nb = len(scene.vert_off) - 1
+    faces = np.ascontiguousarray(m.faces, np.uint32)
+    edges = np.ascontiguousarray(m.edges, np.uint32)
+    fae, eaf = topology.mesh_topology(edges, faces)
+    va, ea, fa = topology.rest_areas(m.x, edges, faces, eaf)
+    M4 = np.zeros((nb, 4, 4))
+    vol = np.zeros(nb)
+    fb = scene.body[faces[:, 0]]
+    order = np.argsort(fb, kind="stable")
+    foff = np.searchsorted(fb[order], np.arange(nb + 1))
+    for b in range(nb):
+        fsel = faces[order[foff[b]:foff[b + 1]]]
+        Mb, MI, I = dyadic_mass(scene.xbar, fsel, mat.density)
+        M4[b] = mass4(Mb, MI, I)
+        vol[b] = Mb / mat.density
+    q0 = scene.q0.reshape(nb, 12)
+    vprop = (np.uint32(2) | (scene.body.astype(np.uint32) << np.uint32(8))).astype(np.uint32)
+    return BodySetup(
+        nb=nb, nv=m.nv, vert_off=np.asarray(scene.vert_off, np.uint32), body=np.asarray(scene.body, np.uint32),
+        xbar=np.ascontiguousarray(scene.xbar, np.float32), mass4=M4, volume=vol, kappa=np.full(nb, mat.stiffness),
+        q0=np.ascontiguousarray(q0, np.float64), x=np.ascontiguousarray(m.x, np.float32), edges=edges, faces=faces,
+        face_adj_edges=fae, edge_adj_faces=eaf, mass=(va * mat.density * mat.thickness).astype(np.float32),
+        d_hat=np.full(m.nv, mat.d_hat, np.float32), offset=np.full(m.nv, 0.5 * mat.contact_offset, np.float32),
+        vprop=vprop, rest_vert_area=va, rest_edge_area=ea, rest_face_area=fa)

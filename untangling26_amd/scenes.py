@@ -246,3 +246,100 @@ def draped_cloth(seed: int = 0, n: int = 64, side: float = 1.0, radius: float = 
         st[b, n * n:] = sv
     mesh = Mesh(x, faces, name=f"draped_cloth_{n}x{n}_ico{level}")
     return mesh, rest.astype(np.float32), n * n, st
+
+
+def subdivided_box(n: int = 8, half: float = 1.0) -> tuple[np.ndarray, np.ndarray]:
+    """Surface of the cube [-half, half]^3 with every face an n x n grid of quads
+    (two triangles each, outward normals): 6 n^2 + 2 vertices... precisely
+    (n+1)^3 - (n-1)^3 vertices and 12 n^2 triangles."""
+    idx = {}
+    verts = []
+
+    def vid(p):
+        k = tuple(int(c) for c in p)
+        if k not in idx:
+            idx[k] = len(verts)
+            verts.append(k)
+        return idx[k]
+
+    faces = []
+    # each face: fixed axis a at side s (0 or n), the other two axes (u, v) ordered for an outward normal
+    for a in range(3):
+        u, v = (a + 1) % 3, (a + 2) % 3
+        for s in (0, n):
+            for i in range(n):
+                for j in range(n):
+                    q = []
+                    for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = [0, 0, 0]
+                        p[a], p[u], p[v] = s, i + di, j + dj
+                        q.append(vid(p))
+                    if s == 0:
+                        q = q[::-1]
+                    faces += [(q[0], q[1], q[2]), (q[0], q[2], q[3])]
+    x = (np.asarray(verts, np.float64) / n * 2.0 - 1.0) * half
+    return x, np.asarray(faces, np.uint32)
+
+
+def _random_rotation(rng) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+@dataclass
+class RigidScene:
+    mesh: Mesh  # all bodies' vertices (body by body) at q0
+    body: np.ndarray  # uint32 [V]: body of each vertex
+    xbar: np.ndarray  # float32 [V, 3]: model position of each vertex (body frame, centred)
+    q0: np.ndarray  # float64 [B, 4, 3]: p, then the columns a1, a2, a3 of A (x = p + A xbar)
+    radius: np.ndarray  # float64 [B]
+    kind: np.ndarray  # uint8 [B]: 0 icosphere, 1 box
+    vert_off: np.ndarray  # uint32 [B + 1]
+
+
+def rigid_bodies(seed: int = 0, n_bodies: int = 512, cube: float = 2.0, r_lo: float = 0.05, r_hi: float = 0.2,
+                 max_overlap: float = 0.3, ico_level: int = 3, box_n: int = 8) -> RigidScene:
+    """Affine-body workload: bodies alternating icosphere (level 3, 642 vertices) and
+    8x8x8-subdivided box (386 vertices, half-extent = the radius), radii U(r_lo, r_hi),
+    random orientations, centres drawn uniformly in the cube; a centre is kept when
+    it overlaps no placed body by more than max_overlap of the smaller radius
+    (centre distance >= r_i + r_j - max_overlap min(r_i, r_j))."""
+    rng = np.random.default_rng(seed)
+    sv, sf = icosphere(ico_level, 1.0)
+    bv, bf = subdivided_box(box_n, 1.0)
+    centres, radii = [], []
+    attempts = 0
+    while len(centres) < n_bodies:
+        attempts += 1
+        if attempts > 2_000_000:
+            raise RuntimeError("rigid_bodies: could not place the bodies")
+        r = rng.uniform(r_lo, r_hi)
+        c = rng.uniform(r, cube - r, 3)
+        if centres:
+            C = np.asarray(centres)
+            R = np.asarray(radii)
+            d = np.linalg.norm(C - c, axis=1)
+            if np.any(d < r + R - max_overlap * np.minimum(r, R)):
+                continue
+        centres.append(c)
+        radii.append(r)
+    xs, fs, body, xbar, q0, kinds, off = [], [], [], [], [], [], [0]
+    for b in range(n_bodies):
+        kind = b % 2
+        v, f = (sv, sf) if kind == 0 else (bv, bf)
+        model = v * radii[b]
+        Rm = _random_rotation(rng)
+        xs.append(centres[b] + model @ Rm.T)
+        fs.append(f + off[-1])
+        body.append(np.full(len(v), b, np.uint32))
+        xbar.append(model)
+        q0.append(np.stack([centres[b], Rm[:, 0], Rm[:, 1], Rm[:, 2]]))
+        kinds.append(kind)
+        off.append(off[-1] + len(v))
+    mesh = Mesh(np.concatenate(xs), np.concatenate(fs), name=f"rigid_bodies_{n_bodies}")
+    return RigidScene(mesh, np.concatenate(body), np.concatenate(xbar).astype(np.float32), np.asarray(q0),
+                      np.asarray(radii), np.asarray(kinds, np.uint8), np.asarray(off, np.uint32))
