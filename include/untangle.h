@@ -456,6 +456,9 @@ typedef struct ut_solver_desc {
 	const float*	rest_x;		 /* [3V] rest positions for the PRP rest geodesics; NULL = x0 */
 	const float*	friction_mu; /* [V] friction coefficient per vertex (sa_contact_active_verts_friction_coeff,
 									the material's friction_mu, default 0.5); NULL: no friction */
+	const struct ut_abd_desc* abd; /* affine bodies: every vertex belongs to a body, the step solves in
+									  q-space (NULL: cloth, the SELL-64 pattern fields are required) */
+	const double*			  q0;  /* [12 nb] initial q of the affine bodies */
 } ut_solver_desc;
 
 /* SceneParams fields the step uses (scene_params.h:24-157). */
@@ -522,6 +525,10 @@ UT_API int ut_solver_profile(ut_solver* s, int what, uint32_t reps, float* avg_m
  * the last Newton iteration (the positions the last detection, contours and
  * PRP saw; sa_x_iter_start), 2 = start of the last step (sa_x_step_start). */
 UT_API int ut_solver_get_positions(ut_solver* s, int which, float* x);
+/* Affine bodies: q and its velocity (FP64 [12 nb] device arrays; either may be NULL).
+ * ut_solver_set_q also recomputes the vertex positions x = x(q). */
+UT_API int ut_solver_get_q(ut_solver* s, double* q, double* vq);
+UT_API int ut_solver_set_q(ut_solver* s, const double* q, const double* vq);
 /* The PRP response pairs of the last Newton iteration (target_point_template_pairs
  * and target_point_template_pairs_indices, intersection_resolver2.cpp:2105-2221):
  * *count = their number; the first min(count, cap) records (64 B each) and index

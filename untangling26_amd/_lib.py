@@ -66,6 +66,8 @@ SIGNATURES = {
     "ut_abd_pcg_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
     "ut_abd_pcg_destroy": (C.c_int, [vp]),
     "ut_abd_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_int, C.c_uint32, C.c_double, vp]),
+    "ut_solver_get_q": (C.c_int, [vp, vp, vp]),
+    "ut_solver_set_q": (C.c_int, [vp, vp, vp]),
     "ut_solver_profile": (C.c_int, [vp, C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
     "ut_prp_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32] + [vp] * 10 + [C.POINTER(vp)]),
     "ut_prp_destroy": (C.c_int, [vp]),
@@ -129,7 +131,8 @@ class SolverDesc(C.Structure):
                           "rest_vert_area", "rest_edge_area", "rest_face_area", "dm", "mu_lambda", "hinges",
                           "rest_angle", "bend_stiff_area")] + [
         ("nslices", C.c_uint32), ("nslots", C.c_uint32)] + [
-        (n, vp) for n in ("slice_off", "col", "diag_slot", "face_slots", "hinge_slots", "rest_x", "friction_mu")]
+        (n, vp) for n in ("slice_off", "col", "diag_slot", "face_slots", "hinge_slots", "rest_x", "friction_mu",
+                          "abd", "q0")]
 
 
 class PotentialArgs(C.Structure):

@@ -969,6 +969,11 @@ UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_so
 			d.hinge_slots = desc->hinge_slots;
 			d.rest_x = desc->rest_x;
 			d.friction_mu = desc->friction_mu;
+			if (desc->abd)
+			{
+				d.abd = abd_desc(desc->abd);
+				d.q0 = desc->q0;
+			}
 			h->s.init(&ctx->c, d, to_params(*params), x0, v0);
 		});
 	if (rc != ut::kOk)
@@ -1362,6 +1367,41 @@ UT_API int ut_solver_get_positions(ut_solver* s, int which, float* x)
 			const float* src = which == 0 ? s->s.x.get() : which == 1 ? s->s.x_iter_start.get() : s->s.x_step_start.get();
 			UT_HIP_CHECK(
 				hipMemcpyAsync(x, src, 12 * (size_t)s->s.d.nv, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+		});
+}
+
+UT_API int ut_solver_get_q(ut_solver* s, double* q, double* vq)
+{
+	if (!s || (!q && !vq))
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			UT_REQUIRE(s->s.abd_mode(), "solver: no affine bodies");
+			const size_t bytes = 96 * (size_t)s->s.d.abd.nb;
+			if (q)
+				UT_HIP_CHECK(hipMemcpyAsync(q, s->s.q.get(), bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+			if (vq)
+				UT_HIP_CHECK(hipMemcpyAsync(vq, s->s.vq.get(), bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+		});
+}
+
+UT_API int ut_solver_set_q(ut_solver* s, const double* q, const double* vq)
+{
+	if (!s || (!q && !vq))
+		return ut::kInvalid;
+	return guarded(s->s.ctx,
+		[&]
+		{
+			UT_REQUIRE(s->s.abd_mode(), "solver: no affine bodies");
+			const size_t bytes = 96 * (size_t)s->s.d.abd.nb;
+			if (q)
+			{
+				UT_HIP_CHECK(hipMemcpyAsync(s->s.q.get(), q, bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
+				ut::abd_q_to_x(s->s.ctx, s->s.d.abd, s->s.q.get(), s->s.x.get(), nullptr);
+			}
+			if (vq)
+				UT_HIP_CHECK(hipMemcpyAsync(s->s.vq.get(), vq, bytes, hipMemcpyDeviceToDevice, s->s.ctx->stream));
 		});
 }
 

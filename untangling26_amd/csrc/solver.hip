@@ -75,6 +75,50 @@ __global__ void k_zero_u32(uint32_t* p, uint32_t n)
 		p[i] = 0u;
 }
 
+// affine bodies: q_tilde = q + h v_q + h^2 g on the translation (fn_predict_position's
+// affine branch, newton_solver.cpp:600-640; quasi-static drops v_q)
+__global__ void k_abd_predict(uint32_t nb, const double* __restrict__ q, const double* __restrict__ vq, double h,
+	double gx, double gy, double gz, int quasi_static, double* __restrict__ q_tilde)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= 12 * nb)
+		return;
+	const uint32_t k = i % 12;
+	const double   g = k == 0 ? gx : k == 1 ? gy : k == 2 ? gz : 0.0;
+	const double   vp = (quasi_static ? 0.0 : vq[i]) + h * g;
+	q_tilde[i] = q[i] + h * vp;
+}
+
+__global__ void k_abd_trial(uint32_t n, const double* __restrict__ q0, const double* __restrict__ dq,
+	const float* __restrict__ toi, double* __restrict__ q)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	double t = 1.0;
+	if (toi)
+	{
+		const float f = *toi;
+		t = f != 1.0f ? 0.9 * (double)f : 1.0;
+	}
+	q[i] = q0[i] + t * dq[i];
+}
+
+__global__ void k_abd_velocity(uint32_t n, const double* __restrict__ q, const double* __restrict__ q0, double h,
+	double damping, double* __restrict__ vq)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		vq[i] = (q[i] - q0[i]) / h * exp(-damping * h);
+}
+
+__global__ void k_neg(uint32_t n, const double* __restrict__ a, double* __restrict__ b)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		b[i] = -a[i];
+}
+
 static uint32_t g(size_t n) { return (uint32_t)((n + 255) / 256); }
 
 void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& params, const float* x0, const float* v0)
@@ -83,7 +127,14 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 	d = desc;
 	p = params;
 	UT_REQUIRE(d.nv && d.nf && d.ne && d.edges && d.faces && d.mass && d.d_hat && d.offset, "solver: missing mesh data");
-	UT_REQUIRE(d.slice_off && d.col && d.diag_slot && d.face_slots, "solver: missing block pattern");
+	if (abd_mode())
+	{
+		UT_REQUIRE(d.abd.nv == d.nv && d.abd.body && d.abd.xbar && d.abd.mass4 && d.abd.volume && d.abd.kappa && d.q0,
+			"solver: affine bodies need every vertex's body, model position, the mass matrices and q0");
+		UT_REQUIRE(d.vprop, "solver: affine bodies need the rigid property word of every vertex (kPropRigid | body << 8)");
+	}
+	else
+		UT_REQUIRE(d.slice_off && d.col && d.diag_slot && d.face_slots, "solver: missing block pattern");
 	const size_t n3 = 3 * (size_t)d.nv;
 	x.reserve(n3);
 	v.reserve(n3);
@@ -106,7 +157,20 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 		UT_HIP_CHECK(hipMemsetAsync(v.get(), 0, 4 * n3, ctx->stream));
 	face_tree.init(ctx, kTreeFace, d.nf);
 	edge_tree.init(ctx, kTreeEdge, d.ne);
-	pcg.init(ctx, d.nv, d.nslices, d.nslots, d.slice_off, d.col, d.diag_slot);
+	if (abd_mode())
+	{
+		const size_t nq = 12 * (size_t)d.abd.nb;
+		for (DevBuf<double>* b : { &q, &vq, &q_step_start, &q_iter_start, &q_tilde, &gq, &rhs_q, &dq, &q_trial })
+			b->reserve(nq);
+		blocks.reserve(144 * (size_t)d.abd.nb);
+		abd_energy.reserve(2);
+		UT_HIP_CHECK(hipMemcpyAsync(q.get(), d.q0, 8 * nq, hipMemcpyDeviceToDevice, ctx->stream));
+		UT_HIP_CHECK(hipMemsetAsync(vq.get(), 0, 8 * nq, ctx->stream));
+		abd_q_to_x(ctx, d.abd, q.get(), x.get(), nullptr);
+		abd_pcg.init(ctx, d.abd.nb);
+	}
+	else
+		pcg.init(ctx, d.nv, d.nslices, d.nslots, d.slice_off, d.col, d.diag_slot);
 	if (d.face_adj_edges && d.edge_adj_faces)
 	{
 		contours.init(ctx, d.nv, d.ne, d.nf, d.edges, d.faces, d.face_adj_edges, d.edge_adj_faces);
@@ -276,6 +340,28 @@ void NewtonSolver::detect(StepReport& rep)
 	throw Error(kCapacity, "contact lists still out of range after 10 resize attempts");
 }
 
+// device_ccd_line_search (newton_solver.cpp:1446-1483): the global time of impact
+// from x_iter_start to x_end over the swept trees, pairs across a contour skipped
+void NewtonSolver::ccd_line_search()
+{
+	face_tree.update_leaves_and_refit(d.offset, x_iter_start.get(), x_end.get(), d.faces);
+	edge_tree.update_leaves_and_refit(d.offset, x_iter_start.get(), x_end.get(), d.edges);
+	ContactArgs c;
+	c.num_verts = d.nv;
+	c.num_edges = d.ne;
+	c.x = x_iter_start.get();
+	c.x_end = x_end.get();
+	c.edges = d.edges;
+	c.faces = d.faces;
+	c.vprop = d.vprop;
+	c.d_hat = d.d_hat;
+	c.offset = d.offset;
+	c.toi = toi.get();
+	c.filter = regions.view();
+	contact_query(ctx, 2, c, &face_tree, nullptr, nullptr, 0);
+	contact_query(ctx, 3, c, &edge_tree, nullptr, nullptr, 0);
+}
+
 StepReport NewtonSolver::physics_step_GPU()
 {
 	StepReport	   rep;
@@ -291,13 +377,23 @@ StepReport NewtonSolver::physics_step_GPU()
 		edge_tree.reduce_aabb(x.get(), d.edges);
 		edge_tree.construct_tree();
 	}
-	k_predict<<<g(d.nv), 256, 0, s>>>(d.nv, x.get(), v.get(), h, p.gravity[0], p.gravity[1], p.gravity[2], d.vprop,
-		p.quasi_static, x_tilde.get());
+	if (abd_mode())
+	{
+		const uint32_t nq = 12 * d.abd.nb;
+		UT_HIP_CHECK(hipMemcpyAsync(q_step_start.get(), q.get(), 8 * (size_t)nq, hipMemcpyDeviceToDevice, s));
+		k_abd_predict<<<g(nq), 256, 0, s>>>(d.abd.nb, q.get(), vq.get(), h, p.gravity[0], p.gravity[1],
+			p.gravity[2], p.quasi_static, q_tilde.get());
+	}
+	else
+		k_predict<<<g(d.nv), 256, 0, s>>>(d.nv, x.get(), v.get(), h, p.gravity[0], p.gravity[1], p.gravity[2],
+			d.vprop, p.quasi_static, x_tilde.get());
 	const bool fp64 = p.pcg.precision == kPcgFp64;
 	float	   t_det = 0, t_ct = 0, t_asm = 0, t_pcg = 0, t_ccd = 0, t_prp = 0;
 	for (int it = 0; it < std::max(1, p.nonlinear_iters); ++it)
 	{
 		UT_HIP_CHECK(hipMemcpyAsync(x_iter_start.get(), x.get(), 4 * n3, hipMemcpyDeviceToDevice, s));
+		if (abd_mode())
+			UT_HIP_CHECK(hipMemcpyAsync(q_iter_start.get(), q.get(), 96 * (size_t)d.abd.nb, hipMemcpyDeviceToDevice, s));
 		UT_HIP_CHECK(hipEventRecord(ev[1], s));
 		uint32_t ncontacts = 0;
 		if (p.use_self_collision)
@@ -390,6 +486,57 @@ StepReport NewtonSolver::physics_step_GPU()
 			UT_HIP_CHECK(hipStreamSynchronize(s));
 		}
 		UT_HIP_CHECK(hipEventRecord(ev[3], s));
+		if (abd_mode())
+		{
+			// q-space assembly and solve (the affine-body branch of the Newton step)
+			const uint32_t	nq = 12 * d.abd.nb;
+			AbdAssembleArgs aa;
+			aa.q = q.get();
+			aa.q_tilde = q_tilde.get();
+			aa.inv_h2 = 1.0 / (h * h);
+			aa.inertia_grad_scale = p.quasi_static ? 0.0 : 1.0;
+			aa.inertia_hess_scale = (p.quasi_static && !p.qs_inertia_hessian) ? 0.0 : 1.0;
+			aa.contacts = ncontacts ? contact_pairs.get() : nullptr;
+			aa.contact_count = counts.get() + 3;
+			aa.contact_cap = ncontacts ? cap_contacts : 0u;
+			aa.grad = gq.get();
+			aa.blocks = blocks.get();
+			aa.energy = abd_energy.get();
+			abd_assemble(ctx, d.abd, aa);
+			k_neg<<<g(nq), 256, 0, s>>>(nq, gq.get(), rhs_q.get());
+			UT_HIP_CHECK(hipEventRecord(ev[4], s));
+			const AbdPcgReport pr = abd_pcg.solve(d.abd, blocks.get(), ncontacts ? contact_pairs.get() : nullptr,
+				ncontacts ? counts.get() + 3 : nullptr, ncontacts ? cap_contacts : 0u, rhs_q.get(), dq.get(),
+				p.pcg.precision == kPcgFp64 ? kPcgFp64 : kPcgMixed, p.pcg.max_iters, p.pcg.rel_tol);
+			rep.pcg_iterations += pr.iterations;
+			rep.pcg_refinements += pr.refinements;
+			rep.pcg_relative_residual = pr.relative_residual;
+			rep.pcg_converged = pr.converged;
+			UT_HIP_CHECK(hipEventRecord(ev[5], s));
+			// trial positions x(q + dq), CCD, then q += alpha dq and x = x(q)
+			k_abd_trial<<<g(nq), 256, 0, s>>>(nq, q_iter_start.get(), dq.get(), nullptr, q_trial.get());
+			abd_q_to_x(ctx, d.abd, q_trial.get(), x_end.get(), nullptr);
+			toi_reset(ctx, toi.get());
+			if (p.use_self_collision && p.use_ccd)
+				ccd_line_search();
+			k_abd_trial<<<g(nq), 256, 0, s>>>(nq, q_iter_start.get(), dq.get(), toi.get(), q.get());
+			abd_q_to_x(ctx, d.abd, q.get(), x.get(), nullptr);
+			UT_HIP_CHECK(hipEventRecord(ev[6], s));
+			UT_HIP_CHECK(hipEventSynchronize(ev[6]));
+			float ms;
+			UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[1], ev[2]));
+			t_det += ms;
+			UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[2], ev[3]));
+			t_ct += ms;
+			UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[3], ev[4]));
+			t_asm += ms;
+			UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[4], ev[5]));
+			t_pcg += ms;
+			UT_HIP_CHECK(hipEventElapsedTime(&ms, ev[5], ev[6]));
+			t_ccd += ms;
+			has_system = true;
+			continue;
+		}
 		AssembleArgs a;
 		a.nv = d.nv;
 		a.nf = d.nf;
@@ -438,24 +585,7 @@ StepReport NewtonSolver::physics_step_GPU()
 		k_trial<<<g(n3), 256, 0, s>>>(n3, x_iter_start.get(), dx.get(), x_end.get());
 		toi_reset(ctx, toi.get());
 		if (p.use_self_collision && p.use_ccd)
-		{
-			face_tree.update_leaves_and_refit(d.offset, x_iter_start.get(), x_end.get(), d.faces);
-			edge_tree.update_leaves_and_refit(d.offset, x_iter_start.get(), x_end.get(), d.edges);
-			ContactArgs c;
-			c.num_verts = d.nv;
-			c.num_edges = d.ne;
-			c.x = x_iter_start.get();
-			c.x_end = x_end.get();
-			c.edges = d.edges;
-			c.faces = d.faces;
-			c.vprop = d.vprop;
-			c.d_hat = d.d_hat;
-			c.offset = d.offset;
-			c.toi = toi.get();
-			c.filter = regions.view();
-			contact_query(ctx, 2, c, &face_tree, nullptr, nullptr, 0);
-			contact_query(ctx, 3, c, &edge_tree, nullptr, nullptr, 0);
-		}
+			ccd_line_search();
 		k_apply_step<<<g(n3), 256, 0, s>>>(n3, x_iter_start.get(), dx.get(), toi.get(), x.get());
 		UT_HIP_CHECK(hipEventRecord(ev[6], s));
 		UT_HIP_CHECK(hipEventSynchronize(ev[6]));
@@ -472,9 +602,18 @@ StepReport NewtonSolver::physics_step_GPU()
 		t_ccd += ms;
 	}
 	k_update_velocity<<<g(n3), 256, 0, s>>>(n3, x.get(), x_step_start.get(), h, p.damping, v.get());
+	if (abd_mode())
+		k_abd_velocity<<<g(12 * (size_t)d.abd.nb), 256, 0, s>>>(12 * d.abd.nb, q.get(), q_step_start.get(), h,
+			p.damping, vq.get());
 	UT_HIP_CHECK(hipEventRecord(ev[7], s));
 	UT_HIP_CHECK(hipMemcpyAsync(&rep.toi, toi.get(), 4, hipMemcpyDeviceToHost, s));
-	UT_HIP_CHECK(hipMemcpyAsync(rep.energy, energy.get(), 24, hipMemcpyDeviceToHost, s));
+	if (abd_mode())
+	{
+		rep.energy[2] = 0.0;
+		UT_HIP_CHECK(hipMemcpyAsync(rep.energy, abd_energy.get(), 16, hipMemcpyDeviceToHost, s));
+	}
+	else
+		UT_HIP_CHECK(hipMemcpyAsync(rep.energy, energy.get(), 24, hipMemcpyDeviceToHost, s));
 	UT_HIP_CHECK(hipEventSynchronize(ev[7]));
 	UT_HIP_CHECK(hipStreamSynchronize(s));
 	float total;
@@ -503,6 +642,7 @@ void NewtonSolver::set_params(const SolverParams& np)
 PcgReport NewtonSolver::resolve_last_system(int precision, float* ms)
 {
 	UT_REQUIRE(has_system, "solver: no system assembled yet (run a step first)");
+	UT_REQUIRE(!abd_mode(), "solver: the probe re-solve is for the cloth system (affine bodies solve in q-space)");
 	UT_REQUIRE(precision == kPcgFp64 || precision == kPcgMixed, "solver: probe precision must be FP64 or mixed");
 	UT_REQUIRE(precision == kPcgFp64 || p.pcg.precision != kPcgFp64,
 		"solver: the FP32 matrix copy exists only when the solver runs the mixed or FP32 PCG");

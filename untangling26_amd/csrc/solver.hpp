@@ -9,6 +9,7 @@
 #include "pcg.hpp"
 #include "prp.hpp"
 #include "regions.hpp"
+#include "abd.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -47,6 +48,9 @@ struct SolverDesc
 	const uint32_t* diag_slot = nullptr;
 	const uint32_t* face_slots = nullptr;
 	const uint32_t* hinge_slots = nullptr;
+	// affine bodies (nb > 0): every vertex belongs to a body and the step solves in q-space
+	AbdDesc		  abd;
+	const double* q0 = nullptr; // [12 nb]
 };
 
 struct SolverParams
@@ -115,6 +119,10 @@ struct NewtonSolver
 	DevBuf<uint32_t>	ef_ids;
 	DevBuf<uint32_t>	counts; // [0] VF, [1] EE, [2] EF, [3] total contacts, [5] friction pairs
 	DevBuf<ContactRec>	fric_pairs; // the first iteration's proximity records (lagged friction)
+	// affine bodies: q, its velocity, step / iteration starts, inertia target, gradient, blocks, step
+	DevBuf<double>		q, vq, q_step_start, q_iter_start, q_tilde, gq, rhs_q, dq, q_trial, blocks, abd_energy;
+	AbdPcg				abd_pcg;
+	bool				abd_mode() const { return d.abd.nb > 0; }
 	uint32_t			nfric = 0;
 	DevBuf<uint32_t>	vert_adj_off, vert_adj_edges; // edges around each vertex (ascending)
 	uint32_t*			h_counts = nullptr;
@@ -140,6 +148,7 @@ struct NewtonSolver
 
 private:
 	void detect(StepReport& rep);
+	void ccd_line_search();
 	void build_vert_adj_edges(); // sa_vert_adj_edges_csr (init_mesh_data.cpp), once at set-up
 	void ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef, uint32_t keep = 0);
 };

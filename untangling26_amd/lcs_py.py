@@ -30,7 +30,7 @@ import numpy as np
 import torch
 
 from . import initializer
-from ._lib import PcgParams, PrpParams, SolverDesc, SolverParams, StepReport, lib, ptr
+from ._lib import AbdDesc, PcgParams, PrpParams, SolverDesc, SolverParams, StepReport, lib, ptr
 from .contours import (CONTOUR_TYPES, CT_ADJ, CT_ADJ_OFF, CT_COMP_MIN, CT_CONTOUR_INDEX, CT_CONTOUR_TYPE,
                        CT_MESH_CLOSED, CT_MESH_INDEX, CT_PAIR_IDS, CT_PAIRS)
 from .runtime import Context
@@ -71,6 +71,11 @@ class Float3:
 
     def __repr__(self):
         return f"Float3({self.x}, {self.y}, {self.z})"
+
+
+def mat_friction_mu(mat):
+    """MaterialBase::default_friction_mu (physical_material.h:66) unless the material sets one."""
+    return float(getattr(mat, "friction_mu", 0.5))
 
 
 def _float3(v) -> Float3:
@@ -202,8 +207,18 @@ class WorldData:
     def _unsupported_material(self, kind):
         raise RuntimeError(f"{kind} materials are not part of this build (cloth only)")
 
-    def set_physics_material_rigid(self, *a, **k):
-        self._unsupported_material("rigid")
+    def set_physics_material_rigid(self, model="Orthogonality", thickness=1e-3, stiffness=1e6, density=1e3, mass=0.0,
+                                   d_hat=1e-3, contact_offset=0.0):
+        """A rigid body as one 12-DOF affine body (python_bindings.cpp:98-118, RigidMaterial
+        defaults of physical_material.h:108-120); the orthogonality model only."""
+        if model not in ("Orthogonality", "ABD"):
+            raise RuntimeError(f"rigid model {model!r} is not part of this build (Orthogonality only)")
+        if mass:
+            raise RuntimeError("a prescribed rigid body mass is not part of this build (density only)")
+        self.material_type = MaterialType.Rigid
+        self.material = initializer.RigidMaterial(stiffness=stiffness, density=density, d_hat=d_hat,
+                                                  contact_offset=contact_offset, thickness=thickness)
+        return self
 
     def set_physics_material_tet(self, *a, **k):
         self._unsupported_material("tetrahedral")
@@ -267,13 +282,17 @@ class WorldData:
         in radians and the rows those functions give (X and Y rotate by -angle in the
         textbook convention, Z by +angle)."""
         x = self.vertices.astype(np.float64) * np.array(tuple(self.scale))
+        x = x @ self.rotation_matrix().T + np.array(tuple(self.translation))
+        return x.astype(np.float32)
+
+    def rotation_matrix(self):
+        """rotate(r) of make_model_matrix (Core/affine_position.h:40-100)."""
         rx, ry, rz = tuple(self.rotation)
         cx, sx, cy, sy, cz, sz = np.cos(rx), np.sin(rx), np.cos(ry), np.sin(ry), np.cos(rz), np.sin(rz)
         X = np.array([[1, 0, 0], [0, cx, sx], [0, -sx, cx]])
         Y = np.array([[cy, 0, -sy], [0, 1, 0], [sy, 0, cy]])
         Z = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
-        x = x @ (X @ Y @ Z).T + np.array(tuple(self.translation))
-        return x.astype(np.float32)
+        return X @ Y @ Z
 
 
 class PyPRPDebugInfo:
@@ -401,6 +420,11 @@ class NewtonSolver:
         if not self._worlds:
             raise RuntimeError("no world data registered")
         self._check_supported()
+        rigid = [w.material_type == MaterialType.Rigid for w in self._worlds]
+        if any(rigid):
+            if not all(rigid):
+                raise RuntimeError("cloth and rigid bodies in one scene are not part of this build")
+            return self._init_rigid()
         mats = {tuple(vars(w.material).items()) for w in self._worlds}
         if len(mats) > 1:
             raise RuntimeError("one cloth material for all meshes in this build")
@@ -456,6 +480,83 @@ class NewtonSolver:
         self._handle = h
         self._desc = d
 
+    def _init_rigid(self):
+        """Every world one affine body: model positions xbar = scale * vertices, q0 = (translation,
+        the columns of the world rotation), the solid's mass matrix from its closed surface."""
+        from .scenes import RigidScene
+
+        mats = {tuple(vars(w.material).items()) for w in self._worlds}
+        if len(mats) > 1:
+            raise RuntimeError("one rigid material for all bodies in this build")
+        xs, fs, body, xbar, q0, off = [], [], [], [], [], [0]
+        for b, w in enumerate(self._worlds):
+            model = w.vertices.astype(np.float64) * np.array(tuple(w.scale))
+            R = w.rotation_matrix()
+            t = np.array(tuple(w.translation), np.float64)
+            xs.append(model @ R.T + t)
+            fs.append(w.triangles.astype(np.int64) + off[-1])
+            body.append(np.full(len(model), b, np.uint32))
+            xbar.append(model)
+            q0.append(np.stack([t, R[:, 0], R[:, 1], R[:, 2]]))
+            off.append(off[-1] + len(model))
+        mesh = Mesh(np.concatenate(xs), np.concatenate(fs).astype(np.uint32))
+        scene = RigidScene(mesh, np.concatenate(body), np.concatenate(xbar).astype(np.float32), np.asarray(q0),
+                           np.zeros(len(q0)), np.zeros(len(q0), np.uint8), np.asarray(off, np.uint32))
+        mat = initializer.RigidMaterial(**vars(self._worlds[0].material))
+        mat.d_hat = self._config.d_hat
+        st = initializer.setup_bodies(scene, mat)
+        self._setup = st
+        dv = self._ctx.device
+
+        def up(a, dtype=None):
+            t = torch.from_numpy(np.ascontiguousarray(a))
+            if t.dtype == torch.uint32:
+                t = t.view(torch.int32)
+            return t.to(dv) if dtype is None else t.to(dtype).to(dv)
+
+        names = dict(edges=st.edges, faces=st.faces, face_adj_edges=st.face_adj_edges,
+                     edge_adj_faces=st.edge_adj_faces, mass=st.mass, d_hat=st.d_hat, offset=st.offset,
+                     vprop=st.vprop, rest_vert_area=st.rest_vert_area, rest_edge_area=st.rest_edge_area,
+                     rest_face_area=st.rest_face_area, rest_x=st.x,
+                     friction_mu=np.full(st.nv, mat_friction_mu(mat), np.float32))
+        self._dev = {k: up(v) for k, v in names.items()}
+        self._dev.update(vert_off=up(st.vert_off), body=up(st.body), xbar=up(st.xbar),
+                         mass4=up(st.mass4.reshape(-1), torch.float64), volume=up(st.volume, torch.float64),
+                         kappa=up(st.kappa, torch.float64), q0=up(st.q0.reshape(-1), torch.float64))
+        a = AbdDesc()
+        a.nb, a.nv = st.nb, st.nv
+        for k in ("vert_off", "body", "xbar", "mass4", "volume", "kappa"):
+            setattr(a, k, ptr(self._dev[k]))
+        d = SolverDesc()
+        d.nv, d.ne, d.nf, d.nh = st.nv, len(st.edges), len(st.faces), 0
+        for k in names:
+            setattr(d, k, ptr(self._dev[k]))
+        d.abd = C.cast(C.pointer(a), C.c_void_p)
+        d.q0 = ptr(self._dev["q0"])
+        x0 = up(st.x)
+        self._x0 = x0.clone()
+        self._q0 = self._dev["q0"].clone()
+        self._material_x = np.zeros((st.nv, 2), np.float32)
+        h = C.c_void_p()
+        self._ctx.check(lib().ut_solver_create(self._ctx.handle, C.byref(d), C.byref(self._params()), ptr(x0),
+                                               None, C.byref(h)))
+        self._handle = h
+        self._desc, self._abd_desc = d, a
+        self._rigid = True
+
+    def q(self):
+        """Affine bodies: the device q [nb, 12] (p, a1, a2, a3 per body)."""
+        self._require()
+        out = torch.empty(12 * self._setup.nb, dtype=torch.float64, device=self._ctx.device)
+        self._ctx.check(lib().ut_solver_get_q(self._handle, ptr(out), None))
+        return out.view(-1, 12)
+
+    def set_q(self, q, vq=None):
+        self._require()
+        qq = q.detach().to(torch.float64).reshape(-1).contiguous()
+        vv = torch.zeros_like(qq) if vq is None else vq.detach().to(torch.float64).reshape(-1).contiguous()
+        self._ctx.check(lib().ut_solver_set_q(self._handle, ptr(qq), ptr(vv)))
+
     def __del__(self):
         try:
             if self._handle:
@@ -489,7 +590,10 @@ class NewtonSolver:
     def restart_system(self):
         """Positions back to the registered ones, velocities zero."""
         self._require()
-        self.set_state(self._x0, torch.zeros_like(self._x0))
+        if getattr(self, "_rigid", False):
+            self.set_q(self._q0)
+        else:
+            self.set_state(self._x0, torch.zeros_like(self._x0))
         self._last_report = None
         self._config.current_frame = 0
 
@@ -530,6 +634,8 @@ class NewtonSolver:
         return v
 
     def set_state(self, x: torch.Tensor, v: torch.Tensor | None = None):
+        if getattr(self, "_rigid", False):
+            raise RuntimeError("affine bodies: set the body state with set_q (positions follow q)")
         self._ctx.check(lib().ut_solver_set_state(self._handle, ptr(x.contiguous()),
                                                   None if v is None else ptr(v.contiguous())))
 
