@@ -35,10 +35,18 @@ def test_unsupported_features_are_named(field, value):
     assert c.unsupported()
 
 
-def test_unsupported_materials_raise():
+def test_materials():
     w = lcs_py.WorldData("a", np.zeros((3, 3)), [[0, 1, 2]])
+    # rigid bodies: the orthogonality model (RigidMaterial defaults, physical_material.h:108-120)
+    w.set_physics_material_rigid(thickness=1e-3, stiffness=1e4)
+    assert w.material_type == lcs_py.MaterialType.Rigid and w.material.stiffness == 1e4
+    assert lcs_py.WorldData("b", np.zeros((3, 3)), [[0, 1, 2]]).set_physics_material_rigid().material.stiffness == 1e6
     with pytest.raises(RuntimeError):
-        w.set_physics_material_rigid(thickness=1e-3, stiffness=1e4)
+        w.set_physics_material_rigid(model="Spring")
+    with pytest.raises(RuntimeError):
+        w.set_physics_material_tet()
+    with pytest.raises(RuntimeError):
+        w.set_physics_material_rod()
     w.set_physics_material_cloth(thickness=2e-3, d_hat=5e-4)
     assert w.material.thickness == 2e-3 and w.material.d_hat == 5e-4
 

@@ -135,7 +135,7 @@ def setup_cloth(mesh: Mesh, mat: ClothMaterial | None = None, x_rest: np.ndarray
 class RigidMaterial:
     """RigidMaterial fields used by the affine-body path: orthogonality stiffness
     (RigidMaterial::stiffness), density, contact d_hat and offset."""
-    stiffness: float = 1e5
+    stiffness: float = 1e6
     density: float = 1e3
     d_hat: float = 1e-3
     contact_offset: float = 0.0
