@@ -412,6 +412,10 @@ typedef struct ut_abd_assemble_args {
 	double*			energy;
 } ut_abd_assemble_args;
 UT_API int ut_abd_assemble(ut_ctx* ctx, const ut_abd_desc* desc, const ut_abd_assemble_args* args);
+/* gq = J^T gx (accumulate != 0: gq += J^T gx): a vertex-space gradient [3 nv] gathered onto
+ * the bodies [12 nb] (no reference counterpart: the differentiable potential's chain
+ * rule through x = p + A xbar). */
+UT_API int ut_abd_gather(ut_ctx* ctx, const ut_abd_desc* desc, const double* gx, double* gq, int accumulate);
 /* y = A p with the blocks and the contacts between bodies (matrix-free), FP64. */
 UT_API int ut_abd_apply(ut_ctx* ctx, const ut_abd_desc* desc, const double* blocks, const void* contacts,
 	const uint32_t* contact_count, uint32_t contact_cap, const double* p, double* y);

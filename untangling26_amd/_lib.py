@@ -63,6 +63,7 @@ SIGNATURES = {
     "ut_abd_q_to_x": (C.c_int, [vp, vp, vp, vp, vp]),
     "ut_abd_assemble": (C.c_int, [vp, vp, vp]),
     "ut_abd_apply": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint32, vp, vp]),
+    "ut_abd_gather": (C.c_int, [vp, vp, vp, vp, C.c_int]),
     "ut_abd_pcg_create": (C.c_int, [vp, C.c_uint32, C.POINTER(vp)]),
     "ut_abd_pcg_destroy": (C.c_int, [vp]),
     "ut_abd_pcg_solve": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint32, vp, vp, C.c_int, C.c_uint32, C.c_double, vp]),

@@ -48,7 +48,7 @@ from .narrow_phase import BARRIER, QUADRATIC, NarrowPhasesDetector
 from .pcg import ConjugateGradientSolver
 from .runtime import Context
 
-__all__ = ["IncrementalPotential", "IPFunction", "BARRIER", "QUADRATIC"]
+__all__ = ["IncrementalPotential", "IPFunction", "AbdPotential", "AbdFunction", "BARRIER", "QUADRATIC"]
 
 
 def _up(a, device):
@@ -288,3 +288,138 @@ class IPFunction(torch.autograd.Function):
         else:
             gx = torch.stack([p.gradient(xb) * g[b] for b, xb in enumerate(x)])
         return gx.to(x.dtype), None
+
+
+class AbdPotential:
+    """The incremental potential of affine bodies (initializer.BodySetup) as a function
+    of q [nb, 12]:
+
+        E(q) = sum_b (q_b - qt_b)^T (M4_b (x) I3) (q_b - qt_b) / (2 h^2)   (abd_inertia_energy.cpp:25-140)
+             + sum_b kappa_b V_b ||A_b^T A_b - I||_F^2                    (abd_ortho_energy.cpp:25-140)
+             + contact barrier and lagged friction at the vertex positions x(q) = p + A xbar
+
+    Forward and backward run ut_abd_assemble (inertia and orthogonality), the
+    proximity DCD at x(q) and ut_contact_potential (distances re-solved at x); the
+    contact gradient reaches q through J^T (ut_abd_gather).  hvp applies the q-space
+    Newton matrix (body blocks + the contacts' couplings, ut_abd_apply).  q may be
+    FP32 or FP64; with FP64 q, x(q) is evaluated in FP64."""
+
+    def __init__(self, ctx: Context, setup, q_tilde=None, inv_h2=3600.0, kappa=1e8, energy=BARRIER,
+                 quasi_static=False, friction_mu=None, q_lag=None, contact_cap=1 << 16):
+        from .abd import AffineBodies
+
+        self.ctx, self.st = ctx, setup
+        dv = ctx.device
+        self.bodies = AffineBodies(ctx, setup)
+        self.nb, self.nv = setup.nb, setup.nv
+        st = setup
+        names = dict(edges=st.edges, faces=st.faces, d_hat=st.d_hat, offset=st.offset, vprop=st.vprop,
+                     va=st.rest_vert_area, ea=st.rest_edge_area, fa=st.rest_face_area)
+        self.d = {k: _up(v, dv) for k, v in names.items()}
+        self.q_tilde = None if q_tilde is None else torch.as_tensor(
+            np.asarray(q_tilde, np.float64).reshape(-1), device=dv).contiguous()
+        self.inv_h2, self.kappa, self.energy_type = float(inv_h2), float(kappa), int(energy)
+        self.quasi_static = bool(quasi_static)
+        self.narrow = NarrowPhasesDetector(ctx, self.nv, self.d["edges"], self.d["faces"], self.d["d_hat"],
+                                           self.d["offset"], self.d["va"], self.d["ea"], self.d["fa"],
+                                           vprop=self.d["vprop"], kappa=kappa, energy=energy)
+        self.face_tree = LBVH(ctx, TREE_FACE, len(st.faces))
+        self.edge_tree = LBVH(ctx, TREE_EDGE, len(st.edges))
+        self.cl = _ContactList(dv, contact_cap)
+        self.fl = None
+        self.comb = _ContactList(dv, 2 * contact_cap)
+        self.gx = torch.zeros(3 * self.nv, dtype=torch.float64, device=dv)
+        self.pot = torch.zeros(2 + 512, dtype=torch.float64, device=dv)
+        self._x = None
+        self.x_lag32 = self.x_lag64 = None
+        if friction_mu is not None:
+            mu = np.broadcast_to(np.asarray(friction_mu, np.float32), (self.nv,))
+            self.mu = torch.as_tensor(np.ascontiguousarray(mu), device=dv)
+            if q_lag is None:
+                raise ValueError("friction needs the lagged state q_lag")
+            ql = torch.as_tensor(np.asarray(q_lag, np.float64).reshape(-1), device=dv)
+            self.x_lag64 = self.bodies.q_to_x(ql, fp64=True).reshape(-1).contiguous()
+            self.x_lag32 = self.x_lag64.to(torch.float32)
+            self.fl = _ContactList(dv, contact_cap)
+            self._detect(self.x_lag32, self.fl)
+            self.ctx.check(lib().ut_friction_init(self.ctx.handle, ptr(self.fl.pairs), ptr(self.fl.count),
+                                                  self.fl.cap, ptr(self.mu)))
+
+    _detect = IncrementalPotential._detect
+
+    def _positions(self, q):
+        """(x32, x64 or None) at q."""
+        if q.dtype == torch.float64:
+            x64 = self.bodies.q_to_x(q, fp64=True).reshape(-1).contiguous()
+            return x64.to(torch.float32), x64
+        return self.bodies.q_to_x(q).reshape(-1).contiguous(), None
+
+    def _prepare(self, q):
+        x32, x64 = self._positions(q)
+        if self._x is None or not torch.equal(self._x, x32):
+            self._detect(x32, self.cl)
+            self._x = x32.clone()
+        return x32, x64
+
+    _potential = IncrementalPotential._potential
+    _newton_contacts = IncrementalPotential._newton_contacts
+
+    def _bodies(self, q, contacts=None, blocks=False):
+        c = contacts
+        return self.bodies.assemble(q, self.q_tilde, self.inv_h2, None if c is None else c.pairs,
+                                    None if c is None else c.count, 0 if c is None else c.cap,
+                                    quasi_static=self.quasi_static, blocks=blocks)
+
+    def energies(self, q):
+        """(inertia, orthogonality, contact, friction) as FP64 tensors."""
+        x32, x64 = self._prepare(q)
+        _, _, e = self._bodies(q)
+        e = e.clone()
+        contact = self._potential(self.cl, x32, x64, TERM_BARRIER, None)[0]
+        friction = (self._potential(self.fl, x32, x64, TERM_FRICTION, None)[1] if self.fl is not None
+                    else torch.zeros((), dtype=torch.float64, device=self.ctx.device))
+        return torch.stack([e[0], e[1], contact, friction])
+
+    def energy(self, q):
+        return self.energies(q).sum()
+
+    def gradient(self, q):
+        """dE/dq, FP64 [nb, 12]."""
+        x32, x64 = self._prepare(q)
+        g, _, _ = self._bodies(q)
+        self.gx.zero_()
+        self._potential(self.cl, x32, x64, TERM_BARRIER, self.gx)
+        if self.fl is not None:
+            self._potential(self.fl, x32, x64, TERM_FRICTION, self.gx)
+        self.ctx.check(lib().ut_abd_gather(self.ctx.handle, C.byref(self.bodies.desc), ptr(self.gx), ptr(g), 1))
+        return g.view(-1, 12).clone()
+
+    def hvp(self, q, v):
+        """The q-space Newton matrix times v, FP64 [nb, 12]."""
+        x32, x64 = self._prepare(q)
+        comb = self._newton_contacts(x32, x64)
+        self._bodies(q, comb, blocks=True)
+        return self.bodies.apply(v, comb.pairs, comb.count, comb.cap).view(-1, 12)
+
+
+class AbdFunction(torch.autograd.Function):
+    """E = AbdFunction.apply(q, potential): q [nb, 12] or [B, nb, 12]; backward multiplies
+    the hand-written gradient dE/dq."""
+
+    @staticmethod
+    def forward(ctx, q, potential: AbdPotential):
+        ctx.potential = potential
+        ctx.save_for_backward(q)
+        if q.dim() == 2:
+            return potential.energy(q)
+        return torch.stack([potential.energy(qb) for qb in q])
+
+    @staticmethod
+    def backward(ctx, g):
+        (q,) = ctx.saved_tensors
+        p = ctx.potential
+        if q.dim() == 2:
+            gq = p.gradient(q) * g
+        else:
+            gq = torch.stack([p.gradient(qb) * g[b] for b, qb in enumerate(q)])
+        return gq.to(q.dtype), None

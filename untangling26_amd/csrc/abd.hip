@@ -500,6 +500,37 @@ __global__ void k_abd_axpy(uint32_t n, const float* __restrict__ d, double* __re
 		x[i] += (double)d[i];
 }
 
+// J^T g: per body, sum over its vertices of (1, xbar_v) (x) g_v (one workgroup per
+// body, vertices [vert_off[b], vert_off[b + 1]), fixed-order wave and block sums)
+__global__ __launch_bounds__(256) void k_abd_gather(const uint32_t* __restrict__ vert_off,
+	const float* __restrict__ xbar, const double* __restrict__ gx, double* __restrict__ gq, int accumulate)
+{
+	const uint32_t b = blockIdx.x;
+	const uint32_t v0 = vert_off[b], v1 = vert_off[b + 1];
+	double		   acc[12] = {};
+	for (uint32_t v = v0 + threadIdx.x; v < v1; v += blockDim.x)
+	{
+		const double w[4] = { 1.0, (double)xbar[3 * v], (double)xbar[3 * v + 1], (double)xbar[3 * v + 2] };
+		for (int k = 0; k < 4; ++k)
+			for (int r = 0; r < 3; ++r)
+				acc[3 * k + r] += w[k] * gx[3 * (size_t)v + r];
+	}
+	__shared__ double part[4][12];
+	for (int k = 0; k < 12; ++k)
+	{
+		const double t = wave_sum(acc[k]);
+		if (__lane_id() == 0)
+			part[threadIdx.x / kWave][k] = t;
+	}
+	__syncthreads();
+	if (threadIdx.x < 12)
+	{
+		const double t = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] + part[3][threadIdx.x];
+		double&		 o = gq[12 * (size_t)b + threadIdx.x];
+		o = accumulate ? o + t : t;
+	}
+}
+
 __global__ void k_zero_d(size_t n, double* p)
 {
 	const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -539,6 +570,13 @@ void abd_apply(Context* ctx, const AbdDesc& d, const double* blocks, const Conta
 	k_abd_apply_blocks<<<div_up(12 * d.nb, 256), 256, 0, s>>>(d.nb, blocks, p, y);
 	if (contacts && count && cap)
 		k_abd_apply_cross<<<512, 256, 0, s>>>(contacts, count, cap, d.body, d.xbar, p, y);
+	UT_LAUNCH_CHECK();
+}
+
+void abd_gather(Context* ctx, const AbdDesc& d, const double* gx, double* gq, bool accumulate)
+{
+	UT_REQUIRE(d.vert_off && d.xbar && gx && gq, "abd gather: null input");
+	k_abd_gather<<<d.nb, 256, 0, ctx->stream>>>(d.vert_off, d.xbar, gx, gq, accumulate ? 1 : 0);
 	UT_LAUNCH_CHECK();
 }
 

@@ -49,6 +49,8 @@ struct AbdPcgReport
 void abd_q_to_x(Context* ctx, const AbdDesc& d, const double* q, float* x32, double* x64);
 // gradient (and blocks) of inertia + orthogonality + contacts in q-space; grad / blocks are overwritten
 void abd_assemble(Context* ctx, const AbdDesc& d, const AbdAssembleArgs& a);
+// gq (+)= J^T gx: a vertex-space gradient gathered onto the bodies
+void abd_gather(Context* ctx, const AbdDesc& d, const double* gx, double* gq, bool accumulate);
 // y = A p: the blocks plus the contacts between bodies (matrix-free)
 void abd_apply(Context* ctx, const AbdDesc& d, const double* blocks, const ContactRec* contacts, const uint32_t* count,
 	uint32_t cap, const double* p, double* y);

@@ -764,6 +764,13 @@ UT_API int ut_abd_assemble(ut_ctx* ctx, const ut_abd_desc* desc, const ut_abd_as
 		});
 }
 
+UT_API int ut_abd_gather(ut_ctx* ctx, const ut_abd_desc* desc, const double* gx, double* gq, int accumulate)
+{
+	if (!ctx || !desc || !gx || !gq || !desc->vert_off)
+		return ut::kInvalid;
+	return guarded(&ctx->c, [&] { ut::abd_gather(&ctx->c, abd_desc(desc), gx, gq, accumulate != 0); });
+}
+
 UT_API int ut_abd_apply(ut_ctx* ctx, const ut_abd_desc* desc, const double* blocks, const void* contacts,
 	const uint32_t* contact_count, uint32_t contact_cap, const double* p, double* y)
 {
