@@ -213,7 +213,7 @@ def summarize(reps):
     out["prp_rays"] = {k: float(np.mean([r[f"prp_rays_{k}"] for r in reps])) for k in ("vf", "ee", "fv")}
     out["prp_hits"] = float(np.mean([r["prp_hits"] for r in reps]))
     out["ee"] = {k: float(np.sum([r[f"prp_ee_{k}"] for r in reps]))
-                 for k in ("launches", "requests", "visits", "leaf_tests", "hits", "cand_tests")}
+                 for k in ("launches", "requests", "visits", "leaf_tests", "hits", "cand_tests", "wave_steps")}
     out["ee"]["ms"] = float(np.sum([r["ms_prp_ee"] for r in reps]))
     return out
 
@@ -357,6 +357,10 @@ def main():
                               "hits_per_launch": mixed["ee"]["hits"] / max(mixed["ee"]["launches"], 1),
                               "candidate_leaf_tests_per_launch":
                                   mixed["ee"]["cand_tests"] / max(mixed["ee"]["launches"], 1),
+                              # node visits per lane-step of the traversal loop: 1.0 when every lane of
+                              # every wave visits a node at every step (no divergence, no idle lanes)
+                              "lane_utilisation": round(mixed["ee"]["visits"]
+                                                        / max(64.0 * mixed["ee"]["wave_steps"], 1.0), 4),
                               "bytes_per_unit": EE_BYTES},
             "roofline_pcg_spmv": {"kernel": "k_pcg_spmv<float>", "bound": "hbm", "achieved": round(achieved, 2),
                                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

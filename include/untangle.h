@@ -505,6 +505,8 @@ typedef struct ut_step_report {
 	float	 ms_prp_ee;
 	uint64_t prp_ee_requests, prp_ee_visits, prp_ee_leaf_tests, prp_ee_hits;
 	uint64_t prp_ee_cand_tests; /* leaves whose target passed the candidate check (edge-edge tests run) */
+	uint64_t prp_ee_wave_steps; /* traversal loop steps summed over waves: visits / (64 x steps) is the
+								   fraction of lanes that visit a node per step */
 } ut_step_report;
 
 UT_API int ut_solver_create(ut_ctx* ctx, const ut_solver_desc* desc, const ut_solver_params* params, const float* x0,

@@ -39,7 +39,7 @@ def _rep(ms, ef, conv):
              prp_responses=10, prp_levels=3, prp_slots=5, prp_hits=7, ms_prp_ee=1.0)
     for k in ("vf", "ee", "fv"):
         r[f"prp_rays_{k}"] = 11
-    for k in ("launches", "requests", "visits", "leaf_tests", "hits", "cand_tests"):
+    for k in ("launches", "requests", "visits", "leaf_tests", "hits", "cand_tests", "wave_steps"):
         r[f"prp_ee_{k}"] = 2
     return r
 

@@ -175,7 +175,7 @@ class StepReport(C.Structure):
                 ("prp_rays_vf", C.c_uint64), ("prp_rays_ee", C.c_uint64), ("prp_rays_fv", C.c_uint64),
                 ("prp_ee_launches", C.c_uint32), ("ms_prp_ee", C.c_float), ("prp_ee_requests", C.c_uint64),
                 ("prp_ee_visits", C.c_uint64), ("prp_ee_leaf_tests", C.c_uint64), ("prp_ee_hits", C.c_uint64),
-                ("prp_ee_cand_tests", C.c_uint64)]
+                ("prp_ee_cand_tests", C.c_uint64), ("prp_ee_wave_steps", C.c_uint64)]
 
 
 class PrpChoice(C.Structure):

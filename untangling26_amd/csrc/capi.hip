@@ -1039,7 +1039,7 @@ UT_API int ut_solver_step(ut_solver* s, ut_step_report* report)
 		report->prp_ee_launches = r.prp_ee.launches, report->ms_prp_ee = r.prp_ee.ms;
 		report->prp_ee_requests = r.prp_ee.requests, report->prp_ee_visits = r.prp_ee.visits;
 		report->prp_ee_leaf_tests = r.prp_ee.leaf_tests, report->prp_ee_hits = r.prp_ee.hits;
-		report->prp_ee_cand_tests = r.prp_ee.cand_tests;
+		report->prp_ee_cand_tests = r.prp_ee.cand_tests, report->prp_ee_wave_steps = r.prp_ee.wave_steps;
 	}
 	return ut::kOk;
 }

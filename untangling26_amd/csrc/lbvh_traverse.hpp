@@ -58,6 +58,7 @@ __device__ __forceinline__ box3 query_box(int kind,
 struct TravCount
 {
 	unsigned long long visits = 0, tests = 0;
+	unsigned long long steps = 0; // loop steps of the wave (node visits / (64 steps) = lane utilisation)
 };
 
 // Runs the traversal for query q (valid when active).  narrow(valid, q, leaf)
@@ -100,8 +101,10 @@ __device__ __forceinline__ void traverse_staged_test(TravSharedT<LdsDepth>& sh,
 	uint32_t node = 0;
 	bool	 go = active;
 	uint32_t visits = 0;
+	uint32_t steps = 0; // wave-uniform
 	while (__ballot(go) != 0ull)
 	{
+		++steps;
 		bool	 e0 = false, e1 = false;
 		uint32_t o0 = 0, o1 = 0;
 		if (go && ++visits > max_visits)
@@ -179,6 +182,7 @@ __device__ __forceinline__ void traverse_staged_test(TravSharedT<LdsDepth>& sh,
 			v += __shfl_xor(v, o, kWave);
 		count->visits += v;
 		count->tests += tests;
+		count->steps += steps;
 	}
 }
 

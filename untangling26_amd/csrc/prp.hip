@@ -454,7 +454,7 @@ struct RayArgs
 	const uint64_t* ecand; // candidate edge bitmaps per (contour, role), ewords each
 	const uint64_t* fcand; // candidate face bitmaps per (contour, role), fwords each
 	uint32_t		ewords, fwords;
-	unsigned long long* tstat; // [3 kinds][4]: node visits, leaf tests, hits, candidate leaf tests
+	unsigned long long* tstat; // [3 kinds][4]: node visits, leaf tests, hits, candidate leaf tests; [12 + kind]: wave steps
 };
 
 __device__ __forceinline__ uint32_t slot_of(const uint32_t* off, uint32_t nslots, uint32_t r)
@@ -600,6 +600,7 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 	flush_stat(&a.tstat[4 * Kind], tc.visits);
 	flush_stat(&a.tstat[4 * Kind + 1], tc.tests);
 	flush_stat(&a.tstat[4 * Kind + 2], hit_acc);
+	flush_stat(&a.tstat[12 + Kind], tc.steps);
 }
 
 // EE (kind 1): source edge swept along dir by ray_max (h:597-613) against candidate edges of role 1
@@ -613,6 +614,8 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 // plane) that the computed crossing lies more than the margin off the swept
 // strip.  The oracle applies the same test.
 constexpr float kEeBeamMargin = 1e-4f;
+// rounding margin of the EE sweep's node culls (FP32 dot products of ~1 m coordinates)
+constexpr float kEeNodeMargin = 1e-4f;
 __device__ __forceinline__ bool ee_in_beam(f3 pn, f3 dir, f3 p0, f3 p1, f3 q0, f3 q1)
 {
 	f3			wn = cross3(pn, dir);
@@ -635,11 +638,12 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 	unsigned long long hit_acc = 0, cand_acc = 0; // lane 0: hits and candidate leaf tests of the wave
 	for (uint32_t base = xcd_block() * blockDim.x; base < total; base += gridDim.x * blockDim.x)
 	{
-		const uint32_t	   r = base + threadIdx.x;
-		bool			   active = r < total;
-		box3			   sw = box_empty(), cb = box_empty();
-		f3			   pn = mk3(0.0f, 0.0f, 0.0f), wn = mk3(0.0f, 0.0f, 0.0f);
-		float		   pd = 0.0f, wlo = -1e30f, whi = 1e30f;
+		const uint32_t r = base + threadIdx.x;
+		bool		   active = r < total;
+		// per-request node-cull constants (see the test below)
+		box3  ib = box_empty();
+		f3	  pn = mk3(0.0f, 0.0f, 0.0f), wn = mk3(0.0f, 0.0f, 0.0f);
+		float slab_lo = -1e30f, slab_hi = 1e30f, beam_lo = -1e30f, beam_hi = 1e30f;
 		if (active)
 		{
 			const uint32_t slot = slot_of(off, a.nslots, r);
@@ -656,17 +660,24 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 			}
 			if (rmax <= 0.0f)
 				active = false;
-			sw = box_add(box_add(box_add(box_of(p0), p1), p0 + dir * rmax), p1 + dir * rmax);
+			box3 sw = box_add(box_add(box_add(box_of(p0), p1), p0 + dir * rmax), p1 + dir * rmax);
 			sw = box_inflate(sw, 1e-5f);
 			const float* tb = a.tbox + 12 * (size_t)slot;
-			cb = box3{ mk3(tb[0], tb[1], tb[2]), mk3(tb[3], tb[4], tb[5]) };
+			// every candidate target edge lies in the candidate box, so a kept edge meets the swept
+			// box inside it: one test against the intersection replaces the two box tests
+			ib.lo = mk3(fmaxf(sw.lo.x, tb[0]), fmaxf(sw.lo.y, tb[1]), fmaxf(sw.lo.z, tb[2]));
+			ib.hi = mk3(fminf(sw.hi.x, tb[3]), fminf(sw.hi.y, tb[4]), fminf(sw.hi.z, tb[5]));
+			if (ib.lo.x > ib.hi.x || ib.lo.y > ib.hi.y || ib.lo.z > ib.hi.z)
+				active = false; // no candidate edge meets the sweep
 			// the leaf's plane prune (same normal): a kept target edge has a point within 1e-6 of the plane
 			pn = cross3(p1 - p0, dir);
 			const float pl = sqrtf(dot3(pn, pn));
 			if (pl >= 1e-10f)
 			{
 				pn = pn * (1.0f / pl);
-				pd = dot3(pn, p0);
+				const float pd = dot3(pn, p0);
+				slab_lo = pd - (1e-6f + kEeNodeMargin);
+				slab_hi = pd + (1e-6f + kEeNodeMargin);
 				// in-plane axis across the sweep: w . dir = 0, so the swept parallelogram
 				// p0 + t e1 + s dir covers only [min, max] of w . p0, w . p1 along it
 				wn = cross3(pn, dir);
@@ -675,8 +686,8 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 				{
 					wn = wn * (1.0f / wl);
 					const float w0 = dot3(wn, p0), w1 = dot3(wn, p1);
-					wlo = fminf(w0, w1);
-					whi = fmaxf(w0, w1);
+					beam_lo = fminf(w0, w1) - (kEeBeamMargin + kEeNodeMargin);
+					beam_hi = fmaxf(w0, w1) + (kEeBeamMargin + kEeNodeMargin);
 				}
 				else
 					wn = mk3(0.0f, 0.0f, 0.0f);
@@ -684,30 +695,30 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 			else
 				pn = mk3(0.0f, 0.0f, 0.0f); // no plane prune at the leaf either
 		}
-		// Node culling.  A node can hold a kept target edge only if its box meets (1) the swept box,
-		// (2) the box of the slot's candidate target edges (every kept edge is a candidate and lies in
-		// it) and (3) the slab |n.x - pd| <= 1e-6 that the leaf's plane prune requires of one point of
-		// the edge.  (4) The leaf keeps only an edge whose endpoints' w . q range meets
-		// [min, max] of w . p0, w . p1 widened by kEeBeamMargin, w the in-plane axis across the
-		// sweep (ee_in_beam; a crossing at X = p0 + t e1 + s dir with t in [0, 1] has w . X in
-		// that range).  (2)-(4) hold for the leaf box of any kept edge and so for its ancestors;
-		// kEeNodeMargin covers the FP32 rounding of the node and leaf dot products (~1e-6 here).
-		constexpr float kEeNodeMargin = 1e-4f;
-		auto			test = [&](const box3& b)
+		// Node culling.  A node can hold a kept target edge only if its box meets (1) the swept box
+		// and (2) the box of the slot's candidate target edges -- tested together as their
+		// intersection ib, since every kept edge is a candidate and lies in (2) -- and (3) the slab
+		// |n.x - pd| <= 1e-6 that the leaf's plane prune requires of one point of the edge.
+		// (4) The leaf keeps only an edge whose endpoints' w . q range meets [min, max] of
+		// w . p0, w . p1 widened by kEeBeamMargin, w the in-plane axis across the sweep
+		// (ee_in_beam; a crossing at X = p0 + t e1 + s dir with t in [0, 1] has w . X in that range).
+		// (1)-(4) hold for the leaf box of any kept edge and so for its ancestors.  (3) and (4) test
+		// the range of n . x (w . x) over the box, whose extremes lie at the corners picked by the
+		// signs of n (w); kEeNodeMargin covers the FP32 rounding of the node and leaf dot products
+		// (~1e-6 here), so the culls stay conservative.
+		auto test = [&](const box3& b)
 		{
-			const bool in_sw = (sw.lo.x <= b.hi.x) & (sw.hi.x >= b.lo.x) & (sw.lo.y <= b.hi.y)
-				& (sw.hi.y >= b.lo.y) & (sw.lo.z <= b.hi.z) & (sw.hi.z >= b.lo.z);
-			const bool in_cb = (cb.lo.x <= b.hi.x) & (cb.hi.x >= b.lo.x) & (cb.lo.y <= b.hi.y)
-				& (cb.hi.y >= b.lo.y) & (cb.lo.z <= b.hi.z) & (cb.hi.z >= b.lo.z);
-			const f3	c = (b.lo + b.hi) * 0.5f, h = (b.hi - b.lo) * 0.5f;
-			const float dc = dot3(pn, c) - pd;
-			const float rr = fabsf(pn.x) * h.x + fabsf(pn.y) * h.y + fabsf(pn.z) * h.z;
-			const bool	in_slab = (dc - rr <= 1e-6f + kEeNodeMargin) & (dc + rr >= -1e-6f - kEeNodeMargin);
-			const float wc = dot3(wn, c);
-			const float wr = fabsf(wn.x) * h.x + fabsf(wn.y) * h.y + fabsf(wn.z) * h.z;
-			const bool	in_beam =
-				(wc + wr >= wlo - kEeBeamMargin - kEeNodeMargin) & (wc - wr <= whi + kEeBeamMargin + kEeNodeMargin);
-			return in_sw & in_cb & in_slab & in_beam;
+			const bool	in_box = (ib.lo.x <= b.hi.x) & (ib.hi.x >= b.lo.x) & (ib.lo.y <= b.hi.y)
+				& (ib.hi.y >= b.lo.y) & (ib.lo.z <= b.hi.z) & (ib.hi.z >= b.lo.z);
+			const float pmax = fmaf(pn.x, pn.x >= 0.0f ? b.hi.x : b.lo.x,
+				fmaf(pn.y, pn.y >= 0.0f ? b.hi.y : b.lo.y, pn.z * (pn.z >= 0.0f ? b.hi.z : b.lo.z)));
+			const float pmin = fmaf(pn.x, pn.x >= 0.0f ? b.lo.x : b.hi.x,
+				fmaf(pn.y, pn.y >= 0.0f ? b.lo.y : b.hi.y, pn.z * (pn.z >= 0.0f ? b.lo.z : b.hi.z)));
+			const float wmax = fmaf(wn.x, wn.x >= 0.0f ? b.hi.x : b.lo.x,
+				fmaf(wn.y, wn.y >= 0.0f ? b.hi.y : b.lo.y, wn.z * (wn.z >= 0.0f ? b.hi.z : b.lo.z)));
+			const float wmin = fmaf(wn.x, wn.x >= 0.0f ? b.lo.x : b.hi.x,
+				fmaf(wn.y, wn.y >= 0.0f ? b.lo.y : b.hi.y, wn.z * (wn.z >= 0.0f ? b.lo.z : b.hi.z)));
+			return in_box & (pmax >= slab_lo) & (pmin <= slab_hi) & (wmax >= beam_lo) & (wmin <= beam_hi);
 		};
 		auto narrow = [&](bool valid, uint32_t rr, uint32_t g)
 		{
@@ -785,6 +796,7 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 	flush_stat(&a.tstat[4 + 1], tc.tests);
 	flush_stat(&a.tstat[4 + 2], hit_acc);
 	flush_stat(&a.tstat[4 + 3], cand_acc);
+	flush_stat(&a.tstat[12 + 1], tc.steps);
 }
 
 // ------------------------------------------------------------------ hit graph
@@ -1960,7 +1972,7 @@ void PrpResolver::init(Context* c, uint32_t nv_, uint32_t ne_, uint32_t nf_, con
 	counters.reserve(8);
 	status.reserve(2);
 	spill.reserve((size_t)kRayBlocks * kTravThreads * (kStackTotal - kRayStackLds));
-	tstat.reserve(12);
+	tstat.reserve(16);
 	if (!ee_ev[0])
 		for (auto& e : ee_ev)
 			UT_HIP_CHECK(hipEventCreate(&e));
@@ -2265,7 +2277,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 			faces, hits.get(), list_count.get(), hit_cap / kHitLists, status.get(), spill.get(), nullptr, slot_tbox.get(),
 			ecand.get(),
 			fcand.get(), ewords, fwords, tstat.get() };
-		k_prp_fill_u64<<<1, 64, 0, s>>>(12, reinterpret_cast<uint64_t*>(tstat.get()), 0ull);
+		k_prp_fill_u64<<<1, 64, 0, s>>>(16, reinterpret_cast<uint64_t*>(tstat.get()), 0ull);
 		k_prp_fill_u32<<<div_up(kHitLists * kListStride, 256), 256, 0, s>>>(kHitLists * kListStride, list_count.get(), 0u);
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get(), 1u);	  // [0] tree health
 		k_prp_fill_u32<<<1, 64, 0, s>>>(1, status.get() + 1, 0u); // [1] stack overflows
@@ -2278,7 +2290,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		UT_HIP_CHECK(hipEventRecord(ee_ev[1], s));
 		UT_LAUNCH_CHECK();
 		uint32_t		   st[2], tot[3];
-		unsigned long long ts[12];
+		unsigned long long ts[16];
 		UT_HIP_CHECK(hipMemcpyAsync(ts, tstat.get(), sizeof(ts), hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(lc.data(), list_count.get(), 4 * lc.size(), hipMemcpyDeviceToHost, s));
 		UT_HIP_CHECK(hipMemcpyAsync(st, status.get(), 8, hipMemcpyDeviceToHost, s));
@@ -2298,6 +2310,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 			ee_prof.leaf_tests += ts[5]; // leaves reached by the sweep (each staged and tested)
 			ee_prof.hits += ts[6];
 			ee_prof.cand_tests += ts[7]; // of which candidates of role 1: the edge-edge test runs
+			ee_prof.wave_steps += ts[13];
 		}
 		UT_REQUIRE(st[0] == 1u, "prp: ray traversal revisited a node (corrupt tree)");
 		UT_REQUIRE(st[1] == 0u, "prp: ray traversal stack overflow");

@@ -86,6 +86,7 @@ struct PrpKernelProfile
 	uint64_t requests = 0, visits = 0, hits = 0;
 	uint64_t leaf_tests = 0; // leaves the traversal reached (each checked against the candidate set)
 	uint64_t cand_tests = 0; // of which candidates: the edge-edge test runs on these
+	uint64_t wave_steps = 0; // traversal loop steps summed over waves (64 lanes each)
 };
 
 struct PrpStats
@@ -138,7 +139,7 @@ struct PrpResolver
 	uint64_t		 ray_count[3] = { 0, 0, 0 }; // ray requests cast since the last resolve began
 	PrpKernelProfile ee_prof;
 	hipEvent_t		 ee_ev[2] = { nullptr, nullptr };
-	DevBuf<unsigned long long> tstat; // [3 kinds][4] traversal counters of one evaluation
+	DevBuf<unsigned long long> tstat; // [3 kinds][4] traversal counters of one evaluation, [12 + kind] wave steps
 	DevBuf<PrpSlot>	 slots;
 	DevBuf<float>	 slot_tbox;				 // [6 * slots] box of the candidate target edges (EE culling)
 	// raw hits of an evaluation: kHitLists sub-lists of hit_cap / kHitLists entries,

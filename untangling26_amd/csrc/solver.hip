@@ -429,7 +429,7 @@ StepReport NewtonSolver::physics_step_GPU()
 					rep.prp_ee.launches += ps.ee.launches, rep.prp_ee.ms += ps.ee.ms;
 					rep.prp_ee.requests += ps.ee.requests, rep.prp_ee.visits += ps.ee.visits;
 					rep.prp_ee.leaf_tests += ps.ee.leaf_tests, rep.prp_ee.hits += ps.ee.hits;
-					rep.prp_ee.cand_tests += ps.ee.cand_tests;
+					rep.prp_ee.cand_tests += ps.ee.cand_tests, rep.prp_ee.wave_steps += ps.ee.wave_steps;
 					t_prp += ps.ms_prepare + ps.ms_levels + ps.ms_final;
 				}
 				else
