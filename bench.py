@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="cloth resolution (vertices per side)")
     ap.add_argument("--dirs", type=int, default=64, help="candidate directions per contour")
     ap.add_argument("--pcg-iters", type=int, default=1000)
+    ap.add_argument("--inner-tol", type=float, default=1e-2,
+                    help="relative tolerance of each FP32 inner solve of the mixed-precision PCG")
     ap.add_argument("--max-repair", type=int, default=200, help="step cap of the repair episode (cli.py --max-iters)")
     ap.add_argument("--probe-every", type=int, default=3,
                     help="re-solve every n-th repair step's system at both precisions (0: never)")
@@ -72,6 +74,7 @@ def make_solver(args, precision):
     cfg.ignore_near_zero_dist_pairs = False
     cfg.pcg_iter_count = args.pcg_iters
     cfg.pcg_rel_tol = 1e-6
+    cfg.pcg_inner_tol = args.inner_tol
     cfg.pcg_precision = precision
     cfg.prp_directions = args.dirs
     s.init_solver()

@@ -194,6 +194,11 @@ UT_API int ut_graph_components_static(ut_ctx* ctx, uint32_t n, const uint32_t* r
 
 enum { UT_PCG_FP64 = 0, UT_PCG_MIXED = 1, UT_PCG_FP32 = 2 };
 enum { UT_PREC_BLOCK_JACOBI = 0, UT_PREC_POINT_JACOBI = 1 };
+/* Krylov recurrence: UT_PCG_CLASSIC is precond_cg.cpp:706-914's loop (two
+ * reductions per iteration); UT_PCG_SINGLE_REDUCTION is the Chronopoulos-Gear
+ * form of the same method (one reduction point per iteration, two launches
+ * instead of three; same iterates in exact arithmetic). */
+enum { UT_PCG_CLASSIC = 0, UT_PCG_SINGLE_REDUCTION = 1 };
 
 typedef struct ut_pcg_params {
 	int		 precision;		 /* UT_PCG_* */
@@ -204,6 +209,7 @@ typedef struct ut_pcg_params {
 	uint32_t max_refine;	 /* mixed: FP64 refinement steps */
 	double	 regularization; /* lambda: A + lambda * mean(diag block) I (precond_cg.cpp:286-339) */
 	uint32_t check_every;	 /* iterations between host convergence polls */
+	int		 recurrence;	 /* UT_PCG_CLASSIC or UT_PCG_SINGLE_REDUCTION */
 } ut_pcg_params;
 
 /* PcgSolveReport (precond_cg.h:18-28). failure: 0 none, 1 non-finite rhs,

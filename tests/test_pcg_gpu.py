@@ -3,7 +3,9 @@
 Tolerances: SpMV in FP64 within 1e-12 relative of the oracle's block product;
 every PCG mode must reach the requested true relative residual (recomputed
 in FP64); the FP64 path's solution is within 1e-9 relative of the oracle's
-(same algorithm, different summation order); the mixed path's solution is
+(same algorithm, different summation order), with either Krylov recurrence
+(the reference's classic loop or the single-reduction form, whose iterates are
+the classic ones in exact arithmetic); the mixed path's solution is
 within 1e-5 relative of the FP64 solution at rel_tol 1e-8 (FP32-rounded
 matrix: error bounded by cond * 2^-24 * ||A|| perturbation, measured here).
 """
@@ -14,7 +16,10 @@ import torch
 import oracle
 from pcg_cases import spring_system
 from untangling26_amd import bsr, scenes, topology
-from untangling26_amd.pcg import BLOCK_JACOBI, FP32, FP64, MIXED, POINT_JACOBI, ConjugateGradientSolver
+from untangling26_amd.pcg import (BLOCK_JACOBI, CLASSIC, FP32, FP64, MIXED, POINT_JACOBI, SINGLE_REDUCTION,
+                                  ConjugateGradientSolver)
+
+RECURRENCES = [CLASSIC, SINGLE_REDUCTION]
 
 pytestmark = pytest.mark.gpu
 
@@ -52,9 +57,10 @@ def test_spmv_matches_oracle(ctx, name):
     assert np.linalg.norm(y32.cpu().numpy() - want) <= 1e-6 * np.linalg.norm(want)
 
 
+@pytest.mark.parametrize("recurrence", RECURRENCES)
 @pytest.mark.parametrize("precond", [BLOCK_JACOBI, POINT_JACOBI])
 @pytest.mark.parametrize("name", ["small", "two_sheets"])
-def test_pcg_fp64_matches_oracle(ctx, name, precond):
+def test_pcg_fp64_matches_oracle(ctx, name, precond, recurrence):
     m, pat, rows, cols, blocks, b = _system(name, k_stretch=10.0)
     rp, ci, bl = oracle.block_csr(m.nv, rows, cols, blocks)
     xo, ito, relo, convo, _ = oracle.pcg(m.nv, rp, ci, bl, b, precond=precond, max_iters=2000, rel_tol=1e-10)
@@ -62,25 +68,28 @@ def test_pcg_fp64_matches_oracle(ctx, name, precond):
     solver = ConjugateGradientSolver(ctx, pat)
     x = torch.zeros(3 * m.nv, dtype=torch.float64, device="cuda")
     rep = solver.device_solve(_vals(pat, rows, cols, blocks, np.float64), torch.from_numpy(b).cuda(), x,
-                              precision=FP64, preconditioner=precond, max_iters=2000, rel_tol=1e-10, check_every=4)
+                              precision=FP64, preconditioner=precond, max_iters=2000, rel_tol=1e-10, check_every=4,
+                              recurrence=recurrence)
     assert rep["converged"] and rep["relative_residual"] <= 1e-10 and rep["failure"] == 0
-    assert abs(rep["iterations"] - ito) <= 2
+    assert abs(rep["iterations"] - ito) <= 2, (rep["iterations"], ito)
     xg = x.cpu().numpy()
     assert np.linalg.norm(xg - xo) <= 1e-9 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("recurrence", RECURRENCES)
 @pytest.mark.parametrize("name", ["two_sheets", "folded_cloth"])
-def test_pcg_mixed_reaches_fp64_solution(ctx, name):
+def test_pcg_mixed_reaches_fp64_solution(ctx, name, recurrence):
     m, pat, rows, cols, blocks, b = _system(name)
     solver = ConjugateGradientSolver(ctx, pat)
     B = torch.from_numpy(b).cuda()
     x64 = torch.zeros_like(B)
     r64 = solver.device_solve(_vals(pat, rows, cols, blocks, np.float64), B, x64, precision=FP64, max_iters=5000,
-                              rel_tol=1e-8)
+                              rel_tol=1e-8, recurrence=recurrence)
     xm = torch.zeros_like(B)
     v32 = _vals(pat, rows, cols, blocks, np.float32)
     v64 = _vals(pat, rows, cols, blocks, np.float64)
-    rm = solver.device_solve(v32, B, xm, precision=MIXED, max_iters=5000, rel_tol=1e-8, inner_tol=1e-2, vals64=v64)
+    rm = solver.device_solve(v32, B, xm, precision=MIXED, max_iters=5000, rel_tol=1e-8, inner_tol=1e-2, vals64=v64,
+                             recurrence=recurrence)
     assert r64["converged"] and rm["converged"], (r64, rm)
     assert rm["relative_residual"] <= 1e-8 and rm["refinements"] >= 2
     # FP32 inner solves with FP64 residuals of the FP64 matrix: both solve the same system to
@@ -92,7 +101,8 @@ def test_pcg_mixed_reaches_fp64_solution(ctx, name):
     # without the FP64 matrix the refinement converges to the FP32-rounded system instead:
     # its residual measured against the FP64 matrix stays near the rounding level
     xf = torch.zeros_like(B)
-    rf = solver.device_solve(v32, B, xf, precision=MIXED, max_iters=5000, rel_tol=1e-8, inner_tol=1e-2)
+    rf = solver.device_solve(v32, B, xf, precision=MIXED, max_iters=5000, rel_tol=1e-8, inner_tol=1e-2,
+                             recurrence=recurrence)
     assert rf["converged"]
     y = torch.zeros_like(B)
     solver.spmv(v64, xf, y)
@@ -102,9 +112,16 @@ def test_pcg_mixed_reaches_fp64_solution(ctx, name):
     assert res64 <= 2e-8 and res64 < res32
 
 
-def test_pcg_fp32_mode_and_edge_cases(ctx):
+@pytest.mark.parametrize("recurrence", RECURRENCES)
+def test_pcg_fp32_mode_and_edge_cases(ctx, recurrence):
     m, pat, rows, cols, blocks, b = _system("two_sheets", k_stretch=10.0)
     solver = ConjugateGradientSolver(ctx, pat)
+    solve = solver.device_solve
+
+    def dsolve(*a, **k):
+        return solve(*a, recurrence=recurrence, **k)
+
+    solver.device_solve = dsolve
     v32 = _vals(pat, rows, cols, blocks, np.float32)
     B = torch.from_numpy(b).cuda()
     x = torch.zeros_like(B)
@@ -151,14 +168,15 @@ def test_pcg_regularization(ctx):
     assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
 
 
+@pytest.mark.parametrize("recurrence", RECURRENCES)
 @pytest.mark.parametrize("energy", [0, 1])
-def test_pcg_with_contacts_matches_oracle(ctx, energy):
+def test_pcg_with_contacts_matches_oracle(ctx, energy, recurrence):
     """Newton system of two parallel sheets with 3.5k contacts inside d_hat: the
     assembled blocks plus the matrix-free contact off-diagonal blocks.  The
     iteration applies the contact blocks through per-vertex contact lists (no
     atomics).  The FP64 iterate after a fixed 60 iterations equals the oracle's
     within 1e-9 (longer runs drift apart by summation order on this stiffer
-    system: both converge, in iteration counts a few apart); the converged
+    system: both converge, in iteration counts up to ~6% apart); the converged
     FP64 and mixed solutions satisfy the system, checked with the atomic form
     of the contact product."""
     from autograd_cases import ip_case
@@ -174,20 +192,26 @@ def test_pcg_with_contacts_matches_oracle(ctx, energy):
     V64 = torch.from_numpy(np.ascontiguousarray(v)).cuda().reshape(-1)
     xo, ito, _, _, _ = oracle.pcg_sell(st.pattern, v, -g, recs, max_iters=60, rel_tol=1e-300)
     x = torch.zeros_like(B)
-    rep = solver.device_solve(V64, B, x, precision=FP64, max_iters=60, rel_tol=1e-300, check_every=4)
+    rep = solver.device_solve(V64, B, x, precision=FP64, max_iters=60, rel_tol=1e-300, check_every=4,
+                              recurrence=recurrence)
     assert rep["iterations"] == ito == 60
-    assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-9 * np.linalg.norm(xo)
+    err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+    assert err <= 1e-9, err
     _, ito, _, convo, _ = oracle.pcg_sell(st.pattern, v, -g, recs, max_iters=3000, rel_tol=1e-10)
     assert convo
     x.zero_()
-    rep = solver.device_solve(V64, B, x, precision=FP64, max_iters=3000, rel_tol=1e-10, check_every=4)
+    rep = solver.device_solve(V64, B, x, precision=FP64, max_iters=3000, rel_tol=1e-10, check_every=4,
+                              recurrence=recurrence)
     assert rep["converged"] and rep["relative_residual"] <= 1e-10, rep
-    assert abs(rep["iterations"] - ito) <= 0.05 * ito, (rep["iterations"], ito)
+    # finite-precision CG on this stiff system: counts move with the rounding of the recurrence
+    # (classic 857 vs the oracle's 857 +- a few; single reduction 808 measured)
+    assert abs(rep["iterations"] - ito) <= 0.1 * ito, (rep["iterations"], ito)
     y = torch.zeros_like(B)
     solver.spmv(V64, x, y)
     assert (B - y).norm().item() <= 2e-10 * B.norm().item()
     xm = torch.zeros_like(B)
-    rm = solver.device_solve(V64.float(), B, xm, precision=MIXED, max_iters=3000, rel_tol=1e-8, vals64=V64)
+    rm = solver.device_solve(V64.float(), B, xm, precision=MIXED, max_iters=3000, rel_tol=1e-8, vals64=V64,
+                             recurrence=recurrence)
     assert rm["converged"] and rm["relative_residual"] <= 1e-8, rm
     solver.spmv(V64, xm, y)
     assert (B - y).norm().item() <= 2e-8 * B.norm().item()

@@ -97,7 +97,7 @@ SIGNATURES = {
 class PcgParams(C.Structure):
     _fields_ = [("precision", C.c_int), ("preconditioner", C.c_int), ("max_iters", C.c_uint32),
                 ("rel_tol", C.c_double), ("inner_tol", C.c_double), ("max_refine", C.c_uint32),
-                ("regularization", C.c_double), ("check_every", C.c_uint32)]
+                ("regularization", C.c_double), ("check_every", C.c_uint32), ("recurrence", C.c_int)]
 
 
 class PcgReport(C.Structure):

@@ -607,6 +607,9 @@ static int pcg_solve_impl(ut_pcg* pcg, const ut_pcg_params* params, const void* 
 			 p.max_refine = params->max_refine;
 			 p.regularization = params->regularization;
 			 p.check_every = params->check_every ? params->check_every : 8;
+			 UT_REQUIRE(params->recurrence == ut::kPcgClassic || params->recurrence == ut::kPcgSingleReduction,
+				 "pcg: unknown recurrence");
+			 p.recurrence = params->recurrence;
 			 rep = pcg->s.solve(p, vals, b, x, vals64);
 		 });
 	if (rc != ut::kOk)
@@ -899,6 +902,8 @@ ut::PcgParams to_pcg(const ut_pcg_params& q)
 	p.max_refine = q.max_refine;
 	p.regularization = q.regularization;
 	p.check_every = q.check_every ? q.check_every : 8;
+	UT_REQUIRE(q.recurrence == ut::kPcgClassic || q.recurrence == ut::kPcgSingleReduction, "pcg: unknown recurrence");
+	p.recurrence = q.recurrence;
 	return p;
 }
 ut::SolverParams to_params(const ut_solver_params& q)

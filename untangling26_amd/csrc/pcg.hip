@@ -21,7 +21,8 @@ constexpr int kPcgThreads = 256;
 constexpr int kSlice = 64;
 
 // ---- reductions -------------------------------------------------------------
-__device__ __forceinline__ void block_partial(double v, double* out)
+// out[slot] = the workgroup's sum of v (slot = the workgroup index unless given)
+__device__ __forceinline__ void block_partial_at(double v, double* out, uint32_t slot)
 {
 	__shared__ double ws[kPcgThreads / kWave];
 	v = wave_sum(v);
@@ -33,10 +34,11 @@ __device__ __forceinline__ void block_partial(double v, double* out)
 		double s = 0.0;
 		for (int q = 0; q < kPcgThreads / kWave; ++q)
 			s += ws[q];
-		out[blockIdx.x] = s;
+		out[slot] = s;
 	}
 	__syncthreads(); // ws is reused by the next call
 }
+__device__ __forceinline__ void block_partial(double v, double* out) { block_partial_at(v, out, blockIdx.x); }
 
 // Reads the stop flag once per workgroup so that every thread of the group
 // takes the same branch (block 0 of the same launch may be writing it).
@@ -153,7 +155,7 @@ __device__ __forceinline__ V3<TA> row_spmv(uint32_t row, const uint32_t* __restr
 	// trips per batch rather than two per slot.  Slots past the width reload the
 	// last one and are skipped in the sums; the width is the same for the 64 rows
 	// of a slice (one wave), so that guard is uniform.
-	constexpr int  B = sizeof(TA) == 4 ? 8 : 4;
+	constexpr int  B = sizeof(TA) == 4 ? 16 : 8;
 	const uint32_t s = row / kSlice, lane = row % kSlice;
 	const uint32_t off = slice_off[s];
 	const uint32_t w = (slice_off[s + 1] - off) / kSlice;
@@ -409,6 +411,11 @@ struct PcgParts
 	double* rr;		// [nb]
 	double* bb;		// [nb]
 };
+// doubles of the partial-sum buffer: 7 arrays of nb, 8 norm slots, 2 x 2 recurrence
+// scalars (+ 4 spare), the contact partials (layout() below)
+constexpr size_t partials_size(uint32_t nb) { return 7 * (size_t)nb + 16 + kContactBlocks; }
+// contact workgroups of the single-reduction SpMV (one partial each, at most kContactBlocks)
+inline uint32_t cg_contact_groups(uint32_t nc) { return std::min<uint32_t>(div_up(nc, kPcgThreads), kContactBlocks); }
 
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_init(uint32_t n, const double* __restrict__ rhs,
@@ -456,6 +463,10 @@ __global__ void k_pcg_check0(uint32_t nb, PcgParts parts, uint32_t* flags)
 }
 
 // ---- iteration ------------------------------------------------------------------
+// Contact entries of a row loaded per round: a vertex on a contour can sit in dozens
+// of PRP response records, and each round is one dependent memory round trip.
+constexpr int kCtBatch = 16;
+
 template <typename T>
 __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint32_t* __restrict__ slice_off,
 	const uint32_t* __restrict__ col, const T* __restrict__ vals, uint32_t nslots, const T* __restrict__ p,
@@ -479,15 +490,15 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_spmv(uint32_t n, const uint
 			a.y += ls * pr.y;
 			a.z += ls * pr.z;
 		}
-		if (ct_off) // contact off-diagonal blocks, this vertex's entries in contact order, 4 loads in flight
-			for (uint32_t e = ct_off[row], end = ct_off[row + 1]; e < end; e += 4)
+		if (ct_off) // contact off-diagonal blocks, this vertex's entries in contact order, kCtBatch loads in flight
+			for (uint32_t e = ct_off[row], end = ct_off[row + 1]; e < end; e += kCtBatch)
 			{
-				V3<T> c[4];
+				V3<T> c[kCtBatch];
 #pragma unroll
-				for (int j = 0; j < 4; ++j)
+				for (int j = 0; j < kCtBatch; ++j)
 					c[j] = ld_contrib(contrib, min(e + j, end - 1));
 #pragma unroll
-				for (int j = 0; j < 4; ++j)
+				for (int j = 0; j < kCtBatch; ++j)
 					if (e + j < end)
 					{
 						a.x += c[j].x;
@@ -635,6 +646,262 @@ __global__ __launch_bounds__(kPcgThreads) void k_pcg_dir(uint32_t it, uint32_t n
 	}
 }
 
+// ---- single-reduction recurrence (Chronopoulos & Gear, 1989) -------------------
+// The same Krylov iterates as the classic recurrence above, with u = M^-1 r,
+// w = A u, p = u + beta p and s = A p = w + beta s kept as vectors:
+//   gamma = (r, u), delta = (w, u), beta = gamma / gamma_prev,
+//   alpha = gamma / (delta - beta gamma / alpha_prev).
+// Both scalars come from dot products taken at one point of the iteration, so
+// an iteration is two launches:
+//   k_cg_spmv  rows: w_sell = (A_sell + lambda) u and the partials of (u, w_sell);
+//              contact groups: each contact's off-diagonal products of u (into
+//              the per-vertex entry lists) and its part of u^T A_off u;
+//   k_cg_vec   rows: w = w_sell + the row's contact entries, then p, s, x, r, u
+//              and the partials of (r, u), (r, r).
+// The contacts only read u, which the previous launch wrote, so their products
+// run beside the rows' SpMV instead of in a launch of their own.  s, r and u
+// alternate between two buffers by iteration parity (a launch reads one and
+// writes the other).
+struct CgScal
+{
+	double alpha, gamma;
+};
+
+template <typename T>
+__device__ __forceinline__ void ld_minv(const T* __restrict__ minv, uint32_t n, uint32_t row, T (&m)[9])
+{
+#pragma unroll
+	for (int k = 0; k < 9; ++k)
+		m[k] = minv[k * (size_t)n + row];
+}
+
+// Sum of a row's contact entries, in list order (kCtBatch loads per round trip).
+template <typename T>
+__device__ __forceinline__ V3<T> row_contacts(const uint32_t* __restrict__ ct_off, const T* __restrict__ contrib,
+	uint32_t row)
+{
+	V3<T> a{ 0, 0, 0 };
+	for (uint32_t e = ct_off[row], end = ct_off[row + 1]; e < end; e += kCtBatch)
+	{
+		V3<T> c[kCtBatch];
+#pragma unroll
+		for (int j = 0; j < kCtBatch; ++j)
+			c[j] = ld_contrib(contrib, min(e + j, end - 1));
+#pragma unroll
+		for (int j = 0; j < kCtBatch; ++j)
+			if (e + j < end)
+			{
+				a.x += c[j].x;
+				a.y += c[j].y;
+				a.z += c[j].z;
+			}
+	}
+	return a;
+}
+
+// x = 0, r = b, u = M^-1 r and the partials of (r, u), (r, r), (b, b).
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_cg_init(uint32_t n, const double* __restrict__ rhs,
+	const T* __restrict__ minv, T* __restrict__ x, T* __restrict__ r, T* __restrict__ u, double* __restrict__ gamma_out,
+	double* __restrict__ rr_out, double* __restrict__ bb_out, uint32_t* __restrict__ flags)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	double		   g = 0.0, rr = 0.0, bb = 0.0;
+	if (i < n)
+	{
+		const V3<T> rv{ (T)rhs[3 * i], (T)rhs[3 * i + 1], (T)rhs[3 * i + 2] };
+		const V3<T> uv = apply_minv(minv, n, i, rv);
+		stv(x, i, V3<T>{ 0, 0, 0 });
+		stv(r, i, rv);
+		stv(u, i, uv);
+		g = dotd(rv, uv);
+		rr = dotd(rv, rv);
+		bb = rhs[3 * i] * rhs[3 * i] + rhs[3 * i + 1] * rhs[3 * i + 1] + rhs[3 * i + 2] * rhs[3 * i + 2];
+	}
+	block_partial(g, gamma_out);
+	block_partial(rr, rr_out);
+	block_partial(bb, bb_out);
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+	{
+		flags[0] = 0u;
+		flags[1] = 0u;
+		flags[2] = 0u;
+	}
+}
+
+// Workgroups [0, nb) are rows, [nb, nb + ncb) contacts (grid-stride over nc).
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_cg_spmv(uint32_t n, uint32_t nb, const uint32_t* __restrict__ slice_off,
+	const uint32_t* __restrict__ col, const T* __restrict__ vals, uint32_t nslots, const T* __restrict__ u,
+	T* __restrict__ w, double lambda, const T* __restrict__ scale, double* __restrict__ delta_rows,
+	double* __restrict__ delta_ct, const uint32_t* __restrict__ flags, const ContactRec* __restrict__ pairs,
+	uint32_t nc, const uint32_t* __restrict__ pos, T* __restrict__ contrib)
+{
+	const uint32_t stop = __hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	if (blockIdx.x < nb)
+	{
+		const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+		double		   d = 0.0;
+		V3<T>		   a{ 0, 0, 0 }, ur{ 0, 0, 0 };
+		if (row < n)
+		{
+			a = row_spmv<T, T, T>(row, slice_off, col, vals, nslots, u);
+			ur = ldv(u, row);
+			if (lambda != 0.0)
+			{
+				const T ls = (T)lambda * scale[row];
+				a.x += ls * ur.x;
+				a.y += ls * ur.y;
+				a.z += ls * ur.z;
+			}
+		}
+		if (stop)
+			return;
+		if (row < n)
+		{
+			stv(w, row, a);
+			d = dotd(ur, a);
+		}
+		block_partial(d, delta_rows);
+		return;
+	}
+	if (stop)
+		return;
+	const uint32_t cb = blockIdx.x - nb, ncb = gridDim.x - nb;
+	double		   d = 0.0;
+	for (uint32_t i = cb * blockDim.x + threadIdx.x; i < nc; i += ncb * blockDim.x)
+	{
+		const ContactRec c = pairs[i];
+		V3<T>			 uc[4];
+		for (int k = 0; k < 4; ++k)
+			uc[k] = ldv(u, c.idx[k] & kIndexMask);
+		v3<double> o[4];
+		d += contact_offdiag(c, uc, o);
+		for (int k = 0; k < 4; ++k)
+			st_contrib(contrib, pos[4 * i + k], (double)(T)o[k].x, (double)(T)o[k].y, (double)(T)o[k].z);
+	}
+	block_partial_at(d, delta_ct, cb);
+}
+
+template <typename T>
+struct CgVecArgs
+{
+	uint32_t		n, nb, ncb;
+	double			tol2;
+	const T*		minv;
+	T*				x;
+	T*				p;
+	const T*		w;		 // w_sell of k_cg_spmv
+	const uint32_t* ct_off;	 // per-vertex contact entry lists, or null without contacts
+	const T*		contrib;
+	const T*		r_in;
+	const T*		s_in;
+	const T*		u_in;
+	T*				r_out;
+	T*				s_out;
+	T*				u_out;
+	const double*	gamma_in;  // [nb] partials of (r, u) after the previous update
+	const double*	delta_in;  // [nb] partials of (u, w_sell)
+	const double*	delta_ct;  // [ncb] contact partials of u^T A_off u
+	const double*	rr_in;	   // [nb] partials of (r, r)
+	const double*	bb;		   // [nb] partials of (b, b)
+	double*			gamma_out;
+	double*			rr_out;
+	const CgScal*	scal_in;   // alpha and gamma of the previous update
+	CgScal*			scal_out;
+	uint32_t*		flags;
+};
+
+// Update `it` + 1 of the single-reduction recurrence, or the stop when the
+// residual of update `it` meets the tolerance (flags[1] = updates done).
+template <typename T>
+__global__ __launch_bounds__(kPcgThreads) void k_cg_vec(uint32_t it, CgVecArgs<T> a)
+{
+	const bool	   first = it == 0;
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	// the row's loads, its contact entries included, are issued before the partial sums are waited for
+	V3<T> uv{ 0, 0, 0 }, pv{ 0, 0, 0 }, wv{ 0, 0, 0 }, sv{ 0, 0, 0 }, xv{ 0, 0, 0 }, rv{ 0, 0, 0 };
+	T	  mi[9] = {};
+	if (i < a.n)
+	{
+		uv = ldv(a.u_in, i);
+		wv = ldv(a.w, i);
+		rv = ldv(a.r_in, i);
+		xv = ldv(a.x, i);
+		if (!first)
+		{
+			pv = ldv(a.p, i);
+			sv = ldv(a.s_in, i);
+		}
+		ld_minv(a.minv, a.n, i, mi);
+		if (a.ct_off)
+		{
+			const V3<T> c = row_contacts(a.ct_off, a.contrib, i);
+			wv = V3<T>{ wv.x + c.x, wv.y + c.y, wv.z + c.z };
+		}
+	}
+	const GroupSums<5> gs = group_sums<5>({ a.gamma_in, a.delta_in, a.delta_ct, a.rr_in, a.bb },
+		{ a.nb, a.nb, a.ncb, a.nb, a.nb }, a.flags);
+	if (gs.stop)
+		return;
+	const double gamma = gs.v[0], delta = gs.v[1] + gs.v[2], rr = gs.v[3], bb = gs.v[4];
+	const bool	 lead = blockIdx.x == 0 && threadIdx.x == 0;
+	if (!isfinite(rr) || rr <= a.tol2 * bb)
+	{
+		if (lead)
+		{
+			if (!isfinite(rr))
+				a.flags[2] = 2u;
+			a.flags[1] = it;
+			a.flags[0] = 1u;
+		}
+		return;
+	}
+	double beta = 0.0, den = delta;
+	if (!first)
+	{
+		const CgScal prev = *a.scal_in;
+		beta = gamma / prev.gamma;
+		den = delta - beta * gamma / prev.alpha;
+	}
+	const double alpha = gamma / den;
+	if (!(isfinite(alpha) && isfinite(beta) && den > 0.0))
+	{
+		if (lead)
+		{
+			a.flags[2] = 2u; // breakdown: u^T A u (or its recurrence) <= 0 or non-finite
+			a.flags[1] = it;
+			a.flags[0] = 1u;
+		}
+		return;
+	}
+	if (lead)
+	{
+		*a.scal_out = CgScal{ alpha, gamma };
+		a.flags[1] = it + 1;
+	}
+	const T al = (T)alpha, be = (T)beta;
+	double	gn = 0.0, rn = 0.0;
+	if (i < a.n)
+	{
+		const V3<T> pn = first ? uv : V3<T>{ uv.x + be * pv.x, uv.y + be * pv.y, uv.z + be * pv.z };
+		const V3<T> sn = first ? wv : V3<T>{ wv.x + be * sv.x, wv.y + be * sv.y, wv.z + be * sv.z };
+		xv = V3<T>{ xv.x + al * pn.x, xv.y + al * pn.y, xv.z + al * pn.z };
+		rv = V3<T>{ rv.x - al * sn.x, rv.y - al * sn.y, rv.z - al * sn.z };
+		const V3<T> un{ mi[0] * rv.x + mi[1] * rv.y + mi[2] * rv.z, mi[3] * rv.x + mi[4] * rv.y + mi[5] * rv.z,
+			mi[6] * rv.x + mi[7] * rv.y + mi[8] * rv.z };
+		stv(a.p, i, pn);
+		stv(a.x, i, xv);
+		stv(a.r_out, i, rv);
+		stv(a.s_out, i, sn);
+		stv(a.u_out, i, un);
+		gn = dotd(rv, un);
+		rn = dotd(rv, rv);
+	}
+	block_partial(gn, a.gamma_out);
+	block_partial(rn, a.rr_out);
+}
+
 // ---- FP64 residual and helpers -----------------------------------------------------
 // res = b - A x with FP64 accumulation (TV = stored value type); partial ||res||^2.
 template <typename TV>
@@ -733,10 +1000,10 @@ void PcgSolver::init(Context* c, uint32_t n_, uint32_t nslices_, uint32_t nslots
 	col = cl;
 	diag_slot = ds;
 	const uint32_t nb = div_up(n, kPcgThreads);
-	// x, r, z, p, p2, q (3n each), minv (9n), scale (n), r64 (3n), scale64 (n), d64 spare (3n) in doubles
-	// + 256 B of alignment slack per carved array (11 arrays)
-	work.reserve(sizeof(double) * ((size_t)6 * 3 * n + 9 * n + n + 3 * n + n + 3 * n) + 11 * 256);
-	partials.reserve(8 * (size_t)nb + 8 + kContactBlocks);
+	// x, p, w, r[2], s[2], u[2] (3n each), minv (9n), scale (n), r64 (3n), scale64 (n), d64 spare (3n) in
+	// doubles + 256 B of alignment slack per carved array (14 arrays)
+	work.reserve(sizeof(double) * ((size_t)9 * 3 * n + 9 * n + n + 3 * n + n + 3 * n) + 14 * 256);
+	partials.reserve(partials_size(nb));
 	flags.reserve(4);
 	if (!h_flags)
 		UT_HIP_CHECK(hipHostMalloc((void**)&h_flags, 16, hipHostMallocDefault));
@@ -797,7 +1064,9 @@ namespace
 template <typename T>
 struct Vecs
 {
-	T *x, *r, *z, *p, *p2, *q, *minv, *scale;
+	// classic recurrence: x, r = r[0], z = u[0], directions p / u[1] (ping-pong), q = w;
+	// single reduction: x, p, w and r, s, u by iteration parity
+	T *x, *p, *w, *r[2], *s[2], *u[2], *minv, *scale;
 	double *r64, *scale64, *spare;
 };
 template <typename T>
@@ -812,11 +1081,14 @@ Vecs<T> carve(uint8_t* base, uint32_t n)
 		return p;
 	};
 	v.x = (T*)take(3 * (size_t)n, sizeof(T));
-	v.r = (T*)take(3 * (size_t)n, sizeof(T));
-	v.z = (T*)take(3 * (size_t)n, sizeof(T));
 	v.p = (T*)take(3 * (size_t)n, sizeof(T));
-	v.p2 = (T*)take(3 * (size_t)n, sizeof(T));
-	v.q = (T*)take(3 * (size_t)n, sizeof(T));
+	v.w = (T*)take(3 * (size_t)n, sizeof(T));
+	for (int k = 0; k < 2; ++k)
+	{
+		v.r[k] = (T*)take(3 * (size_t)n, sizeof(T));
+		v.s[k] = (T*)take(3 * (size_t)n, sizeof(T));
+		v.u[k] = (T*)take(3 * (size_t)n, sizeof(T));
+	}
 	v.minv = (T*)take(9 * (size_t)n, sizeof(T));
 	v.scale = (T*)take(n, sizeof(T));
 	v.r64 = (double*)take(3 * (size_t)n, sizeof(double));
@@ -825,6 +1097,26 @@ Vecs<T> carve(uint8_t* base, uint32_t n)
 	return v;
 }
 
+// Partial-sum arrays (nb = one entry per workgroup of rows) in `partials`.
+struct PcgLayout
+{
+	PcgParts parts;
+	double*	 rr2;	   // [nb] second (r, r) buffer of the single-reduction recurrence
+	double*	 res_part; // [nb] FP64 residual norm
+	double*	 d_norm;   // [8]
+	CgScal*	 scal;	   // [2] by iteration parity
+};
+PcgLayout layout(double* pb, uint32_t nb)
+{
+	PcgLayout l;
+	l.parts = PcgParts{ pb, pb + 7 * (size_t)nb + 16, { pb + nb, pb + 2 * (size_t)nb }, pb + 3 * (size_t)nb,
+		pb + 4 * (size_t)nb };
+	l.res_part = pb + 5 * (size_t)nb;
+	l.rr2 = pb + 6 * (size_t)nb;
+	l.d_norm = pb + 7 * (size_t)nb;
+	l.scal = reinterpret_cast<CgScal*>(pb + 7 * (size_t)nb + 8);
+	return l;
+}
 // Runs PCG iterations of one (inner) solve until done or the budget is used.
 template <typename T>
 uint32_t run_iterations(PcgSolver& s, const Vecs<T>& v, const T* vals, PcgParts parts, double lambda, double tol,
@@ -845,14 +1137,50 @@ uint32_t run_iterations(PcgSolver& s, const Vecs<T>& v, const T* vals, PcgParts 
 		for (uint32_t k = 0; k < chunk; ++k, ++it)
 		{
 			// iteration it uses direction buffer it & 1 and writes the next one to the other
-			T* const p = (it & 1) ? v.p2 : v.p;
-			T* const pn = (it & 1) ? v.p : v.p2;
-			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(s.n, s.slice_off, s.col, vals, s.nslots, p, v.q, lambda,
+			T* const p = (it & 1) ? v.u[1] : v.p;
+			T* const pn = (it & 1) ? v.p : v.u[1];
+			k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(s.n, s.slice_off, s.col, vals, s.nslots, p, v.w, lambda,
 				v.scale, parts, s.flags.get(), nce ? s.ct_off.get() : nullptr, contrib);
 			k_pcg_update<T><<<nb, kPcgThreads, 0, st>>>(
-				it, s.n, nb, v.minv, v.x, v.r, v.z, p, v.q, parts, s.flags.get());
+				it, s.n, nb, v.minv, v.x, v.r[0], v.u[0], p, v.w, parts, s.flags.get());
 			k_pcg_dir<T><<<std::max(nb, div_up(nce, kPcgThreads)), kPcgThreads, 0, st>>>(
-				it, s.n, nb, tol2, v.z, p, pn, parts, s.flags.get(), s.contacts, nce, s.ct_pos.get(), contrib);
+				it, s.n, nb, tol2, v.u[0], p, pn, parts, s.flags.get(), s.contacts, nce, s.ct_pos.get(), contrib);
+		}
+		UT_LAUNCH_CHECK();
+		UT_HIP_CHECK(hipMemcpyAsync(s.h_flags, s.flags.get(), 12, hipMemcpyDeviceToHost, st));
+		UT_HIP_CHECK(hipStreamSynchronize(st));
+		if (s.h_flags[0])
+			break;
+	}
+	return s.h_flags[1];
+}
+
+// Single-reduction iterations of one (inner) solve, after cg_start: update `it`
+// reads r, s, u[it & 1] and writes [(it + 1) & 1]; the SpMV then takes the new u.
+template <typename T>
+uint32_t run_iterations_cg(PcgSolver& s, const Vecs<T>& v, const T* vals, const PcgLayout& L, double lambda,
+	double tol, uint32_t budget, uint32_t check_every)
+{
+	hipStream_t	   st = s.ctx->stream;
+	const uint32_t nb = div_up(s.n, kPcgThreads);
+	const uint32_t nce = s.ct_entries / 4;
+	const uint32_t ncb = cg_contact_groups(nce);
+	T*			   contrib = reinterpret_cast<T*>(s.ct_contrib.get());
+	double* const  rrb[2] = { L.parts.rr, L.rr2 };
+	uint32_t	   it = 0;
+	while (it < budget)
+	{
+		const uint32_t chunk = std::min(check_every, budget - it);
+		for (uint32_t k = 0; k < chunk; ++k, ++it)
+		{
+			const uint32_t c = it & 1, o = c ^ 1;
+			CgVecArgs<T>   a{ s.n, nb, std::max(ncb, 1u), tol * tol, v.minv, v.x, v.p, v.w,
+				  nce ? s.ct_off.get() : nullptr, contrib, v.r[c], v.s[c], v.u[c], v.r[o], v.s[o], v.u[o],
+				  L.parts.rz[o], L.parts.pq, L.parts.pqc, rrb[o], L.parts.bb, L.parts.rz[c], rrb[c], L.scal + c,
+				  L.scal + o, s.flags.get() };
+			k_cg_vec<T><<<nb, kPcgThreads, 0, st>>>(it, a);
+			k_cg_spmv<T><<<nb + ncb, kPcgThreads, 0, st>>>(s.n, nb, s.slice_off, s.col, vals, s.nslots, v.u[o], v.w,
+				lambda, v.scale, L.parts.pq, L.parts.pqc, s.flags.get(), s.contacts, nce, s.ct_pos.get(), contrib);
 		}
 		UT_LAUNCH_CHECK();
 		UT_HIP_CHECK(hipMemcpyAsync(s.h_flags, s.flags.get(), 12, hipMemcpyDeviceToHost, st));
@@ -870,10 +1198,11 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 	hipStream_t	   st = s.ctx->stream;
 	const uint32_t n = s.n, nb = div_up(n, kPcgThreads);
 	Vecs<T>		   v = carve<T>(s.work.get(), n);
-	double*		   pb = s.partials.get();
-	PcgParts	   parts{ pb, pb + 8 * nb + 8, { pb + nb, pb + 2 * nb }, pb + 3 * nb, pb + 4 * nb };
-	double*		   res_part = pb + 5 * nb;
-	double*		   d_norm = pb + 6 * nb;
+	const PcgLayout L = layout(s.partials.get(), nb);
+	const PcgParts	parts = L.parts;
+	double* const	res_part = L.res_part;
+	double* const	d_norm = L.d_norm;
+	const bool		cg = prm.recurrence == kPcgSingleReduction;
 	// the contact part of p^T A p is in the SpMV's own partials
 	UT_HIP_CHECK(hipMemsetAsync(parts.pqc, 0, sizeof(double) * kContactBlocks, st));
 	s.prepare_contacts();
@@ -927,11 +1256,35 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 	uint32_t used = 0;
 	double	 rn = bn;
 	bool	 tol_reached = false;
+	// x = 0, r = rhs, the first direction (classic) or u and w = A u (single reduction)
+	auto start = [&](const double* rhs)
+	{
+		if (cg)
+		{
+			const uint32_t nce = s.ct_entries / 4;
+			T*			   contrib = reinterpret_cast<T*>(s.ct_contrib.get());
+			k_cg_init<T><<<nb, kPcgThreads, 0, st>>>(n, rhs, v.minv, v.x, v.r[0], v.u[0], parts.rz[1], L.rr2, parts.bb,
+				s.flags.get());
+			k_pcg_check0<<<1, kPcgThreads, 0, st>>>(nb, parts, s.flags.get());
+			k_cg_spmv<T><<<nb + cg_contact_groups(nce), kPcgThreads, 0, st>>>(n, nb, s.slice_off, s.col, vals, s.nslots,
+				v.u[0], v.w, prm.regularization, v.scale, parts.pq, parts.pqc, s.flags.get(), s.contacts, nce,
+				s.ct_pos.get(), contrib);
+		}
+		else
+		{
+			k_pcg_init<T><<<nb, kPcgThreads, 0, st>>>(n, rhs, v.minv, v.x, v.r[0], v.u[0], v.p, parts, s.flags.get());
+			k_pcg_check0<<<1, kPcgThreads, 0, st>>>(nb, parts, s.flags.get());
+		}
+	};
+	auto iterate = [&](double tol, uint32_t budget) -> uint32_t
+	{
+		return cg ? run_iterations_cg<T>(s, v, vals, L, prm.regularization, tol, budget, prm.check_every)
+				  : run_iterations<T>(s, v, vals, parts, prm.regularization, tol, budget, prm.check_every);
+	};
 	if (!mixed)
 	{
-		k_pcg_init<T><<<nb, kPcgThreads, 0, st>>>(n, b, v.minv, v.x, v.r, v.z, v.p, parts, s.flags.get());
-		k_pcg_check0<<<1, kPcgThreads, 0, st>>>(nb, parts, s.flags.get());
-		used = run_iterations<T>(s, v, vals, parts, prm.regularization, prm.rel_tol, prm.max_iters, prm.check_every);
+		start(b);
+		used = iterate(prm.rel_tol, prm.max_iters);
 		if (s.h_flags[2])
 			rep.failure = 2;
 		// converged by the iteration's own criterion (recursive residual), as the reference does
@@ -946,11 +1299,10 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 		{
 			if (!(rn > prm.rel_tol * bn))
 				break;
-			k_pcg_init<T><<<nb, kPcgThreads, 0, st>>>(n, v.r64, v.minv, v.x, v.r, v.z, v.p, parts, s.flags.get());
-			k_pcg_check0<<<1, kPcgThreads, 0, st>>>(nb, parts, s.flags.get());
+			start(v.r64);
 			// the inner solve only has to gain the remaining factor, bounded by inner_tol
 			const double want = std::max(prm.inner_tol, 0.5 * prm.rel_tol * bn / rn);
-			used += run_iterations<T>(s, v, vals, parts, prm.regularization, want, prm.max_iters - used, prm.check_every);
+			used += iterate(want, prm.max_iters - used);
 			if (s.h_flags[2] == 1u)
 			{
 				rep.failure = 2;
@@ -1038,18 +1390,17 @@ float profile_spmv_typed(PcgSolver& s, const T* vals, uint32_t reps)
 	hipStream_t	   st = s.ctx->stream;
 	const uint32_t n = s.n, nb = div_up(n, kPcgThreads);
 	Vecs<T>		   v = carve<T>(s.work.get(), n);
-	double*		   pb = s.partials.get();
-	PcgParts	   parts{ pb, pb + 8 * nb + 8, { pb + nb, pb + 2 * nb }, pb + 3 * nb, pb + 4 * nb };
+	const PcgParts parts = layout(s.partials.get(), nb).parts;
 	k_pcg_zero_flags<<<1, 64, 0, st>>>(s.flags.get());
 	hipEvent_t a, b;
 	UT_HIP_CHECK(hipEventCreate(&a));
 	UT_HIP_CHECK(hipEventCreate(&b));
 	k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
-		n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get(), nullptr, (const T*)nullptr);
+		n, s.slice_off, s.col, vals, s.nslots, v.p, v.w, 0.0, v.scale, parts, s.flags.get(), nullptr, (const T*)nullptr);
 	UT_HIP_CHECK(hipEventRecord(a, st));
 	for (uint32_t i = 0; i < reps; ++i)
 		k_pcg_spmv<T><<<nb, kPcgThreads, 0, st>>>(
-			n, s.slice_off, s.col, vals, s.nslots, v.p, v.q, 0.0, v.scale, parts, s.flags.get(), nullptr, (const T*)nullptr);
+			n, s.slice_off, s.col, vals, s.nslots, v.p, v.w, 0.0, v.scale, parts, s.flags.get(), nullptr, (const T*)nullptr);
 	UT_HIP_CHECK(hipEventRecord(b, st));
 	UT_HIP_CHECK(hipEventSynchronize(b));
 	float ms = 0.0f;

@@ -30,6 +30,12 @@ enum PcgPrecision : int
 	kPcgMixed = 1,
 	kPcgFp32 = 2,
 };
+// Krylov recurrence of the iteration (the iterates are the same in exact arithmetic)
+enum PcgRecurrence : int
+{
+	kPcgClassic = 0,		 // precond_cg.cpp's loop: two reductions, three launches per iteration
+	kPcgSingleReduction = 1, // Chronopoulos-Gear: one reduction point, two launches per iteration
+};
 enum PcgPreconditioner : int
 {
 	kPrecBlockJacobi = 0, // inverse of each 3x3 diagonal block
@@ -46,6 +52,7 @@ struct PcgParams
 	uint32_t max_refine = 20;	   // mixed: refinement steps
 	double	 regularization = 0.0; // lambda: q += lambda * mean(diag) * p, precond_cg.cpp:286-339
 	uint32_t check_every = 8;	   // iterations launched between host convergence checks
+	int		 recurrence = kPcgClassic;
 };
 
 struct PcgReport

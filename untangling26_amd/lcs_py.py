@@ -85,7 +85,7 @@ def _float3(v) -> Float3:
 class SceneParams:
     """SceneParams (Solver/SimulationCore/scene_params.h:24-157) with the reference's
     field names and defaults for the fields the step reads, and this build's additions
-    (pcg_precision, pcg_rel_tol, pcg_inner_tol, pcg_block_jacobi, prp_directions,
+    (pcg_precision, pcg_rel_tol, pcg_inner_tol, pcg_block_jacobi, pcg_single_reduction, prp_directions,
     use_region_filter).  ``gravity`` and ``floor`` accept a Float3 or any 3-sequence."""
 
     _DEFAULTS = dict(
@@ -109,6 +109,9 @@ class SceneParams:
         PRP_solid_interior_adjacency_dist_eps=1e-4,
         # additions of this build
         pcg_precision=PCG_MIXED, pcg_rel_tol=1e-6, pcg_inner_tol=1e-2, pcg_block_jacobi=True,
+        # the single-reduction (Chronopoulos-Gear) form of the PCG loop: two launches per
+        # iteration instead of three; False runs precond_cg.cpp's recurrence
+        pcg_single_reduction=True,
         prp_directions=10,  # 10 = the reference's direction generator; more appends a Fibonacci set
         use_region_filter=True,  # filter_invalid_pairs / filter_broad_phase_pairs under use_untangling
     )
@@ -405,7 +408,7 @@ class NewtonSolver:
         p.use_ccd = int(c.use_ccd_linesearch)
         p.use_untangling = int(c.use_untangling)
         p.pcg = PcgParams(c.pcg_precision, 0 if c.pcg_block_jacobi else 1, c.pcg_iter_count, c.pcg_rel_tol,
-                          c.pcg_inner_tol, 20, 0.0, 8)
+                          c.pcg_inner_tol, 20, 0.0, 8, int(bool(c.pcg_single_reduction)))
         p.use_prp = int(c.use_untangling_PRP)
         p.quasi_static = int(c.use_quasi_static_mode)
         p.prp = PrpParams(c.untangling_process_contours_count, c.prp_directions, c.PRP_cpu_contour_batch_size,

@@ -10,6 +10,9 @@ from .runtime import Context
 
 FP64, MIXED, FP32 = 0, 1, 2
 BLOCK_JACOBI, POINT_JACOBI = 0, 1
+# Krylov recurrence: the reference's loop (precond_cg.cpp:706-914), or its
+# single-reduction (Chronopoulos-Gear) form with two launches per iteration
+CLASSIC, SINGLE_REDUCTION = 0, 1
 
 
 class ConjugateGradientSolver:
@@ -39,7 +42,8 @@ class ConjugateGradientSolver:
             pass
 
     def device_solve(self, vals, b, x, precision=MIXED, preconditioner=BLOCK_JACOBI, max_iters=100, rel_tol=1e-6,
-                     inner_tol=1e-2, max_refine=20, regularization=0.0, check_every=8, vals64=None):
+                     inner_tol=1e-2, max_refine=20, regularization=0.0, check_every=8, vals64=None,
+                     recurrence=CLASSIC):
         """Solves A x = b; returns the PcgSolveReport as a dict (also kept in last_solve_report).
         In MIXED mode `vals64` (the FP64 matrix) makes the refinement residual FP64, so the
         result solves the FP64 system; without it the FP32 matrix is refined against."""
@@ -47,7 +51,7 @@ class ConjugateGradientSolver:
         assert vals.dtype == want and vals.is_contiguous(), f"values must be {want}"
         assert b.dtype == torch.float64 and x.dtype == torch.float64
         p = PcgParams(precision, preconditioner, max_iters, rel_tol, inner_tol, max_refine, regularization,
-                      check_every)
+                      check_every, recurrence)
         r = PcgReport()
         if vals64 is not None:
             assert precision == MIXED and vals64.dtype == torch.float64 and vals64.is_contiguous()
