@@ -1,0 +1,65 @@
+"""Bit-reproducibility of the untangling step on the device.
+
+The proximity records are put in (kind, query, leaf) order after detection,
+the PRP's kept hits and response records keep the sorted hit order, the
+assembly adds faces and hinges by colour class (no two elements of a class
+share a vertex) and gathers the contacts per vertex, and every PCG sum has a
+fixed order; so two solvers given the same state and parameters produce the
+same bits, step after step.  Two steps of the folded cloth (48 x 48, the CLI's
+untangling configuration: PRP with 16 directions, region filter, friction, one
+Newton iteration, and a second run with two Newton iterations so the lagged
+friction records take part) are run twice and compared bit for bit: positions,
+EF pair and contour counts, PCG iterations and residual."""
+import numpy as np
+import pytest
+import torch
+
+from untangling26_amd import scenes
+from untangling26_amd.lcs_py import PCG_FP64, PCG_MIXED, NewtonSolver
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(precision, newton_iters):
+    m = scenes.folded_cloth(0, n=48)
+    rest = scenes.folded_cloth_rest(n=48)
+    s = NewtonSolver()
+    s.init_device("hip")
+    s.register_world_data(s.create_world_data_from_array("fold48", m.x, m.faces, rest))
+    c = s.get_config()
+    c.use_untangling = True
+    c.use_quasi_static_mode = True
+    c.gravity = (0.0, 0.0, 0.0)
+    c.contact_energy_type = 0
+    c.use_untangling_PRP = True
+    c.untangling_response_depth = 1.0
+    c.nonlinear_iter_count = newton_iters
+    c.use_ccd_linesearch = True
+    c.ignore_near_zero_dist_pairs = False
+    c.pcg_iter_count = 400
+    c.pcg_precision = precision
+    c.prp_directions = 16
+    s.init_solver()
+    return s
+
+
+def _run(precision, newton_iters, steps=2):
+    s = _solver(precision, newton_iters)
+    out = []
+    for _ in range(steps):
+        r = s.physics_step_gpu()
+        torch.cuda.synchronize()
+        out.append((s.positions().cpu().numpy().copy(),
+                    {k: r[k] for k in ("ef_total", "contours", "pcg_iterations", "pcg_relative_residual",
+                                       "prp_responses", "vf_pairs", "ee_pairs")}))
+    return out
+
+
+@pytest.mark.parametrize("precision,newton_iters", [(PCG_MIXED, 1), (PCG_FP64, 2)])
+def test_two_runs_are_bit_identical(ctx, precision, newton_iters):
+    a = _run(precision, newton_iters)
+    b = _run(precision, newton_iters)
+    assert a[0][1]["ef_total"] > 0 and a[0][1]["prp_responses"] > 0
+    for k, ((xa, ra), (xb, rb)) in enumerate(zip(a, b)):
+        assert ra == rb, (k, ra, rb)
+        assert np.array_equal(xa.view(np.uint32), xb.view(np.uint32)), f"step {k}: positions differ"

@@ -14,6 +14,7 @@
 #include "contact_math.hpp"
 #include "friction_math.hpp"
 #include "lbvh_traverse.hpp"
+#include "radix_sort.hpp"
 
 #include <algorithm>
 
@@ -606,6 +607,81 @@ void friction_lagged(Context* ctx, const ContactRec* pairs, const uint32_t* coun
 	else
 		k_friction_lagged<float><<<256, 256, 0, ctx->stream>>>(pairs, count, cap, x, x_start, out);
 	UT_LAUNCH_CHECK();
+}
+
+namespace
+{
+__global__ void k_cs_leaf_keys(uint32_t n, const uint2* __restrict__ ids, uint32_t* __restrict__ keys,
+	uint32_t* __restrict__ vals)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+	{
+		keys[i] = ids[i].y;
+		vals[i] = i;
+	}
+}
+
+__global__ void k_cs_query_keys(uint32_t n, uint32_t nvf, int query_bits, const uint2* __restrict__ ids,
+	const uint32_t* __restrict__ perm, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < n)
+	{
+		const uint32_t i = perm[j];
+		keys[j] = ((i >= nvf ? 1u : 0u) << query_bits) | ids[i].x;
+		vals[j] = i;
+	}
+}
+
+__global__ void k_cs_gather(uint32_t n, const uint32_t* __restrict__ perm, const ContactRec* __restrict__ recs,
+	const uint2* __restrict__ ids, ContactRec* __restrict__ out, uint2* __restrict__ out_ids)
+{
+	const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < n)
+	{
+		out[j] = recs[perm[j]];
+		out_ids[j] = ids[perm[j]];
+	}
+}
+} // namespace
+
+void ContactSorter::sort(Context* ctx, ContactRec* recs, uint32_t* ids32, uint32_t n, uint32_t nvf, int query_bits,
+	int leaf_bits)
+{
+	UT_REQUIRE(query_bits >= 1 && query_bits <= 31 && leaf_bits >= 1 && leaf_bits <= 32,
+		"contact sort: id bits out of range");
+	if (n < 2)
+		return;
+	const hipStream_t s = ctx->stream;
+	for (int k = 0; k < 2; ++k)
+	{
+		keys[k].reserve(n);
+		vals[k].reserve(n);
+	}
+	hist.reserve(radix_sort_hist_size(n));
+	tmp.reserve(n);
+	tmp_ids.reserve(2 * (size_t)n);
+	const uint2* ids = reinterpret_cast<const uint2*>(ids32);
+	// LSD: stable by leaf, then stable by (kind, query)
+	k_cs_leaf_keys<<<div_up(n, 256), 256, 0, s>>>(n, ids, keys[0].get(), vals[0].get());
+	RadixSortBuffers a;
+	a.keys[0] = keys[0].get(), a.keys[1] = keys[1].get();
+	a.vals[0] = vals[0].get(), a.vals[1] = vals[1].get();
+	a.hist = hist.get();
+	radix_sort_pairs(a, n, leaf_bits, s);
+	const int r = a.result, o = r ^ 1;
+	k_cs_query_keys<<<div_up(n, 256), 256, 0, s>>>(n, nvf, query_bits, ids, a.vals[r], a.keys[o], a.vals[o]);
+	RadixSortBuffers b;
+	b.keys[0] = a.keys[o], b.keys[1] = a.keys[r];
+	b.vals[0] = a.vals[o], b.vals[1] = a.vals[r];
+	b.hist = hist.get();
+	radix_sort_pairs(b, n, query_bits + 1, s);
+	k_cs_gather<<<div_up(n, 256), 256, 0, s>>>(n, b.vals[b.result], recs, ids, tmp.get(),
+		reinterpret_cast<uint2*>(tmp_ids.get()));
+	UT_LAUNCH_CHECK();
+	UT_HIP_CHECK(hipMemcpyAsync(recs, tmp.get(), sizeof(ContactRec) * (size_t)n, hipMemcpyDeviceToDevice, s));
+	UT_HIP_CHECK(hipMemcpyAsync(ids32, tmp_ids.get(), 8 * (size_t)n, hipMemcpyDeviceToDevice, s));
 }
 
 } // namespace ut

@@ -44,6 +44,20 @@ struct ContactArgs
 void contact_query(Context* ctx, int kind, const ContactArgs& a, Lbvh* tree, const uint32_t* broad,
 	const uint32_t* broad_count, uint32_t broad_cap);
 void toi_reset(Context* ctx, float* toi);
+
+// Puts the proximity records [0, n) and their (query, leaf) ids in the order of
+// (kind, query, leaf), kind 0 (VF) for [0, nvf) and 1 (EE) after: the detection
+// appends with atomics, so its order changes from run to run, while everything
+// summed over the records afterwards (assembly, the PCG's contact products)
+// must see one fixed order to be bit-reproducible.  query_bits / leaf_bits bound
+// the ids.  VF records stay first.
+struct ContactSorter
+{
+	DevBuf<uint32_t>   keys[2], vals[2], hist;
+	DevBuf<ContactRec> tmp;
+	DevBuf<uint32_t>   tmp_ids;
+	void sort(Context* ctx, ContactRec* recs, uint32_t* ids, uint32_t n, uint32_t nvf, int query_bits, int leaf_bits);
+};
 enum PotentialTerms : int
 {
 	kTermBarrier = 1,  // repulsion (barrier or quadratic) re-solved at x

@@ -1,4 +1,6 @@
 #pragma once
+
+#include <vector>
 #include "ut_runtime.hpp"
 #include "contact.hpp"
 
@@ -31,6 +33,21 @@ struct AssembleArgs
 	const ContactRec* contacts = nullptr;
 	const uint32_t*	  contact_count = nullptr;
 	uint32_t		  contact_cap = 0;
+	// Deterministic assembly (optional).  Without it every term is added with FP64
+	// atomics, so the rounding of a sum depends on the order the hardware applies
+	// them in.  With it: faces and hinges in colour classes whose members share no
+	// vertex, applied class after class with plain read-modify-writes (each slot
+	// and gradient entry gets its terms in a fixed order), and the contacts
+	// gathered per vertex from entry lists sorted by vertex (ct_off[v] ..
+	// ct_off[v + 1] of ct_entries, entry e = 4 c + a: corner a of contact c).
+	const uint32_t* face_order = nullptr;	   // [nf] faces grouped by colour
+	const uint32_t* face_color_off = nullptr;  // HOST [face_colors + 1]
+	uint32_t		face_colors = 0;
+	const uint32_t* hinge_order = nullptr;	   // [nh]
+	const uint32_t* hinge_color_off = nullptr; // HOST [hinge_colors + 1]
+	uint32_t		hinge_colors = 0;
+	const uint32_t* ct_off = nullptr;	  // [nv + 1]
+	const uint32_t* ct_entries = nullptr; // sorted entries
 	// outputs
 	const uint32_t* diag_slot = nullptr;
 	double*			grad = nullptr;	  // dE/dx, double[3V] (the Newton rhs is -grad)
@@ -40,5 +57,12 @@ struct AssembleArgs
 };
 
 void assemble(Context* ctx, const AssembleArgs& a);
+
+// Greedy colouring of elements of k vertices each (elem[k * i + j]) so that no two
+// elements of one colour share a vertex: order = elements by colour (ascending
+// index within a colour), color_off = colour offsets.  Returns false when more
+// than 256 colours would be needed.
+bool color_elements(const uint32_t* elem, uint32_t n, int k, uint32_t nv, std::vector<uint32_t>& order,
+	std::vector<uint32_t>& color_off);
 
 } // namespace ut

@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kPcgThreads) void k_contact_contrib(const ContactRe
 	const T* __restrict__ p, const uint32_t* __restrict__ pos, T* __restrict__ contrib,
 	const uint32_t* __restrict__ flags)
 {
-	if (group_stopped(flags))
+	if (flags && group_stopped(flags))
 		return;
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nc)
@@ -908,7 +908,7 @@ template <typename TV>
 __global__ __launch_bounds__(kPcgThreads) void k_residual64(uint32_t n, const uint32_t* __restrict__ slice_off,
 	const uint32_t* __restrict__ col, const TV* __restrict__ vals, uint32_t nslots, const double* __restrict__ x,
 	const double* __restrict__ b, double* __restrict__ res, double lambda, const double* __restrict__ scale64,
-	double* __restrict__ part)
+	double* __restrict__ part, const uint32_t* __restrict__ ct_off, const double* __restrict__ contrib)
 {
 	const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
 	double		   rr = 0.0;
@@ -921,6 +921,13 @@ __global__ __launch_bounds__(kPcgThreads) void k_residual64(uint32_t n, const ui
 			a.x += ls * x[3 * row];
 			a.y += ls * x[3 * row + 1];
 			a.z += ls * x[3 * row + 2];
+		}
+		if (ct_off) // the contacts' off-diagonal products of x, in list order
+		{
+			const V3<double> c = row_contacts(ct_off, contrib, row);
+			a.x += c.x;
+			a.y += c.y;
+			a.z += c.z;
 		}
 		const V3<double> rv{ b[3 * row] - a.x, b[3 * row + 1] - a.y, b[3 * row + 2] - a.z };
 		if (res)
@@ -1022,6 +1029,7 @@ PcgSolver::~PcgSolver()
 void PcgSolver::prepare_contacts()
 {
 	ct_entries = 0;
+	ct_sorted = nullptr;
 	if (!contacts || !contact_count || !contact_cap)
 		return;
 	hipStream_t st = ctx->stream;
@@ -1054,6 +1062,7 @@ void PcgSolver::prepare_contacts()
 		bits += 8;
 	radix_sort_pairs(rs, m, bits, st);
 	k_ct_entry_pos<<<div_up(m, 256), 256, 0, st>>>(m, rs.vals[rs.result], ct_pos.get());
+	ct_sorted = rs.vals[rs.result];
 	k_ct_entry_off<<<div_up(n + 1, 256), 256, 0, st>>>(n, m, rs.keys[rs.result], ct_off.get());
 	UT_LAUNCH_CHECK();
 	ct_entries = m;
@@ -1205,7 +1214,9 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 	const bool		cg = prm.recurrence == kPcgSingleReduction;
 	// the contact part of p^T A p is in the SpMV's own partials
 	UT_HIP_CHECK(hipMemsetAsync(parts.pqc, 0, sizeof(double) * kContactBlocks, st));
-	s.prepare_contacts();
+	if (!s.contacts_prepared)
+		s.prepare_contacts();
+	s.contacts_prepared = false;
 	PcgReport	   rep;
 	const bool	   mixed = prm.precision == kPcgMixed;
 
@@ -1217,18 +1228,21 @@ PcgReport solve_typed(PcgSolver& s, const PcgParams& prm, const T* vals, const d
 	auto residual_norm = [&]() -> double
 	{
 		// the refinement residual uses the FP64 matrix when one is given
+		// the contacts' products of x (FP64) into the entry lists, summed by each row in list
+		// order: the residual has one fixed summation order (the contribution buffer is free
+		// between inner solves, which recompute their own products)
+		const uint32_t nce = s.ct_entries / 4;
+		double*		   c64 = s.ct_contrib.get();
+		if (nce)
+			k_contact_contrib<double><<<div_up(nce, kPcgThreads), kPcgThreads, 0, st>>>(
+				s.contacts, nce, x, s.ct_pos.get(), c64, nullptr);
+		const uint32_t* co = nce ? s.ct_off.get() : nullptr;
 		if (vals64)
-			k_residual64<double><<<nb, kPcgThreads, 0, st>>>(
-				n, s.slice_off, s.col, vals64, s.nslots, x, b, v.r64, prm.regularization, v.scale64, res_part);
+			k_residual64<double><<<nb, kPcgThreads, 0, st>>>(n, s.slice_off, s.col, vals64, s.nslots, x, b, v.r64,
+				prm.regularization, v.scale64, res_part, co, c64);
 		else
-			k_residual64<T><<<nb, kPcgThreads, 0, st>>>(
-				n, s.slice_off, s.col, vals, s.nslots, x, b, v.r64, prm.regularization, v.scale64, res_part);
-		if (s.contacts)
-		{
-			k_contact_offdiag<double><<<kContactBlocks, kPcgThreads, 0, st>>>(
-				s.contacts, s.contact_count, s.contact_cap, x, v.r64, -1.0, nullptr, nullptr);
-			k_norm2_64<<<nb, kPcgThreads, 0, st>>>(n, v.r64, res_part);
-		}
+			k_residual64<T><<<nb, kPcgThreads, 0, st>>>(n, s.slice_off, s.col, vals, s.nslots, x, b, v.r64,
+				prm.regularization, v.scale64, res_part, co, c64);
 		k_sum_partials_out<<<1, kPcgThreads, 0, st>>>(res_part, nb, d_norm);
 		UT_HIP_CHECK(hipMemcpyAsync(s.h_norm, d_norm, 8, hipMemcpyDeviceToHost, st));
 		UT_HIP_CHECK(hipStreamSynchronize(st));

@@ -88,8 +88,13 @@ struct PcgSolver
 	// vertex's entries in order: no atomics, fixed summation order.
 	uint32_t		 ct_entries = 0;
 	DevBuf<uint32_t> ct_keys[2], ct_vals[2], ct_hist, ct_off, ct_pos;
+	const uint32_t*	 ct_sorted = nullptr; // the entries e in sorted order (into ct_vals)
 	DevBuf<double>	 ct_contrib; // [4 * entries] (x, y, z, 0), FP32 or FP64 by the solve's precision
+	// Builds the lists for the current contacts.  solve() builds them itself unless
+	// the caller did since the contacts last changed (contacts_prepared, cleared
+	// by the solve), e.g. to share them with the assembly.
 	void			 prepare_contacts();
+	bool			 contacts_prepared = false;
 
 	DevBuf<uint8_t> work; // vectors (sized for FP64)
 	DevBuf<double>	partials;

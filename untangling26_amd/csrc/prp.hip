@@ -1688,16 +1688,27 @@ __global__ void k_prp_adjacent_contours(uint32_t n, const PrpHit* __restrict__ h
 }
 
 // make_response_from_min_hitinfo (:2105-2221): one CollisionPairTemplate per kept hit
+// of an unsuppressed contour, at its rank among those (flags + exclusive scan: the
+// records keep the final list's order)
+__global__ void k_prp_response_flags(uint32_t n, const uint32_t* __restrict__ hc,
+	const uint8_t* __restrict__ suppressed, uint32_t* __restrict__ flag)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		flag[i] = suppressed[hc[i]] ? 0u : 1u;
+}
+
 __global__ void k_prp_responses(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ hc,
 	const float* __restrict__ cdir, const float* __restrict__ cC, const uint8_t* __restrict__ suppressed,
 	const uint32_t* __restrict__ edges, const uint32_t* __restrict__ faces, const float* __restrict__ va,
-	const float* __restrict__ ea, const float* __restrict__ fa, float kappa, ContactRec* __restrict__ out,
-	uint4* __restrict__ out_meta, uint32_t* __restrict__ out_count, uint32_t cap)
+	const float* __restrict__ ea, const float* __restrict__ fa, float kappa, const uint32_t* __restrict__ pos,
+	ContactRec* __restrict__ out, uint4* __restrict__ out_meta, uint32_t cap)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool	   live = i < n && !suppressed[hc[i]];
-	const uint32_t slot = wave_append(out_count, live);
-	if (!live || slot >= cap)
+	if (i >= n || suppressed[hc[i]])
+		return;
+	const uint32_t slot = pos[i];
+	if (slot >= cap)
 		return;
 	const PrpHit   x = h[i];
 	const uint32_t c = hc[i];
@@ -1739,18 +1750,26 @@ __global__ void k_prp_responses(uint32_t n, const PrpHit* __restrict__ h, const 
 								 : make_uint4(x.src, x.dst, region, 2u);
 }
 
-// copy the kept hits of the chosen slots (root == slot root) to the final list
-__global__ void k_prp_keep(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ slot_root,
-	const PrpSlot* __restrict__ slots, uint32_t c0, PrpHit* __restrict__ out, uint32_t* __restrict__ out_c,
-	uint32_t* __restrict__ count, uint32_t cap)
+// The kept hits of the chosen slots (root == slot root) go to the final list in
+// their sorted order: flags, an exclusive scan, then the copy (no atomics, so the
+// final list -- and the response records made from it -- has one fixed order).
+__global__ void k_prp_keep_flags(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ slot_root,
+	uint32_t* __restrict__ flag)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool	   keep = i < n && h[i].root == slot_root[h[i].slot];
-	const uint32_t s = wave_append(count, keep);
-	if (keep && s < cap)
+	if (i < n)
+		flag[i] = h[i].root == slot_root[h[i].slot] ? 1u : 0u;
+}
+
+__global__ void k_prp_keep(uint32_t n, const PrpHit* __restrict__ h, const uint32_t* __restrict__ slot_root,
+	const PrpSlot* __restrict__ slots, uint32_t c0, const uint32_t* __restrict__ pos, uint32_t base,
+	PrpHit* __restrict__ out, uint32_t* __restrict__ out_c, uint32_t cap)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && h[i].root == slot_root[h[i].slot] && base + pos[i] < cap)
 	{
-		out[s] = h[i];
-		out_c[s] = c0 + slots[h[i].slot].contour;
+		out[base + pos[i]] = h[i];
+		out_c[base + pos[i]] = c0 + slots[h[i].slot].contour;
 	}
 }
 
@@ -2599,12 +2618,17 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 		}
 		if (nh)
 		{
-			k_prp_fill_u32<<<1, 64, 0, s>>>(1, counters.get() + 2, n_final);
+			keep_pos.reserve(nh);
+			scan_tmp.reserve(scan_tmp_size(nh));
+			k_prp_keep_flags<<<div_up(nh, 256), 256, 0, s>>>(nh, hits_sorted.get(), slot_root.get(), keep_pos.get());
+			exclusive_scan(keep_pos.get(), keep_pos.get(), nh, scan_tmp.get(), counters.get() + 2, s);
 			k_prp_keep<<<div_up(nh, 256), 256, 0, s>>>(nh, hits_sorted.get(), slot_root.get(), slots.get(), c0,
-				final_hits.get(), final_contour.get(), counters.get() + 2, final_cap);
+				keep_pos.get(), n_final, final_hits.get(), final_contour.get(), final_cap);
 			UT_LAUNCH_CHECK();
-			UT_HIP_CHECK(hipMemcpyAsync(&n_final, counters.get() + 2, 4, hipMemcpyDeviceToHost, s));
+			uint32_t kept = 0;
+			UT_HIP_CHECK(hipMemcpyAsync(&kept, counters.get() + 2, 4, hipMemcpyDeviceToHost, s));
 			UT_HIP_CHECK(hipStreamSynchronize(s));
+			n_final += kept;
 		}
 		ms_final += std::chrono::duration<double, std::milli>(clk::now() - tl1).count();
 	}
@@ -2658,10 +2682,14 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 		upload(suppressed, sup, s);
 		responses.reserve(n_final);
 		response_meta.reserve(n_final);
-		k_prp_fill_u32<<<1, 64, 0, s>>>(1, counters.get() + 3, 0u);
+		keep_pos.reserve(n_final);
+		scan_tmp.reserve(scan_tmp_size(n_final));
+		k_prp_response_flags<<<div_up(n_final, 256), 256, 0, s>>>(n_final, final_contour.get(), suppressed.get(),
+			keep_pos.get());
+		exclusive_scan(keep_pos.get(), keep_pos.get(), n_final, scan_tmp.get(), counters.get() + 3, s);
 		k_prp_responses<<<div_up(n_final, 256), 256, 0, s>>>(n_final, final_hits.get(), final_contour.get(),
 			contour_dir.get(), contour_c.get(), suppressed.get(), edges, faces, va, ea, fa, params.kappa,
-			responses.get(), response_meta.get(), counters.get() + 3, n_final);
+			keep_pos.get(), responses.get(), response_meta.get(), n_final);
 		UT_LAUNCH_CHECK();
 	}
 	else

@@ -139,6 +139,7 @@ struct PrpResolver
 	uint64_t		 ray_count[3] = { 0, 0, 0 }; // ray requests cast since the last resolve began
 	PrpKernelProfile ee_prof;
 	hipEvent_t		 ee_ev[2] = { nullptr, nullptr };
+	DevBuf<uint32_t>		   keep_pos, scan_tmp; // ranks of kept hits / live responses (flags, then scan)
 	DevBuf<unsigned long long> tstat; // [3 kinds][4] traversal counters of one evaluation, [12 + kind] wave steps
 	DevBuf<PrpSlot>	 slots;
 	DevBuf<float>	 slot_tbox;				 // [6 * slots] box of the candidate target edges (EE culling)

@@ -170,7 +170,10 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 		abd_pcg.init(ctx, d.abd.nb);
 	}
 	else
+	{
 		pcg.init(ctx, d.nv, d.nslices, d.nslots, d.slice_off, d.col, d.diag_slot);
+		build_colors();
+	}
 	if (d.face_adj_edges && d.edge_adj_faces)
 	{
 		contours.init(ctx, d.nv, d.ne, d.nf, d.edges, d.faces, d.face_adj_edges, d.edge_adj_faces);
@@ -193,6 +196,27 @@ void NewtonSolver::init(Context* c, const SolverDesc& desc, const SolverParams& 
 	UT_HIP_CHECK(hipHostMalloc((void**)&h_counts, 32, hipHostMallocDefault));
 	for (auto& e : ev)
 		UT_HIP_CHECK(hipEventCreate(&e));
+}
+
+void NewtonSolver::build_colors()
+{
+	const hipStream_t	  s = ctx->stream;
+	std::vector<uint32_t> f(3 * (size_t)d.nf), h(4 * (size_t)d.nh), fo, ho;
+	UT_HIP_CHECK(hipMemcpyAsync(f.data(), d.faces, 12 * (size_t)d.nf, hipMemcpyDeviceToHost, s));
+	if (d.nh && d.hinges)
+		UT_HIP_CHECK(hipMemcpyAsync(h.data(), d.hinges, 16 * (size_t)d.nh, hipMemcpyDeviceToHost, s));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
+	// more than 256 classes (a vertex in hundreds of elements): atomic assembly
+	colored = color_elements(f.data(), d.nf, 3, d.nv, fo, face_color_off)
+		&& color_elements(h.data(), d.hinges ? d.nh : 0u, 4, d.nv, ho, hinge_color_off);
+	if (!colored)
+		return;
+	face_order.reserve(std::max<size_t>(fo.size(), 1));
+	hinge_order.reserve(std::max<size_t>(ho.size(), 1));
+	UT_HIP_CHECK(hipMemcpyAsync(face_order.get(), fo.data(), 4 * fo.size(), hipMemcpyHostToDevice, s));
+	if (!ho.empty())
+		UT_HIP_CHECK(hipMemcpyAsync(hinge_order.get(), ho.data(), 4 * ho.size(), hipMemcpyHostToDevice, s));
+	UT_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 void NewtonSolver::build_vert_adj_edges()
@@ -333,6 +357,11 @@ void NewtonSolver::detect(StepReport& rep)
 			rep.vf_pairs = h_counts[0];
 			rep.ee_pairs = nc - h_counts[0];
 			rep.ef_pairs = nef;
+			// canonical record order: (kind, query, leaf)
+			const uint32_t mq = std::max(d.nv, d.ne), ml = std::max(d.nf, d.ne);
+			const int	   qb = std::max(1, 32 - __builtin_clz(std::max(mq, 2u) - 1));
+			const int	   lb = std::max(1, 32 - __builtin_clz(std::max(ml, 2u) - 1));
+			csort.sort(ctx, contact_pairs.get(), contact_ids.get(), nc, h_counts[0], qb, lb);
 			return;
 		}
 		ensure_contact_capacity(nc > cap_contacts ? 2 * nc : cap_contacts, nef > cap_ef ? 2 * nef : cap_ef);
@@ -478,6 +507,10 @@ StepReport NewtonSolver::physics_step_GPU()
 		else if (d.friction_mu && nfric > 0)
 		{
 			ensure_contact_capacity(ncontacts + nfric, cap_ef, ncontacts);
+			// detect() cleared the counters of this iteration: the lagged kernel reads its
+			// record count from counts[5], so it is set again here (all nfric records written)
+			h_counts[5] = nfric;
+			UT_HIP_CHECK(hipMemcpyAsync(counts.get() + 5, h_counts + 5, 4, hipMemcpyHostToDevice, s));
 			friction_lagged(ctx, fric_pairs.get(), counts.get() + 5, nfric, x.get(), nullptr, x_step_start.get(),
 				nullptr, contact_pairs.get() + ncontacts);
 			ncontacts += nfric;
@@ -566,12 +599,29 @@ StepReport NewtonSolver::physics_step_GPU()
 		a.vals = vals64.get();
 		a.vals32 = fp64 ? nullptr : vals32.get();
 		a.energy = energy.get();
-		assemble(ctx, a);
-		k_negate<<<g(n3), 256, 0, s>>>(n3, grad.get(), rhs.get());
-		UT_HIP_CHECK(hipEventRecord(ev[4], s));
+		if (colored)
+		{
+			a.face_order = face_order.get();
+			a.face_color_off = face_color_off.data();
+			a.face_colors = (uint32_t)face_color_off.size() - 1;
+			a.hinge_order = hinge_order.get();
+			a.hinge_color_off = hinge_color_off.data();
+			a.hinge_colors = (uint32_t)hinge_color_off.size() - 1;
+		}
+		// the PCG's per-vertex contact lists, built once and shared with the assembly
 		pcg.contacts = ncontacts ? contact_pairs.get() : nullptr;
 		pcg.contact_count = counts.get() + 3;
 		pcg.contact_cap = cap_contacts;
+		pcg.prepare_contacts();
+		pcg.contacts_prepared = true;
+		if (pcg.ct_entries && pcg.ct_sorted)
+		{
+			a.ct_off = pcg.ct_off.get();
+			a.ct_entries = pcg.ct_sorted;
+		}
+		assemble(ctx, a);
+		k_negate<<<g(n3), 256, 0, s>>>(n3, grad.get(), rhs.get());
+		UT_HIP_CHECK(hipEventRecord(ev[4], s));
 		const PcgReport pr =
 			pcg.solve(p.pcg, fp64 ? (const void*)vals64.get() : (const void*)vals32.get(), rhs.get(), dx.get(),
 				p.pcg.precision == kPcgMixed ? vals64.get() : nullptr);

@@ -125,6 +125,12 @@ struct NewtonSolver
 	bool				abd_mode() const { return d.abd.nb > 0; }
 	uint32_t			nfric = 0;
 	DevBuf<uint32_t>	vert_adj_off, vert_adj_edges; // edges around each vertex (ascending)
+	// deterministic assembly: faces and hinges by colour class (no shared vertex
+	// within a class), and the canonical order of the proximity records
+	DevBuf<uint32_t>	  face_order, hinge_order;
+	std::vector<uint32_t> face_color_off, hinge_color_off;
+	bool				  colored = false;
+	ContactSorter		  csort;
 	uint32_t*			h_counts = nullptr;
 	hipEvent_t			ev[8] = {};
 
@@ -150,6 +156,7 @@ private:
 	void detect(StepReport& rep);
 	void ccd_line_search();
 	void build_vert_adj_edges(); // sa_vert_adj_edges_csr (init_mesh_data.cpp), once at set-up
+	void build_colors();		 // colour classes of faces and hinges, once at set-up
 	void ensure_contact_capacity(uint32_t need_contacts, uint32_t need_ef, uint32_t keep = 0);
 };
 
