@@ -32,6 +32,11 @@ constexpr int	   kBfsThreads = 1024;
 // VGPRs, so 6 waves per SIMD), each lane with its own spill stack.  A 16-entry
 // LDS stack keeps 6 workgroups within the 160 KB of LDS per CU.
 constexpr int	   kRayBlocks = 1536;
+// the EE sweep runs at 5 waves per SIMD (its leaf tests keep the request's constants in
+// registers): one resident workgroup of 4 waves per 5 / 4 SIMD slots of each of 256 CUs
+constexpr int	   kEeWavesPerSimd = 5;
+constexpr int	   kEeBlocks = 256 * kEeWavesPerSimd;
+static_assert(kEeBlocks <= kRayBlocks && kEeBlocks % kNumXcd == 0, "the EE grid uses the ray spill stacks");
 constexpr int	   kRayStackLds = 16;
 static_assert(kRayBlocks % kNumXcd == 0, "the ray grid is split evenly over the XCDs");
 constexpr uint32_t kMaxDirs = 64;
@@ -616,18 +621,17 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 constexpr float kEeBeamMargin = 1e-4f;
 // rounding margin of the EE sweep's node culls (FP32 dot products of ~1 m coordinates)
 constexpr float kEeNodeMargin = 1e-4f;
-__device__ __forceinline__ bool ee_in_beam(f3 pn, f3 dir, f3 p0, f3 p1, f3 q0, f3 q1)
+// The beam condition with w = normalize(pn x dir) given (each request computes it
+// once); w = 0 stands for |pn x dir| < 1e-10, where the condition keeps the edge.
+__device__ __forceinline__ bool ee_in_beam(f3 wn, f3 p0, f3 p1, f3 q0, f3 q1)
 {
-	f3			wn = cross3(pn, dir);
-	const float wl = sqrtf(dot3(wn, wn));
-	if (wl < 1e-10f)
+	if (wn.x == 0.0f && wn.y == 0.0f && wn.z == 0.0f)
 		return true;
-	wn = wn * (1.0f / wl);
 	const float w0 = dot3(wn, p0), w1 = dot3(wn, p1), u0 = dot3(wn, q0), u1 = dot3(wn, q1);
 	return fmaxf(u0, u1) >= fminf(w0, w1) - kEeBeamMargin && fminf(u0, u1) <= fmaxf(w0, w1) + kEeBeamMargin;
 }
 
-__global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
+__global__ __launch_bounds__(kTravThreads, kEeWavesPerSimd) void k_prp_ray_edge(RayArgs a)
 {
 	__shared__ TravSharedT<kRayStackLds> sh;
 	const uint32_t		  total = a.off[1 * (a.nslots + 1) + a.nslots];
@@ -640,19 +644,21 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 	{
 		const uint32_t r = base + threadIdx.x;
 		bool		   active = r < total;
-		// per-request node-cull constants (see the test below)
-		box3  ib = box_empty();
-		f3	  pn = mk3(0.0f, 0.0f, 0.0f), wn = mk3(0.0f, 0.0f, 0.0f);
-		float slab_lo = -1e30f, slab_hi = 1e30f, beam_lo = -1e30f, beam_hi = 1e30f;
+		// per-request node-cull constants (see the test below); slot, e, rmax, pn, pd and
+		// wn also serve the leaf tests of this request, which read them from this lane
+		box3	 ib = box_empty();
+		f3		 pn = mk3(0.0f, 0.0f, 0.0f), wn = mk3(0.0f, 0.0f, 0.0f);
+		float	 slab_lo = -1e30f, slab_hi = 1e30f, beam_lo = -1e30f, beam_hi = 1e30f;
+		uint32_t slot = 0, e = 0;
+		float	 rmax = 1.0f, pd = 0.0f;
 		if (active)
 		{
-			const uint32_t slot = slot_of(off, a.nslots, r);
+			slot = slot_of(off, a.nslots, r);
 			const PrpSlot& sl = a.slots[slot];
-			const uint32_t e = a.cand[(size_t)(2 * sl.contour) * (a.V + a.E) + a.V + (r - off[slot])];
+			e = a.cand[(size_t)(2 * sl.contour) * (a.V + a.E) + a.V + (r - off[slot])];
 			const f3	   dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
 			const f3	   p0 = ld3(a.x, a.edges[2 * e]), p1 = ld3(a.x, a.edges[2 * e + 1]);
 			const float	   dl2 = dot3(dir, dir);
-			float		   rmax = 1.0f;
 			if (dl2 > 1e-12f)
 			{
 				const float smin = fminf(dot3(p0, dir), dot3(p1, dir));
@@ -675,7 +681,7 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 			if (pl >= 1e-10f)
 			{
 				pn = pn * (1.0f / pl);
-				const float pd = dot3(pn, p0);
+				pd = dot3(pn, p0);
 				slab_lo = pd - (1e-6f + kEeNodeMargin);
 				slab_hi = pd + (1e-6f + kEeNodeMargin);
 				// in-plane axis across the sweep: w . dir = 0, so the swept parallelogram
@@ -722,50 +728,43 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 		};
 		auto narrow = [&](bool valid, uint32_t rr, uint32_t g)
 		{
-			bool   ok = false;
-			PrpHit h{};
+			// The staged candidate belongs to request rr, made by lane rr - base of this
+			// wave (all 64 lanes are here): its slot, source edge, sweep length, plane
+			// (pn, pd) and beam axis wn come from that lane, bit-identical to a
+			// recomputation from the slot and the edge (same expressions, same inputs).
+			const uint32_t src = (rr - base) & (kWave - 1);
+			const uint32_t qs = __shfl(slot, src, kWave), qe = __shfl(e, src, kWave);
+			const float	   q_rmax = __shfl(rmax, src, kWave), q_pd = __shfl(pd, src, kWave);
+			const f3	   q_pn = mk3(__shfl(pn.x, src, kWave), __shfl(pn.y, src, kWave), __shfl(pn.z, src, kWave));
+			const f3	   q_wn = mk3(__shfl(wn.x, src, kWave), __shfl(wn.y, src, kWave), __shfl(wn.z, src, kWave));
+			bool		   ok = false;
+			PrpHit		   h{};
 			if (valid)
 			{
-				const uint32_t	slot = slot_of(off, a.nslots, rr);
-				const PrpSlot&	sl = a.slots[slot];
+				const PrpSlot&	sl = a.slots[qs];
 				const uint32_t	s1 = 2 * sl.contour + 1;
-				const uint32_t	e = a.cand[(size_t)(2 * sl.contour) * (a.V + a.E) + a.V + (rr - off[slot])];
 				const bool		is_cand = (a.ecand[(size_t)s1 * a.ewords + (g >> 6)] >> (g & 63)) & 1ull;
 				uint32_t		a0 = 0, a1 = 0, b0 = 1, b1 = 1;
 				if (is_cand)
 				{
-					a0 = a.edges[2 * e], a1 = a.edges[2 * e + 1];
+					a0 = a.edges[2 * qe], a1 = a.edges[2 * qe + 1];
 					b0 = a.edges[2 * g], b1 = a.edges[2 * g + 1];
 				}
 				const bool cand = is_cand && a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1;
 				cand_acc += (unsigned long long)__popcll(__ballot(cand)); // lane 0 is valid whenever narrow runs
 				if (cand)
 				{
-					const f3	dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
-					const f3	p0 = ld3(a.x, a0), p1 = ld3(a.x, a1), e1 = p1 - p0;
-					const float dl2 = dot3(dir, dir);
-					float		rmax = 1.0f;
-					if (dl2 > 1e-12f)
-					{
-						const float smin = fminf(dot3(p0, dir), dot3(p1, dir));
-						rmax = fminf(fmaxf((sl.ee_pmax - smin) / dl2 + 1e-5f, 0.0f), 1.0f);
-					}
-					f3			pn = cross3(e1, dir);
-					const float pl = sqrtf(dot3(pn, pn));
-					const bool	use_plane = pl >= 1e-10f;
-					float		pd = 0.0f;
-					if (use_plane)
-					{
-						pn = pn * (1.0f / pl);
-						pd = dot3(pn, p0);
-					}
+					const f3 dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
+					const f3 p0 = ld3(a.x, a0), p1 = ld3(a.x, a1), e1 = p1 - p0;
 					const f3 q0 = ld3(a.x, b0), q1 = ld3(a.x, b1);
-					bool	 keep = true;
+					// pn = 0 when |e1 x dir| < 1e-10 (no plane prune)
+					const bool use_plane = q_pn.x != 0.0f || q_pn.y != 0.0f || q_pn.z != 0.0f;
+					bool	   keep = true;
 					if (use_plane)
 					{
-						const float d0 = dot3(pn, q0) - pd, d1 = dot3(pn, q1) - pd;
+						const float d0 = dot3(q_pn, q0) - q_pd, d1 = dot3(q_pn, q1) - q_pd;
 						keep = !((d0 > 1e-6f && d1 > 1e-6f) || (d0 < -1e-6f && d1 < -1e-6f))
-							&& ee_in_beam(pn, dir, p0, p1, q0, q1);
+							&& ee_in_beam(q_wn, p0, p1, q0, q1);
 					}
 					if (keep)
 					{
@@ -777,10 +776,10 @@ __global__ __launch_bounds__(kTravThreads, 6) void k_prp_ray_edge(RayArgs a)
 							const float s_ = -dot3(rv, cross3(e1, e2)) * inv;
 							const float t1 = -dot3(dir, cross3(rv, e2)) * inv;
 							const float t2 = dot3(dir, cross3(e1, rv)) * inv;
-							if (s_ >= 0.0f && s_ <= rmax && t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f)
+							if (s_ >= 0.0f && s_ <= q_rmax && t1 >= 0.0f && t1 <= 1.0f && t2 >= 0.0f && t2 <= 1.0f)
 							{
 								ok = true;
-								h.slot = slot, h.type = 1u, h.src = e, h.dst = g;
+								h.slot = qs, h.type = 1u, h.src = qe, h.dst = g;
 								h.b0 = 1.0f - t1, h.b1 = 1.0f - t2, h.dist = s_, h.root = 0;
 							}
 						}
@@ -2305,7 +2304,7 @@ uint32_t PrpResolver::eval_slots(const std::vector<PrpSlot>& sl, Lbvh& face_tree
 		k_prp_ray_vert<2><<<kRayBlocks, kTravThreads, 0, s>>>(ra);
 		ra.nodes = edge_tree.nodes.get();
 		UT_HIP_CHECK(hipEventRecord(ee_ev[0], s));
-		k_prp_ray_edge<<<kRayBlocks, kTravThreads, 0, s>>>(ra);
+		k_prp_ray_edge<<<kEeBlocks, kTravThreads, 0, s>>>(ra);
 		UT_HIP_CHECK(hipEventRecord(ee_ev[1], s));
 		UT_LAUNCH_CHECK();
 		uint32_t		   st[2], tot[3];
