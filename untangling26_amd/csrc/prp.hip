@@ -87,37 +87,123 @@ __global__ void k_prp_edge_vert_normals(uint32_t ne, uint32_t nv, const uint32_t
 }
 
 // ------------------------------------------------------------------ distances (:618-720)
-// One workgroup per (contour, role): BFS hop distance, then Bellman-Ford on
-// the rest edge lengths with a work list (same fixpoint as Dijkstra's FP32
-// relaxations: every final value is min over neighbours of d[u] + |u - w|).
+// Neighbour blocks: for every vertex its first eight neighbours with their rest
+// edge lengths, (neighbour, length bits) pairs padded with (kNone, 0), 64 bytes
+// per vertex so one vertex's block is a few 16-byte loads.  Lengths use the
+// expression of the relaxation itself, so the sums round identically.
+constexpr int kAdjWays = 8;
+
+__device__ __forceinline__ float rest_len(const float* __restrict__ rest_x, f3 xu, uint32_t w)
+{
+	const f3 d = xu - ld3(rest_x, w);
+	return sqrtf(dot3(d, d));
+}
+
+__global__ void k_prp_adj_blocks(uint32_t V, const uint32_t* __restrict__ vv_off, const uint32_t* __restrict__ vv,
+	const float* __restrict__ rest_x, uint2* __restrict__ adj)
+{
+	const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+	if (u >= V)
+		return;
+	const uint32_t k0 = vv_off[u], deg = vv_off[u + 1] - k0;
+	const f3	   xu = ld3(rest_x, u);
+	for (uint32_t q = 0; q < kAdjWays; ++q)
+	{
+		uint2 e = make_uint2(kNone, 0u);
+		if (q < deg)
+		{
+			const uint32_t w = vv[k0 + q];
+			e = make_uint2(w, __float_as_uint(rest_len(rest_x, xu, w)));
+		}
+		adj[(size_t)kAdjWays * u + q] = e;
+	}
+}
+
+// One workgroup per (contour, role): the BFS hop distance and the Bellman-Ford
+// geodesic distance on the rest edge lengths, both as label-correcting
+// relaxations over one work list.  Each fixpoint is order independent: the
+// hop count is the minimum over paths of the edge count, and the geodesic the
+// minimum over paths of the path's left-to-right rounded FP32 length (what
+// Dijkstra's FP32 relaxations give), since rounding is monotone.  A vertex is
+// listed for the next round when either value improves, once per round: the
+// "listed" marks are a bitmap in LDS per list where the mesh allows (two
+// alternating lists; a list's bits are cleared as its entries are taken, so
+// it is clean when it is filled again), else generation-stamped words in
+// global memory.  The first entries of each list sit in LDS, the rest in
+// global memory.  The (hop, geodesic) pair of a vertex is one 8-byte word of
+// the workgroup's scratch, unpacked into the hop and geodesic rows at the end.
+constexpr uint32_t kDistListLds = 2048;	  // list entries per round held in LDS
+constexpr uint32_t kDistLdsWords = 16384; // 64 KB of LDS per workgroup: two lists + two bitmaps
+
+uint32_t dist_bitmap_words(uint32_t V)
+{
+	const uint32_t w = (V + 31) / 32;
+	return 2 * kDistListLds + 2 * w <= kDistLdsWords ? w : 0u;
+}
+
 __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint32_t c0,
 	const uint32_t* __restrict__ b_off, const uint32_t* __restrict__ b_verts, const uint32_t* __restrict__ vv_off,
-	const uint32_t* __restrict__ vv, const float* __restrict__ rest_x, uint32_t* __restrict__ hop_all,
-	float* __restrict__ geo_all, uint32_t* __restrict__ queue_all, uint32_t* __restrict__ inq_all,
-	uint32_t* __restrict__ diameter)
+	const uint32_t* __restrict__ vv, const float* __restrict__ rest_x, const uint2* __restrict__ adj,
+	uint32_t* __restrict__ hop_all, float* __restrict__ geo_all, uint32_t* __restrict__ queue_all,
+	uint32_t* __restrict__ inq_all, uint32_t* __restrict__ diameter, uint32_t bitmap_words)
 {
+	extern __shared__ uint32_t dsm[];
 	const uint32_t s = blockIdx.x;
 	const uint32_t bs = 2 * c0 + s; // global (contour, role) slot
-	uint32_t*	   H = hop_all + (size_t)s * V;
-	float*		   G = geo_all + (size_t)s * V;
-	uint32_t*	   Q = queue_all + (size_t)s * 2 * V;
+	uint32_t*	   Q = queue_all + (size_t)s * 4 * V; // list overflow [2][V], then the (hop, geodesic) words [2V]
+	uint2*		   HG = reinterpret_cast<uint2*>(Q + 2 * (size_t)V);
+	uint32_t*	   hg = Q + 2 * (size_t)V; // the same words as 32-bit halves: hop 2v, geodesic 2v + 1
 	uint32_t*	   I = inq_all + (size_t)s * V;
-	__shared__ uint32_t nnext;
+	const bool	   bitmap = bitmap_words > 0;
+	__shared__ uint32_t nnext, dmax;
 	for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
-		H[v] = kHopInf, G[v] = FLT_MAX, I[v] = 0u;
+	{
+		HG[v] = make_uint2(kHopInf, __float_as_uint(FLT_MAX));
+		if (!bitmap)
+			I[v] = 0u;
+	}
+	for (uint32_t k = threadIdx.x; k < 2 * bitmap_words; k += blockDim.x)
+		dsm[2 * kDistListLds + k] = 0u;
+	if (threadIdx.x == 0)
+		dmax = 0;
 	__syncthreads();
+	// entry i of list l
+	auto entry = [&](uint32_t l, uint32_t i) -> uint32_t&
+	{ return i < kDistListLds ? dsm[l * kDistListLds + i] : Q[(size_t)l * V + (i - kDistListLds)]; };
 	const uint32_t b0 = b_off[bs], nb = b_off[bs + 1] - b0;
 	for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
 	{
 		const uint32_t v = b_verts[b0 + i];
-		H[v] = 0u;
-		G[v] = 0.0f;
-		Q[i] = v;
+		HG[v] = make_uint2(0u, 0u);
+		entry(0, i) = v;
 	}
 	__syncthreads();
-	// BFS, level synchronous inside the workgroup
-	uint32_t  ncur = nb, level = 0;
-	uint32_t *cur = Q, *nxt = Q + V;
+	uint32_t ncur = nb, gen = 1, cl = 0; // entries, round stamp, list being taken
+	// relaxation of the edge u -> w (w != kNone) with rest length bits lb; cw is a read of w's word
+	auto relax_edge = [&](uint32_t hu, float gu, uint32_t w, uint32_t lb, uint2 cw)
+	{
+		const uint32_t nh = hu == kHopInf ? kHopInf : hu + 1;
+		const uint32_t ng = __float_as_uint(gu + __uint_as_float(lb));
+		// values only decrease, so a read that is not above the candidate (stale or
+		// not) proves the atomic would change nothing
+		bool better = false;
+		if (nh < cw.x)
+			better = nh < atomicMin(&hg[2 * w], nh);
+		if (ng < cw.y)
+			better = (ng < atomicMin(&hg[2 * w + 1], ng)) || better;
+		if (!better)
+			return;
+		bool add;
+		if (bitmap)
+		{
+			const uint32_t bit = 1u << (w & 31);
+			add = !(atomicOr(&dsm[2 * kDistListLds + (cl ^ 1) * bitmap_words + (w >> 5)], bit) & bit);
+		}
+		else
+			add = atomicExch(&I[w], gen) != gen;
+		if (add)
+			entry(cl ^ 1, atomicAdd(&nnext, 1u)) = w;
+	};
 	while (ncur > 0)
 	{
 		if (threadIdx.x == 0)
@@ -125,68 +211,64 @@ __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint3
 		__syncthreads();
 		for (uint32_t i = threadIdx.x; i < ncur; i += blockDim.x)
 		{
-			const uint32_t u = cur[i];
-			for (uint32_t k = vv_off[u]; k < vv_off[u + 1]; ++k)
+			const uint32_t u = entry(cl, i);
+			if (bitmap) // u leaves the list: its mark is cleared for the list's next filling
+				atomicAnd(&dsm[2 * kDistListLds + cl * bitmap_words + (u >> 5)], ~(1u << (u & 31)));
+			const uint64_t wu = __hip_atomic_load(
+				reinterpret_cast<const uint64_t*>(HG) + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+			const uint32_t hu = (uint32_t)wu;
+			const float	   gu = __uint_as_float((uint32_t)(wu >> 32));
+			const uint4*   blk = reinterpret_cast<const uint4*>(adj + (size_t)kAdjWays * u);
+			uint32_t	   w[kAdjWays], lb[kAdjWays];
+#pragma unroll
+			for (int q = 0; q < kAdjWays / 2; ++q)
 			{
-				const uint32_t w = vv[k];
-				// a plain read skips the atomic for vertices already reached (a stale
-				// read can only show kHopInf, which costs the atomic and nothing else)
-				if (H[w] == kHopInf && atomicCAS(&H[w], kHopInf, level + 1) == kHopInf)
-					nxt[atomicAdd(&nnext, 1u)] = w;
+				const uint4 b = blk[q];
+				w[2 * q] = b.x, lb[2 * q] = b.y, w[2 * q + 1] = b.z, lb[2 * q + 1] = b.w;
+			}
+			uint2 cw[kAdjWays];
+#pragma unroll
+			for (int q = 0; q < kAdjWays; ++q)
+				cw[q] = w[q] != kNone ? HG[w[q]] : make_uint2(0u, 0u);
+#pragma unroll
+			for (int q = 0; q < kAdjWays; ++q)
+				if (w[q] != kNone)
+					relax_edge(hu, gu, w[q], lb[q], cw[q]);
+			// neighbours beyond the block (degree above eight)
+			const uint32_t k1 = vv_off[u + 1];
+			if (k1 - vv_off[u] > kAdjWays)
+			{
+				const f3 xu = ld3(rest_x, u);
+				for (uint32_t k = vv_off[u] + kAdjWays; k < k1; ++k)
+				{
+					const uint32_t x = vv[k];
+					relax_edge(hu, gu, x, __float_as_uint(rest_len(rest_x, xu, x)), HG[x]);
+				}
 			}
 		}
 		__syncthreads();
 		ncur = nnext;
-		uint32_t* t = cur;
-		cur = nxt;
-		nxt = t;
-		++level;
-		__syncthreads();
-	}
-	if (threadIdx.x == 0)
-		diameter[s] = level > 0 ? level - 1 : 0;
-	// Bellman-Ford from the same sources
-	for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
-		Q[i] = b_verts[b0 + i];
-	__syncthreads();
-	ncur = nb;
-	cur = Q, nxt = Q + V;
-	uint32_t gen = 1;
-	while (ncur > 0)
-	{
-		if (threadIdx.x == 0)
-			nnext = 0;
-		__syncthreads();
-		for (uint32_t i = threadIdx.x; i < ncur; i += blockDim.x)
-		{
-			const uint32_t u = cur[i];
-			const float	   du = __uint_as_float(__hip_atomic_load(
-				  reinterpret_cast<uint32_t*>(&G[u]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
-			const f3 xu = ld3(rest_x, u);
-			for (uint32_t k = vv_off[u]; k < vv_off[u + 1]; ++k)
-			{
-				const uint32_t w = vv[k];
-				const f3	   d = xu - ld3(rest_x, w);
-				const float	   nd = du + sqrtf(dot3(d, d));
-				const uint32_t bits = __float_as_uint(nd);
-				uint32_t* const gw = reinterpret_cast<uint32_t*>(&G[w]);
-				// values only decrease, so a plain read that is not above nd (stale or
-				// not) proves the atomic would change nothing
-				if (bits >= *gw)
-					continue;
-				const uint32_t old = atomicMin(gw, bits);
-				if (bits < old && atomicExch(&I[w], gen) != gen)
-					nxt[atomicAdd(&nnext, 1u)] = w;
-			}
-		}
-		__syncthreads();
-		ncur = nnext;
-		uint32_t* t = cur;
-		cur = nxt;
-		nxt = t;
+		cl ^= 1;
 		++gen;
 		__syncthreads();
 	}
+	// unpack into the hop and geodesic rows; the diameter is the largest finite hop
+	uint32_t* H = hop_all + (size_t)s * V;
+	float*	  G = geo_all + (size_t)s * V;
+	uint32_t  m = 0;
+	for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
+	{
+		const uint64_t e = __hip_atomic_load(
+			reinterpret_cast<const uint64_t*>(HG) + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		const uint32_t h = (uint32_t)e;
+		H[v] = h, G[v] = __uint_as_float((uint32_t)(e >> 32));
+		if (h != kHopInf)
+			m = max(m, h);
+	}
+	atomicMax(&dmax, m);
+	__syncthreads();
+	if (threadIdx.x == 0)
+		diameter[s] = dmax;
 }
 
 // ------------------------------------------------------------------ ownership + candidates (:1571-1804)
@@ -856,11 +938,15 @@ __device__ __forceinline__ uint32_t uf_find(uint32_t* p, uint32_t x)
 // up; the link is a device-scope CAS, and a failed CAS returns the element's
 // current parent, from which the walk resumes.  Each failure replaces the
 // larger of the two candidates by a smaller index, so the loop ends.
+__device__ __forceinline__ uint32_t uf_load_cached(const uint32_t* p, uint32_t x)
+{
+	return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 __device__ __forceinline__ uint32_t uf_find_cached(const uint32_t* p, uint32_t x)
 {
 	for (;;)
 	{
-		const uint32_t q = __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+		const uint32_t q = uf_load_cached(p, x);
 		if (q == x)
 			return x;
 		x = q;
@@ -940,6 +1026,42 @@ __device__ __forceinline__ bool edges_adjacent(const HitArgs& a, uint32_t e1, ui
 			return true;
 	}
 	return false;
+}
+
+// Unions hit i's set (through ri, an ancestor of i) with the sets of the hits
+// j[q] (kNone entries are skipped) and returns the shared root.  The parent
+// chains of the j are walked side by side first, so their loads overlap.
+template <int N>
+__device__ __forceinline__ uint32_t uf_union_n(uint32_t* p, uint32_t i, uint32_t ri, const uint32_t (&j)[N])
+{
+	uint32_t r[N], live = 0;
+#pragma unroll
+	for (int q = 0; q < N; ++q)
+	{
+		r[q] = j[q];
+		if (j[q] != kNone && j[q] != i)
+			live |= 1u << q;
+	}
+	const uint32_t linked = live;
+	while (live)
+	{
+		uint32_t up[N];
+#pragma unroll
+		for (int q = 0; q < N; ++q)
+			up[q] = (live >> q) & 1u ? uf_load_cached(p, r[q]) : r[q];
+#pragma unroll
+		for (int q = 0; q < N; ++q)
+		{
+			if (up[q] == r[q])
+				live &= ~(1u << q);
+			r[q] = up[q];
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < N; ++q)
+		if ((linked >> q) & 1u)
+			ri = uf_union_cached(p, ri, r[q]);
+	return ri;
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t k)
@@ -1062,7 +1184,8 @@ __global__ void k_prp_hit_union(HitArgs a)
 				ri = uf_union_cached(a.parent, ri, j);
 		}
 	};
-	// the same for up to four targets held in registers (kNone entries, last, are skipped)
+	// the same for up to four targets held in registers (kNone entries, last, are skipped);
+	// the unions follow the four searches, so their parent walks overlap
 	auto link4 = [&](uint32_t type, uint32_t prim, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3)
 	{
 		const uint32_t key = hit_key(s, type, prim, a.prim_bits);
@@ -1070,14 +1193,12 @@ __global__ void k_prp_hit_union(HitArgs a)
 		if (lo == kNone)
 			return;
 		const uint32_t d[4] = { d0, d1, d2, d3 };
+		uint32_t	   j[4] = { kNone, kNone, kNone, kNone };
 #pragma unroll
 		for (int q = 0; q < 4; ++q)
 			if (d[q] != kNone)
-			{
-				const uint32_t j = gallop_dst(a, key, lo, hi, d[q], &lo);
-				if (j != kNone && j != i)
-					ri = uf_union_cached(a.parent, ri, j);
-			}
+				j[q] = gallop_dst(a, key, lo, hi, d[q], &lo);
+		ri = uf_union_n(a.parent, i, ri, j);
 	};
 	auto cswap = [](uint32_t& x, uint32_t& y)
 	{
@@ -1137,19 +1258,28 @@ __global__ void k_prp_hit_union(HitArgs a)
 		}
 		cswap(t[0], t[1]), cswap(t[2], t[3]), cswap(t[0], t[2]), cswap(t[1], t[3]), cswap(t[1], t[2]);
 		link4(1, e1, t[0], t[1], t[2], t[3]);
-		// EE(e', e2), e' another edge of a face of e1 (e' > e1)
+		// EE(e', e2), e' another edge of a face of e1 (e' > e1); one union batch
+		uint32_t j2[4] = { kNone, kNone, kNone, kNone };
+#pragma unroll
 		for (int k = 0; k < 2; ++k)
 		{
 			const uint32_t f = a.eaf[2 * e1 + k];
 			if (f == kNone)
 				continue;
-			for (int j = 0; j < 3; ++j)
-			{
-				const uint32_t ae = a.fae[3 * f + j];
-				if (ae > e1)
-					link4(1, ae, e2, kNone, kNone, kNone);
-			}
+			const uint32_t x0 = a.fae[3 * f], x1 = a.fae[3 * f + 1], x2 = a.fae[3 * f + 2];
+			// the two edges of f other than e1
+			const uint32_t o[2] = { x0 == e1 ? x1 : x0, (x0 == e1 || x1 == e1) ? x2 : x1 };
+#pragma unroll
+			for (int m = 0; m < 2; ++m)
+				if (o[m] > e1)
+				{
+					const uint32_t key = hit_key(s, 1, o[m], a.prim_bits);
+					uint32_t	   lo = bucket_start(a, s, key);
+					if (lo != kNone)
+						j2[2 * k + m] = gallop_dst(a, key, lo, hi, e2, &lo);
+				}
 		}
+		ri = uf_union_n(a.parent, i, ri, j2);
 	}
 }
 
@@ -2171,12 +2301,17 @@ void PrpResolver::distances(uint32_t c0, uint32_t n)
 	const size_t	  nn = std::max<size_t>(n, params.batch_contours);
 	hop.reserve(2 * nn * V);
 	geo.reserve(2 * nn * V);
-	queue.reserve(4 * nn * V);
+	queue.reserve(8 * nn * V);
 	inq.reserve(2 * nn * V);
 	own.reserve(2 * (size_t)params.batch_contours * V);
 	diameter.reserve(2 * nn);
-	k_prp_distances<<<2 * n, kBfsThreads, 0, s>>>(nv, c0, b_off.get(), b_verts.get(), vv_off.get(), vv.get(), rest_x,
-		hop.get(), geo.get(), queue.get(), inq.get(), diameter.get());
+	adj_blocks.reserve((size_t)2 * kAdjWays * V);
+	uint2* adj = reinterpret_cast<uint2*>(adj_blocks.get());
+	k_prp_adj_blocks<<<div_up(nv, 256), 256, 0, s>>>(nv, vv_off.get(), vv.get(), rest_x, adj);
+	const uint32_t bw = dist_bitmap_words(nv);
+	const size_t   lds = 4 * (2 * (size_t)kDistListLds + 2 * (size_t)bw);
+	k_prp_distances<<<2 * n, kBfsThreads, lds, s>>>(nv, c0, b_off.get(), b_verts.get(), vv_off.get(), vv.get(),
+		rest_x, adj, hop.get(), geo.get(), queue.get(), inq.get(), diameter.get(), bw);
 	UT_LAUNCH_CHECK();
 	h_diam.resize(2 * n);
 	UT_HIP_CHECK(hipMemcpyAsync(h_diam.data(), diameter.get(), 8 * (size_t)n, hipMemcpyDeviceToHost, s));

@@ -126,7 +126,8 @@ struct PrpResolver
 	DevBuf<uint32_t> cpair_off, cpair_rec;	 // per contour: (edge v0, v1, face v0, v1, v2, mesh idx) words
 	DevBuf<uint32_t> hop;					 // [2 * batch][V]
 	DevBuf<float>	 geo, curd;				 // [2 * batch][V]
-	DevBuf<uint32_t> queue, inq;			 // BFS / Bellman-Ford work lists [2 * batch][2V], [2 * batch][V]
+	DevBuf<uint32_t> queue, inq;			 // distance work lists + (hop, geodesic) words [2 * batch][4V], [2 * batch][V]
+	DevBuf<uint32_t> adj_blocks;			 // [V][8] (neighbour, rest length bits) pairs
 	DevBuf<uint32_t> diameter;				 // [2 * contours]
 	// rows of the current batch in hop / geo / h_diam (distances may hold more contours)
 	const uint32_t*	 hop_b = nullptr;
