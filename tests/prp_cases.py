@@ -13,6 +13,12 @@ def prp_fixture(name="two_sheets", total_dirs=10):
     elif name == "two_sheets_s3":
         m = scenes.two_sheets(3)
         rest = m.x
+    elif name in ("deep_sheets_8", "deep_sheets_16"):
+        # deeply crossing small sheets: the chosen k-rings reach the contour's hop
+        # diameter (16 x 16: after growing past the first distance threshold;
+        # 8 x 8: within it)
+        m = scenes.two_sheets(0, n=int(name.rsplit("_", 1)[1]), depth=0.2)
+        rest = m.x
     elif name == "folded_small":
         m = scenes.folded_cloth(0, n=48)
         rest = scenes.folded_cloth_rest(n=48)

@@ -9,8 +9,12 @@ Direction counts: 10 (the reference's generator), 16 (the two-sheet
 configuration) and 64 (the folded-cloth configuration); directions beyond the
 tenth are this build's Fibonacci-sphere extension, restated by the oracle
 (their parity against the reference is unpinned: it has no such directions).
-Scenes: two crossing 32 x 32 sheets, a 48 x 48 fold (29 contours), and a batch
-of four contours of the 256 x 256 fold (130,050 triangles) at radius 8."""
+Scenes: two crossing 32 x 32 sheets, a 48 x 48 fold (29 contours), two deeply
+crossing 8 x 8 and 16 x 16 sheets whose chosen k-rings reach the hop diameter
+(the resolver's distance searches stop at thresholds and resume as rings
+grow; these cases end where the diameter is learned within the first
+threshold and during growth), and a batch of four contours of the 256 x 256
+fold (130,050 triangles) at radius 8."""
 import numpy as np
 import pytest
 import torch
@@ -59,7 +63,8 @@ def _hitset(h):
             for a in h}
 
 
-SCENES = [("two_sheets", 10), ("two_sheets", 16), ("two_sheets", 64), ("folded_small", 10), ("folded_small", 64)]
+SCENES = [("two_sheets", 10), ("two_sheets", 16), ("two_sheets", 64), ("folded_small", 10), ("folded_small", 64),
+          ("deep_sheets_8", 10), ("deep_sheets_16", 10)]
 
 
 @pytest.fixture(scope="module", params=SCENES, ids=[f"{n}-{d}dirs" for n, d in SCENES])

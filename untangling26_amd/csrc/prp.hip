@@ -119,21 +119,40 @@ __global__ void k_prp_adj_blocks(uint32_t V, const uint32_t* __restrict__ vv_off
 	}
 }
 
-// One workgroup per (contour, role): the BFS hop distance and the Bellman-Ford
-// geodesic distance on the rest edge lengths, both as label-correcting
-// relaxations over one work list.  Each fixpoint is order independent: the
-// hop count is the minimum over paths of the edge count, and the geodesic the
-// minimum over paths of the path's left-to-right rounded FP32 length (what
-// Dijkstra's FP32 relaxations give), since rounding is monotone.  A vertex is
-// listed for the next round when either value improves, once per round: the
-// "listed" marks are a bitmap in LDS per list where the mesh allows (two
-// alternating lists; a list's bits are cleared as its entries are taken, so
-// it is clean when it is filled again), else generation-stamped words in
+// One workgroup per listed (contour, role) slot: the BFS hop distance and the
+// Bellman-Ford geodesic distance on the rest edge lengths, both as
+// label-correcting relaxations over one work list.  Each fixpoint is order
+// independent: the hop count is the minimum over paths of the edge count, and
+// the geodesic the minimum over paths of the path's left-to-right rounded
+// FP32 length (what Dijkstra's FP32 relaxations give), since rounding is
+// monotone.
+//
+// The search is truncated and resumable: a vertex is expanded only while its
+// hop label is <= the slot's hop threshold k or its geodesic label <= the
+// slot's radius r.  Every vertex whose true hop is <= k + 1 then has its exact
+// hop (the BFS path's vertices all expand), every vertex whose true geodesic
+// is <= r its exact geodesic (a shortest path's prefixes are no longer), and
+// every other label is an upper bound above those limits.  Raising the
+// thresholds resumes the search from the vertices that now qualify and had
+// not (the labels stay in the slot's scratch).  Mode A raises k (from a
+// request) and reports the largest finite hop label, which is the diameter
+// exactly when it is <= k (the BFS died out), and the largest geodesic label
+// on the ring hop <= k, an upper bound of the ring's true geodesics.  Mode B
+// raises r to the larger of the two roles' ring bounds, after which every
+// geodesic that the ownership test compares is exact or provably above the
+// vertex's own distance.  Infinite thresholds give the full fields.
+//
+// A vertex is listed for the next round when either label improves, once per
+// round: the "listed" marks are a bitmap in LDS per list where the mesh allows
+// (two alternating lists; a list's bits are cleared as its entries are taken,
+// so it is clean when it is filled again), else generation-stamped words in
 // global memory.  The first entries of each list sit in LDS, the rest in
-// global memory.  The (hop, geodesic) pair of a vertex is one 8-byte word of
-// the workgroup's scratch, unpacked into the hop and geodesic rows at the end.
+// global memory.  The (hop, geodesic) labels of a vertex are one 8-byte word
+// of the slot's scratch, unpacked into the hop and geodesic rows after each
+// launch.
 constexpr uint32_t kDistListLds = 2048;	  // list entries per round held in LDS
 constexpr uint32_t kDistLdsWords = 16384; // 64 KB of LDS per workgroup: two lists + two bitmaps
+constexpr uint32_t kDistHopAll = 0xFFFFFFFEu; // hop threshold that expands every reached vertex
 
 uint32_t dist_bitmap_words(uint32_t V)
 {
@@ -141,44 +160,83 @@ uint32_t dist_bitmap_words(uint32_t V)
 	return 2 * kDistListLds + 2 * w <= kDistLdsWords ? w : 0u;
 }
 
-__global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint32_t c0,
-	const uint32_t* __restrict__ b_off, const uint32_t* __restrict__ b_verts, const uint32_t* __restrict__ vv_off,
-	const uint32_t* __restrict__ vv, const float* __restrict__ rest_x, const uint2* __restrict__ adj,
-	uint32_t* __restrict__ hop_all, float* __restrict__ geo_all, uint32_t* __restrict__ queue_all,
-	uint32_t* __restrict__ inq_all, uint32_t* __restrict__ diameter, uint32_t bitmap_words)
+struct DistArgs
+{
+	uint32_t		V, c0;
+	const uint32_t* req;   // mode A: (slot, hop threshold, first launch) per workgroup; mode B: (slot, -, -)
+	uint32_t		mode_b;
+	const uint32_t *b_off, *b_verts, *vv_off, *vv;
+	const float*	rest_x;
+	const uint2*	adj;
+	uint32_t*		hop_all; // [slot][V] rows
+	float*			geo_all;
+	uint32_t*		queue_all; // [slot][4V]: list overflow [2][V], then the label words [V] (8 bytes each)
+	uint32_t*		inq_all;   // [slot][V] listed stamps (meshes beyond the LDS bitmap)
+	uint2*			state;	   // [slot] (hop threshold, radius bits) reached so far
+	uint2*			out;	   // [slot] (largest finite hop, ring geodesic bound bits), written by mode A
+	uint32_t		bitmap_words;
+};
+
+__global__ __launch_bounds__(kBfsThreads) void k_prp_distances(DistArgs a)
 {
 	extern __shared__ uint32_t dsm[];
-	const uint32_t s = blockIdx.x;
-	const uint32_t bs = 2 * c0 + s; // global (contour, role) slot
-	uint32_t*	   Q = queue_all + (size_t)s * 4 * V; // list overflow [2][V], then the (hop, geodesic) words [2V]
+	const uint32_t V = a.V;
+	const uint32_t s = a.req[3 * blockIdx.x]; // slot: 2 * local contour + role
+	const bool	   first = !a.mode_b && a.req[3 * blockIdx.x + 2] != 0;
+	const uint32_t bs = 2 * a.c0 + s;		  // global (contour, role) slot
+	uint32_t*	   Q = a.queue_all + (size_t)s * 4 * V;
 	uint2*		   HG = reinterpret_cast<uint2*>(Q + 2 * (size_t)V);
 	uint32_t*	   hg = Q + 2 * (size_t)V; // the same words as 32-bit halves: hop 2v, geodesic 2v + 1
-	uint32_t*	   I = inq_all + (size_t)s * V;
-	const bool	   bitmap = bitmap_words > 0;
-	__shared__ uint32_t nnext, dmax;
-	for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
-	{
-		HG[v] = make_uint2(kHopInf, __float_as_uint(FLT_MAX));
-		if (!bitmap)
-			I[v] = 0u;
-	}
-	for (uint32_t k = threadIdx.x; k < 2 * bitmap_words; k += blockDim.x)
-		dsm[2 * kDistListLds + k] = 0u;
+	uint32_t*	   I = a.inq_all + (size_t)s * V;
+	const bool	   bitmap = a.bitmap_words > 0;
+	// thresholds before and after this launch (a first launch has none before)
+	const uint2	   old = first ? make_uint2(0u, 0u) : a.state[s];
+	uint32_t	   kn = old.x, rn = old.y;
+	if (!a.mode_b)
+		kn = max(kn, a.req[3 * blockIdx.x + 1]);
+	else
+		rn = max(rn, max(a.out[s].y, a.out[s ^ 1u].y)); // non-negative float bits order as the values
+	auto qual_old = [&](uint32_t h, uint32_t g) { return !first && (h <= old.x || g <= old.y); };
+	auto qual_new = [&](uint32_t h, uint32_t g) { return h <= kn || g <= rn; };
+	__shared__ uint32_t nnext, n0, hmax, gmax;
 	if (threadIdx.x == 0)
-		dmax = 0;
-	__syncthreads();
+		nnext = 0, n0 = 0, hmax = 0, gmax = 0;
+	for (uint32_t k = threadIdx.x; k < 2 * a.bitmap_words; k += blockDim.x)
+		dsm[2 * kDistListLds + k] = 0u;
+	if (!bitmap)
+		for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
+			I[v] = 0u;
 	// entry i of list l
 	auto entry = [&](uint32_t l, uint32_t i) -> uint32_t&
 	{ return i < kDistListLds ? dsm[l * kDistListLds + i] : Q[(size_t)l * V + (i - kDistListLds)]; };
-	const uint32_t b0 = b_off[bs], nb = b_off[bs + 1] - b0;
-	for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+	if (first)
 	{
-		const uint32_t v = b_verts[b0 + i];
-		HG[v] = make_uint2(0u, 0u);
-		entry(0, i) = v;
+		for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
+			HG[v] = make_uint2(kHopInf, __float_as_uint(FLT_MAX));
+		__syncthreads();
+		const uint32_t b0 = a.b_off[bs], nb = a.b_off[bs + 1] - b0;
+		for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x)
+		{
+			const uint32_t v = a.b_verts[b0 + i];
+			HG[v] = make_uint2(0u, 0u);
+			entry(0, i) = v;
+		}
+		if (threadIdx.x == 0)
+			n0 = nb;
+	}
+	else
+	{
+		__syncthreads();
+		// resume from the reached vertices that qualify now and did not before
+		for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
+		{
+			const uint2 e = HG[v];
+			if (e.x != kHopInf && !qual_old(e.x, e.y) && qual_new(e.x, e.y))
+				entry(0, atomicAdd(&n0, 1u)) = v;
+		}
 	}
 	__syncthreads();
-	uint32_t ncur = nb, gen = 1, cl = 0; // entries, round stamp, list being taken
+	uint32_t ncur = n0, gen = 1, cl = 0; // entries, round stamp, list being taken
 	// relaxation of the edge u -> w (w != kNone) with rest length bits lb; cw is a read of w's word
 	auto relax_edge = [&](uint32_t hu, float gu, uint32_t w, uint32_t lb, uint2 cw)
 	{
@@ -197,7 +255,7 @@ __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint3
 		if (bitmap)
 		{
 			const uint32_t bit = 1u << (w & 31);
-			add = !(atomicOr(&dsm[2 * kDistListLds + (cl ^ 1) * bitmap_words + (w >> 5)], bit) & bit);
+			add = !(atomicOr(&dsm[2 * kDistListLds + (cl ^ 1) * a.bitmap_words + (w >> 5)], bit) & bit);
 		}
 		else
 			add = atomicExch(&I[w], gen) != gen;
@@ -213,13 +271,15 @@ __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint3
 		{
 			const uint32_t u = entry(cl, i);
 			if (bitmap) // u leaves the list: its mark is cleared for the list's next filling
-				atomicAnd(&dsm[2 * kDistListLds + cl * bitmap_words + (u >> 5)], ~(1u << (u & 31)));
+				atomicAnd(&dsm[2 * kDistListLds + cl * a.bitmap_words + (u >> 5)], ~(1u << (u & 31)));
 			const uint64_t wu = __hip_atomic_load(
 				reinterpret_cast<const uint64_t*>(HG) + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 			const uint32_t hu = (uint32_t)wu;
 			const float	   gu = __uint_as_float((uint32_t)(wu >> 32));
-			const uint4*   blk = reinterpret_cast<const uint4*>(adj + (size_t)kAdjWays * u);
-			uint32_t	   w[kAdjWays], lb[kAdjWays];
+			if (!qual_new(hu, (uint32_t)(wu >> 32)))
+				continue; // beyond both thresholds: waits for a later launch
+			const uint4* blk = reinterpret_cast<const uint4*>(a.adj + (size_t)kAdjWays * u);
+			uint32_t	 w[kAdjWays], lb[kAdjWays];
 #pragma unroll
 			for (int q = 0; q < kAdjWays / 2; ++q)
 			{
@@ -235,14 +295,14 @@ __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint3
 				if (w[q] != kNone)
 					relax_edge(hu, gu, w[q], lb[q], cw[q]);
 			// neighbours beyond the block (degree above eight)
-			const uint32_t k1 = vv_off[u + 1];
-			if (k1 - vv_off[u] > kAdjWays)
+			const uint32_t k1 = a.vv_off[u + 1];
+			if (k1 - a.vv_off[u] > kAdjWays)
 			{
-				const f3 xu = ld3(rest_x, u);
-				for (uint32_t k = vv_off[u] + kAdjWays; k < k1; ++k)
+				const f3 xu = ld3(a.rest_x, u);
+				for (uint32_t k = a.vv_off[u] + kAdjWays; k < k1; ++k)
 				{
-					const uint32_t x = vv[k];
-					relax_edge(hu, gu, x, __float_as_uint(rest_len(rest_x, xu, x)), HG[x]);
+					const uint32_t x = a.vv[k];
+					relax_edge(hu, gu, x, __float_as_uint(rest_len(a.rest_x, xu, x)), HG[x]);
 				}
 			}
 		}
@@ -252,23 +312,31 @@ __global__ __launch_bounds__(kBfsThreads) void k_prp_distances(uint32_t V, uint3
 		++gen;
 		__syncthreads();
 	}
-	// unpack into the hop and geodesic rows; the diameter is the largest finite hop
-	uint32_t* H = hop_all + (size_t)s * V;
-	float*	  G = geo_all + (size_t)s * V;
-	uint32_t  m = 0;
+	// unpack into the hop and geodesic rows: the largest finite hop label and the
+	// largest geodesic label on the ring hop <= k
+	uint32_t* H = a.hop_all + (size_t)s * V;
+	float*	  G = a.geo_all + (size_t)s * V;
+	uint32_t  mh = 0, mg = 0;
 	for (uint32_t v = threadIdx.x; v < V; v += blockDim.x)
 	{
 		const uint64_t e = __hip_atomic_load(
 			reinterpret_cast<const uint64_t*>(HG) + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		const uint32_t h = (uint32_t)e;
-		H[v] = h, G[v] = __uint_as_float((uint32_t)(e >> 32));
+		const uint32_t h = (uint32_t)e, g = (uint32_t)(e >> 32);
+		H[v] = h, G[v] = __uint_as_float(g);
 		if (h != kHopInf)
-			m = max(m, h);
+			mh = max(mh, h);
+		if (h <= kn)
+			mg = max(mg, g);
 	}
-	atomicMax(&dmax, m);
+	atomicMax(&hmax, mh);
+	atomicMax(&gmax, mg);
 	__syncthreads();
 	if (threadIdx.x == 0)
-		diameter[s] = dmax;
+	{
+		a.state[s] = make_uint2(kn, rn);
+		if (!a.mode_b)
+			a.out[s] = make_uint2(hmax, gmax);
+	}
 }
 
 // ------------------------------------------------------------------ ownership + candidates (:1571-1804)
@@ -2294,7 +2362,9 @@ void PrpResolver::directions(uint32_t c0, uint32_t n)
 	}
 }
 
-void PrpResolver::distances(uint32_t c0, uint32_t n)
+// Distance state for the (contour, role) slots of n contours: rows, label words,
+// work lists, thresholds, and the neighbour blocks (rest lengths).
+void PrpResolver::dist_reserve(uint32_t n)
 {
 	const hipStream_t s = ctx->stream;
 	const size_t	  V = nv;
@@ -2304,18 +2374,52 @@ void PrpResolver::distances(uint32_t c0, uint32_t n)
 	queue.reserve(8 * nn * V);
 	inq.reserve(2 * nn * V);
 	own.reserve(2 * (size_t)params.batch_contours * V);
-	diameter.reserve(2 * nn);
+	dist_state.reserve(4 * nn);
+	dist_out.reserve(4 * nn);
 	adj_blocks.reserve((size_t)2 * kAdjWays * V);
-	uint2* adj = reinterpret_cast<uint2*>(adj_blocks.get());
-	k_prp_adj_blocks<<<div_up(nv, 256), 256, 0, s>>>(nv, vv_off.get(), vv.get(), rest_x, adj);
+	k_prp_adj_blocks<<<div_up(nv, 256), 256, 0, s>>>(nv, vv_off.get(), vv.get(), rest_x,
+		reinterpret_cast<uint2*>(adj_blocks.get()));
+	UT_LAUNCH_CHECK();
+}
+
+// One distance launch over the listed slots of contours from c0 (triples: slot,
+// hop threshold, first launch); mode B raises the geodesic radii instead.
+void PrpResolver::dist_pass(uint32_t c0, const std::vector<uint32_t>& req, bool mode_b)
+{
+	if (req.empty())
+		return;
+	const hipStream_t s = ctx->stream;
+	upload(dist_req, req, s);
 	const uint32_t bw = dist_bitmap_words(nv);
 	const size_t   lds = 4 * (2 * (size_t)kDistListLds + 2 * (size_t)bw);
-	k_prp_distances<<<2 * n, kBfsThreads, lds, s>>>(nv, c0, b_off.get(), b_verts.get(), vv_off.get(), vv.get(),
-		rest_x, adj, hop.get(), geo.get(), queue.get(), inq.get(), diameter.get(), bw);
+	DistArgs	   da{ nv, c0, dist_req.get(), mode_b ? 1u : 0u, b_off.get(), b_verts.get(), vv_off.get(), vv.get(), rest_x,
+		  reinterpret_cast<const uint2*>(adj_blocks.get()), hop.get(), geo.get(), queue.get(), inq.get(),
+		  reinterpret_cast<uint2*>(dist_state.get()), reinterpret_cast<uint2*>(dist_out.get()), bw };
+	k_prp_distances<<<(uint32_t)(req.size() / 3), kBfsThreads, lds, s>>>(da);
 	UT_LAUNCH_CHECK();
-	h_diam.resize(2 * n);
-	UT_HIP_CHECK(hipMemcpyAsync(h_diam.data(), diameter.get(), 8 * (size_t)n, hipMemcpyDeviceToHost, s));
+}
+
+// (largest finite hop, ring geodesic bound) of the first 2n slots
+void PrpResolver::dist_results(uint32_t n)
+{
+	const hipStream_t s = ctx->stream;
+	h_dist_out.resize(4 * (size_t)n);
+	UT_HIP_CHECK(hipMemcpyAsync(h_dist_out.data(), dist_out.get(), 16 * (size_t)n, hipMemcpyDeviceToHost, s));
 	UT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+// full hop and geodesic fields of contours [c0, c0 + n) in batch rows 0 .. 2n - 1
+void PrpResolver::distances(uint32_t c0, uint32_t n)
+{
+	dist_reserve(n);
+	std::vector<uint32_t> req;
+	for (uint32_t sl = 0; sl < 2 * n; ++sl)
+		req.insert(req.end(), { sl, kDistHopAll, 1u });
+	dist_pass(c0, req, false);
+	dist_results(n);
+	h_diam.resize(2 * n);
+	for (uint32_t sl = 0; sl < 2 * n; ++sl)
+		h_diam[sl] = h_dist_out[2 * sl];
 }
 
 void PrpResolver::prepare(const float* x, const ContourBuilder& ct, uint32_t c0, uint32_t n)
@@ -2594,10 +2698,10 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 	if (final_cap < 1024)
 		final_cap = 1024;
 	uint32_t n_final = 0;
-	// hop and rest-geodesic distances of every processed contour in one launch
-	// (2 * active workgroups); the batches below read their rows of it
+	// hop and rest-geodesic distances per batch, searched only as far as the
+	// k-rings reach (the thresholds of the distance kernel)
 	if (active)
-		distances(0, active);
+		dist_reserve(std::min(active, params.batch_contours));
 	const auto t1 = clk::now();
 	double	   ms_levels = 0.0, ms_final = 0.0;
 	for (uint32_t c0 = 0; c0 < active; c0 += params.batch_contours)
@@ -2605,18 +2709,43 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 		const uint32_t n = std::min(params.batch_contours, active - c0);
 		batch0 = c0, batch_n = n;
 		const auto tl0 = clk::now();
-		begin_batch(c0);
+		begin_batch(0);
 		directions(c0, n);
+		// distances out to each slot's first k-ring: hop threshold max(3, ceil(sqrt(boundary))),
+		// the largest k that initial_k can choose.  A slot's diameter is known once its
+		// search died out within the threshold (largest hop <= threshold); until then it is
+		// above the threshold, and D stands for it as "infinite" in every comparison below.
+		std::vector<uint32_t> kthr(2 * n), D(2 * n), req;
+		std::vector<uint8_t>  dknown(2 * n);
+		for (uint32_t sl = 0; sl < 2 * n; ++sl)
+		{
+			const uint32_t nb = h_b_off[2 * c0 + sl + 1] - h_b_off[2 * c0 + sl];
+			kthr[sl] = std::max(3u, (uint32_t)std::ceil(std::sqrt((float)nb)));
+			req.insert(req.end(), { sl, kthr[sl], 1u });
+		}
+		dist_pass(c0, req, false);
+		dist_results(n);
+		auto learn = [&](uint32_t sl)
+		{
+			dknown[sl] = h_dist_out[2 * sl] <= kthr[sl];
+			D[sl] = dknown[sl] ? h_dist_out[2 * sl] : UINT32_MAX;
+		};
+		for (uint32_t sl = 0; sl < 2 * n; ++sl)
+			learn(sl);
+		// geodesic radii covering both roles' first rings
+		req.clear();
+		for (uint32_t sl = 0; sl < 2 * n; ++sl)
+			req.insert(req.end(), { sl, 0u, 0u });
+		dist_pass(c0, req, true);
 		// adaptive k-ring driver (evaluate_current_runtime_directions_template :1058-1230)
-		std::vector<uint32_t>				 ks(n), kd(n), D0(n), D1(n);
+		std::vector<uint32_t>				 ks(n), kd(n);
 		std::vector<std::vector<ComboState>> R(n);
-		std::vector<uint8_t>				 live(n, 1);
+		std::vector<uint8_t>				 live(n, 1), grow(2 * n), extended(n);
 		for (uint32_t lc = 0; lc < n; ++lc)
 		{
 			const uint32_t c = c0 + lc;
-			D0[lc] = diam_b[2 * lc], D1[lc] = diam_b[2 * lc + 1];
-			ks[lc] = initial_k(h_b_off[2 * c + 1] - h_b_off[2 * c], D0[lc]);
-			kd[lc] = initial_k(h_b_off[2 * c + 2] - h_b_off[2 * c + 1], D1[lc]);
+			ks[lc] = initial_k(h_b_off[2 * c + 1] - h_b_off[2 * c], D[2 * lc]);
+			kd[lc] = initial_k(h_b_off[2 * c + 2] - h_b_off[2 * c + 1], D[2 * lc + 1]);
 			R[lc].assign(h_dirs[c].size() / 3, ComboState{});
 			choices[c].levels = 0;
 		}
@@ -2654,9 +2783,10 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 				r.present = true;
 				r.s = res[j];
 				r.ks = ks[lc], r.kd = kd[lc];
-				const bool full = ks[lc] >= D0[lc] && kd[lc] >= D1[lc];
-				const bool src_ok = r.s.max_hop_src < ks[lc] || ks[lc] >= D0[lc];
-				const bool dst_ok = r.s.max_hop_dst < kd[lc] || kd[lc] >= D1[lc];
+				const uint32_t D0 = D[2 * lc], D1 = D[2 * lc + 1];
+				const bool	   full = ks[lc] >= D0 && kd[lc] >= D1;
+				const bool	   src_ok = r.s.max_hop_src < ks[lc] || ks[lc] >= D0;
+				const bool	   dst_ok = r.s.max_hop_dst < kd[lc] || kd[lc] >= D1;
 				r.eligible = r.s.usable && src_ok && dst_ok;
 				r.converged = r.eligible || full;
 			}
@@ -2675,29 +2805,65 @@ uint32_t PrpResolver::resolve(const float* x, const ContourBuilder& ct, Lbvh& fa
 					for (auto& r : rr)
 						if (!r.converged && r.present && response_objective(r.s) >= best)
 							r.pruned = true;
-				bool pending = false, gs = false, gd = false;
+				bool		   pending = false, gs = false, gd = false;
+				const uint32_t D0 = D[2 * lc], D1 = D[2 * lc + 1];
 				for (auto& r : rr)
 				{
 					if (r.converged || r.pruned)
 						continue;
 					pending = true;
-					gs = gs || (ks[lc] < D0[lc] && (!r.s.usable || r.s.max_hop_src >= ks[lc]));
-					gd = gd || (kd[lc] < D1[lc] && (!r.s.usable || r.s.max_hop_dst >= kd[lc]));
+					gs = gs || (ks[lc] < D0 && (!r.s.usable || r.s.max_hop_src >= ks[lc]));
+					gd = gd || (kd[lc] < D1 && (!r.s.usable || r.s.max_hop_dst >= kd[lc]));
 				}
 				if (!pending)
 				{
 					live[lc] = 0;
 					continue;
 				}
-				const uint32_t ns_ = gs ? grow_k(ks[lc], D0[lc]) : ks[lc];
-				const uint32_t nd_ = gd ? grow_k(kd[lc], D1[lc]) : kd[lc];
+				grow[2 * lc] = gs, grow[2 * lc + 1] = gd;
+			}
+			// a growing ring whose diameter is not known yet: search out to the candidate
+			// k first, so that grow_k sees the diameter whenever it is below the candidate
+			req.clear();
+			std::fill(extended.begin(), extended.end(), 0);
+			for (uint32_t lc = 0; lc < n; ++lc)
+				for (uint32_t r = 0; r < 2 && live[lc]; ++r)
+				{
+					const uint32_t sl = 2 * lc + r, k = r ? kd[lc] : ks[lc];
+					if (grow[sl] && !dknown[sl] && 2 * k > kthr[sl])
+					{
+						kthr[sl] = 2 * k;
+						req.insert(req.end(), { sl, kthr[sl], 0u });
+						extended[lc] = 1;
+					}
+				}
+			if (!req.empty())
+			{
+				dist_pass(c0, req, false);
+				dist_results(n);
+				for (size_t q = 0; q < req.size(); q += 3)
+					learn(req[q]);
+			}
+			req.clear();
+			for (uint32_t lc = 0; lc < n; ++lc)
+			{
+				if (!live[lc])
+					continue;
+				const uint32_t ns_ = grow[2 * lc] ? grow_k(ks[lc], D[2 * lc]) : ks[lc];
+				const uint32_t nd_ = grow[2 * lc + 1] ? grow_k(kd[lc], D[2 * lc + 1]) : kd[lc];
+				grow[2 * lc] = grow[2 * lc + 1] = 0;
 				if (ns_ <= ks[lc] && nd_ <= kd[lc])
 				{
 					live[lc] = 0; // cannot grow further (the reference raises here)
 					continue;
 				}
 				ks[lc] = ns_, kd[lc] = nd_;
+				if (extended[lc])
+					req.insert(req.end(), { 2 * lc, 0u, 0u, 2 * lc + 1, 0u, 0u });
 			}
+			// geodesic radii covering both roles' grown rings (a ring that grew inside an
+			// unchanged hop threshold is covered by the radii set for that threshold)
+			dist_pass(c0, req, true);
 		}
 		// selection (prp_select_response)
 		std::vector<PrpSlot>  chosen;

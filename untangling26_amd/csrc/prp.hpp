@@ -128,7 +128,8 @@ struct PrpResolver
 	DevBuf<float>	 geo, curd;				 // [2 * batch][V]
 	DevBuf<uint32_t> queue, inq;			 // distance work lists + (hop, geodesic) words [2 * batch][4V], [2 * batch][V]
 	DevBuf<uint32_t> adj_blocks;			 // [V][8] (neighbour, rest length bits) pairs
-	DevBuf<uint32_t> diameter;				 // [2 * contours]
+	DevBuf<uint32_t> dist_req;				 // distance launch requests (slot, hop threshold, first)
+	DevBuf<uint32_t> dist_state, dist_out;	 // per slot: (hop threshold, radius), (largest hop, ring geodesic bound)
 	// rows of the current batch in hop / geo / h_diam (distances may hold more contours)
 	const uint32_t*	 hop_b = nullptr;
 	const float*	 geo_b = nullptr;
@@ -190,7 +191,7 @@ struct PrpResolver
 
 	// host-side contour data of the last prepare
 	uint32_t						nc = 0, batch0 = 0, batch_n = 0, total_pairs = 0;
-	std::vector<uint32_t>			h_cpair_off, h_cpairs, h_pair_words, h_b_off, h_b_verts, h_diam;
+	std::vector<uint32_t>			h_cpair_off, h_cpairs, h_pair_words, h_b_off, h_b_verts, h_diam, h_dist_out;
 	std::vector<uint32_t>			h_contour_of_pair, h_mesh_of_pair, h_adj, h_adjn, h_adj_off;
 	std::vector<std::vector<float>> h_dirs;
 	std::vector<float>				h_va;
@@ -211,6 +212,9 @@ struct PrpResolver
 private:
 	void build_contour_tables(const ContourBuilder& ct);
 	void distances(uint32_t c0, uint32_t n);
+	void dist_reserve(uint32_t n);
+	void dist_pass(uint32_t c0, const std::vector<uint32_t>& req, bool mode_b);
+	void dist_results(uint32_t n);
 	void begin_batch(uint32_t first);
 	void directions(uint32_t c0, uint32_t n);
 	void ensure_hits(uint32_t cap);
