@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256, help="cloth resolution (vertices per side)")
     ap.add_argument("--dirs", type=int, default=64, help="candidate directions per contour")
+    ap.add_argument("--prp-batch", type=int, default=128,
+                    help="contours evaluated together by the PRP (PRP_cpu_contour_batch_size, the reference's default 32); "
+                         "a throughput setting: the step's results do not depend on it")
     ap.add_argument("--pcg-iters", type=int, default=1000)
     ap.add_argument("--inner-tol", type=float, default=1e-2,
                     help="relative tolerance of each FP32 inner solve of the mixed-precision PCG")
@@ -77,6 +80,7 @@ def make_solver(args, precision):
     cfg.pcg_inner_tol = args.inner_tol
     cfg.pcg_precision = precision
     cfg.prp_directions = args.dirs
+    cfg.PRP_cpu_contour_batch_size = getattr(args, "prp_batch", 32)
     s.init_solver()
     return s, m, np.asarray(rest, np.float32)
 
@@ -325,6 +329,7 @@ def main():
                          "offset by a quarter cell in-plane (coincident layers make the EF predicates "
                          "degenerate), z += 0.08 sin(2 pi u) sin(pi v) + N(0, 0.005)",
             "pcg": "mixed FP32 + FP64 refinement",
+            "prp_contour_batch": args.prp_batch,
             "timed_steps": "K repair steps from the tangled state; a step that finds the fold "
                            "intersection-free restarts the repair from the tangled state",
             "timed_restarts": leg["restarts"],
