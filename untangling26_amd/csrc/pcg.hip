@@ -70,9 +70,10 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 	return r;
 }
 
-// K sums of partial arrays (cnt[k] entries each) and the stop flag, read by the
-// first wave in one pass so that the loads of all arrays are in flight together;
-// each sum has the order of sum_partials, so every workgroup gets identical
+// K sums of partial arrays (cnt[k] entries each) and the stop flag.  Every
+// thread of the group sums the entries at its index plus multiples of the
+// group size (two strides' loads issued together), the waves reduce, and the
+// four wave sums are added in a fixed order, so every workgroup gets identical
 // values.  One call per kernel (its LDS slots are not re-fenced).
 template <int K>
 struct GroupSums
@@ -84,40 +85,51 @@ template <int K>
 __device__ __forceinline__ GroupSums<K> group_sums(const double* const (&part)[K], const uint32_t (&cnt)[K],
 	const uint32_t* flags)
 {
-	__shared__ double	s_val[K];
+	constexpr int		kWaves = kPcgThreads / kWave;
+	__shared__ double	s_wave[kWaves][K];
 	__shared__ uint32_t s_stop;
-	if (threadIdx.x < kWave)
+	if (threadIdx.x == 0)
+		s_stop = __hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	double	 acc[K];
+	uint32_t m = 0;
+#pragma unroll
+	for (int k = 0; k < K; ++k)
 	{
-		if (threadIdx.x == 0)
-			s_stop = __hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		double	 acc[K];
-		uint32_t m = 0;
+		acc[k] = 0.0;
+		m = max(m, cnt[k]);
+	}
+	for (uint32_t base = threadIdx.x; base < m; base += 2 * kPcgThreads)
+	{
+		double v[K][2];
 #pragma unroll
 		for (int k = 0; k < K; ++k)
-		{
-			acc[k] = 0.0;
-			m = max(m, cnt[k]);
-		}
-		for (uint32_t i = threadIdx.x; i < m; i += kWave)
-		{
 #pragma unroll
-			for (int k = 0; k < K; ++k)
-				if (i < cnt[k])
-					acc[k] += part[k][i];
-		}
+			for (int j = 0; j < 2; ++j)
+			{
+				const uint32_t i = base + j * kPcgThreads;
+				v[k][j] = i < cnt[k] ? part[k][i] : 0.0;
+			}
 #pragma unroll
 		for (int k = 0; k < K; ++k)
-		{
-			const double s = wave_sum(acc[k]);
-			if (threadIdx.x == 0)
-				s_val[k] = s;
-		}
+			acc[k] += v[k][0] + v[k][1];
+	}
+#pragma unroll
+	for (int k = 0; k < K; ++k)
+	{
+		const double s = wave_sum(acc[k]);
+		if (__lane_id() == 0)
+			s_wave[threadIdx.x / kWave][k] = s;
 	}
 	__syncthreads();
 	GroupSums<K> r;
 #pragma unroll
 	for (int k = 0; k < K; ++k)
-		r.v[k] = s_val[k];
+	{
+		double t = s_wave[0][k];
+		for (int w = 1; w < kWaves; ++w)
+			t += s_wave[w][k];
+		r.v[k] = t;
+	}
 	r.stop = s_stop != 0u;
 	return r;
 }
