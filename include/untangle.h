@@ -616,7 +616,11 @@ UT_API int ut_prp_resolve(ut_prp* p, const float* x, ut_contours* ct, ut_lbvh* f
 	ut_prp_stats* stats);
 /* Parity hooks: prepare contour batch [c0, c0 + n) (distances, directions),
  * read its directions and distances, evaluate slots and read the sorted hits of
- * the last evaluation, read the face/edge/vertex normals. */
+ * the last evaluation, read the face/edge/vertex normals.  ut_prp_prepare
+ * computes the full hop and rest-geodesic fields (and the hop diameters) that
+ * ut_prp_distances returns; ut_prp_resolve searches each field only as far as
+ * its k-rings reach (exact up to those limits), so read distances after a
+ * prepare, not after a resolve. */
 UT_API int ut_prp_prepare(ut_prp* p, const float* x, ut_contours* ct, uint32_t c0, uint32_t n);
 UT_API int ut_prp_directions(ut_prp* p, uint32_t contour, float* dirs, uint32_t* count);
 UT_API int ut_prp_distances(ut_prp* p, uint32_t local_contour, uint32_t role, uint32_t* hop, float* geo,
