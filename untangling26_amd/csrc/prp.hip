@@ -1238,12 +1238,15 @@ __global__ void k_prp_hit_union(HitArgs a)
 	const uint32_t s = L.slot, hi = a.range[2 * s + 1];
 	uint32_t	   ri = i; // an ancestor of i: where the next union's find of i starts
 	// the hits (type, prim) -> d for targets in ascending order: one bucket lookup,
-	// then galloping searches that resume where the previous one ended
-	auto link_list = [&](uint32_t type, uint32_t prim, const uint32_t* __restrict__ d, uint32_t n)
+	// then galloping searches that resume where the previous one ended.  A search
+	// in hit i's own bucket for targets above i's target starts at i + 1 instead
+	// (the bucket is sorted by target), with no table probe.
+	auto link_list = [&](uint32_t type, uint32_t prim, const uint32_t* __restrict__ d, uint32_t n,
+						 uint32_t from = kNone)
 	{
 		const uint32_t key = hit_key(s, type, prim, a.prim_bits);
-		uint32_t	   lo = bucket_start(a, s, key);
-		if (lo == kNone)
+		uint32_t	   lo = from != kNone ? from : bucket_start(a, s, key);
+		if (lo == kNone || lo >= hi)
 			return;
 		for (uint32_t q = 0; q < n; ++q)
 		{
@@ -1254,11 +1257,12 @@ __global__ void k_prp_hit_union(HitArgs a)
 	};
 	// the same for up to four targets held in registers (kNone entries, last, are skipped);
 	// the unions follow the four searches, so their parent walks overlap
-	auto link4 = [&](uint32_t type, uint32_t prim, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3)
+	auto link4 = [&](uint32_t type, uint32_t prim, uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3,
+					 uint32_t from = kNone)
 	{
 		const uint32_t key = hit_key(s, type, prim, a.prim_bits);
-		uint32_t	   lo = bucket_start(a, s, key);
-		if (lo == kNone)
+		uint32_t	   lo = from != kNone ? from : bucket_start(a, s, key);
+		if (lo == kNone || lo >= hi)
 			return;
 		const uint32_t d[4] = { d0, d1, d2, d3 };
 		uint32_t	   j[4] = { kNone, kNone, kNone, kNone };
@@ -1295,13 +1299,14 @@ __global__ void k_prp_hit_union(HitArgs a)
 	{
 		const uint32_t f = L.src, v = L.dst;
 		// (VF(u, f'), u a vertex of f, f' a face of v: found from the VF side)
-		// FV(f, u), u adjacent to v (u > v): the neighbours of v are listed ascending
+		// FV(f, u), u adjacent to v (u > v): the neighbours of v are listed ascending;
+		// i's own bucket, after i
 		const uint32_t q0 = a.vv_off[v], q1 = a.vv_off[v + 1];
 		uint32_t	   q = q0;
 		while (q < q1 && a.vv[q] <= v)
 			++q;
 		if (q < q1)
-			link_list(2, f, a.vv + q, q1 - q);
+			link_list(2, f, a.vv + q, q1 - q, i + 1);
 		// EE(e, e'), e an edge of f, e' an edge of v (listed ascending)
 		for (int k = 0; k < 3; ++k)
 			link_list(1, a.fae[3 * f + k], a.ve + a.ve_off[v], a.ve_off[v + 1] - a.ve_off[v]);
@@ -1325,7 +1330,7 @@ __global__ void k_prp_hit_union(HitArgs a)
 			t[2 * q2 + 1] = o1 > e2 ? o1 : kNone;
 		}
 		cswap(t[0], t[1]), cswap(t[2], t[3]), cswap(t[0], t[2]), cswap(t[1], t[3]), cswap(t[1], t[2]);
-		link4(1, e1, t[0], t[1], t[2], t[3]);
+		link4(1, e1, t[0], t[1], t[2], t[3], i + 1); // i's own bucket, after i
 		// EE(e', e2), e' another edge of a face of e1 (e' > e1); one union batch
 		uint32_t j2[4] = { kNone, kNone, kNone, kNone };
 #pragma unroll
