@@ -656,12 +656,9 @@ struct RayReq
 // VF (kind 0) and FV (kind 2): segment from a vertex along +dir (VF) or -dir (FV), length 1,
 // against candidate faces of the other role (LineIntersection, t in [0, 1], incident faces skipped).
 template <int Kind>
-__global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
+__global__ __launch_bounds__(kTravThreads, kRayBlocks / 256) void k_prp_ray_vert(RayArgs a)
 {
 	__shared__ TravSharedT<kRayStackLds> sh;
-	// request of each lane (slot, source vertex), read by the leaf tests of the lane's
-	// wave: a staged candidate of request r belongs to lane r - base
-	__shared__ uint32_t req_slot[kTravThreads], req_src[kTravThreads];
 	const uint32_t		total = a.off[Kind * (a.nslots + 1) + a.nslots];
 	const uint32_t*		off = a.off + Kind * (a.nslots + 1);
 	const uint32_t		spill_slot = blockIdx.x * blockDim.x + threadIdx.x;
@@ -684,16 +681,10 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 			q.dir = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
 			q.o = ld3(a.x, q.src);
 			q.tmax = 1.0f;
-			req_slot[threadIdx.x] = q.slot;
-			req_src[threadIdx.x] = q.src;
 			// the candidate target faces (role 1 for VF rays, role 0 for FV rays) lie in this box
 			const float* tb = a.tbox + 12 * (size_t)q.slot + (Kind == 0 ? 0 : 6);
 			cb = box3{ mk3(tb[0], tb[1], tb[2]), mk3(tb[3], tb[4], tb[5]) };
 		}
-		// the wave's own lanes wrote the entries its leaf tests read
-		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-		__builtin_amdgcn_wave_barrier();
-		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 		const f3 cast = Kind == 0 ? q.dir : -q.dir;
 		const f3 inv = mk3(1.0f / cast.x, 1.0f / cast.y, 1.0f / cast.z);
 		auto	 test = [&](const box3& b)
@@ -713,22 +704,24 @@ __global__ __launch_bounds__(kTravThreads) void k_prp_ray_vert(RayArgs a)
 		};
 		auto narrow = [&](bool valid, uint32_t rr, uint32_t f)
 		{
-			// the lane's own request is r; staged candidates carry their request id
-			bool   ok = false;
-			PrpHit h{};
+			// staged candidates carry their request id rr, which belongs to lane rr - base
+			// of this wave: its slot, contour, source vertex, direction and origin come
+			// from that lane's registers (every lane shuffles, valid or not)
+			const uint32_t src = (rr - base) & (kWave - 1);
+			const uint32_t slot = __shfl(q.slot, src, kWave), contour = __shfl(q.contour, src, kWave);
+			const uint32_t v = __shfl(q.src, src, kWave);
+			const f3	   d = mk3(__shfl(q.dir.x, src, kWave), __shfl(q.dir.y, src, kWave), __shfl(q.dir.z, src, kWave));
+			const f3	   o = mk3(__shfl(q.o.x, src, kWave), __shfl(q.o.y, src, kWave), __shfl(q.o.z, src, kWave));
+			bool		   ok = false;
+			PrpHit		   h{};
 			if (valid)
 			{
-				const uint32_t slot = req_slot[rr - base];
-				const PrpSlot& sl = a.slots[slot];
-				const uint32_t s_dst = 2 * sl.contour + (Kind == 0 ? 1u : 0u);
-				const uint32_t v = req_src[rr - base];
+				const uint32_t s_dst = 2 * contour + (Kind == 0 ? 1u : 0u);
 				const bool		is_cand = (a.fcand[(size_t)s_dst * a.fwords + (f >> 6)] >> (f & 63)) & 1ull;
 				const uint32_t t0 = is_cand ? a.faces[3 * f] : v, t1 = is_cand ? a.faces[3 * f + 1] : v,
 							   t2 = is_cand ? a.faces[3 * f + 2] : v;
 				if (is_cand && v != t0 && v != t1 && v != t2)
 				{
-					const f3 d = mk3(sl.dir[0], sl.dir[1], sl.dir[2]);
-					const f3 o = ld3(a.x, v);
 					const f3 e = Kind == 0 ? o + d * 1.0f : o + (-d) * 1.0f;
 					f3		 bary;
 					float	 t;
